@@ -1,0 +1,343 @@
+"""ctypes front-end of the C oracle (oracle/mjw_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by ``bench.py``'s cpu_baseline leg, as the checker.  The product package
+``montjoie_amd`` never imports it.
+
+The oracle restates Montjoie's explicit leapfrog path in plain FP64 C; each C
+routine cites the reference file:line it follows.  This module only marshals
+numpy arrays and assembles the structured synthetic problems used by the
+tests (regular / perturbed meshes of the unit square and cube).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "libmjw_oracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_bp = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle shared library with the committed Makefile."""
+    subprocess.run(["make", "-C", _HERE, "oracle"] + (["-B"] if force else []),
+                   check=True, stdout=subprocess.DEVNULL)
+    return _SO
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_SO):
+        build()
+    L = C.CDLL(_SO)
+    L.or_gl.argtypes = [C.c_int, _dp, _dp]
+    L.or_gl.restype = C.c_int
+    L.or_gl_deriv.argtypes = [C.c_int, _dp, _dp]
+    L.or_phi.argtypes = [C.c_int, _dp, C.c_int, C.c_double]
+    L.or_phi.restype = C.c_double
+    L.or_uniform.argtypes = [C.c_uint64, C.c_uint64]
+    L.or_uniform.restype = C.c_double
+    L.or_mesh_vertices.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
+    L.or_mesh_nodes.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, _bp]
+    L.or_mesh_nodes.restype = C.c_int
+    L.or_geometry.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_geometry.restype = C.c_double
+    L.or_mass.argtypes = [C.c_long, C.c_int, C.c_long, _ip, _dp, _bp, _dp, _dp]
+    L.or_color.argtypes = [C.c_long, C.c_int, C.c_long, _ip, _ip]
+    L.or_color.restype = C.c_int
+    L.or_stiffness.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, _ip, _dp, C.c_void_p,
+                               C.c_int, C.c_double, _dp, C.c_double, _dp]
+    L.or_stiffness_elastic.argtypes = [C.c_int, C.c_long, C.c_long, _ip, _dp, _dp, _dp, _dp,
+                                       _ip, C.c_int, C.c_double, _dp, C.c_double, _dp]
+    L.or_ricker.argtypes = [C.c_double, C.c_double]
+    L.or_ricker.restype = C.c_double
+    L.or_pulse_table.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+    L.or_dirac.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip, C.c_long, _bp, _dp, C.c_int,
+                           _dp, _dp]
+    L.or_dirac.restype = C.c_int
+    L.or_receiver.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.or_receiver.restype = C.c_long
+    L.or_circle_points.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+    L.or_leapfrog.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, _ip,
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                              _ip, C.c_int, _dp, _bp, C.c_void_p, C.c_void_p, C.c_double,
+                              C.c_int, C.c_void_p, C.c_void_p, C.c_int, _lp, _dp, C.c_void_p,
+                              _dp, _dp, _dp]
+    L.or_lambda_max.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, _ip,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                _ip, C.c_int, _dp, _bp, C.c_int, C.c_uint64]
+    L.or_lambda_max.restype = C.c_double
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# --------------------------------------------------------------------------
+# 1-D rule
+# --------------------------------------------------------------------------
+
+def gl(r: int):
+    x = np.zeros(r + 1)
+    w = np.zeros(r + 1)
+    if lib().or_gl(r, x, w):
+        raise ValueError("bad order")
+    return x, w
+
+
+def gl_deriv(r: int):
+    x, _ = gl(r)
+    D = np.zeros((r + 1) * (r + 1))
+    lib().or_gl_deriv(r, x, D)
+    return D.reshape(r + 1, r + 1)
+
+
+def phi(r: int, i: int, t: float) -> float:
+    x, _ = gl(r)
+    return lib().or_phi(r, x, i, t)
+
+
+def ricker(f0: float, t: float) -> float:
+    return lib().or_ricker(f0, t)
+
+
+def pulse_table(f0: float, t_begin: float, dt: float, nsteps: int) -> np.ndarray:
+    p = np.zeros(nsteps)
+    lib().or_pulse_table(f0, t_begin, dt, nsteps, p)
+    return p
+
+
+def circle_points(cx, cy, radius, npts):
+    p = np.zeros(2 * npts)
+    lib().or_circle_points(cx, cy, radius, npts, p)
+    return p.reshape(npts, 2)
+
+
+def uniform(seed: int, idx: int) -> float:
+    return lib().or_uniform(seed, idx)
+
+
+# --------------------------------------------------------------------------
+# Structured problems
+# --------------------------------------------------------------------------
+
+@dataclass
+class Mesh:
+    dim: int
+    nx: int
+    r: int
+    pert: float
+    seed: int
+    vert: np.ndarray = field(repr=False)
+    xyz: np.ndarray = field(repr=False)
+    elem_nodes: np.ndarray = field(repr=False)
+    dirichlet: np.ndarray = field(repr=False)
+
+    @property
+    def n(self):
+        return self.r + 1
+
+    @property
+    def nd(self):
+        return self.n ** self.dim
+
+    @property
+    def E(self):
+        return self.nx ** self.dim
+
+    @property
+    def N(self):
+        return (self.nx * self.r + 1) ** self.dim
+
+
+def mesh(dim: int, nx: int, r: int, pert: float = 0.0, seed: int = 0) -> Mesh:
+    L = lib()
+    nv = (nx + 1) ** dim
+    vert = np.zeros(nv * dim)
+    L.or_mesh_vertices(dim, nx, pert, seed, vert)
+    N = (nx * r + 1) ** dim
+    E = nx ** dim
+    nd = (r + 1) ** dim
+    xyz = np.zeros(N * dim)
+    en = np.zeros(E * nd, dtype=np.int32)
+    dr = np.zeros(N, dtype=np.uint8)
+    if L.or_mesh_nodes(dim, nx, r, vert, xyz, en, dr):
+        raise ValueError("mesh")
+    return Mesh(dim, nx, r, pert, seed, vert, xyz.reshape(N, dim), en.reshape(E, nd), dr)
+
+
+def layered(m: Mesh, values) -> np.ndarray:
+    """Per-element value of horizontal layers: layer = ez*len(values)//nx
+    (y in 2-D, z in 3-D)."""
+    E = m.E
+    e = np.arange(E)
+    ez = e // (m.nx ** (m.dim - 1))
+    lay = (ez * len(values)) // m.nx
+    return np.asarray(values, dtype=np.float64)[lay]
+
+
+def geometry(m: Mesh, mu, rho, elastic: bool = False):
+    """Returns (G, Mloc) for the scalar case or (jw, dfinv) for elasticity."""
+    L = lib()
+    mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, np.float64), (m.E,)))
+    rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float64), (m.E,)))
+    nc = 3 if m.dim == 2 else 6
+    Mloc = np.zeros(m.E * m.nd)
+    if not elastic:
+        G = np.zeros(m.E * m.nd * nc)
+        jmin = L.or_geometry(m.dim, m.nx, m.r, m.vert, mu, rho, _ptr(G), _ptr(Mloc), None, None)
+        if jmin <= 0:
+            raise ValueError("degenerate element")
+        return G, Mloc
+    jw = np.zeros(m.E * m.nd)
+    dfi = np.zeros(m.E * m.nd * m.dim * m.dim)
+    jmin = L.or_geometry(m.dim, m.nx, m.r, m.vert, mu, rho, None, _ptr(Mloc), _ptr(jw), _ptr(dfi))
+    if jmin <= 0:
+        raise ValueError("degenerate element")
+    return jw, dfi, Mloc
+
+
+def mass(m: Mesh, Mloc, dirichlet=True):
+    M = np.zeros(m.N)
+    Mi = np.zeros(m.N)
+    dr = m.dirichlet if dirichlet else np.zeros(m.N, np.uint8)
+    lib().or_mass(m.E, m.nd, m.N, m.elem_nodes.reshape(-1), Mloc, dr, M, Mi)
+    return M, Mi
+
+
+def color(m: Mesh):
+    c = np.zeros(m.E, dtype=np.int32)
+    nc = lib().or_color(m.E, m.nd, m.N, m.elem_nodes.reshape(-1), c)
+    return c, nc
+
+
+def stiffness(m: Mesh, G, u, colors=None, alpha=1.0):
+    y = np.zeros(m.N)
+    if colors is None:
+        lib().or_stiffness(m.dim, m.r, m.E, m.N, m.elem_nodes.reshape(-1), G, None, 0, alpha,
+                           np.ascontiguousarray(u, dtype=np.float64), 0.0, y)
+    else:
+        c, nc = colors
+        lib().or_stiffness(m.dim, m.r, m.E, m.N, m.elem_nodes.reshape(-1), G, _ptr(c), nc,
+                           alpha, np.ascontiguousarray(u, dtype=np.float64), 0.0, y)
+    return y
+
+
+def stiffness_elastic(m: Mesh, jw, dfi, lam, mu, u, colors, alpha=1.0):
+    c, nc = colors
+    y = np.zeros(3 * m.N)
+    lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float64), (m.E,)))
+    mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, np.float64), (m.E,)))
+    lib().or_stiffness_elastic(m.r, m.E, m.N, m.elem_nodes.reshape(-1), jw, dfi, lam, mu, c, nc,
+                               alpha, np.ascontiguousarray(u, dtype=np.float64), 0.0, y)
+    return y
+
+
+def dirac(m: Mesh, p, pol=(1.0,)):
+    pol = np.asarray(pol, np.float64)
+    b = np.zeros(m.N * len(pol))
+    cnt = lib().or_dirac(m.dim, m.nx, m.r, m.vert, m.elem_nodes.reshape(-1), m.N, m.dirichlet,
+                         np.asarray(p, np.float64), len(pol), pol, b)
+    if cnt == 0:
+        raise ValueError("source point outside the mesh")
+    return b
+
+
+def receivers(m: Mesh, pts):
+    pts = np.asarray(pts, np.float64).reshape(-1, m.dim)
+    elems = np.zeros(len(pts), dtype=np.int64)
+    W = np.zeros((len(pts), m.nd))
+    for i, p in enumerate(pts):
+        w = np.zeros(m.nd)
+        e = lib().or_receiver(m.dim, m.nx, m.r, m.vert, np.ascontiguousarray(p), w)
+        if e < 0:
+            raise ValueError(f"receiver {i} outside the mesh")
+        elems[i] = e
+        W[i] = w
+    return elems, W
+
+
+@dataclass
+class Problem:
+    """A structured acoustic (ncomp=1) or elastic (ncomp=3) problem."""
+    mesh: Mesh
+    ncomp: int
+    G: np.ndarray = None
+    jw: np.ndarray = None
+    dfi: np.ndarray = None
+    lam: np.ndarray = None
+    mu: np.ndarray = None
+    rho: np.ndarray = None
+    M: np.ndarray = None
+    Minv: np.ndarray = None
+    colors: tuple = None
+
+
+def acoustic_problem(m: Mesh, mu_e, rho_e) -> Problem:
+    G, Mloc = geometry(m, mu_e, rho_e)
+    M, Mi = mass(m, Mloc)
+    return Problem(m, 1, G=G, mu=np.broadcast_to(np.asarray(mu_e, float), (m.E,)).copy(),
+                   rho=np.broadcast_to(np.asarray(rho_e, float), (m.E,)).copy(), M=M, Minv=Mi,
+                   colors=color(m))
+
+
+def elastic_problem(m: Mesh, lam_e, mu_e, rho_e) -> Problem:
+    jw, dfi, Mloc = geometry(m, mu_e, rho_e, elastic=True)
+    M, Mi = mass(m, Mloc)
+    return Problem(m, 3, jw=jw, dfi=dfi,
+                   lam=np.ascontiguousarray(np.broadcast_to(np.asarray(lam_e, float), (m.E,))),
+                   mu=np.ascontiguousarray(np.broadcast_to(np.asarray(mu_e, float), (m.E,))),
+                   rho=np.broadcast_to(np.asarray(rho_e, float), (m.E,)).copy(), M=M, Minv=Mi,
+                   colors=color(m))
+
+
+def apply(P: Problem, u):
+    if P.ncomp == 1:
+        return stiffness(P.mesh, P.G, u, P.colors)
+    return stiffness_elastic(P.mesh, P.jw, P.dfi, P.lam, P.mu, u, P.colors)
+
+
+def lambda_max(P: Problem, m_iter: int = 120, seed: int = 7) -> float:
+    m = P.mesh
+    c, nc = P.colors
+    return lib().or_lambda_max(m.dim, m.r, P.ncomp, m.E, m.N, m.elem_nodes.reshape(-1), _ptr(P.G),
+                               _ptr(P.jw), _ptr(P.dfi), _ptr(P.lam), _ptr(P.mu), c, nc, P.M,
+                               m.dirichlet, m_iter, seed)
+
+
+def leapfrog(P: Problem, b, pulse, dt, nsteps, rec_pts=None, rec_comp=None, u0=None, du0=None):
+    """Returns (U^N, U^{N+1}, traces[(N+1), nrec])."""
+    m = P.mesh
+    c, nc = P.colors
+    NT = m.N * P.ncomp
+    if rec_pts is not None and len(rec_pts):
+        re, rw = receivers(m, rec_pts)
+    else:
+        re, rw = np.zeros(0, np.int64), np.zeros(0)
+    nrec = len(re)
+    rc = None if rec_comp is None else np.ascontiguousarray(rec_comp, dtype=np.int32)
+    uN = np.zeros(NT)
+    uN1 = np.zeros(NT)
+    tr = np.zeros(((nsteps + 1) * max(nrec, 1)))
+    pulse = None if pulse is None else np.ascontiguousarray(pulse, np.float64)
+    lib().or_leapfrog(m.dim, m.r, P.ncomp, m.E, m.N, m.elem_nodes.reshape(-1), _ptr(P.G),
+                      _ptr(P.jw), _ptr(P.dfi), _ptr(P.lam), _ptr(P.mu), c, nc, P.M, m.dirichlet,
+                      _ptr(b), _ptr(pulse), dt, nsteps, _ptr(u0), _ptr(du0), nrec,
+                      np.ascontiguousarray(re), np.ascontiguousarray(rw.reshape(-1)), _ptr(rc),
+                      uN, uN1, tr)
+    return uN, uN1, tr[: (nsteps + 1) * nrec].reshape(nsteps + 1, nrec)
