@@ -1,0 +1,219 @@
+"""Benchmark: FP32 vs FP64 leapfrog on the Q5 3-D acoustic workload.
+
+Workload (one GPU): unit cube, 48^3 Q5 hexahedra with interior vertices
+displaced by <= 0.1 h (seed 2024), velocity in three horizontal layers
+c_k ~ U[1,3] (seed 2025), rho = 1, mu = c^2, Dirichlet boundary, Dirac at
+the centre with a Ricker pulse f0 = 8, dt = 0.5 x CFL limit, 16 receivers.
+A step is one fused leapfrog step over the whole mesh (8 colour passes of
+the stiffness kernel + source/receiver tail), inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--no-cpu]
+
+Prints one JSON line.  `value` is the FP32 (state and geometry) path's
+GDoF-steps/s; the FP64 path, the FP32-with-FP64-geometry path and their
+rel-L2 errors against FP64 are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(nx=48, order=5):
+    import montjoie_amd as M
+    m = M.regular_mesh(3, nx, order, 0.1, 2024)
+    c = np.array([1.0 + 2.0 * _unif(2025, k) for k in range(3)])
+    mu = M.layered_values(m, c * c)
+    return m, mu, c
+
+
+def _unif(seed, idx):
+    # same splitmix64 stream as the mesh perturbation (montjoie_amd/csrc/mjw_host.cpp)
+    M64 = (1 << 64) - 1
+
+    def mix(z):
+        z = (z + 0x9E3779B97F4A7C15) & M64
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+        return z ^ (z >> 31)
+    h = mix(mix(seed) ^ ((idx * 0xD1B54A32D192ED03) & M64))
+    return (h >> 11) * (1.0 / 9007199254740992.0)
+
+
+def receivers(n=16, seed=77):
+    return np.array([[0.1 + 0.8 * _unif(seed, 3 * i + a) for a in range(3)] for i in range(n)])
+
+
+def algorithmic_bytes_per_step(E, nd, N, state_bytes, geom_bytes, ngeom=6):
+    """geometry + element map + (read U, read/write U^{n-1 -> n+1}, read dt^2/M)."""
+    return E * nd * (ngeom * geom_bytes + 4) + N * 4 * state_bytes
+
+
+def flops_per_step(E, n):
+    # gradient + divergence 2*3*2n per node, tensor 2*9, update ~6 per node
+    nd = n ** 3
+    return E * nd * (12 * n + 18 + 6)
+
+
+def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10):
+    import torch
+    import montjoie_amd as M
+    P = M.WaveProblem(m, 1.0, mu, precision=prec, nshots=nshots)
+    srcs = [(0.5, 0.5, 0.5)] * nshots
+    P.set_sources(srcs, shot=np.arange(nshots))
+    P.set_receivers(receivers())
+    total = warmup + steps + profile_steps
+    P.set_pulse(M.ricker_pulse_table(f0, 0.0, dt, total))
+    P.leapfrog_init(dt)
+    if warmup:
+        P.advance(warmup, traces=False, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tr = P.advance(steps, traces=True, graph=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = P.last_timing()[0]
+    uN, _ = P.state()
+    prof = P.profile(profile_steps)
+    info = P.info()
+    P.close()
+    return dict(wall=wall, dev_ms_step=dev_ms, traces=tr, uN=uN, prof=prof, info=info)
+
+
+def cpu_reference(nx=24, order=5, steps=30, dt=1e-3):
+    """Montjoie's own solver (oracle/_ref, built from the reference sources)
+    on a bounded sample of the same workload family, one core."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "mjref_acous.x")
+    if not os.path.exists(exe):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from oracle import oracle as O
+    import make_ref_fixtures as F
+    with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "gpurun_out")
+                                     if os.path.isdir(os.path.join(ROOT, "gpurun_out")) else None) as d:
+        m = O.mesh(3, nx, order, 0.1, 2024)
+        c = [1.0 + 2.0 * _unif(2025, k) for k in range(3)]
+        lay = ((np.arange(m.E) // nx ** 2) * 3) // nx
+        F.write_medit(os.path.join(d, "mesh.mesh"), m, lay + 1, bnd_ref=4)
+        ini = ["TypeElement = HEXAHEDRON_LOBATTO", "TypeEquation = ACOUSTIC", "MeshPath = ./",
+               "Frequency = 8.0 0.0", "FileMesh = mesh.mesh", "TypeMesh = HEXA",
+               f"OrderDiscretization = {order}", "ConditionReference = 4 DIRICHLET"]
+        ini += [f"MateriauDielec = {k + 1} ISOTROPE 1.0 {c[k] ** 2!r} 0.0" for k in range(3)]
+        ini += ["TypeSource = SRC_DIRAC", "OriginePhase = 0.5 0.5 0.5",
+                f"ParametersOutputPoint = 0.0 {steps * dt + dt / 2!r} {dt!r}",
+                "FileOutputPoint = diffracPoint.dat totalPoint.dat", "SismoPoint = 0.3 0.4 0.6",
+                "OutputFormat = BINARY DOUBLE", "ElectricOrMagnetic = 0", f"TimeStep = {dt!r}",
+                f"TimeInterval = 0.0 {(steps - 0.5) * dt!r}", "OrderTimeScheme = 2 LEAP_FROG",
+                "TemporalSource = RICKER", "PrintLevel = 1"]
+        open(os.path.join(d, "case.ini"), "w").write("\n".join(ini) + "\n")
+        r = subprocess.run([exe, "case.ini", "3"], cwd=d, stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True, timeout=600)
+        mt = re.search(r"Time to complete time iterations : ([0-9.eE+-]+)", r.stdout)
+        if r.returncode != 0 or not mt:
+            return None
+        secs = float(mt.group(1))
+        return dict(value=m.N * steps / secs / 1e9, unit="GDoF-steps/s", cores=1,
+                    kind="reference",
+                    sample=f"Montjoie's own leapfrog (oracle/_ref, g++ -O2), {nx}^3 Q{order} "
+                           f"perturbed layered cube ({m.N} DoF), {steps} steps, time loop "
+                           f"{secs:.2f} s, one core")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {a.gpus}); the multi-GPU sharding "
+              "is a design note (DESIGN.md), not a run", file=sys.stderr)
+        sys.exit(2)
+    import montjoie_amd as M
+    t_setup = time.perf_counter()
+    m, mu, c = workload(a.nx)
+    # dt = 0.5 x CFL limit from FP64 Lanczos on D^{-1} A
+    P = M.WaveProblem(m, 1.0, mu, precision="fp64")
+    lam = P.lambda_max(60, 1)
+    P.close()
+    dt = 0.5 * 2.0 / np.sqrt(lam)
+    print(f"# setup: N={m.N} E={m.E} lambda_max={lam:.6e} dt={dt:.6e} "
+          f"({time.perf_counter() - t_setup:.1f} s)", file=sys.stderr, flush=True)
+    res = {}
+    for prec in ("fp64", "fp32_g64", "fp32"):
+        res[prec] = run_gpu(m, mu, dt, prec, a.steps, a.warmup)
+        r = res[prec]
+        print(f"# {prec}: {r['wall'] / a.steps * 1e3:.4f} ms/step wall, "
+              f"{r['dev_ms_step']:.4f} ms/step device, stiffness launch "
+              f"{r['prof']['stiffness_launch_ms']:.4f} ms", file=sys.stderr, flush=True)
+    ref = res["fp64"]
+    for prec in ("fp32_g64", "fp32"):
+        r = res[prec]
+        r["err_state"] = float(np.linalg.norm(r["uN"] - ref["uN"]) / np.linalg.norm(ref["uN"]))
+        r["err_traces"] = float(np.linalg.norm(r["traces"] - ref["traces"]) /
+                                np.linalg.norm(ref["traces"]))
+    gdof = lambda r: m.N * a.steps / r["wall"] / 1e9  # noqa: E731
+    head = res["fp32"]
+    nd = 216
+    ncol = head["info"]["ncolors"]
+    bytes_step = algorithmic_bytes_per_step(m.E, nd, m.N, 4, 4)
+    launch_bytes = bytes_step / ncol
+    achieved = launch_bytes / (head["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "GDoF-steps/s, Q5 hex acoustic, FP32 vs FP64 at rel-L2 err; gradient/forward cost",
+        "value": gdof(head),
+        "unit": "GDoF-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": head["wall"] / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"3-D acoustic Q5 hexahedra {a.nx}^3 unit cube, vertices "
+                               "perturbed <=0.1h (seed 2024), 3 layers c~U[1,3] (seed 2025), "
+                               "Dirac centre, Ricker f0=8, dt=0.5 CFL",
+                   "n_dof": m.N, "n_elem": m.E, "order": 5, "dt": dt, "shots": 1,
+                   "parallelism": "single GPU"},
+        "fp64": {"value": gdof(ref), "ms_per_step": ref["wall"] / a.steps * 1e3},
+        "fp32_g64": {"value": gdof(res["fp32_g64"]),
+                     "ms_per_step": res["fp32_g64"]["wall"] / a.steps * 1e3,
+                     "rel_l2_state_vs_fp64": res["fp32_g64"]["err_state"],
+                     "rel_l2_traces_vs_fp64": res["fp32_g64"]["err_traces"]},
+        "fp32_rel_l2_state_vs_fp64": head["err_state"],
+        "fp32_rel_l2_traces_vs_fp64": head["err_traces"],
+        "fp32_over_fp64_speedup": gdof(head) / gdof(ref),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_scalar3d<6,EPB,float,float,kLeap> (one colour pass)",
+                     "algorithmic_bytes_per_launch": launch_bytes,
+                     "launch_ms": head["prof"]["stiffness_launch_ms"],
+                     "fp64_launch_ms": ref["prof"]["stiffness_launch_ms"],
+                     "fp64_achieved_GBs": algorithmic_bytes_per_step(m.E, nd, m.N, 8, 8) / ncol /
+                     (ref["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_reference()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,248 @@
+/*
+ * mjwave.h -- C ABI of the MI355X (gfx950) explicit spectral-element wave
+ * solver: continuous Gauss-Lobatto Q_r elements on quadrilaterals and
+ * hexahedra, lumped (diagonal) mass, second-order leapfrog, for the scalar
+ * acoustic and the isotropic elastic wave equations.
+ *
+ * The entry points are the operator / time-scheme plug-in surface that
+ * Montjoie's leapfrog driver calls (paths relative to MONTJOIE/src of the
+ * reference):
+ *   - VirtualOdeSystem<T> operators (Instationary/TimeSchemes.hxx:35-151,
+ *     implemented by VarInstationary_Base, Instationary/VarInstationaryInline.cxx:
+ *     174-242): ApplyOperatorKh / Dh / DhMinusdtSh, SolveOperatorDh /
+ *     DhPlusdtSh, AddScalarTimeSource, SetDirichletCondition;
+ *   - VirtualTimeScheme<T> leapfrog (ModifiedEquationIterator,
+ *     Instationary/ModifiedEquation.hxx:78-114, .cxx:99-124,295-345):
+ *     SetInitialConditionS, Advance, GetIterate;
+ *   - the matrix-free stiffness plug-in FemMatrixFreeClass_Base<T>::MltAddFree
+ *     (Computation/FemMatrixFreeClass.hxx:281-287) with its setup
+ *     (VarGeometryProblem.cxx:648-790, Elliptic/Helmholtz/VarHelmholtz.cxx:53-124,
+ *     Hyperbolic/Acoustic/TimeAcoustic.cxx:889-1004).
+ *
+ * Conventions
+ *   - Every function returns an int status: MJW_OK (0) or a negative code;
+ *     the message is available from mjw_last_error(handle) (thread-local for
+ *     handle == NULL).  The reference aborts instead (e.g. TimeAcoustic.cxx:
+ *     243-249); the C++ adapter include/mjwave_adapter.hpp restores that.
+ *   - Host pointers are caller-owned and never retained.  Functions with the
+ *     suffix _dev take device pointers (hipMalloc / torch CUDA tensors) that
+ *     live on the handle's device and are used on the handle's stream.
+ *   - One handle owns one HIP stream; calls on one handle serialise, distinct
+ *     handles are independent.
+ *   - Global vectors: node-major, components blocked as in Montjoie
+ *     (u[m*N + g], Mesh/NumberMap.cxx:693).  With S > 1 shots each shot is
+ *     one such vector, shots consecutive (u[(s*ncomp + m)*N + g]).
+ *   - Element-local nodes q = i + n*j (+ n*n*k), n = order+1, i along xi_1;
+ *     1-D nodes are the Gauss-Lobatto points of [0,1].
+ */
+#ifndef MJWAVE_H
+#define MJWAVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MJW_OK 0
+#define MJW_ERR_ARG -1
+#define MJW_ERR_HIP -2
+#define MJW_ERR_STATE -3
+#define MJW_ERR_UNSUPPORTED -4
+#define MJW_ERR_NOMEM -5
+#define MJW_ERR_GEOMETRY -6
+
+/* precision of the bulk time stepping */
+#define MJW_FP64 0      /* state, geometry, arithmetic in FP64                        */
+#define MJW_FP32 1      /* FP32 state and arithmetic; geometry computed in FP64,
+                           stored rounded to FP32                                     */
+#define MJW_FP32_G64 2  /* FP32 state; geometry stored and applied in FP64             */
+
+typedef struct mjw_problem mjw_problem;
+
+/* ---------------------------------------------------------------- version */
+int mjw_version(void);
+const char *mjw_last_error(const mjw_problem *p);
+
+/* --------------------------------------------------------------- lifecycle */
+/*
+ * Create a problem on `device`.  Replaces Montjoie's setup chain for the
+ * explicit path: ComputeMeshAndFiniteElement + ComputeMassMatrix
+ * (VarGeometryProblem.cxx:648-790 geometry, VarHelmholtz.cxx:53-124
+ * coefficients, TimeAcoustic.cxx:889-1004 lumped mass) + ComputeStiffnessMatrix
+ * (VarInstationary.cxx:1303-1420).
+ *   dim 2|3, order 1..7, ncomp 1 (acoustic) | 3 (elastic, dim 3 only);
+ *   node_xyz[n_nodes*dim] positions of all global GL nodes;
+ *   elem_nodes[n_elem*(order+1)^dim] element -> global node map;
+ *   rho[n_elem], mu[n_elem] per element; lambda[n_elem] for elasticity
+ *   (NULL otherwise);  dirichlet[n_nodes] 1 on homogeneous-Dirichlet nodes
+ *   (NULL: none);  nshots >= 1 independent right-hand sides sharing the mesh;
+ *   precision MJW_FP64 | MJW_FP32 | MJW_FP32_G64.
+ * Geometric factors J w DF^{-1} mu DF^{-T} (acoustic) or J w, DF^{-1}
+ * (elastic) are computed on the device in FP64 from node_xyz
+ * (isoparametric map, exact for multilinear elements).
+ */
+int mjw_create(mjw_problem **out, int device, int dim, int order, int ncomp, int64_t n_nodes,
+               const double *node_xyz, int64_t n_elem, const int32_t *elem_nodes,
+               const double *rho, const double *mu, const double *lambda,
+               const uint8_t *dirichlet, int nshots, int precision);
+int mjw_destroy(mjw_problem *p);
+
+/* problem sizes: [0]=dim [1]=order [2]=ncomp [3]=n_nodes [4]=n_elem [5]=ncolors
+ * [6]=nshots [7]=precision [8]=device bytes held */
+int mjw_info(const mjw_problem *p, int64_t *info9);
+
+/* Tuning: elements per workgroup of the stiffness kernel (0 = default).
+ * Used by the autotuner (montjoie_amd/tuner.py). */
+int mjw_set_tile(mjw_problem *p, int elems_per_block);
+int mjw_get_tile(const mjw_problem *p, int *elems_per_block);
+
+/* ---------------------------------------------------------------- operators */
+/*
+ * Y = beta*Y + alpha*K*U with K = -A (stiffness), all nodes, one shot
+ * (ApplyOperatorKh, VarInstationaryInline.cxx:174-179 -> MltAddScalarH1,
+ * Computation/ProdMatVectScalarH1.cxx:679-727; elastic MltAddVectorH1,
+ * ProdMatVectH1.cxx:1228-1276).  Vectors of length n_nodes*ncomp, FP64.
+ */
+int mjw_apply_kh(mjw_problem *p, double alpha, const double *U, double beta, double *Y);
+/* same on device vectors in the problem's state precision (double or float) */
+int mjw_apply_kh_dev(mjw_problem *p, double alpha, const void *U, double beta, void *Y);
+
+/* Y = beta*Y + alpha*D*U, D = lumped mass with Dirichlet rows 1
+ * (ApplyOperatorDh, ContinuousUnsteadyMassMatrix.cxx:1329-1346). */
+int mjw_apply_dh(mjw_problem *p, double alpha, const double *U, double beta, double *Y);
+/* Y = D^{-1} Y (SolveOperatorDh / SolveOperatorDhPlusdtSh with S_h = 0,
+ * ContinuousUnsteadyMassMatrix.cxx:1376-1420). */
+int mjw_solve_dh(mjw_problem *p, double *Y);
+/* lumped mass diagonal (n_nodes doubles, Dirichlet rows 1) */
+int mjw_get_mass(mjw_problem *p, double *D);
+
+/* ------------------------------------------------------ sources, receivers */
+/*
+ * Point (Dirac) sources: for source k, shot index shot[k] (NULL: all 0),
+ * position pts[k*dim], polarisation pol[k*ncomp] (NULL: 1).  b_i = pol
+ * phi_i(F^{-1}(x)) averaged over the elements containing x, Dirichlet rows
+ * dropped, |b_i| <= eps ||b|| dropped (AddDiracSourceGen,
+ * Source/DefineSourceElliptic.cxx:2439-2725; VarInstationary.cxx:620-675).
+ */
+int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, const double *pol,
+                    const int32_t *shot);
+/*
+ * Time factor f_n applied at leapfrog step n (the pulse evaluated at loop
+ * time t_n + dt, ModifiedEquation.cxx:316; zero after 2.5/f0,
+ * VarInstationary.cxx:2394).  table[nsteps] FP64, same for all shots.
+ */
+int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table);
+/*
+ * Ricker pulse table through Montjoie's degree-24 Gauss-Lobatto interpolator
+ * of the Ricker primitive (TimeSourceHyperbolic, Source/TimeSource.cxx:
+ * 33-51,818-880; window eps VarInstationary.cxx:993; cut-off :416-420):
+ * out[n] = pulse(t_begin + n*dt + dt), n < nsteps.  Host only, FP64.
+ */
+int mjw_ricker_pulse_table(double f0, double t_begin, double dt, int nsteps, double *out);
+/*
+ * Receivers: u(x_r) = sum_i phi_i(x_r) u_i on component comp[r] (NULL: 0)
+ * (Output/GridInterpolation.cxx SismoPoint / SismoCircle, :836-850,
+ * :2721-2761).  Traces are recorded for every shot.
+ */
+int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts, const int32_t *comp);
+
+/* ---------------------------------------------------------------- leapfrog */
+/*
+ * Set dt and initial state (SetInitialConditionS, ModifiedEquation.cxx:99-124):
+ * U^0 = u0, U^1 = U^0 + dt du0 + dt^2/2 D^{-1}(-A U^0), Dirichlet nodes 0.
+ * u0 / du0: host FP64 [nshots][n_nodes*ncomp] or NULL (zero).  The step
+ * counter is reset to 0 and t_n = t_begin + n*dt.
+ */
+int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, const double *u0,
+                      const double *du0);
+/*
+ * Advance nsteps steps (Advance, ModifiedEquation.cxx:295-345):
+ *   U^{n+1} = 2U^n - U^{n-1} + dt^2 D^{-1}(f_n b - A U^n), Dirichlet rows 0,
+ * recording R U^n before each step (GiveIterate, ModifiedEquation.cxx:44).
+ * traces: host FP64 [nsteps][nshots][nrec] or NULL.  Step loops are captured
+ * in a HIP graph when use_graph != 0.
+ */
+int mjw_leapfrog_advance(mjw_problem *p, int nsteps, double *traces, int use_graph);
+/* traces of the current iterate only (GiveFinalIterate): [nshots][nrec] */
+int mjw_record(mjw_problem *p, double *traces);
+/* current pair (U^n, U^{n+1}) as FP64 host vectors [nshots][n_nodes*ncomp];
+ * either may be NULL (GetIterate / GetIterate(k)). */
+int mjw_get_state(mjw_problem *p, double *Un, double *Un1);
+/* device pointers of U^n and U^{n+1} (state precision, shots consecutive) */
+int mjw_state_dev(mjw_problem *p, void **Un, void **Un1);
+/* current step index n */
+int mjw_get_step(const mjw_problem *p, int64_t *n);
+
+/*
+ * Discrete leapfrog energy per shot, FP64 accumulation:
+ * E^{n+1/2} = 1/2 ||(U^{n+1}-U^n)/dt||_D^2 + 1/2 (U^{n+1})^T A U^n,
+ * conserved by the scheme without source.  Non-finite values are reported
+ * as NaN (blow-up detection, cf. NormeMaxSolution VarInstationary.cxx:362-373).
+ * energy[nshots].
+ */
+int mjw_energy(mjw_problem *p, double *energy);
+
+/*
+ * lambda_max(D^{-1} A) by Lanczos (FP64, device), dt_max = 2/sqrt(lambda)
+ * (EvaluateCFL_Generic, Instationary/EvaluationCflWave.cxx:717-760; the
+ * ARPACK call is replaced by Lanczos).
+ */
+int mjw_lambda_max(mjw_problem *p, int iters, uint64_t seed, double *lambda_max);
+
+/* ------------------------------------------------------- adjoint gradient */
+/*
+ * Gradient of a receiver misfit with respect to per-element mu
+ * (m_e = mu_e = c_e^2 with rho = 1).  No counterpart in the reference (it has
+ * no time-domain adjoint); built on the same kernels.
+ *   mjw_adjoint_forward: runs nsteps leapfrog steps from the initialised state
+ *   keeping nsnap checkpoints (pairs U^k, U^{k+1}) and returns the traces
+ *   [nsteps+1][nshots][nrec] (rows 0..nsteps).
+ *   mjw_adjoint_backward: given dJ/d(trace) with the same shape, runs the
+ *   discrete adjoint leapfrog backward, recomputing forward segments from the
+ *   checkpoints, and accumulates grad[n_elem] (FP64, original element order).
+ */
+int mjw_adjoint_forward(mjw_problem *p, int nsteps, int nsnap, double *traces);
+int mjw_adjoint_backward(mjw_problem *p, const double *dJ_dtrace, double *grad);
+
+/* --------------------------------------------------------- synthetic meshes */
+/*
+ * Regular Q_r mesh of [0,1]^dim with nx elements per direction (Montjoie's
+ * REGULAR mesh, Mesh/Mesh3D.cxx:8406-8486), interior vertices displaced by
+ * at most pert*h (seeded splitmix64; pert = 0: affine mesh).  Node positions
+ * are the multilinear images of the GL nodes.  Sizes from mjw_mesh_regular_size.
+ * Outputs: xyz[N*dim], elem_nodes[E*(r+1)^dim], dirichlet[N] (boundary of the
+ * unit square/cube).
+ */
+int mjw_mesh_regular_size(int dim, int nx, int order, int64_t *n_nodes, int64_t *n_elem);
+int mjw_mesh_regular(int dim, int nx, int order, double pert, uint64_t seed, double *xyz,
+                     int32_t *elem_nodes, uint8_t *dirichlet);
+
+/* ------------------------------------------------------------- batching */
+/*
+ * Many independent problems (different meshes, orders, sizes) advanced
+ * together: each step runs every problem's colour passes as one launch per
+ * colour over a flattened work list, the whole step loop captured in one HIP
+ * graph.  All problems must share the precision; each keeps its own dt,
+ * sources, receivers and state (initialise them with mjw_leapfrog_init).
+ */
+typedef struct mjw_batch mjw_batch;
+int mjw_batch_create(mjw_batch **out, int nprob, mjw_problem *const *probs);
+int mjw_batch_advance(mjw_batch *b, int nsteps, int use_graph);
+int mjw_batch_destroy(mjw_batch *b);
+
+/* ------------------------------------------------------------- timing */
+/* Average device duration (ms) of the most recent leapfrog_advance's step,
+ * measured with HIP events on the problem's stream; [0]=per step,
+ * [1]=stiffness kernels per step.  For benchmarks. */
+int mjw_last_timing(const mjw_problem *p, double *ms2);
+/* Advance nsteps eager steps with a HIP event pair around every launch on the
+ * problem's stream: out[0] = mean stiffness-kernel launch (one colour) ms,
+ * out[1] = mean tail-kernel ms, out[2] = mean step ms, out[3] = launches per
+ * step.  Advances the state like mjw_leapfrog_advance (no traces). */
+int mjw_profile(mjw_problem *p, int nsteps, double *out4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1025 @@
+// C ABI of the wave solver (include/mjwave.h): problem setup, operators,
+// sources / receivers, the fused leapfrog step loop (optionally captured in a
+// HIP graph), energy and CFL.  One problem = one device + one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mjwave.h"
+#include "mjw_internal.hpp"
+
+using namespace mjw;
+
+namespace {
+thread_local std::string g_err;
+}
+
+struct mjw_problem {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int dim = 3, order = 1, n = 2, ncomp = 1, nshots = 1, prec = MJW_FP64;
+  int state = kF64, geom = kG64;
+  int64_t N = 0, E = 0;
+  int nd = 0, P = 0;
+  int ncolor = 0;
+  std::vector<int64_t> coff;    // colour offsets in sorted order
+  std::vector<int64_t> perm;    // sorted -> original element
+  int epb = 0;
+  // host copies (original element order)
+  std::vector<double> xyz;
+  std::vector<int32_t> en;
+  std::vector<uint8_t> dir;
+  std::vector<double> bbox;     // E * 2 * dim
+  std::vector<double> mass;     // N, Dirichlet rows 1
+  // device
+  int32_t *d_map = nullptr;
+  void *d_G = nullptr;
+  double *d_M = nullptr, *d_Minv = nullptr, *d_isqM = nullptr;
+  void *d_sdiag = nullptr;
+  uint8_t *d_dir = nullptr;
+  double *d_lam = nullptr, *d_mu = nullptr, *d_rho = nullptr;
+  void *d_D = nullptr;
+  void *d_U[2] = {nullptr, nullptr};
+  int cur = 0;
+  void *d_tmp = nullptr, *d_tmp2 = nullptr;
+  double *d_part = nullptr, *d_red = nullptr;
+  // leapfrog
+  double dt = 0.0, t_begin = 0.0;
+  bool inited = false;
+  int64_t *d_ctl = nullptr;
+  int64_t step = 0;
+  double *d_pulse = nullptr;
+  int64_t npulse = 0;
+  int nsrc = 0;
+  int64_t *d_src_dof = nullptr;
+  int32_t *d_src_shot = nullptr;
+  double *d_src_val = nullptr;
+  int nrec = 0;
+  int32_t *d_rec_gid = nullptr, *d_rec_comp = nullptr;
+  double *d_rec_w = nullptr;
+  double *d_traces = nullptr;
+  int64_t traces_cap = 0;
+  // graph of an even number of steps, valid for one parity / traces buffer
+  hipGraphExec_t gexec = nullptr;
+  int g_steps = 0, g_parity = -1;
+  double *g_traces = nullptr;
+  bool g_with_traces = false;
+  double ms_step = 0.0, ms_stiff = 0.0;
+  size_t bytes = 0;
+  std::string err;
+  // adjoint
+  std::vector<void *> snaps;
+  int adj_nsteps = 0, adj_nsnap = 0;
+  std::vector<int64_t> snap_step;
+};
+
+#define FAIL(p, code, msg)                  \
+  do {                                      \
+    std::string _m = (msg);                 \
+    if (p) (p)->err = _m; else g_err = _m;  \
+    return (code);                          \
+  } while (0)
+
+#define HIPCHK(p, x)                                                                 \
+  do {                                                                               \
+    hipError_t _e = (x);                                                             \
+    if (_e != hipSuccess)                                                            \
+      FAIL(p, MJW_ERR_HIP, std::string("HIP error: ") + hipGetErrorString(_e) + " at " + \
+                               __FILE__ + ":" + std::to_string(__LINE__));           \
+  } while (0)
+
+#define KCHK(p, x)                                                                   \
+  do {                                                                               \
+    int _e = (x);                                                                    \
+    if (_e != 0)                                                                     \
+      FAIL(p, MJW_ERR_HIP, std::string("kernel launch failed (") + std::to_string(_e) + \
+                               ") at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+static size_t ssize(const mjw_problem *p) { return p->state == kF64 ? 8 : 4; }
+static int64_t nstate(const mjw_problem *p) { return (int64_t)p->nshots * p->ncomp * p->N; }
+
+template <typename T>
+static hipError_t dalloc(mjw_problem *p, T **ptr, size_t bytes) {
+  hipError_t e = hipMalloc((void **)ptr, bytes ? bytes : 8);
+  if (e == hipSuccess) p->bytes += bytes;
+  return e;
+}
+
+static void dfree(void *ptr) {
+  if (ptr) (void)hipFree(ptr);
+}
+
+extern "C" int mjw_version(void) { return 10000; }
+
+extern "C" const char *mjw_last_error(const mjw_problem *p) {
+  return p ? p->err.c_str() : g_err.c_str();
+}
+
+// ------------------------------------------------------------------ meshes
+extern "C" int mjw_mesh_regular_size(int dim, int nx, int order, int64_t *n_nodes,
+                                     int64_t *n_elem) {
+  if ((dim != 2 && dim != 3) || nx < 1 || order < 1 || order > kMaxOrder || !n_nodes || !n_elem)
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_mesh_regular_size: bad arguments");
+  mesh_regular_size(dim, nx, order, n_nodes, n_elem);
+  if (*n_nodes >= (int64_t)1 << 31)
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mesh too large for 31-bit node ids");
+  return MJW_OK;
+}
+
+extern "C" int mjw_mesh_regular(int dim, int nx, int order, double pert, uint64_t seed,
+                                double *xyz, int32_t *elem_nodes, uint8_t *dirichlet) {
+  int64_t N, E;
+  int rc = mjw_mesh_regular_size(dim, nx, order, &N, &E);
+  if (rc) return rc;
+  if (!xyz || !elem_nodes || !dirichlet || !(pert >= 0.0) || pert >= 0.5)
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_mesh_regular: bad arguments");
+  mesh_regular(dim, nx, order, pert, seed, xyz, elem_nodes, dirichlet);
+  return MJW_OK;
+}
+
+extern "C" int mjw_ricker_pulse_table(double f0, double t_begin, double dt, int nsteps,
+                                      double *out) {
+  if (!(f0 > 0) || !(dt > 0) || nsteps < 0 || (!out && nsteps > 0))
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_ricker_pulse_table: bad arguments");
+  ricker_pulse_table(f0, t_begin, dt, nsteps, out);
+  return MJW_OK;
+}
+
+// --------------------------------------------------------------- lifecycle
+static void release(mjw_problem *p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  void *ptrs[] = {p->d_map, p->d_G, p->d_M, p->d_Minv, p->d_isqM, p->d_sdiag, p->d_dir,
+                  p->d_lam, p->d_mu, p->d_rho, p->d_D, p->d_U[0], p->d_U[1], p->d_tmp,
+                  p->d_tmp2, p->d_part, p->d_red, p->d_ctl, p->d_pulse, p->d_src_dof,
+                  p->d_src_shot, p->d_src_val, p->d_rec_gid, p->d_rec_comp, p->d_rec_w,
+                  p->d_traces};
+  for (void *q : ptrs) dfree(q);
+  for (void *q : p->snaps) dfree(q);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+extern "C" int mjw_destroy(mjw_problem *p) {
+  release(p);
+  return MJW_OK;
+}
+
+extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int ncomp,
+                          int64_t n_nodes, const double *node_xyz, int64_t n_elem,
+                          const int32_t *elem_nodes, const double *rho, const double *mu,
+                          const double *lambda, const uint8_t *dirichlet, int nshots,
+                          int precision) {
+  if (!out) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_create: out is NULL");
+  *out = nullptr;
+  if (dim != 2 && dim != 3) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "dim must be 2 or 3");
+  if (order < 1 || order > kMaxOrder)
+    FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED, "order must be in 1..7");
+  if (ncomp != 1 && !(ncomp == 3 && dim == 3))
+    FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED, "ncomp must be 1, or 3 in 3-D");
+  if (n_nodes <= 0 || n_nodes >= ((int64_t)1 << 31) || n_elem <= 0 || !node_xyz || !elem_nodes ||
+      !rho || !mu || (ncomp == 3 && !lambda))
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_create: bad sizes or NULL arrays");
+  if (nshots < 1 || nshots > 4096) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "bad nshots");
+  if (precision != MJW_FP64 && precision != MJW_FP32 && precision != MJW_FP32_G64)
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "bad precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    FAIL((mjw_problem *)nullptr, MJW_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= ndev) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "bad device");
+
+  mjw_problem *p = new mjw_problem();
+  p->device = device;
+  p->dim = dim;
+  p->order = order;
+  p->n = order + 1;
+  p->ncomp = ncomp;
+  p->nshots = nshots;
+  p->prec = precision;
+  p->state = precision == MJW_FP64 ? kF64 : kF32;
+  p->geom = precision == MJW_FP32 ? kG32 : kG64;
+  p->N = n_nodes;
+  p->E = n_elem;
+  p->P = p->n * p->n;
+  p->nd = dim == 2 ? p->P : p->P * p->n;
+  const int nd = p->nd;
+  auto fail = [&](int code, const std::string &m) {
+    g_err = m;
+    release(p);
+    return code;
+  };
+  for (int64_t i = 0; i < n_elem * nd; ++i)
+    if (elem_nodes[i] < 0 || elem_nodes[i] >= n_nodes)
+      return fail(MJW_ERR_ARG, "elem_nodes entry out of range");
+  for (int64_t e = 0; e < n_elem; ++e)
+    if (!(rho[e] > 0) || !std::isfinite(mu[e]) || !(mu[e] > 0) ||
+        (ncomp == 3 && !std::isfinite(lambda[e])))
+      return fail(MJW_ERR_ARG, "media must be finite with rho, mu > 0");
+
+  if (hipSetDevice(device) != hipSuccess) return fail(MJW_ERR_HIP, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(MJW_ERR_HIP, "hipStreamCreate failed");
+
+  // host copies
+  p->xyz.assign(node_xyz, node_xyz + n_nodes * dim);
+  p->en.assign(elem_nodes, elem_nodes + n_elem * nd);
+  p->dir.assign(n_nodes, 0);
+  if (dirichlet)
+    for (int64_t g = 0; g < n_nodes; ++g) p->dir[g] = dirichlet[g] ? 1 : 0;
+  p->bbox.resize(n_elem * 2 * dim);
+  for (int64_t e = 0; e < n_elem; ++e) {
+    double *b = &p->bbox[e * 2 * dim];
+    for (int a = 0; a < dim; ++a) { b[a] = 1e300; b[dim + a] = -1e300; }
+    for (int q = 0; q < nd; ++q) {
+      const double *x = &p->xyz[(int64_t)p->en[e * nd + q] * dim];
+      for (int a = 0; a < dim; ++a) {
+        b[a] = std::min(b[a], x[a]);
+        b[dim + a] = std::max(b[dim + a], x[a]);
+      }
+    }
+  }
+
+  // colouring, colour-sorted order, first-toucher flags, plane-major map
+  std::vector<int> color;
+  const int nc = color_elements(n_elem, nd, n_nodes, elem_nodes, color);
+  if (nc < 0) return fail(MJW_ERR_ARG, "colouring failed (more than 64 colours)");
+  p->ncolor = nc;
+  p->coff.assign(nc + 1, 0);
+  for (int64_t e = 0; e < n_elem; ++e) p->coff[color[e] + 1]++;
+  for (int c = 0; c < nc; ++c) p->coff[c + 1] += p->coff[c];
+  p->perm.resize(n_elem);
+  {
+    std::vector<int64_t> pos(p->coff.begin(), p->coff.end() - 1);
+    for (int64_t e = 0; e < n_elem; ++e) p->perm[pos[color[e]]++] = e;
+  }
+  std::vector<int32_t> hmap((size_t)n_elem * nd);
+  {
+    std::vector<uint8_t> seen(n_nodes, 0);
+    const int K = dim == 2 ? 1 : p->n;
+    for (int64_t es = 0; es < n_elem; ++es) {
+      const int64_t e = p->perm[es];
+      for (int k = 0; k < K; ++k)
+        for (int ij = 0; ij < p->P; ++ij) {
+          const int q = ij + p->P * k;
+          const int32_t g = elem_nodes[e * nd + q];
+          uint32_t v = (uint32_t)g;
+          if (!seen[g]) { seen[g] = 1; v |= 0x80000000u; }
+          hmap[((size_t)k * n_elem + es) * p->P + ij] = (int32_t)v;
+        }
+    }
+  }
+  std::vector<double> rs(n_elem), ms(n_elem), ls(ncomp == 3 ? n_elem : 1);
+  for (int64_t es = 0; es < n_elem; ++es) {
+    rs[es] = rho[p->perm[es]];
+    ms[es] = mu[p->perm[es]];
+    if (ncomp == 3) ls[es] = lambda[p->perm[es]];
+  }
+  // 1-D rule
+  double xg[kMaxN], wg[kMaxN], Dg[kMaxN * kMaxN];
+  gl_rule(order, xg, wg);
+  gl_deriv(order, xg, Dg);
+  std::vector<float> Df(p->n * p->n);
+  for (int i = 0; i < p->n * p->n; ++i) Df[i] = (float)Dg[i];
+
+  const int ngc = ncomp == 3 ? 9 : (dim == 2 ? 3 : 6);
+  const size_t gsz = (size_t)n_elem * nd * ngc * (p->geom == kG64 ? 8 : 4);
+  double *d_xyz = nullptr, *d_Dg = nullptr, *d_w = nullptr, *d_Mloc = nullptr;
+  int *d_bad = nullptr;
+  hipStream_t st = p->stream;
+#define CK(x)                                                                \
+  do {                                                                       \
+    if ((x) != hipSuccess) {                                                 \
+      dfree(d_xyz); dfree(d_Dg); dfree(d_w); dfree(d_Mloc); dfree(d_bad);    \
+      return fail(MJW_ERR_HIP, std::string("setup HIP failure at line ") +   \
+                                   std::to_string(__LINE__));                \
+    }                                                                        \
+  } while (0)
+  CK(dalloc(p, &p->d_map, hmap.size() * 4));
+  CK(hipMemcpyAsync(p->d_map, hmap.data(), hmap.size() * 4, hipMemcpyHostToDevice, st));
+  CK(dalloc(p, &p->d_rho, n_elem * 8));
+  CK(dalloc(p, &p->d_mu, n_elem * 8));
+  CK(hipMemcpyAsync(p->d_rho, rs.data(), n_elem * 8, hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(p->d_mu, ms.data(), n_elem * 8, hipMemcpyHostToDevice, st));
+  if (ncomp == 3) {
+    CK(dalloc(p, &p->d_lam, n_elem * 8));
+    CK(hipMemcpyAsync(p->d_lam, ls.data(), n_elem * 8, hipMemcpyHostToDevice, st));
+  }
+  CK(dalloc(p, &p->d_D, p->n * p->n * ssize(p)));
+  if (p->state == kF64)
+    CK(hipMemcpyAsync(p->d_D, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
+  else
+    CK(hipMemcpyAsync(p->d_D, Df.data(), p->n * p->n * 4, hipMemcpyHostToDevice, st));
+  CK(dalloc(p, &p->d_G, gsz));
+  CK(hipMalloc(&d_xyz, n_nodes * dim * 8));
+  CK(hipMalloc(&d_Dg, p->n * p->n * 8));
+  CK(hipMalloc(&d_w, p->n * 8));
+  CK(hipMalloc(&d_Mloc, (size_t)n_elem * nd * 8));
+  CK(hipMalloc(&d_bad, 4));
+  CK(hipMemsetAsync(d_bad, 0, 4, st));
+  CK(hipMemcpyAsync(d_xyz, node_xyz, n_nodes * dim * 8, hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(d_Dg, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(d_w, wg, p->n * 8, hipMemcpyHostToDevice, st));
+  if (launch_geometry_full(dim, order, ncomp, n_elem, p->d_map, d_xyz, d_Dg, d_w, p->d_rho,
+                           p->d_mu, p->geom == kG64 ? (double *)p->d_G : nullptr,
+                           p->geom == kG32 ? (float *)p->d_G : nullptr, d_Mloc, d_bad, st))
+    CK(hipErrorLaunchFailure);
+  CK(dalloc(p, &p->d_M, n_nodes * 8));
+  for (int c = 0; c < nc; ++c)
+    if (launch_mass_assemble(p->d_map, d_Mloc, n_elem, dim, order, p->coff[c], p->coff[c + 1],
+                             p->d_M, st))
+      CK(hipErrorLaunchFailure);
+  int bad = 0;
+  p->mass.resize(n_nodes);
+  CK(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, st));
+  CK(hipMemcpyAsync(p->mass.data(), p->d_M, n_nodes * 8, hipMemcpyDeviceToHost, st));
+  CK(hipStreamSynchronize(st));
+  dfree(d_xyz); dfree(d_Dg); dfree(d_w); dfree(d_Mloc); dfree(d_bad);
+  d_xyz = d_Dg = d_w = d_Mloc = nullptr;
+  d_bad = nullptr;
+  if (bad) return fail(MJW_ERR_GEOMETRY, "non-positive Jacobian: degenerate or inverted element");
+  // Dirichlet rows of the mass are 1 (TimeAcoustic.cxx:936-937)
+  std::vector<double> minv(n_nodes), isq(n_nodes);
+  for (int64_t g = 0; g < n_nodes; ++g) {
+    if (p->dir[g]) p->mass[g] = 1.0;
+    minv[g] = 1.0 / p->mass[g];
+    isq[g] = p->dir[g] ? 0.0 : 1.0 / std::sqrt(p->mass[g]);
+  }
+  CK(hipMemcpyAsync(p->d_M, p->mass.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
+  CK(dalloc(p, &p->d_Minv, n_nodes * 8));
+  CK(dalloc(p, &p->d_isqM, n_nodes * 8));
+  CK(hipMemcpyAsync(p->d_Minv, minv.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(p->d_isqM, isq.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
+  CK(dalloc(p, &p->d_dir, n_nodes));
+  CK(hipMemcpyAsync(p->d_dir, p->dir.data(), n_nodes, hipMemcpyHostToDevice, st));
+  // state
+  const size_t sb = (size_t)nstate(p) * ssize(p);
+  CK(dalloc(p, &p->d_U[0], sb));
+  CK(dalloc(p, &p->d_U[1], sb));
+  CK(dalloc(p, &p->d_tmp, sb));
+  CK(dalloc(p, &p->d_tmp2, sb));
+  CK(hipMemsetAsync(p->d_U[0], 0, sb, st));
+  CK(hipMemsetAsync(p->d_U[1], 0, sb, st));
+  CK(dalloc(p, &p->d_sdiag, n_nodes * ssize(p)));
+  CK(hipMemsetAsync(p->d_sdiag, 0, n_nodes * ssize(p), st));
+  CK(dalloc(p, &p->d_part, (size_t)reduction_scratch_doubles(nshots) * 8));
+  CK(dalloc(p, &p->d_red, (size_t)4 * nshots * 8));
+  CK(dalloc(p, &p->d_ctl, 2 * 8));
+  CK(hipMemsetAsync(p->d_ctl, 0, 16, st));
+  CK(hipStreamSynchronize(st));
+#undef CK
+  p->epb = default_tile(dim, order, ncomp, p->state);
+  *out = p;
+  return MJW_OK;
+}
+
+extern "C" int mjw_info(const mjw_problem *p, int64_t *info) {
+  if (!p || !info) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_info: NULL");
+  info[0] = p->dim; info[1] = p->order; info[2] = p->ncomp; info[3] = p->N; info[4] = p->E;
+  info[5] = p->ncolor; info[6] = p->nshots; info[7] = p->prec; info[8] = (int64_t)p->bytes;
+  return MJW_OK;
+}
+
+extern "C" int mjw_set_tile(mjw_problem *p, int epb) {
+  if (!p) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL problem");
+  if (epb == 0) { p->epb = default_tile(p->dim, p->order, p->ncomp, p->state); return MJW_OK; }
+  std::vector<int> v = tile_choices(p->dim, p->order, p->ncomp);
+  if (std::find(v.begin(), v.end(), epb) == v.end())
+    FAIL(p, MJW_ERR_ARG, "unsupported elements-per-block for this order");
+  if (epb != p->epb && p->gexec) {
+    (void)hipGraphExecDestroy(p->gexec);
+    p->gexec = nullptr;
+  }
+  p->epb = epb;
+  return MJW_OK;
+}
+
+extern "C" int mjw_get_tile(const mjw_problem *p, int *epb) {
+  if (!p || !epb) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
+  *epb = p->epb;
+  return MJW_OK;
+}
+
+// --------------------------------------------------------------- operators
+static StiffArgs stiff_args(mjw_problem *p, int mode, const void *U, void *Y, int nshots) {
+  StiffArgs a{};
+  a.dim = p->dim; a.order = p->order; a.ncomp = p->ncomp; a.epb = p->epb;
+  a.state = p->state; a.geom = p->geom; a.mode = mode; a.E = p->E;
+  a.map = p->d_map; a.G = p->d_G; a.Dm = p->d_D; a.lam = p->d_lam; a.mu = p->d_mu;
+  a.U = U; a.Y = Y; a.sdiag = p->d_sdiag; a.N = p->N; a.nshots = nshots;
+  a.alpha = 1.0; a.beta = 0.0;
+  return a;
+}
+
+// Y = beta*Y + alpha*A*U over all colours
+static int apply_A(mjw_problem *p, double alpha, const void *U, double beta, void *Y, int nshots) {
+  for (int c = 0; c < p->ncolor; ++c) {
+    // beta is applied once per node, by its first toucher (flag in the map)
+    StiffArgs a = stiff_args(p, kApply, U, Y, nshots);
+    a.alpha = alpha;
+    a.beta = beta;
+    a.e0 = p->coff[c];
+    a.e1 = p->coff[c + 1];
+    int rc = launch_stiffness(a, p->stream);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" int mjw_apply_kh_dev(mjw_problem *p, double alpha, const void *U, double beta,
+                                void *Y) {
+  if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_kh_dev: NULL");
+  HIPCHK(p, hipSetDevice(p->device));
+  KCHK(p, apply_A(p, -alpha, U, beta, Y, p->nshots));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  return MJW_OK;
+}
+
+// host FP64 single vector -> device state precision
+static int upload_vec(mjw_problem *p, const double *h, void *dst, int64_t n) {
+  double *tmp = nullptr;
+  if (p->state == kF64)
+    return (int)hipMemcpyAsync(dst, h, n * 8, hipMemcpyHostToDevice, p->stream);
+  if (hipMalloc(&tmp, n * 8) != hipSuccess) return -1;
+  int rc = (int)hipMemcpyAsync(tmp, h, n * 8, hipMemcpyHostToDevice, p->stream);
+  if (!rc) rc = launch_convert_from_f64(p->state, n, tmp, dst, p->stream);
+  if (!rc) rc = (int)hipStreamSynchronize(p->stream);
+  dfree(tmp);
+  return rc;
+}
+
+static int download_vec(mjw_problem *p, const void *src, double *h, int64_t n) {
+  if (p->state == kF64) {
+    int rc = (int)hipMemcpyAsync(h, src, n * 8, hipMemcpyDeviceToHost, p->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(p->stream);
+    return rc;
+  }
+  double *tmp = nullptr;
+  if (hipMalloc(&tmp, n * 8) != hipSuccess) return -1;
+  int rc = launch_convert_to_f64(p->state, n, src, tmp, p->stream);
+  if (!rc) rc = (int)hipMemcpyAsync(h, tmp, n * 8, hipMemcpyDeviceToHost, p->stream);
+  if (!rc) rc = (int)hipStreamSynchronize(p->stream);
+  dfree(tmp);
+  return rc;
+}
+
+extern "C" int mjw_apply_kh(mjw_problem *p, double alpha, const double *U, double beta,
+                            double *Y) {
+  if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_kh: NULL");
+  HIPCHK(p, hipSetDevice(p->device));
+  const int64_t n = (int64_t)p->ncomp * p->N;
+  KCHK(p, upload_vec(p, U, p->d_tmp, n));
+  if (beta != 0.0)
+    KCHK(p, upload_vec(p, Y, p->d_tmp2, n));
+  else
+    HIPCHK(p, hipMemsetAsync(p->d_tmp2, 0, n * ssize(p), p->stream));
+  KCHK(p, apply_A(p, -alpha, p->d_tmp, beta, p->d_tmp2, 1));
+  KCHK(p, download_vec(p, p->d_tmp2, Y, n));
+  return MJW_OK;
+}
+
+extern "C" int mjw_apply_dh(mjw_problem *p, double alpha, const double *U, double beta,
+                            double *Y) {
+  if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_dh: NULL");
+  const int64_t n = (int64_t)p->ncomp * p->N;
+  for (int64_t i = 0; i < n; ++i) {
+    const double d = alpha * (p->mass[i % p->N] * U[i]);
+    Y[i] = beta == 0.0 ? d : beta * Y[i] + d;
+  }
+  return MJW_OK;
+}
+
+extern "C" int mjw_solve_dh(mjw_problem *p, double *Y) {
+  if (!p || !Y) FAIL(p, MJW_ERR_ARG, "mjw_solve_dh: NULL");
+  const int64_t n = (int64_t)p->ncomp * p->N;
+  for (int64_t i = 0; i < n; ++i) Y[i] *= 1.0 / p->mass[i % p->N];
+  return MJW_OK;
+}
+
+extern "C" int mjw_get_mass(mjw_problem *p, double *D) {
+  if (!p || !D) FAIL(p, MJW_ERR_ARG, "mjw_get_mass: NULL");
+  std::copy(p->mass.begin(), p->mass.end(), D);
+  return MJW_OK;
+}
+
+// ------------------------------------------------------ sources, receivers
+// elements whose bounding box contains x (with a relative tolerance)
+static bool in_bbox(const mjw_problem *p, int64_t e, const double *x) {
+  const double *b = &p->bbox[e * 2 * p->dim];
+  double h = 0;
+  for (int a = 0; a < p->dim; ++a) h = std::max(h, b[p->dim + a] - b[a]);
+  const double tol = 1e-9 * h;
+  for (int a = 0; a < p->dim; ++a)
+    if (x[a] < b[a] - tol || x[a] > b[p->dim + a] + tol) return false;
+  return true;
+}
+
+static bool locate_in(const mjw_problem *p, int64_t e, const double *x, double *phi) {
+  std::vector<double> xq((size_t)p->nd * p->dim);
+  for (int q = 0; q < p->nd; ++q)
+    for (int a = 0; a < p->dim; ++a)
+      xq[(size_t)q * p->dim + a] = p->xyz[(int64_t)p->en[e * p->nd + q] * p->dim + a];
+  double xi[3];
+  if (!locate_point(p->dim, p->order, xq.data(), x, xi)) return false;
+  tensor_basis(p->dim, p->order, xi, phi);
+  return true;
+}
+
+extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, const double *pol,
+                               const int32_t *shot) {
+  if (!p || nsrc < 0 || (nsrc > 0 && !pts)) FAIL(p, MJW_ERR_ARG, "mjw_set_sources: bad args");
+  HIPCHK(p, hipSetDevice(p->device));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  std::vector<int64_t> dof;
+  std::vector<int32_t> sh;
+  std::vector<double> val;
+  std::vector<double> phi(p->nd);
+  for (int k = 0; k < nsrc; ++k) {
+    const int s = shot ? shot[k] : 0;
+    if (s < 0 || s >= p->nshots) FAIL(p, MJW_ERR_ARG, "source shot index out of range");
+    const double *x = pts + (size_t)k * p->dim;
+    std::unordered_map<int64_t, double> b;  // node -> sum of phi
+    int cnt = 0;
+    for (int64_t e = 0; e < p->E; ++e) {
+      if (!in_bbox(p, e, x)) continue;
+      if (!locate_in(p, e, x, phi.data())) continue;
+      for (int q = 0; q < p->nd; ++q) b[p->en[e * p->nd + q]] += phi[q];
+      ++cnt;
+    }
+    if (cnt == 0) FAIL(p, MJW_ERR_ARG, "source point " + std::to_string(k) + " outside the mesh");
+    // average over the elements holding the point (DefineSourceElliptic.cxx:2712-2725),
+    // drop Dirichlet rows and values below eps*||b|| (VarInstationary.cxx:637-658)
+    std::vector<std::pair<int64_t, double>> ent(b.begin(), b.end());
+    std::sort(ent.begin(), ent.end());
+    double nrm = 0.0;
+    for (int m = 0; m < p->ncomp; ++m) {
+      const double pm = pol ? pol[(size_t)k * p->ncomp + m] : 1.0;
+      for (auto &kv : ent) {
+        double v = pm * kv.second;
+        if (cnt > 1) v *= 1.0 / cnt;
+        if (p->dir[kv.first]) v = 0.0;
+        nrm += v * v;
+      }
+    }
+    const double thr = 2.220446049250313e-16 * std::sqrt(nrm);
+    for (int m = 0; m < p->ncomp; ++m) {
+      const double pm = pol ? pol[(size_t)k * p->ncomp + m] : 1.0;
+      for (auto &kv : ent) {
+        double v = pm * kv.second;
+        if (cnt > 1) v *= 1.0 / cnt;
+        if (p->dir[kv.first] || std::fabs(v) <= thr) continue;
+        dof.push_back((int64_t)m * p->N + kv.first);
+        sh.push_back(s);
+        val.push_back(v);
+      }
+    }
+  }
+  dfree(p->d_src_dof); dfree(p->d_src_shot); dfree(p->d_src_val);
+  p->d_src_dof = nullptr; p->d_src_shot = nullptr; p->d_src_val = nullptr;
+  p->nsrc = (int)dof.size();
+  if (p->nsrc) {
+    HIPCHK(p, dalloc(p, &p->d_src_dof, dof.size() * 8));
+    HIPCHK(p, dalloc(p, &p->d_src_shot, sh.size() * 4));
+    HIPCHK(p, dalloc(p, &p->d_src_val, val.size() * 8));
+    HIPCHK(p, hipMemcpy(p->d_src_dof, dof.data(), dof.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(p, hipMemcpy(p->d_src_shot, sh.data(), sh.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(p, hipMemcpy(p->d_src_val, val.data(), val.size() * 8, hipMemcpyHostToDevice));
+  }
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  return MJW_OK;
+}
+
+extern "C" int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table) {
+  if (!p || nsteps < 0 || (nsteps > 0 && !table)) FAIL(p, MJW_ERR_ARG, "mjw_set_pulse: bad args");
+  HIPCHK(p, hipSetDevice(p->device));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  dfree(p->d_pulse);
+  p->d_pulse = nullptr;
+  p->npulse = nsteps;
+  if (nsteps) {
+    HIPCHK(p, dalloc(p, &p->d_pulse, (size_t)nsteps * 8));
+    HIPCHK(p, hipMemcpy(p->d_pulse, table, (size_t)nsteps * 8, hipMemcpyHostToDevice));
+  }
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  return MJW_OK;
+}
+
+extern "C" int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts,
+                                 const int32_t *comp) {
+  if (!p || nrec < 0 || (nrec > 0 && !pts)) FAIL(p, MJW_ERR_ARG, "mjw_set_receivers: bad args");
+  HIPCHK(p, hipSetDevice(p->device));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  std::vector<int32_t> gid((size_t)nrec * p->nd), cm(nrec, 0);
+  std::vector<double> w((size_t)nrec * p->nd);
+  for (int r = 0; r < nrec; ++r) {
+    const double *x = pts + (size_t)r * p->dim;
+    cm[r] = comp ? comp[r] : 0;
+    if (cm[r] < 0 || cm[r] >= p->ncomp) FAIL(p, MJW_ERR_ARG, "receiver component out of range");
+    int64_t found = -1;
+    for (int64_t e = 0; e < p->E && found < 0; ++e) {
+      if (!in_bbox(p, e, x)) continue;
+      if (locate_in(p, e, x, &w[(size_t)r * p->nd])) found = e;
+    }
+    if (found < 0) FAIL(p, MJW_ERR_ARG, "receiver " + std::to_string(r) + " outside the mesh");
+    for (int q = 0; q < p->nd; ++q) gid[(size_t)r * p->nd + q] = p->en[found * p->nd + q];
+  }
+  dfree(p->d_rec_gid); dfree(p->d_rec_w); dfree(p->d_rec_comp);
+  p->d_rec_gid = nullptr; p->d_rec_w = nullptr; p->d_rec_comp = nullptr;
+  p->nrec = nrec;
+  if (nrec) {
+    HIPCHK(p, dalloc(p, &p->d_rec_gid, gid.size() * 4));
+    HIPCHK(p, dalloc(p, &p->d_rec_w, w.size() * 8));
+    HIPCHK(p, dalloc(p, &p->d_rec_comp, cm.size() * 4));
+    HIPCHK(p, hipMemcpy(p->d_rec_gid, gid.data(), gid.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(p, hipMemcpy(p->d_rec_w, w.data(), w.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(p, hipMemcpy(p->d_rec_comp, cm.data(), cm.size() * 4, hipMemcpyHostToDevice));
+  }
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  return MJW_OK;
+}
+
+// ---------------------------------------------------------------- leapfrog
+extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, const double *u0,
+                                 const double *du0) {
+  if (!p || !(dt > 0.0) || !std::isfinite(dt)) FAIL(p, MJW_ERR_ARG, "mjw_leapfrog_init: bad dt");
+  HIPCHK(p, hipSetDevice(p->device));
+  hipStream_t st = p->stream;
+  p->dt = dt;
+  p->t_begin = t_begin;
+  const int64_t ns = nstate(p);
+  const int64_t n1 = (int64_t)p->ncomp * p->N;
+  KCHK(p, launch_make_sdiag(p->state, p->N, p->d_M, p->d_dir, dt * dt, p->d_sdiag, st));
+  p->cur = 0;
+  void *U0 = p->d_U[0], *U1 = p->d_U[1];
+  if (u0) {
+    for (int s = 0; s < p->nshots; ++s)
+      KCHK(p, upload_vec(p, u0 + (size_t)s * n1, (char *)U0 + (size_t)s * n1 * ssize(p), n1));
+  } else {
+    HIPCHK(p, hipMemsetAsync(U0, 0, ns * ssize(p), st));
+  }
+  KCHK(p, launch_mask(p->state, ns, p->nshots, p->ncomp, p->N, p->d_dir, U0, st));
+  // U1 = ((0.5 dt) D^{-1}(-A U0) + du0) dt + U0   (ModifiedEquation.cxx:112-118)
+  KCHK(p, apply_A(p, -1.0, U0, 0.0, p->d_tmp, p->nshots));
+  KCHK(p, launch_scale_by(p->state, ns, p->nshots, p->ncomp, p->N, p->d_Minv, 0.5 * dt, p->d_tmp,
+                          st));
+  if (du0) {
+    for (int s = 0; s < p->nshots; ++s)
+      KCHK(p, upload_vec(p, du0 + (size_t)s * n1, (char *)p->d_tmp2 + (size_t)s * n1 * ssize(p),
+                         n1));
+    KCHK(p, launch_axpby(p->state, ns, 1.0, p->d_tmp2, 1.0, p->d_tmp, st));
+  }
+  HIPCHK(p, hipMemcpyAsync(U1, U0, ns * ssize(p), hipMemcpyDeviceToDevice, st));
+  KCHK(p, launch_axpby(p->state, ns, dt, p->d_tmp, 1.0, U1, st));
+  KCHK(p, launch_mask(p->state, ns, p->nshots, p->ncomp, p->N, p->d_dir, U1, st));
+  p->step = 0;
+  int64_t ctl[2] = {0, 0};
+  HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, st));
+  HIPCHK(p, hipStreamSynchronize(st));
+  p->inited = true;
+  return MJW_OK;
+}
+
+// one fused step: colour passes (U^{n+1} over U^{n-1}) + tail
+static int enqueue_step(mjw_problem *p, int parity, double *traces) {
+  void *U = p->d_U[parity], *Y = p->d_U[parity ^ 1];
+  for (int c = 0; c < p->ncolor; ++c) {
+    StiffArgs a = stiff_args(p, kLeap, U, Y, p->nshots);
+    a.e0 = p->coff[c];
+    a.e1 = p->coff[c + 1];
+    int rc = launch_stiffness(a, p->stream);
+    if (rc) return rc;
+  }
+  TailArgs t{};
+  t.state = p->state; t.ctl = p->d_ctl; t.pulse = p->d_pulse; t.npulse = p->npulse;
+  t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot; t.src_val = p->d_src_val;
+  t.sdiag = p->d_sdiag; t.nrec = traces ? p->nrec : 0; t.nd = p->nd; t.rec_gid = p->d_rec_gid;
+  t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = U; t.P = Y;
+  t.traces = (traces && p->nrec) ? traces : nullptr; t.N = p->N; t.nshots = p->nshots;
+  t.advance = 1;
+  return launch_tail_n(t, p->ncomp, p->stream);
+}
+
+static int ensure_traces(mjw_problem *p, int64_t nsteps) {
+  const int64_t need = nsteps * p->nshots * std::max(p->nrec, 1);
+  if (need <= p->traces_cap) return 0;
+  dfree(p->d_traces);
+  p->d_traces = nullptr;
+  p->traces_cap = 0;
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (hipMalloc(&p->d_traces, need * 8) != hipSuccess) return -1;
+  p->traces_cap = need;
+  return 0;
+}
+
+static const int kGraphSteps = 16;  // even
+
+static int build_graph(mjw_problem *p, bool with_traces) {
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return (int)e;
+  int rc = 0;
+  for (int s = 0; s < kGraphSteps && !rc; ++s)
+    rc = enqueue_step(p, (p->cur + s) & 1, with_traces ? p->d_traces : nullptr);
+  e = hipStreamEndCapture(p->stream, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return (int)e;
+  e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) { p->gexec = nullptr; return (int)e; }
+  p->g_steps = kGraphSteps;
+  p->g_parity = p->cur;
+  p->g_traces = p->d_traces;
+  p->g_with_traces = with_traces;
+  return 0;
+}
+
+extern "C" int mjw_leapfrog_advance(mjw_problem *p, int nsteps, double *traces, int use_graph) {
+  if (!p || nsteps < 0) FAIL(p, MJW_ERR_ARG, "mjw_leapfrog_advance: bad args");
+  if (!p->inited) FAIL(p, MJW_ERR_STATE, "mjw_leapfrog_init has not been called");
+  if (nsteps == 0) return MJW_OK;
+  HIPCHK(p, hipSetDevice(p->device));
+  hipStream_t st = p->stream;
+  const bool want = traces && p->nrec > 0;
+  if (want) KCHK(p, ensure_traces(p, nsteps));
+  int64_t ctl[2] = {p->step, p->step};
+  HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, st));
+  hipEvent_t ev0, ev1;
+  HIPCHK(p, hipEventCreate(&ev0));
+  HIPCHK(p, hipEventCreate(&ev1));
+  HIPCHK(p, hipEventRecord(ev0, st));
+  int done = 0;
+  if (use_graph && nsteps >= kGraphSteps) {
+    if (p->cur != 0) {  // graphs are captured at even parity
+      KCHK(p, enqueue_step(p, p->cur, want ? p->d_traces : nullptr));
+      p->cur ^= 1;
+      ++done;
+    }
+    if (nsteps - done >= kGraphSteps) {
+      if (!p->gexec || p->g_parity != p->cur || p->g_with_traces != want ||
+          (want && p->g_traces != p->d_traces))
+        KCHK(p, build_graph(p, want));
+      while (nsteps - done >= kGraphSteps) {
+        HIPCHK(p, hipGraphLaunch(p->gexec, st));
+        done += kGraphSteps;
+      }
+    }
+  }
+  for (; done < nsteps; ++done) {
+    KCHK(p, enqueue_step(p, p->cur, want ? p->d_traces : nullptr));
+    p->cur ^= 1;
+  }
+  HIPCHK(p, hipEventRecord(ev1, st));
+  HIPCHK(p, hipEventSynchronize(ev1));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, ev0, ev1);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  p->ms_step = ms / nsteps;
+  p->step += nsteps;
+  if (want)
+    HIPCHK(p, hipMemcpy(traces, p->d_traces, (size_t)nsteps * p->nshots * p->nrec * 8,
+                        hipMemcpyDeviceToHost));
+  return MJW_OK;
+}
+
+extern "C" int mjw_record(mjw_problem *p, double *traces) {
+  if (!p || !traces) FAIL(p, MJW_ERR_ARG, "mjw_record: NULL");
+  if (!p->inited) FAIL(p, MJW_ERR_STATE, "not initialised");
+  if (p->nrec == 0) return MJW_OK;
+  HIPCHK(p, hipSetDevice(p->device));
+  KCHK(p, ensure_traces(p, 1));
+  int64_t ctl[2] = {p->step, p->step};
+  HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, p->stream));
+  TailArgs t{};
+  t.state = p->state; t.ctl = p->d_ctl; t.nrec = p->nrec; t.nd = p->nd; t.rec_gid = p->d_rec_gid;
+  t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = p->d_U[p->cur]; t.P = nullptr;
+  t.traces = p->d_traces; t.N = p->N; t.nshots = p->nshots; t.advance = 0;
+  KCHK(p, launch_tail_n(t, p->ncomp, p->stream));
+  HIPCHK(p, hipMemcpyAsync(traces, p->d_traces, (size_t)p->nshots * p->nrec * 8,
+                           hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  return MJW_OK;
+}
+
+extern "C" int mjw_get_state(mjw_problem *p, double *Un, double *Un1) {
+  if (!p) FAIL(p, MJW_ERR_ARG, "NULL problem");
+  HIPCHK(p, hipSetDevice(p->device));
+  const int64_t ns = nstate(p);
+  if (Un) KCHK(p, download_vec(p, p->d_U[p->cur], Un, ns));
+  if (Un1) KCHK(p, download_vec(p, p->d_U[p->cur ^ 1], Un1, ns));
+  return MJW_OK;
+}
+
+extern "C" int mjw_state_dev(mjw_problem *p, void **Un, void **Un1) {
+  if (!p) FAIL(p, MJW_ERR_ARG, "NULL problem");
+  if (Un) *Un = p->d_U[p->cur];
+  if (Un1) *Un1 = p->d_U[p->cur ^ 1];
+  return MJW_OK;
+}
+
+extern "C" int mjw_get_step(const mjw_problem *p, int64_t *n) {
+  if (!p || !n) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
+  *n = p->step;
+  return MJW_OK;
+}
+
+extern "C" int mjw_last_timing(const mjw_problem *p, double *ms2) {
+  if (!p || !ms2) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
+  ms2[0] = p->ms_step;
+  ms2[1] = p->ms_stiff;
+  return MJW_OK;
+}
+
+extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
+  if (!p || !out || nsteps < 1) FAIL(p, MJW_ERR_ARG, "mjw_profile: bad args");
+  if (!p->inited) FAIL(p, MJW_ERR_STATE, "not initialised");
+  HIPCHK(p, hipSetDevice(p->device));
+  hipStream_t st = p->stream;
+  const int nev = nsteps * (p->ncolor + 1) + 1;
+  std::vector<hipEvent_t> ev(nev);
+  for (auto &e : ev) HIPCHK(p, hipEventCreate(&e));
+  int64_t ctl[2] = {p->step, p->step};
+  HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, st));
+  int k = 0;
+  HIPCHK(p, hipEventRecord(ev[k++], st));
+  for (int s = 0; s < nsteps; ++s) {
+    void *U = p->d_U[p->cur], *Y = p->d_U[p->cur ^ 1];
+    for (int c = 0; c < p->ncolor; ++c) {
+      StiffArgs a = stiff_args(p, kLeap, U, Y, p->nshots);
+      a.e0 = p->coff[c];
+      a.e1 = p->coff[c + 1];
+      KCHK(p, launch_stiffness(a, st));
+      HIPCHK(p, hipEventRecord(ev[k++], st));
+    }
+    TailArgs t{};
+    t.state = p->state; t.ctl = p->d_ctl; t.pulse = p->d_pulse; t.npulse = p->npulse;
+    t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot;
+    t.src_val = p->d_src_val; t.sdiag = p->d_sdiag; t.nrec = 0; t.nd = p->nd; t.U = U; t.P = Y;
+    t.N = p->N; t.nshots = p->nshots; t.advance = 1;
+    KCHK(p, launch_tail_n(t, p->ncomp, st));
+    HIPCHK(p, hipEventRecord(ev[k++], st));
+    p->cur ^= 1;
+  }
+  HIPCHK(p, hipEventSynchronize(ev[k - 1]));
+  double stiff = 0, tail = 0;
+  k = 1;
+  for (int s = 0; s < nsteps; ++s) {
+    for (int c = 0; c < p->ncolor; ++c, ++k) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, ev[k - 1], ev[k]);
+      stiff += ms;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[k - 1], ev[k]);
+    tail += ms;
+    ++k;
+  }
+  float tot = 0;
+  (void)hipEventElapsedTime(&tot, ev[0], ev[nev - 1]);
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  p->step += nsteps;
+  out[0] = stiff / (nsteps * (double)p->ncolor);
+  out[1] = tail / nsteps;
+  out[2] = tot / nsteps;
+  out[3] = p->ncolor + 1;
+  p->ms_stiff = stiff / nsteps;
+  return MJW_OK;
+}
+
+extern "C" int mjw_energy(mjw_problem *p, double *energy) {
+  if (!p || !energy) FAIL(p, MJW_ERR_ARG, "mjw_energy: NULL");
+  if (!p->inited) FAIL(p, MJW_ERR_STATE, "not initialised");
+  HIPCHK(p, hipSetDevice(p->device));
+  hipStream_t st = p->stream;
+  const int S = p->nshots;
+  void *Un = p->d_U[p->cur], *Un1 = p->d_U[p->cur ^ 1];
+  KCHK(p, apply_A(p, 1.0, Un, 0.0, p->d_tmp, S));
+  KCHK(p, launch_dot_part(p->state, p->N, p->ncomp, S, nullptr, Un1, p->d_tmp, p->d_red,
+                          p->d_part, st));
+  KCHK(p, launch_dot_diff_part(p->state, p->N, p->ncomp, S, p->d_M, Un1, Un, p->d_red + S,
+                               p->d_part, st));
+  std::vector<double> h(2 * S);
+  HIPCHK(p, hipMemcpyAsync(h.data(), p->d_red, 2 * S * 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(p, hipStreamSynchronize(st));
+  for (int s = 0; s < S; ++s) {
+    const double e = 0.5 * h[S + s] / (p->dt * p->dt) + 0.5 * h[s];
+    energy[s] = std::isfinite(e) ? e : std::numeric_limits<double>::quiet_NaN();
+  }
+  return MJW_OK;
+}
+
+// largest eigenvalue of a symmetric tridiagonal matrix (Sturm bisection)
+static double tri_max(const std::vector<double> &al, const std::vector<double> &be) {
+  const int m = (int)al.size();
+  double lo = 1e300, hi = -1e300;
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? std::fabs(be[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(be[i]) : 0.0);
+    lo = std::min(lo, al[i] - r);
+    hi = std::max(hi, al[i] + r);
+  }
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    int below = 0;
+    double d = 1.0;
+    for (int i = 0; i < m; ++i) {
+      d = al[i] - mid - (i > 0 ? be[i - 1] * be[i - 1] / d : 0.0);
+      if (d == 0.0) d = -1e-300;
+      if (d < 0) ++below;
+    }
+    if (below == m) hi = mid; else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+extern "C" int mjw_lambda_max(mjw_problem *p, int iters, uint64_t seed, double *lambda_max) {
+  if (!p || !lambda_max || iters < 2) FAIL(p, MJW_ERR_ARG, "mjw_lambda_max: bad args");
+  HIPCHK(p, hipSetDevice(p->device));
+  hipStream_t st = p->stream;
+  const int64_t n1 = (int64_t)p->ncomp * p->N;
+  // Lanczos on D^{-1/2} A D^{-1/2} restricted to free nodes, single vector,
+  // vectors in the state precision, scalars in FP64.
+  void *q = nullptr, *qo = nullptr, *w = nullptr, *t = nullptr;
+  const size_t b = n1 * ssize(p);
+  HIPCHK(p, hipMalloc(&q, b));
+  HIPCHK(p, hipMalloc(&qo, b));
+  HIPCHK(p, hipMalloc(&w, b));
+  HIPCHK(p, hipMalloc(&t, b));
+  std::vector<double> h0(n1);
+  uint64_t z = seed * 0x9E3779B97F4A7C15ULL + 12345;
+  for (int64_t i = 0; i < n1; ++i) {
+    z ^= z << 13; z ^= z >> 7; z ^= z << 17;
+    h0[i] = p->dir[i % p->N] ? 0.0 : ((double)(z >> 11) / 9007199254740992.0 - 0.5);
+  }
+  double nrm = 0;
+  for (double v : h0) nrm += v * v;
+  nrm = std::sqrt(nrm);
+  for (double &v : h0) v /= nrm;
+  int rc = upload_vec(p, h0.data(), q, n1);
+  if (!rc) rc = (int)hipMemsetAsync(qo, 0, b, st);
+  std::vector<double> al, be;
+  double beta = 0.0, red[2];
+  for (int k = 0; k < iters && !rc; ++k) {
+    rc = (int)hipMemcpyAsync(t, q, b, hipMemcpyDeviceToDevice, st);
+    if (!rc) rc = launch_scale_by(p->state, n1, 1, p->ncomp, p->N, p->d_isqM, 1.0, t, st);
+    if (!rc) rc = apply_A(p, 1.0, t, 0.0, w, 1);
+    if (!rc) rc = launch_scale_by(p->state, n1, 1, p->ncomp, p->N, p->d_isqM, 1.0, w, st);
+    if (!rc) rc = launch_dot_part(p->state, p->N, p->ncomp, 1, nullptr, w, q, p->d_red, p->d_part, st);
+    if (!rc) rc = (int)hipMemcpyAsync(red, p->d_red, 8, hipMemcpyDeviceToHost, st);
+    if (!rc) rc = (int)hipStreamSynchronize(st);
+    if (rc) break;
+    const double alpha = red[0];
+    al.push_back(alpha);
+    // w = w - alpha q - beta qo
+    rc = launch_axpby(p->state, n1, -alpha, q, 1.0, w, st);
+    if (!rc) rc = launch_axpby(p->state, n1, -beta, qo, 1.0, w, st);
+    if (!rc) rc = launch_dot_part(p->state, p->N, p->ncomp, 1, nullptr, w, w, p->d_red, p->d_part, st);
+    if (!rc) rc = (int)hipMemcpyAsync(red, p->d_red, 8, hipMemcpyDeviceToHost, st);
+    if (!rc) rc = (int)hipStreamSynchronize(st);
+    if (rc) break;
+    beta = std::sqrt(red[0]);
+    if (k + 1 == iters || beta < 1e-300) break;
+    be.push_back(beta);
+    std::swap(q, qo);  // qo <- q
+    rc = launch_axpby(p->state, n1, 1.0 / beta, w, 0.0, q, st);
+  }
+  dfree(q); dfree(qo); dfree(w); dfree(t);
+  if (rc) FAIL(p, MJW_ERR_HIP, "Lanczos iteration failed");
+  be.resize(al.size() > 0 ? al.size() - 1 : 0);
+  *lambda_max = tri_max(al, be);
+  return MJW_OK;
+}
+
+// ------------------------------------------------- adjoint (not yet built)
+extern "C" int mjw_adjoint_forward(mjw_problem *p, int, int, double *) {
+  FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint not available in this build");
+}
+extern "C" int mjw_adjoint_backward(mjw_problem *p, const double *, double *) {
+  FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint not available in this build");
+}
+
+// ------------------------------------------------------- batch (not yet built)
+struct mjw_batch {
+  std::vector<mjw_problem *> probs;
+};
+extern "C" int mjw_batch_create(mjw_batch **out, int, mjw_problem *const *) {
+  if (out) *out = nullptr;
+  FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED, "batching not available in this build");
+}
+extern "C" int mjw_batch_advance(mjw_batch *, int, int) {
+  FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED, "batching not available in this build");
+}
+extern "C" int mjw_batch_destroy(mjw_batch *b) {
+  delete b;
+  return MJW_OK;
+}

@@ -1,0 +1,137 @@
+// Internal declarations shared by the host setup (mjw_host.cpp), the kernels
+// (mjw_kernels.hip) and the C ABI (mjw_api.hip).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mjw {
+
+constexpr int kMaxOrder = 7;      // Q1..Q7 elements (n = order+1 <= 8)
+constexpr int kMaxN = kMaxOrder + 1;
+
+// ----------------------------------------------------------------- host math
+// Gauss-Lobatto rule of [0,1] for `order` (order+1 points, weights sum 1).
+void gl_rule(int order, double *x, double *w);
+// D[a*n+b] = phi_b'(x_a) for the GL Lagrange basis.
+void gl_deriv(int order, const double *x, double *D);
+// Lagrange basis values phi_i(t) and derivatives on the GL nodes.
+void lagrange(int order, const double *x, double t, double *phi, double *dphi);
+// Ricker pulse table through the degree-24 GL interpolator.
+void ricker_pulse_table(double f0, double t_begin, double dt, int nsteps, double *out);
+
+// ----------------------------------------------------------------- host mesh
+void mesh_regular_size(int dim, int nx, int order, int64_t *N, int64_t *E);
+void mesh_regular(int dim, int nx, int order, double pert, uint64_t seed, double *xyz,
+                  int32_t *elem_nodes, uint8_t *dirichlet);
+
+// Greedy first-fit colouring of elements (no two elements of one colour share
+// a node).  Returns number of colours or -1.
+int color_elements(int64_t E, int nd, int64_t N, const int32_t *elem_nodes,
+                   std::vector<int> &color);
+
+// Point location in an isoparametric Q_r element: Newton on
+// x(xi) = sum_q x_q phi_q(xi).  Returns true when the reference point lies in
+// [-tol, 1+tol]^dim (xi clamped to [0,1]).
+bool locate_point(int dim, int order, const double *xq /*nd*dim*/, const double *p,
+                  double *xi);
+// tensor basis values phi_q(xi) (nd values)
+void tensor_basis(int dim, int order, const double *xi, double *phi);
+
+// ------------------------------------------------------------ device launch
+enum StateType { kF64 = 0, kF32 = 1 };
+enum GeomType { kG64 = 0, kG32 = 1 };
+enum Mode { kApply = 0, kLeap = 1 };
+
+struct StiffArgs {
+  int dim, order, ncomp, epb;         // geometry, elements per block
+  int state;                          // StateType
+  int geom;                           // GeomType
+  int mode;                           // Mode
+  int64_t E;                          // plane stride (all elements)
+  int64_t e0, e1;                     // element range (one colour)
+  const int32_t *map;                 // plane-major [K][E][P], bit31 = first toucher
+  const void *G;                      // geometry planes
+  const void *Dm;                     // n*n derivative matrix, state precision
+  const double *lam, *mu;             // elastic per element (sorted order), FP64
+  const void *U;                      // state in
+  void *Y;                            // out (apply) / previous->next (leapfrog)
+  const void *sdiag;                  // dt^2 / M per node (leapfrog)
+  double alpha, beta;                 // apply
+  int64_t N;                          // nodes
+  int nshots;
+};
+// returns 0 or a HIP error code; `stream` is a hipStream_t
+int launch_stiffness(const StiffArgs &a, void *stream);
+// list of supported elements-per-block for (dim, order, ncomp)
+std::vector<int> tile_choices(int dim, int order, int ncomp);
+int default_tile(int dim, int order, int ncomp, int state);
+
+// FP64 geometry: acoustic tensor or elastic sqrt(Jw) DF^{-1} planes, Mloc[E*nd]
+int launch_geometry_full(int dim, int order, int ncomp, int64_t E, const int32_t *map,
+                         const double *xyz, const double *D, const double *w, const double *rho,
+                         const double *mu, double *G64, float *G32, double *Mloc, int *bad,
+                         void *stream);
+// mass assembly of one colour range: M[g] (=|+=) Mloc
+int launch_mass_assemble(const int32_t *map, const double *Mloc, int64_t E, int dim, int order,
+                         int64_t e0, int64_t e1, double *M, void *stream);
+
+struct TailArgs {
+  int state;
+  int64_t *ctl;                  // [0] = step, [1] = base
+  const double *pulse;           // table
+  int64_t npulse;
+  int nsrc;                      // entries
+  const int64_t *src_dof;        // dof (m*N+g)
+  const int32_t *src_shot;
+  const double *src_val;
+  const void *sdiag;             // state precision, per node
+  int nrec, nd;
+  const int32_t *rec_gid;        // nrec*nd global node ids
+  const double *rec_w;           // nrec*nd
+  const int32_t *rec_comp;
+  const void *U;                 // U^n
+  void *P;                       // U^{n+1} being formed (null: no injection)
+  double *traces;                // [step-base][S][nrec] (null: none)
+  int64_t N;
+  int nshots;
+  int advance;                   // 1: increment step after
+};
+int launch_tail_n(const TailArgs &a, int ncomp, void *stream);
+
+// elementwise helpers (state precision); vectors are contiguous
+int launch_axpby(int state, int64_t n, double a, const void *x, double b, void *y, void *stream);
+int launch_convert_to_f64(int state, int64_t n, const void *x, double *y, void *stream);
+int launch_convert_from_f64(int state, int64_t n, const double *x, void *y, void *stream);
+// y[i] *= c*d[i % N]
+int launch_scale_by(int state, int64_t n, int S, int ncomp, int64_t N, const double *d, double c,
+                    void *y, void *stream);
+// y[i] = 0 where dir[i % N]
+int launch_mask(int state, int64_t n, int S, int ncomp, int64_t N, const uint8_t *dir, void *y,
+                void *stream);
+int launch_make_sdiag(int state, int64_t N, const double *M, const uint8_t *dir, double dt2,
+                      void *s, void *stream);
+// per shot: out[s] = sum_i w[i%N] x_i y_i  /  sum_i w[i%N] (a_i-b_i)^2, FP64,
+// deterministic; part = scratch of reduction_scratch_doubles(S) doubles
+int launch_dot_part(int state, int64_t N, int ncomp, int S, const double *w, const void *x,
+                    const void *y, double *out, double *part, void *stream);
+int launch_dot_diff_part(int state, int64_t N, int ncomp, int S, const double *w, const void *a,
+                         const void *b, double *out, double *part, void *stream);
+int reduction_scratch_doubles(int S);
+
+// adjoint: per-element gradient accumulation grad[e] += c * u_e^T A0_e lam_e
+struct GradArgs {
+  int dim, order, ncomp, state, geom;
+  int64_t E, N;
+  const int32_t *map;
+  const void *G;
+  const double *lam, *mu;
+  const void *U;       // forward state
+  const void *L;       // adjoint state
+  double *grad;        // E (sorted order)
+  double coef;
+  int nshots;
+};
+int launch_gradient(const GradArgs &a, void *stream);
+
+}  // namespace mjw

@@ -1,0 +1,209 @@
+// Sum-factorised stiffness kernels for isotropic elastodynamics on
+// Gauss-Lobatto Q_r hexahedra, gfx950.
+//
+// Restates MltAdd_ElementH1 (Computation/ProdMatVectH1.cxx:674-944) for the
+// continuous second-order case: three reference gradients (Rh^T, :745-751),
+// physical gradient DF^{-T} grad (:762-777), isotropic stress
+// sigma = lambda tr(e) I + 2 mu e (ApplyTensorStiffness,
+// Elliptic/Elastic/VarElastic.cxx:22-30), J w DF^{-1} sigma (:797-820), three
+// divergences (:839-856), scatter into the component-blocked vector
+// (Mesh/NumberMap.cxx:693).
+//
+// The per-node geometry is stored as M = sqrt(J w) DF^{-1} (9 values, plane-
+// major [k][9][E][n*n]): grad' = M^T g = sqrt(Jw) grad u, sigma' = C:grad'
+// = sqrt(Jw) sigma, v = M sigma' = J w DF^{-1} sigma -- one 3x3 tensor per
+// node instead of J w and J DF^{-1}.  Mapping, colouring, fused leapfrog and
+// shot loop as in the scalar kernels (mjw_kernels_scalar.hip).
+#include <hip/hip_runtime.h>
+
+#include "mjw_internal.hpp"
+
+namespace mjw {
+
+template <int N, int EPB, typename T, typename TG, int MODE>
+__global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    const double *__restrict__ lam_e, const double *__restrict__ mu_e, int64_t E, int64_t e0,
+    int64_t e1, const T *__restrict__ U, T *__restrict__ Y, const T *__restrict__ sdiag,
+    T alpha, T beta, int64_t N_nodes, int nshots) {
+  constexpr int P = N * N, NN = N * N * N;
+  using A = TG;
+  __shared__ T lds[6 * EPB * NN];
+  const int t = threadIdx.x;
+  const int el = t / P, ij = t - el * P;
+  const int i = ij % N, j = ij / N;
+  const int64_t e = e0 + (int64_t)blockIdx.x * EPB + el;
+  const bool act = e < e1;
+  T Di[N], Dj[N], Dti[N], Dtj[N];
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    Di[l] = Dm[i * N + l];
+    Dj[l] = Dm[j * N + l];
+    Dti[l] = Dm[l * N + i];
+    Dtj[l] = Dm[l * N + j];
+  }
+  int32_t gid[N];
+  A lam = 0, mu = 0;
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) gid[k] = map[(k * E + e) * P + ij];
+    lam = (A)lam_e[e];
+    mu = (A)mu_e[e];
+  }
+  // LDS planes: [6][EPB][NN]; u_m uses planes 0..2, v_{m,0} planes 0..2,
+  // v_{m,1} planes 3..5
+  T *pl[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) pl[c] = lds + (c * EPB + el) * NN;
+  for (int s = 0; s < nshots; ++s) {
+    const T *Us = U + (int64_t)s * 3 * N_nodes;
+    T *Ys = Y + (int64_t)s * 3 * N_nodes;
+    T u[3][N];
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          u[m][k] = Us[m * N_nodes + (gid[k] & 0x7fffffff)];
+          pl[m][k * P + ij] = u[m][k];
+        }
+    }
+    __syncthreads();
+    T v0[3][N], v1[3][N], v2[3][N];
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        A g[3][3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+          for (int l = 0; l < N; ++l) {
+            a0 += Di[l] * pl[m][k * P + j * N + l];
+            a1 += Dj[l] * pl[m][k * P + l * N + i];
+            a2 += Dm[k * N + l] * u[m][l];
+          }
+          g[m][0] = a0;
+          g[m][1] = a1;
+          g[m][2] = a2;
+        }
+        A M[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) M[c] = (A)G[((k * 9 + c) * E + e) * P + ij];
+        A du[3][3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            du[m][a] = M[0 * 3 + a] * g[m][0] + M[1 * 3 + a] * g[m][1] + M[2 * 3 + a] * g[m][2];
+        const A tr = du[0][0] + du[1][1] + du[2][2];
+        A sg[3][3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            sg[m][a] = mu * (du[m][a] + du[a][m]) + (m == a ? lam * tr : A(0));
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          v0[m][k] = (T)(M[0] * sg[m][0] + M[1] * sg[m][1] + M[2] * sg[m][2]);
+          v1[m][k] = (T)(M[3] * sg[m][0] + M[4] * sg[m][1] + M[5] * sg[m][2]);
+          v2[m][k] = (T)(M[6] * sg[m][0] + M[7] * sg[m][1] + M[8] * sg[m][2]);
+        }
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          pl[m][k * P + ij] = v0[m][k];
+          pl[3 + m][k * P + ij] = v1[m][k];
+        }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const int32_t mm = gid[k];
+        const int64_t g = mm & 0x7fffffff;
+        const bool first = mm < 0;
+        T sgd = 0;
+        if (MODE == kLeap) sgd = sdiag[g];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          T o = 0;
+#pragma unroll
+          for (int l = 0; l < N; ++l) {
+            o += Dti[l] * pl[m][k * P + j * N + l];
+            o += Dtj[l] * pl[3 + m][k * P + l * N + i];
+            o += Dm[l * N + k] * v2[m][l];
+          }
+          T *y = Ys + m * N_nodes + g;
+          if (MODE == kLeap) {
+            *y = first ? (T(2) * u[m][k] - *y - sgd * o) : (*y - sgd * o);
+          } else {
+            *y = first ? (beta == T(0) ? alpha * o : beta * *y + alpha * o) : (*y + alpha * o);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int N, int EPB, typename T, typename TG>
+static int run_elastic(const StiffArgs &a, hipStream_t st) {
+  const int64_t ne = a.e1 - a.e0;
+  if (ne <= 0) return 0;
+  const unsigned blocks = (unsigned)((ne + EPB - 1) / EPB);
+  const unsigned threads = EPB * N * N;
+  const T *D = static_cast<const T *>(a.Dm);
+  const T *U = static_cast<const T *>(a.U);
+  T *Y = static_cast<T *>(a.Y);
+  const T *s = static_cast<const T *>(a.sdiag);
+  const TG *G = static_cast<const TG *>(a.G);
+  if (a.mode == kLeap)
+    hipLaunchKernelGGL((k_elastic3d<N, EPB, T, TG, kLeap>), dim3(blocks), dim3(threads), 0, st,
+                       a.map, G, D, a.lam, a.mu, a.E, a.e0, a.e1, U, Y, s, (T)a.alpha,
+                       (T)a.beta, a.N, a.nshots);
+  else
+    hipLaunchKernelGGL((k_elastic3d<N, EPB, T, TG, kApply>), dim3(blocks), dim3(threads), 0, st,
+                       a.map, G, D, a.lam, a.mu, a.E, a.e0, a.e1, U, Y, s, (T)a.alpha,
+                       (T)a.beta, a.N, a.nshots);
+  return (int)hipGetLastError();
+}
+
+template <int N, int EPB>
+static int run_elastic_prec(const StiffArgs &a, hipStream_t st) {
+  if (a.state == kF64) return run_elastic<N, EPB, double, double>(a, st);
+  if (a.geom == kG64) return run_elastic<N, EPB, float, double>(a, st);
+  return run_elastic<N, EPB, float, float>(a, st);
+}
+
+// LDS = 6*EPB*n^3*sizeof(T) <= 64 KiB
+#define MJW_TILES_EL(X) \
+  X(2, 16) X(2, 32) X(3, 7) X(3, 14) X(4, 4) X(4, 8) X(5, 5) X(5, 10) X(6, 2) X(6, 5) X(7, 1) \
+  X(7, 2) X(8, 1)
+
+int launch_stiffness_elastic(const StiffArgs &a, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int n = a.order + 1;
+#define X(NN, EE) \
+  if (n == NN && a.epb == EE) return run_elastic_prec<NN, EE>(a, st);
+  MJW_TILES_EL(X)
+#undef X
+  return -1000;
+}
+
+std::vector<int> tile_choices_elastic(int order) {
+  std::vector<int> v;
+  const int n = order + 1;
+#define X(NN, EE) \
+  if (n == NN) v.push_back(EE);
+  MJW_TILES_EL(X)
+#undef X
+  return v;
+}
+
+}  // namespace mjw

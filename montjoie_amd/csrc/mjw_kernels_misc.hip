@@ -1,0 +1,414 @@
+// Setup and small per-step kernels: FP64 geometry precompute, lumped-mass
+// assembly, source injection + receiver interpolation ("tail" of a leapfrog
+// step), vector helpers and deterministic FP64 reductions.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "mjw_internal.hpp"
+
+namespace mjw {
+
+int launch_stiffness_scalar(const StiffArgs &a, void *stream);
+int launch_stiffness_elastic(const StiffArgs &a, void *stream);
+std::vector<int> tile_choices_scalar(int dim, int order);
+std::vector<int> tile_choices_elastic(int order);
+
+int launch_stiffness(const StiffArgs &a, void *stream) {
+  if (a.ncomp == 3) return launch_stiffness_elastic(a, stream);
+  return launch_stiffness_scalar(a, stream);
+}
+
+std::vector<int> tile_choices(int dim, int order, int ncomp) {
+  if (ncomp == 3) return tile_choices_elastic(order);
+  return tile_choices_scalar(dim, order);
+}
+
+int default_tile(int dim, int order, int ncomp, int state) {
+  (void)state;
+  std::vector<int> v = tile_choices(dim, order, ncomp);
+  return v.empty() ? 0 : v[0];
+}
+
+// ---------------------------------------------------------------------------
+// Geometry (FP64).  Per element-local node: DF = d x / d xi from the Q_r
+// interpolant of the node positions (exact for multilinear elements; Montjoie
+// evaluates DF of the element's geometric transformation, Fb.DFjElem,
+// Harmonic/VarGeometryProblem.cxx:693-756), J = det DF, w = tensor GL weight,
+// then either the acoustic tensor J w mu DF^{-1} DF^{-T}
+// (Elliptic/Helmholtz/VarHelmholtz.cxx:88-117) or the elastic factor
+// sqrt(J w) DF^{-1}, and the mass coefficient rho J w (VarHelmholtz.cxx:115).
+// ---------------------------------------------------------------------------
+__global__ void k_geometry(int dim, int n, int ncomp, int64_t E, const int32_t *__restrict__ map,
+                           const double *__restrict__ xyz, const double *__restrict__ D,
+                           const double *__restrict__ wgl, const double *__restrict__ rho,
+                           const double *__restrict__ mu, double *__restrict__ G64,
+                           float *__restrict__ G32, double *__restrict__ Mloc,
+                           int *__restrict__ bad) {
+  const int P = n * n;
+  const int nd = dim == 2 ? P : P * n;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= E * nd) return;
+  const int64_t e = id / nd;
+  const int q = (int)(id - e * nd);
+  const int i = q % n, j = (q / n) % n, k = q / P;
+  auto node = [&](int a, int b, int c) -> int64_t {
+    const int ij = a + n * b;
+    return (int64_t)(map[((int64_t)c * E + e) * P + ij] & 0x7fffffff);
+  };
+  double DF[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int l = 0; l < n; ++l) {
+    const double d0 = D[i * n + l], d1 = D[j * n + l];
+    const int64_t g0 = node(l, j, k), g1 = node(i, l, k);
+    for (int a = 0; a < dim; ++a) {
+      DF[a * dim + 0] += d0 * xyz[g0 * dim + a];
+      DF[a * dim + 1] += d1 * xyz[g1 * dim + a];
+    }
+    if (dim == 3) {
+      const double d2 = D[k * n + l];
+      const int64_t g2 = node(i, j, l);
+      for (int a = 0; a < 3; ++a) DF[a * 3 + 2] += d2 * xyz[g2 * 3 + a];
+    }
+  }
+  double J, Di[9];
+  if (dim == 2) {
+    J = DF[0] * DF[3] - DF[1] * DF[2];
+    Di[0] = DF[3] / J; Di[1] = -DF[1] / J; Di[2] = -DF[2] / J; Di[3] = DF[0] / J;
+  } else {
+    const double c00 = DF[4] * DF[8] - DF[5] * DF[7];
+    const double c01 = DF[5] * DF[6] - DF[3] * DF[8];
+    const double c02 = DF[3] * DF[7] - DF[4] * DF[6];
+    J = DF[0] * c00 + DF[1] * c01 + DF[2] * c02;
+    Di[0] = c00 / J; Di[1] = (DF[2] * DF[7] - DF[1] * DF[8]) / J; Di[2] = (DF[1] * DF[5] - DF[2] * DF[4]) / J;
+    Di[3] = c01 / J; Di[4] = (DF[0] * DF[8] - DF[2] * DF[6]) / J; Di[5] = (DF[2] * DF[3] - DF[0] * DF[5]) / J;
+    Di[6] = c02 / J; Di[7] = (DF[1] * DF[6] - DF[0] * DF[7]) / J; Di[8] = (DF[0] * DF[4] - DF[1] * DF[3]) / J;
+  }
+  if (!(J > 0.0)) atomicOr(bad, 1);
+  const double w = wgl[i] * wgl[j] * (dim == 3 ? wgl[k] : 1.0);
+  const double Jw = J * w;
+  Mloc[id] = rho[e] * Jw;
+  const int ij = i + n * j;
+  if (ncomp == 1) {
+    const double c = Jw * mu[e];
+    if (dim == 2) {
+      double g[3];
+      g[0] = c * (Di[0] * Di[0] + Di[1] * Di[1]);
+      g[1] = c * (Di[0] * Di[2] + Di[1] * Di[3]);
+      g[2] = c * (Di[2] * Di[2] + Di[3] * Di[3]);
+      for (int cc = 0; cc < 3; ++cc) {
+        const int64_t o = ((int64_t)cc * E + e) * P + ij;
+        if (G64) G64[o] = g[cc];
+        if (G32) G32[o] = (float)g[cc];
+      }
+    } else {
+      double g[6];
+      const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+      for (int cc = 0; cc < 6; ++cc) {
+        const double *ra = Di + 3 * ia[cc], *rb = Di + 3 * ib[cc];
+        g[cc] = c * (ra[0] * rb[0] + ra[1] * rb[1] + ra[2] * rb[2]);
+      }
+      for (int cc = 0; cc < 6; ++cc) {
+        const int64_t o = (((int64_t)k * 6 + cc) * E + e) * P + ij;
+        if (G64) G64[o] = g[cc];
+        if (G32) G32[o] = (float)g[cc];
+      }
+    }
+  } else {
+    const double sq = sqrt(Jw);
+    for (int cc = 0; cc < 9; ++cc) {
+      const int64_t o = (((int64_t)k * 9 + cc) * E + e) * P + ij;
+      const double v = sq * Di[cc];
+      if (G64) G64[o] = v;
+      if (G32) G32[o] = (float)v;
+    }
+  }
+}
+
+int launch_geometry_full(int dim, int order, int ncomp, int64_t E, const int32_t *map,
+                         const double *xyz, const double *D, const double *w, const double *rho,
+                         const double *mu, double *G64, float *G32, double *Mloc, int *bad,
+                         void *stream) {
+  const int n = order + 1;
+  const int nd = dim == 2 ? n * n : n * n * n;
+  const int64_t tot = E * nd;
+  const int bs = 256;
+  hipLaunchKernelGGL(k_geometry, dim3((unsigned)((tot + bs - 1) / bs)), dim3(bs), 0,
+                     static_cast<hipStream_t>(stream), dim, n, ncomp, E, map, xyz, D, w, rho, mu,
+                     G64, G32, Mloc, bad);
+  return (int)hipGetLastError();
+}
+
+// lumped mass of one colour: first toucher writes, others add (deterministic)
+__global__ void k_mass(const int32_t *__restrict__ map, const double *__restrict__ Mloc,
+                       int64_t E, int n, int dim, int64_t e0, int64_t e1, double *__restrict__ M) {
+  const int P = n * n, nd = dim == 2 ? P : P * n;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (e1 - e0) * nd) return;
+  const int64_t e = e0 + id / nd;
+  const int q = (int)(id % nd);
+  const int ij = q % P, k = q / P;
+  const int32_t m = map[((int64_t)k * E + e) * P + ij];
+  const int64_t g = m & 0x7fffffff;
+  const double v = Mloc[e * nd + q];
+  if (m < 0)
+    M[g] = v;
+  else
+    M[g] += v;
+}
+
+int launch_mass_assemble(const int32_t *map, const double *Mloc, int64_t E, int dim, int order,
+                         int64_t e0, int64_t e1, double *M, void *stream) {
+  const int n = order + 1;
+  const int nd = dim == 2 ? n * n : n * n * n;
+  const int64_t tot = (e1 - e0) * nd;
+  if (tot <= 0) return 0;
+  hipLaunchKernelGGL(k_mass, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), map, Mloc, E, n, dim, e0, e1, M);
+  return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Tail of a leapfrog step (one block): Dirac source injection into U^{n+1}
+// (AddScalarSourceAtTime, VarInstationary.cxx:2391-2409, scaled by dt^2/M as
+// the solve does, ModifiedEquation.cxx:316-333) and receiver interpolation of
+// U^n (GiveIterate -> WriteSnapshot, VarInstationary.cxx:798-853).  The step
+// index lives on the device so a captured graph can be replayed.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail(TailArgs a, int ncomp) {
+  const int64_t step = a.ctl[0];
+  const int64_t base = a.ctl[1];
+  const T *U = static_cast<const T *>(a.U);
+  T *P = static_cast<T *>(a.P);
+  const T *sd = static_cast<const T *>(a.sdiag);
+  const double pulse = (a.pulse && step < a.npulse) ? a.pulse[step] : 0.0;
+  const int64_t NT = (int64_t)ncomp * a.N;
+  if (pulse != 0.0 && P) {
+    for (int idx = threadIdx.x; idx < a.nsrc; idx += blockDim.x) {
+      const int64_t dof = a.src_dof[idx];
+      const int s = a.src_shot[idx];
+      const int64_t g = dof % a.N;
+      T *y = P + (int64_t)s * NT + dof;
+      *y += sd[g] * (T)(pulse * a.src_val[idx]);
+    }
+  }
+  if (a.traces) {
+    const int tot = a.nrec * a.nshots;
+    for (int it = threadIdx.x; it < tot; it += blockDim.x) {
+      const int r = it % a.nrec, s = it / a.nrec;
+      const T *Us = U + (int64_t)s * NT + (int64_t)a.rec_comp[r] * a.N;
+      double acc = 0.0;
+      for (int q = 0; q < a.nd; ++q)
+        acc += a.rec_w[(int64_t)r * a.nd + q] * (double)Us[a.rec_gid[(int64_t)r * a.nd + q]];
+      a.traces[((step - base) * a.nshots + s) * a.nrec + r] = acc;
+    }
+  }
+  __syncthreads();
+  if (a.advance && threadIdx.x == 0) a.ctl[0] = step + 1;
+}
+
+int launch_tail_n(const TailArgs &a, int ncomp, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (a.state == kF64)
+    hipLaunchKernelGGL(k_tail<double>, dim3(1), dim3(256), 0, st, a, ncomp);
+  else
+    hipLaunchKernelGGL(k_tail<float>, dim3(1), dim3(256), 0, st, a, ncomp);
+  return (int)hipGetLastError();
+}
+
+
+// ------------------------------------------------------------ vector helpers
+template <typename T>
+__global__ void k_axpby(int64_t n, double a, const T *__restrict__ x, double b, T *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (T)(b == 0.0 ? a * (double)x[i] : a * (double)x[i] + b * (double)y[i]);
+}
+
+static unsigned grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+int launch_axpby(int state, int64_t n, double a, const void *x, double b, void *y, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(grid_for(n)), dim3(256), 0, st, n, a,
+                       (const double *)x, b, (double *)y);
+  else
+    hipLaunchKernelGGL(k_axpby<float>, dim3(grid_for(n)), dim3(256), 0, st, n, a,
+                       (const float *)x, b, (float *)y);
+  return (int)hipGetLastError();
+}
+
+template <typename A, typename B>
+__global__ void k_convert(int64_t n, const A *__restrict__ x, B *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (B)x[i];
+}
+
+int launch_convert_to_f64(int state, int64_t n, const void *x, double *y, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    return (int)hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(256), 0, st, n,
+                     (const float *)x, y);
+  return (int)hipGetLastError();
+}
+
+int launch_convert_from_f64(int state, int64_t n, const double *x, void *y, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    return (int)hipMemcpyAsync(y, x, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(256), 0, st, n, x,
+                     (float *)y);
+  return (int)hipGetLastError();
+}
+
+// y[(s*ncomp+m)*N+g] *= c*d[g]
+template <typename T>
+__global__ void k_scale_by(int64_t n, int64_t N, const double *__restrict__ d, double c,
+                           T *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (T)((double)y[i] * (c * d[i % N]));
+}
+
+int launch_scale_by(int state, int64_t n, int S, int ncomp, int64_t N, const double *d, double c,
+                    void *y, void *stream) {
+  (void)S;
+  (void)ncomp;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_scale_by<double>, dim3(grid_for(n)), dim3(256), 0, st, n, N, d, c,
+                       (double *)y);
+  else
+    hipLaunchKernelGGL(k_scale_by<float>, dim3(grid_for(n)), dim3(256), 0, st, n, N, d, c,
+                       (float *)y);
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_mask(int64_t n, int64_t N, const uint8_t *__restrict__ dir, T *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (dir[i % N]) y[i] = T(0);
+}
+
+int launch_mask(int state, int64_t n, int S, int ncomp, int64_t N, const uint8_t *dir, void *y,
+                void *stream) {
+  (void)S;
+  (void)ncomp;
+  if (!dir) return 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_mask<double>, dim3(grid_for(n)), dim3(256), 0, st, n, N, dir,
+                       (double *)y);
+  else
+    hipLaunchKernelGGL(k_mask<float>, dim3(grid_for(n)), dim3(256), 0, st, n, N, dir,
+                       (float *)y);
+  return (int)hipGetLastError();
+}
+
+// s = dt^2 / M on free nodes, 0 on Dirichlet nodes
+template <typename T>
+__global__ void k_sdiag(int64_t N, const double *__restrict__ M, const uint8_t *__restrict__ dir,
+                        double dt2, T *__restrict__ s) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s[i] = (dir && dir[i]) ? T(0) : (T)(dt2 / M[i]);
+}
+
+int launch_make_sdiag(int state, int64_t N, const double *M, const uint8_t *dir, double dt2,
+                      void *s, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_sdiag<double>, dim3(grid_for(N)), dim3(256), 0, st, N, M, dir, dt2,
+                       (double *)s);
+  else
+    hipLaunchKernelGGL(k_sdiag<float>, dim3(grid_for(N)), dim3(256), 0, st, N, M, dir, dt2,
+                       (float *)s);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------- reductions
+// Deterministic two-pass FP64 reduction per shot: fixed grid, per-block
+// partials summed by one block in a fixed order.
+constexpr int kRedBlocks = 1024;
+
+template <typename T, int DIFF>
+__global__ __launch_bounds__(256) void k_dot_partial(int64_t per_shot, int64_t N,
+                                                     const double *__restrict__ w,
+                                                     const T *__restrict__ x,
+                                                     const T *__restrict__ y,
+                                                     double *__restrict__ part) {
+  const int s = blockIdx.y;
+  const T *xs = x + (int64_t)s * per_shot;
+  const T *ys = y + (int64_t)s * per_shot;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per_shot;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double wi = w ? w[i % N] : 1.0;
+    if (DIFF) {
+      const double d = (double)xs[i] - (double)ys[i];
+      acc += wi * d * d;
+    } else {
+      acc += wi * (double)xs[i] * (double)ys[i];
+    }
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(int64_t)s * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_dot_final(int nb, const double *__restrict__ part,
+                                                   double *__restrict__ out) {
+  const int s = blockIdx.x;
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) acc += part[(int64_t)s * nb + i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[s] = sh[0];
+}
+
+template <int DIFF>
+static int dot_impl(int state, int64_t N, int ncomp, int S, const double *w, const void *x,
+                    const void *y, double *out, double *part, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t per = (int64_t)ncomp * N;
+  int nb = (int)std::min<int64_t>(kRedBlocks, (per + 255) / 256);
+  if (nb < 1) nb = 1;
+  if (state == kF64)
+    hipLaunchKernelGGL((k_dot_partial<double, DIFF>), dim3(nb, S), dim3(256), 0, st, per, N, w,
+                       (const double *)x, (const double *)y, part);
+  else
+    hipLaunchKernelGGL((k_dot_partial<float, DIFF>), dim3(nb, S), dim3(256), 0, st, per, N, w,
+                       (const float *)x, (const float *)y, part);
+  hipLaunchKernelGGL(k_dot_final, dim3(S), dim3(256), 0, st, nb, part, out);
+  return (int)hipGetLastError();
+}
+
+int launch_dot_part(int state, int64_t N, int ncomp, int S, const double *w, const void *x,
+                    const void *y, double *out, double *part, void *stream) {
+  return dot_impl<0>(state, N, ncomp, S, w, x, y, out, part, stream);
+}
+int launch_dot_diff_part(int state, int64_t N, int ncomp, int S, const double *w, const void *a,
+                         const void *b, double *out, double *part, void *stream) {
+  return dot_impl<1>(state, N, ncomp, S, w, a, b, out, part, stream);
+}
+int reduction_scratch_doubles(int S) { return kRedBlocks * S; }
+
+}  // namespace mjw

@@ -1,0 +1,296 @@
+// Sum-factorised stiffness kernels for the scalar (acoustic) wave equation on
+// Gauss-Lobatto Q_r quadrilaterals / hexahedra, gfx950.
+//
+// Restates, per element, MltAdd_ElementH1_Scalar (Computation/
+// ProdMatVectScalarH1.cxx:256-668) on the Lobatto path: gather (NumberMap.cxx:
+// 661-699), reference gradient Rh^T u (HexahedronLobatto.cxx:257-281), per-node
+// tensor J w DF^{-1} mu DF^{-T} (ProdMatVectScalarH1.cxx:472-495), divergence
+// Rh v (HexahedronLobatto.cxx:217-251), scatter-add (NumberMap.cxx:745-777).
+//
+// Layout / mapping (MI355X):
+//  * elements are sorted by colour; one launch covers one colour, so the
+//    scatter has no conflicts and needs no atomics;
+//  * 3-D: one thread per (element, i, j) column, looping k; the column's
+//    values stay in registers (xi_3 contractions) and the element is staged
+//    in LDS for the xi_1 / xi_2 contractions.  2-D: one thread per node;
+//  * map and geometry are plane-major ([k][comp][E][n*n]) so each wavefront
+//    load is one contiguous run;
+//  * kLeap fuses the lumped-mass leapfrog update: the first element to touch
+//    a node (bit 31 of its map entry, lowest colour) writes
+//    Y = 2U - Y - s*out, later ones Y -= s*out, with s = dt^2/M (0 on
+//    Dirichlet nodes), so U^{n+1} overwrites U^{n-1} in the same pass;
+//  * kApply computes Y = beta*Y + alpha*A*U (first toucher applies beta);
+//  * several shots (right-hand sides) share the mesh: the geometry of the
+//    column is loaded once into registers and reused for every shot (shot-
+//    major state, each shot a contiguous vector).
+#include <hip/hip_runtime.h>
+
+#include "mjw_internal.hpp"
+
+namespace mjw {
+
+template <typename T, typename TG>
+struct Mix {  // geometry product in the geometry's precision
+  using A = TG;
+};
+
+template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST>
+__global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
+    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots) {
+  constexpr int P = N * N, NN = N * N * N;
+  using A = typename Mix<T, TG>::A;
+  __shared__ T lu[EPB * NN];
+  __shared__ T lv[EPB * NN];
+  const int t = threadIdx.x;
+  const int el = t / P, ij = t - el * P;
+  const int i = ij % N, j = ij / N;
+  const int64_t e = e0 + (int64_t)blockIdx.x * EPB + el;
+  const bool act = e < e1;
+  T Di[N], Dj[N], Dti[N], Dtj[N];
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    Di[l] = Dm[i * N + l];
+    Dj[l] = Dm[j * N + l];
+    Dti[l] = Dm[l * N + i];
+    Dtj[l] = Dm[l * N + j];
+  }
+  int32_t gid[N];
+  A g6[HOIST ? N : 1][6];  // geometry of the column, kept across shots
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      gid[k] = map[(k * E + e) * P + ij];
+      if constexpr (HOIST) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) g6[k][c] = (A)G[((k * 6 + c) * E + e) * P + ij];
+      }
+    }
+  }
+  T *su = lu + el * NN, *sv = lv + el * NN;
+  for (int s = 0; s < nshots; ++s) {
+    const T *Us = U + (int64_t)s * N_nodes;
+    T *Ys = Y + (int64_t)s * N_nodes;
+    T u[N];
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        u[k] = Us[gid[k] & 0x7fffffff];
+        su[k * P + ij] = u[k];
+      }
+    }
+    __syncthreads();
+    T v1[N], v2[N], v3[N];
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        T a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          a1 += Di[l] * su[k * P + j * N + l];
+          a2 += Dj[l] * su[k * P + l * N + i];
+          a3 += Dm[k * N + l] * u[l];
+        }
+        const A b1 = a1, b2 = a2, b3 = a3;
+        A gk[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+          gk[c] = HOIST ? g6[HOIST ? k : 0][c] : (A)G[((k * 6 + c) * E + e) * P + ij];
+        v1[k] = (T)(gk[0] * b1 + gk[1] * b2 + gk[2] * b3);
+        v2[k] = (T)(gk[1] * b1 + gk[3] * b2 + gk[4] * b3);
+        v3[k] = (T)(gk[2] * b1 + gk[4] * b2 + gk[5] * b3);
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        su[k * P + ij] = v1[k];
+        sv[k * P + ij] = v2[k];
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        T o = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          o += Dti[l] * su[k * P + j * N + l];
+          o += Dtj[l] * sv[k * P + l * N + i];
+          o += Dm[l * N + k] * v3[l];
+        }
+        const int32_t m = gid[k];
+        const int64_t g = m & 0x7fffffff;
+        const bool first = m < 0;
+        if (MODE == kLeap) {
+          const T sg = sdiag[g];
+          Ys[g] = first ? (T(2) * u[k] - Ys[g] - sg * o) : (Ys[g] - sg * o);
+        } else {
+          Ys[g] = first ? (beta == T(0) ? alpha * o : beta * Ys[g] + alpha * o) : (Ys[g] + alpha * o);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int N, int EPB, typename T, typename TG, int MODE>
+__global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
+    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots) {
+  constexpr int P = N * N;
+  using A = typename Mix<T, TG>::A;
+  __shared__ T lu[EPB * P];
+  __shared__ T lv[EPB * P];
+  const int t = threadIdx.x;
+  const int el = t / P, q = t - el * P;
+  const int i = q % N, j = q / N;
+  const int64_t e = e0 + (int64_t)blockIdx.x * EPB + el;
+  const bool act = e < e1;
+  T Di[N], Dj[N], Dti[N], Dtj[N];
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    Di[l] = Dm[i * N + l];
+    Dj[l] = Dm[j * N + l];
+    Dti[l] = Dm[l * N + i];
+    Dtj[l] = Dm[l * N + j];
+  }
+  int32_t gid = 0;
+  A g3[3] = {0, 0, 0};
+  if (act) {
+    gid = map[e * P + q];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g3[c] = (A)G[(c * E + e) * P + q];
+  }
+  T *su = lu + el * P, *sv = lv + el * P;
+  const int64_t g = gid & 0x7fffffff;
+  const bool first = gid < 0;
+  for (int s = 0; s < nshots; ++s) {
+    const T *Us = U + (int64_t)s * N_nodes;
+    T *Ys = Y + (int64_t)s * N_nodes;
+    T u = 0;
+    if (act) {
+      u = Us[g];
+      su[q] = u;
+    }
+    __syncthreads();
+    T v1 = 0, v2 = 0;
+    if (act) {
+      T a1 = 0, a2 = 0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        a1 += Di[l] * su[j * N + l];
+        a2 += Dj[l] * su[l * N + i];
+      }
+      const A b1 = a1, b2 = a2;
+      v1 = (T)(g3[0] * b1 + g3[1] * b2);
+      v2 = (T)(g3[1] * b1 + g3[2] * b2);
+    }
+    __syncthreads();
+    if (act) {
+      su[q] = v1;
+      sv[q] = v2;
+    }
+    __syncthreads();
+    if (act) {
+      T o = 0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        o += Dti[l] * su[j * N + l];
+        o += Dtj[l] * sv[l * N + i];
+      }
+      if (MODE == kLeap) {
+        const T sg = sdiag[g];
+        Ys[g] = first ? (T(2) * u - Ys[g] - sg * o) : (Ys[g] - sg * o);
+      } else {
+        Ys[g] = first ? (beta == T(0) ? alpha * o : beta * Ys[g] + alpha * o) : (Ys[g] + alpha * o);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------- dispatch
+template <int N, int EPB, typename T, typename TG>
+static int run_scalar(const StiffArgs &a, hipStream_t st) {
+  const int64_t ne = a.e1 - a.e0;
+  if (ne <= 0) return 0;
+  const unsigned blocks = (unsigned)((ne + EPB - 1) / EPB);
+  const unsigned threads = EPB * N * N;
+  const T *D = static_cast<const T *>(a.Dm);
+  const T *U = static_cast<const T *>(a.U);
+  T *Y = static_cast<T *>(a.Y);
+  const T *s = static_cast<const T *>(a.sdiag);
+  const TG *G = static_cast<const TG *>(a.G);
+  const T al = (T)a.alpha, be = (T)a.beta;
+  if (a.dim == 3) {
+    if (a.mode == kLeap) {
+      if (a.nshots > 1)
+        hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kLeap, true>), dim3(blocks), dim3(threads),
+                           0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+      else
+        hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kLeap, false>), dim3(blocks), dim3(threads),
+                           0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+    } else {
+      hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kApply, false>), dim3(blocks), dim3(threads),
+                         0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+    }
+  } else {
+    if (a.mode == kLeap)
+      hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kLeap>), dim3(blocks), dim3(threads), 0, st,
+                         a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+    else
+      hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kApply>), dim3(blocks), dim3(threads), 0, st,
+                         a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+  }
+  return (int)hipGetLastError();
+}
+
+template <int N, int EPB>
+static int run_prec(const StiffArgs &a, hipStream_t st) {
+  if (a.state == kF64) return run_scalar<N, EPB, double, double>(a, st);
+  if (a.geom == kG64) return run_scalar<N, EPB, float, double>(a, st);
+  return run_scalar<N, EPB, float, float>(a, st);
+}
+
+// tile (elements per block) candidates; the first is the default.
+// 3-D: threads = EPB*n^2, LDS = 2*EPB*n^3*sizeof(T) <= 64 KiB.
+#define MJW_TILES3(X) \
+  X(2, 16) X(2, 32) X(3, 7) X(3, 14) X(4, 8) X(4, 4) X(4, 16) X(5, 10) X(5, 5) X(5, 20) \
+  X(6, 7) X(6, 14) X(6, 16) X(7, 5) X(7, 10) X(8, 2) X(8, 4)
+#define MJW_TILES2(X) \
+  X(2, 64) X(2, 16) X(3, 28) X(3, 7) X(4, 16) X(4, 4) X(5, 10) X(5, 20) X(6, 7) X(6, 14) \
+  X(7, 5) X(7, 10) X(8, 4) X(8, 1)
+
+int launch_stiffness_scalar(const StiffArgs &a, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int n = a.order + 1;
+#define X(NN, EE) \
+  if (n == NN && a.epb == EE) return run_prec<NN, EE>(a, st);
+  if (a.dim == 3) {
+    MJW_TILES3(X)
+  } else {
+    MJW_TILES2(X)
+  }
+#undef X
+  return -1000;
+}
+
+std::vector<int> tile_choices_scalar(int dim, int order) {
+  std::vector<int> v;
+  const int n = order + 1;
+#define X(NN, EE) \
+  if (n == NN) v.push_back(EE);
+  if (dim == 3) {
+    MJW_TILES3(X)
+  } else {
+    MJW_TILES2(X)
+  }
+#undef X
+  return v;
+}
+
+}  // namespace mjw

@@ -1,0 +1,250 @@
+"""Host-side mirror of Montjoie's explicit wave-solver interface on top of the
+C ABI (include/mjwave.h).
+
+``WaveProblem`` plays the role of ``HyperbolicProblem<AcousticEquation<Dim>>``
+/ ``HyperbolicProblem<TimeElasticEquation<Dimension3>>`` for the explicit
+path (Instationary/VarInstationary.cxx:1629-1831 RunAll): it owns the mesh,
+media, Dirichlet nodes, Dirac sources with a Ricker pulse, receivers and the
+leapfrog state, and exposes the VirtualOdeSystem operators
+(Instationary/TimeSchemes.hxx:35-151) and the leapfrog scheme
+(ModifiedEquationIterator, Instationary/ModifiedEquation.hxx:78-114).
+
+Every computation runs in the HIP library; this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, iptr, u8ptr
+
+
+@dataclass
+class Mesh:
+    """Q_r mesh: global GL node positions and element -> node map."""
+    dim: int
+    order: int
+    xyz: np.ndarray          # (N, dim) float64
+    elem_nodes: np.ndarray   # (E, (order+1)^dim) int32, local q = i + n j (+ n^2 k)
+    dirichlet: np.ndarray    # (N,) uint8
+    nx: int = 0              # elements per direction for regular meshes
+
+    @property
+    def N(self) -> int:
+        return self.xyz.shape[0]
+
+    @property
+    def E(self) -> int:
+        return self.elem_nodes.shape[0]
+
+    @property
+    def nd(self) -> int:
+        return (self.order + 1) ** self.dim
+
+
+def regular_mesh(dim: int, nx: int, order: int, pert: float = 0.0, seed: int = 0) -> Mesh:
+    """REGULAR mesh of [0,1]^dim (Mesh/Mesh3D.cxx:8406-8486) with nx elements
+    per direction; interior vertices displaced by <= pert*h (seeded)."""
+    L = _lib.lib()
+    N = C.c_int64()
+    E = C.c_int64()
+    check(L.mjw_mesh_regular_size(dim, nx, order, C.byref(N), C.byref(E)), None, "mesh size")
+    nd = (order + 1) ** dim
+    xyz = np.zeros((N.value, dim))
+    en = np.zeros((E.value, nd), dtype=np.int32)
+    dr = np.zeros(N.value, dtype=np.uint8)
+    check(L.mjw_mesh_regular(dim, nx, order, float(pert), int(seed), dptr(xyz), iptr(en),
+                             u8ptr(dr)), None, "mesh")
+    return Mesh(dim, order, xyz, en, dr, nx)
+
+
+def layered_values(mesh: Mesh, values) -> np.ndarray:
+    """Per-element values of horizontal layers of a regular mesh: layer =
+    ez*len(values)//nx (ez = element index along the last axis)."""
+    e = np.arange(mesh.E)
+    ez = e // (mesh.nx ** (mesh.dim - 1))
+    return np.asarray(values, np.float64)[(ez * len(values)) // mesh.nx]
+
+
+def ricker_pulse_table(f0: float, t_begin: float, dt: float, nsteps: int) -> np.ndarray:
+    """f_n = pulse(t_begin + n dt + dt) through Montjoie's GL-24 interpolator."""
+    out = np.zeros(nsteps)
+    check(_lib.lib().mjw_ricker_pulse_table(f0, t_begin, dt, nsteps, dptr(out)), None, "pulse")
+    return out
+
+
+class WaveProblem:
+    """One acoustic (ncomp=1) or elastic (ncomp=3) problem on one device."""
+
+    def __init__(self, mesh: Mesh, rho, mu, lam=None, ncomp: int = 1, nshots: int = 1,
+                 precision: str = "fp64", device: int = 0, dirichlet=True):
+        self.mesh = mesh
+        self.ncomp = ncomp
+        self.nshots = nshots
+        self.precision = precision
+        E = mesh.E
+        rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float64), (E,)))
+        mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, np.float64), (E,)))
+        lamv = None
+        if ncomp == 3:
+            lamv = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float64), (E,)))
+        xyz = np.ascontiguousarray(mesh.xyz, np.float64)
+        en = np.ascontiguousarray(mesh.elem_nodes, np.int32)
+        dr = np.ascontiguousarray(mesh.dirichlet, np.uint8) if dirichlet is True else (
+            None if dirichlet is False or dirichlet is None else
+            np.ascontiguousarray(dirichlet, np.uint8))
+        self._h = C.c_void_p()
+        L = _lib.lib()
+        check(L.mjw_create(C.byref(self._h), device, mesh.dim, mesh.order, ncomp, mesh.N,
+                           dptr(xyz), E, iptr(en), dptr(rho), dptr(mu), dptr(lamv), u8ptr(dr),
+                           nshots, _lib.PRECISIONS[precision]), None, "mjw_create")
+        self.dt = None
+        self.nrec = 0
+
+    # ------------------------------------------------------------ lifecycle
+    def close(self):
+        if self._h:
+            _lib.lib().mjw_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        check(rc, self._h, what)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 9)()
+        self._chk(_lib.lib().mjw_info(self._h, a), "info")
+        k = ["dim", "order", "ncomp", "n_nodes", "n_elem", "ncolors", "nshots", "precision",
+             "device_bytes"]
+        return dict(zip(k, list(a)))
+
+    def set_tile(self, epb: int):
+        self._chk(_lib.lib().mjw_set_tile(self._h, int(epb)), "set_tile")
+
+    def tile(self) -> int:
+        v = C.c_int()
+        self._chk(_lib.lib().mjw_get_tile(self._h, C.byref(v)), "get_tile")
+        return v.value
+
+    # ------------------------------------------------------------ operators
+    @property
+    def ndof(self) -> int:
+        return self.mesh.N * self.ncomp
+
+    def apply_kh(self, U, alpha=1.0, beta=0.0, Y=None) -> np.ndarray:
+        """Y = beta Y + alpha K U, K = -A (ApplyOperatorKh)."""
+        U = np.ascontiguousarray(U, np.float64)
+        Y = np.zeros(self.ndof) if Y is None else np.ascontiguousarray(Y, np.float64).copy()
+        self._chk(_lib.lib().mjw_apply_kh(self._h, alpha, dptr(U), beta, dptr(Y)), "apply_kh")
+        return Y
+
+    def apply_kh_dev(self, U_ptr: int, Y_ptr: int, alpha=1.0, beta=0.0):
+        self._chk(_lib.lib().mjw_apply_kh_dev(self._h, alpha, C.c_void_p(U_ptr), beta,
+                                              C.c_void_p(Y_ptr)), "apply_kh_dev")
+
+    def mass(self) -> np.ndarray:
+        D = np.zeros(self.mesh.N)
+        self._chk(_lib.lib().mjw_get_mass(self._h, dptr(D)), "get_mass")
+        return D
+
+    # ---------------------------------------------------- sources/receivers
+    def set_sources(self, pts, pol=None, shot=None):
+        pts = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, self.mesh.dim))
+        k = pts.shape[0]
+        polv = None if pol is None else np.ascontiguousarray(
+            np.asarray(pol, np.float64).reshape(k, self.ncomp))
+        sh = None if shot is None else np.ascontiguousarray(np.asarray(shot, np.int32))
+        self._chk(_lib.lib().mjw_set_sources(self._h, k, dptr(pts), dptr(polv), iptr(sh)),
+                  "set_sources")
+
+    def set_pulse(self, table):
+        t = np.ascontiguousarray(table, np.float64)
+        self._chk(_lib.lib().mjw_set_pulse(self._h, len(t), dptr(t)), "set_pulse")
+
+    def set_receivers(self, pts, comp=None):
+        pts = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, self.mesh.dim))
+        cm = None if comp is None else np.ascontiguousarray(np.asarray(comp, np.int32))
+        self._chk(_lib.lib().mjw_set_receivers(self._h, pts.shape[0], dptr(pts), iptr(cm)),
+                  "set_receivers")
+        self.nrec = pts.shape[0]
+
+    # ------------------------------------------------------------ leapfrog
+    def leapfrog_init(self, dt: float, t_begin: float = 0.0, u0=None, du0=None):
+        u0a = None if u0 is None else np.ascontiguousarray(u0, np.float64)
+        du0a = None if du0 is None else np.ascontiguousarray(du0, np.float64)
+        self._chk(_lib.lib().mjw_leapfrog_init(self._h, t_begin, dt, dptr(u0a), dptr(du0a)),
+                  "leapfrog_init")
+        self.dt = dt
+
+    def advance(self, nsteps: int, traces: bool = True, graph: bool = True):
+        """Returns traces [nsteps, nshots, nrec] of U^n before each step."""
+        tr = np.zeros((nsteps, self.nshots, max(self.nrec, 0))) if (traces and self.nrec) else None
+        self._chk(_lib.lib().mjw_leapfrog_advance(self._h, nsteps, dptr(tr), int(bool(graph))),
+                  "advance")
+        return tr
+
+    def record(self) -> np.ndarray:
+        tr = np.zeros((self.nshots, self.nrec))
+        if self.nrec:
+            self._chk(_lib.lib().mjw_record(self._h, dptr(tr)), "record")
+        return tr
+
+    def run(self, nsteps: int, graph: bool = True):
+        """Receiver traces for rows n = 0..nsteps (RunSecondOrderScheme)."""
+        tr = self.advance(nsteps, traces=True, graph=graph)
+        last = self.record()
+        if tr is None:
+            return None
+        return np.concatenate([tr, last[None]], axis=0)
+
+    def state(self):
+        n = self.nshots * self.ndof
+        a = np.zeros(n)
+        b = np.zeros(n)
+        self._chk(_lib.lib().mjw_get_state(self._h, dptr(a), dptr(b)), "get_state")
+        return a.reshape(self.nshots, self.ndof), b.reshape(self.nshots, self.ndof)
+
+    def state_dev(self):
+        a = C.c_void_p()
+        b = C.c_void_p()
+        self._chk(_lib.lib().mjw_state_dev(self._h, C.byref(a), C.byref(b)), "state_dev")
+        return a.value, b.value
+
+    def step(self) -> int:
+        v = C.c_int64()
+        self._chk(_lib.lib().mjw_get_step(self._h, C.byref(v)), "get_step")
+        return v.value
+
+    def energy(self) -> np.ndarray:
+        e = np.zeros(self.nshots)
+        self._chk(_lib.lib().mjw_energy(self._h, dptr(e)), "energy")
+        return e
+
+    def lambda_max(self, iters: int = 80, seed: int = 1) -> float:
+        v = C.c_double()
+        self._chk(_lib.lib().mjw_lambda_max(self._h, iters, seed, C.byref(v)), "lambda_max")
+        return v.value
+
+    def profile(self, nsteps: int = 10) -> dict:
+        """Per-launch device times (HIP events on the problem's stream)."""
+        a = (C.c_double * 4)()
+        self._chk(_lib.lib().mjw_profile(self._h, nsteps, a), "profile")
+        return {"stiffness_launch_ms": a[0], "tail_ms": a[1], "step_ms": a[2],
+                "launches_per_step": int(a[3])}
+
+    def last_timing(self):
+        a = (C.c_double * 2)()
+        self._chk(_lib.lib().mjw_last_timing(self._h, a), "last_timing")
+        return a[0], a[1]

@@ -1,0 +1,125 @@
+"""Shared definitions of the synthetic test problems and helpers to rebuild
+the problems stored in tests/golden/*.npz (made by
+tests/golden/make_ref_fixtures.py from Montjoie's own solvers)."""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@dataclass
+class Case:
+    name: str
+    dim: int
+    nx: int
+    r: int
+    pert: float
+    seed: int
+    elastic: bool
+    mu_e: np.ndarray
+    rho_e: np.ndarray
+    lam_e: np.ndarray | None
+    f0: float
+    dt: float
+    nsteps: int
+    src: np.ndarray
+    pol: np.ndarray | None
+    rec: np.ndarray
+    times: np.ndarray | None = None
+    traces: np.ndarray | None = None
+
+
+def golden_names():
+    return sorted(f[:-4] for f in os.listdir(GOLD) if f.startswith("ref") and f.endswith(".npz"))
+
+
+def load(name: str) -> Case:
+    z = np.load(os.path.join(GOLD, name + ".npz"))
+    dim, nx = int(z["dim"]), int(z["nx"])
+    E = nx ** dim
+    e = np.arange(E)
+    layers = z["layers"]
+    lay = ((e // nx ** (dim - 1)) * len(layers)) // nx
+    elastic = bool(z["elastic"])
+    mu_e = layers[lay]
+    rho_e = z["rho_layers"][lay] if elastic else np.ones(E)
+    lam_e = z["lam_layers"][lay] if elastic else None
+    return Case(name, dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]), elastic, mu_e,
+                rho_e, lam_e, float(z["f0"]), float(z["dt"]), int(z["nsteps"]), z["src"],
+                z["pol"] if elastic else None, z["rec"], z["times"], z["traces"])
+
+
+def oracle_run(c: Case, nsteps: int | None = None):
+    """Run the case through the C oracle: returns (uN, uN1, traces[n, rec])."""
+    nsteps = c.nsteps if nsteps is None else nsteps
+    m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    if c.elastic:
+        P = O.elastic_problem(m, c.lam_e, c.mu_e, c.rho_e)
+        b = O.dirac(m, c.src, c.pol)
+        rc = np.zeros(0, np.int32)
+    else:
+        P = O.acoustic_problem(m, c.mu_e, c.rho_e)
+        b = O.dirac(m, c.src)
+        rc = None
+    pulse = O.pulse_table(c.f0, 0.0, c.dt, nsteps)
+    if c.elastic:
+        # every receiver on the three components, component-major
+        pts = np.vstack([c.rec] * 3)
+        comp = np.repeat(np.arange(3, dtype=np.int32), len(c.rec))
+        return O.leapfrog(P, b, pulse, c.dt, nsteps, pts, comp)
+    return O.leapfrog(P, b, pulse, c.dt, nsteps, c.rec, rc)
+
+
+def reference_traces(c: Case) -> np.ndarray:
+    """Montjoie's traces re-ordered like oracle_run's (rows n = 0..nsteps)."""
+    tr = c.traces
+    if c.elastic:
+        nr = len(c.rec)
+        # elastic SismoPoint rows start with the displacement of each point
+        # (3 components per point), followed by 9 further fields per point
+        tr = tr[:, :3 * nr].reshape(tr.shape[0], nr, 3).transpose(0, 2, 1)
+        tr = tr.reshape(tr.shape[0], 3 * nr)
+    return tr
+
+
+def rel_l2(a, b) -> float:
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def gpu_problem(c: Case, precision: str = "fp64", nshots: int = 1, srcs=None):
+    """The same case on the HIP path (montjoie_amd); sources default to the
+    case's single source replicated for every shot."""
+    import montjoie_amd as M
+    m = M.regular_mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    if c.elastic:
+        P = M.WaveProblem(m, c.rho_e, c.mu_e, lam=c.lam_e, ncomp=3, nshots=nshots,
+                          precision=precision)
+    else:
+        P = M.WaveProblem(m, c.rho_e, c.mu_e, nshots=nshots, precision=precision)
+    if srcs is None:
+        srcs = [c.src] * nshots
+    pol = None if not c.elastic else np.vstack([c.pol] * len(srcs))
+    P.set_sources(np.asarray(srcs), pol=pol, shot=np.arange(len(srcs)) % nshots)
+    if c.elastic:
+        P.set_receivers(np.vstack([c.rec] * 3), comp=np.repeat(np.arange(3), len(c.rec)))
+    else:
+        P.set_receivers(c.rec)
+    return P
+
+
+def gpu_run(c: Case, precision="fp64", nsteps=None, graph=True, nshots=1, srcs=None):
+    import montjoie_amd as M
+    nsteps = c.nsteps if nsteps is None else nsteps
+    P = gpu_problem(c, precision, nshots, srcs)
+    P.set_pulse(M.ricker_pulse_table(c.f0, 0.0, c.dt, nsteps))
+    P.leapfrog_init(c.dt)
+    tr = P.run(nsteps, graph=graph)
+    return P, tr

@@ -1,0 +1,85 @@
+"""The C ABI library: loads, exports every symbol declared in include/*.h,
+and its host-only entry points (mesh generator, Ricker table, argument
+checks) agree with the oracle.  No compute call needs a GPU here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import montjoie_amd as M
+from montjoie_amd import _lib
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if f.endswith(".h"):
+            txt = open(os.path.join(inc, f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(mjw_\w+)\s*\(", txt))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # the Python binding types every declared symbol
+    assert set(names) <= set(_lib.SIGNATURES)
+
+
+def test_version():
+    assert _lib.lib().mjw_version() >= 10000
+
+
+@pytest.mark.parametrize("dim,nx,r,pert", [(2, 5, 3, 0.0), (2, 4, 6, 0.2), (3, 3, 4, 0.1),
+                                           (3, 2, 7, 0.05)])
+def test_mesh_generator_matches_oracle(dim, nx, r, pert):
+    m = M.regular_mesh(dim, nx, r, pert, seed=17)
+    o = O.mesh(dim, nx, r, pert, 17)
+    assert np.array_equal(m.xyz, o.xyz)
+    assert np.array_equal(m.elem_nodes, o.elem_nodes)
+    assert np.array_equal(m.dirichlet, o.dirichlet)
+
+
+@pytest.mark.parametrize("f0,dt,n", [(10.0, 1e-3, 300), (8.0, 0.00731, 200), (3.0, 0.01, 100)])
+def test_ricker_table_matches_oracle(f0, dt, n):
+    a = M.ricker_pulse_table(f0, 0.0, dt, n)
+    b = O.pulse_table(f0, 0.0, dt, n)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-13 * np.abs(b).max())
+
+
+def test_create_rejects_bad_arguments_without_device():
+    L = _lib.lib()
+    h = C.c_void_p()
+    xyz = np.zeros(6)
+    en = np.zeros(4, np.int32)
+    rho = np.ones(1)
+    # order 9 is unsupported -> error code, message, no crash
+    rc = L.mjw_create(C.byref(h), 0, 2, 9, 1, 3, _lib.dptr(xyz), 1, _lib.iptr(en),
+                      _lib.dptr(rho), _lib.dptr(rho), None, None, 1, 0)
+    assert rc == -4 and b"order" in L.mjw_last_error(None)
+    rc = L.mjw_create(C.byref(h), 0, 4, 2, 1, 3, _lib.dptr(xyz), 1, _lib.iptr(en),
+                      _lib.dptr(rho), _lib.dptr(rho), None, None, 1, 0)
+    assert rc == -1
+    with pytest.raises(M.MjwError):
+        M.ricker_pulse_table(-1.0, 0.0, 1e-3, 10)
+
+
+def test_product_does_not_import_oracle():
+    """The product package never routes through the checker."""
+    pkg = os.path.join(ROOT, "montjoie_amd")
+    for dp, _, fs in os.walk(pkg):
+        for f in fs:
+            if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
+                txt = open(os.path.join(dp, f)).read()
+                assert "import oracle" not in txt
+                assert "from oracle" not in txt and "mjw_oracle" not in txt

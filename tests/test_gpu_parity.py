@@ -1,0 +1,150 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and with
+Montjoie's own solutions (tests/golden).  Tolerances: FP64 paths 1e-10
+relative L2 against Montjoie (the requirement) and 1e-12 against the
+oracle; FP32 paths are compared with the FP64 result and their error is
+bounded explicitly."""
+import numpy as np
+import pytest
+
+import cases as CS
+import montjoie_amd as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dim,r,nx,pert", [(2, 1, 5, 0.0), (2, 4, 4, 0.15), (2, 7, 2, 0.1),
+                                           (3, 1, 4, 0.1), (3, 2, 3, 0.1), (3, 3, 3, 0.1),
+                                           (3, 4, 2, 0.1), (3, 5, 2, 0.1), (3, 6, 2, 0.1),
+                                           (3, 7, 2, 0.05)])
+def test_apply_matches_oracle_fp64(dim, r, nx, pert):
+    m = M.regular_mesh(dim, nx, r, pert, seed=5)
+    o = O.mesh(dim, nx, r, pert, 5)
+    mu = 1.0 + (np.arange(m.E) % 3)
+    rho = 1.0 + 0.5 * (np.arange(m.E) % 2)
+    P = M.WaveProblem(m, rho, mu)
+    G, Mloc = O.geometry(o, mu, rho)
+    cols = O.color(o)
+    rng = np.random.default_rng(r)
+    u = rng.standard_normal(m.N)
+    y0 = rng.standard_normal(m.N)
+    ref = -O.stiffness(o, G, u, cols)
+    got = P.apply_kh(u)
+    assert CS.rel_l2(got, ref) < 1e-12
+    # beta / alpha handling: Y = 0.5 Y + 2 K u
+    got2 = P.apply_kh(u, alpha=2.0, beta=0.5, Y=y0)
+    assert CS.rel_l2(got2, 0.5 * y0 + 2 * ref) < 1e-12
+    Mo, _ = O.mass(o, Mloc)
+    np.testing.assert_allclose(P.mass(), Mo, rtol=1e-13)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 3e-6), ("fp32_g64", 3e-6)])
+def test_apply_fp32(prec, tol):
+    m = M.regular_mesh(3, 3, 5, 0.1, seed=5)
+    o = O.mesh(3, 3, 5, 0.1, 5)
+    P = M.WaveProblem(m, 1.0, 2.0, precision=prec)
+    G, _ = O.geometry(o, 2.0, 1.0)
+    u = np.random.default_rng(0).standard_normal(m.N)
+    ref = -O.stiffness(o, G, u, O.color(o))
+    assert CS.rel_l2(P.apply_kh(u), ref) < tol
+
+
+@pytest.mark.parametrize("r,nx", [(1, 3), (2, 2), (3, 2), (4, 2), (5, 1), (6, 1), (7, 1)])
+def test_elastic_apply_matches_oracle(r, nx):
+    m = M.regular_mesh(3, nx, r, 0.1, seed=6)
+    o = O.mesh(3, nx, r, 0.1, 6)
+    lam = 1.0 + (np.arange(m.E) % 3)
+    mu = 2.0 + (np.arange(m.E) % 2)
+    P = M.WaveProblem(m, 1.0, mu, lam=lam, ncomp=3)
+    Po = O.elastic_problem(o, lam, mu, 1.0)
+    u = np.random.default_rng(r).standard_normal(3 * m.N)
+    assert CS.rel_l2(P.apply_kh(u), -O.apply(Po, u)) < 1e-12
+
+
+@pytest.mark.parametrize("name", CS.golden_names())
+def test_leapfrog_fp64_matches_montjoie(name):
+    c = CS.load(name)
+    P, tr = CS.gpu_run(c, "fp64")
+    ref = CS.reference_traces(c)
+    err_ref = CS.rel_l2(tr[:, 0, :], ref)
+    _, _, otr = CS.oracle_run(c)
+    err_or = CS.rel_l2(tr[:, 0, :], otr)
+    print(f"{name}: rel-L2 vs Montjoie {err_ref:.2e}, vs oracle {err_or:.2e}")
+    assert err_ref < 1e-10
+    assert err_or < 1e-12
+
+
+def test_leapfrog_final_state_matches_oracle():
+    c = CS.load("ref3d_q5_pert")
+    P, _ = CS.gpu_run(c, "fp64")
+    uN, uN1 = P.state()
+    oN, oN1, _ = CS.oracle_run(c)
+    assert CS.rel_l2(uN[0], oN) < 1e-12
+    assert CS.rel_l2(uN1[0], oN1) < 1e-12
+
+
+def test_graph_equals_eager_bitwise():
+    c = CS.load("ref3d_q3_pert")
+    _, a = CS.gpu_run(c, "fp64", graph=True)
+    _, b = CS.gpu_run(c, "fp64", graph=False)
+    assert np.array_equal(a, b)
+
+
+def test_run_to_run_deterministic():
+    c = CS.load("ref2d_q4_n8")
+    _, a = CS.gpu_run(c, "fp32")
+    _, b = CS.gpu_run(c, "fp32")
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp32_g64"])
+def test_fp32_error_against_fp64(prec):
+    c = CS.load("ref3d_q5_pert")
+    _, ref = CS.gpu_run(c, "fp64")
+    _, tr = CS.gpu_run(c, prec)
+    err = CS.rel_l2(tr, ref)
+    print(f"{prec}: traces rel-L2 vs FP64 {err:.2e}")
+    assert err < 1e-5
+
+
+def test_multishot_equals_single_shots():
+    c = CS.load("ref3d_q3_pert")
+    srcs = [c.src, (0.3, 0.6, 0.5), (0.55, 0.35, 0.62), (0.5, 0.5, 0.3)]
+    _, multi = CS.gpu_run(c, "fp64", nshots=4, srcs=srcs, nsteps=60)
+    for s, x in enumerate(srcs):
+        _, one = CS.gpu_run(c, "fp64", nshots=1, srcs=[x], nsteps=60)
+        assert np.array_equal(multi[:, s, :], one[:, 0, :])
+
+
+def test_every_tile_gives_identical_results():
+    c = CS.load("ref3d_q5_pert")
+    P = CS.gpu_problem(c, "fp64")
+    u = np.random.default_rng(1).standard_normal(P.ndof)
+    outs = []
+    for epb in (7, 14, 16):
+        P.set_tile(epb)
+        outs.append(P.apply_kh(u))
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def test_energy_conserved_fp64():
+    m = M.regular_mesh(3, 3, 4, 0.1, seed=9)
+    P = M.WaveProblem(m, 1.0, 1.0 + np.arange(m.E) % 2)
+    lam = P.lambda_max(100)
+    dt = 0.5 * 2 / np.sqrt(lam)
+    u0 = np.random.default_rng(4).standard_normal(m.N) * (1 - m.dirichlet)
+    P.leapfrog_init(dt, u0=u0)
+    E = []
+    for _ in range(4):
+        P.advance(50, traces=False)
+        E.append(P.energy()[0])
+    assert max(E) - min(E) < 1e-11 * max(E)
+
+
+def test_lambda_max_matches_oracle():
+    m = M.regular_mesh(2, 16, 4)
+    P = M.WaveProblem(m, 1.0, 1.0)
+    o = O.mesh(2, 16, 4)
+    lo = O.lambda_max(O.acoustic_problem(o, 1.0, 1.0), 150, 3)
+    lg = P.lambda_max(150)
+    assert abs(lg - lo) < 1e-7 * lo

@@ -1,0 +1,241 @@
+"""The oracle pinned: against the reference's own known-answer tables and
+solutions (tests/golden, made from Montjoie's sources), and against what the
+mathematics fixes.  CPU only."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cases as CS
+from oracle import oracle as O
+
+GOLD = CS.GOLD
+
+
+# ---------------------------------------------------------------- quadrature
+def test_gl_closed_forms():
+    # GaussLobattoPoints.cxx:62-85
+    x, w = O.gl(1)
+    assert np.array_equal(x, [0, 1]) and np.array_equal(w, [0.5, 0.5])
+    x, w = O.gl(2)
+    assert np.allclose(x, [0, 0.5, 1], rtol=0, atol=0) and np.allclose(w, [1 / 6, 2 / 3, 1 / 6],
+                                                                         rtol=1e-16)
+    x, w = O.gl(3)
+    a = (5 - np.sqrt(5)) / 10
+    assert np.allclose(x, [0, a, 1 - a, 1], atol=1e-15)
+    assert np.allclose(w, [1 / 12, 5 / 12, 5 / 12, 1 / 12], atol=1e-15)
+
+
+def test_gl_against_solin_table():
+    """Known-answer table shipped with the reference (Table_Quadrature.txt)."""
+    tab = json.load(open(os.path.join(GOLD, "gl_solin.json")))["rules"]
+    assert len(tab) >= 10
+    for n, rows in tab.items():
+        r = int(n) - 1
+        t = np.array([float(a) for a, _ in rows])[::-1]
+        wt = np.array([float(b) for _, b in rows])[::-1]
+        x, w = O.gl(r)
+        np.testing.assert_allclose(x, 0.5 * (1 + t), atol=2e-16)
+        np.testing.assert_allclose(w, 0.5 * wt, atol=3e-16)
+
+
+@pytest.mark.parametrize("r", range(1, 9))
+def test_gl_exactness(r):
+    """GL with r+1 points integrates x^k exactly for k <= 2r-1
+    (QuadratureTest.cc:144-171 pattern); x^{2r} is not exact."""
+    x, w = O.gl(r)
+    for k in range(2 * r):
+        assert abs(np.dot(w, x ** k) - 1.0 / (k + 1)) < 1e-14
+    assert abs(np.dot(w, x ** (2 * r)) - 1.0 / (2 * r + 1)) > 1e-12
+
+
+@pytest.mark.parametrize("r", range(1, 9))
+def test_derivative_matrix(r):
+    """D exact on polynomials of degree <= r, rows sum to 0, and equal to a
+    finite-difference derivative of the product-form basis."""
+    x, _ = O.gl(r)
+    D = O.gl_deriv(r)
+    assert np.abs(D.sum(axis=1)).max() < 1e-11
+    for k in range(r + 1):
+        np.testing.assert_allclose(D @ x ** k, k * x ** max(k - 1, 0) if k else 0 * x, atol=1e-10)
+    h = 1e-6
+    for b in range(r + 1):
+        fd = [(O.phi(r, b, t + h) - O.phi(r, b, t - h)) / (2 * h) for t in x]
+        np.testing.assert_allclose(D[:, b], fd, atol=1e-6 * max(1, np.abs(D).max()))
+
+
+def test_pulse_matches_closed_form_ricker():
+    """The GL-24 interpolant's derivative tracks the analytic Ricker
+    (TimeSource.cxx:55-67) to interpolation accuracy and is 0 after 2.5/f0."""
+    f0, dt, n = 10.0, 1e-3, 400
+    p = O.pulse_table(f0, 0.0, dt, n)
+    t = np.arange(n) * dt + dt
+    exact = np.array([O.ricker(f0, s) for s in t])
+    live = t < 2.5 / f0
+    assert np.abs(p[live] - exact[live]).max() < 1e-6
+    assert np.all(p[~live] == 0.0)
+
+
+# ---------------------------------------------------------------- operators
+@pytest.mark.parametrize("dim,r,pert", [(2, 1, 0.0), (2, 4, 0.15), (3, 2, 0.1), (3, 5, 0.1)])
+def test_stiffness_invariants(dim, r, pert):
+    """A 1 = 0 without Dirichlet, symmetry u^T A v = v^T A u, and
+    u^T A u = int mu |grad u|^2 exactly for linear u."""
+    m = O.mesh(dim, 3, r, pert, 3)
+    mu = 1.0 + np.arange(m.E) % 3
+    G, _ = O.geometry(m, mu, 1.0)
+    cols = O.color(m)
+    one = np.ones(m.N)
+    assert np.abs(O.stiffness(m, G, one, cols)).max() < 1e-11
+    rng = np.random.default_rng(0)
+    u, v = rng.standard_normal((2, m.N))
+    Au, Av = O.stiffness(m, G, u, cols), O.stiffness(m, G, v, cols)
+    assert abs(v @ Au - u @ Av) < 1e-11 * np.abs(u) @ np.abs(Av)
+    a = np.arange(1, dim + 1) * 0.7
+    lin = m.xyz @ a
+    # exact energy: sum_e mu_e |a|^2 vol_e
+    _, Mloc = O.geometry(m, mu, 1.0)
+    vol_e = Mloc.reshape(m.E, -1).sum(axis=1)
+    exact = np.sum(mu * vol_e) * (a @ a)
+    assert abs(lin @ O.stiffness(m, G, lin, cols) - exact) < 1e-11 * exact
+
+
+@pytest.mark.parametrize("dim,r", [(2, 1), (2, 3), (3, 1), (3, 2)])
+def test_stiffness_equals_assembled_matrix(dim, r):
+    """Matrix-free product == column-by-column assembled matrix built from
+    brute-force element integrals with dense Lagrange derivatives
+    (test_matrix_vector_product.cc:111-200 pattern), affine mesh, mu = 1."""
+    nx = 2
+    m = O.mesh(dim, nx, r)
+    G, _ = O.geometry(m, 1.0, 1.0)
+    h = 1.0 / nx
+    x, w = O.gl(r)
+    D = O.gl_deriv(r)
+    n = r + 1
+    # reference-element stiffness on [0,h]^dim: sum_q w_q h^dim grad phi_a . grad phi_b
+    nd = n ** dim
+    idx = np.array(np.unravel_index(np.arange(nd), (n,) * dim, order="F")).T
+    K = np.zeros((nd, nd))
+    for q in range(nd):
+        wq = np.prod(w[idx[q]]) * h ** dim
+        grads = np.zeros((nd, dim))
+        for a in range(nd):
+            for d in range(dim):
+                g = D[idx[q][d], idx[a][d]] / h
+                for d2 in range(dim):
+                    if d2 != d:
+                        g *= 1.0 if idx[a][d2] == idx[q][d2] else 0.0
+                grads[a, d] = g
+        K += wq * grads @ grads.T
+    A = np.zeros((m.N, m.N))
+    for e in range(m.E):
+        en = m.elem_nodes[e]
+        A[np.ix_(en, en)] += K
+    cols = O.color(m)
+    for j in range(0, m.N, max(1, m.N // 7)):
+        ej = np.zeros(m.N)
+        ej[j] = 1.0
+        np.testing.assert_allclose(O.stiffness(m, G, ej, cols), A[:, j], atol=1e-11)
+
+
+def test_colouring_is_conflict_free():
+    m = O.mesh(3, 4, 2, 0.1, 1)
+    c, nc = O.color(m)
+    assert nc == 8
+    for col in range(nc):
+        nodes = m.elem_nodes[c == col].ravel()
+        assert len(np.unique(nodes)) == len(nodes)
+
+
+def test_mass_sums_to_volume():
+    m = O.mesh(3, 3, 3, 0.12, 4)
+    _, Mloc = O.geometry(m, 1.0, 2.5)
+    M, _ = O.mass(m, Mloc, dirichlet=False)
+    assert abs(M.sum() - 2.5) < 1e-13
+
+
+def test_dirac_at_vertex_is_unit_vector_and_partition_of_unity():
+    m = O.mesh(2, 8, 4)
+    b = O.dirac(m, (0.5, 0.5))
+    assert np.count_nonzero(b) == 1 and abs(b.max() - 1.0) < 1e-15
+    m3 = O.mesh(3, 3, 3, 0.1, 2)
+    b3 = O.dirac(m3, (0.47, 0.51, 0.53))
+    assert abs(b3.sum() - 1.0) < 1e-13
+    # exact for polynomials: sum_i b_i p(x_i) = p(x0) for p of degree <= r
+    p = lambda X: 1 + X[..., 0] ** 2 - 2 * X[..., 1] * X[..., 2]  # noqa: E731
+    assert abs(b3 @ p(m3.xyz) - p(np.array([0.47, 0.51, 0.53]))) < 1e-12
+
+
+# ---------------------------------------------------------------- leapfrog
+def test_energy_conserved_without_source():
+    """E^{n+1/2} = 1/2|(U^{n+1}-U^n)/dt|_M^2 + 1/2 U^{n+1} A U^n is constant."""
+    m = O.mesh(2, 6, 3, 0.1, 8)
+    P = O.acoustic_problem(m, 1.0 + np.arange(m.E) % 2, 1.0)
+    rng = np.random.default_rng(1)
+    u0 = rng.standard_normal(m.N) * (1 - m.dirichlet)
+    lam = O.lambda_max(P)
+    dt = 0.5 * 2 / np.sqrt(lam)
+    Es = []
+    for ns in (5, 60, 200):
+        uN, uN1, _ = O.leapfrog(P, None, None, dt, ns, None, None, u0)
+        E = 0.5 * np.sum(P.M * (uN1 - uN) ** 2) / dt ** 2 + 0.5 * uN1 @ O.apply(P, uN)
+        Es.append(E)
+    assert max(Es) - min(Es) < 1e-12 * max(Es)
+
+
+def test_leapfrog_second_order():
+    """Error against a 4x finer time step shrinks like dt^2
+    (explicit_scheme.cc:632-749 pattern)."""
+    m = O.mesh(2, 4, 3)
+    P = O.acoustic_problem(m, 1.0, 1.0)
+    u0 = np.sin(np.pi * m.xyz[:, 0]) * np.sin(np.pi * m.xyz[:, 1])
+    T = 0.2
+
+    def run(dt):
+        n = int(round(T / dt))
+        uN, _, _ = O.leapfrog(P, None, None, dt, n, None, None, u0)
+        return uN
+
+    ref = run(T / 1600)
+    e1 = np.linalg.norm(run(T / 100) - ref)
+    e2 = np.linalg.norm(run(T / 200) - ref)
+    assert abs(np.log2(e1 / e2) - 2.0) < 0.05
+
+
+@pytest.mark.parametrize("name", CS.golden_names())
+def test_oracle_matches_montjoie(name):
+    """Traces of Montjoie's own solver (tests/golden, 15 significant digits)."""
+    c = CS.load(name)
+    _, _, tr = CS.oracle_run(c)
+    ref = CS.reference_traces(c)
+    assert tr.shape == ref.shape
+    assert CS.rel_l2(tr, ref) < 1e-12
+
+
+def test_lambda_max_bounds_cfl():
+    """dt just above 2/sqrt(lambda_max) blows up, just below stays bounded."""
+    m = O.mesh(2, 4, 3)
+    P = O.acoustic_problem(m, 1.0, 1.0)
+    lam = O.lambda_max(P)
+    rng = np.random.default_rng(2)
+    u0 = rng.standard_normal(m.N) * (1 - m.dirichlet)
+    dmax = 2 / np.sqrt(lam)
+    ok, _, _ = O.leapfrog(P, None, None, 0.98 * dmax, 3000, None, None, u0)
+    bad, _, _ = O.leapfrog(P, None, None, 1.02 * dmax, 3000, None, None, u0)
+    assert np.abs(ok).max() < 1e3 * np.abs(u0).max()
+    assert not (np.abs(bad).max() < 1e6 * np.abs(u0).max())  # inf / nan count as blow-up
+
+
+def test_elastic_rigid_motions_and_symmetry():
+    m = O.mesh(3, 2, 2, 0.1, 5)
+    P = O.elastic_problem(m, 2.0, 1.5, 1.0)
+    # rigid translation and infinitesimal rotation are in the kernel
+    t = np.concatenate([np.full(m.N, 1.0), np.full(m.N, -2.0), np.full(m.N, 0.5)])
+    X = m.xyz
+    rot = np.concatenate([-X[:, 1], X[:, 0], 0 * X[:, 0]])
+    for v in (t, rot):
+        assert np.abs(O.apply(P, v)).max() < 1e-11
+    rng = np.random.default_rng(3)
+    u, v = rng.standard_normal((2, 3 * m.N))
+    assert abs(v @ O.apply(P, u) - u @ O.apply(P, v)) < 1e-10
