@@ -689,11 +689,14 @@ extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, cons
   return MJW_OK;
 }
 
-// one fused step: colour passes (U^{n+1} over U^{n-1}) + tail
+// One fused leapfrog step n (Advance, ModifiedEquation.cxx:295-345): with
+// U^n in d_U[parity] and U^{n+1} in d_U[parity^1], the colour passes apply
+// the stiffness to U^{n+1} and overwrite U^n with U^{n+2}; the tail adds the
+// source and records the traces of U^{n+1} (row n+1).
 static int enqueue_step(mjw_problem *p, int parity, double *traces) {
-  void *U = p->d_U[parity], *Y = p->d_U[parity ^ 1];
+  void *Ucur = p->d_U[parity ^ 1], *Y = p->d_U[parity];
   for (int c = 0; c < p->ncolor; ++c) {
-    StiffArgs a = stiff_args(p, kLeap, U, Y, p->nshots);
+    StiffArgs a = stiff_args(p, kLeap, Ucur, Y, p->nshots);
     a.e0 = p->coff[c];
     a.e1 = p->coff[c + 1];
     int rc = launch_stiffness(a, p->stream);
@@ -703,14 +706,24 @@ static int enqueue_step(mjw_problem *p, int parity, double *traces) {
   t.state = p->state; t.ctl = p->d_ctl; t.pulse = p->d_pulse; t.npulse = p->npulse;
   t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot; t.src_val = p->d_src_val;
   t.sdiag = p->d_sdiag; t.nrec = traces ? p->nrec : 0; t.nd = p->nd; t.rec_gid = p->d_rec_gid;
-  t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = U; t.P = Y;
+  t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = Ucur; t.P = Y;
   t.traces = (traces && p->nrec) ? traces : nullptr; t.N = p->N; t.nshots = p->nshots;
   t.advance = 1;
+  t.row_offset = 1;
+  return launch_tail_n(t, p->ncomp, p->stream);
+}
+
+// traces of U^n (d_U[cur]) into row step - base (no source, no step change)
+static int enqueue_record(mjw_problem *p, double *traces) {
+  TailArgs t{};
+  t.state = p->state; t.ctl = p->d_ctl; t.nrec = p->nrec; t.nd = p->nd; t.rec_gid = p->d_rec_gid;
+  t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = p->d_U[p->cur]; t.P = nullptr;
+  t.traces = traces; t.N = p->N; t.nshots = p->nshots; t.advance = 0; t.row_offset = 0;
   return launch_tail_n(t, p->ncomp, p->stream);
 }
 
 static int ensure_traces(mjw_problem *p, int64_t nsteps) {
-  const int64_t need = nsteps * p->nshots * std::max(p->nrec, 1);
+  const int64_t need = (nsteps + 1) * p->nshots * std::max(p->nrec, 1);
   if (need <= p->traces_cap) return 0;
   dfree(p->d_traces);
   p->d_traces = nullptr;
@@ -757,6 +770,7 @@ extern "C" int mjw_leapfrog_advance(mjw_problem *p, int nsteps, double *traces, 
   hipEvent_t ev0, ev1;
   HIPCHK(p, hipEventCreate(&ev0));
   HIPCHK(p, hipEventCreate(&ev1));
+  if (want) KCHK(p, enqueue_record(p, p->d_traces));  // row 0 = U^n
   HIPCHK(p, hipEventRecord(ev0, st));
   int done = 0;
   if (use_graph && nsteps >= kGraphSteps) {
@@ -801,11 +815,7 @@ extern "C" int mjw_record(mjw_problem *p, double *traces) {
   KCHK(p, ensure_traces(p, 1));
   int64_t ctl[2] = {p->step, p->step};
   HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, p->stream));
-  TailArgs t{};
-  t.state = p->state; t.ctl = p->d_ctl; t.nrec = p->nrec; t.nd = p->nd; t.rec_gid = p->d_rec_gid;
-  t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = p->d_U[p->cur]; t.P = nullptr;
-  t.traces = p->d_traces; t.N = p->N; t.nshots = p->nshots; t.advance = 0;
-  KCHK(p, launch_tail_n(t, p->ncomp, p->stream));
+  KCHK(p, enqueue_record(p, p->d_traces));
   HIPCHK(p, hipMemcpyAsync(traces, p->d_traces, (size_t)p->nshots * p->nrec * 8,
                            hipMemcpyDeviceToHost, p->stream));
   HIPCHK(p, hipStreamSynchronize(p->stream));
@@ -854,7 +864,7 @@ extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
   int k = 0;
   HIPCHK(p, hipEventRecord(ev[k++], st));
   for (int s = 0; s < nsteps; ++s) {
-    void *U = p->d_U[p->cur], *Y = p->d_U[p->cur ^ 1];
+    void *U = p->d_U[p->cur ^ 1], *Y = p->d_U[p->cur];
     for (int c = 0; c < p->ncolor; ++c) {
       StiffArgs a = stiff_args(p, kLeap, U, Y, p->nshots);
       a.e0 = p->coff[c];
@@ -866,7 +876,7 @@ extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
     t.state = p->state; t.ctl = p->d_ctl; t.pulse = p->d_pulse; t.npulse = p->npulse;
     t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot;
     t.src_val = p->d_src_val; t.sdiag = p->d_sdiag; t.nrec = 0; t.nd = p->nd; t.U = U; t.P = Y;
-    t.N = p->N; t.nshots = p->nshots; t.advance = 1;
+    t.N = p->N; t.nshots = p->nshots; t.advance = 1; t.row_offset = 1;
     KCHK(p, launch_tail_n(t, p->ncomp, st));
     HIPCHK(p, hipEventRecord(ev[k++], st));
     p->cur ^= 1;

@@ -96,6 +96,7 @@ struct TailArgs {
   int64_t N;
   int nshots;
   int advance;                   // 1: increment step after
+  int row_offset;                // trace row = step - base + row_offset
 };
 int launch_tail_n(const TailArgs &a, int ncomp, void *stream);
 

@@ -168,11 +168,12 @@ int launch_mass_assemble(const int32_t *map, const double *Mloc, int64_t E, int 
 }
 
 // ---------------------------------------------------------------------------
-// Tail of a leapfrog step (one block): Dirac source injection into U^{n+1}
-// (AddScalarSourceAtTime, VarInstationary.cxx:2391-2409, scaled by dt^2/M as
-// the solve does, ModifiedEquation.cxx:316-333) and receiver interpolation of
-// U^n (GiveIterate -> WriteSnapshot, VarInstationary.cxx:798-853).  The step
-// index lives on the device so a captured graph can be replayed.
+// Tail of leapfrog step n (one block): Dirac source injection into U^{n+2}
+// being formed (AddScalarSourceAtTime at t_n + dt, VarInstationary.cxx:
+// 2391-2409, scaled by dt^2/M as the solve does, ModifiedEquation.cxx:316-333)
+// and receiver interpolation of U^{n+1}, the iterate the next step hands to
+// GiveIterate (VarInstationary.cxx:798-853).  The step index lives on the
+// device so a captured graph can be replayed.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_tail(TailArgs a, int ncomp) {
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(256) void k_tail(TailArgs a, int ncomp) {
       double acc = 0.0;
       for (int q = 0; q < a.nd; ++q)
         acc += a.rec_w[(int64_t)r * a.nd + q] * (double)Us[a.rec_gid[(int64_t)r * a.nd + q]];
-      a.traces[((step - base) * a.nshots + s) * a.nrec + r] = acc;
+      a.traces[((step - base + a.row_offset) * a.nshots + s) * a.nrec + r] = acc;
     }
   }
   __syncthreads();
