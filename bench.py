@@ -26,6 +26,10 @@ import time
 
 import numpy as np
 
+# torch ships its own HIP runtime under the same soname as ROCm's; load it
+# before libmjwave.so so both use one runtime in this process
+import torch  # noqa: E402
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -69,7 +73,6 @@ def flops_per_step(E, n):
 
 
 def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10):
-    import torch
     import montjoie_amd as M
     P = M.WaveProblem(m, 1.0, mu, precision=prec, nshots=nshots)
     srcs = [(0.5, 0.5, 0.5)] * nshots
