@@ -142,6 +142,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32_g64", "fp32"], default=None,
+                    help="run one precision only (profiling runs)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 1 GiB device copy (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}); the multi-GPU sharding "
@@ -157,13 +161,22 @@ def main():
     dt = 0.5 * 2.0 / np.sqrt(lam)
     print(f"# setup: N={m.N} E={m.E} lambda_max={lam:.6e} dt={dt:.6e} "
           f"({time.perf_counter() - t_setup:.1f} s)", file=sys.stderr, flush=True)
+    if a.calib_copy:
+        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("# calibration: 3 x 1 GiB device copies", file=sys.stderr, flush=True)
     res = {}
-    for prec in ("fp64", "fp32_g64", "fp32"):
+    for prec in ((a.only,) if a.only else ("fp64", "fp32_g64", "fp32")):
         res[prec] = run_gpu(m, mu, dt, prec, a.steps, a.warmup)
         r = res[prec]
         print(f"# {prec}: {r['wall'] / a.steps * 1e3:.4f} ms/step wall, "
               f"{r['dev_ms_step']:.4f} ms/step device, stiffness launch "
               f"{r['prof']['stiffness_launch_ms']:.4f} ms", file=sys.stderr, flush=True)
+    if a.only:
+        return
     ref = res["fp64"]
     for prec in ("fp32_g64", "fp32"):
         r = res[prec]

@@ -16,6 +16,7 @@
 // shot loop as in the scalar kernels (mjw_kernels_scalar.hip).
 #include <hip/hip_runtime.h>
 
+#include "mjw_device.hpp"
 #include "mjw_internal.hpp"
 
 namespace mjw {
@@ -34,19 +35,16 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
   const int i = ij % N, j = ij / N;
   const int64_t e = e0 + (int64_t)blockIdx.x * EPB + el;
   const bool act = e < e1;
-  T Di[N], Dj[N], Dti[N], Dtj[N];
-#pragma unroll
-  for (int l = 0; l < N; ++l) {
-    Di[l] = Dm[i * N + l];
-    Dj[l] = Dm[j * N + l];
-    Dti[l] = Dm[l * N + i];
-    Dtj[l] = Dm[l * N + j];
-  }
+  const int64_t eb = e0 + (int64_t)blockIdx.x * EPB;
+  const int64_t plane = E * P;
+  const uint32_t lo = (uint32_t)(el * P + ij);
+  const uint32_t g_ps = (uint32_t)(plane * sizeof(TG));
   int32_t gid[N];
   A lam = 0, mu = 0;
   if (act) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) gid[k] = map[(k * E + e) * P + ij];
+    for (int k = 0; k < N; ++k)
+      gid[k] = bld<int32_t>(mkres(map + eb * P), lo * 4u, (uint32_t)(k * plane * 4));
     lam = (A)lam_e[e];
     mu = (A)mu_e[e];
   }
@@ -55,22 +53,30 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
   T *pl[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) pl[c] = lds + (c * EPB + el) * NN;
+  const rsrc_t rs = mkres(sdiag);
+  const uint32_t comp_ps = (uint32_t)(N_nodes * sizeof(T));
   for (int s = 0; s < nshots; ++s) {
-    const T *Us = U + (int64_t)s * 3 * N_nodes;
-    T *Ys = Y + (int64_t)s * 3 * N_nodes;
+    const rsrc_t ru = mkres(U + (int64_t)s * 3 * N_nodes);
+    const rsrc_t ry = mkres(Y + (int64_t)s * 3 * N_nodes);
     T u[3][N];
     if (act) {
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-          u[m][k] = Us[m * N_nodes + (gid[k] & 0x7fffffff)];
+          u[m][k] = bld<T>(ru, (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T), m * comp_ps);
           pl[m][k * P + ij] = u[m][k];
         }
     }
     __syncthreads();
     T v0[3][N], v1[3][N], v2[3][N];
     if (act) {
+      T Di[N], Dj[N];
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        Di[l] = Dm[i * N + l];
+        Dj[l] = Dm[j * N + l];
+      }
 #pragma unroll
       for (int k = 0; k < N; ++k) {
         A g[3][3];
@@ -88,8 +94,9 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
           g[m][2] = a2;
         }
         A M[9];
+        const rsrc_t rg = mkres(G + (k * 9) * plane + eb * P);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) M[c] = (A)G[((k * 9 + c) * E + e) * P + ij];
+        for (int c = 0; c < 9; ++c) M[c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
         A du[3][3];
 #pragma unroll
         for (int m = 0; m < 3; ++m)
@@ -123,13 +130,19 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
     }
     __syncthreads();
     if (act) {
+      T Dti[N], Dtj[N];
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        Dti[l] = Dm[l * N + i];
+        Dtj[l] = Dm[l * N + j];
+      }
 #pragma unroll
       for (int k = 0; k < N; ++k) {
         const int32_t mm = gid[k];
-        const int64_t g = mm & 0x7fffffff;
+        const uint32_t off = (uint32_t)(mm & 0x7fffffff) * (uint32_t)sizeof(T);
         const bool first = mm < 0;
         T sgd = 0;
-        if (MODE == kLeap) sgd = sdiag[g];
+        if (MODE == kLeap) sgd = bld<T>(rs, off, 0u);
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
           T o = 0;
@@ -139,12 +152,14 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
             o += Dtj[l] * pl[3 + m][k * P + l * N + i];
             o += Dm[l * N + k] * v2[m][l];
           }
-          T *y = Ys + m * N_nodes + g;
+          const T y = bld<T>(ry, off, m * comp_ps);
+          T r;
           if (MODE == kLeap) {
-            *y = first ? (T(2) * u[m][k] - *y - sgd * o) : (*y - sgd * o);
+            r = first ? (T(2) * u[m][k] - y - sgd * o) : (y - sgd * o);
           } else {
-            *y = first ? (beta == T(0) ? alpha * o : beta * *y + alpha * o) : (*y + alpha * o);
+            r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
           }
+          bst<T>(r, ry, off, m * comp_ps);
         }
       }
     }

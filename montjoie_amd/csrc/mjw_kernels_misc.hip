@@ -175,8 +175,10 @@ int launch_mass_assemble(const int32_t *map, const double *Mloc, int64_t E, int 
 // GiveIterate (VarInstationary.cxx:798-853).  The step index lives on the
 // device so a captured graph can be replayed.
 // ---------------------------------------------------------------------------
+constexpr int kTailThreads = 1024;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_tail(TailArgs a, int ncomp) {
+__global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs a, int ncomp) {
   const int64_t step = a.ctl[0];
   const int64_t base = a.ctl[1];
   const T *U = static_cast<const T *>(a.U);
@@ -194,14 +196,18 @@ __global__ __launch_bounds__(256) void k_tail(TailArgs a, int ncomp) {
     }
   }
   if (a.traces) {
+    // one wavefront per (receiver, shot), lanes over the element's nodes
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int tot = a.nrec * a.nshots;
-    for (int it = threadIdx.x; it < tot; it += blockDim.x) {
+    for (int it = wave; it < tot; it += nw) {
       const int r = it % a.nrec, s = it / a.nrec;
       const T *Us = U + (int64_t)s * NT + (int64_t)a.rec_comp[r] * a.N;
       double acc = 0.0;
-      for (int q = 0; q < a.nd; ++q)
+      for (int q = lane; q < a.nd; q += 64)
         acc += a.rec_w[(int64_t)r * a.nd + q] * (double)Us[a.rec_gid[(int64_t)r * a.nd + q]];
-      a.traces[((step - base + a.row_offset) * a.nshots + s) * a.nrec + r] = acc;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lane == 0) a.traces[((step - base + a.row_offset) * a.nshots + s) * a.nrec + r] = acc;
     }
   }
   __syncthreads();
@@ -211,12 +217,11 @@ __global__ __launch_bounds__(256) void k_tail(TailArgs a, int ncomp) {
 int launch_tail_n(const TailArgs &a, int ncomp, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (a.state == kF64)
-    hipLaunchKernelGGL(k_tail<double>, dim3(1), dim3(256), 0, st, a, ncomp);
+    hipLaunchKernelGGL(k_tail<double>, dim3(1), dim3(kTailThreads), 0, st, a, ncomp);
   else
-    hipLaunchKernelGGL(k_tail<float>, dim3(1), dim3(256), 0, st, a, ncomp);
+    hipLaunchKernelGGL(k_tail<float>, dim3(1), dim3(kTailThreads), 0, st, a, ncomp);
   return (int)hipGetLastError();
 }
-
 
 // ------------------------------------------------------------ vector helpers
 template <typename T>

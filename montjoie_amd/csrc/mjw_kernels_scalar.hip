@@ -25,6 +25,7 @@
 //    major state, each shot a contiguous vector).
 #include <hip/hip_runtime.h>
 
+#include "mjw_device.hpp"
 #include "mjw_internal.hpp"
 
 namespace mjw {
@@ -46,43 +47,49 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
   const int t = threadIdx.x;
   const int el = t / P, ij = t - el * P;
   const int i = ij % N, j = ij / N;
-  const int64_t e = e0 + (int64_t)blockIdx.x * EPB + el;
-  const bool act = e < e1;
-  T Di[N], Dj[N], Dti[N], Dtj[N];
-#pragma unroll
-  for (int l = 0; l < N; ++l) {
-    Di[l] = Dm[i * N + l];
-    Dj[l] = Dm[j * N + l];
-    Dti[l] = Dm[l * N + i];
-    Dtj[l] = Dm[l * N + j];
-  }
+  // block-uniform resources; per-lane byte offsets are 32-bit
+  const int64_t eb = e0 + (int64_t)blockIdx.x * EPB;
+  const int64_t plane = E * P;
+  const uint32_t lo = (uint32_t)(el * P + ij);
+  const bool act = eb + el < e1;
+  const rsrc_t rmap = mkres(map + eb * P);
+  const uint32_t map_ps = (uint32_t)(plane * 4);
+  const uint32_t g_ps = (uint32_t)(plane * sizeof(TG));
   int32_t gid[N];
   A g6[HOIST ? N : 1][6];  // geometry of the column, kept across shots
   if (act) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      gid[k] = map[(k * E + e) * P + ij];
+      gid[k] = bld<int32_t>(rmap, lo * 4u, k * map_ps);
       if constexpr (HOIST) {
+        const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) g6[k][c] = (A)G[((k * 6 + c) * E + e) * P + ij];
+        for (int c = 0; c < 6; ++c) g6[k][c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
       }
     }
   }
+  const rsrc_t rs = mkres(sdiag);
   T *su = lu + el * NN, *sv = lv + el * NN;
   for (int s = 0; s < nshots; ++s) {
-    const T *Us = U + (int64_t)s * N_nodes;
-    T *Ys = Y + (int64_t)s * N_nodes;
+    const rsrc_t ru = mkres(U + (int64_t)s * N_nodes);
+    const rsrc_t ry = mkres(Y + (int64_t)s * N_nodes);
     T u[N];
     if (act) {
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        u[k] = Us[gid[k] & 0x7fffffff];
+        u[k] = bld<T>(ru, (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
         su[k * P + ij] = u[k];
       }
     }
     __syncthreads();
     T v1[N], v2[N], v3[N];
     if (act) {
+      T Di[N], Dj[N];
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        Di[l] = Dm[i * N + l];
+        Dj[l] = Dm[j * N + l];
+      }
 #pragma unroll
       for (int k = 0; k < N; ++k) {
         T a1 = 0, a2 = 0, a3 = 0;
@@ -94,9 +101,14 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
         }
         const A b1 = a1, b2 = a2, b3 = a3;
         A gk[6];
+        if constexpr (HOIST) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
-          gk[c] = HOIST ? g6[HOIST ? k : 0][c] : (A)G[((k * 6 + c) * E + e) * P + ij];
+          for (int c = 0; c < 6; ++c) gk[c] = g6[HOIST ? k : 0][c];
+        } else {
+          const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
+#pragma unroll
+          for (int c = 0; c < 6; ++c) gk[c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
+        }
         v1[k] = (T)(gk[0] * b1 + gk[1] * b2 + gk[2] * b3);
         v2[k] = (T)(gk[1] * b1 + gk[3] * b2 + gk[4] * b3);
         v3[k] = (T)(gk[2] * b1 + gk[4] * b2 + gk[5] * b3);
@@ -112,6 +124,12 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
     }
     __syncthreads();
     if (act) {
+      T Dti[N], Dtj[N];
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        Dti[l] = Dm[l * N + i];
+        Dtj[l] = Dm[l * N + j];
+      }
 #pragma unroll
       for (int k = 0; k < N; ++k) {
         T o = 0;
@@ -122,14 +140,17 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
           o += Dm[l * N + k] * v3[l];
         }
         const int32_t m = gid[k];
-        const int64_t g = m & 0x7fffffff;
+        const uint32_t off = (uint32_t)(m & 0x7fffffff) * (uint32_t)sizeof(T);
         const bool first = m < 0;
+        const T y = bld<T>(ry, off, 0u);
+        T r;
         if (MODE == kLeap) {
-          const T sg = sdiag[g];
-          Ys[g] = first ? (T(2) * u[k] - Ys[g] - sg * o) : (Ys[g] - sg * o);
+          const T sg = bld<T>(rs, off, 0u);
+          r = first ? (T(2) * u[k] - y - sg * o) : (y - sg * o);
         } else {
-          Ys[g] = first ? (beta == T(0) ? alpha * o : beta * Ys[g] + alpha * o) : (Ys[g] + alpha * o);
+          r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
         }
+        bst<T>(r, ry, off, 0u);
       }
     }
     __syncthreads();
@@ -148,8 +169,10 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
   const int t = threadIdx.x;
   const int el = t / P, q = t - el * P;
   const int i = q % N, j = q / N;
-  const int64_t e = e0 + (int64_t)blockIdx.x * EPB + el;
-  const bool act = e < e1;
+  const int64_t eb = e0 + (int64_t)blockIdx.x * EPB;
+  const int64_t plane = E * P;
+  const uint32_t lo = (uint32_t)(el * P + q);
+  const bool act = eb + el < e1;
   T Di[N], Dj[N], Dti[N], Dtj[N];
 #pragma unroll
   for (int l = 0; l < N; ++l) {
@@ -161,19 +184,22 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
   int32_t gid = 0;
   A g3[3] = {0, 0, 0};
   if (act) {
-    gid = map[e * P + q];
+    gid = bld<int32_t>(mkres(map + eb * P), lo * 4u, 0u);
+    const rsrc_t rg = mkres(G + eb * P);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) g3[c] = (A)G[(c * E + e) * P + q];
+    for (int c = 0; c < 3; ++c)
+      g3[c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * (uint32_t)(plane * sizeof(TG)));
   }
   T *su = lu + el * P, *sv = lv + el * P;
-  const int64_t g = gid & 0x7fffffff;
+  const uint32_t off = (uint32_t)(gid & 0x7fffffff) * (uint32_t)sizeof(T);
   const bool first = gid < 0;
+  const rsrc_t rs = mkres(sdiag);
   for (int s = 0; s < nshots; ++s) {
-    const T *Us = U + (int64_t)s * N_nodes;
-    T *Ys = Y + (int64_t)s * N_nodes;
+    const rsrc_t ru = mkres(U + (int64_t)s * N_nodes);
+    const rsrc_t ry = mkres(Y + (int64_t)s * N_nodes);
     T u = 0;
     if (act) {
-      u = Us[g];
+      u = bld<T>(ru, off, 0u);
       su[q] = u;
     }
     __syncthreads();
@@ -202,12 +228,15 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
         o += Dti[l] * su[j * N + l];
         o += Dtj[l] * sv[l * N + i];
       }
+      const T y = bld<T>(ry, off, 0u);
+      T r;
       if (MODE == kLeap) {
-        const T sg = sdiag[g];
-        Ys[g] = first ? (T(2) * u - Ys[g] - sg * o) : (Ys[g] - sg * o);
+        const T sg = bld<T>(rs, off, 0u);
+        r = first ? (T(2) * u - y - sg * o) : (y - sg * o);
       } else {
-        Ys[g] = first ? (beta == T(0) ? alpha * o : beta * Ys[g] + alpha * o) : (Ys[g] + alpha * o);
+        r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
       }
+      bst<T>(r, ry, off, 0u);
     }
     __syncthreads();
   }
