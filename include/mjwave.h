@@ -96,6 +96,10 @@ int mjw_info(const mjw_problem *p, int64_t *info9);
  * Used by the autotuner (montjoie_amd/tuner.py). */
 int mjw_set_tile(mjw_problem *p, int elems_per_block);
 int mjw_get_tile(const mjw_problem *p, int *elems_per_block);
+/* Block order of the stiffness launches: blocks that the hardware places on
+ * one XCD (round-robin over nxcd dies) get one contiguous run of elements, so
+ * neighbours share an L2.  nxcd = 1 disables it; default 8 (MI355X). */
+int mjw_set_xcd_order(mjw_problem *p, int nxcd);
 
 /* ---------------------------------------------------------------- operators */
 /*

@@ -30,6 +30,7 @@ SIGNATURES = {
     "mjw_info": (C.c_int, [_vp, _i64p]),
     "mjw_set_tile": (C.c_int, [_vp, C.c_int]),
     "mjw_get_tile": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "mjw_set_xcd_order": (C.c_int, [_vp, C.c_int]),
     "mjw_apply_kh": (C.c_int, [_vp, C.c_double, _dp, C.c_double, _dp]),
     "mjw_apply_kh_dev": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp]),
     "mjw_apply_dh": (C.c_int, [_vp, C.c_double, _dp, C.c_double, _dp]),

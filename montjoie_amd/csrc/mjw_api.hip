@@ -33,6 +33,7 @@ struct mjw_problem {
   std::vector<int64_t> coff;    // colour offsets in sorted order
   std::vector<int64_t> perm;    // sorted -> original element
   int epb = 0;
+  int nxcd = 8;                 // MI355X: 8 XCDs, round-robin block dispatch
   // host copies (original element order)
   std::vector<double> xyz;
   std::vector<int32_t> en;
@@ -415,6 +416,16 @@ extern "C" int mjw_set_tile(mjw_problem *p, int epb) {
   return MJW_OK;
 }
 
+extern "C" int mjw_set_xcd_order(mjw_problem *p, int nxcd) {
+  if (!p || nxcd < 1 || nxcd > 64) FAIL(p, MJW_ERR_ARG, "mjw_set_xcd_order: bad args");
+  if (nxcd != p->nxcd && p->gexec) {
+    (void)hipGraphExecDestroy(p->gexec);
+    p->gexec = nullptr;
+  }
+  p->nxcd = nxcd;
+  return MJW_OK;
+}
+
 extern "C" int mjw_get_tile(const mjw_problem *p, int *epb) {
   if (!p || !epb) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
   *epb = p->epb;
@@ -428,7 +439,7 @@ static StiffArgs stiff_args(mjw_problem *p, int mode, const void *U, void *Y, in
   a.state = p->state; a.geom = p->geom; a.mode = mode; a.E = p->E;
   a.map = p->d_map; a.G = p->d_G; a.Dm = p->d_D; a.lam = p->d_lam; a.mu = p->d_mu;
   a.U = U; a.Y = Y; a.sdiag = p->d_sdiag; a.N = p->N; a.nshots = nshots;
-  a.alpha = 1.0; a.beta = 0.0;
+  a.alpha = 1.0; a.beta = 0.0; a.nxcd = p->nxcd;
   return a;
 }
 

@@ -45,4 +45,13 @@ __device__ __forceinline__ void bst<double>(double v, rsrc_t r, uint32_t vo, uin
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, v), r, vo, so, 0);
 }
 
+// Logical block index such that the blocks the hardware places on one XCD
+// (block b runs on XCD b % nxcd) process one contiguous range of logical
+// blocks: neighbouring elements then share that XCD's L2.  nxcd <= 1: identity.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned nxcd) {
+  if (nxcd <= 1) return b;
+  const unsigned q = nb / nxcd, r = nb % nxcd, x = b % nxcd, i = b / nxcd;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 }  // namespace mjw

@@ -60,6 +60,7 @@ struct StiffArgs {
   double alpha, beta;                 // apply
   int64_t N;                          // nodes
   int nshots;
+  int nxcd;                           // XCD-aware block order (1 = off)
 };
 // returns 0 or a HIP error code; `stream` is a hipStream_t
 int launch_stiffness(const StiffArgs &a, void *stream);

@@ -26,16 +26,16 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
     const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
     const double *__restrict__ lam_e, const double *__restrict__ mu_e, int64_t E, int64_t e0,
     int64_t e1, const T *__restrict__ U, T *__restrict__ Y, const T *__restrict__ sdiag,
-    T alpha, T beta, int64_t N_nodes, int nshots) {
+    T alpha, T beta, int64_t N_nodes, int nshots, int nxcd) {
   constexpr int P = N * N, NN = N * N * N;
   using A = TG;
   __shared__ T lds[6 * EPB * NN];
   const int t = threadIdx.x;
   const int el = t / P, ij = t - el * P;
   const int i = ij % N, j = ij / N;
-  const int64_t e = e0 + (int64_t)blockIdx.x * EPB + el;
+  const int64_t e = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * EPB + el;
   const bool act = e < e1;
-  const int64_t eb = e0 + (int64_t)blockIdx.x * EPB;
+  const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * EPB;
   const int64_t plane = E * P;
   const uint32_t lo = (uint32_t)(el * P + ij);
   const uint32_t g_ps = (uint32_t)(plane * sizeof(TG));
@@ -181,11 +181,11 @@ static int run_elastic(const StiffArgs &a, hipStream_t st) {
   if (a.mode == kLeap)
     hipLaunchKernelGGL((k_elastic3d<N, EPB, T, TG, kLeap>), dim3(blocks), dim3(threads), 0, st,
                        a.map, G, D, a.lam, a.mu, a.E, a.e0, a.e1, U, Y, s, (T)a.alpha,
-                       (T)a.beta, a.N, a.nshots);
+                       (T)a.beta, a.N, a.nshots, a.nxcd);
   else
     hipLaunchKernelGGL((k_elastic3d<N, EPB, T, TG, kApply>), dim3(blocks), dim3(threads), 0, st,
                        a.map, G, D, a.lam, a.mu, a.E, a.e0, a.e1, U, Y, s, (T)a.alpha,
-                       (T)a.beta, a.N, a.nshots);
+                       (T)a.beta, a.N, a.nshots, a.nxcd);
   return (int)hipGetLastError();
 }
 

@@ -39,7 +39,7 @@ template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST>
 __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
     const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
     int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
-    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots) {
+    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd) {
   constexpr int P = N * N, NN = N * N * N;
   using A = typename Mix<T, TG>::A;
   __shared__ T lu[EPB * NN];
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
   const int el = t / P, ij = t - el * P;
   const int i = ij % N, j = ij / N;
   // block-uniform resources; per-lane byte offsets are 32-bit
-  const int64_t eb = e0 + (int64_t)blockIdx.x * EPB;
+  const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * EPB;
   const int64_t plane = E * P;
   const uint32_t lo = (uint32_t)(el * P + ij);
   const bool act = eb + el < e1;
@@ -161,7 +161,7 @@ template <int N, int EPB, typename T, typename TG, int MODE>
 __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
     const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
     int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
-    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots) {
+    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd) {
   constexpr int P = N * N;
   using A = typename Mix<T, TG>::A;
   __shared__ T lu[EPB * P];
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
   const int t = threadIdx.x;
   const int el = t / P, q = t - el * P;
   const int i = q % N, j = q / N;
-  const int64_t eb = e0 + (int64_t)blockIdx.x * EPB;
+  const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * EPB;
   const int64_t plane = E * P;
   const uint32_t lo = (uint32_t)(el * P + q);
   const bool act = eb + el < e1;
@@ -259,21 +259,21 @@ static int run_scalar(const StiffArgs &a, hipStream_t st) {
     if (a.mode == kLeap) {
       if (a.nshots > 1)
         hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kLeap, true>), dim3(blocks), dim3(threads),
-                           0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+                           0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
       else
         hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kLeap, false>), dim3(blocks), dim3(threads),
-                           0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+                           0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
     } else {
       hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kApply, false>), dim3(blocks), dim3(threads),
-                         0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+                         0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
     }
   } else {
     if (a.mode == kLeap)
       hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kLeap>), dim3(blocks), dim3(threads), 0, st,
-                         a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+                         a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
     else
       hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kApply>), dim3(blocks), dim3(threads), 0, st,
-                         a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots);
+                         a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
   }
   return (int)hipGetLastError();
 }

@@ -133,6 +133,9 @@ class WaveProblem:
     def set_tile(self, epb: int):
         self._chk(_lib.lib().mjw_set_tile(self._h, int(epb)), "set_tile")
 
+    def set_xcd_order(self, nxcd: int):
+        self._chk(_lib.lib().mjw_set_xcd_order(self._h, int(nxcd)), "set_xcd_order")
+
     def tile(self) -> int:
         v = C.c_int()
         self._chk(_lib.lib().mjw_get_tile(self._h, C.byref(v)), "get_tile")

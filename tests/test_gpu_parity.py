@@ -148,3 +148,18 @@ def test_lambda_max_matches_oracle():
     lo = O.lambda_max(O.acoustic_problem(o, 1.0, 1.0), 150, 3)
     lg = P.lambda_max(150)
     assert abs(lg - lo) < 1e-7 * lo
+
+
+def test_xcd_block_order_is_result_neutral():
+    """The XCD-aware block order changes which die runs an element, not the
+    arithmetic: results are bitwise identical."""
+    c = CS.load("ref3d_q5_pert")
+    P = CS.gpu_problem(c, "fp64")
+    u = np.random.default_rng(2).standard_normal(P.ndof)
+    P.set_xcd_order(1)
+    a = P.apply_kh(u)
+    P.set_xcd_order(8)
+    b = P.apply_kh(u)
+    P.set_xcd_order(3)
+    d = P.apply_kh(u)
+    assert np.array_equal(a, b) and np.array_equal(a, d)
