@@ -199,15 +199,24 @@ int mjw_lambda_max(mjw_problem *p, int iters, uint64_t seed, double *lambda_max)
  * Gradient of a receiver misfit with respect to per-element mu
  * (m_e = mu_e = c_e^2 with rho = 1).  No counterpart in the reference (it has
  * no time-domain adjoint); built on the same kernels.
- *   mjw_adjoint_forward: runs nsteps leapfrog steps from the initialised state
- *   keeping nsnap checkpoints (pairs U^k, U^{k+1}) and returns the traces
- *   [nsteps+1][nshots][nrec] (rows 0..nsteps).
+ *   mjw_adjoint_forward: from a zero initial state (mjw_leapfrog_init with
+ *   u0 = du0 = NULL), computes U^0..U^nsteps keeping nsnap checkpoint pairs
+ *   (U^k, U^{k+1}) and returns the traces [nsteps+1][nrec] (rows 0..nsteps).
  *   mjw_adjoint_backward: given dJ/d(trace) with the same shape, runs the
- *   discrete adjoint leapfrog backward, recomputing forward segments from the
- *   checkpoints, and accumulates grad[n_elem] (FP64, original element order).
+ *   discrete adjoint leapfrog backward; the forward iterates are rebuilt in
+ *   reverse time between checkpoints (the leapfrog is time reversible) in the
+ *   same fused kernel as the adjoint, and grad[n_elem] = dJ/dmu_e is
+ *   accumulated in FP64 (original element order).  Single-shot acoustic.
  */
 int mjw_adjoint_forward(mjw_problem *p, int nsteps, int nsnap, double *traces);
 int mjw_adjoint_backward(mjw_problem *p, const double *dJ_dtrace, double *grad);
+/* device time (ms) of the last adjoint forward [0] and backward [1] */
+int mjw_adjoint_timing(const mjw_problem *p, double *ms2);
+/* Replace the per-element mu of an acoustic problem (rescales the stored
+ * geometric factors; rho and the mass are unchanged).  Media update used by
+ * the gradient / inversion loop; no counterpart in the reference (Montjoie
+ * rebuilds the problem from the data file). */
+int mjw_set_mu(mjw_problem *p, const double *mu);
 
 /* --------------------------------------------------------- synthetic meshes */
 /*

@@ -1,0 +1,186 @@
+// mjwave_adapter.hpp -- header-only C++ adapter over the C ABI (mjwave.h),
+// shaped like the two Montjoie interfaces the explicit driver talks to:
+//
+//  * the operator system VirtualOdeSystem<T> (Instationary/TimeSchemes.hxx:35-151,
+//    implemented by VarInstationary_Base, VarInstationaryInline.cxx:174-242):
+//    ApplyOperatorKh / ApplyOperatorDh / ApplyOperatorDhMinusdtSh /
+//    SolveOperatorDh / SolveOperatorDhPlusdtSh / AddScalarTimeSource /
+//    SetDirichletCondition;
+//  * the leapfrog time scheme ModifiedEquationIterator<T>
+//    (Instationary/ModifiedEquation.hxx:78-114, .cxx:99-124,295-345):
+//    SetInitialConditionS / Advance / GetIterate / GetCFL / GetNumberOfIterates.
+//
+// Errors: the C ABI returns status codes; like the reference (cout + abort(),
+// e.g. Hyperbolic/Acoustic/TimeAcoustic.cxx:243-249) this adapter prints the
+// message and aborts, unless MJW_ADAPTER_THROW is defined (then it throws
+// std::runtime_error).
+//
+// Vectors are std::vector<double> or any contiguous container exposing
+// data() / size(); host memory stays owned by the caller.
+#pragma once
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mjwave.h"
+
+namespace mjw {
+
+inline void check(int rc, const mjw_problem *p, const char *what) {
+  if (rc == MJW_OK) return;
+  std::string msg = std::string(what) + " failed (" + std::to_string(rc) + "): " +
+                    mjw_last_error(p);
+#ifdef MJW_ADAPTER_THROW
+  throw std::runtime_error(msg);
+#else
+  std::fprintf(stderr, "%s\n", msg.c_str());
+  std::abort();
+#endif
+}
+
+// Mesh handed over by the caller: positions of all global GL nodes and the
+// element -> node map in local order q = i + n j (+ n^2 k).
+struct MeshView {
+  int dim = 3, order = 1;
+  int64_t n_nodes = 0, n_elem = 0;
+  const double *xyz = nullptr;
+  const int32_t *elem_nodes = nullptr;
+  const uint8_t *dirichlet = nullptr;  // may be null (no Dirichlet node)
+};
+
+// The explicit second-order wave problem D u'' + A u = f(t) b on one device.
+class HipWaveSystem {
+ public:
+  HipWaveSystem(const MeshView &m, const double *rho, const double *mu,
+                const double *lambda = nullptr, int ncomp = 1, int nshots = 1,
+                int precision = MJW_FP64, int device = 0)
+      : ncomp_(ncomp), nshots_(nshots), ndof_(m.n_nodes * ncomp) {
+    check(mjw_create(&p_, device, m.dim, m.order, ncomp, m.n_nodes, m.xyz, m.n_elem,
+                     m.elem_nodes, rho, mu, lambda, m.dirichlet, nshots, precision),
+          nullptr, "mjw_create");
+  }
+  ~HipWaveSystem() { mjw_destroy(p_); }
+  HipWaveSystem(const HipWaveSystem &) = delete;
+  HipWaveSystem &operator=(const HipWaveSystem &) = delete;
+
+  mjw_problem *handle() const { return p_; }
+  int64_t GetNbDof() const { return ndof_; }
+
+  // Y = beta Y + alpha K U, K = -A  (ApplyOperatorKh)
+  template <class V>
+  void ApplyOperatorKh(double alpha, double /*t*/, const V &U, double beta, V &Y) {
+    check(mjw_apply_kh(p_, alpha, U.data(), beta, Y.data()), p_, "ApplyOperatorKh");
+  }
+  // Y = beta Y + alpha D U  (ApplyOperatorDh; D - dt/2 S = D since S = 0)
+  template <class V>
+  void ApplyOperatorDh(double alpha, double /*t*/, const V &U, double beta, V &Y) {
+    check(mjw_apply_dh(p_, alpha, U.data(), beta, Y.data()), p_, "ApplyOperatorDh");
+  }
+  template <class V>
+  void ApplyOperatorDhMinusdtSh(double alpha, double t, const V &U, double beta, V &Y) {
+    ApplyOperatorDh(alpha, t, U, beta, Y);
+  }
+  // Y = D^{-1} Y  (SolveOperatorDh / SolveOperatorDhPlusdtSh with S = 0)
+  template <class V>
+  void SolveOperatorDh(V &Y) {
+    check(mjw_solve_dh(p_, Y.data()), p_, "SolveOperatorDh");
+  }
+  template <class V>
+  void SolveOperatorDhPlusdtSh(V &Y) {
+    SolveOperatorDh(Y);
+  }
+  // homogeneous Dirichlet: zero the Dirichlet rows (SetDirichletCondition)
+  template <class V>
+  void SetDirichletCondition(double /*t*/, int /*n*/, V &Y, const std::vector<uint8_t> &dir) {
+    for (size_t i = 0; i < Y.size(); ++i)
+      if (dir[i % dir.size()]) Y[i] = 0.0;
+  }
+
+  // Dirac sources with a Ricker pulse (SRC_DIRAC + TemporalSource = RICKER)
+  void SetDiracSources(const std::vector<double> &pts, int dim,
+                       const std::vector<double> &pol = {}, const std::vector<int32_t> &shot = {}) {
+    const int nsrc = (int)(pts.size() / dim);
+    check(mjw_set_sources(p_, nsrc, pts.data(), pol.empty() ? nullptr : pol.data(),
+                          shot.empty() ? nullptr : shot.data()),
+          p_, "SetDiracSources");
+  }
+  void SetRickerPulse(double f0, double t_begin, double dt, int nsteps) {
+    std::vector<double> tab(nsteps);
+    check(mjw_ricker_pulse_table(f0, t_begin, dt, nsteps, tab.data()), nullptr, "Ricker");
+    check(mjw_set_pulse(p_, nsteps, tab.data()), p_, "SetRickerPulse");
+  }
+  void SetReceivers(const std::vector<double> &pts, int dim, const std::vector<int32_t> &comp = {}) {
+    const int nrec = (int)(pts.size() / dim);
+    check(mjw_set_receivers(p_, nrec, pts.data(), comp.empty() ? nullptr : comp.data()), p_,
+          "SetReceivers");
+    nrec_ = nrec;
+  }
+  int GetNbReceivers() const { return nrec_; }
+  int GetNbShots() const { return nshots_; }
+
+  // CFL: dt_max = 2 / sqrt(lambda_max(D^{-1} A))  (EvaluateCFL_Generic)
+  double EvaluateCFL(int iters = 80) {
+    double lam = 0.0;
+    check(mjw_lambda_max(p_, iters, 1, &lam), p_, "EvaluateCFL");
+    return 2.0 / std::sqrt(lam);
+  }
+
+ private:
+  mjw_problem *p_ = nullptr;
+  int ncomp_, nshots_;
+  int64_t ndof_;
+  int nrec_ = 0;
+};
+
+// Order-2 leapfrog (ModifiedEquationIterator) whose iterates live on the
+// device; Advance may fuse several steps (AdvanceN) and the receiver traces
+// recorded at every step play the role of GiveIterate's seismograms.
+class HipLeapFrogScheme {
+ public:
+  explicit HipLeapFrogScheme(HipWaveSystem &sys) : sys_(sys) {}
+
+  // U^0 = u0, U^1 = U^0 + dt du0 + dt^2/2 D^{-1} K U^0 (+ Dirichlet)
+  template <class V>
+  void SetInitialConditionS(double t, double dt, const V &u0, const V &du0) {
+    check(mjw_leapfrog_init(sys_.handle(), t, dt, u0.size() ? u0.data() : nullptr,
+                            du0.size() ? du0.data() : nullptr),
+          sys_.handle(), "SetInitialConditionS");
+    dt_ = dt;
+  }
+  void SetInitialConditionZero(double t, double dt) {
+    check(mjw_leapfrog_init(sys_.handle(), t, dt, nullptr, nullptr), sys_.handle(),
+          "SetInitialConditionS");
+    dt_ = dt;
+  }
+  // one step (Advance); traces of the iterate before the step appended
+  void Advance(double /*t*/, int /*n*/) { AdvanceN(1, nullptr); }
+  // n fused steps; traces: [n][nshots][nrec] or null
+  void AdvanceN(int n, double *traces, bool graph = true) {
+    check(mjw_leapfrog_advance(sys_.handle(), n, traces, graph ? 1 : 0), sys_.handle(),
+          "Advance");
+  }
+  // U^n and U^{n+1} (GetIterate(0), GetIterate(1)); all shots
+  void GetIterates(std::vector<double> &Un, std::vector<double> &Un1) {
+    Un.resize(sys_.GetNbDof() * sys_.GetNbShots());
+    Un1.resize(Un.size());
+    check(mjw_get_state(sys_.handle(), Un.data(), Un1.data()), sys_.handle(), "GetIterate");
+  }
+  std::vector<double> Energy() {
+    std::vector<double> e(sys_.GetNbShots());
+    check(mjw_energy(sys_.handle(), e.data()), sys_.handle(), "Energy");
+    return e;
+  }
+  static double GetCFL() { return 2.0; }
+  static int GetNumberOfIterates() { return 2; }
+  double GetTimeStep() const { return dt_; }
+
+ private:
+  HipWaveSystem &sys_;
+  double dt_ = 0.0;
+};
+
+}  // namespace mjw

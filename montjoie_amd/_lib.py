@@ -58,6 +58,8 @@ SIGNATURES = {
     "mjw_batch_destroy": (C.c_int, [_vp]),
     "mjw_last_timing": (C.c_int, [_vp, _dp]),
     "mjw_profile": (C.c_int, [_vp, C.c_int, _dp]),
+    "mjw_adjoint_timing": (C.c_int, [_vp, _dp]),
+    "mjw_set_mu": (C.c_int, [_vp, _dp]),
 }
 
 _lib = None

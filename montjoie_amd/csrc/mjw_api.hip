@@ -77,9 +77,19 @@ struct mjw_problem {
   size_t bytes = 0;
   std::string err;
   // adjoint
-  std::vector<void *> snaps;
-  int adj_nsteps = 0, adj_nsnap = 0;
-  std::vector<int64_t> snap_step;
+  bool zero_ic = true;
+  std::vector<void *> snaps;          // pairs (U^k, U^{k+1}), 2*N*ncomp each
+  std::vector<int64_t> snap_step;     // k of each pair
+  int64_t adj_nsteps = 0;
+  bool adj_ready = false;
+  void *d_pair[2] = {nullptr, nullptr};
+  double *d_grad = nullptr, *d_adjg = nullptr;
+  int64_t adjg_cap = 0;
+  int nadj = 0;                       // receiver transpose (CSR over unique nodes)
+  int32_t *d_adj_node = nullptr, *d_adj_ptr = nullptr, *d_adj_rec = nullptr;
+  double *d_adj_w = nullptr;
+  hipGraphExec_t bexec = nullptr;     // two backward steps
+  double ms_adj_fwd = 0.0, ms_adj_bwd = 0.0;
 };
 
 #define FAIL(p, code, msg)                  \
@@ -168,6 +178,10 @@ static void release(mjw_problem *p) {
                   p->d_traces};
   for (void *q : ptrs) dfree(q);
   for (void *q : p->snaps) dfree(q);
+  void *aptrs[] = {p->d_pair[0], p->d_pair[1], p->d_grad, p->d_adjg, p->d_adj_node,
+                   p->d_adj_ptr, p->d_adj_rec, p->d_adj_w};
+  for (void *q : aptrs) dfree(q);
+  if (p->bexec) (void)hipGraphExecDestroy(p->bexec);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
@@ -439,7 +453,7 @@ static StiffArgs stiff_args(mjw_problem *p, int mode, const void *U, void *Y, in
   a.state = p->state; a.geom = p->geom; a.mode = mode; a.E = p->E;
   a.map = p->d_map; a.G = p->d_G; a.Dm = p->d_D; a.lam = p->d_lam; a.mu = p->d_mu;
   a.U = U; a.Y = Y; a.sdiag = p->d_sdiag; a.N = p->N; a.nshots = nshots;
-  a.alpha = 1.0; a.beta = 0.0; a.nxcd = p->nxcd;
+  a.alpha = 1.0; a.beta = 0.0; a.nxcd = p->nxcd; a.grad = nullptr;
   return a;
 }
 
@@ -658,6 +672,44 @@ extern "C" int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts,
   dfree(p->d_rec_gid); dfree(p->d_rec_w); dfree(p->d_rec_comp);
   p->d_rec_gid = nullptr; p->d_rec_w = nullptr; p->d_rec_comp = nullptr;
   p->nrec = nrec;
+  {
+    // receiver transpose R^T as CSR over the unique nodes it touches (the
+    // adjoint source), entries of one node in (receiver, local node) order
+    std::vector<std::pair<int64_t, std::pair<int32_t, double>>> ent;
+    for (int r = 0; r < nrec; ++r)
+      for (int q = 0; q < p->nd; ++q) {
+        const double wv = w[(size_t)r * p->nd + q];
+        if (wv != 0.0)
+          ent.push_back({(int64_t)cm[r] * p->N + gid[(size_t)r * p->nd + q], {r, wv}});
+      }
+    std::stable_sort(ent.begin(), ent.end(),
+                     [](const auto &x, const auto &y) { return x.first < y.first; });
+    std::vector<int32_t> node, ptr(1, 0), rec;
+    std::vector<double> wt;
+    for (size_t i = 0; i < ent.size(); ++i) {
+      if (i == 0 || ent[i].first != ent[i - 1].first) {
+        if (i) ptr.push_back((int32_t)rec.size());
+        node.push_back((int32_t)ent[i].first);
+      }
+      rec.push_back(ent[i].second.first);
+      wt.push_back(ent[i].second.second);
+    }
+    if (!ent.empty()) ptr.push_back((int32_t)rec.size());
+    dfree(p->d_adj_node); dfree(p->d_adj_ptr); dfree(p->d_adj_rec); dfree(p->d_adj_w);
+    p->d_adj_node = nullptr; p->d_adj_ptr = nullptr; p->d_adj_rec = nullptr; p->d_adj_w = nullptr;
+    p->nadj = (int)node.size();
+    if (p->nadj) {
+      HIPCHK(p, dalloc(p, &p->d_adj_node, node.size() * 4));
+      HIPCHK(p, dalloc(p, &p->d_adj_ptr, ptr.size() * 4));
+      HIPCHK(p, dalloc(p, &p->d_adj_rec, rec.size() * 4));
+      HIPCHK(p, dalloc(p, &p->d_adj_w, wt.size() * 8));
+      HIPCHK(p, hipMemcpy(p->d_adj_node, node.data(), node.size() * 4, hipMemcpyHostToDevice));
+      HIPCHK(p, hipMemcpy(p->d_adj_ptr, ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
+      HIPCHK(p, hipMemcpy(p->d_adj_rec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
+      HIPCHK(p, hipMemcpy(p->d_adj_w, wt.data(), wt.size() * 8, hipMemcpyHostToDevice));
+    }
+    if (p->bexec) { (void)hipGraphExecDestroy(p->bexec); p->bexec = nullptr; }
+  }
   if (nrec) {
     HIPCHK(p, dalloc(p, &p->d_rec_gid, gid.size() * 4));
     HIPCHK(p, dalloc(p, &p->d_rec_w, w.size() * 8));
@@ -678,6 +730,8 @@ extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, cons
   hipStream_t st = p->stream;
   p->dt = dt;
   p->t_begin = t_begin;
+  p->zero_ic = (u0 == nullptr && du0 == nullptr);
+  p->adj_ready = false;
   const int64_t ns = nstate(p);
   const int64_t n1 = (int64_t)p->ncomp * p->N;
   KCHK(p, launch_make_sdiag(p->state, p->N, p->d_M, p->d_dir, dt * dt, p->d_sdiag, st));
@@ -1032,12 +1086,235 @@ extern "C" int mjw_lambda_max(mjw_problem *p, int iters, uint64_t seed, double *
   return MJW_OK;
 }
 
-// ------------------------------------------------- adjoint (not yet built)
-extern "C" int mjw_adjoint_forward(mjw_problem *p, int, int, double *) {
-  FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint not available in this build");
+// ------------------------------------------------------------ adjoint
+// Discrete adjoint of the leapfrog recurrence (zero initial state):
+//   U^{k+1} = 2U^k - U^{k-1} - S A U^k + S f_{k-1} b,   S = dt^2 M^{-1}
+// gives, for nu_k = S lambda_k and h^k = R^T dJ/d(trace_k),
+//   nu_{N-1} = S h^N, nu_N = 0,  nu_{j-1} = 2nu_j - nu_{j+1} - S A nu_j + S h^j,
+//   dJ/dmu_e = - sum_{j=1}^{N-1} nu_j^T (A_e / mu_e) U^j.
+// The recurrence is time reversible: U^{j-1} = 2U^j - U^{j+1} - S A U^j +
+// S f_{j-1} b, so the backward pass rebuilds U^j in reverse alongside nu_j in
+// the same fused two-vector kernel (geometry read once for both) and
+// re-anchors U on the checkpoint pairs stored by the forward pass.
+
+extern "C" int mjw_set_mu(mjw_problem *p, const double *mu) {
+  if (!p || !mu) FAIL(p, MJW_ERR_ARG, "mjw_set_mu: NULL");
+  if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_mu: acoustic problems only");
+  HIPCHK(p, hipSetDevice(p->device));
+  std::vector<double> old(p->E), ms(p->E), sc(p->E);
+  HIPCHK(p, hipMemcpy(old.data(), p->d_mu, p->E * 8, hipMemcpyDeviceToHost));
+  for (int64_t es = 0; es < p->E; ++es) {
+    ms[es] = mu[p->perm[es]];
+    if (!(ms[es] > 0) || !std::isfinite(ms[es])) FAIL(p, MJW_ERR_ARG, "mu must be finite, > 0");
+    sc[es] = ms[es] / old[es];
+  }
+  double *d_sc = nullptr;
+  HIPCHK(p, hipMalloc(&d_sc, p->E * 8));
+  HIPCHK(p, hipMemcpy(d_sc, sc.data(), p->E * 8, hipMemcpyHostToDevice));
+  const int ngc = p->dim == 2 ? 3 : 6;
+  const int K = p->dim == 2 ? 1 : p->n;
+  int rc = launch_scale_planes(p->geom == kG64 ? 0 : 1, p->d_G, (int64_t)K * ngc, p->E, p->P, d_sc,
+                               p->stream);
+  if (!rc) rc = (int)hipMemcpyAsync(p->d_mu, ms.data(), p->E * 8, hipMemcpyHostToDevice, p->stream);
+  if (!rc) rc = (int)hipStreamSynchronize(p->stream);
+  dfree(d_sc);
+  if (rc) FAIL(p, MJW_ERR_HIP, "mjw_set_mu: rescale failed");
+  p->adj_ready = false;
+  return MJW_OK;
 }
-extern "C" int mjw_adjoint_backward(mjw_problem *p, const double *, double *) {
-  FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint not available in this build");
+
+extern "C" int mjw_adjoint_forward(mjw_problem *p, int nsteps, int nsnap, double *traces) {
+  if (!p || nsteps < 2 || nsnap < 1 || !traces) FAIL(p, MJW_ERR_ARG, "mjw_adjoint_forward: bad args");
+  if (p->ncomp != 1 || p->nshots != 1)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: single-shot acoustic problems only");
+  if (!p->inited || p->step != 0 || !p->zero_ic)
+    FAIL(p, MJW_ERR_STATE, "adjoint: call mjw_leapfrog_init with zero initial state first");
+  if (p->nrec == 0) FAIL(p, MJW_ERR_STATE, "adjoint: no receivers");
+  HIPCHK(p, hipSetDevice(p->device));
+  for (void *q : p->snaps) dfree(q);
+  p->snaps.clear();
+  p->snap_step.clear();
+  const int64_t N1 = nsteps - 1;                      // steps run: U^0..U^N
+  const int64_t K = (N1 + nsnap - 1) / nsnap;         // segment length
+  const size_t vb = (size_t)p->N * ssize(p);
+  hipEvent_t e0, e1;
+  HIPCHK(p, hipEventCreate(&e0));
+  HIPCHK(p, hipEventCreate(&e1));
+  HIPCHK(p, hipEventRecord(e0, p->stream));
+  for (int64_t k = 0; k < N1; k += K) {
+    void *s = nullptr;
+    HIPCHK(p, hipMalloc(&s, 2 * vb));
+    p->snaps.push_back(s);
+    p->snap_step.push_back(k);
+    HIPCHK(p, hipMemcpyAsync(s, p->d_U[p->cur], vb, hipMemcpyDeviceToDevice, p->stream));
+    HIPCHK(p, hipMemcpyAsync((char *)s + vb, p->d_U[p->cur ^ 1], vb, hipMemcpyDeviceToDevice,
+                             p->stream));
+    const int64_t m = std::min(K, N1 - k);
+    int rc = mjw_leapfrog_advance(p, (int)m, traces + k * p->nrec, 1);
+    if (rc) return rc;
+  }
+  {  // row N-1 = R U^{N-1} (current iterate)
+    int rc = mjw_record(p, traces + N1 * p->nrec);
+    if (rc) return rc;
+  }
+  {  // row N = R U^N (the next iterate, cur^1)
+    KCHK(p, ensure_traces(p, 1));
+    int64_t ctl[2] = {0, 0};
+    HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, p->stream));
+    TailArgs t{};
+    t.state = p->state; t.ctl = p->d_ctl; t.nrec = p->nrec; t.nd = p->nd; t.rec_gid = p->d_rec_gid;
+    t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = p->d_U[p->cur ^ 1]; t.P = nullptr;
+    t.traces = p->d_traces; t.N = p->N; t.nshots = 1; t.advance = 0; t.row_offset = 0;
+    KCHK(p, launch_tail_n(t, p->ncomp, p->stream));
+    HIPCHK(p, hipMemcpyAsync(traces + (size_t)nsteps * p->nrec, p->d_traces, (size_t)p->nrec * 8,
+                             hipMemcpyDeviceToHost, p->stream));
+  }
+  HIPCHK(p, hipEventRecord(e1, p->stream));
+  HIPCHK(p, hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->ms_adj_fwd = ms;
+  p->adj_nsteps = nsteps;
+  p->adj_ready = true;
+  return MJW_OK;
+}
+
+// one backward step j: buffers cur (U^j, nu_j) and other (U^{j+1}, nu_{j+1})
+static int enqueue_bwd_step(mjw_problem *p, void *cur, void *other) {
+  for (int c = 0; c < p->ncolor; ++c) {
+    StiffArgs a = stiff_args(p, kLeap, cur, other, 2);
+    a.e0 = p->coff[c];
+    a.e1 = p->coff[c + 1];
+    a.grad = p->d_grad;
+    int rc = launch_stiffness(a, p->stream);
+    if (rc) return rc;
+  }
+  TailArgs t{};
+  t.state = p->state; t.ctl = p->d_ctl; t.pulse = p->d_pulse; t.npulse = p->npulse;
+  t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot; t.src_val = p->d_src_val;
+  t.sdiag = p->d_sdiag; t.nrec = 0; t.nd = p->nd; t.U = cur; t.P = other; t.traces = nullptr;
+  t.N = p->N; t.nshots = 2; t.advance = -1; t.row_offset = 0;
+  t.nadj = p->nadj; t.adj_node = p->d_adj_node; t.adj_ptr = p->d_adj_ptr; t.adj_rec = p->d_adj_rec;
+  t.adj_w = p->d_adj_w; t.adj_g = p->d_adjg; t.adj_row = 1; t.adj_nrec = p->nrec; t.adj_shot = 1;
+  return launch_tail_n(t, p->ncomp, p->stream);
+}
+
+extern "C" int mjw_adjoint_backward(mjw_problem *p, const double *dJ, double *grad) {
+  if (!p || !dJ || !grad) FAIL(p, MJW_ERR_ARG, "mjw_adjoint_backward: NULL");
+  if (!p->adj_ready) FAIL(p, MJW_ERR_STATE, "adjoint: run mjw_adjoint_forward first");
+  HIPCHK(p, hipSetDevice(p->device));
+  hipStream_t st = p->stream;
+  const int64_t Nn = p->adj_nsteps;                    // traces rows 0..Nn
+  const size_t vb = (size_t)p->N * ssize(p);
+  const int64_t rows = Nn + 1;
+  if (rows * p->nrec > p->adjg_cap) {
+    dfree(p->d_adjg);
+    p->d_adjg = nullptr;
+    HIPCHK(p, hipMalloc(&p->d_adjg, rows * p->nrec * 8));
+    p->adjg_cap = rows * p->nrec;
+    if (p->bexec) { (void)hipGraphExecDestroy(p->bexec); p->bexec = nullptr; }
+  }
+  for (int b = 0; b < 2; ++b)
+    if (!p->d_pair[b]) {
+      HIPCHK(p, dalloc(p, &p->d_pair[b], 2 * vb));
+      if (p->bexec) { (void)hipGraphExecDestroy(p->bexec); p->bexec = nullptr; }
+    }
+  if (!p->d_grad) HIPCHK(p, dalloc(p, &p->d_grad, p->E * 8));
+  HIPCHK(p, hipMemcpyAsync(p->d_adjg, dJ, rows * p->nrec * 8, hipMemcpyHostToDevice, st));
+  HIPCHK(p, hipMemsetAsync(p->d_grad, 0, p->E * 8, st));
+  hipEvent_t e0, e1;
+  HIPCHK(p, hipEventCreate(&e0));
+  HIPCHK(p, hipEventCreate(&e1));
+  HIPCHK(p, hipEventRecord(e0, st));
+  // cur = (U^{N-1}, nu_{N-1} = S h^N), other = (U^N, nu_N = 0)
+  void *cur = p->d_pair[0], *oth = p->d_pair[1];
+  HIPCHK(p, hipMemcpyAsync(cur, p->d_U[p->cur], vb, hipMemcpyDeviceToDevice, st));
+  HIPCHK(p, hipMemcpyAsync(oth, p->d_U[p->cur ^ 1], vb, hipMemcpyDeviceToDevice, st));
+  HIPCHK(p, hipMemsetAsync((char *)cur + vb, 0, vb, st));
+  HIPCHK(p, hipMemsetAsync((char *)oth + vb, 0, vb, st));
+  {
+    int64_t ctl[2] = {Nn - 1, 0};   // adj row = step + 1 = N
+    HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, st));
+    TailArgs t{};
+    t.state = p->state; t.ctl = p->d_ctl; t.sdiag = p->d_sdiag; t.U = cur; t.P = cur;
+    t.N = p->N; t.nshots = 2; t.advance = 0;
+    t.nadj = p->nadj; t.adj_node = p->d_adj_node; t.adj_ptr = p->d_adj_ptr;
+    t.adj_rec = p->d_adj_rec; t.adj_w = p->d_adj_w; t.adj_g = p->d_adjg; t.adj_row = 1;
+    t.adj_nrec = p->nrec; t.adj_shot = 1;
+    KCHK(p, launch_tail_n(t, p->ncomp, st));
+  }
+  // step counter = j - 1 (pulse index of the reverse step), decremented by the tail
+  {
+    int64_t ctl[2] = {Nn - 2, 0};
+    HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, st));
+  }
+  // re-anchoring points: after the step j = k_s + 1 the current pair holds
+  // (U^{k_s}, U^{k_s+1})
+  std::vector<int> anchor(Nn + 1, -1);
+  for (size_t s = 0; s < p->snap_step.size(); ++s) anchor[p->snap_step[s]] = (int)s;
+  int64_t j = Nn - 1;
+  while (j >= 1) {
+    // run backward steps until the next anchor (k_s = j - 1)
+    int64_t stop = j;  // last j of this run
+    while (stop >= 1 && anchor[stop - 1] < 0) --stop;
+    // steps j, j-1, ..., stop (stop = k_s + 1 or 1)
+    if (stop < 1) stop = 1;
+    int64_t cnt = j - stop + 1;
+    bool parity0 = true;  // cur == d_pair[0]
+    (void)parity0;
+    // two-step graph when the buffers are in their canonical order
+    while (cnt >= 2 && cur == p->d_pair[0]) {
+      if (!p->bexec) {
+        hipGraph_t g = nullptr;
+        HIPCHK(p, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_bwd_step(p, p->d_pair[0], p->d_pair[1]);
+        if (!rc) rc = enqueue_bwd_step(p, p->d_pair[1], p->d_pair[0]);
+        hipError_t ee = hipStreamEndCapture(st, &g);
+        if (rc || ee != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          FAIL(p, MJW_ERR_HIP, "adjoint graph capture failed");
+        }
+        ee = hipGraphInstantiate(&p->bexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ee != hipSuccess) { p->bexec = nullptr; FAIL(p, MJW_ERR_HIP, "graph instantiate"); }
+      }
+      HIPCHK(p, hipGraphLaunch(p->bexec, st));
+      cnt -= 2;
+      j -= 2;
+    }
+    while (cnt > 0) {
+      KCHK(p, enqueue_bwd_step(p, cur, oth));
+      std::swap(cur, oth);
+      --cnt;
+      --j;
+    }
+    if (j >= 0 && anchor[j] >= 0 && j < Nn - 1) {
+      // cur holds (U^j, nu_j), oth holds (U^{j+1}, nu_{j+1}): restore U exactly
+      void *s = p->snaps[anchor[j]];
+      HIPCHK(p, hipMemcpyAsync(cur, s, vb, hipMemcpyDeviceToDevice, st));
+      HIPCHK(p, hipMemcpyAsync(oth, (char *)s + vb, vb, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  HIPCHK(p, hipEventRecord(e1, st));
+  HIPCHK(p, hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->ms_adj_bwd = ms;
+  std::vector<double> gs(p->E);
+  HIPCHK(p, hipMemcpy(gs.data(), p->d_grad, p->E * 8, hipMemcpyDeviceToHost));
+  for (int64_t es = 0; es < p->E; ++es) grad[p->perm[es]] = gs[es];
+  return MJW_OK;
+}
+
+extern "C" int mjw_adjoint_timing(const mjw_problem *p, double *ms2) {
+  if (!p || !ms2) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
+  ms2[0] = p->ms_adj_fwd;
+  ms2[1] = p->ms_adj_bwd;
+  return MJW_OK;
 }
 
 // ------------------------------------------------------- batch (not yet built)

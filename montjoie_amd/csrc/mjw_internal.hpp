@@ -61,6 +61,7 @@ struct StiffArgs {
   int64_t N;                          // nodes
   int nshots;
   int nxcd;                           // XCD-aware block order (1 = off)
+  double *grad;                       // adjoint step: per-element gradient (sorted), or null
 };
 // returns 0 or a HIP error code; `stream` is a hipStream_t
 int launch_stiffness(const StiffArgs &a, void *stream);
@@ -96,8 +97,19 @@ struct TailArgs {
   double *traces;                // [step-base][S][nrec] (null: none)
   int64_t N;
   int nshots;
-  int advance;                   // 1: increment step after
+  int advance;                   // +1 / -1: step change after, 0: none
   int row_offset;                // trace row = step - base + row_offset
+  // adjoint injection: U^{n+1}[node] += s[node] * sum_k w_k g[step][rec_k]
+  // (receiver transpose, CSR over unique nodes); applied to shot adj_shot
+  int nadj;                      // unique nodes
+  const int32_t *adj_node;
+  const int32_t *adj_ptr;        // nadj+1
+  const int32_t *adj_rec;
+  const double *adj_w;
+  const double *adj_g;           // [row][adj_nrec]
+  int64_t adj_row;               // row of adj_g = step + adj_row
+  int adj_nrec;
+  int adj_shot;
 };
 int launch_tail_n(const TailArgs &a, int ncomp, void *stream);
 
@@ -120,6 +132,9 @@ int launch_dot_part(int state, int64_t N, int ncomp, int S, const double *w, con
 int launch_dot_diff_part(int state, int64_t N, int ncomp, int S, const double *w, const void *a,
                          const void *b, double *out, double *part, void *stream);
 int reduction_scratch_doubles(int S);
+// G planes *= sc[e] (per element, sorted order); g32: float planes
+int launch_scale_planes(int g32, void *G, int64_t nplanes, int64_t E, int P, const double *sc,
+                        void *stream);
 
 // adjoint: per-element gradient accumulation grad[e] += c * u_e^T A0_e lam_e
 struct GradArgs {

@@ -210,8 +210,20 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs a, int ncomp) {
       if (lane == 0) a.traces[((step - base + a.row_offset) * a.nshots + s) * a.nrec + r] = acc;
     }
   }
+  if (a.nadj && P) {
+    // adjoint source: receiver transpose of dJ/dtrace at this step, unique
+    // nodes in CSR form so the sum per node has a fixed order
+    const double *gr = a.adj_g + (step + a.adj_row) * a.adj_nrec;
+    T *Ps = P + (int64_t)a.adj_shot * NT;
+    for (int idx = threadIdx.x; idx < a.nadj; idx += blockDim.x) {
+      const int32_t node = a.adj_node[idx];
+      double v = 0.0;
+      for (int k = a.adj_ptr[idx]; k < a.adj_ptr[idx + 1]; ++k) v += a.adj_w[k] * gr[a.adj_rec[k]];
+      Ps[node] += sd[node] * (T)v;
+    }
+  }
   __syncthreads();
-  if (a.advance && threadIdx.x == 0) a.ctl[0] = step + 1;
+  if (a.advance && threadIdx.x == 0) a.ctl[0] = step + a.advance;
 }
 
 int launch_tail_n(const TailArgs &a, int ncomp, void *stream) {
@@ -337,6 +349,28 @@ int launch_make_sdiag(int state, int64_t N, const double *M, const uint8_t *dir,
   else
     hipLaunchKernelGGL(k_sdiag<float>, dim3(grid_for(N)), dim3(256), 0, st, N, M, dir, dt2,
                        (float *)s);
+  return (int)hipGetLastError();
+}
+
+// G[(plane*E + e)*P + q] *= sc[e] over all planes (media update)
+template <typename T>
+__global__ void k_scale_planes(T *__restrict__ G, int64_t n, int64_t E, int P,
+                               const double *__restrict__ sc) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    G[i] = (T)((double)G[i] * sc[(i / P) % E]);
+}
+
+int launch_scale_planes(int g32, void *G, int64_t nplanes, int64_t E, int P, const double *sc,
+                        void *stream) {
+  const int64_t n = nplanes * E * P;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (g32)
+    hipLaunchKernelGGL(k_scale_planes<float>, dim3(grid_for(n)), dim3(256), 0, st, (float *)G, n,
+                       E, P, sc);
+  else
+    hipLaunchKernelGGL(k_scale_planes<double>, dim3(grid_for(n)), dim3(256), 0, st, (double *)G,
+                       n, E, P, sc);
   return (int)hipGetLastError();
 }
 

@@ -35,15 +35,21 @@ struct Mix {  // geometry product in the geometry's precision
   using A = TG;
 };
 
-template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST>
+template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD = false>
 __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
     const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
     int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
-    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd) {
+    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd,
+    double *__restrict__ grad = nullptr, const double *__restrict__ mu_e = nullptr) {
+  // GRAD (adjoint step, nshots == 2: shot 0 = forward iterate U^j rebuilt
+  // backwards, shot 1 = adjoint nu_j): grad[e] -= (A_e nu_e . U_e) / mu_e
   constexpr int P = N * N, NN = N * N * N;
   using A = typename Mix<T, TG>::A;
   __shared__ T lu[EPB * NN];
   __shared__ T lv[EPB * NN];
+  __shared__ double gpart[GRAD ? EPB * P : 1];
+  T ufw[GRAD ? N : 1];  // shot-0 values of the column (forward iterate)
+  double gacc = 0.0;
   const int t = threadIdx.x;
   const int el = t / P, ij = t - el * P;
   const int i = ij % N, j = ij / N;
@@ -151,21 +157,40 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
           r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
         }
         bst<T>(r, ry, off, 0u);
+        if constexpr (GRAD) {
+          if (s == 0) ufw[k] = u[k];
+          else gacc += (double)o * (double)ufw[k];
+        }
       }
     }
     __syncthreads();
   }
+  if constexpr (GRAD) {
+    // per-element sum in a fixed order (deterministic), FP64
+    gpart[t] = act ? gacc : 0.0;
+    __syncthreads();
+    if (act && ij == 0) {
+      double sum = 0.0;
+      for (int q = 0; q < P; ++q) sum += gpart[el * P + q];
+      const int64_t e = eb + el;
+      grad[e] -= sum / mu_e[e];
+    }
+  }
 }
 
-template <int N, int EPB, typename T, typename TG, int MODE>
+template <int N, int EPB, typename T, typename TG, int MODE, bool GRAD = false>
 __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
     const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
     int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
-    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd) {
+    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd,
+    double *__restrict__ grad = nullptr, const double *__restrict__ mu_e = nullptr) {
   constexpr int P = N * N;
   using A = typename Mix<T, TG>::A;
   __shared__ T lu[EPB * P];
   __shared__ T lv[EPB * P];
+  __shared__ double gpart[GRAD ? EPB * P : 1];
+  T ufw = 0;
+  double gacc = 0.0;
   const int t = threadIdx.x;
   const int el = t / P, q = t - el * P;
   const int i = q % N, j = q / N;
@@ -237,8 +262,22 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
         r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
       }
       bst<T>(r, ry, off, 0u);
+      if constexpr (GRAD) {
+        if (s == 0) ufw = u;
+        else gacc += (double)o * (double)ufw;
+      }
     }
     __syncthreads();
+  }
+  if constexpr (GRAD) {
+    gpart[t] = act ? gacc : 0.0;
+    __syncthreads();
+    if (act && q == 0) {
+      double sum = 0.0;
+      for (int qq = 0; qq < P; ++qq) sum += gpart[el * P + qq];
+      const int64_t e = eb + el;
+      grad[e] -= sum / mu_e[e];
+    }
   }
 }
 
@@ -255,6 +294,18 @@ static int run_scalar(const StiffArgs &a, hipStream_t st) {
   const T *s = static_cast<const T *>(a.sdiag);
   const TG *G = static_cast<const TG *>(a.G);
   const T al = (T)a.alpha, be = (T)a.beta;
+  if (a.grad) {
+    if (a.mode != kLeap || a.nshots != 2) return -1001;
+    if (a.dim == 3)
+      hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kLeap, true, true>), dim3(blocks),
+                         dim3(threads), 0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N,
+                         a.nshots, a.nxcd, a.grad, a.mu);
+    else
+      hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kLeap, true>), dim3(blocks), dim3(threads), 0,
+                         st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd,
+                         a.grad, a.mu);
+    return (int)hipGetLastError();
+  }
   if (a.dim == 3) {
     if (a.mode == kLeap) {
       if (a.nshots > 1)

@@ -1013,3 +1013,86 @@ double or_lambda_max(int dim, int r, int ncomp, long E, long N, const int *elem_
   free(Q); free(wv); free(t1); free(sq); free(al); free(be);
   return lmax;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint gradient (FP64 reference)                                        */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Gradient of a receiver misfit with respect to per-element mu for the
+ * acoustic leapfrog with zero initial state.  No counterpart in the reference
+ * (it has no time-domain adjoint); derived from the leapfrog recurrence of
+ * ModifiedEquation.cxx:295-345 written as
+ *   U^{k+1} = 2U^k - U^{k-1} + S(-A U^k + f_{k-1} b),  S = dt^2 D^{-1} (0 on
+ *   Dirichlet rows), U^0 = U^1 = 0,  traces d^k = R U^k, k = 0..N.
+ * With h^k = R^T dJ/dd^k the scaled multipliers nu satisfy
+ *   nu_N = 0, nu_{N-1} = S h^N, nu_{j-1} = 2nu_j - nu_{j+1} + S(-A nu_j + h^j),
+ * and dJ/dmu_e = -sum_{j=1}^{N-1} nu_j^T (A_e/mu_e) U^j.  All forward states
+ * are stored (small problems only).  Pinned by central finite differences of
+ * the misfit in tests/test_oracle.py.
+ */
+void or_adjoint_acoustic(int dim, int r, long E, long N, const int *elem_nodes, const double *G,
+                         const double *mu, const int *color, int ncolor, const double *Dm,
+                         const unsigned char *dirichlet, const double *b, const double *pulse,
+                         double dt, int nsteps, int nrec, const long *rec_elem,
+                         const double *rec_w, const double *dJ, double *grad) {
+  int n = r + 1, nd = (dim == 2) ? n * n : n * n * n;
+  int nc = (dim == 2) ? 3 : 6;
+  double *U = (double *)calloc((size_t)N * (nsteps + 1), sizeof(double));
+  double *S = (double *)malloc(sizeof(double) * N);
+  double *t1 = (double *)malloc(sizeof(double) * N);
+  double *nu0 = (double *)calloc(N, sizeof(double)), *nu1 = (double *)calloc(N, sizeof(double));
+  double *h = (double *)malloc(sizeof(double) * N);
+  for (long g = 0; g < N; g++) S[g] = dirichlet[g] ? 0.0 : dt * dt / Dm[g];
+  for (int k = 1; k < nsteps; k++) {
+    double *Uk = U + (long)k * N, *Ukm = U + (long)(k - 1) * N, *Ukp = U + (long)(k + 1) * N;
+    or_stiffness(dim, r, E, N, elem_nodes, G, color, ncolor, 1.0, Uk, 0.0, t1);
+    for (long g = 0; g < N; g++) {
+      double f = -t1[g] + (b ? pulse[k - 1] * b[g] : 0.0);
+      Ukp[g] = 2.0 * Uk[g] - Ukm[g] + S[g] * f;
+    }
+  }
+#define OR_HVEC(row)                                                          \
+  do {                                                                        \
+    for (long g = 0; g < N; g++) h[g] = 0.0;                                  \
+    for (int s = 0; s < nrec; s++) {                                          \
+      long e = rec_elem[s];                                                   \
+      for (int q = 0; q < nd; q++)                                            \
+        h[elem_nodes[e * nd + q]] += rec_w[(long)s * nd + q] * dJ[(long)(row)*nrec + s]; \
+    }                                                                         \
+  } while (0)
+  /* nu1 = nu_{j+1}, nu0 = nu_j; start j = N-1 */
+  for (long g = 0; g < N; g++) nu1[g] = 0.0;
+  OR_HVEC(nsteps);
+  for (long g = 0; g < N; g++) nu0[g] = S[g] * h[g];
+  for (long e = 0; e < E; e++) grad[e] = 0.0;
+  double ue[OR_MAXN * OR_MAXN * OR_MAXN], ve[OR_MAXN * OR_MAXN * OR_MAXN],
+      oe[OR_MAXN * OR_MAXN * OR_MAXN];
+  double x[OR_MAXN1D], w[OR_MAXN1D], D[OR_MAXN1D * OR_MAXN1D];
+  or_gl(r, x, w);
+  or_gl_deriv(r, x, D);
+  for (int j = nsteps - 1; j >= 1; j--) {
+    const double *Uj = U + (long)j * N;
+    /* gradient term -nu_j^T A_e^0 U^j */
+    for (long e = 0; e < E; e++) {
+      for (int q = 0; q < nd; q++) {
+        ue[q] = Uj[elem_nodes[e * nd + q]];
+        ve[q] = nu0[elem_nodes[e * nd + q]];
+      }
+      local_apply(dim, n, D, G + e * nd * nc, ve, oe);
+      double s = 0.0;
+      for (int q = 0; q < nd; q++) s += oe[q] * ue[q];
+      grad[e] -= s / mu[e];
+    }
+    if (j == 1) break;
+    or_stiffness(dim, r, E, N, elem_nodes, G, color, ncolor, 1.0, nu0, 0.0, t1);
+    OR_HVEC(j);
+    for (long g = 0; g < N; g++) {
+      double nm = 2.0 * nu0[g] - nu1[g] + S[g] * (-t1[g] + h[g]);
+      nu1[g] = nu0[g];
+      nu0[g] = nm;
+    }
+  }
+#undef OR_HVEC
+  free(U); free(S); free(t1); free(nu0); free(nu1); free(h);
+}

@@ -80,6 +80,9 @@ def lib():
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 _ip, C.c_int, _dp, _bp, C.c_int, C.c_uint64]
     L.or_lambda_max.restype = C.c_double
+    L.or_adjoint_acoustic.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, _ip, _dp, _dp, _ip,
+                                      C.c_int, _dp, _bp, C.c_void_p, C.c_void_p, C.c_double,
+                                      C.c_int, C.c_int, _lp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -341,3 +344,23 @@ def leapfrog(P: Problem, b, pulse, dt, nsteps, rec_pts=None, rec_comp=None, u0=N
                       np.ascontiguousarray(re), np.ascontiguousarray(rw.reshape(-1)), _ptr(rc),
                       uN, uN1, tr)
     return uN, uN1, tr[: (nsteps + 1) * nrec].reshape(nsteps + 1, nrec)
+
+
+def adjoint_gradient(P: Problem, b, pulse, dt, nsteps, rec_pts, dJ):
+    """FP64 reference gradient dJ/dmu_e (zero initial state), dJ[(nsteps+1), nrec]."""
+    m = P.mesh
+    c, nc = P.colors
+    re, rw = receivers(m, rec_pts)
+    grad = np.zeros(m.E)
+    lib().or_adjoint_acoustic(m.dim, m.r, m.E, m.N, m.elem_nodes.reshape(-1), P.G,
+                              np.ascontiguousarray(P.mu, np.float64), c, nc, P.M, m.dirichlet,
+                              _ptr(b), _ptr(np.ascontiguousarray(pulse, np.float64)), dt, nsteps,
+                              len(re), np.ascontiguousarray(re), np.ascontiguousarray(rw.reshape(-1)),
+                              np.ascontiguousarray(dJ, np.float64).reshape(-1), grad)
+    return grad
+
+
+def forward_traces(P: Problem, b, pulse, dt, nsteps, rec_pts):
+    """Traces rows 0..nsteps of the zero-initial-state leapfrog."""
+    _, _, tr = leapfrog(P, b, pulse, dt, nsteps, rec_pts)
+    return tr

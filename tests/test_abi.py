@@ -83,3 +83,16 @@ def test_product_does_not_import_oracle():
                 txt = open(os.path.join(dp, f)).read()
                 assert "import oracle" not in txt
                 assert "from oracle" not in txt and "mjw_oracle" not in txt
+
+
+def test_cpp_adapter_builds_and_links(tmp_path):
+    """include/mjwave_adapter.hpp compiles against the headers and links
+    against libmjwave.so (run on the GPU by tests/test_gpu_parity.py)."""
+    import subprocess
+    exe = tmp_path / "adapter_example"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "adapter_example.cpp"), "-o", str(exe),
+                        "-L", os.path.join(ROOT, "montjoie_amd"), "-lmjwave",
+                        "-Wl,-rpath," + os.path.join(ROOT, "montjoie_amd")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

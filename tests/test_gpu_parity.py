@@ -163,3 +163,23 @@ def test_xcd_block_order_is_result_neutral():
     P.set_xcd_order(3)
     d = P.apply_kh(u)
     assert np.array_equal(a, b) and np.array_equal(a, d)
+
+
+def test_cpp_adapter_runs_on_gpu(tmp_path):
+    """The C++ adapter (HipWaveSystem + HipLeapFrogScheme) reproduces the
+    oracle's traces (2-D Q4 8x8, 200 steps)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "adapter_example"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "adapter_example.cpp"), "-o", str(exe),
+                    "-L", os.path.join(root, "montjoie_amd"), "-lmjwave",
+                    "-Wl,-rpath," + os.path.join(root, "montjoie_amd")], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=100, check=True)
+    got = np.array([[float(v) for v in line.split()] for line in out.stdout.strip().splitlines()])
+    m = O.mesh(2, 8, 4)
+    P = O.acoustic_problem(m, 1.0, 1.0)
+    _, _, tr = O.leapfrog(P, O.dirac(m, (0.5, 0.5)), O.pulse_table(10.0, 0.0, 1e-3, 200), 1e-3,
+                          200, [[0.8, 0.5], [0.3, 0.4]])
+    np.testing.assert_allclose(got, tr[100:103], rtol=1e-12, atol=1e-20)
