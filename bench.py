@@ -96,6 +96,128 @@ def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10):
     return dict(wall=wall, dev_ms_step=dev_ms, traces=tr, uN=uN, prof=prof, info=info)
 
 
+def shot_sources(nshots, seed=2027):
+    return np.array([[0.25 + 0.5 * _unif(seed, 3 * s + a) for a in range(3)]
+                     for s in range(nshots)])
+
+
+def bench_shots(m, mu, dt, steps, warmup, nshots=16):
+    """FP32 batch of `nshots` Dirac shots in one problem (geometry read once per
+    element for all shots) against the FP64 path run shot by shot."""
+    import montjoie_amd as M
+    srcs = shot_sources(nshots)
+    out = {}
+    for prec, ns in (("fp32", nshots), ("fp32_g64", nshots), ("fp64", 1)):
+        P = M.WaveProblem(m, 1.0, mu, precision=prec, nshots=ns)
+        P.set_sources(srcs[:ns], shot=np.arange(ns))
+        P.set_receivers(receivers())
+        P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, warmup + steps))
+        P.leapfrog_init(dt)
+        if warmup:
+            P.advance(warmup, traces=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        P.advance(steps, traces=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[prec] = dict(wall=wall, shots=ns, dev_ms=P.last_timing()[0])
+        P.close()
+    t64_batch = out["fp64"]["wall"] * nshots           # FP64 path, one shot at a time
+    return {
+        "shots": nshots,
+        "fp32_batch_s": out["fp32"]["wall"],
+        "fp32_g64_batch_s": out["fp32_g64"]["wall"],
+        "fp64_one_shot_s": out["fp64"]["wall"],
+        "fp64_path_16_shots_s": t64_batch,
+        "fp32_batch_gdof_steps_s": m.N * steps * nshots / out["fp32"]["wall"] / 1e9,
+        "fp32_g64_batch_gdof_steps_s": m.N * steps * nshots / out["fp32_g64"]["wall"] / 1e9,
+        "time_to_solution_ratio_fp64_over_fp32_batch": t64_batch / out["fp32"]["wall"],
+        "time_to_solution_ratio_fp64_over_fp32_g64_batch": t64_batch / out["fp32_g64"]["wall"],
+    }
+
+
+def smooth_field(m, seed, ng=4):
+    """Per-element sum of ng seeded Gaussians, scaled to [0, 1]."""
+    X = m.xyz[m.elem_nodes].mean(axis=1)
+    f = np.zeros(m.E)
+    for g in range(ng):
+        a = 2 * _unif(seed, 10 * g) - 1
+        c = np.array([0.2 + 0.6 * _unif(seed, 10 * g + 1 + k) for k in range(3)])
+        s = 0.1 + 0.15 * _unif(seed, 10 * g + 5)
+        f += a * np.exp(-np.sum((X - c) ** 2, axis=1) / (2 * s * s))
+    return (f - f.min()) / (f.max() - f.min())
+
+
+def bench_adjoint(nx=40, order=4, steps=2000, nsnap=20, ndir=8):
+    """Gradient of an L2 receiver misfit w.r.t. per-element c^2 on a 40^3 Q4
+    cube: cost of forward + adjoint over a plain forward solve (FP32), FD check
+    of the FP64 gradient and FP32 vs FP64 gradient error."""
+    import montjoie_amd as M
+    from montjoie_amd.adjoint import AdjointSolver
+    m = M.regular_mesh(3, nx, order, 0.0, 0)
+    mtrue = 1.0 + 8.0 * smooth_field(m, 2029)
+    m0 = mtrue * (1.0 + 0.1 * (2 * smooth_field(m, 2030) - 1))
+    g1 = np.linspace(0.1, 0.9, 8)
+    rec = np.array([[x, y, 0.95] for y in g1 for x in g1])
+    P64 = M.WaveProblem(m, 1.0, m0, precision="fp64")
+    dt = 0.5 * 2.0 / np.sqrt(P64.lambda_max(60, 2))
+    res = {"workload": f"{nx}^3 Q{order} cube, mu = c^2 smooth in [1,9] (seed 2029), model "
+                       f"perturbed 10% (seed 2030), {steps} steps, 64 receivers (8x8 at z=0.95), "
+                       f"{nsnap} checkpoints", "dt": dt, "n_dof": m.N}
+    solvers = {}
+    for prec in ("fp32", "fp64"):
+        P = P64 if prec == "fp64" else M.WaveProblem(m, 1.0, m0, precision=prec)
+        P.set_sources([(0.5, 0.5, 0.5)])
+        P.set_receivers(rec)
+        P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, steps + 4))
+        solvers[prec] = (P, AdjointSolver(P, dt, steps, nsnap))
+    grads = {}
+    for prec in ("fp32", "fp64"):
+        P, S = solvers[prec]
+        dobs = S.forward(mtrue)
+        S.set_mu(m0)
+        # plain forward solve (same steps, traces on)
+        P.leapfrog_init(dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        P.advance(steps, traces=True)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        tr = S.forward()
+        g = S.backward(tr - dobs)
+        torch.cuda.synchronize()
+        t_grad = time.perf_counter() - t0
+        dev = S.timing()
+        grads[prec] = g
+        res[prec] = {"forward_s": t_fwd, "forward_plus_adjoint_s": t_grad,
+                     "gradient_over_forward_cost": t_grad / t_fwd,
+                     "device_ms_adjoint_forward": dev[0], "device_ms_adjoint_backward": dev[1]}
+        if prec == "fp64":
+            P, S = solvers["fp64"]
+            J = lambda mu: 0.5 * np.sum((S.forward(mu) - dobs) ** 2)  # noqa: E731
+            rng = np.random.default_rng(2031)
+            errs2, errs4 = [], []
+            for _ in range(ndir):
+                d = rng.standard_normal(m.E)
+                h = 1e-4 * np.linalg.norm(m0) / np.linalg.norm(d)
+                jp, jm = J(m0 + h * d), J(m0 - h * d)
+                jp2, jm2 = J(m0 + 2 * h * d), J(m0 - 2 * h * d)
+                fd2 = (jp - jm) / (2 * h)
+                fd4 = (-jp2 + 8 * jp - 8 * jm + jm2) / (12 * h)
+                errs2.append(abs(fd2 - g @ d) / abs(fd4))
+                errs4.append(abs(fd4 - g @ d) / abs(fd4))
+            res["fd_check_fp64_max_rel_err"] = float(max(errs4))
+            res["fd_stencil"] = "central, 4th order, h = 1e-4 ||m|| / ||d||"
+            res["fd2_check_fp64_max_rel_err"] = float(max(errs2))
+            res["fd_directions"] = ndir
+    res["fp32_vs_fp64_gradient_rel_l2"] = float(np.linalg.norm(grads["fp32"] - grads["fp64"]) /
+                                               np.linalg.norm(grads["fp64"]))
+    for P, _ in solvers.values():
+        P.close()
+    return res
+
+
 def cpu_reference(nx=24, order=5, steps=30, dt=1e-3):
     """Montjoie's own solver (oracle/_ref, built from the reference sources)
     on a bounded sample of the same workload family, one core."""
@@ -146,6 +268,8 @@ def main():
                     help="run one precision only (profiling runs)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--quick", action="store_true",
+                    help="skip the 16-shot batch and the adjoint measurement")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}); the multi-GPU sharding "
@@ -226,6 +350,11 @@ def main():
                      (ref["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9},
         "cpu_baseline": None,
     }
+    if not a.quick:
+        out["batch_16_shots"] = bench_shots(m, mu, dt, a.steps, a.warmup)
+        print(f"# 16 shots: {out['batch_16_shots']}", file=sys.stderr, flush=True)
+        out["adjoint"] = bench_adjoint()
+        print(f"# adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_reference()
     print(json.dumps(out))
