@@ -100,6 +100,8 @@ int mjw_get_tile(const mjw_problem *p, int *elems_per_block);
  * one XCD (round-robin over nxcd dies) get one contiguous run of elements, so
  * neighbours share an L2.  nxcd = 1 disables it; default 8 (MI355X). */
 int mjw_set_xcd_order(mjw_problem *p, int nxcd);
+/* supported elements-per-block values (up to cap); returns their count */
+int mjw_tile_choices(const mjw_problem *p, int *out, int cap);
 
 /* ---------------------------------------------------------------- operators */
 /*
@@ -240,8 +242,19 @@ int mjw_mesh_regular(int dim, int nx, int order, double pert, uint64_t seed, dou
  * sources, receivers and state (initialise them with mjw_leapfrog_init).
  */
 typedef struct mjw_batch mjw_batch;
+/* single-shot 3-D acoustic problems, orders 1..7, same device and precision */
 int mjw_batch_create(mjw_batch **out, int nprob, mjw_problem *const *probs);
+/* advance every problem nsteps (all at the same step parity); traces of each
+ * problem are kept on the device, rows 0..nsteps-1 = R U^n before each step */
 int mjw_batch_advance(mjw_batch *b, int nsteps, int use_graph);
+/* traces [nsteps][nrec] of problem iprob from the last mjw_batch_advance */
+int mjw_batch_traces(mjw_batch *b, int iprob, double *traces);
+/* tuning: order_mode 0 = work items by decreasing element order (default),
+ * 1 = problem by problem; nxcd reserved */
+int mjw_batch_config(mjw_batch *b, int nxcd, int order_mode);
+/* device ms per step of the last mjw_batch_advance */
+int mjw_batch_timing(const mjw_batch *b, double *ms);
+const char *mjw_batch_last_error(const mjw_batch *b);
 int mjw_batch_destroy(mjw_batch *b);
 
 /* ------------------------------------------------------------- timing */

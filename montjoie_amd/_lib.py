@@ -56,6 +56,11 @@ SIGNATURES = {
     "mjw_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
     "mjw_batch_advance": (C.c_int, [_vp, C.c_int, C.c_int]),
     "mjw_batch_destroy": (C.c_int, [_vp]),
+    "mjw_batch_traces": (C.c_int, [_vp, C.c_int, _dp]),
+    "mjw_batch_config": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "mjw_batch_timing": (C.c_int, [_vp, _dp]),
+    "mjw_batch_last_error": (C.c_char_p, [_vp]),
+    "mjw_tile_choices": (C.c_int, [_vp, C.POINTER(C.c_int), C.c_int]),
     "mjw_last_timing": (C.c_int, [_vp, _dp]),
     "mjw_profile": (C.c_int, [_vp, C.c_int, _dp]),
     "mjw_adjoint_timing": (C.c_int, [_vp, _dp]),

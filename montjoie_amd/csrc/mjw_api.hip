@@ -1317,18 +1317,273 @@ extern "C" int mjw_adjoint_timing(const mjw_problem *p, double *ms2) {
   return MJW_OK;
 }
 
-// ------------------------------------------------------- batch (not yet built)
+// ------------------------------------------------------------------ batch
+// Independent 3-D acoustic problems (uneven sizes and orders) advanced
+// together: per step one stiffness launch per colour index over the work
+// items of every problem having that colour (k_batch3d dispatches on the
+// element order), then one tail block per problem; the step loop is captured
+// in a HIP graph.
 struct mjw_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
   std::vector<mjw_problem *> probs;
+  int state = kF64, geom = kG64;
+  int ncolor = 0;
+  int order_mode = 0;                 // 0: items by decreasing order, 1: problem-major
+  BatchProb *d_probs = nullptr;
+  std::vector<BatchItem *> d_items;   // per colour
+  std::vector<int> nitems;
+  TailArgs *d_tail[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec = nullptr;
+  bool g_traces = false;
+  int64_t last_nsteps = 0;
+  double ms_step = 0.0;
+  std::string err;
 };
-extern "C" int mjw_batch_create(mjw_batch **out, int, mjw_problem *const *) {
-  if (out) *out = nullptr;
-  FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED, "batching not available in this build");
+
+static void batch_free_items(mjw_batch *b) {
+  for (BatchItem *q : b->d_items) dfree(q);
+  b->d_items.clear();
+  b->nitems.clear();
 }
-extern "C" int mjw_batch_advance(mjw_batch *, int, int) {
-  FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED, "batching not available in this build");
+
+static int batch_build_items(mjw_batch *b) {
+  batch_free_items(b);
+  b->ncolor = 0;
+  for (mjw_problem *p : b->probs) b->ncolor = std::max(b->ncolor, p->ncolor);
+  for (int c = 0; c < b->ncolor; ++c) {
+    std::vector<std::pair<int, BatchItem>> items;  // (order key, item)
+    for (size_t i = 0; i < b->probs.size(); ++i) {
+      mjw_problem *p = b->probs[i];
+      if (c >= p->ncolor) continue;
+      const int epb = batch_epb(p->order);
+      for (int64_t eb = p->coff[c]; eb < p->coff[c + 1]; eb += epb) {
+        BatchItem it{};
+        it.prob = (int32_t)i;
+        it.eb = eb;
+        it.e1 = p->coff[c + 1];
+        items.push_back({b->order_mode == 0 ? -p->order : (int)i, it});
+      }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const auto &x, const auto &y) { return x.first < y.first; });
+    std::vector<BatchItem> h(items.size());
+    for (size_t k = 0; k < items.size(); ++k) h[k] = items[k].second;
+    BatchItem *d = nullptr;
+    if (hipMalloc(&d, std::max<size_t>(1, h.size()) * sizeof(BatchItem)) != hipSuccess) return -1;
+    if (!h.empty() &&
+        hipMemcpy(d, h.data(), h.size() * sizeof(BatchItem), hipMemcpyHostToDevice) != hipSuccess)
+      return -1;
+    b->d_items.push_back(d);
+    b->nitems.push_back((int)h.size());
+  }
+  return 0;
 }
+
+#define BFAIL(b, code, msg)               \
+  do {                                    \
+    if (b) (b)->err = (msg);              \
+    g_err = (msg);                        \
+    return (code);                        \
+  } while (0)
+
+extern "C" int mjw_batch_create(mjw_batch **out, int nprob, mjw_problem *const *probs) {
+  if (!out || nprob < 1 || !probs) BFAIL((mjw_batch *)nullptr, MJW_ERR_ARG, "mjw_batch_create: bad args");
+  *out = nullptr;
+  for (int i = 0; i < nprob; ++i) {
+    const mjw_problem *p = probs[i];
+    if (!p) BFAIL((mjw_batch *)nullptr, MJW_ERR_ARG, "mjw_batch_create: NULL problem");
+    if (p->dim != 3 || p->ncomp != 1 || p->nshots != 1)
+      BFAIL((mjw_batch *)nullptr, MJW_ERR_UNSUPPORTED, "batch: single-shot 3-D acoustic problems");
+    if (p->device != probs[0]->device || p->state != probs[0]->state ||
+        p->geom != probs[0]->geom)
+      BFAIL((mjw_batch *)nullptr, MJW_ERR_ARG, "batch: problems must share device and precision");
+  }
+  mjw_batch *b = new mjw_batch();
+  b->device = probs[0]->device;
+  b->state = probs[0]->state;
+  b->geom = probs[0]->geom;
+  b->probs.assign(probs, probs + nprob);
+  if (hipSetDevice(b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    BFAIL((mjw_batch *)nullptr, MJW_ERR_HIP, "batch: stream creation failed");
+  }
+  std::vector<BatchProb> hp(nprob);
+  for (int i = 0; i < nprob; ++i) {
+    mjw_problem *p = probs[i];
+    BatchProb &d = hp[i];
+    d.map = p->d_map; d.G = p->d_G; d.Dm = p->d_D; d.sdiag = p->d_sdiag;
+    d.buf[0] = p->d_U[0]; d.buf[1] = p->d_U[1];
+    d.E = p->E; d.N = p->N; d.order = p->order;
+  }
+  int rc = (int)hipMalloc(&b->d_probs, nprob * sizeof(BatchProb));
+  if (!rc) rc = (int)hipMemcpy(b->d_probs, hp.data(), nprob * sizeof(BatchProb), hipMemcpyHostToDevice);
+  if (!rc) rc = (int)hipMalloc(&b->d_tail[0], nprob * sizeof(TailArgs));
+  if (!rc) rc = (int)hipMalloc(&b->d_tail[1], nprob * sizeof(TailArgs));
+  if (!rc) rc = batch_build_items(b);
+  if (rc) {
+    mjw_batch_destroy(b);
+    BFAIL((mjw_batch *)nullptr, MJW_ERR_HIP, "batch: setup failed");
+  }
+  *out = b;
+  return MJW_OK;
+}
+
+extern "C" int mjw_batch_config(mjw_batch *b, int nxcd, int order_mode) {
+  if (!b || order_mode < 0 || order_mode > 1) BFAIL(b, MJW_ERR_ARG, "mjw_batch_config: bad args");
+  (void)nxcd;  // block order of the batch launches follows the work list
+  if (order_mode != b->order_mode) {
+    b->order_mode = order_mode;
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (batch_build_items(b)) BFAIL(b, MJW_ERR_HIP, "batch: work list rebuild failed");
+  }
+  return MJW_OK;
+}
+
+static int batch_enqueue_step(mjw_batch *b, int parity) {
+  for (int c = 0; c < b->ncolor; ++c) {
+    int rc = launch_batch3d(b->state, b->geom, b->d_probs, b->d_items[c], b->nitems[c], parity,
+                            b->stream);
+    if (rc) return rc;
+  }
+  return launch_tail_batch(b->state, b->d_tail[parity], (int)b->probs.size(), b->stream);
+}
+
+extern "C" int mjw_batch_advance(mjw_batch *b, int nsteps, int use_graph) {
+  if (!b || nsteps < 0) BFAIL(b, MJW_ERR_ARG, "mjw_batch_advance: bad args");
+  if (nsteps == 0) return MJW_OK;
+  if (hipSetDevice(b->device) != hipSuccess) BFAIL(b, MJW_ERR_HIP, "hipSetDevice");
+  const int parity = b->probs[0]->cur;
+  bool traces = false;
+  for (mjw_problem *p : b->probs) {
+    if (!p->inited) BFAIL(b, MJW_ERR_STATE, "batch: every problem needs mjw_leapfrog_init");
+    if (p->cur != parity) BFAIL(b, MJW_ERR_STATE, "batch: problems must be at the same parity");
+    traces = traces || p->nrec > 0;
+  }
+  // per-problem tail arguments for both parities (read by the kernels at run time)
+  std::vector<TailArgs> ta[2];
+  for (int par = 0; par < 2; ++par) ta[par].resize(b->probs.size());
+  for (size_t i = 0; i < b->probs.size(); ++i) {
+    mjw_problem *p = b->probs[i];
+    if (hipStreamSynchronize(p->stream) != hipSuccess) BFAIL(b, MJW_ERR_HIP, "sync");
+    if (p->nrec && ensure_traces(p, nsteps)) BFAIL(b, MJW_ERR_NOMEM, "batch: traces buffer");
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+    for (int par = 0; par < 2; ++par) {
+      TailArgs &t = ta[par][i];
+      t = TailArgs{};
+      t.state = p->state; t.ctl = p->d_ctl; t.pulse = p->d_pulse; t.npulse = p->npulse;
+      t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot;
+      t.src_val = p->d_src_val; t.sdiag = p->d_sdiag; t.nrec = p->nrec; t.nd = p->nd;
+      t.rec_gid = p->d_rec_gid; t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp;
+      t.U = p->d_U[par ^ 1]; t.P = p->d_U[par]; t.traces = p->nrec ? p->d_traces : nullptr;
+      t.N = p->N; t.nshots = 1; t.advance = 1; t.row_offset = 1;
+    }
+    int64_t ctl[2] = {p->step, p->step};
+    if (hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+      BFAIL(b, MJW_ERR_HIP, "ctl");
+    if (p->nrec) {  // row 0 = U^n
+      TailArgs r{};
+      r.state = p->state; r.ctl = p->d_ctl; r.nrec = p->nrec; r.nd = p->nd;
+      r.rec_gid = p->d_rec_gid; r.rec_w = p->d_rec_w; r.rec_comp = p->d_rec_comp;
+      r.U = p->d_U[p->cur]; r.traces = p->d_traces; r.N = p->N; r.nshots = 1;
+      if (launch_tail_n(r, 1, b->stream)) BFAIL(b, MJW_ERR_HIP, "batch record");
+    }
+  }
+  for (int par = 0; par < 2; ++par)
+    if (hipMemcpyAsync(b->d_tail[par], ta[par].data(), ta[par].size() * sizeof(TailArgs),
+                       hipMemcpyHostToDevice, b->stream) != hipSuccess)
+      BFAIL(b, MJW_ERR_HIP, "tail args");
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipEventRecord(e0, b->stream);
+  int done = 0, cur = parity;
+  if (use_graph && nsteps >= kGraphSteps) {
+    if (cur != 0) {
+      if (batch_enqueue_step(b, cur)) BFAIL(b, MJW_ERR_HIP, "batch step");
+      cur ^= 1;
+      ++done;
+    }
+    if (nsteps - done >= kGraphSteps) {
+      if (!b->gexec) {
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+          BFAIL(b, MJW_ERR_HIP, "capture");
+        int rc = 0;
+        for (int s = 0; s < kGraphSteps && !rc; ++s) rc = batch_enqueue_step(b, s & 1);
+        hipError_t ee = hipStreamEndCapture(b->stream, &g);
+        if (rc || ee != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          BFAIL(b, MJW_ERR_HIP, "batch graph capture");
+        }
+        ee = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ee != hipSuccess) { b->gexec = nullptr; BFAIL(b, MJW_ERR_HIP, "batch graph"); }
+      }
+      while (nsteps - done >= kGraphSteps) {
+        if (hipGraphLaunch(b->gexec, b->stream) != hipSuccess) BFAIL(b, MJW_ERR_HIP, "graph launch");
+        done += kGraphSteps;
+      }
+    }
+  }
+  for (; done < nsteps; ++done) {
+    if (batch_enqueue_step(b, cur)) BFAIL(b, MJW_ERR_HIP, "batch step");
+    cur ^= 1;
+  }
+  (void)hipEventRecord(e1, b->stream);
+  if (hipEventSynchronize(e1) != hipSuccess) BFAIL(b, MJW_ERR_HIP, "batch run failed");
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->ms_step = ms / nsteps;
+  for (mjw_problem *p : b->probs) {
+    p->cur = cur;
+    p->step += nsteps;
+  }
+  b->last_nsteps = nsteps;
+  return MJW_OK;
+}
+
+extern "C" int mjw_batch_traces(mjw_batch *b, int iprob, double *traces) {
+  if (!b || iprob < 0 || iprob >= (int)b->probs.size() || !traces)
+    BFAIL(b, MJW_ERR_ARG, "mjw_batch_traces: bad args");
+  mjw_problem *p = b->probs[iprob];
+  if (!p->nrec) return MJW_OK;
+  if (hipMemcpy(traces, p->d_traces, (size_t)b->last_nsteps * p->nrec * 8,
+                hipMemcpyDeviceToHost) != hipSuccess)
+    BFAIL(b, MJW_ERR_HIP, "copy");
+  return MJW_OK;
+}
+
+extern "C" int mjw_batch_timing(const mjw_batch *b, double *ms) {
+  if (!b || !ms) BFAIL((mjw_batch *)nullptr, MJW_ERR_ARG, "NULL");
+  *ms = b->ms_step;
+  return MJW_OK;
+}
+
+extern "C" const char *mjw_batch_last_error(const mjw_batch *b) {
+  return b ? b->err.c_str() : g_err.c_str();
+}
+
 extern "C" int mjw_batch_destroy(mjw_batch *b) {
+  if (!b) return MJW_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  batch_free_items(b);
+  dfree(b->d_probs);
+  dfree(b->d_tail[0]);
+  dfree(b->d_tail[1]);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return MJW_OK;
+}
+
+extern "C" int mjw_tile_choices(const mjw_problem *p, int *out, int cap) {
+  if (!p || (!out && cap > 0)) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
+  std::vector<int> v = tile_choices(p->dim, p->order, p->ncomp);
+  for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
+  return (int)v.size();
 }

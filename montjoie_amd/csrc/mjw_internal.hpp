@@ -151,4 +151,22 @@ struct GradArgs {
 };
 int launch_gradient(const GradArgs &a, void *stream);
 
+// ---------------------------------------------------------------- batching
+struct BatchProb {          // device descriptor of one problem of a batch
+  const int32_t *map;
+  const void *G, *Dm, *sdiag;
+  void *buf[2];             // the problem's two state buffers
+  int64_t E, N;
+  int order, pad;
+};
+struct BatchItem {          // one block of work: elements [eb, min(eb+EPB, e1))
+  int32_t prob, pad;
+  int64_t eb, e1;
+};
+int batch_epb(int order);
+int launch_batch3d(int state, int geom, const BatchProb *probs, const BatchItem *items,
+                   int nitems, int parity, void *stream);
+// one tail block per problem: args[blockIdx.x]
+int launch_tail_batch(int state, const TailArgs *args, int nprob, void *stream);
+
 }  // namespace mjw

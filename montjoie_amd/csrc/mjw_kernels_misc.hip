@@ -178,7 +178,7 @@ int launch_mass_assemble(const int32_t *map, const double *Mloc, int64_t E, int 
 constexpr int kTailThreads = 1024;
 
 template <typename T>
-__global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs a, int ncomp) {
+__device__ __forceinline__ void tail_body(const TailArgs &a, int ncomp) {
   const int64_t step = a.ctl[0];
   const int64_t base = a.ctl[1];
   const T *U = static_cast<const T *>(a.U);
@@ -224,6 +224,25 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs a, int ncomp) {
   }
   __syncthreads();
   if (a.advance && threadIdx.x == 0) a.ctl[0] = step + a.advance;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs a, int ncomp) {
+  tail_body<T>(a, ncomp);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTailThreads) void k_tail_batch(const TailArgs *__restrict__ args) {
+  tail_body<T>(args[blockIdx.x], 1);
+}
+
+int launch_tail_batch(int state, const TailArgs *args, int nprob, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_tail_batch<double>, dim3(nprob), dim3(kTailThreads), 0, st, args);
+  else
+    hipLaunchKernelGGL(k_tail_batch<float>, dim3(nprob), dim3(kTailThreads), 0, st, args);
+  return (int)hipGetLastError();
 }
 
 int launch_tail_n(const TailArgs &a, int ncomp, void *stream) {

@@ -35,29 +35,28 @@ struct Mix {  // geometry product in the geometry's precision
   using A = TG;
 };
 
-template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD = false>
-__global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
-    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
-    int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
-    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd,
-    double *__restrict__ grad = nullptr, const double *__restrict__ mu_e = nullptr) {
+// Body of the 3-D scalar stiffness pass for the elements [eb, min(eb+EPB, e1))
+// of one colour; threads t >= EPB*n^2 of the block idle (they only meet the
+// barriers).  lu, lv: 2*EPB*n^3 values of LDS; gpart: EPB*n^2 doubles (GRAD).
+template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD>
+__device__ __forceinline__ void scalar3d_body(
+    T *lu, T *lv, double *gpart, const int32_t *__restrict__ map, const TG *__restrict__ G,
+    const T *__restrict__ Dm, int64_t E, int64_t eb, int64_t e1, const T *__restrict__ U,
+    T *__restrict__ Y, const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots,
+    double *__restrict__ grad, const double *__restrict__ mu_e) {
   // GRAD (adjoint step, nshots == 2: shot 0 = forward iterate U^j rebuilt
   // backwards, shot 1 = adjoint nu_j): grad[e] -= (A_e nu_e . U_e) / mu_e
   constexpr int P = N * N, NN = N * N * N;
   using A = typename Mix<T, TG>::A;
-  __shared__ T lu[EPB * NN];
-  __shared__ T lv[EPB * NN];
-  __shared__ double gpart[GRAD ? EPB * P : 1];
   T ufw[GRAD ? N : 1];  // shot-0 values of the column (forward iterate)
   double gacc = 0.0;
   const int t = threadIdx.x;
   const int el = t / P, ij = t - el * P;
   const int i = ij % N, j = ij / N;
   // block-uniform resources; per-lane byte offsets are 32-bit
-  const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * EPB;
   const int64_t plane = E * P;
   const uint32_t lo = (uint32_t)(el * P + ij);
-  const bool act = eb + el < e1;
+  const bool act = el < EPB && eb + el < e1;
   const rsrc_t rmap = mkres(map + eb * P);
   const uint32_t map_ps = (uint32_t)(plane * 4);
   const uint32_t g_ps = (uint32_t)(plane * sizeof(TG));
@@ -167,7 +166,7 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
   }
   if constexpr (GRAD) {
     // per-element sum in a fixed order (deterministic), FP64
-    gpart[t] = act ? gacc : 0.0;
+    if (t < EPB * P) gpart[t] = act ? gacc : 0.0;
     __syncthreads();
     if (act && ij == 0) {
       double sum = 0.0;
@@ -176,6 +175,20 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
       grad[e] -= sum / mu_e[e];
     }
   }
+}
+
+template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD = false>
+__global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
+    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd,
+    double *__restrict__ grad = nullptr, const double *__restrict__ mu_e = nullptr) {
+  __shared__ T lu[EPB * N * N * N];
+  __shared__ T lv[EPB * N * N * N];
+  __shared__ double gpart[GRAD ? EPB * N * N : 1];
+  const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * EPB;
+  scalar3d_body<N, EPB, T, TG, MODE, HOIST, GRAD>(lu, lv, gpart, map, G, Dm, E, eb, e1, U, Y,
+                                                  sdiag, alpha, beta, N_nodes, nshots, grad, mu_e);
 }
 
 template <int N, int EPB, typename T, typename TG, int MODE, bool GRAD = false>
@@ -279,6 +292,73 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
       grad[e] -= sum / mu_e[e];
     }
   }
+}
+
+// ------------------------------------------------------------------ batch
+// Many independent 3-D acoustic problems (any mesh size, orders 1..7) in one
+// launch per colour: block b takes work item b = (problem, first element of
+// its block); the element order selects the compiled body (blocks of 256
+// threads, floor(256 / n^2) elements per block).  parity selects which of
+// the problem's two state buffers holds U^{n+1} (read) and U^n (overwritten
+// by U^{n+2}), as in the single-problem step.
+constexpr int kBatchThreads = 256;
+template <int N>
+struct BatchEpb {
+  static constexpr int value = kBatchThreads / (N * N);
+};
+
+template <typename T, typename TG>
+__global__ __launch_bounds__(kBatchThreads) void k_batch3d(const BatchProb *__restrict__ probs,
+                                                           const BatchItem *__restrict__ items,
+                                                           int parity) {
+  constexpr int kLds = 2 * 4 * 512;  // max over orders of 2*EPB*n^3
+  __shared__ T lds[kLds];
+  const BatchItem it = items[blockIdx.x];
+  const BatchProb &p = probs[it.prob];
+  const T *U = static_cast<const T *>(p.buf[parity ^ 1]);
+  T *Y = static_cast<T *>(p.buf[parity]);
+  const TG *G = static_cast<const TG *>(p.G);
+  const T *D = static_cast<const T *>(p.Dm);
+  const T *s = static_cast<const T *>(p.sdiag);
+#define MJW_BATCH_CASE(ORD)                                                                     \
+  case ORD: {                                                                                  \
+    constexpr int N = ORD + 1, EPB = BatchEpb<N>::value;                                       \
+    scalar3d_body<N, EPB, T, TG, kLeap, false, false>(lds, lds + EPB * N * N * N, nullptr,     \
+                                                      p.map, G, D, p.E, it.eb, it.e1, U, Y, s,  \
+                                                      T(0), T(0), p.N, 1, nullptr, nullptr);   \
+  } break;
+  switch (p.order) {
+    MJW_BATCH_CASE(1)
+    MJW_BATCH_CASE(2)
+    MJW_BATCH_CASE(3)
+    MJW_BATCH_CASE(4)
+    MJW_BATCH_CASE(5)
+    MJW_BATCH_CASE(6)
+    MJW_BATCH_CASE(7)
+    default: break;
+  }
+#undef MJW_BATCH_CASE
+}
+
+int batch_epb(int order) {
+  const int n = order + 1;
+  return kBatchThreads / (n * n);
+}
+
+int launch_batch3d(int state, int geom, const BatchProb *probs, const BatchItem *items,
+                   int nitems, int parity, void *stream) {
+  if (nitems <= 0) return 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL((k_batch3d<double, double>), dim3(nitems), dim3(kBatchThreads), 0, st,
+                       probs, items, parity);
+  else if (geom == kG64)
+    hipLaunchKernelGGL((k_batch3d<float, double>), dim3(nitems), dim3(kBatchThreads), 0, st,
+                       probs, items, parity);
+  else
+    hipLaunchKernelGGL((k_batch3d<float, float>), dim3(nitems), dim3(kBatchThreads), 0, st,
+                       probs, items, parity);
+  return (int)hipGetLastError();
 }
 
 // --------------------------------------------------------------- dispatch

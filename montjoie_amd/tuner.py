@@ -1,0 +1,72 @@
+"""Project autotuner: picks the launch shape of the stiffness kernels by
+timing candidates on the actual problem (HIP events per launch).
+
+* single problems: elements per block (``mjw_tile_choices``) x XCD block
+  order (on / off);
+* batches: work-list order (by decreasing element order / problem-major).
+
+Every candidate computes the same arithmetic per node (the colour order of
+the scatter is unchanged), so tuning never changes results; it advances the
+problems' state, so tune before ``leapfrog_init`` of the production run (the
+tuner re-initialises with a zero state when given ``dt``).  Results are
+cached per (dim, order, ncomp, precision, element count) for the process.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import _lib
+
+_CACHE: dict = {}
+
+
+def tile_choices(P) -> list[int]:
+    buf = (C.c_int * 16)()
+    n = _lib.lib().mjw_tile_choices(P.handle, buf, 16)
+    return [buf[i] for i in range(max(n, 0))]
+
+
+def tune_problem(P, steps: int = 8, dt: float | None = None, use_cache: bool = True) -> dict:
+    info = P.info()
+    key = (info["dim"], info["order"], info["ncomp"], info["precision"], info["n_elem"],
+           info["nshots"])
+    if use_cache and key in _CACHE:
+        best = _CACHE[key]
+    else:
+        if dt is not None:
+            P.leapfrog_init(dt)
+        results = []
+        for epb in tile_choices(P):
+            for nxcd in (1, 8):
+                P.set_tile(epb)
+                P.set_xcd_order(nxcd)
+                P.profile(2)  # warm
+                r = P.profile(steps)
+                results.append((r["stiffness_launch_ms"], epb, nxcd))
+        results.sort()
+        best = {"stiffness_launch_ms": results[0][0], "tile": results[0][1],
+                "nxcd": results[0][2], "candidates": results}
+        _CACHE[key] = best
+    P.set_tile(best["tile"])
+    P.set_xcd_order(best["nxcd"])
+    if dt is not None:
+        P.leapfrog_init(dt)
+    return best
+
+
+def tune_batch(B, steps: int = 6, dt: float | None = None) -> dict:
+    results = []
+    for mode in (0, 1):
+        B.config(mode)
+        if dt is not None:
+            for P in B.problems:
+                P.leapfrog_init(dt)
+        B.advance(2, graph=False)
+        B.advance(steps, graph=False)
+        results.append((B.ms_per_step(), mode))
+    results.sort()
+    B.config(results[0][1])
+    if dt is not None:
+        for P in B.problems:
+            P.leapfrog_init(dt)
+    return {"ms_per_step": results[0][0], "order_mode": results[0][1], "candidates": results}
