@@ -276,7 +276,102 @@ def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028,
             "setup_s": t_setup, "autotuned": tuned and {"order_mode": tuned["order_mode"]}}
 
 
-def cpu_reference(nx=24, order=5, steps=30, dt=1e-3):
+def sphere_dirs(n, seed=2027):
+    """n unit vectors, uniform on the sphere (z ~ U[-1,1], phi ~ U[0,2pi))."""
+    out = []
+    for s in range(n):
+        z = 2 * _unif(seed, 1000 + 2 * s) - 1
+        ph = 2 * np.pi * _unif(seed, 1001 + 2 * s)
+        r = np.sqrt(max(0.0, 1 - z * z))
+        out.append([r * np.cos(ph), r * np.sin(ph), z])
+    return np.array(out)
+
+
+def bench_elastic(nx=32, order=4, steps=800, nshots=16, warmup=10):
+    """Isotropic elastodynamics, Q4 hexahedra nx^3 (vertices perturbed <= 0.1h,
+    seed 2024), 3 layers lambda, mu ~ U[1,4], rho ~ U[1,2] (seed 2026);
+    16 Dirac shots at U[0.25,0.75]^3 with directions uniform on the sphere
+    (seed 2027), Ricker f0 = 8, dt = 0.5 CFL.  FP32 16-shot batch against
+    the FP64 path (batch for the error, one shot for the shot-by-shot time)."""
+    import montjoie_amd as M
+    m = M.regular_mesh(3, nx, order, 0.1, 2024)
+    lam = M.layered_values(m, [1.0 + 3.0 * _unif(2026, k) for k in range(3)])
+    mu = M.layered_values(m, [1.0 + 3.0 * _unif(2026, 3 + k) for k in range(3)])
+    rho = M.layered_values(m, [1.0 + _unif(2026, 6 + k) for k in range(3)])
+    srcs, pol = shot_sources(nshots), sphere_dirs(nshots)
+    rec = np.repeat(receivers(), 3, axis=0)
+    comp = np.tile(np.arange(3, dtype=np.int32), len(rec) // 3)
+    P = M.WaveProblem(m, rho, mu, lam=lam, ncomp=3, precision="fp64")
+    dt = 0.5 * 2.0 / np.sqrt(P.lambda_max(60, 1))
+    P.close()
+    res = {}
+    for prec, ns in (("fp32", nshots), ("fp64", nshots), ("fp64_1", 1)):
+        P = M.WaveProblem(m, rho, mu, lam=lam, ncomp=3, nshots=ns,
+                          precision=prec.split("_")[0])
+        P.set_sources(srcs[:ns], pol=pol[:ns], shot=np.arange(ns))
+        P.set_receivers(rec, comp)
+        P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, warmup + steps))
+        P.leapfrog_init(dt)
+        if warmup:
+            P.advance(warmup, traces=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr = P.advance(steps, traces=True)
+        torch.cuda.synchronize()
+        res[prec] = dict(wall=time.perf_counter() - t0, traces=tr, uN=P.state()[1])
+        P.close()
+        print(f"# elastic {prec}: {res[prec]['wall']:.3f} s", file=sys.stderr, flush=True)
+    ndof = 3 * m.N
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+    t64_path = res["fp64_1"]["wall"] * nshots
+    return {
+        "workload": f"elastic Q{order} {nx}^3, 3 layers, {nshots} shots, {steps} steps",
+        "n_dof": ndof, "dt": dt,
+        "fp32_batch_s": res["fp32"]["wall"],
+        "fp32_batch_gdof_steps_s": ndof * steps * nshots / res["fp32"]["wall"] / 1e9,
+        "fp64_batch_gdof_steps_s": ndof * steps * nshots / res["fp64"]["wall"] / 1e9,
+        "fp64_one_shot_gdof_steps_s": ndof * steps / res["fp64_1"]["wall"] / 1e9,
+        "time_to_solution_ratio_fp64_shot_by_shot_over_fp32_batch": t64_path / res["fp32"]["wall"],
+        "fp32_rel_l2_state_vs_fp64": rel(res["fp32"]["uN"], res["fp64"]["uN"]),
+        "fp32_rel_l2_traces_vs_fp64": rel(res["fp32"]["traces"], res["fp64"]["traces"]),
+    }
+
+
+def bench_2d(nx=64, order=4, steps=1000):
+    """2-D acoustic Q4, nx^2 unit square, Dirichlet, Dirac at the centre (a
+    vertex), Ricker f0 = 10, dt = 0.5 CFL, 16 receivers on the circle of
+    radius 0.3; FP64 and FP32, graph-captured step loop against one
+    launch sequence per step."""
+    import montjoie_amd as M
+    m = M.regular_mesh(2, nx, order)
+    ang = 2 * np.pi * np.arange(16) / 16
+    rec = np.stack([0.5 + 0.3 * np.cos(ang), 0.5 + 0.3 * np.sin(ang)], axis=1)
+    P = M.WaveProblem(m, 1.0, 1.0, precision="fp64")
+    dt = 0.5 * 2.0 / np.sqrt(P.lambda_max(60, 1))
+    P.close()
+    out = {"workload": f"2-D acoustic Q{order} {nx}^2, {steps} steps", "n_dof": m.N, "dt": dt}
+    for prec in ("fp64", "fp32"):
+        for graph in (True, False):
+            P = M.WaveProblem(m, 1.0, 1.0, precision=prec)
+            P.set_sources([(0.5, 0.5)])
+            P.set_receivers(rec)
+            P.set_pulse(M.ricker_pulse_table(10.0, 0.0, dt, steps + 32))
+            P.leapfrog_init(dt)
+            P.advance(32, traces=False, graph=graph)
+            P.leapfrog_init(dt)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            P.advance(steps, traces=True, graph=graph)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            P.close()
+            key = f"{prec}_{'graph' if graph else 'eager'}"
+            out[key] = {"us_per_step": wall / steps * 1e6,
+                        "gdof_steps_s": m.N * steps / wall / 1e9}
+    return out
+
+
+def cpu_reference(nx=24, order=5, steps=150, dt=1e-3):
     """Montjoie's own solver (oracle/_ref, built from the reference sources)
     on a bounded sample of the same workload family, one core."""
     exe = os.path.join(ROOT, "oracle", "_ref", "mjref_acous.x")
@@ -329,6 +424,8 @@ def main():
                     help="skip the 16-shot batch, the adjoint and the many-problem batch")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the 64-problem uneven batch (prints its JSON)")
+    ap.add_argument("--extra-only", choices=["batch", "elastic", "2d", "shots", "adjoint"],
+                    default=None, help="only one secondary workload (prints its JSON)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {a.gpus}): the solver runs each "
@@ -337,7 +434,18 @@ def main():
         sys.exit(2)
     import montjoie_amd as M
     if a.batch_only:
-        print(json.dumps(bench_batch()))
+        a.extra_only = "batch"
+    if a.extra_only:
+        fn = {"batch": bench_batch, "elastic": bench_elastic, "2d": bench_2d,
+              "adjoint": bench_adjoint}.get(a.extra_only)
+        if fn is None:
+            m, mu, _ = workload(a.nx)
+            lam = M.WaveProblem(m, 1.0, mu, precision="fp64")
+            dt = 0.5 * 2.0 / np.sqrt(lam.lambda_max(60, 1))
+            lam.close()
+            print(json.dumps(bench_shots(m, mu, dt, a.steps, a.warmup)))
+        else:
+            print(json.dumps(fn()))
         return
     t_setup = time.perf_counter()
     m, mu, c = workload(a.nx)
@@ -420,6 +528,10 @@ def main():
         print(f"# adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
         out["batch_64_problems"] = bench_batch()
         print(f"# batch: {out['batch_64_problems']}", file=sys.stderr, flush=True)
+        out["elastic_16_shots"] = bench_elastic()
+        print(f"# elastic: {out['elastic_16_shots']}", file=sys.stderr, flush=True)
+        out["acoustic_2d"] = bench_2d()
+        print(f"# 2d: {out['acoustic_2d']}", file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_reference()
     print(json.dumps(out))
