@@ -103,6 +103,25 @@ int mjw_set_xcd_order(mjw_problem *p, int nxcd);
 /* supported elements-per-block values (up to cap); returns their count */
 int mjw_tile_choices(const mjw_problem *p, int *out, int cap);
 
+/*
+ * Geometry of the 3-D acoustic stiffness kernels.
+ *   MJW_GEOM_STORED: the per-node tensor J w mu DF^{-1} DF^{-T} (6 values per
+ *     node) is computed once by mjw_create and read every step;
+ *   MJW_GEOM_TRILINEAR: for meshes of straight-sided hexahedra (every node the
+ *     image of the GL grid under the trilinear map of the element's 8
+ *     vertices, checked by mjw_create to 1e-11 of the element diameter) the
+ *     kernels rebuild DF of that map at each node from 21 coefficients per
+ *     element -- the transformation Montjoie itself differentiates for
+ *     non-curved elements (ComputeLocalMassMatrix, VarGeometryProblem.cxx:
+ *     671-690) -- and read no per-node geometry.
+ * mjw_create selects TRILINEAR whenever the mesh allows it (3-D, ncomp 1),
+ * STORED otherwise.  Results of the two modes agree to rounding.
+ */
+#define MJW_GEOM_STORED 0
+#define MJW_GEOM_TRILINEAR 1
+int mjw_set_geometry_mode(mjw_problem *p, int mode);
+int mjw_get_geometry_mode(const mjw_problem *p, int *mode, int *trilinear_available);
+
 /* ---------------------------------------------------------------- operators */
 /*
  * Y = beta*Y + alpha*K*U with K = -A (stiffness), all nodes, one shot

@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(HERE, "libmjwave.so")
 
 MJW_OK = 0
 MJW_FP64, MJW_FP32, MJW_FP32_G64 = 0, 1, 2
+MJW_GEOM_STORED, MJW_GEOM_TRILINEAR = 0, 1
+GEOMETRY_MODES = {"stored": MJW_GEOM_STORED, "trilinear": MJW_GEOM_TRILINEAR}
 PRECISIONS = {"fp64": MJW_FP64, "fp32": MJW_FP32, "fp32_g64": MJW_FP32_G64}
 
 # every exported symbol of include/mjwave.h with (restype, argtypes)
@@ -27,6 +29,8 @@ SIGNATURES = {
     "mjw_create": (C.c_int, [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int64, _dp,
                              C.c_int64, _i32p, _dp, _dp, _dp, _u8p, C.c_int, C.c_int]),
     "mjw_destroy": (C.c_int, [_vp]),
+    "mjw_set_geometry_mode": (C.c_int, [_vp, C.c_int]),
+    "mjw_get_geometry_mode": (C.c_int, [_vp, _i32p, _i32p]),
     "mjw_info": (C.c_int, [_vp, _i64p]),
     "mjw_set_tile": (C.c_int, [_vp, C.c_int]),
     "mjw_get_tile": (C.c_int, [_vp, C.POINTER(C.c_int)]),

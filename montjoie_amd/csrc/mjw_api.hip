@@ -43,6 +43,11 @@ struct mjw_problem {
   // device
   int32_t *d_map = nullptr;
   void *d_G = nullptr;
+  // trilinear geometry (3-D scalar problems whose elements are straight-sided)
+  bool tri_ok = false;
+  int geo = kGeoStored;
+  std::vector<double> hcoef;          // E*kCoef, sorted order, FP64
+  void *d_coef = nullptr, *d_glxw = nullptr;
   double *d_M = nullptr, *d_Minv = nullptr, *d_isqM = nullptr;
   void *d_sdiag = nullptr;
   uint8_t *d_dir = nullptr;
@@ -171,6 +176,8 @@ static void release(mjw_problem *p) {
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  dfree(p->d_coef);
+  dfree(p->d_glxw);
   void *ptrs[] = {p->d_map, p->d_G, p->d_M, p->d_Minv, p->d_isqM, p->d_sdiag, p->d_dir,
                   p->d_lam, p->d_mu, p->d_rho, p->d_D, p->d_U[0], p->d_U[1], p->d_tmp,
                   p->d_tmp2, p->d_part, p->d_red, p->d_ctl, p->d_pulse, p->d_src_dof,
@@ -189,6 +196,25 @@ static void release(mjw_problem *p) {
 extern "C" int mjw_destroy(mjw_problem *p) {
   release(p);
   return MJW_OK;
+}
+
+// coefficient table and GL points / weights in the geometry precision
+static hipError_t upload_trilinear(mjw_problem *p, const double *xg, const double *wg) {
+  const size_t nc = p->hcoef.size();
+  hipError_t e = hipSuccess;
+  if (p->geom == kG64) {
+    e = hipMemcpy(p->d_coef, p->hcoef.data(), nc * 8, hipMemcpyHostToDevice);
+  } else {
+    std::vector<float> f(p->hcoef.begin(), p->hcoef.end());
+    e = hipMemcpy(p->d_coef, f.data(), nc * 4, hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess || !xg) return e;
+  double gw[2 * kMaxN];
+  for (int i = 0; i < p->n; ++i) { gw[i] = xg[i]; gw[p->n + i] = wg[i]; }
+  if (p->geom == kG64) return hipMemcpy(p->d_glxw, gw, 2 * p->n * 8, hipMemcpyHostToDevice);
+  float gf[2 * kMaxN];
+  for (int i = 0; i < 2 * p->n; ++i) gf[i] = (float)gw[i];
+  return hipMemcpy(p->d_glxw, gf, 2 * p->n * 4, hipMemcpyHostToDevice);
 }
 
 extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int ncomp,
@@ -374,6 +400,19 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   d_xyz = d_Dg = d_w = d_Mloc = nullptr;
   d_bad = nullptr;
   if (bad) return fail(MJW_ERR_GEOMETRY, "non-positive Jacobian: degenerate or inverted element");
+  // straight-sided hexahedra: per-element trilinear coefficients (default)
+  if (dim == 3 && ncomp == 1 &&
+      trilinear_coefficients(order, n_elem, elem_nodes, node_xyz, p->perm.data(), ms.data(),
+                             1e-11, p->hcoef)) {
+    const size_t gb = p->geom == kG64 ? 8 : 4;
+    CK(dalloc(p, &p->d_coef, p->hcoef.size() * gb));
+    CK(dalloc(p, &p->d_glxw, 2 * p->n * gb));
+    CK(upload_trilinear(p, xg, wg));
+    p->tri_ok = true;
+    p->geo = kGeoTrilinear;
+  } else {
+    p->hcoef.clear();
+  }
   // Dirichlet rows of the mass are 1 (TimeAcoustic.cxx:936-937)
   std::vector<double> minv(n_nodes), isq(n_nodes);
   for (int64_t g = 0; g < n_nodes; ++g) {
@@ -440,6 +479,29 @@ extern "C" int mjw_set_xcd_order(mjw_problem *p, int nxcd) {
   return MJW_OK;
 }
 
+extern "C" int mjw_set_geometry_mode(mjw_problem *p, int mode) {
+  if (!p) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL problem");
+  if (mode != MJW_GEOM_STORED && mode != MJW_GEOM_TRILINEAR)
+    FAIL(p, MJW_ERR_ARG, "mjw_set_geometry_mode: bad mode");
+  if (mode == MJW_GEOM_TRILINEAR && !p->tri_ok)
+    FAIL(p, MJW_ERR_UNSUPPORTED,
+         "trilinear geometry needs a 3-D acoustic mesh of straight-sided hexahedra");
+  const int g = mode == MJW_GEOM_TRILINEAR ? kGeoTrilinear : kGeoStored;
+  if (g != p->geo) {
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+    if (p->bexec) { (void)hipGraphExecDestroy(p->bexec); p->bexec = nullptr; }
+  }
+  p->geo = g;
+  return MJW_OK;
+}
+
+extern "C" int mjw_get_geometry_mode(const mjw_problem *p, int *mode, int *trilinear_available) {
+  if (!p || !mode) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
+  *mode = p->geo == kGeoTrilinear ? MJW_GEOM_TRILINEAR : MJW_GEOM_STORED;
+  if (trilinear_available) *trilinear_available = p->tri_ok ? 1 : 0;
+  return MJW_OK;
+}
+
 extern "C" int mjw_get_tile(const mjw_problem *p, int *epb) {
   if (!p || !epb) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
   *epb = p->epb;
@@ -454,6 +516,7 @@ static StiffArgs stiff_args(mjw_problem *p, int mode, const void *U, void *Y, in
   a.map = p->d_map; a.G = p->d_G; a.Dm = p->d_D; a.lam = p->d_lam; a.mu = p->d_mu;
   a.U = U; a.Y = Y; a.sdiag = p->d_sdiag; a.N = p->N; a.nshots = nshots;
   a.alpha = 1.0; a.beta = 0.0; a.nxcd = p->nxcd; a.grad = nullptr;
+  a.geo = p->geo; a.coef = p->d_coef; a.glxw = p->d_glxw;
   return a;
 }
 
@@ -1117,6 +1180,10 @@ extern "C" int mjw_set_mu(mjw_problem *p, const double *mu) {
                                p->stream);
   if (!rc) rc = (int)hipMemcpyAsync(p->d_mu, ms.data(), p->E * 8, hipMemcpyHostToDevice, p->stream);
   if (!rc) rc = (int)hipStreamSynchronize(p->stream);
+  if (!rc && p->tri_ok) {
+    for (int64_t es = 0; es < p->E; ++es) p->hcoef[(size_t)es * kCoef + 21] = ms[es];
+    rc = (int)upload_trilinear(p, nullptr, nullptr);
+  }
   dfree(d_sc);
   if (rc) FAIL(p, MJW_ERR_HIP, "mjw_set_mu: rescale failed");
   p->adj_ready = false;
@@ -1416,6 +1483,7 @@ extern "C" int mjw_batch_create(mjw_batch **out, int nprob, mjw_problem *const *
     d.map = p->d_map; d.G = p->d_G; d.Dm = p->d_D; d.sdiag = p->d_sdiag;
     d.buf[0] = p->d_U[0]; d.buf[1] = p->d_U[1];
     d.E = p->E; d.N = p->N; d.order = p->order;
+    d.geo = p->geo; d.coef = p->d_coef; d.glxw = p->d_glxw;
   }
   int rc = (int)hipMalloc(&b->d_probs, nprob * sizeof(BatchProb));
   if (!rc) rc = (int)hipMemcpy(b->d_probs, hp.data(), nprob * sizeof(BatchProb), hipMemcpyHostToDevice);

@@ -291,6 +291,63 @@ int color_elements(int64_t E, int nd, int64_t N, const int32_t *elem_nodes,
 }
 
 // ---------------------------------------------------------------------------
+// Straight-sided hexahedra: for a non-curved element Montjoie evaluates DF of
+// the transformation built from the element's vertices at the quadrature
+// points (ComputeLocalMassMatrix, Harmonic/VarGeometryProblem.cxx:671-690,
+// GetVerticesElement + DFjElem), i.e. of the trilinear map of the 8 vertices.
+// When every node is that map's image of the GL grid the kernels rebuild DF
+// from 21 coefficients per element instead of reading 6 stored values per
+// node.
+// ---------------------------------------------------------------------------
+bool trilinear_coefficients(int order, int64_t E, const int32_t *elem_nodes, const double *xyz,
+                            const int64_t *perm, const double *mu_sorted, double tol,
+                            std::vector<double> &coef) {
+  const int n = order + 1, nd = n * n * n, r = order;
+  double xg[kMaxN], wg[kMaxN];
+  gl_rule(order, xg, wg);
+  coef.assign((size_t)E * kCoef, 0.0);
+  for (int64_t es = 0; es < E; ++es) {
+    const int32_t *en = elem_nodes + perm[es] * nd;
+    auto X = [&](int i, int j, int k, int d) {
+      return xyz[(int64_t)en[i + n * j + n * n * k] * 3 + d];
+    };
+    double a[8][3];
+    double diam2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      const double x000 = X(0, 0, 0, d), x100 = X(r, 0, 0, d), x010 = X(0, r, 0, d),
+                   x110 = X(r, r, 0, d), x001 = X(0, 0, r, d), x101 = X(r, 0, r, d),
+                   x011 = X(0, r, r, d), x111 = X(r, r, r, d);
+      a[0][d] = x000;
+      a[1][d] = x100 - x000;
+      a[2][d] = x010 - x000;
+      a[3][d] = x001 - x000;
+      a[4][d] = x110 - x100 - x010 + x000;
+      a[5][d] = x011 - x010 - x001 + x000;
+      a[6][d] = x101 - x100 - x001 + x000;
+      a[7][d] = x111 - x110 - x101 - x011 + x100 + x010 + x001 - x000;
+      diam2 += (x111 - x000) * (x111 - x000);
+    }
+    const double lim = tol * std::sqrt(diam2);
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          const double u = xg[i], v = xg[j], w = xg[k];
+          for (int d = 0; d < 3; ++d) {
+            const double x = a[0][d] + a[1][d] * u + a[2][d] * v + a[3][d] * w +
+                             a[4][d] * u * v + a[5][d] * v * w + a[6][d] * u * w +
+                             a[7][d] * u * v * w;
+            if (!(std::fabs(x - X(i, j, k, d)) <= lim)) return false;
+          }
+        }
+    double *c = &coef[(size_t)es * kCoef];
+    for (int m = 1; m < 8; ++m)
+      for (int d = 0; d < 3; ++d) c[(m - 1) * 3 + d] = a[m][d];
+    c[21] = mu_sorted[es];
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------
 // Point location in an isoparametric element (replaces FjInverseProblem::Solve,
 // FiniteElement/PointsReference.cxx:277-360, and the projection of points found
 // slightly outside, DefineSourceElliptic.cxx:2539-2543).

@@ -43,6 +43,19 @@ enum StateType { kF64 = 0, kF32 = 1 };
 enum GeomType { kG64 = 0, kG32 = 1 };
 enum Mode { kApply = 0, kLeap = 1 };
 
+// Trilinear (vertex) geometry of a hexahedron: x(xi) = a0 + a1 xi1 + a2 xi2 +
+// a3 xi3 + a4 xi1 xi2 + a5 xi2 xi3 + a6 xi1 xi3 + a7 xi1 xi2 xi3 on [0,1]^3.
+// Per element (colour-sorted order) kCoef values: a1..a7 (3 components each,
+// slots 0..20), mu (slot 21), 0, 0.
+constexpr int kCoef = 24;
+// Fills coef (E*kCoef, sorted order via perm) and returns true when every
+// element's nodes are the trilinear image of the GL grid (within tol times
+// the element diameter); false otherwise (coef then undefined).
+bool trilinear_coefficients(int order, int64_t E, const int32_t *elem_nodes, const double *xyz,
+                            const int64_t *perm, const double *mu_sorted, double tol,
+                            std::vector<double> &coef);
+enum GeoMode { kGeoStored = 0, kGeoTrilinear = 1 };
+
 struct StiffArgs {
   int dim, order, ncomp, epb;         // geometry, elements per block
   int state;                          // StateType
@@ -62,6 +75,9 @@ struct StiffArgs {
   int nshots;
   int nxcd;                           // XCD-aware block order (1 = off)
   double *grad;                       // adjoint step: per-element gradient (sorted), or null
+  int geo;                            // GeoMode (trilinear: 3-D scalar only)
+  const void *coef;                   // trilinear: E*kCoef values, geometry precision
+  const void *glxw;                   // trilinear: GL points then weights (n each), geometry precision
 };
 // returns 0 or a HIP error code; `stream` is a hipStream_t
 int launch_stiffness(const StiffArgs &a, void *stream);
@@ -136,28 +152,14 @@ int reduction_scratch_doubles(int S);
 int launch_scale_planes(int g32, void *G, int64_t nplanes, int64_t E, int P, const double *sc,
                         void *stream);
 
-// adjoint: per-element gradient accumulation grad[e] += c * u_e^T A0_e lam_e
-struct GradArgs {
-  int dim, order, ncomp, state, geom;
-  int64_t E, N;
-  const int32_t *map;
-  const void *G;
-  const double *lam, *mu;
-  const void *U;       // forward state
-  const void *L;       // adjoint state
-  double *grad;        // E (sorted order)
-  double coef;
-  int nshots;
-};
-int launch_gradient(const GradArgs &a, void *stream);
-
 // ---------------------------------------------------------------- batching
 struct BatchProb {          // device descriptor of one problem of a batch
   const int32_t *map;
   const void *G, *Dm, *sdiag;
   void *buf[2];             // the problem's two state buffers
+  const void *coef, *glxw;  // trilinear geometry (geo == kGeoTrilinear)
   int64_t E, N;
-  int order, pad;
+  int order, geo;
 };
 struct BatchItem {          // one block of work: elements [eb, min(eb+EPB, e1))
   int32_t prob, pad;

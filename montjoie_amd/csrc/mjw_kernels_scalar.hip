@@ -35,15 +35,53 @@ struct Mix {  // geometry product in the geometry's precision
   using A = TG;
 };
 
+// Per-thread constants of one (i, j) column of a trilinear hexahedron: the
+// columns of DF at (x_i, x_j, x_k) are c0 = p0 + x_k q0, c1 = p1 + x_k q1 and
+// c2 (independent of x_k); f = mu w_i w_j.
+template <typename A>
+struct TriCol {
+  A p0[3], q0[3], p1[3], q1[3], c2[3], f;
+};
+
+// G = mu w J DF^{-1} DF^{-T} at layer k: with r_b the rows of the adjugate
+// J DF^{-1} (r0 = c1 x c2, r1 = c2 x c0, r2 = c0 x c1), G_ab = mu w r_a.r_b / J.
+template <typename A>
+__device__ __forceinline__ void tri_geometry(const TriCol<A> &c, A zeta, A wk, A g[6]) {
+  A c0[3], c1[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    c0[d] = c.p0[d] + zeta * c.q0[d];
+    c1[d] = c.p1[d] + zeta * c.q1[d];
+  }
+  const A r0[3] = {c1[1] * c.c2[2] - c1[2] * c.c2[1], c1[2] * c.c2[0] - c1[0] * c.c2[2],
+                   c1[0] * c.c2[1] - c1[1] * c.c2[0]};
+  const A r1[3] = {c.c2[1] * c0[2] - c.c2[2] * c0[1], c.c2[2] * c0[0] - c.c2[0] * c0[2],
+                   c.c2[0] * c0[1] - c.c2[1] * c0[0]};
+  const A r2[3] = {c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2],
+                   c0[0] * c1[1] - c0[1] * c1[0]};
+  const A J = c0[0] * r0[0] + c0[1] * r0[1] + c0[2] * r0[2];
+  const A f = c.f * wk / J;
+  g[0] = f * (r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]);
+  g[1] = f * (r0[0] * r1[0] + r0[1] * r1[1] + r0[2] * r1[2]);
+  g[2] = f * (r0[0] * r2[0] + r0[1] * r2[1] + r0[2] * r2[2]);
+  g[3] = f * (r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
+  g[4] = f * (r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2]);
+  g[5] = f * (r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+}
+
 // Body of the 3-D scalar stiffness pass for the elements [eb, min(eb+EPB, e1))
 // of one colour; threads t >= EPB*n^2 of the block idle (they only meet the
 // barriers).  lu, lv: 2*EPB*n^3 values of LDS; gpart: EPB*n^2 doubles (GRAD).
-template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD>
+// TRI: the per-node tensor is rebuilt from the element's trilinear map
+// (coef: kCoef values per element, staged through lc = EPB*kCoef values of
+// LDS; glxw: GL points then weights) instead of read from G.
+template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD, bool TRI>
 __device__ __forceinline__ void scalar3d_body(
-    T *lu, T *lv, double *gpart, const int32_t *__restrict__ map, const TG *__restrict__ G,
-    const T *__restrict__ Dm, int64_t E, int64_t eb, int64_t e1, const T *__restrict__ U,
-    T *__restrict__ Y, const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots,
-    double *__restrict__ grad, const double *__restrict__ mu_e) {
+    T *lu, T *lv, double *gpart, TG *lc, const int32_t *__restrict__ map,
+    const TG *__restrict__ G, const T *__restrict__ Dm, int64_t E, int64_t eb, int64_t e1,
+    const T *__restrict__ U, T *__restrict__ Y, const T *__restrict__ sdiag, T alpha, T beta,
+    int64_t N_nodes, int nshots, double *__restrict__ grad, const double *__restrict__ mu_e,
+    const TG *__restrict__ coef, const TG *__restrict__ glxw) {
   // GRAD (adjoint step, nshots == 2: shot 0 = forward iterate U^j rebuilt
   // backwards, shot 1 = adjoint nu_j): grad[e] -= (A_e nu_e . U_e) / mu_e
   constexpr int P = N * N, NN = N * N * N;
@@ -62,14 +100,40 @@ __device__ __forceinline__ void scalar3d_body(
   const uint32_t g_ps = (uint32_t)(plane * sizeof(TG));
   int32_t gid[N];
   A g6[HOIST ? N : 1][6];  // geometry of the column, kept across shots
+  TriCol<A> tc;
+  if constexpr (TRI) {
+    for (int q = t; q < EPB * kCoef; q += blockDim.x)
+      lc[q] = eb + q / kCoef < e1 ? coef[eb * kCoef + q] : TG(0);
+    __syncthreads();
+    if (act) {
+      const TG *c = lc + el * kCoef;
+      const A xi = glxw[i], eta = glxw[j];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const A a1 = c[d], a2 = c[3 + d], a3 = c[6 + d], a4 = c[9 + d], a5 = c[12 + d],
+                a6 = c[15 + d], a7 = c[18 + d];
+        tc.p0[d] = a1 + a4 * eta;
+        tc.q0[d] = a6 + a7 * eta;
+        tc.p1[d] = a2 + a4 * xi;
+        tc.q1[d] = a5 + a7 * xi;
+        tc.c2[d] = a3 + a5 * eta + a6 * xi + a7 * (xi * eta);
+      }
+      tc.f = (A)c[21] * ((A)glxw[N + i] * (A)glxw[N + j]);
+    }
+  }
   if (act) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       gid[k] = bld<int32_t>(rmap, lo * 4u, k * map_ps);
       if constexpr (HOIST) {
-        const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
+        if constexpr (TRI) {
+          tri_geometry<A>(tc, (A)glxw[k], (A)glxw[N + k], g6[k]);
+        } else {
+          const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) g6[k][c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
+          for (int c = 0; c < 6; ++c)
+            g6[k][c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
+        }
       }
     }
   }
@@ -109,6 +173,8 @@ __device__ __forceinline__ void scalar3d_body(
         if constexpr (HOIST) {
 #pragma unroll
           for (int c = 0; c < 6; ++c) gk[c] = g6[HOIST ? k : 0][c];
+        } else if constexpr (TRI) {
+          tri_geometry<A>(tc, (A)glxw[k], (A)glxw[N + k], gk);
         } else {
           const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
 #pragma unroll
@@ -177,18 +243,21 @@ __device__ __forceinline__ void scalar3d_body(
   }
 }
 
-template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD = false>
+template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD, bool TRI>
 __global__ __launch_bounds__(EPB *N *N) void k_scalar3d(
     const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
     int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
     const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd,
-    double *__restrict__ grad = nullptr, const double *__restrict__ mu_e = nullptr) {
+    double *__restrict__ grad, const double *__restrict__ mu_e, const TG *__restrict__ coef,
+    const TG *__restrict__ glxw) {
   __shared__ T lu[EPB * N * N * N];
   __shared__ T lv[EPB * N * N * N];
   __shared__ double gpart[GRAD ? EPB * N * N : 1];
+  __shared__ TG lc[TRI ? EPB * kCoef : 1];
   const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * EPB;
-  scalar3d_body<N, EPB, T, TG, MODE, HOIST, GRAD>(lu, lv, gpart, map, G, Dm, E, eb, e1, U, Y,
-                                                  sdiag, alpha, beta, N_nodes, nshots, grad, mu_e);
+  scalar3d_body<N, EPB, T, TG, MODE, HOIST, GRAD, TRI>(lu, lv, gpart, lc, map, G, Dm, E, eb, e1,
+                                                       U, Y, sdiag, alpha, beta, N_nodes, nshots,
+                                                       grad, mu_e, coef, glxw);
 }
 
 template <int N, int EPB, typename T, typename TG, int MODE, bool GRAD = false>
@@ -312,7 +381,9 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch3d(const BatchProb *__re
                                                            const BatchItem *__restrict__ items,
                                                            int parity) {
   constexpr int kLds = 2 * 4 * 512;  // max over orders of 2*EPB*n^3
+  constexpr int kLdsC = (kBatchThreads / 4) * kCoef;  // max EPB (order 1) * kCoef
   __shared__ T lds[kLds];
+  __shared__ TG ldc[kLdsC];
   const BatchItem it = items[blockIdx.x];
   const BatchProb &p = probs[it.prob];
   const T *U = static_cast<const T *>(p.buf[parity ^ 1]);
@@ -320,12 +391,19 @@ __global__ __launch_bounds__(kBatchThreads) void k_batch3d(const BatchProb *__re
   const TG *G = static_cast<const TG *>(p.G);
   const T *D = static_cast<const T *>(p.Dm);
   const T *s = static_cast<const T *>(p.sdiag);
-#define MJW_BATCH_CASE(ORD)                                                                     \
-  case ORD: {                                                                                  \
-    constexpr int N = ORD + 1, EPB = BatchEpb<N>::value;                                       \
-    scalar3d_body<N, EPB, T, TG, kLeap, false, false>(lds, lds + EPB * N * N * N, nullptr,     \
-                                                      p.map, G, D, p.E, it.eb, it.e1, U, Y, s,  \
-                                                      T(0), T(0), p.N, 1, nullptr, nullptr);   \
+  const TG *cf = static_cast<const TG *>(p.coef);
+  const TG *gw = static_cast<const TG *>(p.glxw);
+#define MJW_BATCH_CASE(ORD)                                                                    \
+  case ORD: {                                                                                 \
+    constexpr int N = ORD + 1, EPB = BatchEpb<N>::value;                                      \
+    if (p.geo == kGeoTrilinear)                                                               \
+      scalar3d_body<N, EPB, T, TG, kLeap, false, false, true>(                                \
+          lds, lds + EPB * N * N * N, nullptr, ldc, p.map, G, D, p.E, it.eb, it.e1, U, Y, s,  \
+          T(0), T(0), p.N, 1, nullptr, nullptr, cf, gw);                                      \
+    else                                                                                      \
+      scalar3d_body<N, EPB, T, TG, kLeap, false, false, false>(                               \
+          lds, lds + EPB * N * N * N, nullptr, ldc, p.map, G, D, p.E, it.eb, it.e1, U, Y, s,  \
+          T(0), T(0), p.N, 1, nullptr, nullptr, cf, gw);                                      \
   } break;
   switch (p.order) {
     MJW_BATCH_CASE(1)
@@ -362,6 +440,30 @@ int launch_batch3d(int state, int geom, const BatchProb *probs, const BatchItem 
 }
 
 // --------------------------------------------------------------- dispatch
+template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD, bool TRI>
+static void launch3(const StiffArgs &a, hipStream_t st, unsigned blocks) {
+  hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, MODE, HOIST, GRAD, TRI>), dim3(blocks),
+                     dim3(EPB * N * N), 0, st, a.map, static_cast<const TG *>(a.G),
+                     static_cast<const T *>(a.Dm), a.E, a.e0, a.e1, static_cast<const T *>(a.U),
+                     static_cast<T *>(a.Y), static_cast<const T *>(a.sdiag), (T)a.alpha,
+                     (T)a.beta, a.N, a.nshots, a.nxcd, a.grad, a.mu,
+                     static_cast<const TG *>(a.coef), static_cast<const TG *>(a.glxw));
+}
+
+// stored geometry: several shots keep the column's tensor in registers
+// (HOIST); trilinear geometry recomputes it (no stream to save)
+template <int N, int EPB, typename T, typename TG, bool TRI>
+static void run3(const StiffArgs &a, hipStream_t st, unsigned blocks) {
+  if (a.grad)
+    launch3<N, EPB, T, TG, kLeap, !TRI, true, TRI>(a, st, blocks);
+  else if (a.mode == kLeap && a.nshots > 1 && !TRI)
+    launch3<N, EPB, T, TG, kLeap, true, false, false>(a, st, blocks);
+  else if (a.mode == kLeap)
+    launch3<N, EPB, T, TG, kLeap, false, false, TRI>(a, st, blocks);
+  else
+    launch3<N, EPB, T, TG, kApply, false, false, TRI>(a, st, blocks);
+}
+
 template <int N, int EPB, typename T, typename TG>
 static int run_scalar(const StiffArgs &a, hipStream_t st) {
   const int64_t ne = a.e1 - a.e0;
@@ -374,37 +476,24 @@ static int run_scalar(const StiffArgs &a, hipStream_t st) {
   const T *s = static_cast<const T *>(a.sdiag);
   const TG *G = static_cast<const TG *>(a.G);
   const T al = (T)a.alpha, be = (T)a.beta;
-  if (a.grad) {
-    if (a.mode != kLeap || a.nshots != 2) return -1001;
-    if (a.dim == 3)
-      hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kLeap, true, true>), dim3(blocks),
-                         dim3(threads), 0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N,
-                         a.nshots, a.nxcd, a.grad, a.mu);
-    else
-      hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kLeap, true>), dim3(blocks), dim3(threads), 0,
-                         st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd,
-                         a.grad, a.mu);
-    return (int)hipGetLastError();
-  }
+  if (a.grad && (a.mode != kLeap || a.nshots != 2)) return -1001;
   if (a.dim == 3) {
-    if (a.mode == kLeap) {
-      if (a.nshots > 1)
-        hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kLeap, true>), dim3(blocks), dim3(threads),
-                           0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
-      else
-        hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kLeap, false>), dim3(blocks), dim3(threads),
-                           0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
+    if (a.geo == kGeoTrilinear) {
+      if (!a.coef || !a.glxw) return -1002;
+      run3<N, EPB, T, TG, true>(a, st, blocks);
     } else {
-      hipLaunchKernelGGL((k_scalar3d<N, EPB, T, TG, kApply, false>), dim3(blocks), dim3(threads),
-                         0, st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
+      run3<N, EPB, T, TG, false>(a, st, blocks);
     }
+  } else if (a.grad) {
+    hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kLeap, true>), dim3(blocks), dim3(threads), 0,
+                       st, a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd,
+                       a.grad, a.mu);
+  } else if (a.mode == kLeap) {
+    hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kLeap>), dim3(blocks), dim3(threads), 0, st,
+                       a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
   } else {
-    if (a.mode == kLeap)
-      hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kLeap>), dim3(blocks), dim3(threads), 0, st,
-                         a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
-    else
-      hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kApply>), dim3(blocks), dim3(threads), 0, st,
-                         a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
+    hipLaunchKernelGGL((k_scalar2d<N, EPB, T, TG, kApply>), dim3(blocks), dim3(threads), 0, st,
+                       a.map, G, D, a.E, a.e0, a.e1, U, Y, s, al, be, a.N, a.nshots, a.nxcd);
   }
   return (int)hipGetLastError();
 }

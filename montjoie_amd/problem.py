@@ -130,6 +130,33 @@ class WaveProblem:
              "device_bytes"]
         return dict(zip(k, list(a)))
 
+    def set_mu(self, mu):
+        """Replace the per-element coefficient mu in place (mjw_set_mu)."""
+        mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, np.float64), (self.mesh.E,)))
+        self._chk(_lib.lib().mjw_set_mu(self._h, dptr(mu)), "set_mu")
+
+    def set_geometry(self, mode: str):
+        """'trilinear' (per-element vertex map, default when the mesh allows
+        it) or 'stored' (per-node tensor read every step)."""
+        self._chk(_lib.lib().mjw_set_geometry_mode(self._h, _lib.GEOMETRY_MODES[mode]),
+                  "set_geometry")
+
+    @property
+    def geometry(self) -> str:
+        m = C.c_int32()
+        a = C.c_int32()
+        self._chk(_lib.lib().mjw_get_geometry_mode(self._h, C.byref(m), C.byref(a)),
+                  "get_geometry")
+        return "trilinear" if m.value == _lib.MJW_GEOM_TRILINEAR else "stored"
+
+    @property
+    def trilinear_available(self) -> bool:
+        m = C.c_int32()
+        a = C.c_int32()
+        self._chk(_lib.lib().mjw_get_geometry_mode(self._h, C.byref(m), C.byref(a)),
+                  "get_geometry")
+        return bool(a.value)
+
     def set_tile(self, epb: int):
         self._chk(_lib.lib().mjw_set_tile(self._h, int(epb)), "set_tile")
 
