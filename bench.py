@@ -62,8 +62,28 @@ def receivers(n=16, seed=77):
 
 
 def algorithmic_bytes_per_step(E, nd, N, state_bytes, geom_bytes, ngeom=6):
-    """geometry + element map + (read U, read/write U^{n-1 -> n+1}, read dt^2/M)."""
+    """Stored per-node geometry + element map + (read U, read/write
+    U^{n-1 -> n+1}, read dt^2/M): the per-step byte count of the stored-
+    geometry algorithm, no cache reuse credited."""
     return E * nd * (ngeom * geom_bytes + 4) + N * 4 * state_bytes
+
+
+def trilinear_bytes_per_step(E, nd, N, state_bytes, geom_bytes, ncoef=24):
+    """Same with the per-element trilinear coefficients in place of the
+    per-node geometry: what the trilinear kernels have to move."""
+    return E * (ncoef * geom_bytes + nd * 4) + N * 4 * state_bytes
+
+
+def measured_traffic(name="stiffness_q5_48cube_counters.md"):
+    """HBM bytes per colour pass of the headline kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (line `traffic_bytes_per_launch:`)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def flops_per_step(E, n):
@@ -92,8 +112,9 @@ def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10):
     uN, _ = P.state()
     prof = P.profile(profile_steps)
     info = P.info()
+    geo = P.geometry
     P.close()
-    return dict(wall=wall, dev_ms_step=dev_ms, traces=tr, uN=uN, prof=prof, info=info)
+    return dict(wall=wall, dev_ms_step=dev_ms, traces=tr, uN=uN, prof=prof, info=info, geometry=geo)
 
 
 def shot_sources(nshots, seed=2027):
@@ -485,6 +506,9 @@ def main():
     bytes_step = algorithmic_bytes_per_step(m.E, nd, m.N, 4, 4)
     launch_bytes = bytes_step / ncol
     achieved = launch_bytes / (head["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9
+    tri = head["geometry"] == "trilinear"
+    kernel_bytes = (trilinear_bytes_per_step(m.E, nd, m.N, 4, 4) if tri else bytes_step) / ncol
+    traffic = measured_traffic() if (a.nx == 48 and tri) else None
     out = {
         "metric": "GDoF-steps/s, Q5 hex acoustic, FP32 vs FP64 at rel-L2 err; gradient/forward cost",
         "value": gdof(head),
@@ -512,8 +536,15 @@ def main():
         "fp32_rel_l2_traces_vs_fp64": head["err_traces"],
         "fp32_over_fp64_speedup": gdof(head) / gdof(ref),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_scalar3d<6,EPB,float,float,kLeap> (one colour pass)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "geometry": head["geometry"],
+                     "kernel_bytes_per_launch": kernel_bytes,
+                     "kernel_bytes_frac": kernel_bytes / (head["prof"]["stiffness_launch_ms"] * 1e-3)
+                     / 1e9 / HBM_PEAK_GBS,
+                     "traffic_GBs": None if traffic is None else
+                     traffic / (head["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9,
+                     "kernel": "mjw::k_scalar3d<6, 7, float, float, 1, false, false, %s> "
+                               "(one colour pass)" % ("true" if tri else "false"),
                      "algorithmic_bytes_per_launch": launch_bytes,
                      "launch_ms": head["prof"]["stiffness_launch_ms"],
                      "fp64_launch_ms": ref["prof"]["stiffness_launch_ms"],

@@ -281,10 +281,12 @@ int mjw_batch_destroy(mjw_batch *b);
  * measured with HIP events on the problem's stream; [0]=per step,
  * [1]=stiffness kernels per step.  For benchmarks. */
 int mjw_last_timing(const mjw_problem *p, double *ms2);
-/* Advance nsteps eager steps with a HIP event pair around every launch on the
- * problem's stream: out[0] = mean stiffness-kernel launch (one colour) ms,
- * out[1] = mean tail-kernel ms, out[2] = mean step ms, out[3] = launches per
- * step.  Advances the state like mjw_leapfrog_advance (no traces). */
+/* Advance nsteps eager steps with HIP events on the problem's stream before
+ * and after each step's colour passes (launched back to back) and after its
+ * tail kernel: out[0] = mean stiffness time per colour pass ms (step's
+ * passes / ncolors), out[1] = mean tail-kernel ms, out[2] = mean step ms,
+ * out[3] = launches per step.  Advances the state like mjw_leapfrog_advance
+ * (no traces). */
 int mjw_profile(mjw_problem *p, int nsteps, double *out4);
 
 #ifdef __cplusplus

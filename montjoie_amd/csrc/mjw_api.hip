@@ -995,7 +995,9 @@ extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
   if (!p->inited) FAIL(p, MJW_ERR_STATE, "not initialised");
   HIPCHK(p, hipSetDevice(p->device));
   hipStream_t st = p->stream;
-  const int nev = nsteps * (p->ncolor + 1) + 1;
+  // events only around each step's colour passes and its tail: the colour
+  // passes of one step run back to back, as in the time loop
+  const int nev = 2 * nsteps + 1;
   std::vector<hipEvent_t> ev(nev);
   for (auto &e : ev) HIPCHK(p, hipEventCreate(&e));
   int64_t ctl[2] = {p->step, p->step};
@@ -1009,8 +1011,8 @@ extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
       a.e0 = p->coff[c];
       a.e1 = p->coff[c + 1];
       KCHK(p, launch_stiffness(a, st));
-      HIPCHK(p, hipEventRecord(ev[k++], st));
     }
+    HIPCHK(p, hipEventRecord(ev[k++], st));
     TailArgs t{};
     t.state = p->state; t.ctl = p->d_ctl; t.pulse = p->d_pulse; t.npulse = p->npulse;
     t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot;
@@ -1022,17 +1024,12 @@ extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
   }
   HIPCHK(p, hipEventSynchronize(ev[k - 1]));
   double stiff = 0, tail = 0;
-  k = 1;
   for (int s = 0; s < nsteps; ++s) {
-    for (int c = 0; c < p->ncolor; ++c, ++k) {
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, ev[k - 1], ev[k]);
-      stiff += ms;
-    }
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, ev[k - 1], ev[k]);
+    (void)hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
+    stiff += ms;
+    (void)hipEventElapsedTime(&ms, ev[2 * s + 1], ev[2 * s + 2]);
     tail += ms;
-    ++k;
   }
   float tot = 0;
   (void)hipEventElapsedTime(&tot, ev[0], ev[nev - 1]);
