@@ -130,8 +130,20 @@ int mjw_get_geometry_mode(const mjw_problem *p, int *mode, int *trilinear_availa
  * ProdMatVectH1.cxx:1228-1276).  Vectors of length n_nodes*ncomp, FP64.
  */
 int mjw_apply_kh(mjw_problem *p, double alpha, const double *U, double beta, double *Y);
-/* same on device vectors in the problem's state precision (double or float) */
+/* same on device vectors in the problem's state precision (double or float),
+ * caller's node order, all nshots shots (nshots*ncomp*n_nodes values) */
 int mjw_apply_kh_dev(mjw_problem *p, double alpha, const void *U, double beta, void *Y);
+/*
+ * The device keeps its vectors in a solver node order: nodes grouped by the
+ * set of element colours that touch them, then by first touch in the
+ * kernels' element order, so that each colour pass of the stiffness kernel
+ * reads whole groups and a cache line carries only nodes of that pass.
+ * Every host-vector entry point converts to and from the caller's order;
+ * solver_index[g] (n_nodes values) = position of the caller's node g in the
+ * device vectors returned by mjw_state_dev.  No counterpart in the reference
+ * (Montjoie's vectors follow its own numbering, Mesh/NumberMap.cxx:661-777).
+ */
+int mjw_node_order(const mjw_problem *p, int32_t *solver_index);
 
 /* Y = beta*Y + alpha*D*U, D = lumped mass with Dirichlet rows 1
  * (ApplyOperatorDh, ContinuousUnsteadyMassMatrix.cxx:1329-1346). */
@@ -194,7 +206,8 @@ int mjw_record(mjw_problem *p, double *traces);
 /* current pair (U^n, U^{n+1}) as FP64 host vectors [nshots][n_nodes*ncomp];
  * either may be NULL (GetIterate / GetIterate(k)). */
 int mjw_get_state(mjw_problem *p, double *Un, double *Un1);
-/* device pointers of U^n and U^{n+1} (state precision, shots consecutive) */
+/* device pointers of U^n and U^{n+1} (state precision, shots consecutive,
+ * solver node order: see mjw_node_order) */
 int mjw_state_dev(mjw_problem *p, void **Un, void **Un1);
 /* current step index n */
 int mjw_get_step(const mjw_problem *p, int64_t *n);

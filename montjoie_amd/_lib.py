@@ -31,6 +31,7 @@ SIGNATURES = {
     "mjw_destroy": (C.c_int, [_vp]),
     "mjw_set_geometry_mode": (C.c_int, [_vp, C.c_int]),
     "mjw_get_geometry_mode": (C.c_int, [_vp, _i32p, _i32p]),
+    "mjw_node_order": (C.c_int, [_vp, _i32p]),
     "mjw_info": (C.c_int, [_vp, _i64p]),
     "mjw_set_tile": (C.c_int, [_vp, C.c_int]),
     "mjw_get_tile": (C.c_int, [_vp, C.POINTER(C.c_int)]),
@@ -86,6 +87,10 @@ def lib() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise MjwError(f"{LIB_PATH} is missing: build it with `python -m montjoie_amd.build` "
                        "(there is no CPU fallback)")
+    try:  # torch's HIP runtime first, so that the process holds a single one
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(L, name)

@@ -32,6 +32,10 @@ struct mjw_problem {
   int ncolor = 0;
   std::vector<int64_t> coff;    // colour offsets in sorted order
   std::vector<int64_t> perm;    // sorted -> original element
+  // device vectors use the solver node order: nperm[g] = solver index of
+  // node g of the caller's numbering, ninv = its inverse
+  std::vector<int32_t> nperm, ninv;
+  int32_t *d_nperm = nullptr, *d_ninv = nullptr;
   int epb = 0;
   int nxcd = 8;                 // MI355X: 8 XCDs, round-robin block dispatch
   // host copies (original element order)
@@ -178,6 +182,8 @@ static void release(mjw_problem *p) {
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   dfree(p->d_coef);
   dfree(p->d_glxw);
+  dfree(p->d_nperm);
+  dfree(p->d_ninv);
   void *ptrs[] = {p->d_map, p->d_G, p->d_M, p->d_Minv, p->d_isqM, p->d_sdiag, p->d_dir,
                   p->d_lam, p->d_mu, p->d_rho, p->d_D, p->d_U[0], p->d_U[1], p->d_tmp,
                   p->d_tmp2, p->d_part, p->d_red, p->d_ctl, p->d_pulse, p->d_src_dof,
@@ -315,6 +321,44 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
     std::vector<int64_t> pos(p->coff.begin(), p->coff.end() - 1);
     for (int64_t e = 0; e < n_elem; ++e) p->perm[pos[color[e]]++] = e;
   }
+  // solver node order: nodes grouped by the set of colours whose elements
+  // touch them (each colour pass then reads whole groups, so a cache line
+  // carries only nodes of the current pass), inside a group by first touch
+  // in the kernels' element order
+  {
+    std::vector<uint64_t> mask(n_nodes, 0);
+    for (int64_t e = 0; e < n_elem; ++e)
+      for (int q = 0; q < nd; ++q) mask[elem_nodes[e * nd + q]] |= 1ull << color[e];
+    std::vector<int32_t> by_rank;
+    by_rank.reserve(n_nodes);
+    std::vector<uint8_t> seen(n_nodes, 0);
+    for (int64_t es = 0; es < n_elem; ++es) {
+      const int64_t e = p->perm[es];
+      for (int q = 0; q < nd; ++q) {
+        const int32_t g = elem_nodes[e * nd + q];
+        if (!seen[g]) { seen[g] = 1; by_rank.push_back(g); }
+      }
+    }
+    for (int64_t g = 0; g < n_nodes; ++g)
+      if (!seen[g]) by_rank.push_back((int32_t)g);  // nodes of no element
+    std::vector<uint64_t> masks(mask);
+    std::sort(masks.begin(), masks.end());
+    masks.erase(std::unique(masks.begin(), masks.end()), masks.end());
+    std::vector<int64_t> off(masks.size() + 1, 0);
+    std::vector<int32_t> cid(n_nodes);
+    for (int64_t g = 0; g < n_nodes; ++g) {
+      cid[g] = (int32_t)(std::lower_bound(masks.begin(), masks.end(), mask[g]) - masks.begin());
+      off[cid[g] + 1]++;
+    }
+    for (size_t c = 0; c < masks.size(); ++c) off[c + 1] += off[c];
+    p->nperm.resize(n_nodes);
+    p->ninv.resize(n_nodes);
+    for (int32_t g : by_rank) {
+      const int64_t i = off[cid[g]]++;
+      p->nperm[g] = (int32_t)i;
+      p->ninv[i] = g;
+    }
+  }
   std::vector<int32_t> hmap((size_t)n_elem * nd);
   {
     std::vector<uint8_t> seen(n_nodes, 0);
@@ -325,7 +369,7 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
         for (int ij = 0; ij < p->P; ++ij) {
           const int q = ij + p->P * k;
           const int32_t g = elem_nodes[e * nd + q];
-          uint32_t v = (uint32_t)g;
+          uint32_t v = (uint32_t)p->nperm[g];
           if (!seen[g]) { seen[g] = 1; v |= 0x80000000u; }
           hmap[((size_t)k * n_elem + es) * p->P + ij] = (int32_t)v;
         }
@@ -379,7 +423,10 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   CK(hipMalloc(&d_Mloc, (size_t)n_elem * nd * 8));
   CK(hipMalloc(&d_bad, 4));
   CK(hipMemsetAsync(d_bad, 0, 4, st));
-  CK(hipMemcpyAsync(d_xyz, node_xyz, n_nodes * dim * 8, hipMemcpyHostToDevice, st));
+  std::vector<double> xyz_s((size_t)n_nodes * dim);
+  for (int64_t g = 0; g < n_nodes; ++g)
+    for (int a = 0; a < dim; ++a) xyz_s[(size_t)p->nperm[g] * dim + a] = node_xyz[g * dim + a];
+  CK(hipMemcpyAsync(d_xyz, xyz_s.data(), n_nodes * dim * 8, hipMemcpyHostToDevice, st));
   CK(hipMemcpyAsync(d_Dg, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
   CK(hipMemcpyAsync(d_w, wg, p->n * 8, hipMemcpyHostToDevice, st));
   if (launch_geometry_full(dim, order, ncomp, n_elem, p->d_map, d_xyz, d_Dg, d_w, p->d_rho,
@@ -413,20 +460,35 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   } else {
     p->hcoef.clear();
   }
-  // Dirichlet rows of the mass are 1 (TimeAcoustic.cxx:936-937)
-  std::vector<double> minv(n_nodes), isq(n_nodes);
+  // the assembled mass came back in solver order; host copies are in the
+  // caller's order.  Dirichlet rows of the mass are 1 (TimeAcoustic.cxx:936-937)
+  {
+    std::vector<double> ms_solver(p->mass);
+    for (int64_t g = 0; g < n_nodes; ++g) p->mass[g] = ms_solver[p->nperm[g]];
+  }
+  std::vector<double> msv(n_nodes), minv(n_nodes), isq(n_nodes);
+  std::vector<uint8_t> dirv(n_nodes);
   for (int64_t g = 0; g < n_nodes; ++g) {
     if (p->dir[g]) p->mass[g] = 1.0;
-    minv[g] = 1.0 / p->mass[g];
-    isq[g] = p->dir[g] ? 0.0 : 1.0 / std::sqrt(p->mass[g]);
+    const int64_t i = p->nperm[g];
+    msv[i] = p->mass[g];
+    minv[i] = 1.0 / p->mass[g];
+    isq[i] = p->dir[g] ? 0.0 : 1.0 / std::sqrt(p->mass[g]);
+    dirv[i] = p->dir[g];
   }
-  CK(hipMemcpyAsync(p->d_M, p->mass.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(p->d_M, msv.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
   CK(dalloc(p, &p->d_Minv, n_nodes * 8));
   CK(dalloc(p, &p->d_isqM, n_nodes * 8));
   CK(hipMemcpyAsync(p->d_Minv, minv.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
   CK(hipMemcpyAsync(p->d_isqM, isq.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
   CK(dalloc(p, &p->d_dir, n_nodes));
-  CK(hipMemcpyAsync(p->d_dir, p->dir.data(), n_nodes, hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(p->d_dir, dirv.data(), n_nodes, hipMemcpyHostToDevice, st));
+  CK(dalloc(p, &p->d_nperm, n_nodes * 4));
+  CK(dalloc(p, &p->d_ninv, n_nodes * 4));
+  CK(hipMemcpyAsync(p->d_nperm, p->nperm.data(), n_nodes * 4, hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(p->d_ninv, p->ninv.data(), n_nodes * 4, hipMemcpyHostToDevice, st));
+  // xyz_s, msv, ... are read by the copies above before they go out of scope
+  CK(hipStreamSynchronize(st));
   // state
   const size_t sb = (size_t)nstate(p) * ssize(p);
   CK(dalloc(p, &p->d_U[0], sb));
@@ -539,37 +601,63 @@ extern "C" int mjw_apply_kh_dev(mjw_problem *p, double alpha, const void *U, dou
                                 void *Y) {
   if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_kh_dev: NULL");
   HIPCHK(p, hipSetDevice(p->device));
-  KCHK(p, apply_A(p, -alpha, U, beta, Y, p->nshots));
+  // caller's order -> solver order (gather by ninv) and back (gather by nperm)
+  const int64_t nb = (int64_t)p->nshots * p->ncomp;
+  KCHK(p, launch_gather(p->state, nb, p->N, p->d_ninv, U, p->d_tmp, p->stream));
+  if (beta != 0.0)
+    KCHK(p, launch_gather(p->state, nb, p->N, p->d_ninv, Y, p->d_tmp2, p->stream));
+  else
+    HIPCHK(p, hipMemsetAsync(p->d_tmp2, 0, nb * p->N * ssize(p), p->stream));
+  KCHK(p, apply_A(p, -alpha, p->d_tmp, beta, p->d_tmp2, p->nshots));
+  KCHK(p, launch_gather(p->state, nb, p->N, p->d_nperm, p->d_tmp2, Y, p->stream));
   HIPCHK(p, hipStreamSynchronize(p->stream));
   return MJW_OK;
 }
 
-// host FP64 single vector -> device state precision
+extern "C" int mjw_node_order(const mjw_problem *p, int32_t *solver_index) {
+  if (!p || !solver_index) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_node_order: NULL");
+  std::copy(p->nperm.begin(), p->nperm.end(), solver_index);
+  return MJW_OK;
+}
+
+// host FP64 vectors (caller's node order, n = blocks of N) -> device state
+// precision in solver order
 static int upload_vec(mjw_problem *p, const double *h, void *dst, int64_t n) {
-  double *tmp = nullptr;
+  std::vector<double> hs((size_t)n);
+  for (int64_t b = 0; b < n; b += p->N)
+    for (int64_t g = 0; g < p->N; ++g) hs[b + p->nperm[g]] = h[b + g];
+  int rc = (int)hipStreamSynchronize(p->stream);
+  if (rc) return rc;
   if (p->state == kF64)
-    return (int)hipMemcpyAsync(dst, h, n * 8, hipMemcpyHostToDevice, p->stream);
+    return (int)hipMemcpy(dst, hs.data(), n * 8, hipMemcpyHostToDevice);
+  double *tmp = nullptr;
   if (hipMalloc(&tmp, n * 8) != hipSuccess) return -1;
-  int rc = (int)hipMemcpyAsync(tmp, h, n * 8, hipMemcpyHostToDevice, p->stream);
+  rc = (int)hipMemcpyAsync(tmp, hs.data(), n * 8, hipMemcpyHostToDevice, p->stream);
   if (!rc) rc = launch_convert_from_f64(p->state, n, tmp, dst, p->stream);
   if (!rc) rc = (int)hipStreamSynchronize(p->stream);
   dfree(tmp);
   return rc;
 }
 
+// device state (solver order) -> host FP64 in the caller's node order
 static int download_vec(mjw_problem *p, const void *src, double *h, int64_t n) {
+  std::vector<double> hs((size_t)n);
+  int rc = 0;
   if (p->state == kF64) {
-    int rc = (int)hipMemcpyAsync(h, src, n * 8, hipMemcpyDeviceToHost, p->stream);
+    rc = (int)hipMemcpyAsync(hs.data(), src, n * 8, hipMemcpyDeviceToHost, p->stream);
     if (!rc) rc = (int)hipStreamSynchronize(p->stream);
-    return rc;
+  } else {
+    double *tmp = nullptr;
+    if (hipMalloc(&tmp, n * 8) != hipSuccess) return -1;
+    rc = launch_convert_to_f64(p->state, n, src, tmp, p->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(hs.data(), tmp, n * 8, hipMemcpyDeviceToHost, p->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(p->stream);
+    dfree(tmp);
   }
-  double *tmp = nullptr;
-  if (hipMalloc(&tmp, n * 8) != hipSuccess) return -1;
-  int rc = launch_convert_to_f64(p->state, n, src, tmp, p->stream);
-  if (!rc) rc = (int)hipMemcpyAsync(h, tmp, n * 8, hipMemcpyDeviceToHost, p->stream);
-  if (!rc) rc = (int)hipStreamSynchronize(p->stream);
-  dfree(tmp);
-  return rc;
+  if (rc) return rc;
+  for (int64_t b = 0; b < n; b += p->N)
+    for (int64_t g = 0; g < p->N; ++g) h[b + g] = hs[b + p->nperm[g]];
+  return 0;
 }
 
 extern "C" int mjw_apply_kh(mjw_problem *p, double alpha, const double *U, double beta,
@@ -677,7 +765,7 @@ extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, cons
         double v = pm * kv.second;
         if (cnt > 1) v *= 1.0 / cnt;
         if (p->dir[kv.first] || std::fabs(v) <= thr) continue;
-        dof.push_back((int64_t)m * p->N + kv.first);
+        dof.push_back((int64_t)m * p->N + p->nperm[kv.first]);
         sh.push_back(s);
         val.push_back(v);
       }
@@ -730,7 +818,8 @@ extern "C" int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts,
       if (locate_in(p, e, x, &w[(size_t)r * p->nd])) found = e;
     }
     if (found < 0) FAIL(p, MJW_ERR_ARG, "receiver " + std::to_string(r) + " outside the mesh");
-    for (int q = 0; q < p->nd; ++q) gid[(size_t)r * p->nd + q] = p->en[found * p->nd + q];
+    for (int q = 0; q < p->nd; ++q)
+      gid[(size_t)r * p->nd + q] = p->nperm[p->en[found * p->nd + q]];
   }
   dfree(p->d_rec_gid); dfree(p->d_rec_w); dfree(p->d_rec_comp);
   p->d_rec_gid = nullptr; p->d_rec_w = nullptr; p->d_rec_comp = nullptr;

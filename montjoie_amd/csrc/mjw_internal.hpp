@@ -133,6 +133,9 @@ int launch_tail_n(const TailArgs &a, int ncomp, void *stream);
 int launch_axpby(int state, int64_t n, double a, const void *x, double b, void *y, void *stream);
 int launch_convert_to_f64(int state, int64_t n, const void *x, double *y, void *stream);
 int launch_convert_from_f64(int state, int64_t n, const double *x, void *y, void *stream);
+// y[b*N + j] = x[b*N + idx[j]], b < nb (x and y distinct)
+int launch_gather(int state, int64_t nb, int64_t N, const int32_t *idx, const void *x, void *y,
+                  void *stream);
 // y[i] *= c*d[i % N]
 int launch_scale_by(int state, int64_t n, int S, int ncomp, int64_t N, const double *d, double c,
                     void *y, void *stream);

@@ -305,6 +305,30 @@ int launch_convert_from_f64(int state, int64_t n, const double *x, void *y, void
   return (int)hipGetLastError();
 }
 
+// y[b*N + j] = x[b*N + idx[j]] for nb blocks of N (node renumbering)
+template <typename T>
+__global__ void k_gather(int64_t n, int64_t N, const int32_t *__restrict__ idx,
+                         const T *__restrict__ x, T *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = i / N, j = i - b * N;
+    y[i] = x[b * N + idx[j]];
+  }
+}
+
+int launch_gather(int state, int64_t nb, int64_t N, const int32_t *idx, const void *x, void *y,
+                  void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t n = nb * N;
+  if (state == kF64)
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(256), 0, st, n, N, idx,
+                       (const double *)x, (double *)y);
+  else
+    hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n)), dim3(256), 0, st, n, N, idx,
+                       (const float *)x, (float *)y);
+  return (int)hipGetLastError();
+}
+
 // y[(s*ncomp+m)*N+g] *= c*d[g]
 template <typename T>
 __global__ void k_scale_by(int64_t n, int64_t N, const double *__restrict__ d, double c,

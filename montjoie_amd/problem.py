@@ -246,6 +246,12 @@ class WaveProblem:
         self._chk(_lib.lib().mjw_get_state(self._h, dptr(a), dptr(b)), "get_state")
         return a.reshape(self.nshots, self.ndof), b.reshape(self.nshots, self.ndof)
 
+    def node_order(self) -> np.ndarray:
+        """solver_index[g]: position of node g in the device vectors."""
+        out = np.zeros(self.mesh.N, dtype=np.int32)
+        self._chk(_lib.lib().mjw_node_order(self._h, iptr(out)), "node_order")
+        return out
+
     def state_dev(self):
         a = C.c_void_p()
         b = C.c_void_p()

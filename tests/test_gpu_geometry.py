@@ -132,3 +132,48 @@ def test_set_mu_equals_fresh_problem(mode):
     Q.set_geometry(mode)
     u = np.random.default_rng(2).standard_normal(m.N)
     assert CS.rel_l2(P.apply_kh(u), Q.apply_kh(u)) < 1e-14
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_solver_node_order_and_device_vectors(prec):
+    """Device vectors live in the solver node order; host entry points and
+    mjw_apply_kh_dev take the caller's order."""
+    import torch
+    m = M.regular_mesh(3, 3, 4, 0.1, seed=7)
+    P = M.WaveProblem(m, 1.0, 1.0 + (np.arange(m.E) % 2), precision=prec, nshots=2)
+    idx = P.node_order()
+    assert np.array_equal(np.sort(idx), np.arange(m.N))
+    dt = np.float64 if prec == "fp64" else np.float32
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((2, m.N))
+    y = rng.standard_normal((2, m.N))
+    ref = np.stack([P.apply_kh(u[s], alpha=1.5, beta=0.5, Y=y[s]) for s in range(2)])
+    U = torch.tensor(u.astype(dt), device="cuda")
+    Y = torch.tensor(y.astype(dt), device="cuda")
+    P.apply_kh_dev(U.data_ptr(), Y.data_ptr(), alpha=1.5, beta=0.5)
+    torch.cuda.synchronize()
+    tol = 1e-13 if prec == "fp64" else 1e-5
+    assert CS.rel_l2(Y.cpu().numpy().astype(np.float64), ref) < tol
+    # state_dev: solver order
+    P.set_sources([(0.5, 0.5, 0.5), (0.4, 0.6, 0.5)], shot=[0, 1])
+    P.set_pulse(M.ricker_pulse_table(8.0, 0.0, 1e-3, 40))
+    P.leapfrog_init(1e-3)
+    P.advance(30, traces=False)
+    un, un1 = P.state()
+    a, _ = P.state_dev()
+    dev = device_array(a, 2 * m.N, dt).reshape(2, m.N)
+    np.testing.assert_array_equal(dev[:, idx].astype(np.float64), un)
+
+
+class _DevPtr:
+    """__cuda_array_interface__ view of a raw device pointer."""
+    def __init__(self, ptr, n, dt):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": np.dtype(dt).str,
+                                         "data": (ptr, False), "version": 2}
+
+
+def device_array(ptr, n, dt):
+    import torch
+    t = torch.as_tensor(_DevPtr(ptr, n, dt), device="cuda")
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
