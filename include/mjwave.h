@@ -104,9 +104,10 @@ int mjw_set_xcd_order(mjw_problem *p, int nxcd);
 int mjw_tile_choices(const mjw_problem *p, int *out, int cap);
 
 /*
- * Geometry of the 3-D acoustic stiffness kernels.
- *   MJW_GEOM_STORED: the per-node tensor J w mu DF^{-1} DF^{-T} (6 values per
- *     node) is computed once by mjw_create and read every step;
+ * Geometry of the 3-D stiffness kernels.
+ *   MJW_GEOM_STORED: the per-node factors (acoustic J w mu DF^{-1} DF^{-T},
+ *     6 values; elastic sqrt(J w) DF^{-1}, 9 values) are computed once by
+ *     mjw_create and read every step;
  *   MJW_GEOM_TRILINEAR: for meshes of straight-sided hexahedra (every node the
  *     image of the GL grid under the trilinear map of the element's 8
  *     vertices, checked by mjw_create to 1e-11 of the element diameter) the
@@ -114,8 +115,8 @@ int mjw_tile_choices(const mjw_problem *p, int *out, int cap);
  *     element -- the transformation Montjoie itself differentiates for
  *     non-curved elements (ComputeLocalMassMatrix, VarGeometryProblem.cxx:
  *     671-690) -- and read no per-node geometry.
- * mjw_create selects TRILINEAR whenever the mesh allows it (3-D, ncomp 1),
- * STORED otherwise.  Results of the two modes agree to rounding.
+ * mjw_create selects TRILINEAR whenever the mesh allows it (3-D), STORED
+ * otherwise.  Results of the two modes agree to rounding.
  */
 #define MJW_GEOM_STORED 0
 #define MJW_GEOM_TRILINEAR 1

@@ -448,7 +448,7 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   d_bad = nullptr;
   if (bad) return fail(MJW_ERR_GEOMETRY, "non-positive Jacobian: degenerate or inverted element");
   // straight-sided hexahedra: per-element trilinear coefficients (default)
-  if (dim == 3 && ncomp == 1 &&
+  if (dim == 3 &&
       trilinear_coefficients(order, n_elem, elem_nodes, node_xyz, p->perm.data(), ms.data(),
                              1e-11, p->hcoef)) {
     const size_t gb = p->geom == kG64 ? 8 : 4;
@@ -547,7 +547,7 @@ extern "C" int mjw_set_geometry_mode(mjw_problem *p, int mode) {
     FAIL(p, MJW_ERR_ARG, "mjw_set_geometry_mode: bad mode");
   if (mode == MJW_GEOM_TRILINEAR && !p->tri_ok)
     FAIL(p, MJW_ERR_UNSUPPORTED,
-         "trilinear geometry needs a 3-D acoustic mesh of straight-sided hexahedra");
+         "trilinear geometry needs a 3-D mesh of straight-sided hexahedra");
   const int g = mode == MJW_GEOM_TRILINEAR ? kGeoTrilinear : kGeoStored;
   if (g != p->geo) {
     if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }

@@ -13,7 +13,10 @@
 // major [k][9][E][n*n]): grad' = M^T g = sqrt(Jw) grad u, sigma' = C:grad'
 // = sqrt(Jw) sigma, v = M sigma' = J w DF^{-1} sigma -- one 3x3 tensor per
 // node instead of J w and J DF^{-1}.  Mapping, colouring, fused leapfrog and
-// shot loop as in the scalar kernels (mjw_kernels_scalar.hip).
+// shot loop as in the scalar kernels (mjw_kernels_scalar.hip).  TRI: for
+// straight-sided hexahedra M is rebuilt per node from the element's
+// trilinear map (DF^{-1} = adj(DF)/J, so M = sqrt(w/J) adj(DF)) instead of
+// read from G.
 #include <hip/hip_runtime.h>
 
 #include "mjw_device.hpp"
@@ -21,15 +24,48 @@
 
 namespace mjw {
 
-template <int N, int EPB, typename T, typename TG, int MODE>
+// columns of DF at layer k of one (i, j) column: c0 = p0 + x_k q0,
+// c1 = p1 + x_k q1, c2 fixed (trilinear map coefficients a1..a7)
+template <typename A>
+struct TriColE {
+  A p0[3], q0[3], p1[3], q1[3], c2[3], wij;
+};
+
+template <typename A>
+__device__ __forceinline__ void tri_m(const TriColE<A> &c, A zeta, A wk, A M[9]) {
+  A c0[3], c1[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    c0[d] = c.p0[d] + zeta * c.q0[d];
+    c1[d] = c.p1[d] + zeta * c.q1[d];
+  }
+  const A r0[3] = {c1[1] * c.c2[2] - c1[2] * c.c2[1], c1[2] * c.c2[0] - c1[0] * c.c2[2],
+                   c1[0] * c.c2[1] - c1[1] * c.c2[0]};
+  const A r1[3] = {c.c2[1] * c0[2] - c.c2[2] * c0[1], c.c2[2] * c0[0] - c.c2[0] * c0[2],
+                   c.c2[0] * c0[1] - c.c2[1] * c0[0]};
+  const A r2[3] = {c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2],
+                   c0[0] * c1[1] - c0[1] * c1[0]};
+  const A J = c0[0] * r0[0] + c0[1] * r0[1] + c0[2] * r0[2];
+  const A f = sqrt(c.wij * wk / J);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    M[a] = f * r0[a];
+    M[3 + a] = f * r1[a];
+    M[6 + a] = f * r2[a];
+  }
+}
+
+template <int N, int EPB, typename T, typename TG, int MODE, bool TRI>
 __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
     const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
     const double *__restrict__ lam_e, const double *__restrict__ mu_e, int64_t E, int64_t e0,
     int64_t e1, const T *__restrict__ U, T *__restrict__ Y, const T *__restrict__ sdiag,
-    T alpha, T beta, int64_t N_nodes, int nshots, int nxcd) {
+    T alpha, T beta, int64_t N_nodes, int nshots, int nxcd, const TG *__restrict__ coef,
+    const TG *__restrict__ glxw) {
   constexpr int P = N * N, NN = N * N * N;
   using A = TG;
   __shared__ T lds[6 * EPB * NN];
+  __shared__ TG lc[TRI ? EPB * kCoef : 1];
   const int t = threadIdx.x;
   const int el = t / P, ij = t - el * P;
   const int i = ij % N, j = ij / N;
@@ -41,6 +77,27 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
   const uint32_t g_ps = (uint32_t)(plane * sizeof(TG));
   int32_t gid[N];
   A lam = 0, mu = 0;
+  TriColE<A> tc;
+  if constexpr (TRI) {
+    for (int q = t; q < EPB * kCoef; q += blockDim.x)
+      lc[q] = eb + q / kCoef < e1 ? coef[eb * kCoef + q] : TG(0);
+    __syncthreads();
+    if (act) {
+      const TG *c = lc + el * kCoef;
+      const A xi = glxw[i], eta = glxw[j];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const A a1 = c[d], a2 = c[3 + d], a3 = c[6 + d], a4 = c[9 + d], a5 = c[12 + d],
+                a6 = c[15 + d], a7 = c[18 + d];
+        tc.p0[d] = a1 + a4 * eta;
+        tc.q0[d] = a6 + a7 * eta;
+        tc.p1[d] = a2 + a4 * xi;
+        tc.q1[d] = a5 + a7 * xi;
+        tc.c2[d] = a3 + a5 * eta + a6 * xi + a7 * (xi * eta);
+      }
+      tc.wij = (A)glxw[N + i] * (A)glxw[N + j];
+    }
+  }
   if (act) {
 #pragma unroll
     for (int k = 0; k < N; ++k)
@@ -94,9 +151,13 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
           g[m][2] = a2;
         }
         A M[9];
-        const rsrc_t rg = mkres(G + (k * 9) * plane + eb * P);
+        if constexpr (TRI) {
+          tri_m<A>(tc, (A)glxw[k], (A)glxw[N + k], M);
+        } else {
+          const rsrc_t rg = mkres(G + (k * 9) * plane + eb * P);
 #pragma unroll
-        for (int c = 0; c < 9; ++c) M[c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
+          for (int c = 0; c < 9; ++c) M[c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
+        }
         A du[3][3];
 #pragma unroll
         for (int m = 0; m < 3; ++m)
@@ -167,25 +228,29 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
   }
 }
 
+template <int N, int EPB, typename T, typename TG, int MODE, bool TRI>
+static void launch_el(const StiffArgs &a, hipStream_t st, unsigned blocks) {
+  hipLaunchKernelGGL((k_elastic3d<N, EPB, T, TG, MODE, TRI>), dim3(blocks), dim3(EPB * N * N), 0,
+                     st, a.map, static_cast<const TG *>(a.G), static_cast<const T *>(a.Dm), a.lam,
+                     a.mu, a.E, a.e0, a.e1, static_cast<const T *>(a.U), static_cast<T *>(a.Y),
+                     static_cast<const T *>(a.sdiag), (T)a.alpha, (T)a.beta, a.N, a.nshots,
+                     a.nxcd, static_cast<const TG *>(a.coef), static_cast<const TG *>(a.glxw));
+}
+
 template <int N, int EPB, typename T, typename TG>
 static int run_elastic(const StiffArgs &a, hipStream_t st) {
   const int64_t ne = a.e1 - a.e0;
   if (ne <= 0) return 0;
   const unsigned blocks = (unsigned)((ne + EPB - 1) / EPB);
-  const unsigned threads = EPB * N * N;
-  const T *D = static_cast<const T *>(a.Dm);
-  const T *U = static_cast<const T *>(a.U);
-  T *Y = static_cast<T *>(a.Y);
-  const T *s = static_cast<const T *>(a.sdiag);
-  const TG *G = static_cast<const TG *>(a.G);
-  if (a.mode == kLeap)
-    hipLaunchKernelGGL((k_elastic3d<N, EPB, T, TG, kLeap>), dim3(blocks), dim3(threads), 0, st,
-                       a.map, G, D, a.lam, a.mu, a.E, a.e0, a.e1, U, Y, s, (T)a.alpha,
-                       (T)a.beta, a.N, a.nshots, a.nxcd);
-  else
-    hipLaunchKernelGGL((k_elastic3d<N, EPB, T, TG, kApply>), dim3(blocks), dim3(threads), 0, st,
-                       a.map, G, D, a.lam, a.mu, a.E, a.e0, a.e1, U, Y, s, (T)a.alpha,
-                       (T)a.beta, a.N, a.nshots, a.nxcd);
+  const bool tri = a.geo == kGeoTrilinear;
+  if (tri && (!a.coef || !a.glxw)) return -1002;
+  if (a.mode == kLeap) {
+    if (tri) launch_el<N, EPB, T, TG, kLeap, true>(a, st, blocks);
+    else launch_el<N, EPB, T, TG, kLeap, false>(a, st, blocks);
+  } else {
+    if (tri) launch_el<N, EPB, T, TG, kApply, true>(a, st, blocks);
+    else launch_el<N, EPB, T, TG, kApply, false>(a, st, blocks);
+  }
   return (int)hipGetLastError();
 }
 

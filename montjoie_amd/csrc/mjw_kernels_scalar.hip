@@ -193,6 +193,23 @@ __device__ __forceinline__ void scalar3d_body(
         sv[k * P + ij] = v2[k];
       }
     }
+    // FP32 state: the column's previous values and dt^2/M are loaded before
+    // the scatter loop, whose stores to Y would otherwise order every later
+    // load of Y behind them (no thread of a colour pass touches another's
+    // nodes, so reading early is exact).  FP64 keeps the loads in the loop:
+    // the extra registers would cost more occupancy than they save.
+    constexpr bool PRE = sizeof(T) == 4;
+    T yk[PRE ? N : 1], sk[PRE && MODE == kLeap ? N : 1];
+    if constexpr (PRE) {
+      if (act) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const uint32_t off = (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T);
+          yk[k] = bld<T>(ry, off, 0u);
+          if constexpr (MODE == kLeap) sk[k] = bld<T>(rs, off, 0u);
+        }
+      }
+    }
     __syncthreads();
     if (act) {
       T Dti[N], Dtj[N];
@@ -213,10 +230,10 @@ __device__ __forceinline__ void scalar3d_body(
         const int32_t m = gid[k];
         const uint32_t off = (uint32_t)(m & 0x7fffffff) * (uint32_t)sizeof(T);
         const bool first = m < 0;
-        const T y = bld<T>(ry, off, 0u);
+        const T y = PRE ? yk[PRE ? k : 0] : bld<T>(ry, off, 0u);
         T r;
-        if (MODE == kLeap) {
-          const T sg = bld<T>(rs, off, 0u);
+        if constexpr (MODE == kLeap) {
+          const T sg = PRE ? sk[PRE ? k : 0] : bld<T>(rs, off, 0u);
           r = first ? (T(2) * u[k] - y - sg * o) : (y - sg * o);
         } else {
           r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
@@ -363,6 +380,136 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
   }
 }
 
+// ------------------------------------------------------- thread per node
+// 3-D scalar pass with one thread per element node (EPB elements, EPB*n^3
+// threads per block): every node's loads are independent, so a colour pass
+// has n times more waves in flight than the column kernel to cover the
+// latency of the gathers.  Threads are ordered (k, element, i + n j) so the
+// map and geometry planes are read as EPB*n^2 contiguous values per k.
+// The contractions use the same operand order as the column kernel.
+template <int N, int EPB, typename T, typename TG, int MODE, bool GRAD, bool TRI>
+__global__ __launch_bounds__(EPB *N *N *N) void k_node3d(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
+    const T *__restrict__ sdiag, T alpha, T beta, int64_t N_nodes, int nshots, int nxcd,
+    double *__restrict__ grad, const double *__restrict__ mu_e, const TG *__restrict__ coef,
+    const TG *__restrict__ glxw) {
+  constexpr int P = N * N, NN = P * N, NT = EPB * NN;
+  using A = TG;
+  __shared__ T lu[EPB * NN], l1[EPB * NN], l2[EPB * NN], l3[EPB * NN];
+  __shared__ T sD[N * N];
+  __shared__ TG lc[TRI ? EPB * kCoef : 1];
+  __shared__ double gpart[GRAD ? NT : 1];
+  const int t = threadIdx.x;
+  const int k = t / (EPB * P), rem = t - k * (EPB * P), el = rem / P, ij = rem - el * P;
+  const int i = ij % N, j = ij / N;
+  const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * EPB;
+  const bool act = eb + el < e1;
+  const int64_t plane = E * P;
+  const uint32_t lo = (uint32_t)(el * P + ij);
+  for (int q = t; q < N * N; q += NT) sD[q] = Dm[q];
+  if constexpr (TRI) {
+    for (int q = t; q < EPB * kCoef; q += NT)
+      lc[q] = eb + q / kCoef < e1 ? coef[eb * kCoef + q] : TG(0);
+  }
+  int32_t gid = 0;
+  A g[6] = {0, 0, 0, 0, 0, 0};
+  if (act) {
+    gid = bld<int32_t>(mkres(map + eb * P), lo * 4u, (uint32_t)(k * plane * 4));
+    if constexpr (!TRI) {
+      const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
+      const uint32_t g_ps = (uint32_t)(plane * sizeof(TG));
+#pragma unroll
+      for (int c = 0; c < 6; ++c) g[c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
+    }
+  }
+  __syncthreads();
+  if constexpr (TRI) {
+    if (act) {
+      const TG *c = lc + el * kCoef;
+      const A xi = glxw[i], eta = glxw[j];
+      TriCol<A> tc;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const A a1 = c[d], a2 = c[3 + d], a3 = c[6 + d], a4 = c[9 + d], a5 = c[12 + d],
+                a6 = c[15 + d], a7 = c[18 + d];
+        tc.p0[d] = a1 + a4 * eta;
+        tc.q0[d] = a6 + a7 * eta;
+        tc.p1[d] = a2 + a4 * xi;
+        tc.q1[d] = a5 + a7 * xi;
+        tc.c2[d] = a3 + a5 * eta + a6 * xi + a7 * (xi * eta);
+      }
+      tc.f = (A)c[21] * ((A)glxw[N + i] * (A)glxw[N + j]);
+      tri_geometry<A>(tc, (A)glxw[k], (A)glxw[N + k], g);
+    }
+  }
+  const uint32_t off = (uint32_t)(gid & 0x7fffffff) * (uint32_t)sizeof(T);
+  const bool first = gid < 0;
+  const rsrc_t rs = mkres(sdiag);
+  const int lq = el * NN + k * P + ij;
+  const T *su = lu + el * NN;
+  const T *s1 = l1 + el * NN, *s2 = l2 + el * NN, *s3 = l3 + el * NN;
+  T ufw = 0;
+  double gacc = 0.0;
+  for (int s = 0; s < nshots; ++s) {
+    const rsrc_t ru = mkres(U + (int64_t)s * N_nodes);
+    const rsrc_t ry = mkres(Y + (int64_t)s * N_nodes);
+    T u = 0, y = 0, sg = 0;
+    if (act) {
+      u = bld<T>(ru, off, 0u);
+      y = bld<T>(ry, off, 0u);
+      if constexpr (MODE == kLeap) sg = bld<T>(rs, off, 0u);
+      lu[lq] = u;
+    }
+    __syncthreads();
+    if (act) {
+      T a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        a1 += sD[i * N + l] * su[k * P + j * N + l];
+        a2 += sD[j * N + l] * su[k * P + l * N + i];
+        a3 += sD[k * N + l] * su[l * P + ij];
+      }
+      const A b1 = a1, b2 = a2, b3 = a3;
+      l1[lq] = (T)(g[0] * b1 + g[1] * b2 + g[2] * b3);
+      l2[lq] = (T)(g[1] * b1 + g[3] * b2 + g[4] * b3);
+      l3[lq] = (T)(g[2] * b1 + g[4] * b2 + g[5] * b3);
+    }
+    __syncthreads();
+    if (act) {
+      T o = 0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        o += sD[l * N + i] * s1[k * P + j * N + l];
+        o += sD[l * N + j] * s2[k * P + l * N + i];
+        o += sD[l * N + k] * s3[l * P + ij];
+      }
+      T r;
+      if constexpr (MODE == kLeap)
+        r = first ? (T(2) * u - y - sg * o) : (y - sg * o);
+      else
+        r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
+      bst<T>(r, ry, off, 0u);
+      if constexpr (GRAD) {
+        if (s == 0) ufw = u;
+        else gacc += (double)o * (double)ufw;
+      }
+    }
+  }
+  if constexpr (GRAD) {
+    // per-element sum in a fixed order (deterministic), FP64
+    gpart[t] = act ? gacc : 0.0;
+    __syncthreads();
+    if (act && k == 0 && ij == 0) {
+      double sum = 0.0;
+      for (int kk = 0; kk < N; ++kk)
+        for (int q = 0; q < P; ++q) sum += gpart[kk * EPB * P + el * P + q];
+      const int64_t e = eb + el;
+      grad[e] -= sum / mu_e[e];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ batch
 // Many independent 3-D acoustic problems (any mesh size, orders 1..7) in one
 // launch per colour: block b takes work item b = (problem, first element of
@@ -498,6 +645,49 @@ static int run_scalar(const StiffArgs &a, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+template <int N, int EPB, typename T, typename TG, int MODE, bool GRAD, bool TRI>
+static void launch_node(const StiffArgs &a, hipStream_t st, unsigned blocks) {
+  hipLaunchKernelGGL((k_node3d<N, EPB, T, TG, MODE, GRAD, TRI>), dim3(blocks),
+                     dim3(EPB * N * N * N), 0, st, a.map, static_cast<const TG *>(a.G),
+                     static_cast<const T *>(a.Dm), a.E, a.e0, a.e1, static_cast<const T *>(a.U),
+                     static_cast<T *>(a.Y), static_cast<const T *>(a.sdiag), (T)a.alpha,
+                     (T)a.beta, a.N, a.nshots, a.nxcd, a.grad, a.mu,
+                     static_cast<const TG *>(a.coef), static_cast<const TG *>(a.glxw));
+}
+
+template <int N, int EPB, typename T, typename TG, bool TRI>
+static void run_node3(const StiffArgs &a, hipStream_t st, unsigned blocks) {
+  if (a.grad)
+    launch_node<N, EPB, T, TG, kLeap, true, TRI>(a, st, blocks);
+  else if (a.mode == kLeap)
+    launch_node<N, EPB, T, TG, kLeap, false, TRI>(a, st, blocks);
+  else
+    launch_node<N, EPB, T, TG, kApply, false, TRI>(a, st, blocks);
+}
+
+template <int N, int EPB, typename T, typename TG>
+static int run_node(const StiffArgs &a, hipStream_t st) {
+  const int64_t ne = a.e1 - a.e0;
+  if (ne <= 0) return 0;
+  if (a.dim != 3 || a.ncomp != 1) return -1003;
+  if (a.grad && (a.mode != kLeap || a.nshots != 2)) return -1001;
+  const unsigned blocks = (unsigned)((ne + EPB - 1) / EPB);
+  if (a.geo == kGeoTrilinear) {
+    if (!a.coef || !a.glxw) return -1002;
+    run_node3<N, EPB, T, TG, true>(a, st, blocks);
+  } else {
+    run_node3<N, EPB, T, TG, false>(a, st, blocks);
+  }
+  return (int)hipGetLastError();
+}
+
+template <int N, int EPB>
+static int run_node_prec(const StiffArgs &a, hipStream_t st) {
+  if (a.state == kF64) return run_node<N, EPB, double, double>(a, st);
+  if (a.geom == kG64) return run_node<N, EPB, float, double>(a, st);
+  return run_node<N, EPB, float, float>(a, st);
+}
+
 template <int N, int EPB>
 static int run_prec(const StiffArgs &a, hipStream_t st) {
   if (a.state == kF64) return run_scalar<N, EPB, double, double>(a, st);
@@ -510,6 +700,11 @@ static int run_prec(const StiffArgs &a, hipStream_t st) {
 #define MJW_TILES3(X) \
   X(2, 16) X(2, 32) X(3, 7) X(3, 14) X(4, 8) X(4, 4) X(4, 16) X(5, 10) X(5, 5) X(5, 20) \
   X(6, 7) X(6, 14) X(6, 16) X(7, 5) X(7, 10) X(8, 2) X(8, 4)
+// thread-per-node tiles (elements per block; listed as -EPB):
+// threads = EPB*n^3 <= 1024, LDS = 4*EPB*n^3*sizeof(T) + ...
+#define MJW_NODE_TILES3(X) \
+  X(2, 32) X(2, 64) X(3, 8) X(3, 16) X(4, 4) X(4, 8) X(5, 2) X(5, 4) X(6, 1) X(6, 2) X(6, 4) \
+  X(7, 1) X(7, 2) X(8, 1) X(8, 2)
 #define MJW_TILES2(X) \
   X(2, 64) X(2, 16) X(3, 28) X(3, 7) X(4, 16) X(4, 4) X(5, 10) X(5, 20) X(6, 7) X(6, 14) \
   X(7, 5) X(7, 10) X(8, 4) X(8, 1)
@@ -519,12 +714,16 @@ int launch_stiffness_scalar(const StiffArgs &a, void *stream) {
   const int n = a.order + 1;
 #define X(NN, EE) \
   if (n == NN && a.epb == EE) return run_prec<NN, EE>(a, st);
+#define XN(NN, EE) \
+  if (n == NN && a.epb == -EE) return run_node_prec<NN, EE>(a, st);
   if (a.dim == 3) {
     MJW_TILES3(X)
+    MJW_NODE_TILES3(XN)
   } else {
     MJW_TILES2(X)
   }
 #undef X
+#undef XN
   return -1000;
 }
 
@@ -533,12 +732,16 @@ std::vector<int> tile_choices_scalar(int dim, int order) {
   const int n = order + 1;
 #define X(NN, EE) \
   if (n == NN) v.push_back(EE);
+#define XN(NN, EE) \
+  if (n == NN) v.push_back(-EE);
   if (dim == 3) {
     MJW_TILES3(X)
+    MJW_NODE_TILES3(XN)
   } else {
     MJW_TILES2(X)
   }
 #undef X
+#undef XN
   return v;
 }
 

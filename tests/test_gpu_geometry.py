@@ -40,9 +40,7 @@ def test_modes_detected():
     P2 = M.WaveProblem(M.regular_mesh(2, 3, 3, 0.1, seed=1), 1.0, 1.0)
     assert not P2.trilinear_available and P2.geometry == "stored"
     Pe = M.WaveProblem(m3, 1.0, 1.0, lam=1.0, ncomp=3)
-    assert not Pe.trilinear_available
-    with pytest.raises(MjwError):
-        Pe.set_geometry("trilinear")
+    assert Pe.trilinear_available and Pe.geometry == "trilinear"
     mc = dataclasses.replace(m3, xyz=curve(m3.xyz))
     Pc = M.WaveProblem(mc, 1.0, 1.0)
     assert not Pc.trilinear_available and Pc.geometry == "stored"
@@ -62,6 +60,20 @@ def test_apply_matches_oracle_both_modes(r, nx, mode):
     u = np.random.default_rng(r).standard_normal(m.N)
     ref = -O.stiffness(o, G, u, O.color(o))
     assert CS.rel_l2(P.apply_kh(u), ref) < 1e-12
+
+
+@pytest.mark.parametrize("r,nx", [(1, 3), (2, 2), (3, 2), (4, 2), (5, 1), (6, 1), (7, 1)])
+@pytest.mark.parametrize("mode", ["trilinear", "stored"])
+def test_elastic_apply_both_modes(r, nx, mode):
+    m = M.regular_mesh(3, nx, r, 0.1, seed=6)
+    o = O.mesh(3, nx, r, 0.1, 6)
+    lam = 1.0 + (np.arange(m.E) % 3)
+    mu = 2.0 + (np.arange(m.E) % 2)
+    P = M.WaveProblem(m, 1.0, mu, lam=lam, ncomp=3)
+    P.set_geometry(mode)
+    Po = O.elastic_problem(o, lam, mu, 1.0)
+    u = np.random.default_rng(r).standard_normal(3 * m.N)
+    assert CS.rel_l2(P.apply_kh(u), -O.apply(Po, u)) < 1e-12
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp32_g64"])
@@ -116,6 +128,26 @@ def test_leapfrog_modes_agree(prec, tol):
         P.leapfrog_init(dt)
         tr = P.run(299)
         out[mode] = (tr, P.state()[1])
+    assert CS.rel_l2(out["trilinear"][0], out["stored"][0]) < tol
+    assert CS.rel_l2(out["trilinear"][1], out["stored"][1]) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_elastic_leapfrog_modes_agree(prec, tol):
+    m = M.regular_mesh(3, 3, 3, 0.1, seed=4)
+    out = {}
+    for mode in ("stored", "trilinear"):
+        P = M.WaveProblem(m, 1.0, M.layered_values(m, [1.0, 2.0, 1.5]),
+                          lam=M.layered_values(m, [2.0, 1.0, 3.0]), ncomp=3, nshots=2,
+                          precision=prec)
+        P.set_geometry(mode)
+        P.set_sources([(0.5, 0.5, 0.5), (0.4, 0.6, 0.55)], pol=[(0, 0, 1), (1, 0.5, 0)],
+                      shot=[0, 1])
+        P.set_receivers(np.array([[0.3, 0.4, 0.6], [0.7, 0.2, 0.5]]), comp=[0, 2])
+        dt = 1e-3
+        P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, 200))
+        P.leapfrog_init(dt)
+        out[mode] = (P.run(199), P.state()[1])
     assert CS.rel_l2(out["trilinear"][0], out["stored"][0]) < tol
     assert CS.rel_l2(out["trilinear"][1], out["stored"][1]) < tol
 
