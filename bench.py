@@ -74,6 +74,16 @@ def trilinear_bytes_per_step(E, nd, N, state_bytes, geom_bytes, ncoef=24):
     return E * (ncoef * geom_bytes + nd * 4) + N * 4 * state_bytes
 
 
+def kernel_name(tile, tri, n=6, t="float"):
+    """rocprofv3 name of the FP32 stiffness kernel a tile code selects."""
+    tr = "true" if tri else "false"
+    if tile > 1000:
+        return f"mjw::k_pipe3d<{n}, {tile - 1000}, {t}, {t}, 1, {tr}>"
+    if tile < 0:
+        return f"mjw::k_node3d<{n}, {-tile}, {t}, {t}, 1, false, {tr}>"
+    return f"mjw::k_scalar3d<{n}, {tile}, {t}, {t}, 1, false, false, {tr}>"
+
+
 def measured_traffic(name="stiffness_q5_48cube_counters.md"):
     """HBM bytes per colour pass of the headline kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (line `traffic_bytes_per_launch:`)."""
@@ -92,7 +102,8 @@ def flops_per_step(E, n):
     return E * nd * (12 * n + 18 + 6)
 
 
-def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10):
+def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10, tile=None,
+            nxcd=None):
     import montjoie_amd as M
     P = M.WaveProblem(m, 1.0, mu, precision=prec, nshots=nshots)
     srcs = [(0.5, 0.5, 0.5)] * nshots
@@ -100,6 +111,13 @@ def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10):
     P.set_receivers(receivers())
     total = warmup + steps + profile_steps
     P.set_pulse(M.ricker_pulse_table(f0, 0.0, dt, total))
+    if tile is None:
+        from montjoie_amd import tuner
+        tuned = tuner.tune_problem(P, steps=8, dt=dt)  # launch shape only: result-neutral
+    else:
+        P.set_tile(tile)
+        P.set_xcd_order(nxcd or 8)
+        tuned = {"tile": tile, "nxcd": nxcd or 8}
     P.leapfrog_init(dt)
     if warmup:
         P.advance(warmup, traces=False, graph=True)
@@ -114,7 +132,8 @@ def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10):
     info = P.info()
     geo = P.geometry
     P.close()
-    return dict(wall=wall, dev_ms_step=dev_ms, traces=tr, uN=uN, prof=prof, info=info, geometry=geo)
+    return dict(wall=wall, dev_ms_step=dev_ms, traces=tr, uN=uN, prof=prof, info=info, geometry=geo,
+                tile=tuned["tile"], nxcd=tuned["nxcd"])
 
 
 def shot_sources(nshots, seed=2027):
@@ -443,6 +462,9 @@ def main():
                     help="also run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--quick", action="store_true",
                     help="skip the 16-shot batch, the adjoint and the many-problem batch")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="stiffness tile code (default: autotuned per precision)")
+    ap.add_argument("--xcd", type=int, default=None, help="XCD block order with --tile")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the 64-problem uneven batch (prints its JSON)")
     ap.add_argument("--extra-only", choices=["batch", "elastic", "2d", "shots", "adjoint"],
@@ -486,11 +508,12 @@ def main():
         print("# calibration: 3 x 1 GiB device copies", file=sys.stderr, flush=True)
     res = {}
     for prec in ((a.only,) if a.only else ("fp64", "fp32_g64", "fp32")):
-        res[prec] = run_gpu(m, mu, dt, prec, a.steps, a.warmup)
+        res[prec] = run_gpu(m, mu, dt, prec, a.steps, a.warmup, tile=a.tile, nxcd=a.xcd)
         r = res[prec]
         print(f"# {prec}: {r['wall'] / a.steps * 1e3:.4f} ms/step wall, "
               f"{r['dev_ms_step']:.4f} ms/step device, stiffness launch "
-              f"{r['prof']['stiffness_launch_ms']:.4f} ms", file=sys.stderr, flush=True)
+              f"{r['prof']['stiffness_launch_ms']:.4f} ms, tile {r['tile']} xcd {r['nxcd']}",
+              file=sys.stderr, flush=True)
     if a.only:
         return
     ref = res["fp64"]
@@ -543,8 +566,8 @@ def main():
                      / 1e9 / HBM_PEAK_GBS,
                      "traffic_GBs": None if traffic is None else
                      traffic / (head["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9,
-                     "kernel": "mjw::k_scalar3d<6, 7, float, float, 1, false, false, %s> "
-                               "(one colour pass)" % ("true" if tri else "false"),
+                     "kernel": kernel_name(head["tile"], tri) + " (one colour pass)",
+                     "tile": head["tile"], "xcd_order": head["nxcd"],
                      "algorithmic_bytes_per_launch": launch_bytes,
                      "launch_ms": head["prof"]["stiffness_launch_ms"],
                      "fp64_launch_ms": ref["prof"]["stiffness_launch_ms"],

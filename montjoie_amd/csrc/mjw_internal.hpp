@@ -55,6 +55,9 @@ bool trilinear_coefficients(int order, int64_t E, const int32_t *elem_nodes, con
                             const int64_t *perm, const double *mu_sorted, double tol,
                             std::vector<double> &coef);
 enum GeoMode { kGeoStored = 0, kGeoTrilinear = 1 };
+// tile codes of the 3-D scalar kernels: EPB > 0 column kernel, -EPB one thread
+// per node, kPipeTile + EPB persistent column kernel
+constexpr int kPipeTile = 1000;
 
 struct StiffArgs {
   int dim, order, ncomp, epb;         // geometry, elements per block

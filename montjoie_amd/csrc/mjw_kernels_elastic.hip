@@ -21,39 +21,9 @@
 
 #include "mjw_device.hpp"
 #include "mjw_internal.hpp"
+#include "mjw_tri.hpp"
 
 namespace mjw {
-
-// columns of DF at layer k of one (i, j) column: c0 = p0 + x_k q0,
-// c1 = p1 + x_k q1, c2 fixed (trilinear map coefficients a1..a7)
-template <typename A>
-struct TriColE {
-  A p0[3], q0[3], p1[3], q1[3], c2[3], wij;
-};
-
-template <typename A>
-__device__ __forceinline__ void tri_m(const TriColE<A> &c, A zeta, A wk, A M[9]) {
-  A c0[3], c1[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    c0[d] = c.p0[d] + zeta * c.q0[d];
-    c1[d] = c.p1[d] + zeta * c.q1[d];
-  }
-  const A r0[3] = {c1[1] * c.c2[2] - c1[2] * c.c2[1], c1[2] * c.c2[0] - c1[0] * c.c2[2],
-                   c1[0] * c.c2[1] - c1[1] * c.c2[0]};
-  const A r1[3] = {c.c2[1] * c0[2] - c.c2[2] * c0[1], c.c2[2] * c0[0] - c.c2[0] * c0[2],
-                   c.c2[0] * c0[1] - c.c2[1] * c0[0]};
-  const A r2[3] = {c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2],
-                   c0[0] * c1[1] - c0[1] * c1[0]};
-  const A J = c0[0] * r0[0] + c0[1] * r0[1] + c0[2] * r0[2];
-  const A f = sqrt(c.wij * wk / J);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    M[a] = f * r0[a];
-    M[3 + a] = f * r1[a];
-    M[6 + a] = f * r2[a];
-  }
-}
 
 template <int N, int EPB, typename T, typename TG, int MODE, bool TRI>
 __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
@@ -77,25 +47,14 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
   const uint32_t g_ps = (uint32_t)(plane * sizeof(TG));
   int32_t gid[N];
   A lam = 0, mu = 0;
-  TriColE<A> tc;
+  TriCol<A> tc;
   if constexpr (TRI) {
     for (int q = t; q < EPB * kCoef; q += blockDim.x)
       lc[q] = eb + q / kCoef < e1 ? coef[eb * kCoef + q] : TG(0);
     __syncthreads();
     if (act) {
-      const TG *c = lc + el * kCoef;
-      const A xi = glxw[i], eta = glxw[j];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const A a1 = c[d], a2 = c[3 + d], a3 = c[6 + d], a4 = c[9 + d], a5 = c[12 + d],
-                a6 = c[15 + d], a7 = c[18 + d];
-        tc.p0[d] = a1 + a4 * eta;
-        tc.q0[d] = a6 + a7 * eta;
-        tc.p1[d] = a2 + a4 * xi;
-        tc.q1[d] = a5 + a7 * xi;
-        tc.c2[d] = a3 + a5 * eta + a6 * xi + a7 * (xi * eta);
-      }
-      tc.wij = (A)glxw[N + i] * (A)glxw[N + j];
+      tri_setup<A, TG>(lc + el * kCoef, (A)glxw[i], (A)glxw[j],
+                       (A)glxw[N + i] * (A)glxw[N + j], false, tc);
     }
   }
   if (act) {

@@ -25,8 +25,11 @@
 //    major state, each shot a contiguous vector).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mjw_device.hpp"
 #include "mjw_internal.hpp"
+#include "mjw_tri.hpp"
 
 namespace mjw {
 
@@ -34,40 +37,6 @@ template <typename T, typename TG>
 struct Mix {  // geometry product in the geometry's precision
   using A = TG;
 };
-
-// Per-thread constants of one (i, j) column of a trilinear hexahedron: the
-// columns of DF at (x_i, x_j, x_k) are c0 = p0 + x_k q0, c1 = p1 + x_k q1 and
-// c2 (independent of x_k); f = mu w_i w_j.
-template <typename A>
-struct TriCol {
-  A p0[3], q0[3], p1[3], q1[3], c2[3], f;
-};
-
-// G = mu w J DF^{-1} DF^{-T} at layer k: with r_b the rows of the adjugate
-// J DF^{-1} (r0 = c1 x c2, r1 = c2 x c0, r2 = c0 x c1), G_ab = mu w r_a.r_b / J.
-template <typename A>
-__device__ __forceinline__ void tri_geometry(const TriCol<A> &c, A zeta, A wk, A g[6]) {
-  A c0[3], c1[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    c0[d] = c.p0[d] + zeta * c.q0[d];
-    c1[d] = c.p1[d] + zeta * c.q1[d];
-  }
-  const A r0[3] = {c1[1] * c.c2[2] - c1[2] * c.c2[1], c1[2] * c.c2[0] - c1[0] * c.c2[2],
-                   c1[0] * c.c2[1] - c1[1] * c.c2[0]};
-  const A r1[3] = {c.c2[1] * c0[2] - c.c2[2] * c0[1], c.c2[2] * c0[0] - c.c2[0] * c0[2],
-                   c.c2[0] * c0[1] - c.c2[1] * c0[0]};
-  const A r2[3] = {c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2],
-                   c0[0] * c1[1] - c0[1] * c1[0]};
-  const A J = c0[0] * r0[0] + c0[1] * r0[1] + c0[2] * r0[2];
-  const A f = c.f * wk / J;
-  g[0] = f * (r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2]);
-  g[1] = f * (r0[0] * r1[0] + r0[1] * r1[1] + r0[2] * r1[2]);
-  g[2] = f * (r0[0] * r2[0] + r0[1] * r2[1] + r0[2] * r2[2]);
-  g[3] = f * (r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
-  g[4] = f * (r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2]);
-  g[5] = f * (r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
-}
 
 // Body of the 3-D scalar stiffness pass for the elements [eb, min(eb+EPB, e1))
 // of one colour; threads t >= EPB*n^2 of the block idle (they only meet the
@@ -106,19 +75,8 @@ __device__ __forceinline__ void scalar3d_body(
       lc[q] = eb + q / kCoef < e1 ? coef[eb * kCoef + q] : TG(0);
     __syncthreads();
     if (act) {
-      const TG *c = lc + el * kCoef;
-      const A xi = glxw[i], eta = glxw[j];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const A a1 = c[d], a2 = c[3 + d], a3 = c[6 + d], a4 = c[9 + d], a5 = c[12 + d],
-                a6 = c[15 + d], a7 = c[18 + d];
-        tc.p0[d] = a1 + a4 * eta;
-        tc.q0[d] = a6 + a7 * eta;
-        tc.p1[d] = a2 + a4 * xi;
-        tc.q1[d] = a5 + a7 * xi;
-        tc.c2[d] = a3 + a5 * eta + a6 * xi + a7 * (xi * eta);
-      }
-      tc.f = (A)c[21] * ((A)glxw[N + i] * (A)glxw[N + j]);
+      tri_setup<A, TG>(lc + el * kCoef, (A)glxw[i], (A)glxw[j],
+                       (A)glxw[N + i] * (A)glxw[N + j], true, tc);
     }
   }
   if (act) {
@@ -127,7 +85,7 @@ __device__ __forceinline__ void scalar3d_body(
       gid[k] = bld<int32_t>(rmap, lo * 4u, k * map_ps);
       if constexpr (HOIST) {
         if constexpr (TRI) {
-          tri_geometry<A>(tc, (A)glxw[k], (A)glxw[N + k], g6[k]);
+          tri_tensor<A>(tc, (A)glxw[k], (A)glxw[N + k], g6[k]);
         } else {
           const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
 #pragma unroll
@@ -169,20 +127,27 @@ __device__ __forceinline__ void scalar3d_body(
           a3 += Dm[k * N + l] * u[l];
         }
         const A b1 = a1, b2 = a2, b3 = a3;
-        A gk[6];
-        if constexpr (HOIST) {
-#pragma unroll
-          for (int c = 0; c < 6; ++c) gk[c] = g6[HOIST ? k : 0][c];
-        } else if constexpr (TRI) {
-          tri_geometry<A>(tc, (A)glxw[k], (A)glxw[N + k], gk);
+        if constexpr (TRI && !HOIST) {
+          A w1, w2, w3;
+          tri_apply<A>(tc, (A)glxw[k], (A)glxw[N + k], b1, b2, b3, w1, w2, w3);
+          v1[k] = (T)w1;
+          v2[k] = (T)w2;
+          v3[k] = (T)w3;
         } else {
-          const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
+          A gk[6];
+          if constexpr (HOIST) {
 #pragma unroll
-          for (int c = 0; c < 6; ++c) gk[c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
+            for (int c = 0; c < 6; ++c) gk[c] = g6[HOIST ? k : 0][c];
+          } else {
+            const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+              gk[c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
+          }
+          v1[k] = (T)(gk[0] * b1 + gk[1] * b2 + gk[2] * b3);
+          v2[k] = (T)(gk[1] * b1 + gk[3] * b2 + gk[4] * b3);
+          v3[k] = (T)(gk[2] * b1 + gk[4] * b2 + gk[5] * b3);
         }
-        v1[k] = (T)(gk[0] * b1 + gk[1] * b2 + gk[2] * b3);
-        v2[k] = (T)(gk[1] * b1 + gk[3] * b2 + gk[4] * b3);
-        v3[k] = (T)(gk[2] * b1 + gk[4] * b2 + gk[5] * b3);
       }
     }
     __syncthreads();
@@ -380,6 +345,162 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
   }
 }
 
+// -------------------------------------------------------- persistent column
+// One shot, n >= 5.  The grid holds only as many blocks as fit on the chip;
+// each block walks a contiguous run of groups of EPB elements and issues the
+// element-map and state gathers of the next group while it computes the
+// current one, so the two dependent memory round trips of a group (map, then
+// U) overlap the arithmetic instead of being paid once per resident block.
+// Same operand order as the column kernel (bitwise equal results).  LDS:
+// lu = staged U, l1 / l2 = the xi_1 / xi_2 fluxes (separate arrays so a
+// group needs two barriers), lc = trilinear coefficients, sD = D.
+template <int N, int EPB, typename T, typename TG, int MODE, bool TRI>
+__global__ __launch_bounds__(EPB *N *N) void k_pipe3d(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
+    const T *__restrict__ sdiag, T alpha, T beta, int nxcd, const TG *__restrict__ coef,
+    const TG *__restrict__ glxw) {
+  constexpr int P = N * N, NN = P * N, NT = EPB * P;
+  static_assert(EPB * kCoef <= NT, "one coefficient per thread");
+  using A = TG;
+  __shared__ T lu[EPB * NN], l1[EPB * NN], l2[EPB * NN];
+  __shared__ T sD[N * N];
+  __shared__ TG lc[TRI ? EPB * kCoef : 1];
+  const int t = threadIdx.x;
+  const int el = t / P, ij = t - el * P;
+  const int i = ij % N, j = ij / N;
+  // contiguous, balanced run of groups per block; XCD-aware block order
+  const int64_t ngrp = (e1 - e0 + EPB - 1) / EPB;
+  const int64_t nb = gridDim.x, lb = xcd_block(blockIdx.x, gridDim.x, nxcd);
+  const int64_t q = ngrp / nb, r = ngrp % nb;
+  const int64_t g0 = lb * q + (lb < r ? lb : r), g1 = g0 + q + (lb < r ? 1 : 0);
+  const int64_t plane = E * P;
+  const uint32_t lo = (uint32_t)(el * P + ij);
+  const uint32_t map_ps = (uint32_t)(plane * 4);
+  const uint32_t g_ps = (uint32_t)(plane * sizeof(TG));
+  const rsrc_t ru = mkres(U), ry = mkres(Y), rs = mkres(sdiag);
+  for (int qq = t; qq < N * N; qq += NT) sD[qq] = Dm[qq];
+  int32_t gid[N], gidn[N];
+  T u[N], un[N];
+  TG cn = 0;
+  if (g0 < g1) {
+    const int64_t eb = e0 + g0 * EPB;
+    if (eb + el < e1) {
+      const rsrc_t rm = mkres(map + eb * P);
+#pragma unroll
+      for (int k = 0; k < N; ++k) gidn[k] = bld<int32_t>(rm, lo * 4u, k * map_ps);
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        un[k] = bld<T>(ru, (uint32_t)(gidn[k] & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
+    }
+    if constexpr (TRI)
+      if (t < EPB * kCoef) cn = eb + t / kCoef < e1 ? coef[eb * kCoef + t] : TG(0);
+  }
+  for (int64_t g = g0; g < g1; ++g) {
+    const int64_t eb = e0 + g * EPB;
+    const bool act = eb + el < e1;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      gid[k] = gidn[k];
+      u[k] = un[k];
+    }
+    T *su = lu + el * NN;
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) su[k * P + ij] = u[k];
+    }
+    if constexpr (TRI)
+      if (t < EPB * kCoef) lc[t] = cn;
+    // this group's previous values and dt^2/M (read before any store to Y)
+    T yk[N], sk[MODE == kLeap ? N : 1];
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const uint32_t off = (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T);
+        yk[k] = bld<T>(ry, off, 0u);
+        if constexpr (MODE == kLeap) sk[k] = bld<T>(rs, off, 0u);
+      }
+    }
+    // next group's element map (and coefficients)
+    const bool nxt = g + 1 < g1;
+    const int64_t ebn = eb + EPB;
+    const bool actn = nxt && ebn + el < e1;
+    if (actn) {
+      const rsrc_t rm = mkres(map + ebn * P);
+#pragma unroll
+      for (int k = 0; k < N; ++k) gidn[k] = bld<int32_t>(rm, lo * 4u, k * map_ps);
+    }
+    if constexpr (TRI)
+      if (nxt && t < EPB * kCoef) cn = ebn + t / kCoef < e1 ? coef[ebn * kCoef + t] : TG(0);
+    __syncthreads();
+    T v3[N];
+    if (act) {
+      TriCol<A> tc;
+      if constexpr (TRI) {
+        tri_setup<A, TG>(lc + el * kCoef, (A)glxw[i], (A)glxw[j],
+                         (A)glxw[N + i] * (A)glxw[N + j], true, tc);
+      }
+      T *s1 = l1 + el * NN, *s2 = l2 + el * NN;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        T a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          a1 += sD[i * N + l] * su[k * P + j * N + l];
+          a2 += sD[j * N + l] * su[k * P + l * N + i];
+          a3 += sD[k * N + l] * u[l];
+        }
+        const A b1 = a1, b2 = a2, b3 = a3;
+        if constexpr (TRI) {
+          A w1, w2, w3;
+          tri_apply<A>(tc, (A)glxw[k], (A)glxw[N + k], b1, b2, b3, w1, w2, w3);
+          s1[k * P + ij] = (T)w1;
+          s2[k * P + ij] = (T)w2;
+          v3[k] = (T)w3;
+        } else {
+          A gk[6];
+          const rsrc_t rg = mkres(G + (k * 6) * plane + eb * P);
+#pragma unroll
+          for (int c = 0; c < 6; ++c) gk[c] = (A)bld<TG>(rg, lo * (uint32_t)sizeof(TG), c * g_ps);
+          s1[k * P + ij] = (T)(gk[0] * b1 + gk[1] * b2 + gk[2] * b3);
+          s2[k * P + ij] = (T)(gk[1] * b1 + gk[3] * b2 + gk[4] * b3);
+          v3[k] = (T)(gk[2] * b1 + gk[4] * b2 + gk[5] * b3);
+        }
+      }
+    }
+    // next group's state gathers
+    if (actn) {
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        un[k] = bld<T>(ru, (uint32_t)(gidn[k] & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
+    }
+    __syncthreads();
+    if (act) {
+      const T *s1 = l1 + el * NN, *s2 = l2 + el * NN;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        T o = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          o += sD[l * N + i] * s1[k * P + j * N + l];
+          o += sD[l * N + j] * s2[k * P + l * N + i];
+          o += sD[l * N + k] * v3[l];
+        }
+        const int32_t m = gid[k];
+        const uint32_t off = (uint32_t)(m & 0x7fffffff) * (uint32_t)sizeof(T);
+        const bool first = m < 0;
+        const T y = yk[k];
+        T rr;
+        if constexpr (MODE == kLeap)
+          rr = first ? (T(2) * u[k] - y - sk[k] * o) : (y - sk[k] * o);
+        else
+          rr = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
+        bst<T>(rr, ry, off, 0u);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------- thread per node
 // 3-D scalar pass with one thread per element node (EPB elements, EPB*n^3
 // threads per block): every node's loads are independent, so a colour pass
@@ -414,6 +535,7 @@ __global__ __launch_bounds__(EPB *N *N *N) void k_node3d(
   }
   int32_t gid = 0;
   A g[6] = {0, 0, 0, 0, 0, 0};
+  TriCol<A> tc;
   if (act) {
     gid = bld<int32_t>(mkres(map + eb * P), lo * 4u, (uint32_t)(k * plane * 4));
     if constexpr (!TRI) {
@@ -425,23 +547,9 @@ __global__ __launch_bounds__(EPB *N *N *N) void k_node3d(
   }
   __syncthreads();
   if constexpr (TRI) {
-    if (act) {
-      const TG *c = lc + el * kCoef;
-      const A xi = glxw[i], eta = glxw[j];
-      TriCol<A> tc;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const A a1 = c[d], a2 = c[3 + d], a3 = c[6 + d], a4 = c[9 + d], a5 = c[12 + d],
-                a6 = c[15 + d], a7 = c[18 + d];
-        tc.p0[d] = a1 + a4 * eta;
-        tc.q0[d] = a6 + a7 * eta;
-        tc.p1[d] = a2 + a4 * xi;
-        tc.q1[d] = a5 + a7 * xi;
-        tc.c2[d] = a3 + a5 * eta + a6 * xi + a7 * (xi * eta);
-      }
-      tc.f = (A)c[21] * ((A)glxw[N + i] * (A)glxw[N + j]);
-      tri_geometry<A>(tc, (A)glxw[k], (A)glxw[N + k], g);
-    }
+    if (act)
+      tri_setup<A, TG>(lc + el * kCoef, (A)glxw[i], (A)glxw[j],
+                       (A)glxw[N + i] * (A)glxw[N + j], true, tc);
   }
   const uint32_t off = (uint32_t)(gid & 0x7fffffff) * (uint32_t)sizeof(T);
   const bool first = gid < 0;
@@ -471,9 +579,17 @@ __global__ __launch_bounds__(EPB *N *N *N) void k_node3d(
         a3 += sD[k * N + l] * su[l * P + ij];
       }
       const A b1 = a1, b2 = a2, b3 = a3;
-      l1[lq] = (T)(g[0] * b1 + g[1] * b2 + g[2] * b3);
-      l2[lq] = (T)(g[1] * b1 + g[3] * b2 + g[4] * b3);
-      l3[lq] = (T)(g[2] * b1 + g[4] * b2 + g[5] * b3);
+      if constexpr (TRI) {
+        A w1, w2, w3;
+        tri_apply<A>(tc, (A)glxw[k], (A)glxw[N + k], b1, b2, b3, w1, w2, w3);
+        l1[lq] = (T)w1;
+        l2[lq] = (T)w2;
+        l3[lq] = (T)w3;
+      } else {
+        l1[lq] = (T)(g[0] * b1 + g[1] * b2 + g[2] * b3);
+        l2[lq] = (T)(g[1] * b1 + g[3] * b2 + g[4] * b3);
+        l3[lq] = (T)(g[2] * b1 + g[4] * b2 + g[5] * b3);
+      }
     }
     __syncthreads();
     if (act) {
@@ -681,6 +797,55 @@ static int run_node(const StiffArgs &a, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+template <int N, int EPB, typename T, typename TG, int MODE, bool TRI>
+static int launch_pipe(const StiffArgs &a, hipStream_t st, int64_t ngrp) {
+  // resident blocks on the device (cached per kernel and device)
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return -1004;
+  if (!cached[dev]) {
+    int per_cu = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&k_pipe3d<N, EPB, T, TG, MODE, TRI>),
+            EPB * N * N, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return -1004;
+    cached[dev] = std::max(1, per_cu) * std::max(1, ncu);
+  }
+  const unsigned blocks = (unsigned)std::min<int64_t>(ngrp, cached[dev]);
+  hipLaunchKernelGGL((k_pipe3d<N, EPB, T, TG, MODE, TRI>), dim3(blocks), dim3(EPB * N * N), 0, st,
+                     a.map, static_cast<const TG *>(a.G), static_cast<const T *>(a.Dm), a.E, a.e0,
+                     a.e1, static_cast<const T *>(a.U), static_cast<T *>(a.Y),
+                     static_cast<const T *>(a.sdiag), (T)a.alpha, (T)a.beta, a.nxcd,
+                     static_cast<const TG *>(a.coef), static_cast<const TG *>(a.glxw));
+  return (int)hipGetLastError();
+}
+
+// persistent column kernel for one shot; other cases (several shots, the
+// adjoint step) take the column kernel with the same tile
+template <int N, int EPB, typename T, typename TG>
+static int run_pipe(const StiffArgs &a, hipStream_t st) {
+  const int64_t ne = a.e1 - a.e0;
+  if (ne <= 0) return 0;
+  if (a.dim != 3 || a.ncomp != 1) return -1003;
+  if (a.grad || a.nshots != 1) return run_scalar<N, EPB, T, TG>(a, st);
+  const int64_t ngrp = (ne + EPB - 1) / EPB;
+  const bool tri = a.geo == kGeoTrilinear;
+  if (tri && (!a.coef || !a.glxw)) return -1002;
+  if (a.mode == kLeap)
+    return tri ? launch_pipe<N, EPB, T, TG, kLeap, true>(a, st, ngrp)
+               : launch_pipe<N, EPB, T, TG, kLeap, false>(a, st, ngrp);
+  return tri ? launch_pipe<N, EPB, T, TG, kApply, true>(a, st, ngrp)
+             : launch_pipe<N, EPB, T, TG, kApply, false>(a, st, ngrp);
+}
+
+template <int N, int EPB>
+static int run_pipe_prec(const StiffArgs &a, hipStream_t st) {
+  if (a.state == kF64) return run_pipe<N, EPB, double, double>(a, st);
+  if (a.geom == kG64) return run_pipe<N, EPB, float, double>(a, st);
+  return run_pipe<N, EPB, float, float>(a, st);
+}
+
 template <int N, int EPB>
 static int run_node_prec(const StiffArgs &a, hipStream_t st) {
   if (a.state == kF64) return run_node<N, EPB, double, double>(a, st);
@@ -705,6 +870,8 @@ static int run_prec(const StiffArgs &a, hipStream_t st) {
 #define MJW_NODE_TILES3(X) \
   X(2, 32) X(2, 64) X(3, 8) X(3, 16) X(4, 4) X(4, 8) X(5, 2) X(5, 4) X(6, 1) X(6, 2) X(6, 4) \
   X(7, 1) X(7, 2) X(8, 1) X(8, 2)
+// persistent column tiles (listed as kPipeTile + EPB), n >= 5
+#define MJW_PIPE_TILES3(X) X(5, 5) X(5, 10) X(6, 7) X(6, 14) X(7, 5) X(7, 10) X(8, 4)
 #define MJW_TILES2(X) \
   X(2, 64) X(2, 16) X(3, 28) X(3, 7) X(4, 16) X(4, 4) X(5, 10) X(5, 20) X(6, 7) X(6, 14) \
   X(7, 5) X(7, 10) X(8, 4) X(8, 1)
@@ -716,14 +883,18 @@ int launch_stiffness_scalar(const StiffArgs &a, void *stream) {
   if (n == NN && a.epb == EE) return run_prec<NN, EE>(a, st);
 #define XN(NN, EE) \
   if (n == NN && a.epb == -EE) return run_node_prec<NN, EE>(a, st);
+#define XP(NN, EE) \
+  if (n == NN && a.epb == kPipeTile + EE) return run_pipe_prec<NN, EE>(a, st);
   if (a.dim == 3) {
     MJW_TILES3(X)
     MJW_NODE_TILES3(XN)
+    MJW_PIPE_TILES3(XP)
   } else {
     MJW_TILES2(X)
   }
 #undef X
 #undef XN
+#undef XP
   return -1000;
 }
 
@@ -734,14 +905,18 @@ std::vector<int> tile_choices_scalar(int dim, int order) {
   if (n == NN) v.push_back(EE);
 #define XN(NN, EE) \
   if (n == NN) v.push_back(-EE);
+#define XP(NN, EE) \
+  if (n == NN) v.push_back(kPipeTile + EE);
   if (dim == 3) {
     MJW_TILES3(X)
     MJW_NODE_TILES3(XN)
+    MJW_PIPE_TILES3(XP)
   } else {
     MJW_TILES2(X)
   }
 #undef X
 #undef XN
+#undef XP
   return v;
 }
 

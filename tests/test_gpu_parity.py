@@ -121,7 +121,7 @@ def test_every_tile_gives_identical_results():
     P = CS.gpu_problem(c, "fp64")
     u = np.random.default_rng(1).standard_normal(P.ndof)
     outs = []
-    for epb in (7, 14, 16, -1, -2, -4):   # column tiles, then one thread per node
+    for epb in (7, 14, 16, -1, -2, -4, 1007, 1014):  # column, thread per node, persistent
         P.set_tile(epb)
         outs.append(P.apply_kh(u))
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
