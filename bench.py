@@ -316,6 +316,32 @@ def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028,
             "setup_s": t_setup, "autotuned": tuned and {"order_mode": tuned["order_mode"]}}
 
 
+def bench_host_apply(m, mu, reps=5):
+    """ApplyOperatorKh through host FP64 vectors (mjw_apply_kh: upload U,
+    apply, download Y), the PCIe-inclusive rate of the operator boundary,
+    against the same apply on device-resident vectors (mjw_apply_kh_dev)."""
+    import montjoie_amd as M
+    P = M.WaveProblem(m, 1.0, mu, precision="fp64")
+    u = np.random.default_rng(0).standard_normal(m.N)
+    P.apply_kh(u)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        P.apply_kh(u)
+    host = (time.perf_counter() - t0) / reps
+    U = torch.tensor(u, device="cuda")
+    Y = torch.zeros_like(U)
+    P.apply_kh_dev(U.data_ptr(), Y.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        P.apply_kh_dev(U.data_ptr(), Y.data_ptr())
+    torch.cuda.synchronize()
+    dev = (time.perf_counter() - t0) / reps
+    P.close()
+    return {"precision": "fp64", "host_vectors_ms": host * 1e3, "device_vectors_ms": dev * 1e3,
+            "host_gdof_s": m.N / host / 1e9, "device_gdof_s": m.N / dev / 1e9}
+
+
 def sphere_dirs(n, seed=2027):
     """n unit vectors, uniform on the sphere (z ~ U[-1,1], phi ~ U[0,2pi))."""
     out = []
@@ -575,6 +601,8 @@ def main():
                      (ref["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9},
         "cpu_baseline": None,
     }
+    out["host_operator"] = bench_host_apply(m, mu)
+    print(f"# host operator: {out['host_operator']}", file=sys.stderr, flush=True)
     if not a.quick:
         out["batch_16_shots"] = bench_shots(m, mu, dt, a.steps, a.warmup)
         print(f"# 16 shots: {out['batch_16_shots']}", file=sys.stderr, flush=True)
