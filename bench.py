@@ -445,7 +445,7 @@ def cpu_reference(nx=24, order=5, steps=150, dt=1e-3):
         return None
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     from oracle import oracle as O
-    import make_ref_fixtures as F
+    import decks as F
     with tempfile.TemporaryDirectory(dir=ROOT, prefix=".mjref_") as d:
         m = O.mesh(3, nx, order, 0.1, 2024)
         c = [1.0 + 2.0 * _unif(2025, k) for k in range(3)]

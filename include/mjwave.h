@@ -171,6 +171,10 @@ int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, const double *p
  * VarInstationary.cxx:2394).  table[nsteps] FP64, same for all shots.
  */
 int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table);
+/* Gauss-Lobatto rule on [0,1] of `order` (order+1 points, weights sum 1):
+ * Globatto::ConstructQuadrature(r, QUADRATURE_LOBATTO)
+ * (Quadrature/GaussLobatto.cxx:29-75).  Host only, FP64, order 1..24. */
+int mjw_gauss_lobatto(int order, double *points, double *weights);
 /*
  * Ricker pulse table through Montjoie's degree-24 Gauss-Lobatto interpolator
  * of the Ricker primitive (TimeSourceHyperbolic, Source/TimeSource.cxx:

@@ -32,6 +32,7 @@ SIGNATURES = {
     "mjw_set_geometry_mode": (C.c_int, [_vp, C.c_int]),
     "mjw_get_geometry_mode": (C.c_int, [_vp, _i32p, _i32p]),
     "mjw_node_order": (C.c_int, [_vp, _i32p]),
+    "mjw_gauss_lobatto": (C.c_int, [C.c_int, _dp, _dp]),
     "mjw_info": (C.c_int, [_vp, _i64p]),
     "mjw_set_tile": (C.c_int, [_vp, C.c_int]),
     "mjw_get_tile": (C.c_int, [_vp, C.POINTER(C.c_int)]),

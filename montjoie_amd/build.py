@@ -18,6 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmjwave.so")
+RUNNER = os.path.join(HERE, "mjwave_run")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 ARCH = os.environ.get("MJW_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -59,6 +61,11 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lgomp"])
     os.replace(tmp, LIB)
+    # deck driver (host C++ over the C ABI)
+    tmp = RUNNER + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, os.path.join(HERE, "driver",
+          "mjwave_run.cpp"), "-o", tmp, "-L", HERE, "-lmjwave", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, RUNNER)
     return LIB
 
 

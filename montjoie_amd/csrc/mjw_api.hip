@@ -166,6 +166,13 @@ extern "C" int mjw_mesh_regular(int dim, int nx, int order, double pert, uint64_
   return MJW_OK;
 }
 
+extern "C" int mjw_gauss_lobatto(int order, double *points, double *weights) {
+  if (order < 1 || order > 24 || !points || !weights)
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_gauss_lobatto: order in 1..24, non-NULL arrays");
+  gl_rule(order, points, weights);
+  return MJW_OK;
+}
+
 extern "C" int mjw_ricker_pulse_table(double f0, double t_begin, double dt, int nsteps,
                                       double *out) {
   if (!(f0 > 0) || !(dt > 0) || nsteps < 0 || (!out && nsteps > 0))

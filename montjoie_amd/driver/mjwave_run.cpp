@@ -1,0 +1,657 @@
+// mjwave_run -- runs a Montjoie explicit wave deck (.ini data file) on the
+// MI355X through the C ABI of mjwave.h.
+//
+//   mjwave_run deck.ini [--precision fp64|fp32|fp32_g64] [--device N]
+//                       [--dump-mesh FILE]
+//
+// Replaces, for the explicit leapfrog path, Montjoie's driver chain
+// (Program/Beginner/acous3D.cc -> HyperbolicProblem::RunAll,
+// Instationary/VarInstationary.cxx:1629-1831): read the data file
+// (Output/OutputHarmonic.cxx keyword parsing), build the mesh (REGULAR,
+// Mesh/Mesh2D.cxx:5095-5135, Mesh/Mesh3D.cxx:10468-10514, or a Medit .mesh
+// file, Mesh/Mesh3D.cxx:1027-1060,3734-3770), number the Q_r nodes
+// (Mesh/NumberMap.cxx), apply the media, Dirichlet references, Dirac source
+// with a Ricker pulse and SismoPoint receivers, run RunSecondOrderScheme
+// (ModifiedEquation.cxx:16-53) and write the point seismograms in
+// Montjoie's text format (one row per output time: t then the values, %.15e,
+// tab-separated; Output/GridInterpolation.cxx:3150-3152 sets precision 15).
+//
+// Keywords read: TypeElement (QUADRANGLE_LOBATTO, HEXAHEDRON_LOBATTO),
+// TypeEquation (ACOUSTIC, TIME_ELASTIC), MeshPath, FileMesh, TypeMesh,
+// OrderDiscretization, ConditionReference (DIRICHLET), MateriauDielec
+// (ISOTROPE), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
+// TemporalSource (RICKER), TimeStep, TimeInterval, OrderTimeScheme
+// (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
+// ParametersOutputPoint.  Other keywords are reported and ignored; a value
+// this path does not implement (another scheme, source or condition) stops
+// the run with a message, as Montjoie's own parser does (cout + abort).
+//
+// Elastic decks: Montjoie's rows hold the displacement of every point (3
+// values per point, point-major) followed by 9 further fields per point;
+// this driver writes the displacement part.
+//
+// --dump-mesh writes the assembled mesh (sizes, node positions, element ->
+// node map in local order q = i + n j + n^2 k, Dirichlet flags, media) as raw
+// little-endian arrays and exits without touching the GPU.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <set>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "mjwave.h"
+
+namespace {
+
+[[noreturn]] void die(const std::string &m) {
+  std::fprintf(stderr, "mjwave_run: %s\n", m.c_str());
+  std::exit(1);
+}
+
+void check(int rc, const mjw_problem *p, const char *what) {
+  if (rc != MJW_OK) die(std::string(what) + ": " + mjw_last_error(p));
+}
+
+std::vector<std::string> split(const std::string &s) {
+  std::istringstream is(s);
+  std::vector<std::string> v;
+  std::string w;
+  while (is >> w) v.push_back(w);
+  return v;
+}
+
+double num(const std::string &s) {
+  char *end = nullptr;
+  const double v = std::strtod(s.c_str(), &end);
+  if (end == s.c_str()) die("not a number: " + s);
+  return v;
+}
+
+// ----------------------------------------------------------------- deck
+struct Deck {
+  std::multimap<std::string, std::vector<std::string>> kv;
+  std::string dir;
+  const std::vector<std::string> *get(const std::string &k) const {
+    auto it = kv.find(k);
+    return it == kv.end() ? nullptr : &it->second;
+  }
+  std::vector<std::vector<std::string>> all(const std::string &k) const {
+    std::vector<std::vector<std::string>> out;
+    auto r = kv.equal_range(k);
+    for (auto it = r.first; it != r.second; ++it) out.push_back(it->second);
+    return out;
+  }
+  const std::vector<std::string> &need(const std::string &k) const {
+    const auto *v = get(k);
+    if (!v || v->empty()) die("keyword " + k + " is missing");
+    return *v;
+  }
+};
+
+Deck read_deck(const std::string &path) {
+  std::ifstream in(path);
+  if (!in) die("cannot open " + path);
+  Deck d;
+  const size_t sl = path.find_last_of('/');
+  d.dir = sl == std::string::npos ? "." : path.substr(0, sl);
+  std::string line;
+  while (std::getline(in, line)) {
+    const size_t h = line.find('#');
+    if (h != std::string::npos) line = line.substr(0, h);
+    const size_t eq = line.find('=');
+    if (eq == std::string::npos) continue;
+    std::vector<std::string> key = split(line.substr(0, eq));
+    if (key.size() != 1) continue;
+    d.kv.emplace(key[0], split(line.substr(eq + 1)));
+  }
+  return d;
+}
+
+// ----------------------------------------------------------------- mesh
+// vertices + Q1 elements (quadrilaterals / hexahedra in Medit corner order)
+// + boundary faces (edges / quadrilaterals) with references
+struct Q1Mesh {
+  int dim = 3;
+  std::vector<double> vert;               // nv * dim
+  std::vector<int64_t> elem;              // E * 2^dim vertex ids
+  std::vector<int> elem_ref;
+  std::vector<std::vector<int64_t>> bnd;  // boundary faces (vertex ids)
+  std::vector<int> bnd_ref;
+};
+
+Q1Mesh regular_mesh(int dim, const std::vector<std::string> &p) {
+  // FileMesh = REGULAR nbx [xmin xmax ymin ymax [zmin zmax]] [ref_domain ref_bnd...]
+  Q1Mesh m;
+  m.dim = dim;
+  int nbx = (int)num(p.at(1));
+  if (nbx <= 1) nbx = 2;
+  double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+  const size_t nb = p.size();
+  const size_t need_box = dim == 3 ? 8 : 6;
+  if (nb >= need_box)
+    for (int a = 0; a < dim; ++a) {
+      lo[a] = num(p[2 + 2 * a]);
+      hi[a] = num(p[3 + 2 * a]);
+    }
+  int npt[3] = {nbx, nbx, nbx};
+  for (int a = 1; a < dim; ++a) {
+    const double ratio = (hi[a] - lo[a]) / (hi[0] - lo[0]) * nbx;
+    // 2-D rounds up (Mesh2D.cxx:5109-5110), 3-D to the nearest (Mesh3D.cxx:10486-10487)
+    npt[a] = dim == 2 ? (int)std::ceil(ratio) : (int)std::lround(ratio);
+    if (npt[a] <= 1) npt[a] = 2;
+  }
+  int ref_dom = 1;
+  std::vector<int> rb(2 * dim, 3);
+  const size_t first_ref = need_box;
+  if (nb >= first_ref + 2) {
+    if (dim == 2) ref_dom = (int)num(p[first_ref]);  // 3-D always uses domain 1
+    const int r0 = (int)num(p[first_ref + 1]);
+    std::fill(rb.begin(), rb.end(), r0);
+    if (nb >= first_ref + 1 + 2 * dim)
+      for (int k = 1; k < 2 * dim; ++k) rb[k] = (int)num(p[first_ref + 1 + k]);
+  }
+  const int nx = npt[0] - 1, ny = npt[1] - 1, nz = dim == 3 ? npt[2] - 1 : 0;
+  auto coord = [&](int a, int i) {
+    return i == npt[a] - 1 ? hi[a] : lo[a] + i * (hi[a] - lo[a]) / (npt[a] - 1);
+  };
+  const int pz = dim == 3 ? npt[2] : 1;
+  for (int k = 0; k < pz; ++k)
+    for (int j = 0; j < npt[1]; ++j)
+      for (int i = 0; i < npt[0]; ++i) {
+        m.vert.push_back(coord(0, i));
+        m.vert.push_back(coord(1, j));
+        if (dim == 3) m.vert.push_back(coord(2, k));
+      }
+  auto vid = [&](int i, int j, int k) {
+    return (int64_t)i + (int64_t)npt[0] * (j + (int64_t)npt[1] * k);
+  };
+  if (dim == 2) {
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        for (int64_t v : {vid(i, j, 0), vid(i + 1, j, 0), vid(i + 1, j + 1, 0), vid(i, j + 1, 0)})
+          m.elem.push_back(v);
+        m.elem_ref.push_back(ref_dom);
+      }
+    // boundary references: y = ymin, x = xmax, y = ymax, x = xmin (Mesh2D.cxx:4856-4873)
+    for (int i = 0; i < nx; ++i) {
+      m.bnd.push_back({vid(i, 0, 0), vid(i + 1, 0, 0)});
+      m.bnd_ref.push_back(rb[0]);
+      m.bnd.push_back({vid(i, ny, 0), vid(i + 1, ny, 0)});
+      m.bnd_ref.push_back(rb[2]);
+    }
+    for (int j = 0; j < ny; ++j) {
+      m.bnd.push_back({vid(nx, j, 0), vid(nx, j + 1, 0)});
+      m.bnd_ref.push_back(rb[1]);
+      m.bnd.push_back({vid(0, j, 0), vid(0, j + 1, 0)});
+      m.bnd_ref.push_back(rb[3]);
+    }
+  } else {
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          for (int64_t v : {vid(i, j, k), vid(i + 1, j, k), vid(i + 1, j + 1, k), vid(i, j + 1, k),
+                            vid(i, j, k + 1), vid(i + 1, j, k + 1), vid(i + 1, j + 1, k + 1),
+                            vid(i, j + 1, k + 1)})
+            m.elem.push_back(v);
+          m.elem_ref.push_back(1);
+        }
+    // boundary references: x = xmin, y = ymin, z = zmin, z = zmax, y = ymax,
+    // x = xmax (Mesh3D.cxx:8700-8840)
+    auto quad = [&](int64_t a, int64_t b, int64_t c, int64_t d, int ref) {
+      m.bnd.push_back({a, b, c, d});
+      m.bnd_ref.push_back(ref);
+    };
+    for (int b = 0; b < std::max(ny, nz); ++b)
+      for (int a = 0; a < std::max(nx, ny); ++a) {
+        if (a < ny && b < nz) {
+          quad(vid(0, a, b), vid(0, a + 1, b), vid(0, a + 1, b + 1), vid(0, a, b + 1), rb[0]);
+          quad(vid(nx, a, b), vid(nx, a + 1, b), vid(nx, a + 1, b + 1), vid(nx, a, b + 1), rb[5]);
+        }
+        if (a < nx && b < nz) {
+          quad(vid(a, 0, b), vid(a + 1, 0, b), vid(a + 1, 0, b + 1), vid(a, 0, b + 1), rb[1]);
+          quad(vid(a, ny, b), vid(a + 1, ny, b), vid(a + 1, ny, b + 1), vid(a, ny, b + 1), rb[4]);
+        }
+      }
+    for (int b = 0; b < ny; ++b)
+      for (int a = 0; a < nx; ++a) {
+        quad(vid(a, b, 0), vid(a + 1, b, 0), vid(a + 1, b + 1, 0), vid(a, b + 1, 0), rb[2]);
+        quad(vid(a, b, nz), vid(a + 1, b, nz), vid(a + 1, b + 1, nz), vid(a, b + 1, nz), rb[3]);
+      }
+  }
+  return m;
+}
+
+Q1Mesh medit_mesh(int dim, const std::string &path) {
+  std::ifstream in(path);
+  if (!in) die("cannot open mesh file " + path);
+  Q1Mesh m;
+  m.dim = dim;
+  std::string w;
+  auto read_int = [&]() -> int64_t {
+    int64_t v;
+    if (!(in >> v)) die("bad Medit file " + path);
+    return v;
+  };
+  while (in >> w) {
+    if (w == "Dimension") {
+      const int d = (int)read_int();
+      if (d != dim) die("mesh dimension does not match TypeElement");
+    } else if (w == "Vertices") {
+      const int64_t n = read_int();
+      m.vert.resize(n * dim);
+      for (int64_t i = 0; i < n; ++i) {
+        for (int a = 0; a < dim; ++a) in >> m.vert[i * dim + a];
+        read_int();  // reference
+      }
+    } else if (w == "Hexahedra" || (w == "Quadrilaterals" && dim == 2)) {
+      const int64_t n = read_int();
+      const int nv = dim == 3 ? 8 : 4;
+      for (int64_t e = 0; e < n; ++e) {
+        for (int c = 0; c < nv; ++c) m.elem.push_back(read_int() - 1);
+        m.elem_ref.push_back((int)read_int());
+      }
+    } else if ((w == "Quadrilaterals" && dim == 3) || (w == "Edges" && dim == 2)) {
+      const int64_t n = read_int();
+      const int nv = dim == 3 ? 4 : 2;
+      for (int64_t f = 0; f < n; ++f) {
+        std::vector<int64_t> v(nv);
+        for (int c = 0; c < nv; ++c) v[c] = read_int() - 1;
+        m.bnd.push_back(v);
+        m.bnd_ref.push_back((int)read_int());
+      }
+    } else if (w == "Triangles" || w == "Tetrahedra" || w == "Prisms" || w == "Pyramids") {
+      die("Medit section " + w + ": only quadrilateral / hexahedral meshes are supported");
+    } else if (w == "End") {
+      break;
+    }
+  }
+  if (m.elem.empty()) die("no elements in " + path);
+  return m;
+}
+
+// ------------------------------------------------------- Q_r numbering
+// Global GL nodes shared by neighbouring elements are identified
+// topologically: a node on an edge by the edge's vertex ids and its index
+// counted from the lower id; on a face by the face's lowest-id corner, its
+// lower-id then higher-id neighbour corners, and its indices in that frame.
+struct Key {
+  uint64_t a, b;
+  bool operator==(const Key &o) const { return a == o.a && b == o.b; }
+};
+struct KeyHash {
+  size_t operator()(const Key &k) const {
+    uint64_t h = k.a * 0x9E3779B97F4A7C15ULL ^ (k.b + 0x632BE59BD9B4E019ULL + (k.a << 6));
+    return (size_t)(h ^ (h >> 31));
+  }
+};
+
+struct QrMesh {
+  int dim, order;
+  int64_t N = 0, E = 0;
+  std::vector<double> xyz;
+  std::vector<int32_t> en;
+  std::vector<uint8_t> dir;
+};
+
+QrMesh number_nodes(const Q1Mesh &m, int order, const std::set<int> &dirichlet_refs) {
+  const int dim = m.dim, r = order, n = r + 1, nc = 1 << dim;
+  const int nd = dim == 3 ? n * n * n : n * n;
+  double xg[32], wg[32];
+  check(mjw_gauss_lobatto(order, xg, wg), nullptr, "mjw_gauss_lobatto");
+  QrMesh q;
+  q.dim = dim;
+  q.order = order;
+  q.E = (int64_t)m.elem_ref.size();
+  q.en.assign(q.E * nd, -1);
+  std::unordered_map<Key, int32_t, KeyHash> ids;
+  ids.reserve(q.E * nd / 2);
+  // corner index of (a, b, c) in {0,1}^dim, Medit order
+  auto corner = [&](int a, int b, int c) {
+    static const int c2[2][2] = {{0, 3}, {1, 2}};
+    return c2[a][b] + 4 * c;
+  };
+  std::set<std::vector<int64_t>> dir_faces;
+  for (size_t f = 0; f < m.bnd.size(); ++f)
+    if (dirichlet_refs.count(m.bnd_ref[f])) {
+      std::vector<int64_t> v = m.bnd[f];
+      std::sort(v.begin(), v.end());
+      dir_faces.insert(v);
+    }
+  std::vector<uint8_t> dflag;
+  for (int64_t e = 0; e < q.E; ++e) {
+    const int64_t *V = &m.elem[e * nc];
+    const int K = dim == 3 ? n : 1;
+    for (int k = 0; k < K; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          const int idx[3] = {i, j, k};
+          int side[3] = {-1, -1, -1}, nb = 0;
+          for (int a = 0; a < dim; ++a)
+            if (idx[a] == 0 || idx[a] == r) { side[a] = idx[a] == r; ++nb; }
+          Key key{0, 0};
+          bool shared = true;
+          if (nb == dim) {  // vertex
+            const int64_t g = V[corner(side[0], side[1], dim == 3 ? side[2] : 0)];
+            key = {(1ull << 62) | (uint64_t)g, 0};
+          } else if (nb == dim - 1) {  // edge
+            int d = 0;
+            while (side[d] >= 0) ++d;
+            int s0[3] = {side[0], side[1], side[2]}, s1[3] = {side[0], side[1], side[2]};
+            s0[d] = 0;
+            s1[d] = 1;
+            int64_t ga = V[corner(s0[0], s0[1], dim == 3 ? s0[2] : 0)];
+            int64_t gb = V[corner(s1[0], s1[1], dim == 3 ? s1[2] : 0)];
+            int t = idx[d];
+            if (ga > gb) { std::swap(ga, gb); t = r - t; }
+            key = {(2ull << 62) | ((uint64_t)ga << 31) | (uint64_t)gb, (uint64_t)t};
+          } else if (dim == 3 && nb == 1) {  // face
+            int d0 = 0;
+            while (side[d0] < 0) ++d0;
+            int du = -1, dv = -1;
+            for (int a = 0; a < 3; ++a)
+              if (a != d0) { if (du < 0) du = a; else dv = a; }
+            int64_t g[2][2];
+            for (int a = 0; a < 2; ++a)
+              for (int b = 0; b < 2; ++b) {
+                int s[3];
+                s[d0] = side[d0];
+                s[du] = a;
+                s[dv] = b;
+                g[a][b] = V[corner(s[0], s[1], s[2])];
+              }
+            int ou = 0, ov = 0;
+            for (int a = 0; a < 2; ++a)
+              for (int b = 0; b < 2; ++b)
+                if (g[a][b] < g[ou][ov]) { ou = a; ov = b; }
+            const int su = ou == 0 ? idx[du] : r - idx[du];
+            const int sv = ov == 0 ? idx[dv] : r - idx[dv];
+            const int64_t nu = g[1 - ou][ov], nv = g[ou][1 - ov];
+            int s1, s2;
+            int64_t first, second;
+            if (nu < nv) { s1 = su; s2 = sv; first = nu; second = nv; }
+            else { s1 = sv; s2 = su; first = nv; second = nu; }
+            key = {(3ull << 62) | ((uint64_t)g[ou][ov] << 31) | (uint64_t)first,
+                   ((uint64_t)second << 16) | ((uint64_t)s1 << 8) | (uint64_t)s2};
+          } else {
+            shared = false;
+          }
+          int32_t gid;
+          bool fresh = true;
+          if (shared) {
+            auto it = ids.find(key);
+            if (it != ids.end()) { gid = it->second; fresh = false; }
+            else { gid = (int32_t)q.N; ids.emplace(key, gid); }
+          } else {
+            gid = (int32_t)q.N;
+          }
+          if (fresh) {
+            ++q.N;
+            // position: multilinear map of the element's vertices at the GL point
+            double x[3] = {0, 0, 0};
+            for (int c = 0; c < nc; ++c) {
+              static const int ca[8] = {0, 1, 1, 0, 0, 1, 1, 0}, cb[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                               cc[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+              const double wa = ca[c] ? xg[i] : 1 - xg[i], wb = cb[c] ? xg[j] : 1 - xg[j],
+                           wc = dim == 3 ? (cc[c] ? xg[k] : 1 - xg[k]) : 1.0;
+              for (int a = 0; a < dim; ++a) x[a] += wa * wb * wc * m.vert[V[c] * dim + a];
+            }
+            for (int a = 0; a < dim; ++a) q.xyz.push_back(x[a]);
+            dflag.push_back(0);
+          }
+          q.en[e * nd + i + n * j + n * n * k] = gid;
+        }
+    // Dirichlet: every node of an element face listed as a Dirichlet boundary
+    for (int d0 = 0; d0 < dim; ++d0)
+      for (int sd = 0; sd < 2; ++sd) {
+        std::vector<int64_t> fv;
+        for (int c = 0; c < nc; ++c) {
+          static const int ca[8] = {0, 1, 1, 0, 0, 1, 1, 0}, cb[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                           cc[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+          const int s[3] = {ca[c], cb[c], cc[c]};
+          if (s[d0] == sd) fv.push_back(V[c]);
+        }
+        std::sort(fv.begin(), fv.end());
+        if (!dir_faces.count(fv)) continue;
+        const int K = dim == 3 ? n : 1;
+        for (int k = 0; k < K; ++k)
+          for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+              const int idx[3] = {i, j, k};
+              if (idx[d0] == (sd ? r : 0)) dflag[q.en[e * nd + i + n * j + n * n * k]] = 1;
+            }
+      }
+  }
+  q.dir = dflag;
+  return q;
+}
+
+// ------------------------------------------------------------------ media
+struct Media {
+  std::vector<double> rho, mu, lam;
+};
+
+Media media(const Deck &d, const Q1Mesh &m, bool elastic) {
+  std::map<int, std::vector<double>> mat;
+  for (const auto &v : d.all("MateriauDielec")) {
+    if (v.size() < 5 || v[1] != "ISOTROPE") die("MateriauDielec: only ISOTROPE media are supported");
+    mat[(int)num(v[0])] = {num(v[2]), num(v[3]), num(v[4])};
+  }
+  Media md;
+  for (int ref : m.elem_ref) {
+    auto it = mat.find(ref);
+    if (it == mat.end()) die("no MateriauDielec for element reference " + std::to_string(ref));
+    const auto &p = it->second;
+    if (elastic) {  // ISOTROPE rho lambda mu
+      md.rho.push_back(p[0]);
+      md.lam.push_back(p[1]);
+      md.mu.push_back(p[2]);
+    } else {  // ISOTROPE rho mu sigma
+      if (p[2] != 0.0) die("MateriauDielec: damping (sigma != 0) is not on the explicit path");
+      md.rho.push_back(p[0]);
+      md.mu.push_back(p[1]);
+    }
+  }
+  return md;
+}
+
+template <class T>
+void wr(std::ofstream &o, const std::vector<T> &v) {
+  o.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::fprintf(stderr,
+                 "usage: mjwave_run deck.ini [--precision fp64|fp32|fp32_g64] [--device N] "
+                 "[--dump-mesh FILE]\n");
+    return 2;
+  }
+  std::string deck_path = argv[1], dump;
+  int precision = MJW_FP64, device = 0;
+  for (int a = 2; a < argc; ++a) {
+    const std::string s = argv[a];
+    if (s == "--precision" && a + 1 < argc) {
+      const std::string p = argv[++a];
+      precision = p == "fp32" ? MJW_FP32 : p == "fp32_g64" ? MJW_FP32_G64 : p == "fp64" ? MJW_FP64 : -1;
+      if (precision < 0) die("unknown precision " + p);
+    } else if (s == "--device" && a + 1 < argc) {
+      device = std::atoi(argv[++a]);
+    } else if (s == "--dump-mesh" && a + 1 < argc) {
+      dump = argv[++a];
+    } else {
+      die("unknown option " + s);
+    }
+  }
+  const Deck d = read_deck(deck_path);
+  static const std::set<std::string> known = {
+      "TypeElement", "TypeEquation", "MeshPath", "FileMesh", "TypeMesh", "OrderDiscretization",
+      "ConditionReference", "MateriauDielec", "TypeSource", "OriginePhase", "Polarization",
+      "Frequency", "TemporalSource", "TimeStep", "TimeInterval", "OrderTimeScheme", "SismoPoint",
+      "SismoCircle", "FileOutputPoint", "ParametersOutputPoint", "OutputFormat",
+      "ElectricOrMagnetic", "PrintLevel"};
+  for (const auto &kv : d.kv)
+    if (!known.count(kv.first))
+      std::fprintf(stderr, "mjwave_run: keyword %s ignored\n", kv.first.c_str());
+
+  const std::string te = d.need("TypeElement")[0];
+  int dim;
+  if (te == "QUADRANGLE_LOBATTO") dim = 2;
+  else if (te == "HEXAHEDRON_LOBATTO") dim = 3;
+  else die("TypeElement " + te + ": only QUADRANGLE_LOBATTO / HEXAHEDRON_LOBATTO");
+  const std::string eq = d.need("TypeEquation")[0];
+  const bool elastic = eq == "TIME_ELASTIC";
+  if (!elastic && eq != "ACOUSTIC") die("TypeEquation " + eq + ": only ACOUSTIC / TIME_ELASTIC");
+  if (elastic && dim != 3) die("TIME_ELASTIC needs hexahedra");
+  const int ncomp = elastic ? 3 : 1;
+  const int order = (int)num(d.need("OrderDiscretization")[0]);
+  if (const auto *o = d.get("OrderTimeScheme"))
+    if (o->size() < 2 || num((*o)[0]) != 2 || (*o)[1] != "LEAP_FROG")
+      die("OrderTimeScheme: only 2 LEAP_FROG");
+  if (const auto *s = d.get("TypeSource"))
+    if ((*s)[0] != "SRC_DIRAC") die("TypeSource " + (*s)[0] + ": only SRC_DIRAC");
+  if (const auto *s = d.get("TemporalSource"))
+    if ((*s)[0] != "RICKER") die("TemporalSource " + (*s)[0] + ": only RICKER");
+
+  // mesh
+  const auto &fm = d.need("FileMesh");
+  Q1Mesh q1;
+  if (fm[0] == "REGULAR") {
+    q1 = regular_mesh(dim, fm);
+  } else {
+    std::string mp = d.get("MeshPath") ? d.need("MeshPath")[0] : "./";
+    if (!mp.empty() && mp[0] != '/') mp = d.dir + "/" + mp;
+    q1 = medit_mesh(dim, mp + "/" + fm[0]);
+  }
+  std::set<int> drefs;
+  for (const auto &v : d.all("ConditionReference")) {
+    if (v.size() < 2) die("ConditionReference needs a reference and a condition");
+    if (v.back() != "DIRICHLET") die("ConditionReference " + v.back() + ": only DIRICHLET");
+    for (size_t k = 0; k + 1 < v.size(); ++k) drefs.insert((int)num(v[k]));
+  }
+  const QrMesh qm = number_nodes(q1, order, drefs);
+  const Media md = media(d, q1, elastic);
+  if (!dump.empty()) {
+    std::ofstream o(dump, std::ios::binary);
+    const std::vector<int64_t> head = {dim, order, ncomp, qm.N, qm.E};
+    wr(o, head);
+    wr(o, qm.xyz);
+    wr(o, qm.en);
+    wr(o, qm.dir);
+    wr(o, md.rho);
+    wr(o, md.mu);
+    wr(o, md.lam);
+    return o ? 0 : 1;
+  }
+
+  // time parameters
+  const double dt = num(d.need("TimeStep")[0]);
+  const auto &ti = d.need("TimeInterval");
+  const double t0 = ti.size() >= 2 ? num(ti[0]) : 0.0, t1 = num(ti.size() >= 2 ? ti[1] : ti[0]);
+  const int64_t nsteps = (int64_t)std::ceil((t1 - t0) / dt - 1e-9);  // RunSecondOrderScheme
+  if (nsteps < 1) die("TimeInterval shorter than one TimeStep");
+  const double f0 = num(d.need("Frequency")[0]);
+
+  mjw_problem *p = nullptr;
+  auto tc = std::chrono::steady_clock::now();
+  check(mjw_create(&p, device, dim, order, ncomp, qm.N, qm.xyz.data(), qm.E, qm.en.data(),
+                   md.rho.data(), md.mu.data(), elastic ? md.lam.data() : nullptr, qm.dir.data(),
+                   1, precision),
+        nullptr, "mjw_create");
+  // source
+  {
+    const auto &op = d.need("OriginePhase");
+    if ((int)op.size() < dim) die("OriginePhase needs " + std::to_string(dim) + " coordinates");
+    double x[3];
+    for (int a = 0; a < dim; ++a) x[a] = num(op[a]);
+    std::vector<double> pol(ncomp, 1.0);
+    if (elastic) {
+      const auto &pv = d.need("Polarization");
+      for (int a = 0; a < 3; ++a) pol[a] = num(pv.at(a));
+    }
+    check(mjw_set_sources(p, 1, x, pol.data(), nullptr), p, "source");
+  }
+  std::vector<double> tab(nsteps);
+  check(mjw_ricker_pulse_table(f0, t0, dt, (int)nsteps, tab.data()), nullptr, "Ricker");
+  check(mjw_set_pulse(p, (int)nsteps, tab.data()), p, "pulse");
+  // receivers: SismoPoint, SismoCircle (x0 y0 radius N in 2-D)
+  std::vector<double> pts;
+  int npt = 0;
+  for (const auto &v : d.all("SismoPoint")) {
+    if ((int)v.size() < dim) die("SismoPoint needs " + std::to_string(dim) + " coordinates");
+    for (int a = 0; a < dim; ++a) pts.push_back(num(v[a]));
+    ++npt;
+  }
+  for (const auto &v : d.all("SismoCircle")) {
+    if (dim != 2 || v.size() < 4) die("SismoCircle = x0 y0 radius N (2-D)");
+    const double cx = num(v[0]), cy = num(v[1]), rad = num(v[2]);
+    const int nc = (int)num(v[3]);
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < nc; ++k) {
+      pts.push_back(cx + rad * std::cos(2 * pi * k / nc));
+      pts.push_back(cy + rad * std::sin(2 * pi * k / nc));
+      ++npt;
+    }
+  }
+  std::vector<double> rpts;
+  std::vector<int32_t> comp;
+  for (int m = 0; m < ncomp; ++m)
+    for (int r = 0; r < npt; ++r) {
+      for (int a = 0; a < dim; ++a) rpts.push_back(pts[(size_t)r * dim + a]);
+      comp.push_back(m);
+    }
+  const int nrec = npt * ncomp;
+  if (nrec) check(mjw_set_receivers(p, nrec, rpts.data(), comp.data()), p, "receivers");
+  const double t_setup =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
+
+  // RunSecondOrderScheme: U^0 = U^1 = 0, rows n = 0..nsteps
+  check(mjw_leapfrog_init(p, t0, dt, nullptr, nullptr), p, "leapfrog_init");
+  std::vector<double> tr((size_t)(nsteps + 1) * (nrec ? nrec : 1));
+  auto ts = std::chrono::steady_clock::now();
+  check(mjw_leapfrog_advance(p, (int)nsteps, nrec ? tr.data() : nullptr, 1), p, "advance");
+  if (nrec) check(mjw_record(p, tr.data() + (size_t)nsteps * nrec), p, "record");
+  const double t_loop = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+
+  // seismograms
+  if (nrec && d.get("FileOutputPoint")) {
+    const auto &fo = d.need("FileOutputPoint");
+    const std::string out = fo.size() >= 2 ? fo[1] : fo[0];
+    double ts0 = t0, ts1 = t0 + nsteps * dt, tsd = dt;
+    if (const auto *po = d.get("ParametersOutputPoint")) {
+      if (po->size() >= 3) { ts0 = num((*po)[0]); ts1 = num((*po)[1]); tsd = num((*po)[2]); }
+    }
+    const int64_t every = std::max<int64_t>(1, std::llround(tsd / dt));
+    FILE *f = std::fopen(out.c_str(), "w");
+    if (!f) die("cannot write " + out);
+    for (int64_t s = 0; s <= nsteps; ++s) {
+      const double t = t0 + s * dt;
+      if (t < ts0 - 1e-9 * dt || t > ts1 + 1e-9 * dt) continue;
+      if (std::llround((t - ts0) / dt) % every) continue;
+      std::fprintf(f, "%.15e", t);
+      // point-major: the components of point 0, then of point 1, ...
+      for (int r = 0; r < npt; ++r)
+        for (int m = 0; m < ncomp; ++m)
+          std::fprintf(f, "\t%.15e", tr[(size_t)s * nrec + (size_t)m * npt + r]);
+      std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+  }
+  std::printf("mjwave_run: %s %dD Q%d, %lld nodes, %lld elements, %lld steps\n",
+              elastic ? "elastic" : "acoustic", dim, order, (long long)qm.N, (long long)qm.E,
+              (long long)nsteps);
+  std::printf("Time to set up the problem : %.6f\n", t_setup);
+  std::printf("Time to complete time iterations : %.6f\n", t_loop);
+  mjw_destroy(p);
+  return 0;
+}

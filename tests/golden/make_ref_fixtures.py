@@ -25,119 +25,18 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 from oracle import oracle as O  # noqa: E402
+from tests.golden.decks import write_deck  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
 RUN = os.path.join(REF, "run")
 GOLD = os.path.dirname(os.path.abspath(__file__))
 
 
-def _f(x: float) -> str:
-    return repr(float(x))
-
-
-def write_medit(path, m: O.Mesh, elem_ref, bnd_ref=1):
-    """Medit mesh of the structured (perturbed) mesh: vertices, hexahedra
-    (quadrilaterals in 2-D) with their media reference, boundary faces."""
-    dim, nx = m.dim, m.nx
-    nv1 = nx + 1
-    V = m.vert.reshape(-1, dim)
-    lines = ["MeshVersionFormatted 1", "Dimension", str(dim), "Vertices", str(len(V))]
-    for v in V:
-        lines.append(" ".join(_f(c) for c in v) + " 0")
-    vid = lambda a, b, c=0: 1 + a + nv1 * (b + nv1 * c)  # noqa: E731
-    if dim == 3:
-        lines += ["Hexahedra", str(nx ** 3)]
-        e = 0
-        for c in range(nx):
-            for b in range(nx):
-                for a in range(nx):
-                    cs = [vid(a, b, c), vid(a + 1, b, c), vid(a + 1, b + 1, c), vid(a, b + 1, c),
-                          vid(a, b, c + 1), vid(a + 1, b, c + 1), vid(a + 1, b + 1, c + 1),
-                          vid(a, b + 1, c + 1)]
-                    lines.append(" ".join(map(str, cs)) + f" {int(elem_ref[e])}")
-                    e += 1
-        faces = []
-        for i in range(nx):
-            for j in range(nx):
-                faces.append([vid(i, j, 0), vid(i, j + 1, 0), vid(i + 1, j + 1, 0), vid(i + 1, j, 0)])
-                faces.append([vid(i, j, nx), vid(i + 1, j, nx), vid(i + 1, j + 1, nx), vid(i, j + 1, nx)])
-                faces.append([vid(i, 0, j), vid(i + 1, 0, j), vid(i + 1, 0, j + 1), vid(i, 0, j + 1)])
-                faces.append([vid(i, nx, j), vid(i, nx, j + 1), vid(i + 1, nx, j + 1), vid(i + 1, nx, j)])
-                faces.append([vid(0, i, j), vid(0, i, j + 1), vid(0, i + 1, j + 1), vid(0, i + 1, j)])
-                faces.append([vid(nx, i, j), vid(nx, i + 1, j), vid(nx, i + 1, j + 1), vid(nx, i, j + 1)])
-        lines += ["Quadrilaterals", str(len(faces))]
-        lines += [" ".join(map(str, f)) + f" {bnd_ref}" for f in faces]
-    else:
-        lines += ["Quadrilaterals", str(nx * nx)]
-        e = 0
-        for b in range(nx):
-            for a in range(nx):
-                cs = [vid(a, b), vid(a + 1, b), vid(a + 1, b + 1), vid(a, b + 1)]
-                lines.append(" ".join(map(str, cs)) + f" {int(elem_ref[e])}")
-                e += 1
-        edges = []
-        for i in range(nx):
-            edges += [[vid(i, 0), vid(i + 1, 0)], [vid(nx, i), vid(nx, i + 1)],
-                      [vid(i + 1, nx), vid(i, nx)], [vid(0, i + 1), vid(0, i)]]
-        lines += ["Edges", str(len(edges))]
-        lines += [" ".join(map(str, f)) + f" {bnd_ref}" for f in edges]
-    lines.append("End")
-    with open(path, "w") as fh:
-        fh.write("\n".join(lines) + "\n")
-
-
 def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
              lam_layers=None, rho_layers=None, pol=None, regular=False):
     d = os.path.join(RUN, name)
-    os.makedirs(d, exist_ok=True)
-    m = O.mesh(dim, nx, r, pert, seed)
-    nl = len(layers)
-    e = np.arange(m.E)
-    ez = e // (nx ** (dim - 1))
-    lay = (ez * nl) // nx
-    ini = []
-    if dim == 2:
-        ini.append("TypeElement = QUADRANGLE_LOBATTO")
-    else:
-        ini.append("TypeElement = HEXAHEDRON_LOBATTO")
-    ini.append("TypeEquation = " + ("TIME_ELASTIC" if elastic else "ACOUSTIC"))
-    ini.append("MeshPath = ./")
-    ini.append(f"Frequency = {_f(f0)} 0.0")
-    if regular:
-        ini.append(f"FileMesh = REGULAR {nx + 1} 0.0 1.0 0.0 1.0" + (" 0.0 1.0" if dim == 3 else "")
-                   + " 1" + " 1" * (2 * dim))
-    else:
-        write_medit(os.path.join(d, "mesh.mesh"), m, lay + 1, bnd_ref=nl + 1)
-        ini.append("FileMesh = mesh.mesh")
-    if dim == 3:
-        ini.append("TypeMesh = HEXA")
-    ini.append(f"OrderDiscretization = {r}")
-    bref = 1 if regular else nl + 1
-    ini.append(f"ConditionReference = {bref} DIRICHLET")
-    for k in range(nl):
-        if elastic:
-            ini.append(f"MateriauDielec = {k + 1} ISOTROPE {_f(rho_layers[k])} "
-                       f"{_f(lam_layers[k])} {_f(layers[k])}")
-        else:
-            ini.append(f"MateriauDielec = {k + 1} ISOTROPE 1.0 {_f(layers[k])} 0.0")
-    ini.append("TypeSource = SRC_DIRAC")
-    ini.append("OriginePhase = " + " ".join(_f(c) for c in src))
-    if elastic:
-        ini.append("Polarization = " + " ".join(_f(c) for c in pol))
-    T = (nsteps - 0.5) * dt
-    ini.append(f"ParametersOutputPoint = 0.0 {_f(nsteps * dt + 0.5 * dt)} {_f(dt)}")
-    ini.append("FileOutputPoint = diffracPoint.dat totalPoint.dat")
-    for p in rec:
-        ini.append("SismoPoint = " + " ".join(_f(c) for c in p))
-    ini.append("OutputFormat = BINARY DOUBLE")
-    ini.append("ElectricOrMagnetic = " + ("-1" if elastic else "0"))
-    ini.append(f"TimeStep = {_f(dt)}")
-    ini.append(f"TimeInterval = 0.0 {_f(T)}")
-    ini.append("OrderTimeScheme = 2 LEAP_FROG")
-    ini.append("TemporalSource = RICKER")
-    ini.append("PrintLevel = 0")
-    with open(os.path.join(d, "case.ini"), "w") as fh:
-        fh.write("\n".join(ini) + "\n")
+    write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
+               lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):

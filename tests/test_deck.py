@@ -1,0 +1,100 @@
+"""Deck driver (montjoie_amd/mjwave_run): Montjoie data files and Medit
+meshes of the golden cases (tests/golden/decks.py writes them from the
+parameters stored with each fixture).  CPU: the mesh the driver assembles
+(--dump-mesh, no GPU call) is node for node the oracle's mesh.  GPU: the
+seismogram file it writes matches the one Montjoie wrote for the same deck."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import cases as CS
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUNNER = os.path.join(ROOT, "montjoie_amd", "mjwave_run")
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+import decks  # noqa: E402
+
+
+def _write(c: CS.Case, d):
+    z = np.load(os.path.join(CS.GOLD, c.name + ".npz"))
+    decks.write_deck(str(d), c.dim, c.nx, c.r, c.pert, c.seed, z["layers"], c.f0, c.dt,
+                     c.nsteps, c.src, c.rec, elastic=c.elastic,
+                     lam_layers=z["lam_layers"] if c.elastic else None,
+                     rho_layers=z["rho_layers"] if c.elastic else None,
+                     pol=c.pol, regular=bool(z["regular"]))
+
+
+def _dump(path):
+    raw = open(path, "rb").read()
+    head = np.frombuffer(raw[:40], np.int64)
+    dim, order, ncomp, N, E = (int(v) for v in head)
+    nd = (order + 1) ** dim
+    off = 40
+    out = {"dim": dim, "order": order, "ncomp": ncomp, "N": N, "E": E}
+    for name, dt, n in (("xyz", np.float64, N * dim), ("en", np.int32, E * nd),
+                        ("dir", np.uint8, N), ("rho", np.float64, E), ("mu", np.float64, E),
+                        ("lam", np.float64, E if ncomp == 3 else 0)):
+        size = np.dtype(dt).itemsize * n
+        out[name] = np.frombuffer(raw[off:off + size], dt)
+        off += size
+    assert off == len(raw)
+    out["xyz"] = out["xyz"].reshape(N, dim)
+    out["en"] = out["en"].reshape(E, nd)
+    return out
+
+
+@pytest.mark.parametrize("name", CS.golden_names())
+def test_driver_mesh_is_the_oracle_mesh(name, tmp_path):
+    if not os.path.exists(RUNNER):
+        pytest.fail("montjoie_amd/mjwave_run is not built (python -m montjoie_amd.build)")
+    c = CS.load(name)
+    _write(c, tmp_path)
+    r = subprocess.run([RUNNER, "case.ini", "--dump-mesh", "mesh.bin"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    g = _dump(tmp_path / "mesh.bin")
+    o = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    assert (g["N"], g["E"]) == (o.N, o.E)
+    oen = o.elem_nodes.reshape(o.E, -1)
+    # same conforming numbering up to a permutation of the global nodes
+    perm = np.full(o.N, -1)
+    perm[oen.ravel()] = g["en"].ravel()
+    assert np.array_equal(perm[oen], g["en"]) and np.unique(perm).size == o.N
+    np.testing.assert_allclose(g["xyz"][perm], o.xyz.reshape(o.N, -1), rtol=0, atol=1e-14)
+    assert np.array_equal(g["dir"][perm], o.dirichlet.astype(np.uint8))
+    np.testing.assert_array_equal(g["mu"], c.mu_e)
+    np.testing.assert_array_equal(g["rho"], c.rho_e)
+    if c.elastic:
+        np.testing.assert_array_equal(g["lam"], c.lam_e)
+
+
+def test_driver_rejects_unsupported_decks(tmp_path):
+    c = CS.load("ref2d_q4_n8")
+    _write(c, tmp_path)
+    ini = (tmp_path / "case.ini").read_text()
+    (tmp_path / "bad.ini").write_text(ini.replace("2 LEAP_FROG", "4 RUNGE_KUTTA"))
+    r = subprocess.run([RUNNER, "bad.ini", "--dump-mesh", "x.bin"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "LEAP_FROG" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CS.golden_names())
+def test_driver_seismograms_match_montjoie(name, tmp_path):
+    """FP64 run of the deck: the written totalPoint.dat against Montjoie's
+    (rel-L2 <= 1e-10, the requirement; times exact)."""
+    c = CS.load(name)
+    _write(c, tmp_path)
+    print(f"deck {name}", flush=True)
+    r = subprocess.run([RUNNER, "case.ini"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    out = np.loadtxt(tmp_path / "totalPoint.dat", ndmin=2)
+    np.testing.assert_allclose(out[:, 0], c.times, rtol=1e-15, atol=1e-18)
+    ref = c.traces[:, :out.shape[1] - 1]
+    assert out.shape[0] == ref.shape[0]
+    assert CS.rel_l2(out[:, 1:], ref) < 1e-10
