@@ -34,7 +34,7 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
     const TG *__restrict__ glxw) {
   constexpr int P = N * N, NN = N * N * N;
   using A = TG;
-  __shared__ T lds[6 * EPB * NN];
+  __shared__ T lds[9 * EPB * NN];
   __shared__ TG lc[TRI ? EPB * kCoef : 1];
   const int t = threadIdx.x;
   const int el = t / P, ij = t - el * P;
@@ -64,13 +64,18 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
     lam = (A)lam_e[e];
     mu = (A)mu_e[e];
   }
-  // LDS planes: [6][EPB][NN]; u_m uses planes 0..2, v_{m,0} planes 0..2,
-  // v_{m,1} planes 3..5
-  T *pl[6];
+  // LDS planes [9][EPB][NN]: u_m in planes 0..2, the xi_1 fluxes v_{m,0} in
+  // planes 3..5, the xi_2 fluxes v_{m,1} in planes 6..8 (written as they are
+  // computed, so two barriers per shot); the xi_3 fluxes stay in registers
+  T *pl[9];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) pl[c] = lds + (c * EPB + el) * NN;
+  for (int c = 0; c < 9; ++c) pl[c] = lds + (c * EPB + el) * NN;
   const rsrc_t rs = mkres(sdiag);
   const uint32_t comp_ps = (uint32_t)(N_nodes * sizeof(T));
+  // the scatter's previous values and dt^2/M could be loaded ahead of the
+  // scatter loop (as the scalar kernels do in FP32); here the extra registers
+  // cost more occupancy than the ordering, so PRE is off
+  constexpr bool PRE = false;
   for (int s = 0; s < nshots; ++s) {
     const rsrc_t ru = mkres(U + (int64_t)s * 3 * N_nodes);
     const rsrc_t ry = mkres(Y + (int64_t)s * 3 * N_nodes);
@@ -85,7 +90,7 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
         }
     }
     __syncthreads();
-    T v0[3][N], v1[3][N], v2[3][N];
+    T v2[3][N];
     if (act) {
       T Di[N], Dj[N];
 #pragma unroll
@@ -132,21 +137,23 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
             sg[m][a] = mu * (du[m][a] + du[a][m]) + (m == a ? lam * tr : A(0));
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-          v0[m][k] = (T)(M[0] * sg[m][0] + M[1] * sg[m][1] + M[2] * sg[m][2]);
-          v1[m][k] = (T)(M[3] * sg[m][0] + M[4] * sg[m][1] + M[5] * sg[m][2]);
+          pl[3 + m][k * P + ij] = (T)(M[0] * sg[m][0] + M[1] * sg[m][1] + M[2] * sg[m][2]);
+          pl[6 + m][k * P + ij] = (T)(M[3] * sg[m][0] + M[4] * sg[m][1] + M[5] * sg[m][2]);
           v2[m][k] = (T)(M[6] * sg[m][0] + M[7] * sg[m][1] + M[8] * sg[m][2]);
         }
       }
     }
-    __syncthreads();
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < 3; ++m)
+    T yk[PRE ? 3 : 1][PRE ? N : 1], sk[PRE && MODE == kLeap ? N : 1];
+    if constexpr (PRE) {
+      if (act) {
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-          pl[m][k * P + ij] = v0[m][k];
-          pl[3 + m][k * P + ij] = v1[m][k];
+          const uint32_t off = (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T);
+          if constexpr (MODE == kLeap) sk[k] = bld<T>(rs, off, 0u);
+#pragma unroll
+          for (int m = 0; m < 3; ++m) yk[m][k] = bld<T>(ry, off, m * comp_ps);
         }
+      }
     }
     __syncthreads();
     if (act) {
@@ -162,17 +169,17 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
         const uint32_t off = (uint32_t)(mm & 0x7fffffff) * (uint32_t)sizeof(T);
         const bool first = mm < 0;
         T sgd = 0;
-        if (MODE == kLeap) sgd = bld<T>(rs, off, 0u);
+        if constexpr (MODE == kLeap) sgd = PRE ? sk[PRE ? k : 0] : bld<T>(rs, off, 0u);
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
           T o = 0;
 #pragma unroll
           for (int l = 0; l < N; ++l) {
-            o += Dti[l] * pl[m][k * P + j * N + l];
-            o += Dtj[l] * pl[3 + m][k * P + l * N + i];
+            o += Dti[l] * pl[3 + m][k * P + j * N + l];
+            o += Dtj[l] * pl[6 + m][k * P + l * N + i];
             o += Dm[l * N + k] * v2[m][l];
           }
-          const T y = bld<T>(ry, off, m * comp_ps);
+          const T y = PRE ? yk[PRE ? m : 0][PRE ? k : 0] : bld<T>(ry, off, m * comp_ps);
           T r;
           if (MODE == kLeap) {
             r = first ? (T(2) * u[m][k] - y - sgd * o) : (y - sgd * o);
@@ -183,7 +190,8 @@ __global__ __launch_bounds__(EPB *N *N) void k_elastic3d(
         }
       }
     }
-    __syncthreads();
+    // the next shot writes planes 0..2 (last read before the barrier above)
+    // and planes 3..8 only after its first barrier: no barrier needed here
   }
 }
 
@@ -220,9 +228,9 @@ static int run_elastic_prec(const StiffArgs &a, hipStream_t st) {
   return run_elastic<N, EPB, float, float>(a, st);
 }
 
-// LDS = 6*EPB*n^3*sizeof(T) <= 64 KiB
+// LDS = 9*EPB*n^3*sizeof(T) (+ coefficients) <= 64 KiB in FP64
 #define MJW_TILES_EL(X) \
-  X(2, 16) X(2, 32) X(3, 7) X(3, 14) X(4, 4) X(4, 8) X(5, 5) X(5, 10) X(6, 2) X(6, 5) X(7, 1) \
+  X(2, 16) X(2, 32) X(3, 7) X(3, 14) X(4, 4) X(4, 8) X(5, 5) X(5, 3) X(6, 2) X(6, 3) X(7, 1) \
   X(7, 2) X(8, 1)
 
 int launch_stiffness_elastic(const StiffArgs &a, void *stream) {
