@@ -97,15 +97,29 @@ __device__ __forceinline__ void scalar3d_body(
   }
   const rsrc_t rs = mkres(sdiag);
   T *su = lu + el * NN, *sv = lv + el * NN;
+  // shots share the nodes: the next shot's values are gathered while the
+  // current one is computed
+  T un[N];
+  if (act) {
+    const rsrc_t ru = mkres(U);
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+      un[k] = bld<T>(ru, (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
+  }
   for (int s = 0; s < nshots; ++s) {
-    const rsrc_t ru = mkres(U + (int64_t)s * N_nodes);
     const rsrc_t ry = mkres(Y + (int64_t)s * N_nodes);
     T u[N];
     if (act) {
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        u[k] = bld<T>(ru, (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
+        u[k] = un[k];
         su[k * P + ij] = u[k];
+      }
+      if (s + 1 < nshots) {
+        const rsrc_t ru = mkres(U + (int64_t)(s + 1) * N_nodes);
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+          un[k] = bld<T>(ru, (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
       }
     }
     __syncthreads();
