@@ -248,6 +248,20 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   if (nshots < 1 || nshots > 4096) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "bad nshots");
   if (precision != MJW_FP64 && precision != MJW_FP32 && precision != MJW_FP32_G64)
     FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "bad precision");
+  {
+    // buffer resources of the kernels address at most 2 GiB each (checked
+    // before any array is read)
+    const int64_t P = (int64_t)(order + 1) * (order + 1);
+    const int64_t K = dim == 2 ? 1 : order + 1;
+    const int64_t sb = precision == MJW_FP64 ? 8 : 4, gb = precision == MJW_FP32 ? 4 : 8;
+    const int64_t ngc = ncomp == 3 ? 9 : (dim == 2 ? 3 : 6);
+    const int64_t lim = ((int64_t)1 << 31) - 4096;
+    if (n_nodes * ncomp * sb >= lim || n_elem * P * K * 4 >= lim ||
+        n_elem * P * ngc * gb >= lim)
+      FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED,
+           "problem too large for one device problem (2 GiB extent per array); split it into "
+           "several problems");
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
     FAIL((mjw_problem *)nullptr, MJW_ERR_HIP, "no HIP device available");
@@ -281,17 +295,6 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
         (ncomp == 3 && !std::isfinite(lambda[e])))
       return fail(MJW_ERR_ARG, "media must be finite with rho, mu > 0");
 
-  {
-    // buffer resources of the kernels address at most 2 GiB each
-    const int64_t P = (int64_t)(order + 1) * (order + 1);
-    const int64_t K = dim == 2 ? 1 : order + 1;
-    const int64_t sb = precision == MJW_FP64 ? 8 : 4, gb = precision == MJW_FP32 ? 4 : 8;
-    const int64_t ngc = ncomp == 3 ? 9 : (dim == 2 ? 3 : 6);
-    const int64_t lim = ((int64_t)1 << 31) - 4096;
-    if (n_nodes * ncomp * sb >= lim || n_elem * P * K * 4 >= lim || n_elem * P * ngc * gb >= lim)
-      return fail(MJW_ERR_UNSUPPORTED, "problem too large for one device problem (2 GiB extent); "
-                                       "split it into several problems");
-  }
   if (hipSetDevice(device) != hipSuccess) return fail(MJW_ERR_HIP, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(MJW_ERR_HIP, "hipStreamCreate failed");

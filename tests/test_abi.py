@@ -70,6 +70,10 @@ def test_create_rejects_bad_arguments_without_device():
     rc = L.mjw_create(C.byref(h), 0, 4, 2, 1, 3, _lib.dptr(xyz), 1, _lib.iptr(en),
                       _lib.dptr(rho), _lib.dptr(rho), None, None, 1, 0)
     assert rc == -1
+    # 2 GiB extent per array: refused before any array is read (tiny arrays here)
+    rc = L.mjw_create(C.byref(h), 0, 3, 7, 1, (1 << 31) // 8 + 1, _lib.dptr(xyz), 1,
+                      _lib.iptr(en), _lib.dptr(rho), _lib.dptr(rho), None, None, 1, 0)
+    assert rc == -4 and b"2 GiB" in L.mjw_last_error(None)
     with pytest.raises(M.MjwError):
         M.ricker_pulse_table(-1.0, 0.0, 1e-3, 10)
 
