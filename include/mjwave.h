@@ -29,9 +29,10 @@
  *     live on the handle's device and are used on the handle's stream.
  *   - One handle owns one HIP stream; calls on one handle serialise, distinct
  *     handles are independent.
- *   - Global vectors: node-major, components blocked as in Montjoie
+ *   - Global host vectors: node-major, components blocked as in Montjoie
  *     (u[m*N + g], Mesh/NumberMap.cxx:693).  With S > 1 shots each shot is
- *     one such vector, shots consecutive (u[(s*ncomp + m)*N + g]).
+ *     one such vector, shots consecutive (u[(s*ncomp + m)*N + g]).  Device
+ *     vectors use the solver's own node order and layout (mjw_state_dev).
  *   - Element-local nodes q = i + n*j (+ n*n*k), n = order+1, i along xi_1;
  *     1-D nodes are the Gauss-Lobatto points of [0,1].
  */
@@ -92,8 +93,12 @@ int mjw_destroy(mjw_problem *p);
  * [6]=nshots [7]=precision [8]=device bytes held */
 int mjw_info(const mjw_problem *p, int64_t *info9);
 
-/* Tuning: elements per workgroup of the stiffness kernel (0 = default).
- * Used by the autotuner (montjoie_amd/tuner.py). */
+/* Tuning: launch shape of the time loop's stiffness kernel (0 = default):
+ * for one shot the elements per workgroup (a column tile EPB, -EPB for one
+ * thread per node, 1000 + EPB for the persistent column kernel); for 3-D
+ * acoustic problems with several (interleaved) shots the shot lanes per
+ * column.  mjw_tile_choices lists the valid values.  Used by the autotuner
+ * (montjoie_amd/tuner.py); every shape gives identical results. */
 int mjw_set_tile(mjw_problem *p, int elems_per_block);
 int mjw_get_tile(const mjw_problem *p, int *elems_per_block);
 /* Block order of the stiffness launches: blocks that the hardware places on
@@ -211,8 +216,10 @@ int mjw_record(mjw_problem *p, double *traces);
 /* current pair (U^n, U^{n+1}) as FP64 host vectors [nshots][n_nodes*ncomp];
  * either may be NULL (GetIterate / GetIterate(k)). */
 int mjw_get_state(mjw_problem *p, double *Un, double *Un1);
-/* device pointers of U^n and U^{n+1} (state precision, shots consecutive,
- * solver node order: see mjw_node_order) */
+/* device pointers of U^n and U^{n+1} in the solver's layout (state
+ * precision, solver node order: see mjw_node_order): shots consecutive, except
+ * 3-D acoustic problems with nshots > 1, whose shots are interleaved
+ * (u[g*nshots + s]: one gather serves every shot of a node) */
 int mjw_state_dev(mjw_problem *p, void **Un, void **Un1);
 /* current step index n */
 int mjw_get_step(const mjw_problem *p, int64_t *n);

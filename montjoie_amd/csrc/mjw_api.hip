@@ -35,6 +35,9 @@ struct mjw_problem {
   // device vectors use the solver node order: nperm[g] = solver index of
   // node g of the caller's numbering, ninv = its inverse
   std::vector<int32_t> nperm, ninv;
+  // several shots of a 3-D acoustic problem: interleaved state u[(m*N + g)*S + s]
+  int ilv = 0;
+  int st = 0;                   // shot lanes of the interleaved kernel (0: automatic)
   int32_t *d_nperm = nullptr, *d_ninv = nullptr;
   int epb = 0;
   int nxcd = 8;                 // MI355X: 8 XCDs, round-robin block dispatch
@@ -256,7 +259,9 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
     const int64_t sb = precision == MJW_FP64 ? 8 : 4, gb = precision == MJW_FP32 ? 4 : 8;
     const int64_t ngc = ncomp == 3 ? 9 : (dim == 2 ? 3 : 6);
     const int64_t lim = ((int64_t)1 << 31) - 4096;
-    if (n_nodes * ncomp * sb >= lim || n_elem * P * K * 4 >= lim ||
+    // several 3-D acoustic shots are interleaved: one array of S*N values
+    const int64_t sv = (dim == 3 && ncomp == 1 && nshots > 1) ? nshots : 1;
+    if (sv * n_nodes * ncomp * sb >= lim || n_elem * P * K * 4 >= lim ||
         n_elem * P * ngc * gb >= lim)
       FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED,
            "problem too large for one device problem (2 GiB extent per array); split it into "
@@ -274,6 +279,7 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   p->n = order + 1;
   p->ncomp = ncomp;
   p->nshots = nshots;
+  p->ilv = (dim == 3 && ncomp == 1 && nshots > 1) ? 1 : 0;
   p->prec = precision;
   p->state = precision == MJW_FP64 ? kF64 : kF32;
   p->geom = precision == MJW_FP32 ? kG32 : kG64;
@@ -527,17 +533,29 @@ extern "C" int mjw_info(const mjw_problem *p, int64_t *info) {
   return MJW_OK;
 }
 
+// launch shapes of the time loop's stiffness kernel: elements per block
+// (single shot) or shot lanes per column (interleaved shots)
+static std::vector<int> problem_tiles(const mjw_problem *p) {
+  if (p->ilv) return tile_choices_shots(p->order, p->state, p->geom);
+  return tile_choices(p->dim, p->order, p->ncomp);
+}
+
 extern "C" int mjw_set_tile(mjw_problem *p, int epb) {
   if (!p) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL problem");
-  if (epb == 0) { p->epb = default_tile(p->dim, p->order, p->ncomp, p->state); return MJW_OK; }
-  std::vector<int> v = tile_choices(p->dim, p->order, p->ncomp);
-  if (std::find(v.begin(), v.end(), epb) == v.end())
-    FAIL(p, MJW_ERR_ARG, "unsupported elements-per-block for this order");
-  if (epb != p->epb && p->gexec) {
+  int &slot = p->ilv ? p->st : p->epb;
+  int want = epb;
+  if (epb == 0) {
+    want = p->ilv ? 0 : default_tile(p->dim, p->order, p->ncomp, p->state);
+  } else {
+    std::vector<int> v = problem_tiles(p);
+    if (std::find(v.begin(), v.end(), epb) == v.end())
+      FAIL(p, MJW_ERR_ARG, "unsupported launch shape for this problem");
+  }
+  if (want != slot && p->gexec) {
     (void)hipGraphExecDestroy(p->gexec);
     p->gexec = nullptr;
   }
-  p->epb = epb;
+  slot = want;
   return MJW_OK;
 }
 
@@ -576,7 +594,7 @@ extern "C" int mjw_get_geometry_mode(const mjw_problem *p, int *mode, int *trili
 
 extern "C" int mjw_get_tile(const mjw_problem *p, int *epb) {
   if (!p || !epb) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
-  *epb = p->epb;
+  *epb = p->ilv ? p->st : p->epb;
   return MJW_OK;
 }
 
@@ -589,6 +607,8 @@ static StiffArgs stiff_args(mjw_problem *p, int mode, const void *U, void *Y, in
   a.U = U; a.Y = Y; a.sdiag = p->d_sdiag; a.N = p->N; a.nshots = nshots;
   a.alpha = 1.0; a.beta = 0.0; a.nxcd = p->nxcd; a.grad = nullptr;
   a.geo = p->geo; a.coef = p->d_coef; a.glxw = p->d_glxw;
+  a.ilv = p->ilv && nshots > 1;
+  if (a.ilv) a.epb = p->st;
   return a;
 }
 
@@ -611,15 +631,17 @@ extern "C" int mjw_apply_kh_dev(mjw_problem *p, double alpha, const void *U, dou
                                 void *Y) {
   if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_kh_dev: NULL");
   HIPCHK(p, hipSetDevice(p->device));
-  // caller's order -> solver order (gather by ninv) and back (gather by nperm)
-  const int64_t nb = (int64_t)p->nshots * p->ncomp;
-  KCHK(p, launch_gather(p->state, nb, p->N, p->d_ninv, U, p->d_tmp, p->stream));
+  // caller's layout and numbering -> solver's, apply, and back
+  const int S = p->nshots;
+  KCHK(p, launch_to_solver(p->state, S, p->ncomp, p->N, p->ilv, p->d_ninv, U, p->d_tmp, p->stream));
   if (beta != 0.0)
-    KCHK(p, launch_gather(p->state, nb, p->N, p->d_ninv, Y, p->d_tmp2, p->stream));
+    KCHK(p, launch_to_solver(p->state, S, p->ncomp, p->N, p->ilv, p->d_ninv, Y, p->d_tmp2,
+                             p->stream));
   else
-    HIPCHK(p, hipMemsetAsync(p->d_tmp2, 0, nb * p->N * ssize(p), p->stream));
-  KCHK(p, apply_A(p, -alpha, p->d_tmp, beta, p->d_tmp2, p->nshots));
-  KCHK(p, launch_gather(p->state, nb, p->N, p->d_nperm, p->d_tmp2, Y, p->stream));
+    HIPCHK(p, hipMemsetAsync(p->d_tmp2, 0, nstate(p) * ssize(p), p->stream));
+  KCHK(p, apply_A(p, -alpha, p->d_tmp, beta, p->d_tmp2, S));
+  KCHK(p, launch_from_solver(p->state, S, p->ncomp, p->N, p->ilv, p->d_nperm, p->d_tmp2, Y,
+                             p->stream));
   HIPCHK(p, hipStreamSynchronize(p->stream));
   return MJW_OK;
 }
@@ -630,12 +652,21 @@ extern "C" int mjw_node_order(const mjw_problem *p, int32_t *solver_index) {
   return MJW_OK;
 }
 
-// host FP64 vectors (caller's node order, n = blocks of N) -> device state
-// precision in solver order
-static int upload_vec(mjw_problem *p, const double *h, void *dst, int64_t n) {
+// host FP64 vectors in the caller's layout [Sv][ncomp][N] and numbering <->
+// device vectors in the solver's (state precision; Sv > 1 on a problem with
+// interleaved shots uses the interleaved layout)
+static int64_t solver_slot(const mjw_problem *p, int Sv, int64_t s, int64_t m, int64_t i) {
+  const int64_t per = (int64_t)p->ncomp * p->N, mi = m * p->N + i;
+  return (p->ilv && Sv > 1) ? mi * Sv + s : s * per + mi;
+}
+
+static int upload_vec(mjw_problem *p, const double *h, void *dst, int Sv) {
+  const int64_t per = (int64_t)p->ncomp * p->N, n = per * Sv;
   std::vector<double> hs((size_t)n);
-  for (int64_t b = 0; b < n; b += p->N)
-    for (int64_t g = 0; g < p->N; ++g) hs[b + p->nperm[g]] = h[b + g];
+  for (int64_t s = 0; s < Sv; ++s)
+    for (int64_t m = 0; m < p->ncomp; ++m)
+      for (int64_t g = 0; g < p->N; ++g)
+        hs[solver_slot(p, Sv, s, m, p->nperm[g])] = h[s * per + m * p->N + g];
   int rc = (int)hipStreamSynchronize(p->stream);
   if (rc) return rc;
   if (p->state == kF64)
@@ -649,8 +680,8 @@ static int upload_vec(mjw_problem *p, const double *h, void *dst, int64_t n) {
   return rc;
 }
 
-// device state (solver order) -> host FP64 in the caller's node order
-static int download_vec(mjw_problem *p, const void *src, double *h, int64_t n) {
+static int download_vec(mjw_problem *p, const void *src, double *h, int Sv) {
+  const int64_t per = (int64_t)p->ncomp * p->N, n = per * Sv;
   std::vector<double> hs((size_t)n);
   int rc = 0;
   if (p->state == kF64) {
@@ -665,8 +696,10 @@ static int download_vec(mjw_problem *p, const void *src, double *h, int64_t n) {
     dfree(tmp);
   }
   if (rc) return rc;
-  for (int64_t b = 0; b < n; b += p->N)
-    for (int64_t g = 0; g < p->N; ++g) h[b + g] = hs[b + p->nperm[g]];
+  for (int64_t s = 0; s < Sv; ++s)
+    for (int64_t m = 0; m < p->ncomp; ++m)
+      for (int64_t g = 0; g < p->N; ++g)
+        h[s * per + m * p->N + g] = hs[solver_slot(p, Sv, s, m, p->nperm[g])];
   return 0;
 }
 
@@ -675,13 +708,13 @@ extern "C" int mjw_apply_kh(mjw_problem *p, double alpha, const double *U, doubl
   if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_kh: NULL");
   HIPCHK(p, hipSetDevice(p->device));
   const int64_t n = (int64_t)p->ncomp * p->N;
-  KCHK(p, upload_vec(p, U, p->d_tmp, n));
+  KCHK(p, upload_vec(p, U, p->d_tmp, 1));
   if (beta != 0.0)
-    KCHK(p, upload_vec(p, Y, p->d_tmp2, n));
+    KCHK(p, upload_vec(p, Y, p->d_tmp2, 1));
   else
     HIPCHK(p, hipMemsetAsync(p->d_tmp2, 0, n * ssize(p), p->stream));
   KCHK(p, apply_A(p, -alpha, p->d_tmp, beta, p->d_tmp2, 1));
-  KCHK(p, download_vec(p, p->d_tmp2, Y, n));
+  KCHK(p, download_vec(p, p->d_tmp2, Y, 1));
   return MJW_OK;
 }
 
@@ -900,25 +933,22 @@ extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, cons
   p->cur = 0;
   void *U0 = p->d_U[0], *U1 = p->d_U[1];
   if (u0) {
-    for (int s = 0; s < p->nshots; ++s)
-      KCHK(p, upload_vec(p, u0 + (size_t)s * n1, (char *)U0 + (size_t)s * n1 * ssize(p), n1));
+    KCHK(p, upload_vec(p, u0, U0, p->nshots));
   } else {
     HIPCHK(p, hipMemsetAsync(U0, 0, ns * ssize(p), st));
   }
-  KCHK(p, launch_mask(p->state, ns, p->nshots, p->ncomp, p->N, p->d_dir, U0, st));
+  KCHK(p, launch_mask(p->state, ns, p->nshots, p->ilv, p->N, p->d_dir, U0, st));
   // U1 = ((0.5 dt) D^{-1}(-A U0) + du0) dt + U0   (ModifiedEquation.cxx:112-118)
   KCHK(p, apply_A(p, -1.0, U0, 0.0, p->d_tmp, p->nshots));
-  KCHK(p, launch_scale_by(p->state, ns, p->nshots, p->ncomp, p->N, p->d_Minv, 0.5 * dt, p->d_tmp,
+  KCHK(p, launch_scale_by(p->state, ns, p->nshots, p->ilv, p->N, p->d_Minv, 0.5 * dt, p->d_tmp,
                           st));
   if (du0) {
-    for (int s = 0; s < p->nshots; ++s)
-      KCHK(p, upload_vec(p, du0 + (size_t)s * n1, (char *)p->d_tmp2 + (size_t)s * n1 * ssize(p),
-                         n1));
+    KCHK(p, upload_vec(p, du0, p->d_tmp2, p->nshots));
     KCHK(p, launch_axpby(p->state, ns, 1.0, p->d_tmp2, 1.0, p->d_tmp, st));
   }
   HIPCHK(p, hipMemcpyAsync(U1, U0, ns * ssize(p), hipMemcpyDeviceToDevice, st));
   KCHK(p, launch_axpby(p->state, ns, dt, p->d_tmp, 1.0, U1, st));
-  KCHK(p, launch_mask(p->state, ns, p->nshots, p->ncomp, p->N, p->d_dir, U1, st));
+  KCHK(p, launch_mask(p->state, ns, p->nshots, p->ilv, p->N, p->d_dir, U1, st));
   p->step = 0;
   int64_t ctl[2] = {0, 0};
   HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, st));
@@ -945,7 +975,7 @@ static int enqueue_step(mjw_problem *p, int parity, double *traces) {
   t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot; t.src_val = p->d_src_val;
   t.sdiag = p->d_sdiag; t.nrec = traces ? p->nrec : 0; t.nd = p->nd; t.rec_gid = p->d_rec_gid;
   t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = Ucur; t.P = Y;
-  t.traces = (traces && p->nrec) ? traces : nullptr; t.N = p->N; t.nshots = p->nshots;
+  t.traces = (traces && p->nrec) ? traces : nullptr; t.N = p->N; t.nshots = p->nshots; t.ilv = p->ilv;
   t.advance = 1;
   t.row_offset = 1;
   return launch_tail_n(t, p->ncomp, p->stream);
@@ -956,7 +986,7 @@ static int enqueue_record(mjw_problem *p, double *traces) {
   TailArgs t{};
   t.state = p->state; t.ctl = p->d_ctl; t.nrec = p->nrec; t.nd = p->nd; t.rec_gid = p->d_rec_gid;
   t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = p->d_U[p->cur]; t.P = nullptr;
-  t.traces = traces; t.N = p->N; t.nshots = p->nshots; t.advance = 0; t.row_offset = 0;
+  t.traces = traces; t.N = p->N; t.nshots = p->nshots; t.ilv = p->ilv; t.advance = 0; t.row_offset = 0;
   return launch_tail_n(t, p->ncomp, p->stream);
 }
 
@@ -1064,8 +1094,8 @@ extern "C" int mjw_get_state(mjw_problem *p, double *Un, double *Un1) {
   if (!p) FAIL(p, MJW_ERR_ARG, "NULL problem");
   HIPCHK(p, hipSetDevice(p->device));
   const int64_t ns = nstate(p);
-  if (Un) KCHK(p, download_vec(p, p->d_U[p->cur], Un, ns));
-  if (Un1) KCHK(p, download_vec(p, p->d_U[p->cur ^ 1], Un1, ns));
+  if (Un) KCHK(p, download_vec(p, p->d_U[p->cur], Un, p->nshots));
+  if (Un1) KCHK(p, download_vec(p, p->d_U[p->cur ^ 1], Un1, p->nshots));
   return MJW_OK;
 }
 
@@ -1116,7 +1146,7 @@ extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
     t.state = p->state; t.ctl = p->d_ctl; t.pulse = p->d_pulse; t.npulse = p->npulse;
     t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot;
     t.src_val = p->d_src_val; t.sdiag = p->d_sdiag; t.nrec = 0; t.nd = p->nd; t.U = U; t.P = Y;
-    t.N = p->N; t.nshots = p->nshots; t.advance = 1; t.row_offset = 1;
+    t.N = p->N; t.nshots = p->nshots; t.ilv = p->ilv; t.advance = 1; t.row_offset = 1;
     KCHK(p, launch_tail_n(t, p->ncomp, st));
     HIPCHK(p, hipEventRecord(ev[k++], st));
     p->cur ^= 1;
@@ -1150,9 +1180,9 @@ extern "C" int mjw_energy(mjw_problem *p, double *energy) {
   const int S = p->nshots;
   void *Un = p->d_U[p->cur], *Un1 = p->d_U[p->cur ^ 1];
   KCHK(p, apply_A(p, 1.0, Un, 0.0, p->d_tmp, S));
-  KCHK(p, launch_dot_part(p->state, p->N, p->ncomp, S, nullptr, Un1, p->d_tmp, p->d_red,
+  KCHK(p, launch_dot_part(p->state, p->N, p->ncomp, S, p->ilv, nullptr, Un1, p->d_tmp, p->d_red,
                           p->d_part, st));
-  KCHK(p, launch_dot_diff_part(p->state, p->N, p->ncomp, S, p->d_M, Un1, Un, p->d_red + S,
+  KCHK(p, launch_dot_diff_part(p->state, p->N, p->ncomp, S, p->ilv, p->d_M, Un1, Un, p->d_red + S,
                                p->d_part, st));
   std::vector<double> h(2 * S);
   HIPCHK(p, hipMemcpyAsync(h.data(), p->d_red, 2 * S * 8, hipMemcpyDeviceToHost, st));
@@ -1210,16 +1240,16 @@ extern "C" int mjw_lambda_max(mjw_problem *p, int iters, uint64_t seed, double *
   for (double v : h0) nrm += v * v;
   nrm = std::sqrt(nrm);
   for (double &v : h0) v /= nrm;
-  int rc = upload_vec(p, h0.data(), q, n1);
+  int rc = upload_vec(p, h0.data(), q, 1);
   if (!rc) rc = (int)hipMemsetAsync(qo, 0, b, st);
   std::vector<double> al, be;
   double beta = 0.0, red[2];
   for (int k = 0; k < iters && !rc; ++k) {
     rc = (int)hipMemcpyAsync(t, q, b, hipMemcpyDeviceToDevice, st);
-    if (!rc) rc = launch_scale_by(p->state, n1, 1, p->ncomp, p->N, p->d_isqM, 1.0, t, st);
+    if (!rc) rc = launch_scale_by(p->state, n1, 1, 0, p->N, p->d_isqM, 1.0, t, st);
     if (!rc) rc = apply_A(p, 1.0, t, 0.0, w, 1);
-    if (!rc) rc = launch_scale_by(p->state, n1, 1, p->ncomp, p->N, p->d_isqM, 1.0, w, st);
-    if (!rc) rc = launch_dot_part(p->state, p->N, p->ncomp, 1, nullptr, w, q, p->d_red, p->d_part, st);
+    if (!rc) rc = launch_scale_by(p->state, n1, 1, 0, p->N, p->d_isqM, 1.0, w, st);
+    if (!rc) rc = launch_dot_part(p->state, p->N, p->ncomp, 1, 0, nullptr, w, q, p->d_red, p->d_part, st);
     if (!rc) rc = (int)hipMemcpyAsync(red, p->d_red, 8, hipMemcpyDeviceToHost, st);
     if (!rc) rc = (int)hipStreamSynchronize(st);
     if (rc) break;
@@ -1228,7 +1258,7 @@ extern "C" int mjw_lambda_max(mjw_problem *p, int iters, uint64_t seed, double *
     // w = w - alpha q - beta qo
     rc = launch_axpby(p->state, n1, -alpha, q, 1.0, w, st);
     if (!rc) rc = launch_axpby(p->state, n1, -beta, qo, 1.0, w, st);
-    if (!rc) rc = launch_dot_part(p->state, p->N, p->ncomp, 1, nullptr, w, w, p->d_red, p->d_part, st);
+    if (!rc) rc = launch_dot_part(p->state, p->N, p->ncomp, 1, 0, nullptr, w, w, p->d_red, p->d_part, st);
     if (!rc) rc = (int)hipMemcpyAsync(red, p->d_red, 8, hipMemcpyDeviceToHost, st);
     if (!rc) rc = (int)hipStreamSynchronize(st);
     if (rc) break;
@@ -1747,7 +1777,7 @@ extern "C" int mjw_batch_destroy(mjw_batch *b) {
 
 extern "C" int mjw_tile_choices(const mjw_problem *p, int *out, int cap) {
   if (!p || (!out && cap > 0)) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL");
-  std::vector<int> v = tile_choices(p->dim, p->order, p->ncomp);
+  std::vector<int> v = problem_tiles(p);
   for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
   return (int)v.size();
 }

@@ -78,12 +78,18 @@ struct StiffArgs {
   int nshots;
   int nxcd;                           // XCD-aware block order (1 = off)
   double *grad;                       // adjoint step: per-element gradient (sorted), or null
+  int ilv;                            // several shots interleaved: u[(m*N + g)*S + s]
   int geo;                            // GeoMode (trilinear: 3-D scalar only)
   const void *coef;                   // trilinear: E*kCoef values, geometry precision
   const void *glxw;                   // trilinear: GL points then weights (n each), geometry precision
 };
 // returns 0 or a HIP error code; `stream` is a hipStream_t
 int launch_stiffness(const StiffArgs &a, void *stream);
+// 3-D acoustic, several shots in the interleaved layout; StiffArgs::epb =
+// shot lanes per column (0: automatic)
+int launch_stiffness_shots(const StiffArgs &a, void *stream);
+// supported shot-lane counts, largest first
+std::vector<int> tile_choices_shots(int order, int state, int geom);
 // list of supported elements-per-block for (dim, order, ncomp)
 std::vector<int> tile_choices(int dim, int order, int ncomp);
 int default_tile(int dim, int order, int ncomp, int state);
@@ -129,6 +135,7 @@ struct TailArgs {
   int64_t adj_row;               // row of adj_g = step + adj_row
   int adj_nrec;
   int adj_shot;
+  int ilv;                       // state layout: shots interleaved (1) or consecutive (0)
 };
 int launch_tail_n(const TailArgs &a, int ncomp, void *stream);
 
@@ -136,23 +143,28 @@ int launch_tail_n(const TailArgs &a, int ncomp, void *stream);
 int launch_axpby(int state, int64_t n, double a, const void *x, double b, void *y, void *stream);
 int launch_convert_to_f64(int state, int64_t n, const void *x, double *y, void *stream);
 int launch_convert_from_f64(int state, int64_t n, const double *x, void *y, void *stream);
-// y[b*N + j] = x[b*N + idx[j]], b < nb (x and y distinct)
-int launch_gather(int state, int64_t nb, int64_t N, const int32_t *idx, const void *x, void *y,
-                  void *stream);
-// y[i] *= c*d[i % N]
-int launch_scale_by(int state, int64_t n, int S, int ncomp, int64_t N, const double *d, double c,
+// caller's layout [S][ncomp][N] (caller's node numbering) <-> solver layout
+// (solver node order; shots consecutive, or interleaved [ncomp][N][S] when
+// ilv): to_solver gathers through ninv (solver -> caller node), from_solver
+// through nperm (caller -> solver node); x and y distinct
+int launch_to_solver(int state, int S, int ncomp, int64_t N, int ilv, const int32_t *ninv,
+                     const void *x, void *y, void *stream);
+int launch_from_solver(int state, int S, int ncomp, int64_t N, int ilv, const int32_t *nperm,
+                       const void *x, void *y, void *stream);
+// y[i] *= c*d[node(i)], node(i) = (ilv ? i / S : i) % N
+int launch_scale_by(int state, int64_t n, int S, int ilv, int64_t N, const double *d, double c,
                     void *y, void *stream);
-// y[i] = 0 where dir[i % N]
-int launch_mask(int state, int64_t n, int S, int ncomp, int64_t N, const uint8_t *dir, void *y,
+// y[i] = 0 where dir[node(i)]
+int launch_mask(int state, int64_t n, int S, int ilv, int64_t N, const uint8_t *dir, void *y,
                 void *stream);
 int launch_make_sdiag(int state, int64_t N, const double *M, const uint8_t *dir, double dt2,
                       void *s, void *stream);
 // per shot: out[s] = sum_i w[i%N] x_i y_i  /  sum_i w[i%N] (a_i-b_i)^2, FP64,
 // deterministic; part = scratch of reduction_scratch_doubles(S) doubles
-int launch_dot_part(int state, int64_t N, int ncomp, int S, const double *w, const void *x,
-                    const void *y, double *out, double *part, void *stream);
-int launch_dot_diff_part(int state, int64_t N, int ncomp, int S, const double *w, const void *a,
-                         const void *b, double *out, double *part, void *stream);
+int launch_dot_part(int state, int64_t N, int ncomp, int S, int ilv, const double *w,
+                    const void *x, const void *y, double *out, double *part, void *stream);
+int launch_dot_diff_part(int state, int64_t N, int ncomp, int S, int ilv, const double *w,
+                         const void *a, const void *b, double *out, double *part, void *stream);
 int reduction_scratch_doubles(int S);
 // G planes *= sc[e] (per element, sorted order); g32: float planes
 int launch_scale_planes(int g32, void *G, int64_t nplanes, int64_t E, int P, const double *sc,

@@ -16,6 +16,7 @@ std::vector<int> tile_choices_elastic(int order);
 
 int launch_stiffness(const StiffArgs &a, void *stream) {
   if (a.ncomp == 3) return launch_stiffness_elastic(a, stream);
+  if (a.ilv && a.dim == 3 && !a.grad) return launch_stiffness_shots(a, stream);
   return launch_stiffness_scalar(a, stream);
 }
 
@@ -197,7 +198,7 @@ __device__ __forceinline__ void tail_body(const TailArgs &a, int ncomp) {
       const int64_t dof = a.src_dof[idx];
       const int s = a.src_shot[idx];
       const int64_t g = dof % a.N;
-      T *y = P + (int64_t)s * NT + dof;
+      T *y = P + (a.ilv ? dof * a.nshots + s : (int64_t)s * NT + dof);
       *y += sd[g] * (T)(pulse * a.src_val[idx]);
     }
   }
@@ -207,10 +208,13 @@ __device__ __forceinline__ void tail_body(const TailArgs &a, int ncomp) {
     const int tot = a.nrec * a.nshots;
     for (int it = wave; it < tot; it += nw) {
       const int r = it % a.nrec, s = it / a.nrec;
-      const T *Us = U + (int64_t)s * NT + (int64_t)a.rec_comp[r] * a.N;
+      const int64_t cbase = (int64_t)a.rec_comp[r] * a.N;
       double acc = 0.0;
-      for (int q = lane; q < a.nd; q += 64)
-        acc += a.rec_w[(int64_t)r * a.nd + q] * (double)Us[a.rec_gid[(int64_t)r * a.nd + q]];
+      for (int q = lane; q < a.nd; q += 64) {
+        const int64_t d = cbase + a.rec_gid[(int64_t)r * a.nd + q];
+        acc += a.rec_w[(int64_t)r * a.nd + q] *
+               (double)U[a.ilv ? d * a.nshots + s : (int64_t)s * NT + d];
+      }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
       if (lane == 0) a.traces[((step - base + a.row_offset) * a.nshots + s) * a.nrec + r] = acc;
@@ -311,72 +315,104 @@ int launch_convert_from_f64(int state, int64_t n, const double *x, void *y, void
   return (int)hipGetLastError();
 }
 
-// y[b*N + j] = x[b*N + idx[j]] for nb blocks of N (node renumbering)
+// y[i] *= c*d[node(i)], node(i) = (i / div) % N (div = S interleaved, else 1)
 template <typename T>
-__global__ void k_gather(int64_t n, int64_t N, const int32_t *__restrict__ idx,
-                         const T *__restrict__ x, T *__restrict__ y) {
+__global__ void k_scale_by(int64_t n, int64_t N, int64_t div, const double *__restrict__ d,
+                           double c, T *__restrict__ y) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = i / N, j = i - b * N;
-    y[i] = x[b * N + idx[j]];
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (T)((double)y[i] * (c * d[(i / div) % N]));
+}
+
+int launch_scale_by(int state, int64_t n, int S, int ilv, int64_t N, const double *d, double c,
+                    void *y, void *stream) {
+  const int64_t div = ilv ? S : 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_scale_by<double>, dim3(grid_for(n)), dim3(256), 0, st, n, N, div, d, c,
+                       (double *)y);
+  else
+    hipLaunchKernelGGL(k_scale_by<float>, dim3(grid_for(n)), dim3(256), 0, st, n, N, div, d, c,
+                       (float *)y);
+  return (int)hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_mask(int64_t n, int64_t N, int64_t div, const uint8_t *__restrict__ dir,
+                       T *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (dir[(i / div) % N]) y[i] = T(0);
+}
+
+int launch_mask(int state, int64_t n, int S, int ilv, int64_t N, const uint8_t *dir, void *y,
+                void *stream) {
+  if (!dir) return 0;
+  const int64_t div = ilv ? S : 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_mask<double>, dim3(grid_for(n)), dim3(256), 0, st, n, N, div, dir,
+                       (double *)y);
+  else
+    hipLaunchKernelGGL(k_mask<float>, dim3(grid_for(n)), dim3(256), 0, st, n, N, div, dir,
+                       (float *)y);
+  return (int)hipGetLastError();
+}
+
+// caller layout [S][ncomp][N] <-> solver layout (see mjw_internal.hpp)
+__device__ __forceinline__ int64_t solver_index(int64_t s, int64_t mN_g, int S, int64_t per,
+                                                int ilv) {
+  return ilv ? mN_g * S + s : s * per + mN_g;
+}
+
+template <typename T>
+__global__ void k_to_solver(int S, int ncomp, int64_t N, int ilv, const int32_t *__restrict__ ninv,
+                            const T *__restrict__ x, T *__restrict__ y) {
+  const int64_t per = (int64_t)ncomp * N, n = per * S;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    // u runs over the caller's layout; the value of solver node i = nperm[g]
+    // comes from caller node ninv[i]: write solver slot of (s, m, i)
+    const int64_t s = u / per, r = u - s * per, m = r / N, i = r - m * N;
+    y[solver_index(s, m * N + i, S, per, ilv)] = x[s * per + m * N + ninv[i]];
   }
 }
 
-int launch_gather(int state, int64_t nb, int64_t N, const int32_t *idx, const void *x, void *y,
-                  void *stream) {
+template <typename T>
+__global__ void k_from_solver(int S, int ncomp, int64_t N, int ilv,
+                              const int32_t *__restrict__ nperm, const T *__restrict__ x,
+                              T *__restrict__ y) {
+  const int64_t per = (int64_t)ncomp * N, n = per * S;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = u / per, r = u - s * per, m = r / N, g = r - m * N;
+    y[u] = x[solver_index(s, m * N + nperm[g], S, per, ilv)];
+  }
+}
+
+int launch_to_solver(int state, int S, int ncomp, int64_t N, int ilv, const int32_t *ninv,
+                     const void *x, void *y, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const int64_t n = nb * N;
+  const int64_t n = (int64_t)S * ncomp * N;
   if (state == kF64)
-    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(256), 0, st, n, N, idx,
-                       (const double *)x, (double *)y);
+    hipLaunchKernelGGL(k_to_solver<double>, dim3(grid_for(n)), dim3(256), 0, st, S, ncomp, N, ilv,
+                       ninv, (const double *)x, (double *)y);
   else
-    hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n)), dim3(256), 0, st, n, N, idx,
-                       (const float *)x, (float *)y);
+    hipLaunchKernelGGL(k_to_solver<float>, dim3(grid_for(n)), dim3(256), 0, st, S, ncomp, N, ilv,
+                       ninv, (const float *)x, (float *)y);
   return (int)hipGetLastError();
 }
 
-// y[(s*ncomp+m)*N+g] *= c*d[g]
-template <typename T>
-__global__ void k_scale_by(int64_t n, int64_t N, const double *__restrict__ d, double c,
-                           T *__restrict__ y) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = (T)((double)y[i] * (c * d[i % N]));
-}
-
-int launch_scale_by(int state, int64_t n, int S, int ncomp, int64_t N, const double *d, double c,
-                    void *y, void *stream) {
-  (void)S;
-  (void)ncomp;
+int launch_from_solver(int state, int S, int ncomp, int64_t N, int ilv, const int32_t *nperm,
+                       const void *x, void *y, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t n = (int64_t)S * ncomp * N;
   if (state == kF64)
-    hipLaunchKernelGGL(k_scale_by<double>, dim3(grid_for(n)), dim3(256), 0, st, n, N, d, c,
-                       (double *)y);
+    hipLaunchKernelGGL(k_from_solver<double>, dim3(grid_for(n)), dim3(256), 0, st, S, ncomp, N,
+                       ilv, nperm, (const double *)x, (double *)y);
   else
-    hipLaunchKernelGGL(k_scale_by<float>, dim3(grid_for(n)), dim3(256), 0, st, n, N, d, c,
-                       (float *)y);
-  return (int)hipGetLastError();
-}
-
-template <typename T>
-__global__ void k_mask(int64_t n, int64_t N, const uint8_t *__restrict__ dir, T *__restrict__ y) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    if (dir[i % N]) y[i] = T(0);
-}
-
-int launch_mask(int state, int64_t n, int S, int ncomp, int64_t N, const uint8_t *dir, void *y,
-                void *stream) {
-  (void)S;
-  (void)ncomp;
-  if (!dir) return 0;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (state == kF64)
-    hipLaunchKernelGGL(k_mask<double>, dim3(grid_for(n)), dim3(256), 0, st, n, N, dir,
-                       (double *)y);
-  else
-    hipLaunchKernelGGL(k_mask<float>, dim3(grid_for(n)), dim3(256), 0, st, n, N, dir,
-                       (float *)y);
+    hipLaunchKernelGGL(k_from_solver<float>, dim3(grid_for(n)), dim3(256), 0, st, S, ncomp, N,
+                       ilv, nperm, (const float *)x, (float *)y);
   return (int)hipGetLastError();
 }
 
@@ -429,23 +465,24 @@ int launch_scale_planes(int g32, void *G, int64_t nplanes, int64_t E, int P, con
 constexpr int kRedBlocks = 1024;
 
 template <typename T, int DIFF>
-__global__ __launch_bounds__(256) void k_dot_partial(int64_t per_shot, int64_t N,
+__global__ __launch_bounds__(256) void k_dot_partial(int64_t per_shot, int64_t N, int ilv,
                                                      const double *__restrict__ w,
                                                      const T *__restrict__ x,
                                                      const T *__restrict__ y,
                                                      double *__restrict__ part) {
-  const int s = blockIdx.y;
-  const T *xs = x + (int64_t)s * per_shot;
-  const T *ys = y + (int64_t)s * per_shot;
+  // shot s, entry i (= m*N + g): x[i*S + s] when interleaved, x[s*per + i] else
+  const int s = blockIdx.y, S = gridDim.y;
+  const int64_t base = ilv ? s : (int64_t)s * per_shot, stride = ilv ? S : 1;
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < per_shot;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double wi = w ? w[i % N] : 1.0;
+    const int64_t k = base + i * stride;
     if (DIFF) {
-      const double d = (double)xs[i] - (double)ys[i];
+      const double d = (double)x[k] - (double)y[k];
       acc += wi * d * d;
     } else {
-      acc += wi * (double)xs[i] * (double)ys[i];
+      acc += wi * (double)x[k] * (double)y[k];
     }
   }
   __shared__ double sh[256];
@@ -474,29 +511,29 @@ __global__ __launch_bounds__(256) void k_dot_final(int nb, const double *__restr
 }
 
 template <int DIFF>
-static int dot_impl(int state, int64_t N, int ncomp, int S, const double *w, const void *x,
-                    const void *y, double *out, double *part, void *stream) {
+static int dot_impl(int state, int64_t N, int ncomp, int S, int ilv, const double *w,
+                    const void *x, const void *y, double *out, double *part, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int64_t per = (int64_t)ncomp * N;
   int nb = (int)std::min<int64_t>(kRedBlocks, (per + 255) / 256);
   if (nb < 1) nb = 1;
   if (state == kF64)
-    hipLaunchKernelGGL((k_dot_partial<double, DIFF>), dim3(nb, S), dim3(256), 0, st, per, N, w,
-                       (const double *)x, (const double *)y, part);
+    hipLaunchKernelGGL((k_dot_partial<double, DIFF>), dim3(nb, S), dim3(256), 0, st, per, N, ilv,
+                       w, (const double *)x, (const double *)y, part);
   else
-    hipLaunchKernelGGL((k_dot_partial<float, DIFF>), dim3(nb, S), dim3(256), 0, st, per, N, w,
-                       (const float *)x, (const float *)y, part);
+    hipLaunchKernelGGL((k_dot_partial<float, DIFF>), dim3(nb, S), dim3(256), 0, st, per, N, ilv,
+                       w, (const float *)x, (const float *)y, part);
   hipLaunchKernelGGL(k_dot_final, dim3(S), dim3(256), 0, st, nb, part, out);
   return (int)hipGetLastError();
 }
 
-int launch_dot_part(int state, int64_t N, int ncomp, int S, const double *w, const void *x,
-                    const void *y, double *out, double *part, void *stream) {
-  return dot_impl<0>(state, N, ncomp, S, w, x, y, out, part, stream);
+int launch_dot_part(int state, int64_t N, int ncomp, int S, int ilv, const double *w,
+                    const void *x, const void *y, double *out, double *part, void *stream) {
+  return dot_impl<0>(state, N, ncomp, S, ilv, w, x, y, out, part, stream);
 }
-int launch_dot_diff_part(int state, int64_t N, int ncomp, int S, const double *w, const void *a,
-                         const void *b, double *out, double *part, void *stream) {
-  return dot_impl<1>(state, N, ncomp, S, w, a, b, out, part, stream);
+int launch_dot_diff_part(int state, int64_t N, int ncomp, int S, int ilv, const double *w,
+                         const void *a, const void *b, double *out, double *part, void *stream) {
+  return dot_impl<1>(state, N, ncomp, S, ilv, w, a, b, out, part, stream);
 }
 int reduction_scratch_doubles(int S) { return kRedBlocks * S; }
 

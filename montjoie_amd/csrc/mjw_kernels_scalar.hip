@@ -727,17 +727,15 @@ static void launch3(const StiffArgs &a, hipStream_t st, unsigned blocks) {
                      static_cast<const TG *>(a.coef), static_cast<const TG *>(a.glxw));
 }
 
-// several shots keep the column's tensor in registers (HOIST): loaded once
-// (stored geometry) or built once (trilinear geometry, FP32 state, where the
-// registers fit; the tensor form G b then replaces (mu w/J) R (R^T b), so
-// such shots agree with single-shot runs to rounding, not bit for bit)
+// several shots in the consecutive layout (the adjoint's pair of states)
+// keep the column's stored tensor in registers (HOIST); 3-D problems with
+// several shots take the interleaved kernel (mjw_kernels_shots.hip)
 template <int N, int EPB, typename T, typename TG, bool TRI>
 static void run3(const StiffArgs &a, hipStream_t st, unsigned blocks) {
-  constexpr bool HOIST_TRI = TRI && sizeof(T) == 4;
   if (a.grad)
     launch3<N, EPB, T, TG, kLeap, !TRI, true, TRI>(a, st, blocks);
-  else if (a.mode == kLeap && a.nshots > 1 && (!TRI || HOIST_TRI))
-    launch3<N, EPB, T, TG, kLeap, true, false, TRI>(a, st, blocks);
+  else if (a.mode == kLeap && a.nshots > 1 && !TRI)
+    launch3<N, EPB, T, TG, kLeap, true, false, false>(a, st, blocks);
   else if (a.mode == kLeap)
     launch3<N, EPB, T, TG, kLeap, false, false, TRI>(a, st, blocks);
   else

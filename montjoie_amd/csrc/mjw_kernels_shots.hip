@@ -1,0 +1,267 @@
+// Stiffness pass of a 3-D acoustic problem with several shots (right-hand
+// sides) in the shot-interleaved layout u[g*S + s], gfx950.
+//
+// Same arithmetic per node and shot as the column kernel
+// (mjw_kernels_scalar.hip, restating MltAdd_ElementH1_Scalar,
+// Computation/ProdMatVectScalarH1.cxx:256-668), mapped for many shots: a
+// block takes one element and ST consecutive shots; thread (i + n j, lane)
+// owns column (i, j) of shot s0 + lane and loops k.  The ST lanes of a
+// column gather one contiguous run of ST values per node, so a wavefront's
+// loads of U, Y and dt^2/M are ST times fewer cache lines than ST separate
+// single-shot passes.  The element's geometry is built once per block into
+// LDS and read by every shot: the adjugate rows of DF and mu w / J per node
+// (trilinear map; the flux is the single-shot kernels' f R (R^T b), so
+// results equal single-shot runs bit for bit) or the stored tensor.  LDS
+// state is [k][i + n j][lane] so lanes hit consecutive banks.
+#include <hip/hip_runtime.h>
+
+#include "mjw_device.hpp"
+#include "mjw_internal.hpp"
+#include "mjw_tri.hpp"
+
+namespace mjw {
+
+template <int N, int ST, typename T, typename TG, int MODE, bool TRI>
+__global__ __launch_bounds__(N *N *ST) void k_shots3d(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
+    const T *__restrict__ sdiag, T alpha, T beta, int nshots, int nxcd,
+    const TG *__restrict__ coef, const TG *__restrict__ glxw) {
+  constexpr int P = N * N, NN = P * N;
+  using A = TG;
+  __shared__ T su[NN * ST], s1[NN * ST], s2[NN * ST];
+  constexpr int NG = TRI ? 10 : 6;  // per node: r0 r1 r2 f, or the 6 tensor entries
+  __shared__ A lg[P * N * NG];      // [i + n j][k][NG]
+  __shared__ TG lc[TRI ? kCoef : 1];
+  const int t = threadIdx.x;
+  const int lane = t % ST, ij = t / ST;
+  const int i = ij % N, j = ij / N;
+  const int64_t e = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd);
+  const int s = blockIdx.y * ST + lane;
+  const bool act = e < e1, sact = act && s < nshots;
+  // element tensor into LDS: lanes 0..n-1 of each column build layer k = lane
+  if constexpr (TRI) {
+    for (int q = t; q < kCoef; q += P * ST)
+      if (act) lc[q] = coef[e * kCoef + q];
+    __syncthreads();
+    if (act && lane < N) {
+      TriCol<A> tc;
+      tri_setup<A, TG>(lc, (A)glxw[i], (A)glxw[j], (A)glxw[N + i] * (A)glxw[N + j], true, tc);
+      for (int k = lane; k < N; k += ST) {
+        A *q = lg + (ij * N + k) * NG;
+        tri_rows_f<A>(tc, (A)glxw[k], (A)glxw[N + k], q, q + 3, q + 6, q[9]);
+      }
+    }
+  } else {
+    if (act)
+      for (int k = lane; k < N; k += ST)
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+          lg[(ij * N + k) * NG + c] = (A)G[((int64_t)(k * 6 + c) * E + e) * P + ij];
+  }
+  int32_t gid[N];
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) gid[k] = map[((int64_t)k * E + e) * P + ij];
+  }
+  const rsrc_t ru = mkres(U), ry = mkres(Y), rs = mkres(sdiag);
+  T u[N];
+  if (sact) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const uint32_t g = (uint32_t)(gid[k] & 0x7fffffff);
+      u[k] = bld<T>(ru, (g * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T), 0u);
+      su[(k * P + ij) * ST + lane] = u[k];
+    }
+  }
+  __syncthreads();
+  T v3[N];
+  if (sact) {
+    T Di[N], Dj[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      Di[l] = Dm[i * N + l];
+      Dj[l] = Dm[j * N + l];
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      T a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        a1 += Di[l] * su[(k * P + j * N + l) * ST + lane];
+        a2 += Dj[l] * su[(k * P + l * N + i) * ST + lane];
+        a3 += Dm[k * N + l] * u[l];
+      }
+      const A b1 = a1, b2 = a2, b3 = a3;
+      const A *g = lg + (ij * N + k) * NG;
+      if constexpr (TRI) {
+        A w1, w2, w3;
+        tri_flux<A>(g, g + 3, g + 6, g[9], b1, b2, b3, w1, w2, w3);
+        s1[(k * P + ij) * ST + lane] = (T)w1;
+        s2[(k * P + ij) * ST + lane] = (T)w2;
+        v3[k] = (T)w3;
+      } else {
+        s1[(k * P + ij) * ST + lane] = (T)(g[0] * b1 + g[1] * b2 + g[2] * b3);
+        s2[(k * P + ij) * ST + lane] = (T)(g[1] * b1 + g[3] * b2 + g[4] * b3);
+        v3[k] = (T)(g[2] * b1 + g[4] * b2 + g[5] * b3);
+      }
+    }
+  }
+  // previous values and dt^2/M before the scatter's stores
+  T yk[N], sk[MODE == kLeap ? N : 1];
+  if (sact) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const uint32_t g = (uint32_t)(gid[k] & 0x7fffffff);
+      yk[k] = bld<T>(ry, (g * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T), 0u);
+      if constexpr (MODE == kLeap) sk[k] = bld<T>(rs, g * (uint32_t)sizeof(T), 0u);
+    }
+  }
+  __syncthreads();
+  if (sact) {
+    T Dti[N], Dtj[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      Dti[l] = Dm[l * N + i];
+      Dtj[l] = Dm[l * N + j];
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      T o = 0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        o += Dti[l] * s1[(k * P + j * N + l) * ST + lane];
+        o += Dtj[l] * s2[(k * P + l * N + i) * ST + lane];
+        o += Dm[l * N + k] * v3[l];
+      }
+      const int32_t m = gid[k];
+      const uint32_t off =
+          ((uint32_t)(m & 0x7fffffff) * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T);
+      const bool first = m < 0;
+      const T y = yk[k];
+      T r;
+      if constexpr (MODE == kLeap)
+        r = first ? (T(2) * u[k] - y - sk[k] * o) : (y - sk[k] * o);
+      else
+        r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
+      bst<T>(r, ry, off, 0u);
+    }
+  }
+}
+
+// shots per block (lanes per column): the largest power of two <= 16 with
+// n^2*ST <= 1024 threads and LDS = 3*n^3*ST*sizeof(T) + 10*n^3*sizeof(TG)
+// + kCoef*sizeof(TG) <= 64 KiB; fewer when the problem has fewer shots
+template <int N, typename T, typename TG>
+struct ShotTile {
+  static constexpr int lds(int st) {
+    return 3 * N * N * N * st * (int)sizeof(T) + 10 * N * N * N * (int)sizeof(TG) +
+           kCoef * (int)sizeof(TG);
+  }
+  static constexpr int pick() {
+    int st = 16;
+    while (st > 1 && (lds(st) > 65536 || N * N * st > 1024)) st /= 2;
+    return st;
+  }
+  static constexpr int value = pick();
+};
+
+template <int N, int ST, typename T, typename TG>
+static int run_shots_st(const StiffArgs &a, hipStream_t st) {
+  const int64_t ne = a.e1 - a.e0;
+  if (ne <= 0) return 0;
+  const dim3 grid((unsigned)ne, (unsigned)((a.nshots + ST - 1) / ST));
+  const dim3 block(N * N * ST);
+  const T *D = static_cast<const T *>(a.Dm);
+  const T *U = static_cast<const T *>(a.U);
+  T *Y = static_cast<T *>(a.Y);
+  const T *s = static_cast<const T *>(a.sdiag);
+  const TG *G = static_cast<const TG *>(a.G);
+  const TG *cf = static_cast<const TG *>(a.coef);
+  const TG *gw = static_cast<const TG *>(a.glxw);
+  const bool tri = a.geo == kGeoTrilinear;
+  if (tri && (!cf || !gw)) return -1002;
+#define MJW_SHOTS_LAUNCH(MODE, TRI)                                                              \
+  hipLaunchKernelGGL((k_shots3d<N, ST, T, TG, MODE, TRI>), grid, block, 0, st, a.map, G, D, a.E, \
+                     a.e0, a.e1, U, Y, s, (T)a.alpha, (T)a.beta, a.nshots, a.nxcd, cf, gw)
+  if (a.mode == kLeap) {
+    if (tri) MJW_SHOTS_LAUNCH(kLeap, true);
+    else MJW_SHOTS_LAUNCH(kLeap, false);
+  } else {
+    if (tri) MJW_SHOTS_LAUNCH(kApply, true);
+    else MJW_SHOTS_LAUNCH(kApply, false);
+  }
+#undef MJW_SHOTS_LAUNCH
+  return (int)hipGetLastError();
+}
+
+// a.epb > 0 requests that many shot lanes (a power of two up to the limit);
+// 0 picks the smallest power of two >= nshots within the limit
+template <int N, typename T, typename TG>
+static int run_shots_t(const StiffArgs &a, hipStream_t st) {
+  constexpr int SM = ShotTile<N, T, TG>::value;
+  static_assert(SM >= 1, "shot tile");
+  int want = a.epb;
+  if (want <= 0) {
+    want = 1;
+    while (want < a.nshots && want < SM) want *= 2;
+  }
+  if (want > SM) return -1005;
+  if constexpr (SM >= 16)
+    if (want == 16) return run_shots_st<N, 16, T, TG>(a, st);
+  if constexpr (SM >= 8)
+    if (want == 8) return run_shots_st<N, 8, T, TG>(a, st);
+  if constexpr (SM >= 4)
+    if (want == 4) return run_shots_st<N, 4, T, TG>(a, st);
+  if constexpr (SM >= 2)
+    if (want == 2) return run_shots_st<N, 2, T, TG>(a, st);
+  if (want == 1) return run_shots_st<N, 1, T, TG>(a, st);
+  return -1005;
+}
+
+template <int N>
+static void shot_tiles_n(int state, int geom, std::vector<int> &v) {
+  int sm;
+  if (state == kF64) sm = ShotTile<N, double, double>::value;
+  else if (geom == kG64) sm = ShotTile<N, float, double>::value;
+  else sm = ShotTile<N, float, float>::value;
+  for (int t = sm; t >= 1; t /= 2) v.push_back(t);
+}
+
+std::vector<int> tile_choices_shots(int order, int state, int geom) {
+  std::vector<int> v;
+  switch (order + 1) {
+    case 2: shot_tiles_n<2>(state, geom, v); break;
+    case 3: shot_tiles_n<3>(state, geom, v); break;
+    case 4: shot_tiles_n<4>(state, geom, v); break;
+    case 5: shot_tiles_n<5>(state, geom, v); break;
+    case 6: shot_tiles_n<6>(state, geom, v); break;
+    case 7: shot_tiles_n<7>(state, geom, v); break;
+    case 8: shot_tiles_n<8>(state, geom, v); break;
+    default: break;
+  }
+  return v;
+}
+
+template <int N>
+static int run_shots_n(const StiffArgs &a, hipStream_t st) {
+  if (a.state == kF64) return run_shots_t<N, double, double>(a, st);
+  if (a.geom == kG64) return run_shots_t<N, float, double>(a, st);
+  return run_shots_t<N, float, float>(a, st);
+}
+
+int launch_stiffness_shots(const StiffArgs &a, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (a.order + 1) {
+    case 2: return run_shots_n<2>(a, st);
+    case 3: return run_shots_n<3>(a, st);
+    case 4: return run_shots_n<4>(a, st);
+    case 5: return run_shots_n<5>(a, st);
+    case 6: return run_shots_n<6>(a, st);
+    case 7: return run_shots_n<7>(a, st);
+    case 8: return run_shots_n<8>(a, st);
+    default: return -1000;
+  }
+}
+
+}  // namespace mjw

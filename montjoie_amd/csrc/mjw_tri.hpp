@@ -88,20 +88,34 @@ __device__ __forceinline__ float tri_ratio(float f, float J) {
 }
 __device__ __forceinline__ double tri_ratio(double f, double J) { return f / J; }
 
-// acoustic flux at layer zeta: v = G b with G = mu w J DF^{-1} DF^{-T},
-// evaluated as (mu w / J) R (R^T b), R the matrix of rows r_a
+// adjugate rows r0, r1, r2 and f = mu w / J at layer zeta
 template <typename A>
-__device__ __forceinline__ void tri_apply(const TriCol<A> &t, A zeta, A wk, A b1, A b2, A b3,
-                                          A &v1, A &v2, A &v3) {
-  A r0[3], r1[3], r2[3], J;
+__device__ __forceinline__ void tri_rows_f(const TriCol<A> &t, A zeta, A wk, A r0[3], A r1[3],
+                                           A r2[3], A &f) {
+  A J;
   tri_rows(t, zeta, r0, r1, r2, J);
-  const A f = tri_ratio(t.f * wk, J);
+  f = tri_ratio(t.f * wk, J);
+}
+
+// acoustic flux v = G b with G = mu w J DF^{-1} DF^{-T}, evaluated as
+// f R (R^T b), R the matrix of rows r_a
+template <typename A>
+__device__ __forceinline__ void tri_flux(const A *r0, const A *r1, const A *r2, A f, A b1, A b2,
+                                         A b3, A &v1, A &v2, A &v3) {
   A w[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) w[d] = b1 * r0[d] + b2 * r1[d] + b3 * r2[d];
   v1 = f * dot3(r0, w);
   v2 = f * dot3(r1, w);
   v3 = f * dot3(r2, w);
+}
+
+template <typename A>
+__device__ __forceinline__ void tri_apply(const TriCol<A> &t, A zeta, A wk, A b1, A b2, A b3,
+                                          A &v1, A &v2, A &v3) {
+  A r0[3], r1[3], r2[3], f;
+  tri_rows_f(t, zeta, wk, r0, r1, r2, f);
+  tri_flux(r0, r1, r2, f, b1, b2, b3, v1, v2, v3);
 }
 
 // acoustic tensor at layer zeta (6 components 00 01 02 11 12 22)

@@ -168,8 +168,9 @@ def test_set_mu_equals_fresh_problem(mode):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_solver_node_order_and_device_vectors(prec):
-    """Device vectors live in the solver node order; host entry points and
-    mjw_apply_kh_dev take the caller's order."""
+    """Device vectors live in the solver node order (and, for several 3-D
+    acoustic shots, the interleaved layout); host entry points and
+    mjw_apply_kh_dev take the caller's order and layout."""
     import torch
     m = M.regular_mesh(3, 3, 4, 0.1, seed=7)
     P = M.WaveProblem(m, 1.0, 1.0 + (np.arange(m.E) % 2), precision=prec, nshots=2)
@@ -193,7 +194,8 @@ def test_solver_node_order_and_device_vectors(prec):
     P.advance(30, traces=False)
     un, un1 = P.state()
     a, _ = P.state_dev()
-    dev = device_array(a, 2 * m.N, dt).reshape(2, m.N)
+    # 3-D acoustic with several shots: interleaved, u[g*S + s]
+    dev = device_array(a, 2 * m.N, dt).reshape(m.N, 2).T
     np.testing.assert_array_equal(dev[:, idx].astype(np.float64), un)
 
 

@@ -127,6 +127,25 @@ def test_every_tile_gives_identical_results():
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_every_shot_tile_gives_identical_results(prec):
+    """Interleaved shots: every shot-lane count computes the same numbers."""
+    from montjoie_amd import tuner
+    m = M.regular_mesh(3, 3, 4, 0.1, seed=11)
+    P = M.WaveProblem(m, 1.0, 1.0 + (np.arange(m.E) % 3), precision=prec, nshots=11)
+    rng = np.random.default_rng(5)
+    P.set_sources(0.25 + 0.5 * rng.random((11, 3)), shot=np.arange(11))
+    P.set_receivers(0.2 + 0.6 * rng.random((5, 3)))
+    P.set_pulse(M.ricker_pulse_table(8.0, 0.0, 1e-3, 40))
+    outs = []
+    for t in tuner.tile_choices(P):
+        P.set_tile(t)
+        P.leapfrog_init(1e-3)
+        outs.append(P.run(30))
+    assert len(outs) >= 3
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
 def test_energy_conserved_fp64():
     m = M.regular_mesh(3, 3, 4, 0.1, seed=9)
     P = M.WaveProblem(m, 1.0, 1.0 + np.arange(m.E) % 2)
