@@ -96,8 +96,7 @@ int mjw_info(const mjw_problem *p, int64_t *info9);
 /* Tuning: launch shape of the time loop's stiffness kernel (0 = default):
  * for one shot the elements per workgroup (a column tile EPB, -EPB for one
  * thread per node, 1000 + EPB for the persistent column kernel); for 3-D
- * acoustic problems with several (interleaved) shots the shot lanes per
- * column.  mjw_tile_choices lists the valid values.  Used by the autotuner
+ * problems with several (interleaved) shots the shot lanes per column.  mjw_tile_choices lists the valid values.  Used by the autotuner
  * (montjoie_amd/tuner.py); every shape gives identical results. */
 int mjw_set_tile(mjw_problem *p, int elems_per_block);
 int mjw_get_tile(const mjw_problem *p, int *elems_per_block);
@@ -218,8 +217,8 @@ int mjw_record(mjw_problem *p, double *traces);
 int mjw_get_state(mjw_problem *p, double *Un, double *Un1);
 /* device pointers of U^n and U^{n+1} in the solver's layout (state
  * precision, solver node order: see mjw_node_order): shots consecutive, except
- * 3-D acoustic problems with nshots > 1, whose shots are interleaved
- * (u[g*nshots + s]: one gather serves every shot of a node) */
+ * 3-D problems with nshots > 1, whose shots are interleaved
+ * (u[(m*n_nodes + g)*nshots + s]: one gather serves every shot of a node) */
 int mjw_state_dev(mjw_problem *p, void **Un, void **Un1);
 /* current step index n */
 int mjw_get_step(const mjw_problem *p, int64_t *n);

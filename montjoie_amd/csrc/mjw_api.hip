@@ -35,7 +35,7 @@ struct mjw_problem {
   // device vectors use the solver node order: nperm[g] = solver index of
   // node g of the caller's numbering, ninv = its inverse
   std::vector<int32_t> nperm, ninv;
-  // several shots of a 3-D acoustic problem: interleaved state u[(m*N + g)*S + s]
+  // several shots of a 3-D problem: interleaved state u[(m*N + g)*S + s]
   int ilv = 0;
   int st = 0;                   // shot lanes of the interleaved kernel (0: automatic)
   int32_t *d_nperm = nullptr, *d_ninv = nullptr;
@@ -259,8 +259,8 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
     const int64_t sb = precision == MJW_FP64 ? 8 : 4, gb = precision == MJW_FP32 ? 4 : 8;
     const int64_t ngc = ncomp == 3 ? 9 : (dim == 2 ? 3 : 6);
     const int64_t lim = ((int64_t)1 << 31) - 4096;
-    // several 3-D acoustic shots are interleaved: one array of S*N values
-    const int64_t sv = (dim == 3 && ncomp == 1 && nshots > 1) ? nshots : 1;
+    // several 3-D shots are interleaved: one array of S*ncomp*N values
+    const int64_t sv = (dim == 3 && nshots > 1) ? nshots : 1;
     if (sv * n_nodes * ncomp * sb >= lim || n_elem * P * K * 4 >= lim ||
         n_elem * P * ngc * gb >= lim)
       FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED,
@@ -279,7 +279,7 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   p->n = order + 1;
   p->ncomp = ncomp;
   p->nshots = nshots;
-  p->ilv = (dim == 3 && ncomp == 1 && nshots > 1) ? 1 : 0;
+  p->ilv = (dim == 3 && nshots > 1) ? 1 : 0;
   p->prec = precision;
   p->state = precision == MJW_FP64 ? kF64 : kF32;
   p->geom = precision == MJW_FP32 ? kG32 : kG64;
@@ -536,7 +536,9 @@ extern "C" int mjw_info(const mjw_problem *p, int64_t *info) {
 // launch shapes of the time loop's stiffness kernel: elements per block
 // (single shot) or shot lanes per column (interleaved shots)
 static std::vector<int> problem_tiles(const mjw_problem *p) {
-  if (p->ilv) return tile_choices_shots(p->order, p->state, p->geom);
+  if (p->ilv)
+    return p->ncomp == 3 ? tile_choices_shots_elastic(p->order, p->state, p->geom)
+                         : tile_choices_shots(p->order, p->state, p->geom);
   return tile_choices(p->dim, p->order, p->ncomp);
 }
 

@@ -90,6 +90,9 @@ int launch_stiffness(const StiffArgs &a, void *stream);
 int launch_stiffness_shots(const StiffArgs &a, void *stream);
 // supported shot-lane counts, largest first
 std::vector<int> tile_choices_shots(int order, int state, int geom);
+// isotropic elastic, several shots interleaved
+int launch_stiffness_shots_elastic(const StiffArgs &a, void *stream);
+std::vector<int> tile_choices_shots_elastic(int order, int state, int geom);
 // list of supported elements-per-block for (dim, order, ncomp)
 std::vector<int> tile_choices(int dim, int order, int ncomp);
 int default_tile(int dim, int order, int ncomp, int state);

@@ -15,8 +15,10 @@ std::vector<int> tile_choices_scalar(int dim, int order);
 std::vector<int> tile_choices_elastic(int order);
 
 int launch_stiffness(const StiffArgs &a, void *stream) {
+  if (a.ilv && a.dim == 3 && !a.grad)
+    return a.ncomp == 3 ? launch_stiffness_shots_elastic(a, stream)
+                        : launch_stiffness_shots(a, stream);
   if (a.ncomp == 3) return launch_stiffness_elastic(a, stream);
-  if (a.ilv && a.dim == 3 && !a.grad) return launch_stiffness_shots(a, stream);
   return launch_stiffness_scalar(a, stream);
 }
 

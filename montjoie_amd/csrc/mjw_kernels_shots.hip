@@ -1,5 +1,6 @@
-// Stiffness pass of a 3-D acoustic problem with several shots (right-hand
-// sides) in the shot-interleaved layout u[g*S + s], gfx950.
+// Stiffness passes of 3-D problems with several shots (right-hand sides) in
+// the shot-interleaved layout u[(m*N + g)*S + s], gfx950: acoustic
+// (k_shots3d) and isotropic elastic (k_shots_el).
 //
 // Same arithmetic per node and shot as the column kernel
 // (mjw_kernels_scalar.hip, restating MltAdd_ElementH1_Scalar,
@@ -29,7 +30,9 @@ __global__ __launch_bounds__(N *N *ST) void k_shots3d(
     const TG *__restrict__ coef, const TG *__restrict__ glxw) {
   constexpr int P = N * N, NN = P * N;
   using A = TG;
-  __shared__ T su[NN * ST], s1[NN * ST], s2[NN * ST];
+  // su holds the staged U, then (after a barrier) the xi_1 fluxes
+  __shared__ T su[NN * ST], s2[NN * ST];
+  T *const s1 = su;
   constexpr int NG = TRI ? 10 : 6;  // per node: r0 r1 r2 f, or the 6 tensor entries
   __shared__ A lg[P * N * NG];      // [i + n j][k][NG]
   __shared__ TG lc[TRI ? kCoef : 1];
@@ -75,7 +78,7 @@ __global__ __launch_bounds__(N *N *ST) void k_shots3d(
     }
   }
   __syncthreads();
-  T v3[N];
+  T v1[N], v2[N], v3[N];
   if (sact) {
     T Di[N], Dj[N];
 #pragma unroll
@@ -97,14 +100,22 @@ __global__ __launch_bounds__(N *N *ST) void k_shots3d(
       if constexpr (TRI) {
         A w1, w2, w3;
         tri_flux<A>(g, g + 3, g + 6, g[9], b1, b2, b3, w1, w2, w3);
-        s1[(k * P + ij) * ST + lane] = (T)w1;
-        s2[(k * P + ij) * ST + lane] = (T)w2;
+        v1[k] = (T)w1;
+        v2[k] = (T)w2;
         v3[k] = (T)w3;
       } else {
-        s1[(k * P + ij) * ST + lane] = (T)(g[0] * b1 + g[1] * b2 + g[2] * b3);
-        s2[(k * P + ij) * ST + lane] = (T)(g[1] * b1 + g[3] * b2 + g[4] * b3);
+        v1[k] = (T)(g[0] * b1 + g[1] * b2 + g[2] * b3);
+        v2[k] = (T)(g[1] * b1 + g[3] * b2 + g[4] * b3);
         v3[k] = (T)(g[2] * b1 + g[4] * b2 + g[5] * b3);
       }
+    }
+  }
+  __syncthreads();  // every read of the staged U is done: su takes the xi_1 fluxes
+  if (sact) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      s1[(k * P + ij) * ST + lane] = v1[k];
+      s2[(k * P + ij) * ST + lane] = v2[k];
     }
   }
   // previous values and dt^2/M before the scatter's stores
@@ -150,12 +161,12 @@ __global__ __launch_bounds__(N *N *ST) void k_shots3d(
 }
 
 // shots per block (lanes per column): the largest power of two <= 16 with
-// n^2*ST <= 1024 threads and LDS = 3*n^3*ST*sizeof(T) + 10*n^3*sizeof(TG)
+// n^2*ST <= 1024 threads and LDS = 2*n^3*ST*sizeof(T) + 10*n^3*sizeof(TG)
 // + kCoef*sizeof(TG) <= 64 KiB; fewer when the problem has fewer shots
 template <int N, typename T, typename TG>
 struct ShotTile {
   static constexpr int lds(int st) {
-    return 3 * N * N * N * st * (int)sizeof(T) + 10 * N * N * N * (int)sizeof(TG) +
+    return 2 * N * N * N * st * (int)sizeof(T) + 10 * N * N * N * (int)sizeof(TG) +
            kCoef * (int)sizeof(TG);
   }
   static constexpr int pick() {
@@ -262,6 +273,277 @@ int launch_stiffness_shots(const StiffArgs &a, void *stream) {
     case 8: return run_shots_n<8>(a, st);
     default: return -1000;
   }
+}
+
+}  // namespace mjw
+
+namespace mjw {
+
+// ----------------------------------------------------------------- elastic
+// Isotropic elastodynamics, several shots interleaved (u[((m*N + g)*S + s]):
+// the arithmetic of k_elastic3d (mjw_kernels_elastic.hip, restating
+// MltAdd_ElementH1, Computation/ProdMatVectH1.cxx:674-944) with one element
+// and ST shot lanes per block.  The factor M = sqrt(J w) DF^{-1} of each node
+// is built once per block into LDS (trilinear map or stored planes).  LDS
+// planes [9][k][i + n j][lane]: u_m (0..2), xi_1 fluxes (3..5), xi_2 fluxes
+// (6..8).
+template <int N, int ST, typename T, typename TG, int MODE, bool TRI>
+__global__ __launch_bounds__(N *N *ST) void k_shots_el(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    const double *__restrict__ lam_e, const double *__restrict__ mu_e, int64_t E, int64_t e0,
+    int64_t e1, const T *__restrict__ U, T *__restrict__ Y, const T *__restrict__ sdiag,
+    T alpha, T beta, int64_t N_nodes, int nshots, int nxcd, const TG *__restrict__ coef,
+    const TG *__restrict__ glxw) {
+  constexpr int P = N * N, NN = P * N, PL = NN * ST;
+  using A = TG;
+  __shared__ T lds[9 * PL];
+  __shared__ A lm[P * N * 9];  // [i + n j][k][9]
+  __shared__ TG lc[TRI ? kCoef : 1];
+  const int t = threadIdx.x;
+  const int lane = t % ST, ij = t / ST;
+  const int i = ij % N, j = ij / N;
+  const int64_t e = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd);
+  const int s = blockIdx.y * ST + lane;
+  const bool act = e < e1, sact = act && s < nshots;
+  if constexpr (TRI) {
+    for (int q = t; q < kCoef; q += P * ST)
+      if (act) lc[q] = coef[e * kCoef + q];
+    __syncthreads();
+    if (act && lane < N) {
+      TriCol<A> tc;
+      tri_setup<A, TG>(lc, (A)glxw[i], (A)glxw[j], (A)glxw[N + i] * (A)glxw[N + j], false, tc);
+      for (int k = lane; k < N; k += ST) tri_m<A>(tc, (A)glxw[k], (A)glxw[N + k], lm + (ij * N + k) * 9);
+    }
+  } else {
+    if (act)
+      for (int k = lane; k < N; k += ST)
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+          lm[(ij * N + k) * 9 + c] = (A)G[((int64_t)(k * 9 + c) * E + e) * P + ij];
+  }
+  int32_t gid[N];
+  A lam = 0, mu = 0;
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) gid[k] = map[((int64_t)k * E + e) * P + ij];
+    lam = (A)lam_e[e];
+    mu = (A)mu_e[e];
+  }
+  const rsrc_t ru = mkres(U), ry = mkres(Y), rs = mkres(sdiag);
+  auto slot = [&](int m, int32_t g) {
+    return (((uint32_t)m * (uint32_t)N_nodes + (uint32_t)(g & 0x7fffffff)) * (uint32_t)nshots +
+            (uint32_t)s) * (uint32_t)sizeof(T);
+  };
+  T u[3][N];
+  if (sact) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        u[m][k] = bld<T>(ru, slot(m, gid[k]), 0u);
+        lds[m * PL + (k * P + ij) * ST + lane] = u[m][k];
+      }
+  }
+  __syncthreads();
+  T v2[3][N];
+  if (sact) {
+    T Di[N], Dj[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      Di[l] = Dm[i * N + l];
+      Dj[l] = Dm[j * N + l];
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      A g[3][3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          a0 += Di[l] * lds[m * PL + (k * P + j * N + l) * ST + lane];
+          a1 += Dj[l] * lds[m * PL + (k * P + l * N + i) * ST + lane];
+          a2 += Dm[k * N + l] * u[m][l];
+        }
+        g[m][0] = a0;
+        g[m][1] = a1;
+        g[m][2] = a2;
+      }
+      const A *M = lm + (ij * N + k) * 9;
+      A du[3][3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          du[m][a] = M[0 * 3 + a] * g[m][0] + M[1 * 3 + a] * g[m][1] + M[2 * 3 + a] * g[m][2];
+      const A tr = du[0][0] + du[1][1] + du[2][2];
+      A sg[3][3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          sg[m][a] = mu * (du[m][a] + du[a][m]) + (m == a ? lam * tr : A(0));
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        lds[(3 + m) * PL + (k * P + ij) * ST + lane] =
+            (T)(M[0] * sg[m][0] + M[1] * sg[m][1] + M[2] * sg[m][2]);
+        lds[(6 + m) * PL + (k * P + ij) * ST + lane] =
+            (T)(M[3] * sg[m][0] + M[4] * sg[m][1] + M[5] * sg[m][2]);
+        v2[m][k] = (T)(M[6] * sg[m][0] + M[7] * sg[m][1] + M[8] * sg[m][2]);
+      }
+    }
+  }
+  __syncthreads();
+  if (sact) {
+    T Dti[N], Dtj[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      Dti[l] = Dm[l * N + i];
+      Dtj[l] = Dm[l * N + j];
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const int32_t mm = gid[k];
+      const bool first = mm < 0;
+      T sgd = 0;
+      if constexpr (MODE == kLeap)
+        sgd = bld<T>(rs, (uint32_t)(mm & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        T o = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          o += Dti[l] * lds[(3 + m) * PL + (k * P + j * N + l) * ST + lane];
+          o += Dtj[l] * lds[(6 + m) * PL + (k * P + l * N + i) * ST + lane];
+          o += Dm[l * N + k] * v2[m][l];
+        }
+        const uint32_t off = slot(m, mm);
+        const T y = bld<T>(ry, off, 0u);
+        T r;
+        if constexpr (MODE == kLeap)
+          r = first ? (T(2) * u[m][k] - y - sgd * o) : (y - sgd * o);
+        else
+          r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
+        bst<T>(r, ry, off, 0u);
+      }
+    }
+  }
+}
+
+// lanes: largest power of two <= 16 with n^2*ST <= 1024 threads and
+// LDS = 9*n^3*ST*sizeof(T) + 9*n^3*sizeof(TG) + kCoef*sizeof(TG) <= 64 KiB
+template <int N, typename T, typename TG>
+struct ShotTileEl {
+  static constexpr int lds(int st) {
+    return 9 * N * N * N * st * (int)sizeof(T) + 9 * N * N * N * (int)sizeof(TG) +
+           kCoef * (int)sizeof(TG);
+  }
+  static constexpr int pick() {
+    int st = 16;
+    while (st > 1 && (lds(st) > 65536 || N * N * st > 1024)) st /= 2;
+    return st;
+  }
+  static constexpr int value = lds(1) <= 65536 ? pick() : 0;
+};
+
+template <int N, int ST, typename T, typename TG>
+static int run_shots_el_st(const StiffArgs &a, hipStream_t st) {
+  const int64_t ne = a.e1 - a.e0;
+  if (ne <= 0) return 0;
+  const dim3 grid((unsigned)ne, (unsigned)((a.nshots + ST - 1) / ST));
+  const dim3 block(N * N * ST);
+  const T *D = static_cast<const T *>(a.Dm);
+  const T *U = static_cast<const T *>(a.U);
+  T *Y = static_cast<T *>(a.Y);
+  const T *s = static_cast<const T *>(a.sdiag);
+  const TG *G = static_cast<const TG *>(a.G);
+  const TG *cf = static_cast<const TG *>(a.coef);
+  const TG *gw = static_cast<const TG *>(a.glxw);
+  const bool tri = a.geo == kGeoTrilinear;
+  if (tri && (!cf || !gw)) return -1002;
+#define MJW_SHOTS_EL(MODE, TRI)                                                                  \
+  hipLaunchKernelGGL((k_shots_el<N, ST, T, TG, MODE, TRI>), grid, block, 0, st, a.map, G, D,    \
+                     a.lam, a.mu, a.E, a.e0, a.e1, U, Y, s, (T)a.alpha, (T)a.beta, a.N,        \
+                     a.nshots, a.nxcd, cf, gw)
+  if (a.mode == kLeap) {
+    if (tri) MJW_SHOTS_EL(kLeap, true);
+    else MJW_SHOTS_EL(kLeap, false);
+  } else {
+    if (tri) MJW_SHOTS_EL(kApply, true);
+    else MJW_SHOTS_EL(kApply, false);
+  }
+#undef MJW_SHOTS_EL
+  return (int)hipGetLastError();
+}
+
+template <int N, typename T, typename TG>
+static int run_shots_el_t(const StiffArgs &a, hipStream_t st) {
+  constexpr int SM = ShotTileEl<N, T, TG>::value;
+  if constexpr (SM < 1) {
+    return -1005;
+  } else {
+    int want = a.epb;
+    if (want <= 0) {
+      want = 1;
+      while (want < a.nshots && want < SM) want *= 2;
+    }
+    if (want > SM) return -1005;
+    if constexpr (SM >= 16)
+      if (want == 16) return run_shots_el_st<N, 16, T, TG>(a, st);
+    if constexpr (SM >= 8)
+      if (want == 8) return run_shots_el_st<N, 8, T, TG>(a, st);
+    if constexpr (SM >= 4)
+      if (want == 4) return run_shots_el_st<N, 4, T, TG>(a, st);
+    if constexpr (SM >= 2)
+      if (want == 2) return run_shots_el_st<N, 2, T, TG>(a, st);
+    if (want == 1) return run_shots_el_st<N, 1, T, TG>(a, st);
+    return -1005;
+  }
+}
+
+template <int N>
+static int run_shots_el_n(const StiffArgs &a, hipStream_t st) {
+  if (a.state == kF64) return run_shots_el_t<N, double, double>(a, st);
+  if (a.geom == kG64) return run_shots_el_t<N, float, double>(a, st);
+  return run_shots_el_t<N, float, float>(a, st);
+}
+
+int launch_stiffness_shots_elastic(const StiffArgs &a, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (a.order + 1) {
+    case 2: return run_shots_el_n<2>(a, st);
+    case 3: return run_shots_el_n<3>(a, st);
+    case 4: return run_shots_el_n<4>(a, st);
+    case 5: return run_shots_el_n<5>(a, st);
+    case 6: return run_shots_el_n<6>(a, st);
+    case 7: return run_shots_el_n<7>(a, st);
+    case 8: return run_shots_el_n<8>(a, st);
+    default: return -1000;
+  }
+}
+
+template <int N>
+static void shot_tiles_el_n(int state, int geom, std::vector<int> &v) {
+  int sm;
+  if (state == kF64) sm = ShotTileEl<N, double, double>::value;
+  else if (geom == kG64) sm = ShotTileEl<N, float, double>::value;
+  else sm = ShotTileEl<N, float, float>::value;
+  for (int t = sm; t >= 1; t /= 2) v.push_back(t);
+}
+
+std::vector<int> tile_choices_shots_elastic(int order, int state, int geom) {
+  std::vector<int> v;
+  switch (order + 1) {
+    case 2: shot_tiles_el_n<2>(state, geom, v); break;
+    case 3: shot_tiles_el_n<3>(state, geom, v); break;
+    case 4: shot_tiles_el_n<4>(state, geom, v); break;
+    case 5: shot_tiles_el_n<5>(state, geom, v); break;
+    case 6: shot_tiles_el_n<6>(state, geom, v); break;
+    case 7: shot_tiles_el_n<7>(state, geom, v); break;
+    case 8: shot_tiles_el_n<8>(state, geom, v); break;
+    default: break;
+  }
+  return v;
 }
 
 }  // namespace mjw

@@ -194,7 +194,7 @@ def test_solver_node_order_and_device_vectors(prec):
     P.advance(30, traces=False)
     un, un1 = P.state()
     a, _ = P.state_dev()
-    # 3-D acoustic with several shots: interleaved, u[g*S + s]
+    # 3-D with several shots: interleaved, u[g*S + s]
     dev = device_array(a, 2 * m.N, dt).reshape(m.N, 2).T
     np.testing.assert_array_equal(dev[:, idx].astype(np.float64), un)
 

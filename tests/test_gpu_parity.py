@@ -116,6 +116,17 @@ def test_multishot_equals_single_shots():
         assert np.array_equal(multi[:, s, :], one[:, 0, :])
 
 
+def test_elastic_multishot_equals_single_shots():
+    """Interleaved elastic shots (k_shots_el) against one-shot runs
+    (k_elastic3d): bit for bit."""
+    c = CS.load("ref3d_el_q3")
+    srcs = [c.src, (0.3, 0.6, 0.5), (0.55, 0.35, 0.62)]
+    _, multi = CS.gpu_run(c, "fp64", nshots=3, srcs=srcs, nsteps=50)
+    for s, x in enumerate(srcs):
+        _, one = CS.gpu_run(c, "fp64", nshots=1, srcs=[x], nsteps=50)
+        assert np.array_equal(multi[:, s, :], one[:, 0, :])
+
+
 def test_every_tile_gives_identical_results():
     c = CS.load("ref3d_q5_pert")
     P = CS.gpu_problem(c, "fp64")
