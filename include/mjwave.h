@@ -263,6 +263,19 @@ int mjw_adjoint_timing(const mjw_problem *p, double *ms2);
  * rebuilds the problem from the data file). */
 int mjw_set_mu(mjw_problem *p, const double *mu);
 
+/* Media varying inside the elements (acoustic): rho_q and mu_q hold rho and mu
+ * at every quadrature point, [n_elem][(r+1)^dim] in the caller's element order
+ * and the local node order of mjw_create.  Replaces ScalarPhysicalIndice /
+ * TensorPhysicalIndice with a varying field (PhysicalMedia = rho|mu ref
+ * ISOTROPE SINUS|MESH|..., Elliptic/PhysicalProperty.cxx:3614-3710), whose
+ * values at the quadrature points VarGeometryProblem.cxx:1395-1560 stores
+ * and VarHelmholtz.cxx:73-117 reads for the mass rho J w and the stiffness
+ * tensor J w mu DF^{-1} DF^{-T}.  Rebuilds the geometric factors and the lumped
+ * mass; the problem then uses stored geometry (MJW_GEOM_TRILINEAR is refused),
+ * mjw_set_mu and the adjoint are refused, and mjw_leapfrog_init must be called
+ * again. */
+int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const double *mu_q);
+
 /* --------------------------------------------------------- synthetic meshes */
 /*
  * Regular Q_r mesh of [0,1]^dim with nx elements per direction (Montjoie's

@@ -71,6 +71,7 @@ SIGNATURES = {
     "mjw_profile": (C.c_int, [_vp, C.c_int, _dp]),
     "mjw_adjoint_timing": (C.c_int, [_vp, _dp]),
     "mjw_set_mu": (C.c_int, [_vp, _dp]),
+    "mjw_set_media_nodal": (C.c_int, [_vp, _dp, _dp]),
 }
 
 _lib = None

@@ -53,6 +53,7 @@ struct mjw_problem {
   // trilinear geometry (3-D scalar problems whose elements are straight-sided)
   bool tri_ok = false;
   int geo = kGeoStored;
+  bool nodal_media = false;           // rho, mu given per quadrature point
   std::vector<double> hcoef;          // E*kCoef, sorted order, FP64
   void *d_coef = nullptr, *d_glxw = nullptr;
   double *d_M = nullptr, *d_Minv = nullptr, *d_isqM = nullptr;
@@ -231,6 +232,33 @@ static hipError_t upload_trilinear(mjw_problem *p, const double *xg, const doubl
   float gf[2 * kMaxN];
   for (int i = 0; i < 2 * p->n; ++i) gf[i] = (float)gw[i];
   return hipMemcpy(p->d_glxw, gf, 2 * p->n * 4, hipMemcpyHostToDevice);
+}
+
+// p->mass holds the assembled mass in solver order (as downloaded): convert
+// it to the caller's order, set Dirichlet rows to 1 (TimeAcoustic.cxx:936-937)
+// and upload M, 1/M and 1/sqrt(M) (0 on Dirichlet rows) in solver order.
+static hipError_t finish_mass(mjw_problem *p) {
+  const int64_t n_nodes = p->N;
+  {
+    std::vector<double> ms_solver(p->mass);
+    for (int64_t g = 0; g < n_nodes; ++g) p->mass[g] = ms_solver[p->nperm[g]];
+  }
+  std::vector<double> msv(n_nodes), minv(n_nodes), isq(n_nodes);
+  for (int64_t g = 0; g < n_nodes; ++g) {
+    if (p->dir[g]) p->mass[g] = 1.0;
+    const int64_t i = p->nperm[g];
+    msv[i] = p->mass[g];
+    minv[i] = 1.0 / p->mass[g];
+    isq[i] = p->dir[g] ? 0.0 : 1.0 / std::sqrt(p->mass[g]);
+  }
+  hipError_t e = hipMemcpyAsync(p->d_M, msv.data(), n_nodes * 8, hipMemcpyHostToDevice, p->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(p->d_Minv, minv.data(), n_nodes * 8, hipMemcpyHostToDevice, p->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(p->d_isqM, isq.data(), n_nodes * 8, hipMemcpyHostToDevice, p->stream);
+  // the host vectors must outlive the copies
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  return e;
 }
 
 extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int ncomp,
@@ -446,7 +474,7 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   CK(hipMemcpyAsync(d_Dg, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
   CK(hipMemcpyAsync(d_w, wg, p->n * 8, hipMemcpyHostToDevice, st));
   if (launch_geometry_full(dim, order, ncomp, n_elem, p->d_map, d_xyz, d_Dg, d_w, p->d_rho,
-                           p->d_mu, p->geom == kG64 ? (double *)p->d_G : nullptr,
+                           p->d_mu, nullptr, nullptr, p->geom == kG64 ? (double *)p->d_G : nullptr,
                            p->geom == kG32 ? (float *)p->d_G : nullptr, d_Mloc, d_bad, st))
     CK(hipErrorLaunchFailure);
   CK(dalloc(p, &p->d_M, n_nodes * 8));
@@ -476,27 +504,11 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   } else {
     p->hcoef.clear();
   }
-  // the assembled mass came back in solver order; host copies are in the
-  // caller's order.  Dirichlet rows of the mass are 1 (TimeAcoustic.cxx:936-937)
-  {
-    std::vector<double> ms_solver(p->mass);
-    for (int64_t g = 0; g < n_nodes; ++g) p->mass[g] = ms_solver[p->nperm[g]];
-  }
-  std::vector<double> msv(n_nodes), minv(n_nodes), isq(n_nodes);
-  std::vector<uint8_t> dirv(n_nodes);
-  for (int64_t g = 0; g < n_nodes; ++g) {
-    if (p->dir[g]) p->mass[g] = 1.0;
-    const int64_t i = p->nperm[g];
-    msv[i] = p->mass[g];
-    minv[i] = 1.0 / p->mass[g];
-    isq[i] = p->dir[g] ? 0.0 : 1.0 / std::sqrt(p->mass[g]);
-    dirv[i] = p->dir[g];
-  }
-  CK(hipMemcpyAsync(p->d_M, msv.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
   CK(dalloc(p, &p->d_Minv, n_nodes * 8));
   CK(dalloc(p, &p->d_isqM, n_nodes * 8));
-  CK(hipMemcpyAsync(p->d_Minv, minv.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
-  CK(hipMemcpyAsync(p->d_isqM, isq.data(), n_nodes * 8, hipMemcpyHostToDevice, st));
+  CK(finish_mass(p));
+  std::vector<uint8_t> dirv(n_nodes);
+  for (int64_t g = 0; g < n_nodes; ++g) dirv[p->nperm[g]] = p->dir[g];
   CK(dalloc(p, &p->d_dir, n_nodes));
   CK(hipMemcpyAsync(p->d_dir, dirv.data(), n_nodes, hipMemcpyHostToDevice, st));
   CK(dalloc(p, &p->d_nperm, n_nodes * 4));
@@ -1291,6 +1303,8 @@ extern "C" int mjw_lambda_max(mjw_problem *p, int iters, uint64_t seed, double *
 extern "C" int mjw_set_mu(mjw_problem *p, const double *mu) {
   if (!p || !mu) FAIL(p, MJW_ERR_ARG, "mjw_set_mu: NULL");
   if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_mu: acoustic problems only");
+  if (p->nodal_media)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_mu: the problem has per-quadrature-point media");
   HIPCHK(p, hipSetDevice(p->device));
   std::vector<double> old(p->E), ms(p->E), sc(p->E);
   HIPCHK(p, hipMemcpy(old.data(), p->d_mu, p->E * 8, hipMemcpyDeviceToHost));
@@ -1318,10 +1332,87 @@ extern "C" int mjw_set_mu(mjw_problem *p, const double *mu) {
   return MJW_OK;
 }
 
+// Media varying inside the elements: rho and mu at every quadrature point
+// (PhysicalMedia ... SINUS / MESH / REGULAR_GRID evaluated by the caller,
+// as VarGeometryProblem.cxx:1395-1500 stores them for
+// VarHelmholtz.cxx:73-117).  Rebuilds the stored geometric factors and the
+// lumped mass from the vertices; the per-element trilinear form no longer
+// applies, so the problem switches to stored geometry for good.
+extern "C" int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const double *mu_q) {
+  if (!p || !rho_q || !mu_q) FAIL(p, MJW_ERR_ARG, "mjw_set_media_nodal: NULL");
+  if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_media_nodal: acoustic problems only");
+  HIPCHK(p, hipSetDevice(p->device));
+  const int64_t E = p->E, nd = p->nd, N = p->N;
+  std::vector<double> rq((size_t)E * nd), mq((size_t)E * nd);
+  for (int64_t es = 0; es < E; ++es) {
+    const int64_t e = p->perm[es];
+    for (int64_t q = 0; q < nd; ++q) {
+      const double r = rho_q[e * nd + q], m = mu_q[e * nd + q];
+      if (!(r > 0) || !(m > 0) || !std::isfinite(r) || !std::isfinite(m))
+        FAIL(p, MJW_ERR_ARG, "mjw_set_media_nodal: rho and mu must be finite, > 0");
+      rq[(size_t)es * nd + q] = r;
+      mq[(size_t)es * nd + q] = m;
+    }
+  }
+  double xg[kMaxN], wg[kMaxN], Dg[kMaxN * kMaxN];
+  gl_rule(p->order, xg, wg);
+  gl_deriv(p->order, xg, Dg);
+  std::vector<double> xyz_s((size_t)N * p->dim);
+  for (int64_t g = 0; g < N; ++g)
+    for (int a = 0; a < p->dim; ++a) xyz_s[(size_t)p->nperm[g] * p->dim + a] = p->xyz[g * p->dim + a];
+  double *d_xyz = nullptr, *d_Dg = nullptr, *d_w = nullptr, *d_Mloc = nullptr, *d_rq = nullptr,
+         *d_mq = nullptr;
+  int *d_bad = nullptr;
+  hipStream_t st = p->stream;
+  const size_t bq = (size_t)E * nd * 8;
+  hipError_t e = hipSuccess;
+  auto ok = [&](hipError_t x) { if (e == hipSuccess) e = x; return e == hipSuccess; };
+  ok(hipMalloc(&d_xyz, xyz_s.size() * 8));
+  ok(hipMalloc(&d_Dg, p->n * p->n * 8));
+  ok(hipMalloc(&d_w, p->n * 8));
+  ok(hipMalloc(&d_Mloc, bq));
+  ok(hipMalloc(&d_rq, bq));
+  ok(hipMalloc(&d_mq, bq));
+  ok(hipMalloc(&d_bad, 4));
+  ok(hipMemsetAsync(d_bad, 0, 4, st));
+  ok(hipMemcpyAsync(d_xyz, xyz_s.data(), xyz_s.size() * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_Dg, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_w, wg, p->n * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_rq, rq.data(), bq, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_mq, mq.data(), bq, hipMemcpyHostToDevice, st));
+  if (e == hipSuccess &&
+      launch_geometry_full(p->dim, p->order, 1, E, p->d_map, d_xyz, d_Dg, d_w, p->d_rho, p->d_mu,
+                           d_rq, d_mq, p->geom == kG64 ? (double *)p->d_G : nullptr,
+                           p->geom == kG32 ? (float *)p->d_G : nullptr, d_Mloc, d_bad, st))
+    e = hipErrorLaunchFailure;
+  for (int c = 0; c < p->ncolor && e == hipSuccess; ++c)
+    if (launch_mass_assemble(p->d_map, d_Mloc, E, p->dim, p->order, p->coff[c], p->coff[c + 1],
+                             p->d_M, st))
+      e = hipErrorLaunchFailure;
+  int bad = 0;
+  ok(hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, st));
+  ok(hipMemcpyAsync(p->mass.data(), p->d_M, N * 8, hipMemcpyDeviceToHost, st));
+  ok(hipStreamSynchronize(st));
+  dfree(d_xyz); dfree(d_Dg); dfree(d_w); dfree(d_Mloc); dfree(d_rq); dfree(d_mq); dfree(d_bad);
+  if (e != hipSuccess) FAIL(p, MJW_ERR_HIP, std::string("mjw_set_media_nodal: ") + hipGetErrorString(e));
+  if (bad) FAIL(p, MJW_ERR_GEOMETRY, "non-positive Jacobian: degenerate or inverted element");
+  HIPCHK(p, finish_mass(p));
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (p->bexec) { (void)hipGraphExecDestroy(p->bexec); p->bexec = nullptr; }
+  p->tri_ok = false;
+  p->geo = kGeoStored;
+  p->nodal_media = true;
+  p->inited = false;
+  p->adj_ready = false;
+  return MJW_OK;
+}
+
 extern "C" int mjw_adjoint_forward(mjw_problem *p, int nsteps, int nsnap, double *traces) {
   if (!p || nsteps < 2 || nsnap < 1 || !traces) FAIL(p, MJW_ERR_ARG, "mjw_adjoint_forward: bad args");
   if (p->ncomp != 1 || p->nshots != 1)
     FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: single-shot acoustic problems only");
+  if (p->nodal_media)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: per-element mu only (the problem has nodal media)");
   if (!p->inited || p->step != 0 || !p->zero_ic)
     FAIL(p, MJW_ERR_STATE, "adjoint: call mjw_leapfrog_init with zero initial state first");
   if (p->nrec == 0) FAIL(p, MJW_ERR_STATE, "adjoint: no receivers");

@@ -97,11 +97,13 @@ std::vector<int> tile_choices_shots_elastic(int order, int state, int geom);
 std::vector<int> tile_choices(int dim, int order, int ncomp);
 int default_tile(int dim, int order, int ncomp, int state);
 
-// FP64 geometry: acoustic tensor or elastic sqrt(Jw) DF^{-1} planes, Mloc[E*nd]
+// FP64 geometry: acoustic tensor or elastic sqrt(Jw) DF^{-1} planes, Mloc[E*nd].
+// rho_q / mu_q (E*nd, sorted element order; may be null) replace the
+// per-element rho / mu by values at every quadrature point
 int launch_geometry_full(int dim, int order, int ncomp, int64_t E, const int32_t *map,
                          const double *xyz, const double *D, const double *w, const double *rho,
-                         const double *mu, double *G64, float *G32, double *Mloc, int *bad,
-                         void *stream);
+                         const double *mu, const double *rho_q, const double *mu_q, double *G64,
+                         float *G32, double *Mloc, int *bad, void *stream);
 // mass assembly of one colour range: M[g] (=|+=) Mloc
 int launch_mass_assemble(const int32_t *map, const double *Mloc, int64_t E, int dim, int order,
                          int64_t e0, int64_t e1, double *M, void *stream);

@@ -51,7 +51,8 @@ int default_tile(int dim, int order, int ncomp, int state) {
 __global__ void k_geometry(int dim, int n, int ncomp, int64_t E, const int32_t *__restrict__ map,
                            const double *__restrict__ xyz, const double *__restrict__ D,
                            const double *__restrict__ wgl, const double *__restrict__ rho,
-                           const double *__restrict__ mu, double *__restrict__ G64,
+                           const double *__restrict__ mu, const double *__restrict__ rho_q,
+                           const double *__restrict__ mu_q, double *__restrict__ G64,
                            float *__restrict__ G32, double *__restrict__ Mloc,
                            int *__restrict__ bad) {
   const int P = n * n;
@@ -95,10 +96,12 @@ __global__ void k_geometry(int dim, int n, int ncomp, int64_t E, const int32_t *
   if (!(J > 0.0)) atomicOr(bad, 1);
   const double w = wgl[i] * wgl[j] * (dim == 3 ? wgl[k] : 1.0);
   const double Jw = J * w;
-  Mloc[id] = rho[e] * Jw;
+  // media at this quadrature point: per-node fields when given, else the
+  // element's constants
+  Mloc[id] = (rho_q ? rho_q[id] : rho[e]) * Jw;
   const int ij = i + n * j;
   if (ncomp == 1) {
-    const double c = Jw * mu[e];
+    const double c = Jw * (mu_q ? mu_q[id] : mu[e]);
     if (dim == 2) {
       double g[3];
       g[0] = c * (Di[0] * Di[0] + Di[1] * Di[1]);
@@ -135,15 +138,15 @@ __global__ void k_geometry(int dim, int n, int ncomp, int64_t E, const int32_t *
 
 int launch_geometry_full(int dim, int order, int ncomp, int64_t E, const int32_t *map,
                          const double *xyz, const double *D, const double *w, const double *rho,
-                         const double *mu, double *G64, float *G32, double *Mloc, int *bad,
-                         void *stream) {
+                         const double *mu, const double *rho_q, const double *mu_q, double *G64,
+                         float *G32, double *Mloc, int *bad, void *stream) {
   const int n = order + 1;
   const int nd = dim == 2 ? n * n : n * n * n;
   const int64_t tot = E * nd;
   const int bs = 256;
   hipLaunchKernelGGL(k_geometry, dim3((unsigned)((tot + bs - 1) / bs)), dim3(bs), 0,
                      static_cast<hipStream_t>(stream), dim, n, ncomp, E, map, xyz, D, w, rho, mu,
-                     G64, G32, Mloc, bad);
+                     rho_q, mu_q, G64, G32, Mloc, bad);
   return (int)hipGetLastError();
 }
 

@@ -19,7 +19,7 @@
 // Keywords read: TypeElement (QUADRANGLE_LOBATTO, HEXAHEDRON_LOBATTO),
 // TypeEquation (ACOUSTIC, TIME_ELASTIC), MeshPath, FileMesh, TypeMesh,
 // OrderDiscretization, ConditionReference (DIRICHLET), MateriauDielec
-// (ISOTROPE), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
+// (ISOTROPE), PhysicalMedia (rho / mu, SINUS, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
 // TemporalSource (RICKER), TimeStep, TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
 // ParametersOutputPoint.  Other keywords are reported and ignored; a value
@@ -463,6 +463,108 @@ Media media(const Deck &d, const Q1Mesh &m, bool elastic) {
   return md;
 }
 
+// PhysicalMedia = rho|mu ref ISOTROPE SINUS xmin xmax ymin ymax [zmin zmax]
+//                 c_x c_y [c_z] offset amplitude L_x L_y [L_z] [cte]
+// read as GenericPhysicalIndice::SetInputVaryingMedia does
+// (Elliptic/PhysicalProperty.cxx:3614-3672; the word after ref is skipped for
+// the scalar rho, ScalarPhysicalIndice::SetInputData, and must be ISOTROPE
+// for the tensor mu, PhysicalProperty.cxx:2421-2436).
+struct Sinus {
+  int dim = 3;
+  double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0}, coef[3] = {0, 0, 0}, k[3] = {0, 0, 0};
+  double offset = 0, amp = 0;
+};
+
+Sinus parse_sinus(const std::vector<std::string> &v, int dim) {
+  const std::vector<std::string> p(v.begin() + 2, v.end());
+  const int M = (int)p.size();
+  int nb = 1;
+  if (nb >= M || p[nb] != "SINUS")
+    die("PhysicalMedia " + v[0] + ": only the SINUS field is on this path");
+  if (v[0] == "mu" && p[0] != "ISOTROPE") die("PhysicalMedia mu: only ISOTROPE");
+  ++nb;
+  Sinus s;
+  s.dim = dim;
+  const double two_pi = 6.283185307179586476925286766559;
+  if (nb < M - 3) {
+    s.xmin[0] = num(p[nb++]); s.xmax[0] = num(p[nb++]);
+    s.xmin[1] = num(p[nb++]); s.xmax[1] = num(p[nb++]);
+  }
+  if (dim == 3 && nb < M - 1) { s.xmin[2] = num(p[nb++]); s.xmax[2] = num(p[nb++]); }
+  if (nb < M - 1) { s.coef[0] = num(p[nb++]); s.coef[1] = num(p[nb++]); }
+  if (dim == 3 && nb < M) s.coef[2] = num(p[nb++]);
+  if (nb < M - 3) {
+    s.offset = num(p[nb++]);
+    s.amp = num(p[nb++]);
+    s.k[0] = two_pi / num(p[nb++]);
+    s.k[1] = two_pi / num(p[nb++]);
+  }
+  if (dim == 3 && nb < M) s.k[2] = two_pi / num(p[nb++]);
+  return s;
+}
+
+// the field at a point (PhysicalSinusoidalMedia::ComputeIndexAtPoint,
+// PhysicalProperty.cxx:218-245; box test with the 1e-10 threshold
+// Mesh/NumberMesh.cxx:1621,2727 sets; Gaussian cut-off of RectangleCutOff,
+// PhysicalProperty.cxx:28-95)
+double sinus_at(const Sinus &s, const double *x) {
+  const double th = 1e-10;
+  for (int a = 0; a < s.dim; ++a)
+    if (x[a] < s.xmin[a] - th || x[a] > s.xmax[a] + th) return s.offset;
+  double c = 1.0;
+  for (int a = 0; a < s.dim; ++a) c *= std::sin(x[a] * s.k[a]);
+  bool cut = false;
+  for (int a = 0; a < s.dim; ++a) cut = cut || s.coef[a] != 0.0;
+  if (cut)
+    for (int a = 0; a < s.dim; ++a) {
+      double f[2];
+      const double b[2] = {s.xmin[a], s.xmax[a]};
+      for (int q = 0; q < 2; ++q) {
+        const double d = x[a] - b[q];
+        f[q] = std::fabs(d) > th ? std::exp(-s.coef[a] / (d * d)) : (s.coef[a] != 0.0 ? 0.0 : 1.0);
+      }
+      c *= f[0] * f[1];
+    }
+  return s.offset + s.amp * c;
+}
+
+// rho and mu at every quadrature point (the GL nodes of each element) when a
+// PhysicalMedia line makes them vary; empty otherwise
+struct NodalMedia {
+  std::vector<double> rho, mu;
+};
+
+NodalMedia nodal_media(const Deck &d, const Q1Mesh &m, const QrMesh &qm, const Media &md, int dim,
+                       bool elastic) {
+  NodalMedia nm;
+  const auto lines = d.all("PhysicalMedia");
+  if (lines.empty()) return nm;
+  if (elastic) die("PhysicalMedia: varying media are supported for ACOUSTIC decks only");
+  const int64_t nd = (int64_t)qm.en.size() / qm.E;
+  nm.rho.resize(qm.en.size());
+  nm.mu.resize(qm.en.size());
+  for (int64_t e = 0; e < qm.E; ++e)
+    for (int64_t q = 0; q < nd; ++q) {
+      nm.rho[e * nd + q] = md.rho[e];
+      nm.mu[e * nd + q] = md.mu[e];
+    }
+  for (const auto &v : lines) {
+    if (v.size() < 4) die("PhysicalMedia = name ref ISOTROPE SINUS ...");
+    if (v[0] != "rho" && v[0] != "mu") die("PhysicalMedia " + v[0] + ": only rho and mu");
+    const int ref = (int)num(v[1]);
+    const Sinus s = parse_sinus(v, dim);
+    std::vector<double> &dst = v[0] == "rho" ? nm.rho : nm.mu;
+    for (int64_t e = 0; e < qm.E; ++e) {
+      if (m.elem_ref[e] != ref) continue;
+      for (int64_t q = 0; q < nd; ++q) {
+        const int32_t g = qm.en[e * nd + q];
+        dst[e * nd + q] = sinus_at(s, &qm.xyz[(size_t)g * dim]);
+      }
+    }
+  }
+  return nm;
+}
+
 template <class T>
 void wr(std::ofstream &o, const std::vector<T> &v) {
   o.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
@@ -496,7 +598,7 @@ int main(int argc, char **argv) {
   const Deck d = read_deck(deck_path);
   static const std::set<std::string> known = {
       "TypeElement", "TypeEquation", "MeshPath", "FileMesh", "TypeMesh", "OrderDiscretization",
-      "ConditionReference", "MateriauDielec", "TypeSource", "OriginePhase", "Polarization",
+      "ConditionReference", "MateriauDielec", "PhysicalMedia", "TypeSource", "OriginePhase", "Polarization",
       "Frequency", "TemporalSource", "TimeStep", "TimeInterval", "OrderTimeScheme", "SismoPoint",
       "SismoCircle", "FileOutputPoint", "ParametersOutputPoint", "OutputFormat",
       "ElectricOrMagnetic", "PrintLevel"};
@@ -541,6 +643,7 @@ int main(int argc, char **argv) {
   }
   const QrMesh qm = number_nodes(q1, order, drefs);
   const Media md = media(d, q1, elastic);
+  const NodalMedia nm = nodal_media(d, q1, qm, md, dim, elastic);
   if (!dump.empty()) {
     std::ofstream o(dump, std::ios::binary);
     const std::vector<int64_t> head = {dim, order, ncomp, qm.N, qm.E};
@@ -551,6 +654,8 @@ int main(int argc, char **argv) {
     wr(o, md.rho);
     wr(o, md.mu);
     wr(o, md.lam);
+    wr(o, nm.rho);  // empty unless PhysicalMedia: E * nd each
+    wr(o, nm.mu);
     return o ? 0 : 1;
   }
 
@@ -568,6 +673,8 @@ int main(int argc, char **argv) {
                    md.rho.data(), md.mu.data(), elastic ? md.lam.data() : nullptr, qm.dir.data(),
                    1, precision),
         nullptr, "mjw_create");
+  if (!nm.rho.empty())
+    check(mjw_set_media_nodal(p, nm.rho.data(), nm.mu.data()), p, "PhysicalMedia");
   // source
   {
     const auto &op = d.need("OriginePhase");

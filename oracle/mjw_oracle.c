@@ -317,12 +317,16 @@ static double det_inv(int dim, const double *A, double *Ai) {
  * (Elliptic/Helmholtz/VarHelmholtz.cxx:88-117, second-order branch).
  * Also the elastic factors: Jw and DF^{-1} (row-major) for
  * Computation/ProdMatVectH1.cxx:760-820.
- * Inputs: structured mesh (dim, nx, r, vert), per-element mu, rho.
+ * Inputs: structured mesh (dim, nx, r, vert), mu and rho per element
+ * (nodal = 0) or per element-local node (nodal = 1, [E*nd]: media varying
+ * inside the element, evaluated at the quadrature points as
+ * VarGeometryProblem.cxx:1395-1560 stores them for VarHelmholtz.cxx:73-75).
  * Outputs (any may be NULL): G[E*nd*ncomp], Mloc[E*nd], jw[E*nd],
  * dfinv[E*nd*dim*dim].  Returns min J (negative = degenerate element).
  */
 double or_geometry(int dim, int nx, int r, const double *vert, const double *mu,
-                   const double *rho, double *G, double *Mloc, double *jw, double *dfinv) {
+                   const double *rho, int nodal, double *G, double *Mloc, double *jw,
+                   double *dfinv) {
   int n = r + 1;
   double x[OR_MAXN], w[OR_MAXN];
   if (or_gl(r, x, w)) return -1.0;
@@ -350,7 +354,8 @@ double or_geometry(int dim, int nx, int r, const double *vert, const double *mu,
       if (J < jmin) jmin = J;
       double Jw = J * wq;
       long o = e * nd + q;
-      if (Mloc) Mloc[o] = rho[e] * Jw;
+      const double rho_q = nodal ? rho[o] : rho[e], mu_q = nodal ? mu[o] : mu[e];
+      if (Mloc) Mloc[o] = rho_q * Jw;
       if (jw) jw[o] = Jw;
       if (dfinv)
         for (int a = 0; a < dim * dim; a++) dfinv[o * dim * dim + a] = Di[a];
@@ -361,7 +366,7 @@ double or_geometry(int dim, int nx, int r, const double *vert, const double *mu,
           for (int b = 0; b < dim; b++) {
             double s = 0.0;
             for (int c = 0; c < dim; c++) s += Di[a * dim + c] * Di[b * dim + c];
-            gm[a * dim + b] = Jw * mu[e] * s;
+            gm[a * dim + b] = Jw * mu_q * s;
           }
         if (dim == 2) {
           G[o * 3 + 0] = gm[0]; G[o * 3 + 1] = gm[1]; G[o * 3 + 2] = gm[3];

@@ -52,7 +52,7 @@ def lib():
     L.or_mesh_vertices.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint64, _dp]
     L.or_mesh_nodes.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _ip, _bp]
     L.or_mesh_nodes.restype = C.c_int
-    L.or_geometry.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
+    L.or_geometry.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int,
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.or_geometry.restype = C.c_double
     L.or_mass.argtypes = [C.c_long, C.c_int, C.c_long, _ip, _dp, _bp, _dp, _dp]
@@ -125,6 +125,42 @@ def pulse_table(f0: float, t_begin: float, dt: float, nsteps: int) -> np.ndarray
     return p
 
 
+THRESHOLD = 1e-10  # R2/R3::threshold as Mesh/NumberMesh.cxx:1621,2727 set it
+
+
+def sinus_media(pts, xmin, xmax, coef, offset, amplitude, period):
+    """PhysicalMedia ... SINUS: the sinusoidal field of
+    Elliptic/PhysicalProperty.cxx:218-245 at the points pts[..., dim].  Inside
+    the box [xmin, xmax] (PointInsideBoundingBox, SELDON
+    vector/TinyVectorInline.cxx:1517-1540, with THRESHOLD):
+    offset + amplitude * prod_i sin(2 pi x_i / period_i) * cutoff(x), the cutoff
+    prod_i exp(-c_i / (x_i - xmin_i)^2) exp(-c_i / (x_i - xmax_i)^2) when some
+    c_i != 0 (RectangleCutOff, PhysicalProperty.cxx:28-95; a factor is 0 within
+    THRESHOLD of a face with c_i != 0).  Outside the box: offset."""
+    pts = np.asarray(pts, float)
+    dim = pts.shape[-1]
+    xmin, xmax, coef = (np.asarray(v, float) for v in (xmin, xmax, coef))
+    k = 2.0 * np.pi / np.asarray(period, float)
+    inside = np.all((pts >= xmin - THRESHOLD) & (pts <= xmax + THRESHOLD), axis=-1)
+    val = np.ones(pts.shape[:-1])
+    for i in range(dim):
+        val = val * np.sin(pts[..., i] * k[i])
+    if np.any(coef != 0):
+        for i in range(dim):
+            for b in (xmin[i], xmax[i]):
+                d = pts[..., i] - b
+                near = np.abs(d) <= THRESHOLD
+                f = np.exp(-coef[i] / np.where(near, 1.0, d * d))
+                val = val * np.where(near, 0.0 if coef[i] != 0 else 1.0, f)
+    return np.where(inside, offset + amplitude * val, offset)
+
+
+def quadrature_points(m: "Mesh"):
+    """(E, nd, dim) positions of the element-local GL nodes (the quadrature
+    points of the Lobatto elements)."""
+    return m.xyz[m.elem_nodes]
+
+
 def circle_points(cx, cy, radius, npts):
     p = np.zeros(2 * npts)
     lib().or_circle_points(cx, cy, radius, npts, p)
@@ -194,22 +230,36 @@ def layered(m: Mesh, values) -> np.ndarray:
     return np.asarray(values, dtype=np.float64)[lay]
 
 
+def _media(m: Mesh, v):
+    """Per-element (scalar or (E,)) or per element-local node ((E, nd))."""
+    v = np.asarray(v, np.float64)
+    if v.ndim == 2:
+        return np.ascontiguousarray(v.reshape(m.E * m.nd)), 1
+    return np.ascontiguousarray(np.broadcast_to(v, (m.E,))), 0
+
+
 def geometry(m: Mesh, mu, rho, elastic: bool = False):
-    """Returns (G, Mloc) for the scalar case or (jw, dfinv) for elasticity."""
+    """Returns (G, Mloc) for the scalar case or (jw, dfinv) for elasticity.
+    mu, rho: per element, or (E, nd) arrays of values at the quadrature points
+    (both of the same kind)."""
     L = lib()
-    mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, np.float64), (m.E,)))
-    rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float64), (m.E,)))
+    mu, nod_m = _media(m, mu)
+    rho, nod_r = _media(m, rho)
+    if nod_m != nod_r:
+        raise ValueError("mu and rho must both be per element or both per node")
     nc = 3 if m.dim == 2 else 6
     Mloc = np.zeros(m.E * m.nd)
     if not elastic:
         G = np.zeros(m.E * m.nd * nc)
-        jmin = L.or_geometry(m.dim, m.nx, m.r, m.vert, mu, rho, _ptr(G), _ptr(Mloc), None, None)
+        jmin = L.or_geometry(m.dim, m.nx, m.r, m.vert, mu, rho, nod_m, _ptr(G), _ptr(Mloc), None,
+                             None)
         if jmin <= 0:
             raise ValueError("degenerate element")
         return G, Mloc
     jw = np.zeros(m.E * m.nd)
     dfi = np.zeros(m.E * m.nd * m.dim * m.dim)
-    jmin = L.or_geometry(m.dim, m.nx, m.r, m.vert, mu, rho, None, _ptr(Mloc), _ptr(jw), _ptr(dfi))
+    jmin = L.or_geometry(m.dim, m.nx, m.r, m.vert, mu, rho, nod_m, None, _ptr(Mloc), _ptr(jw),
+                         _ptr(dfi))
     if jmin <= 0:
         raise ValueError("degenerate element")
     return jw, dfi, Mloc
@@ -292,10 +342,12 @@ class Problem:
 
 
 def acoustic_problem(m: Mesh, mu_e, rho_e) -> Problem:
+    """mu_e, rho_e per element, or both (E, nd) at the quadrature points."""
     G, Mloc = geometry(m, mu_e, rho_e)
     M, Mi = mass(m, Mloc)
-    return Problem(m, 1, G=G, mu=np.broadcast_to(np.asarray(mu_e, float), (m.E,)).copy(),
-                   rho=np.broadcast_to(np.asarray(rho_e, float), (m.E,)).copy(), M=M, Minv=Mi,
+    shp = (m.E, m.nd) if np.ndim(mu_e) == 2 else (m.E,)
+    return Problem(m, 1, G=G, mu=np.broadcast_to(np.asarray(mu_e, float), shp).copy(),
+                   rho=np.broadcast_to(np.asarray(rho_e, float), shp).copy(), M=M, Minv=Mi,
                    colors=color(m))
 
 

@@ -50,9 +50,29 @@ def load(name: str) -> Case:
     mu_e = layers[lay]
     rho_e = z["rho_layers"][lay] if elastic else np.ones(E)
     lam_e = z["lam_layers"][lay] if elastic else None
+    srho = z["sinus_rho"] if "sinus_rho" in z.files else np.zeros(0)
+    smu = z["sinus_mu"] if "sinus_mu" in z.files else np.zeros(0)
+    if len(srho) or len(smu):
+        # PhysicalMedia SINUS of medium 1 (every element): rho and mu at the
+        # quadrature points, shape (E, nd)
+        m = O.mesh(dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]))
+        pts = O.quadrature_points(m)
+        rho_e = sinus_field(srho, pts) if len(srho) else np.broadcast_to(rho_e[:, None],
+                                                                         pts.shape[:2]).copy()
+        mu_e = sinus_field(smu, pts) if len(smu) else np.broadcast_to(mu_e[:, None],
+                                                                      pts.shape[:2]).copy()
     return Case(name, dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]), elastic, mu_e,
                 rho_e, lam_e, float(z["f0"]), float(z["dt"]), int(z["nsteps"]), z["src"],
                 z["pol"] if elastic else None, z["rec"], z["times"], z["traces"])
+
+
+def sinus_field(p, pts):
+    """Oracle evaluation of a stored SINUS field (layout: xmin(dim) xmax(dim)
+    cut-off(dim) offset amplitude period(dim)) at pts[..., dim]."""
+    dim = pts.shape[-1]
+    p = np.asarray(p, float)
+    return O.sinus_media(pts, p[:dim], p[dim:2 * dim], p[2 * dim:3 * dim], p[3 * dim],
+                         p[3 * dim + 1], p[3 * dim + 2:])
 
 
 def oracle_run(c: Case, nsteps: int | None = None):
@@ -102,6 +122,10 @@ def gpu_problem(c: Case, precision: str = "fp64", nshots: int = 1, srcs=None):
     if c.elastic:
         P = M.WaveProblem(m, c.rho_e, c.mu_e, lam=c.lam_e, ncomp=3, nshots=nshots,
                           precision=precision)
+    elif c.mu_e.ndim == 2:
+        # media varying inside the elements
+        P = M.WaveProblem(m, 1.0, 1.0, nshots=nshots, precision=precision)
+        P.set_media_nodal(c.rho_e, c.mu_e)
     else:
         P = M.WaveProblem(m, c.rho_e, c.mu_e, nshots=nshots, precision=precision)
     if srcs is None:

@@ -66,9 +66,25 @@ def write_medit(path, m: O.Mesh, elem_ref, bnd_ref=1):
         fh.write("\n".join(lines) + "\n")
 
 
+def sinus_line(name, ref, p):
+    """PhysicalMedia line of a SINUS field; p = [xmin(dim), xmax(dim), coef(dim),
+    offset, amplitude, period(dim)] (the layout stored in the fixtures)."""
+    p = [float(v) for v in p]
+    dim = (len(p) - 2) // 4
+    xmin, xmax, coef = p[:dim], p[dim:2 * dim], p[2 * dim:3 * dim]
+    off, amp, per = p[3 * dim], p[3 * dim + 1], p[3 * dim + 2:]
+    box = []
+    for a in range(dim):
+        box += [xmin[a], xmax[a]]
+    vals = box + coef + [off, amp] + per
+    return f"PhysicalMedia = {name} {ref} ISOTROPE SINUS " + " ".join(_f(v) for v in vals)
+
+
 def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
-               lam_layers=None, rho_layers=None, pol=None, regular=False):
-    """case.ini (+ mesh.mesh unless REGULAR) in directory d."""
+               lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
+               sinus_mu=None):
+    """case.ini (+ mesh.mesh unless REGULAR) in directory d.  sinus_rho /
+    sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1."""
     os.makedirs(d, exist_ok=True)
     m = O.mesh(dim, nx, r, pert, seed)
     nl = len(layers)
@@ -100,6 +116,10 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
                        f"{_f(lam_layers[k])} {_f(layers[k])}")
         else:
             ini.append(f"MateriauDielec = {k + 1} ISOTROPE 1.0 {_f(layers[k])} 0.0")
+    if sinus_rho is not None and len(sinus_rho):
+        ini.append(sinus_line("rho", 1, sinus_rho))
+    if sinus_mu is not None and len(sinus_mu):
+        ini.append(sinus_line("mu", 1, sinus_mu))
     ini.append("TypeSource = SRC_DIRAC")
     ini.append("OriginePhase = " + " ".join(_f(c) for c in src))
     if elastic:

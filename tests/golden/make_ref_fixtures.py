@@ -33,10 +33,12 @@ GOLD = os.path.dirname(os.path.abspath(__file__))
 
 
 def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
-             lam_layers=None, rho_layers=None, pol=None, regular=False):
+             lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
+             sinus_mu=None):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
-               lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular)
+               lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
+               sinus_rho=sinus_rho, sinus_mu=sinus_mu)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):
@@ -53,7 +55,9 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         rec=np.asarray(rec, float), times=times, traces=traces, elastic=elastic,
         lam_layers=np.asarray(lam_layers if lam_layers is not None else [], float),
         rho_layers=np.asarray(rho_layers if rho_layers is not None else [], float),
-        pol=np.asarray(pol if pol is not None else [], float), regular=regular)
+        pol=np.asarray(pol if pol is not None else [], float), regular=regular,
+        sinus_rho=np.asarray(sinus_rho if sinus_rho is not None else [], float),
+        sinus_mu=np.asarray(sinus_mu if sinus_mu is not None else [], float))
     print(f"{name}: {traces.shape} rows, times {times[0]}..{times[-1]}")
 
 
@@ -115,6 +119,30 @@ def ref3d_el_q3():
     rec = seeded_points(3, 8, 79)
     run_case("ref3d_el_q3", 3, 4, 3, 0.1, 7, mu, 8.0, 3e-3, 150, (0.46, 0.55, 0.51), rec,
              elastic=True, lam_layers=lam, rho_layers=rho, pol=(0.3, -0.5, 0.8))
+
+
+@case
+def ref2d_q4_sinus():
+    """8x8 Q4 unit square, rho and mu varying inside the elements
+    (PhysicalMedia SINUS): mu with a Gaussian cut-off on a box inside the
+    domain, rho over the whole square."""
+    circ = O.circle_points(0.5, 0.5, 0.3, 8)
+    # xmin(2) xmax(2) cut-off(2) offset amplitude period(2)
+    mu = [0.15, 0.2, 0.85, 0.9, 0.02, 0.03, 2.0, 1.2, 0.7, 0.9]
+    rho = [0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 1.5, 0.4, 1.3, 2.1]
+    run_case("ref2d_q4_sinus", 2, 8, 4, 0.0, 0, [1.0], 10.0, 1e-3, 200, (0.5, 0.5), circ,
+             regular=True, sinus_rho=rho, sinus_mu=mu)
+
+
+@case
+def ref3d_q3_sinus():
+    """4^3 Q3 perturbed hexahedra, rho and mu varying inside the elements."""
+    rec = seeded_points(3, 10, 80)
+    # xmin(3) xmax(3) cut-off(3) offset amplitude period(3)
+    mu = [0.1, 0.2, 0.15, 0.9, 0.85, 0.8, 0.01, 0.02, 0.015, 2.5, 1.5, 0.8, 1.1, 0.9]
+    rho = [0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 1.2, 0.3, 1.7, 1.3, 2.2]
+    run_case("ref3d_q3_sinus", 3, 4, 3, 0.1, 31, [1.0], 8.0, 4e-3, 150, (0.46, 0.52, 0.49), rec,
+             sinus_rho=rho, sinus_mu=mu)
 
 
 if __name__ == "__main__":

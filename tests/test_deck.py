@@ -25,7 +25,9 @@ def _write(c: CS.Case, d):
                      c.nsteps, c.src, c.rec, elastic=c.elastic,
                      lam_layers=z["lam_layers"] if c.elastic else None,
                      rho_layers=z["rho_layers"] if c.elastic else None,
-                     pol=c.pol, regular=bool(z["regular"]))
+                     pol=c.pol, regular=bool(z["regular"]),
+                     sinus_rho=z["sinus_rho"] if "sinus_rho" in z.files else None,
+                     sinus_mu=z["sinus_mu"] if "sinus_mu" in z.files else None)
 
 
 def _dump(path):
@@ -41,7 +43,11 @@ def _dump(path):
         size = np.dtype(dt).itemsize * n
         out[name] = np.frombuffer(raw[off:off + size], dt)
         off += size
-    assert off == len(raw)
+    # media varying inside the elements (PhysicalMedia): rho, mu per quadrature point
+    nq = (len(raw) - off) // 16
+    assert nq in (0, E * nd) and off + 16 * nq == len(raw)
+    out["rho_q"] = np.frombuffer(raw[off:off + 8 * nq], np.float64).reshape(-1, nd)
+    out["mu_q"] = np.frombuffer(raw[off + 8 * nq:], np.float64).reshape(-1, nd)
     out["xyz"] = out["xyz"].reshape(N, dim)
     out["en"] = out["en"].reshape(E, nd)
     return out
@@ -66,8 +72,17 @@ def test_driver_mesh_is_the_oracle_mesh(name, tmp_path):
     assert np.array_equal(perm[oen], g["en"]) and np.unique(perm).size == o.N
     np.testing.assert_allclose(g["xyz"][perm], o.xyz.reshape(o.N, -1), rtol=0, atol=1e-14)
     assert np.array_equal(g["dir"][perm], o.dirichlet.astype(np.uint8))
-    np.testing.assert_array_equal(g["mu"], c.mu_e)
-    np.testing.assert_array_equal(g["rho"], c.rho_e)
+    if c.mu_e.ndim == 2:
+        # PhysicalMedia SINUS evaluated by the driver at its quadrature points
+        # against the oracle's field at the oracle's (same elements and
+        # local order; positions equal to 1e-14)
+        assert g["mu_q"].shape == c.mu_e.shape
+        np.testing.assert_allclose(g["mu_q"], c.mu_e, rtol=1e-13, atol=0)
+        np.testing.assert_allclose(g["rho_q"], c.rho_e, rtol=1e-13, atol=0)
+    else:
+        assert g["mu_q"].size == 0
+        np.testing.assert_array_equal(g["mu"], c.mu_e)
+        np.testing.assert_array_equal(g["rho"], c.rho_e)
     if c.elastic:
         np.testing.assert_array_equal(g["lam"], c.lam_e)
 

@@ -1,0 +1,93 @@
+"""Media varying inside the elements (mjw_set_media_nodal): rho and mu given
+at every quadrature point, as Montjoie stores a PhysicalMedia field for the
+mass and stiffness integrals.  The operator and the mass are checked against
+the oracle fed the same per-node fields (FP64 1e-12, FP32 3e-6); a field that
+is constant on each element reproduces the per-element problem; the calls
+that need per-element media refuse a nodal problem.  Whole runs of decks
+with SINUS media against Montjoie's own seismograms are in
+test_gpu_parity.py and test_deck.py (golden cases ref*_sinus)."""
+import numpy as np
+import pytest
+
+import cases as CS
+import montjoie_amd as M
+from montjoie_amd._lib import MjwError
+from montjoie_amd.adjoint import AdjointSolver
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def fields(o, seed):
+    """Smooth positive rho, mu at the oracle's quadrature points."""
+    x = O.quadrature_points(o)
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 1.5, size=(2, x.shape[-1]))
+    mu = 2.0 + np.prod(np.sin(3.0 * a[0] * x), axis=-1)
+    rho = 1.5 + 0.5 * np.prod(np.cos(2.0 * a[1] * x), axis=-1)
+    return rho, mu
+
+
+@pytest.mark.parametrize("dim,r,nx,pert", [(2, 1, 5, 0.0), (2, 4, 3, 0.15), (3, 2, 3, 0.1),
+                                           (3, 5, 2, 0.1), (3, 7, 1, 0.05)])
+def test_nodal_media_apply_and_mass_match_oracle(dim, r, nx, pert):
+    m = M.regular_mesh(dim, nx, r, pert, seed=8)
+    o = O.mesh(dim, nx, r, pert, 8)
+    rho, mu = fields(o, r)
+    print(f"nodal media dim={dim} r={r} nx={nx}", flush=True)
+    P = M.WaveProblem(m, 1.0, 1.0)
+    P.set_media_nodal(rho, mu)
+    assert P.geometry == "stored"
+    G, Mloc = O.geometry(o, mu, rho)
+    u = np.random.default_rng(r).standard_normal(m.N)
+    ref = -O.stiffness(o, G, u, O.color(o))
+    assert CS.rel_l2(P.apply_kh(u), ref) < 1e-12
+    Mo, _ = O.mass(o, Mloc)
+    np.testing.assert_allclose(P.mass(), Mo, rtol=1e-13)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp32_g64"])
+def test_nodal_media_fp32(prec):
+    m = M.regular_mesh(3, 3, 5, 0.1, seed=5)
+    o = O.mesh(3, 3, 5, 0.1, 5)
+    rho, mu = fields(o, 3)
+    P = M.WaveProblem(m, 1.0, 1.0, precision=prec)
+    P.set_media_nodal(rho, mu)
+    G, _ = O.geometry(o, mu, rho)
+    u = np.random.default_rng(0).standard_normal(m.N)
+    assert CS.rel_l2(P.apply_kh(u), -O.stiffness(o, G, u, O.color(o))) < 3e-6
+
+
+def test_elementwise_constant_field_is_the_per_element_problem():
+    m = M.regular_mesh(3, 3, 4, 0.1, seed=2)
+    rng = np.random.default_rng(4)
+    mu_e = rng.uniform(1, 3, m.E)
+    rho_e = rng.uniform(1, 2, m.E)
+    nd = m.elem_nodes.shape[1]
+    Pe = M.WaveProblem(m, rho_e, mu_e)
+    Pe.set_geometry("stored")
+    Pn = M.WaveProblem(m, 1.0, 1.0)
+    Pn.set_media_nodal(np.repeat(rho_e[:, None], nd, 1), np.repeat(mu_e[:, None], nd, 1))
+    u = rng.standard_normal(m.N)
+    np.testing.assert_array_equal(Pn.apply_kh(u), Pe.apply_kh(u))
+    np.testing.assert_array_equal(Pn.mass(), Pe.mass())
+
+
+def test_nodal_media_refusals():
+    m = M.regular_mesh(3, 2, 3, 0.1, seed=1)
+    o = O.mesh(3, 2, 3, 0.1, 1)
+    rho, mu = fields(o, 1)
+    P = M.WaveProblem(m, 1.0, 1.0)
+    with pytest.raises(MjwError):
+        P.set_media_nodal(rho, -mu)
+    P.set_media_nodal(rho, mu)
+    with pytest.raises(MjwError):
+        P.set_geometry("trilinear")
+    with pytest.raises(MjwError):
+        P.set_mu(np.ones(m.E))
+    P.set_receivers(np.array([[0.5, 0.5, 0.5]]))
+    with pytest.raises(MjwError):
+        AdjointSolver(P, 1e-3, 10, nsnap=2).forward()
+    Pel = M.WaveProblem(m, 1.0, 1.0, lam=1.0, ncomp=3)
+    with pytest.raises(MjwError):
+        Pel.set_media_nodal(rho, mu)

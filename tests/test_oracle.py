@@ -266,3 +266,24 @@ def test_oracle_adjoint_gradient_matches_finite_differences():
         assert abs(fd - g @ dm) < 1e-6 * abs(fd)
     # a planted sign error in the recursion would flip or scale the result
     assert abs(g @ g) > 0
+
+
+def test_sinus_media_closed_forms():
+    """PhysicalMedia SINUS: no cut-off gives offset + amplitude prod sin(2 pi
+    x / L); outside the box the offset; on a face with a cut-off the offset;
+    the cut-off factor is prod exp(-c / d^2) over both faces (the whole field
+    is pinned against Montjoie by the ref*_sinus fixtures)."""
+    x = np.array([[0.3, 0.7], [0.1, 0.45], [1.2, 0.5], [0.25, 0.6]])
+    f = O.sinus_media(x, [0, 0], [1, 1], [0, 0], 2.0, 0.5, [0.8, 1.3])
+    want = 2.0 + 0.5 * np.sin(2 * np.pi * x[:, 0] / 0.8) * np.sin(2 * np.pi * x[:, 1] / 1.3)
+    want[2] = 2.0
+    np.testing.assert_allclose(f, want, rtol=1e-15)
+    c = [0.01, 0.02]
+    g = O.sinus_media(x, [0.25, 0.2], [0.9, 0.8], c, 2.0, 0.5, [0.8, 1.3])
+    p = x[0]
+    cut = np.prod([np.exp(-c[a] / (p[a] - lo) ** 2) * np.exp(-c[a] / (p[a] - hi) ** 2)
+                   for a, (lo, hi) in enumerate([(0.25, 0.9), (0.2, 0.8)])])
+    np.testing.assert_allclose(g[0], 2.0 + 0.5 * np.sin(2 * np.pi * 0.3 / 0.8)
+                               * np.sin(2 * np.pi * 0.7 / 1.3) * cut, rtol=1e-14)
+    assert g[1] == 2.0 and g[2] == 2.0  # outside the box
+    assert g[3] == 2.0  # on the face x = 0.25, where the cut-off vanishes
