@@ -5,14 +5,16 @@
 // Same arithmetic per node and shot as the column kernel
 // (mjw_kernels_scalar.hip, restating MltAdd_ElementH1_Scalar,
 // Computation/ProdMatVectScalarH1.cxx:256-668), mapped for many shots: a
-// block takes one element and ST consecutive shots; thread (i + n j, lane)
-// owns column (i, j) of shot s0 + lane and loops k.  The ST lanes of a
-// column gather one contiguous run of ST values per node, so a wavefront's
-// loads of U, Y and dt^2/M are ST times fewer cache lines than ST separate
-// single-shot passes.  The element's geometry is built once per block into
-// LDS and read by every shot: the adjugate rows of DF and mu w / J per node
-// (trilinear map; the flux is the single-shot kernels' f R (R^T b), so
-// results equal single-shot runs bit for bit) or the stored tensor.  LDS
+// block takes a run of R consecutive elements of a colour (R = 1, 2, 4) and
+// ST consecutive shots; thread (i + n j, lane) owns column (i, j) of shot
+// s0 + lane and loops k.  The ST lanes of a column gather one contiguous run
+// of ST values per node, so a wavefront's loads of U, Y and dt^2/M are ST
+// times fewer cache lines than ST separate single-shot passes.  Each
+// element's geometry is built once into LDS, one thread per node, and read
+// by every shot: the adjugate rows of DF and mu w / J per node (trilinear
+// map; the flux is the single-shot kernels' f R (R^T b), so results equal
+// single-shot runs bit for bit) or the stored tensor.  With R > 1 the next
+// element's map and U are loaded while the current one is computed.  LDS
 // state is [k][i + n j][lane] so lanes hit consecutive banks.
 #include <hip/hip_runtime.h>
 
@@ -22,7 +24,7 @@
 
 namespace mjw {
 
-template <int N, int ST, typename T, typename TG, int MODE, bool TRI>
+template <int N, int ST, int R, typename T, typename TG, int MODE, bool TRI>
 __global__ __launch_bounds__(N *N *ST) void k_shots3d(
     const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
     int64_t E, int64_t e0, int64_t e1, const T *__restrict__ U, T *__restrict__ Y,
@@ -35,127 +37,151 @@ __global__ __launch_bounds__(N *N *ST) void k_shots3d(
   T *const s1 = su;
   constexpr int NG = TRI ? 10 : 6;  // per node: r0 r1 r2 f, or the 6 tensor entries
   __shared__ A lg[P * N * NG];      // [i + n j][k][NG]
-  __shared__ TG lc[TRI ? kCoef : 1];
+  // D and D^T: read from LDS inside the element loop (not held in registers)
+  __shared__ T sD[P], sDt[P];
   const int t = threadIdx.x;
   const int lane = t % ST, ij = t / ST;
   const int i = ij % N, j = ij / N;
-  const int64_t e = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd);
+  for (int q = t; q < P; q += P * ST) {
+    sD[q] = Dm[q];
+    sDt[q] = Dm[(q % N) * N + q / N];
+  }
+  // this block's run of R consecutive elements of the colour
+  const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * R;
+  const int64_t ee = eb + R < e1 ? eb + R : e1;
   const int s = blockIdx.y * ST + lane;
-  const bool act = e < e1, sact = act && s < nshots;
-  // element tensor into LDS: lanes 0..n-1 of each column build layer k = lane
-  if constexpr (TRI) {
-    for (int q = t; q < kCoef; q += P * ST)
-      if (act) lc[q] = coef[e * kCoef + q];
-    __syncthreads();
-    if (act && lane < N) {
-      TriCol<A> tc;
-      tri_setup<A, TG>(lc, (A)glxw[i], (A)glxw[j], (A)glxw[N + i] * (A)glxw[N + j], true, tc);
-      for (int k = lane; k < N; k += ST) {
-        A *q = lg + (ij * N + k) * NG;
-        tri_rows_f<A>(tc, (A)glxw[k], (A)glxw[N + k], q, q + 3, q + 6, q[9]);
-      }
-    }
-  } else {
-    if (act)
-      for (int k = lane; k < N; k += ST)
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-          lg[(ij * N + k) * NG + c] = (A)G[((int64_t)(k * 6 + c) * E + e) * P + ij];
-  }
-  int32_t gid[N];
-  if (act) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) gid[k] = map[((int64_t)k * E + e) * P + ij];
-  }
+  const bool sok = s < nshots;
   const rsrc_t ru = mkres(U), ry = mkres(Y), rs = mkres(sdiag);
+  int32_t gid[N];
   T u[N];
-  if (sact) {
+  if (eb < ee) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const uint32_t g = (uint32_t)(gid[k] & 0x7fffffff);
-      u[k] = bld<T>(ru, (g * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T), 0u);
-      su[(k * P + ij) * ST + lane] = u[k];
-    }
+    for (int k = 0; k < N; ++k) gid[k] = map[((int64_t)k * E + eb) * P + ij];
+    if (sok)
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        u[k] = bld<T>(ru, ((uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)nshots + (uint32_t)s) *
+                              (uint32_t)sizeof(T), 0u);
   }
-  __syncthreads();
-  T v1[N], v2[N], v3[N];
-  if (sact) {
-    T Di[N], Dj[N];
+  for (int64_t e = eb; e < ee; ++e) {
+    const bool nxt = R > 1 && e + 1 < ee;
+    // stage U; one thread per node builds the element's geometry
+    if (sok)
 #pragma unroll
-    for (int l = 0; l < N; ++l) {
-      Di[l] = Dm[i * N + l];
-      Dj[l] = Dm[j * N + l];
-    }
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      T a1 = 0, a2 = 0, a3 = 0;
-#pragma unroll
-      for (int l = 0; l < N; ++l) {
-        a1 += Di[l] * su[(k * P + j * N + l) * ST + lane];
-        a2 += Dj[l] * su[(k * P + l * N + i) * ST + lane];
-        a3 += Dm[k * N + l] * u[l];
-      }
-      const A b1 = a1, b2 = a2, b3 = a3;
-      const A *g = lg + (ij * N + k) * NG;
+      for (int k = 0; k < N; ++k) su[(k * P + ij) * ST + lane] = u[k];
+    for (int gq = t; gq < NN; gq += P * ST) {  // node gq = i + n j + n^2 k
+      const int gij = gq % P, gk = gq / P;
+      A *q = lg + (gij * N + gk) * NG;
       if constexpr (TRI) {
-        A w1, w2, w3;
-        tri_flux<A>(g, g + 3, g + 6, g[9], b1, b2, b3, w1, w2, w3);
-        v1[k] = (T)w1;
-        v2[k] = (T)w2;
-        v3[k] = (T)w3;
+        const TG *c = coef + e * kCoef;
+        const int gi = gij % N, gj = gij / N;
+        TriCol<A> tc;
+        tri_setup<A, TG>(c, (A)glxw[gi], (A)glxw[gj], (A)glxw[N + gi] * (A)glxw[N + gj], true, tc);
+        tri_rows_f<A>(tc, (A)glxw[gk], (A)glxw[N + gk], q, q + 3, q + 6, q[9]);
       } else {
-        v1[k] = (T)(g[0] * b1 + g[1] * b2 + g[2] * b3);
-        v2[k] = (T)(g[1] * b1 + g[3] * b2 + g[4] * b3);
-        v3[k] = (T)(g[2] * b1 + g[4] * b2 + g[5] * b3);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) q[c] = (A)G[((int64_t)(gk * 6 + c) * E + e) * P + gij];
       }
     }
-  }
-  __syncthreads();  // every read of the staged U is done: su takes the xi_1 fluxes
-  if (sact) {
+    // next element's map; this element's previous values and dt^2/M
+    int32_t gn[N];
+    if (nxt)
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      s1[(k * P + ij) * ST + lane] = v1[k];
-      s2[(k * P + ij) * ST + lane] = v2[k];
+      for (int k = 0; k < N; ++k) gn[k] = map[((int64_t)k * E + e + 1) * P + ij];
+    T yk[N], sk[MODE == kLeap ? N : 1];
+    if (sok) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const uint32_t g = (uint32_t)(gid[k] & 0x7fffffff);
+        yk[k] = bld<T>(ry, (g * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T), 0u);
+        if constexpr (MODE == kLeap) sk[k] = bld<T>(rs, g * (uint32_t)sizeof(T), 0u);
+      }
     }
-  }
-  // previous values and dt^2/M before the scatter's stores
-  T yk[N], sk[MODE == kLeap ? N : 1];
-  if (sact) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const uint32_t g = (uint32_t)(gid[k] & 0x7fffffff);
-      yk[k] = bld<T>(ry, (g * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T), 0u);
-      if constexpr (MODE == kLeap) sk[k] = bld<T>(rs, g * (uint32_t)sizeof(T), 0u);
-    }
-  }
-  __syncthreads();
-  if (sact) {
-    T Dti[N], Dtj[N];
-#pragma unroll
-    for (int l = 0; l < N; ++l) {
-      Dti[l] = Dm[l * N + i];
-      Dtj[l] = Dm[l * N + j];
-    }
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      T o = 0;
+    __syncthreads();
+    T v1[N], v2[N], v3[N];
+    if (sok) {
+      T Di[N], Dj[N];
 #pragma unroll
       for (int l = 0; l < N; ++l) {
-        o += Dti[l] * s1[(k * P + j * N + l) * ST + lane];
-        o += Dtj[l] * s2[(k * P + l * N + i) * ST + lane];
-        o += Dm[l * N + k] * v3[l];
+        Di[l] = sD[i * N + l];
+        Dj[l] = sD[j * N + l];
       }
-      const int32_t m = gid[k];
-      const uint32_t off =
-          ((uint32_t)(m & 0x7fffffff) * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T);
-      const bool first = m < 0;
-      const T y = yk[k];
-      T r;
-      if constexpr (MODE == kLeap)
-        r = first ? (T(2) * u[k] - y - sk[k] * o) : (y - sk[k] * o);
-      else
-        r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
-      bst<T>(r, ry, off, 0u);
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        T a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          a1 += Di[l] * su[(k * P + j * N + l) * ST + lane];
+          a2 += Dj[l] * su[(k * P + l * N + i) * ST + lane];
+          a3 += Dm[k * N + l] * u[l];
+        }
+        const A b1 = a1, b2 = a2, b3 = a3;
+        const A *g = lg + (ij * N + k) * NG;
+        if constexpr (TRI) {
+          A w1, w2, w3;
+          tri_flux<A>(g, g + 3, g + 6, g[9], b1, b2, b3, w1, w2, w3);
+          v1[k] = (T)w1;
+          v2[k] = (T)w2;
+          v3[k] = (T)w3;
+        } else {
+          v1[k] = (T)(g[0] * b1 + g[1] * b2 + g[2] * b3);
+          v2[k] = (T)(g[1] * b1 + g[3] * b2 + g[4] * b3);
+          v3[k] = (T)(g[2] * b1 + g[4] * b2 + g[5] * b3);
+        }
+      }
+    }
+    // next element's U, in flight during the divergence
+    T un[N];
+    if (nxt && sok)
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        un[k] = bld<T>(ru, ((uint32_t)(gn[k] & 0x7fffffff) * (uint32_t)nshots + (uint32_t)s) *
+                               (uint32_t)sizeof(T), 0u);
+    __syncthreads();  // every read of the staged U and the geometry is done
+    if (sok) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        s1[(k * P + ij) * ST + lane] = v1[k];
+        s2[(k * P + ij) * ST + lane] = v2[k];
+      }
+    }
+    __syncthreads();
+    if (sok) {
+      T Dti[N], Dtj[N];
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        Dti[l] = sDt[i * N + l];
+        Dtj[l] = sDt[j * N + l];
+      }
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        T o = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          o += Dti[l] * s1[(k * P + j * N + l) * ST + lane];
+          o += Dtj[l] * s2[(k * P + l * N + i) * ST + lane];
+          o += Dm[l * N + k] * v3[l];
+        }
+        const int32_t m = gid[k];
+        const uint32_t off =
+            ((uint32_t)(m & 0x7fffffff) * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T);
+        const bool first = m < 0;
+        const T y = yk[k];
+        T r;
+        if constexpr (MODE == kLeap)
+          r = first ? (T(2) * u[k] - y - sk[k] * o) : (y - sk[k] * o);
+        else
+          r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
+        bst<T>(r, ry, off, 0u);
+      }
+    }
+    if (nxt) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        gid[k] = gn[k];
+        u[k] = un[k];
+      }
+      __syncthreads();  // the fluxes are read before the next element is staged
     }
   }
 }
@@ -177,11 +203,11 @@ struct ShotTile {
   static constexpr int value = pick();
 };
 
-template <int N, int ST, typename T, typename TG>
+template <int N, int ST, int R, typename T, typename TG>
 static int run_shots_st(const StiffArgs &a, hipStream_t st) {
   const int64_t ne = a.e1 - a.e0;
   if (ne <= 0) return 0;
-  const dim3 grid((unsigned)ne, (unsigned)((a.nshots + ST - 1) / ST));
+  const dim3 grid((unsigned)((ne + R - 1) / R), (unsigned)((a.nshots + ST - 1) / ST));
   const dim3 block(N * N * ST);
   const T *D = static_cast<const T *>(a.Dm);
   const T *U = static_cast<const T *>(a.U);
@@ -192,41 +218,58 @@ static int run_shots_st(const StiffArgs &a, hipStream_t st) {
   const TG *gw = static_cast<const TG *>(a.glxw);
   const bool tri = a.geo == kGeoTrilinear;
   if (tri && (!cf || !gw)) return -1002;
-#define MJW_SHOTS_LAUNCH(MODE, TRI)                                                              \
-  hipLaunchKernelGGL((k_shots3d<N, ST, T, TG, MODE, TRI>), grid, block, 0, st, a.map, G, D, a.E, \
-                     a.e0, a.e1, U, Y, s, (T)a.alpha, (T)a.beta, a.nshots, a.nxcd, cf, gw)
+#define MJW_SHOTS_LAUNCH(RR, MODE, TRI)                                                        \
+  hipLaunchKernelGGL((k_shots3d<N, ST, RR, T, TG, MODE, TRI>), grid, block, 0, st, a.map, G, D, \
+                     a.E, a.e0, a.e1, U, Y, s, (T)a.alpha, (T)a.beta, a.nshots, a.nxcd, cf, gw)
   if (a.mode == kLeap) {
-    if (tri) MJW_SHOTS_LAUNCH(kLeap, true);
-    else MJW_SHOTS_LAUNCH(kLeap, false);
+    if (tri) MJW_SHOTS_LAUNCH(R, kLeap, true);
+    else MJW_SHOTS_LAUNCH(R, kLeap, false);
   } else {
-    if (tri) MJW_SHOTS_LAUNCH(kApply, true);
-    else MJW_SHOTS_LAUNCH(kApply, false);
+    // operator applications (setup, CFL, energy) run one element per block
+    if (R != 1) return -1005;
+    if (tri) MJW_SHOTS_LAUNCH(1, kApply, true);
+    else MJW_SHOTS_LAUNCH(1, kApply, false);
   }
 #undef MJW_SHOTS_LAUNCH
   return (int)hipGetLastError();
 }
 
-// a.epb > 0 requests that many shot lanes (a power of two up to the limit);
-// 0 picks the smallest power of two >= nshots within the limit
+// launch shape code: ST shot lanes per column and runs of R elements per
+// block, code = ST for R = 1, 100 R + ST otherwise (R = 2, 4 with ST >= 8);
+// 0 picks the smallest power of two >= nshots within the limit, R = 1
 template <int N, typename T, typename TG>
 static int run_shots_t(const StiffArgs &a, hipStream_t st) {
   constexpr int SM = ShotTile<N, T, TG>::value;
   static_assert(SM >= 1, "shot tile");
-  int want = a.epb;
-  if (want <= 0) {
+  int want = a.epb % 100, R = a.epb / 100;
+  if (a.epb <= 0) {
     want = 1;
+    R = 0;
     while (want < a.nshots && want < SM) want *= 2;
   }
   if (want > SM) return -1005;
+  // operator applications take the one-element form of the same lanes
+  if (a.mode != kLeap) R = 0;
   if constexpr (SM >= 16)
-    if (want == 16) return run_shots_st<N, 16, T, TG>(a, st);
+    if (want == 16) {
+      if (R == 4) return run_shots_st<N, 16, 4, T, TG>(a, st);
+      if (R == 2) return run_shots_st<N, 16, 2, T, TG>(a, st);
+      if (R == 0) return run_shots_st<N, 16, 1, T, TG>(a, st);
+      return -1005;
+    }
   if constexpr (SM >= 8)
-    if (want == 8) return run_shots_st<N, 8, T, TG>(a, st);
+    if (want == 8) {
+      if (R == 4) return run_shots_st<N, 8, 4, T, TG>(a, st);
+      if (R == 2) return run_shots_st<N, 8, 2, T, TG>(a, st);
+      if (R == 0) return run_shots_st<N, 8, 1, T, TG>(a, st);
+      return -1005;
+    }
+  if (R != 0) return -1005;
   if constexpr (SM >= 4)
-    if (want == 4) return run_shots_st<N, 4, T, TG>(a, st);
+    if (want == 4) return run_shots_st<N, 4, 1, T, TG>(a, st);
   if constexpr (SM >= 2)
-    if (want == 2) return run_shots_st<N, 2, T, TG>(a, st);
-  if (want == 1) return run_shots_st<N, 1, T, TG>(a, st);
+    if (want == 2) return run_shots_st<N, 2, 1, T, TG>(a, st);
+  if (want == 1) return run_shots_st<N, 1, 1, T, TG>(a, st);
   return -1005;
 }
 
@@ -236,7 +279,13 @@ static void shot_tiles_n(int state, int geom, std::vector<int> &v) {
   if (state == kF64) sm = ShotTile<N, double, double>::value;
   else if (geom == kG64) sm = ShotTile<N, float, double>::value;
   else sm = ShotTile<N, float, float>::value;
-  for (int t = sm; t >= 1; t /= 2) v.push_back(t);
+  for (int t = sm; t >= 1; t /= 2) {
+    v.push_back(t);
+    if (t >= 8) {
+      v.push_back(200 + t);
+      v.push_back(400 + t);
+    }
+  }
 }
 
 std::vector<int> tile_choices_shots(int order, int state, int geom) {
@@ -309,17 +358,23 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
     for (int q = t; q < kCoef; q += P * ST)
       if (act) lc[q] = coef[e * kCoef + q];
     __syncthreads();
-    if (act && lane < N) {
-      TriCol<A> tc;
-      tri_setup<A, TG>(lc, (A)glxw[i], (A)glxw[j], (A)glxw[N + i] * (A)glxw[N + j], false, tc);
-      for (int k = lane; k < N; k += ST) tri_m<A>(tc, (A)glxw[k], (A)glxw[N + k], lm + (ij * N + k) * 9);
-    }
+    // one thread per node gq = i + n j + n^2 k (the first n^3 threads)
+    if (act)
+      for (int gq = t; gq < NN; gq += P * ST) {
+        const int gij = gq % P, gk = gq / P, gi = gij % N, gj = gij / N;
+        TriCol<A> tc;
+        tri_setup<A, TG>(lc, (A)glxw[gi], (A)glxw[gj], (A)glxw[N + gi] * (A)glxw[N + gj], false,
+                         tc);
+        tri_m<A>(tc, (A)glxw[gk], (A)glxw[N + gk], lm + (gij * N + gk) * 9);
+      }
   } else {
     if (act)
-      for (int k = lane; k < N; k += ST)
+      for (int gq = t; gq < NN; gq += P * ST) {
+        const int gij = gq % P, gk = gq / P;
 #pragma unroll
         for (int c = 0; c < 9; ++c)
-          lm[(ij * N + k) * 9 + c] = (A)G[((int64_t)(k * 9 + c) * E + e) * P + ij];
+          lm[(gij * N + gk) * 9 + c] = (A)G[((int64_t)(gk * 9 + c) * E + e) * P + gij];
+      }
   }
   int32_t gid[N];
   A lam = 0, mu = 0;
