@@ -89,14 +89,19 @@ __global__ __launch_bounds__(N *N *ST) void k_shots3d(
 #pragma unroll
       for (int k = 0; k < N; ++k) gn[k] = map[((int64_t)k * E + e + 1) * P + ij];
     T yk[N], sk[MODE == kLeap ? N : 1];
-    if (sok) {
+    auto load_y = [&]() {
+      if (sok) {
 #pragma unroll
-      for (int k = 0; k < N; ++k) {
-        const uint32_t g = (uint32_t)(gid[k] & 0x7fffffff);
-        yk[k] = bld<T>(ry, (g * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T), 0u);
-        if constexpr (MODE == kLeap) sk[k] = bld<T>(rs, g * (uint32_t)sizeof(T), 0u);
+        for (int k = 0; k < N; ++k) {
+          const uint32_t g = (uint32_t)(gid[k] & 0x7fffffff);
+          yk[k] = bld<T>(ry, (g * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T), 0u);
+          if constexpr (MODE == kLeap) sk[k] = bld<T>(rs, g * (uint32_t)sizeof(T), 0u);
+        }
       }
-    }
+    };
+    // runs load Y early (hidden behind the gradient); single elements load it
+    // after the gradient, which keeps fewer registers live
+    if constexpr (R > 1) load_y();
     __syncthreads();
     T v1[N], v2[N], v3[N];
     if (sok) {
@@ -130,6 +135,7 @@ __global__ __launch_bounds__(N *N *ST) void k_shots3d(
         }
       }
     }
+    if constexpr (R == 1) load_y();
     // next element's U, in flight during the divergence
     T un[N];
     if (nxt && sok)

@@ -20,7 +20,7 @@
 // TypeEquation (ACOUSTIC, TIME_ELASTIC), MeshPath, FileMesh, TypeMesh,
 // OrderDiscretization, ConditionReference (DIRICHLET), MateriauDielec
 // (ISOTROPE), PhysicalMedia (rho / mu, SINUS, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
-// TemporalSource (RICKER), TimeStep, TimeInterval, OrderTimeScheme
+// TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
 // ParametersOutputPoint.  Other keywords are reported and ignored; a value
 // this path does not implement (another scheme, source or condition) stops
@@ -659,12 +659,10 @@ int main(int argc, char **argv) {
     return o ? 0 : 1;
   }
 
-  // time parameters
-  const double dt = num(d.need("TimeStep")[0]);
+  const auto &tsv = d.need("TimeStep");
+  const bool auto_dt = tsv[0] == "AUTO";
   const auto &ti = d.need("TimeInterval");
   const double t0 = ti.size() >= 2 ? num(ti[0]) : 0.0, t1 = num(ti.size() >= 2 ? ti[1] : ti[0]);
-  const int64_t nsteps = (int64_t)std::ceil((t1 - t0) / dt - 1e-9);  // RunSecondOrderScheme
-  if (nsteps < 1) die("TimeInterval shorter than one TimeStep");
   const double f0 = num(d.need("Frequency")[0]);
 
   mjw_problem *p = nullptr;
@@ -675,6 +673,27 @@ int main(int argc, char **argv) {
         nullptr, "mjw_create");
   if (!nm.rho.empty())
     check(mjw_set_media_nodal(p, nm.rho.data(), nm.mu.data()), p, "PhysicalMedia");
+  // time step: given, or TimeStep = AUTO: 0.999 times the leapfrog limit
+  // 2 / sqrt(lambda_max(D^-1 K)) (VarInstationary.cxx:91-103,1229-1240,
+  // EvaluateCFL_Generic, EvaluationCflWave.cxx:717-760, P(0) = 2), lambda_max
+  // by Lanczos run until it repeats to 1e-13 (ARPACK in the reference)
+  double dt;
+  if (auto_dt) {
+    double lam = 0.0, prev = -1.0;
+    const int64_t nfree = (int64_t)ncomp * qm.N;
+    for (int it = 64; it <= 4096; it *= 2) {
+      check(mjw_lambda_max(p, (int)std::min<int64_t>(it, std::max<int64_t>(nfree, 2)), 1, &lam), p,
+            "TimeStep = AUTO");
+      if (std::fabs(lam - prev) <= 1e-13 * lam || it >= nfree) break;
+      prev = lam;
+    }
+    dt = 0.999 * 2.0 / std::sqrt(lam);
+    std::printf("Maximal time step = %.15e\n", 2.0 / std::sqrt(lam));
+  } else {
+    dt = num(tsv[0]);
+  }
+  const int64_t nsteps = (int64_t)std::ceil((t1 - t0) / dt - 1e-9);  // RunSecondOrderScheme
+  if (nsteps < 1) die("TimeInterval shorter than one TimeStep");
   // source
   {
     const auto &op = d.need("OriginePhase");

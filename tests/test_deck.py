@@ -113,3 +113,46 @@ def test_driver_seismograms_match_montjoie(name, tmp_path):
     ref = c.traces[:, :out.shape[1] - 1]
     assert out.shape[0] == ref.shape[0]
     assert CS.rel_l2(out[:, 1:], ref) < 1e-10
+
+
+def _dense_lambda_max(c: CS.Case) -> float:
+    """Largest eigenvalue of D^{-1/2} K D^{-1/2} on the free nodes, dense."""
+    m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    P = O.acoustic_problem(m, c.mu_e, c.rho_e)
+    free = np.flatnonzero(m.dirichlet == 0)
+    K = np.zeros((free.size, free.size))
+    for a, g in enumerate(free):
+        e = np.zeros(m.N)
+        e[g] = 1.0
+        K[:, a] = O.apply(P, e)[free]
+    s = 1.0 / np.sqrt(P.M[free])
+    return float(np.linalg.eigvalsh(s[:, None] * K * s[None, :]).max())
+
+
+@pytest.mark.gpu
+def test_driver_time_step_auto(tmp_path):
+    """TimeStep = AUTO: dt = 0.999 * 2 / sqrt(lambda_max(D^-1 K)) (Montjoie's
+    rule; its ARPACK is not in the build, so lambda_max is checked against a
+    dense eigensolver of the oracle's matrix), and the run at that dt against
+    the oracle."""
+    c = CS.load("ref2d_q4_n8")
+    _write(c, tmp_path)
+    ini = (tmp_path / "case.ini").read_text()
+    sub = {"TimeStep": "TimeStep = AUTO",
+           "ParametersOutputPoint": "ParametersOutputPoint = 0.0 1.0 0.0"}  # every step
+    (tmp_path / "case.ini").write_text(
+        "\n".join(sub.get(ln.split(" ")[0], ln) for ln in ini.splitlines()) + "\n")
+    print("deck ref2d_q4_n8 TimeStep = AUTO", flush=True)
+    r = subprocess.run([RUNNER, "case.ini"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    out = np.loadtxt(tmp_path / "totalPoint.dat", ndmin=2)
+    dt = out[1, 0] - out[0, 0]
+    want = 0.999 * 2.0 / np.sqrt(_dense_lambda_max(c))
+    assert abs(dt - want) <= 1e-12 * want
+    T = (c.nsteps - 0.5) * c.dt  # TimeInterval of the deck
+    c.dt = float(out[1, 0])
+    c.nsteps = int(np.ceil(T / c.dt - 1e-9))
+    assert out.shape[0] == c.nsteps + 1
+    _, _, tr = CS.oracle_run(c)
+    assert CS.rel_l2(out[:, 1:], tr[:, :out.shape[1] - 1]) < 1e-10
