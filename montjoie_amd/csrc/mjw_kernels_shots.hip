@@ -454,6 +454,18 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
       }
     }
   }
+  // previous values and dt^2/M, in flight across the barrier (all read
+  // before this thread's first store to Y)
+  T yk[3][N], sk[MODE == kLeap ? N : 1];
+  if (sact) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) yk[m][k] = bld<T>(ry, slot(m, gid[k]), 0u);
+      if constexpr (MODE == kLeap)
+        sk[k] = bld<T>(rs, (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
+    }
+  }
   __syncthreads();
   if (sact) {
     T Dti[N], Dtj[N];
@@ -467,8 +479,7 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
       const int32_t mm = gid[k];
       const bool first = mm < 0;
       T sgd = 0;
-      if constexpr (MODE == kLeap)
-        sgd = bld<T>(rs, (uint32_t)(mm & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
+      if constexpr (MODE == kLeap) sgd = sk[k];
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         T o = 0;
@@ -479,7 +490,7 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
           o += Dm[l * N + k] * v2[m][l];
         }
         const uint32_t off = slot(m, mm);
-        const T y = bld<T>(ry, off, 0u);
+        const T y = yk[m][k];
         T r;
         if constexpr (MODE == kLeap)
           r = first ? (T(2) * u[m][k] - y - sgd * o) : (y - sgd * o);
