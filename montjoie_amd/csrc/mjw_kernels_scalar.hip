@@ -300,9 +300,12 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
   for (int s = 0; s < nshots; ++s) {
     const rsrc_t ru = mkres(U + (int64_t)s * N_nodes);
     const rsrc_t ry = mkres(Y + (int64_t)s * N_nodes);
-    T u = 0;
+    // this node's previous value and dt^2/M with U: in flight during the pass
+    T u = 0, y = 0, sg = 0;
     if (act) {
       u = bld<T>(ru, off, 0u);
+      y = bld<T>(ry, off, 0u);
+      if (MODE == kLeap) sg = bld<T>(rs, off, 0u);
       su[q] = u;
     }
     __syncthreads();
@@ -331,10 +334,8 @@ __global__ __launch_bounds__(EPB *N *N) void k_scalar2d(
         o += Dti[l] * su[j * N + l];
         o += Dtj[l] * sv[l * N + i];
       }
-      const T y = bld<T>(ry, off, 0u);
       T r;
       if (MODE == kLeap) {
-        const T sg = bld<T>(rs, off, 0u);
         r = first ? (T(2) * u - y - sg * o) : (y - sg * o);
       } else {
         r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
