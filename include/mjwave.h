@@ -212,6 +212,17 @@ int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, const double *u
 int mjw_leapfrog_advance(mjw_problem *p, int nsteps, double *traces, int use_graph);
 /* traces of the current iterate only (GiveFinalIterate): [nshots][nrec] */
 int mjw_record(mjw_problem *p, double *traces);
+/*
+ * Probes: the current iterate (the one mjw_record samples) at arbitrary
+ * points, every component, for snapshot grids (SismoPlane / SismoLine
+ * outputs, Output/GridInterpolation.cxx:4975-5166 builds the points,
+ * GenerateGridPoints; WriteSnapshot, Instationary/VarInstationary.cxx:798-853,
+ * samples them).  mjw_set_probes locates the points (inside[i] = 0 for a
+ * point outside the mesh, whose value is then 0, as TranslatePoint leaves it);
+ * mjw_probe writes values[(s*npts + i)*ncomp + m] in FP64.
+ */
+int mjw_set_probes(mjw_problem *p, int npts, const double *pts, uint8_t *inside);
+int mjw_probe(mjw_problem *p, double *values);
 /* current pair (U^n, U^{n+1}) as FP64 host vectors [nshots][n_nodes*ncomp];
  * either may be NULL (GetIterate / GetIterate(k)). */
 int mjw_get_state(mjw_problem *p, double *Un, double *Un1);

@@ -72,6 +72,8 @@ SIGNATURES = {
     "mjw_adjoint_timing": (C.c_int, [_vp, _dp]),
     "mjw_set_mu": (C.c_int, [_vp, _dp]),
     "mjw_set_media_nodal": (C.c_int, [_vp, _dp, _dp]),
+    "mjw_set_probes": (C.c_int, [_vp, C.c_int, _dp, _vp]),
+    "mjw_probe": (C.c_int, [_vp, _dp]),
 }
 
 _lib = None

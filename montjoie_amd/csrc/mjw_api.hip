@@ -46,6 +46,15 @@ struct mjw_problem {
   std::vector<int32_t> en;
   std::vector<uint8_t> dir;
   std::vector<double> bbox;     // E * 2 * dim
+  // point location: uniform buckets over the mesh box, each listing (in
+  // element order) the elements whose box meets it; built on first use
+  std::vector<int32_t> loc_ptr, loc_elem;
+  int loc_n[3] = {0, 0, 0};
+  double loc_lo[3] = {0, 0, 0}, loc_h[3] = {1, 1, 1};
+  // probes: points where mjw_probe samples the current iterate
+  int nprobe = 0;
+  int32_t *d_pr_gid = nullptr;
+  double *d_pr_w = nullptr, *d_pr_out = nullptr;
   std::vector<double> mass;     // N, Dirichlet rows 1
   // device
   int32_t *d_map = nullptr;
@@ -199,7 +208,7 @@ static void release(mjw_problem *p) {
                   p->d_lam, p->d_mu, p->d_rho, p->d_D, p->d_U[0], p->d_U[1], p->d_tmp,
                   p->d_tmp2, p->d_part, p->d_red, p->d_ctl, p->d_pulse, p->d_src_dof,
                   p->d_src_shot, p->d_src_val, p->d_rec_gid, p->d_rec_comp, p->d_rec_w,
-                  p->d_traces};
+                  p->d_traces, p->d_pr_gid, p->d_pr_w, p->d_pr_out};
   for (void *q : ptrs) dfree(q);
   for (void *q : p->snaps) dfree(q);
   void *aptrs[] = {p->d_pair[0], p->d_pair[1], p->d_grad, p->d_adjg, p->d_adj_node,
@@ -779,6 +788,70 @@ static bool locate_in(const mjw_problem *p, int64_t e, const double *x, double *
   return true;
 }
 
+// buckets of about one element each per direction over the mesh box
+static void build_locator(mjw_problem *p) {
+  if (!p->loc_ptr.empty()) return;
+  const int dim = p->dim;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t e = 0; e < p->E; ++e)
+    for (int a = 0; a < dim; ++a) {
+      lo[a] = std::min(lo[a], p->bbox[e * 2 * dim + a]);
+      hi[a] = std::max(hi[a], p->bbox[e * 2 * dim + dim + a]);
+    }
+  const int per = std::max(1, (int)std::llround(std::pow((double)p->E, 1.0 / dim)));
+  int64_t nb = 1;
+  for (int a = 0; a < 3; ++a) {
+    p->loc_n[a] = a < dim ? std::min(per, 1024) : 1;
+    p->loc_lo[a] = a < dim ? lo[a] : 0.0;
+    p->loc_h[a] = a < dim ? std::max((hi[a] - lo[a]) / p->loc_n[a], 1e-300) : 1.0;
+    nb *= p->loc_n[a];
+  }
+  auto range = [&](int a, double x0, double x1, int &i0, int &i1) {
+    i0 = (int)std::floor((x0 - p->loc_lo[a]) / p->loc_h[a]);
+    i1 = (int)std::floor((x1 - p->loc_lo[a]) / p->loc_h[a]);
+    i0 = std::max(0, std::min(p->loc_n[a] - 1, i0));
+    i1 = std::max(0, std::min(p->loc_n[a] - 1, i1));
+  };
+  std::vector<int32_t> cnt(nb + 1, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1) {
+      for (int64_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+      p->loc_ptr = cnt;
+      p->loc_elem.assign(cnt[nb], 0);
+    }
+    std::vector<int32_t> fill(p->loc_ptr.begin(), p->loc_ptr.end());
+    for (int64_t e = 0; e < p->E; ++e) {
+      const double *b = &p->bbox[e * 2 * dim];
+      double h = 0;
+      for (int a = 0; a < dim; ++a) h = std::max(h, b[dim + a] - b[a]);
+      const double tol = 1e-9 * h;
+      int r0[3] = {0, 0, 0}, r1[3] = {0, 0, 0};
+      for (int a = 0; a < dim; ++a) range(a, b[a] - tol, b[dim + a] + tol, r0[a], r1[a]);
+      for (int z = r0[2]; z <= r1[2]; ++z)
+        for (int y = r0[1]; y <= r1[1]; ++y)
+          for (int x = r0[0]; x <= r1[0]; ++x) {
+            const int64_t k = x + (int64_t)p->loc_n[0] * (y + (int64_t)p->loc_n[1] * z);
+            if (pass == 0) ++cnt[k + 1];
+            else p->loc_elem[fill[k]++] = (int32_t)e;
+          }
+    }
+  }
+}
+
+// elements whose box holds x, in element order (empty outside the mesh box)
+static std::pair<const int32_t *, const int32_t *> candidates(mjw_problem *p, const double *x) {
+  build_locator(p);
+  int64_t k = 0, stride = 1;
+  for (int a = 0; a < p->dim; ++a) {
+    const double t = (x[a] - p->loc_lo[a]) / p->loc_h[a];
+    if (!(t >= -1e-6 && t <= p->loc_n[a] + 1e-6)) return {nullptr, nullptr};
+    const int i = std::max(0, std::min(p->loc_n[a] - 1, (int)std::floor(t)));
+    k += i * stride;
+    stride *= p->loc_n[a];
+  }
+  return {p->loc_elem.data() + p->loc_ptr[k], p->loc_elem.data() + p->loc_ptr[k + 1]};
+}
+
 extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, const double *pol,
                                const int32_t *shot) {
   if (!p || nsrc < 0 || (nsrc > 0 && !pts)) FAIL(p, MJW_ERR_ARG, "mjw_set_sources: bad args");
@@ -794,7 +867,9 @@ extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, cons
     const double *x = pts + (size_t)k * p->dim;
     std::unordered_map<int64_t, double> b;  // node -> sum of phi
     int cnt = 0;
-    for (int64_t e = 0; e < p->E; ++e) {
+    const auto cand = candidates(p, x);
+    for (const int32_t *ce = cand.first; ce != cand.second; ++ce) {
+      const int64_t e = *ce;
       if (!in_bbox(p, e, x)) continue;
       if (!locate_in(p, e, x, phi.data())) continue;
       for (int q = 0; q < p->nd; ++q) b[p->en[e * p->nd + q]] += phi[q];
@@ -870,9 +945,10 @@ extern "C" int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts,
     cm[r] = comp ? comp[r] : 0;
     if (cm[r] < 0 || cm[r] >= p->ncomp) FAIL(p, MJW_ERR_ARG, "receiver component out of range");
     int64_t found = -1;
-    for (int64_t e = 0; e < p->E && found < 0; ++e) {
-      if (!in_bbox(p, e, x)) continue;
-      if (locate_in(p, e, x, &w[(size_t)r * p->nd])) found = e;
+    const auto cand = candidates(p, x);
+    for (const int32_t *ce = cand.first; ce != cand.second && found < 0; ++ce) {
+      if (!in_bbox(p, *ce, x)) continue;
+      if (locate_in(p, *ce, x, &w[(size_t)r * p->nd])) found = *ce;
     }
     if (found < 0) FAIL(p, MJW_ERR_ARG, "receiver " + std::to_string(r) + " outside the mesh");
     for (int q = 0; q < p->nd; ++q)
@@ -1099,6 +1175,52 @@ extern "C" int mjw_record(mjw_problem *p, double *traces) {
   HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, p->stream));
   KCHK(p, enqueue_record(p, p->d_traces));
   HIPCHK(p, hipMemcpyAsync(traces, p->d_traces, (size_t)p->nshots * p->nrec * 8,
+                           hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  return MJW_OK;
+}
+
+extern "C" int mjw_set_probes(mjw_problem *p, int npts, const double *pts, uint8_t *inside) {
+  if (!p || npts < 0 || (npts > 0 && !pts)) FAIL(p, MJW_ERR_ARG, "mjw_set_probes: bad args");
+  HIPCHK(p, hipSetDevice(p->device));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  std::vector<int32_t> gid((size_t)npts * p->nd, 0);
+  std::vector<double> w((size_t)npts * p->nd, 0.0);
+  for (int r = 0; r < npts; ++r) {
+    const double *x = pts + (size_t)r * p->dim;
+    int64_t found = -1;
+    const auto cand = candidates(p, x);
+    for (const int32_t *ce = cand.first; ce != cand.second && found < 0; ++ce) {
+      if (!in_bbox(p, *ce, x)) continue;
+      if (locate_in(p, *ce, x, &w[(size_t)r * p->nd])) found = *ce;
+    }
+    if (found >= 0) {
+      for (int q = 0; q < p->nd; ++q)
+        gid[(size_t)r * p->nd + q] = p->nperm[p->en[found * p->nd + q]];
+    } else {
+      std::fill(w.begin() + (size_t)r * p->nd, w.begin() + (size_t)(r + 1) * p->nd, 0.0);
+    }
+    if (inside) inside[r] = found >= 0 ? 1 : 0;
+  }
+  dfree(p->d_pr_gid); dfree(p->d_pr_w); dfree(p->d_pr_out);
+  p->d_pr_gid = nullptr; p->d_pr_w = nullptr; p->d_pr_out = nullptr;
+  p->nprobe = npts;
+  if (npts == 0) return MJW_OK;
+  HIPCHK(p, dalloc(p, &p->d_pr_gid, gid.size() * 4));
+  HIPCHK(p, dalloc(p, &p->d_pr_w, w.size() * 8));
+  HIPCHK(p, dalloc(p, &p->d_pr_out, (size_t)npts * p->nshots * p->ncomp * 8));
+  HIPCHK(p, hipMemcpy(p->d_pr_gid, gid.data(), gid.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(p, hipMemcpy(p->d_pr_w, w.data(), w.size() * 8, hipMemcpyHostToDevice));
+  return MJW_OK;
+}
+
+extern "C" int mjw_probe(mjw_problem *p, double *values) {
+  if (!p || !values) FAIL(p, MJW_ERR_ARG, "mjw_probe: NULL");
+  if (p->nprobe == 0) return MJW_OK;
+  HIPCHK(p, hipSetDevice(p->device));
+  KCHK(p, launch_probe(p->state, p->d_U[p->cur], p->ilv, p->nshots, p->ncomp, p->N, p->nprobe,
+                       p->nd, p->d_pr_gid, p->d_pr_w, p->d_pr_out, p->stream));
+  HIPCHK(p, hipMemcpyAsync(values, p->d_pr_out, (size_t)p->nprobe * p->nshots * p->ncomp * 8,
                            hipMemcpyDeviceToHost, p->stream));
   HIPCHK(p, hipStreamSynchronize(p->stream));
   return MJW_OK;

@@ -104,6 +104,9 @@ int launch_geometry_full(int dim, int order, int ncomp, int64_t E, const int32_t
                          const double *xyz, const double *D, const double *w, const double *rho,
                          const double *mu, const double *rho_q, const double *mu_q, double *G64,
                          float *G32, double *Mloc, int *bad, void *stream);
+// state at probe points (FP64 output, [nshots][npts][ncomp])
+int launch_probe(int state, const void *U, int ilv, int nshots, int ncomp, int64_t N, int npts,
+                 int nd, const int32_t *gid, const double *w, double *out, void *stream);
 // mass assembly of one colour range: M[g] (=|+=) Mloc
 int launch_mass_assemble(const int32_t *map, const double *Mloc, int64_t E, int dim, int order,
                          int64_t e0, int64_t e1, double *M, void *stream);

@@ -150,6 +150,45 @@ int launch_geometry_full(int dim, int order, int ncomp, int64_t E, const int32_t
   return (int)hipGetLastError();
 }
 
+// values of the state at probe points: out[(s*npts + r)*ncomp + m] =
+// sum_q w[r][q] u_m(s)[gid[r][q]] (FP64 sums in a fixed order); one thread
+// per (point, shot, component)
+template <typename T>
+__global__ void k_probe(const T *__restrict__ U, int ilv, int nshots, int ncomp, int64_t N,
+                        int npts, int nd, const int32_t *__restrict__ gid,
+                        const double *__restrict__ w, double *__restrict__ out) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t tot = (int64_t)npts * nshots * ncomp;
+  if (id >= tot) return;
+  const int m = (int)(id % ncomp);
+  const int64_t rs = id / ncomp;
+  const int r = (int)(rs % npts), s = (int)(rs / npts);
+  const int64_t NT = (int64_t)ncomp * N;
+  double acc = 0.0;
+  for (int q = 0; q < nd; ++q) {
+    const double wq = w[(int64_t)r * nd + q];
+    if (wq == 0.0) continue;
+    const int64_t d = (int64_t)m * N + gid[(int64_t)r * nd + q];
+    acc += wq * (double)U[ilv ? d * nshots + s : (int64_t)s * NT + d];
+  }
+  out[id] = acc;
+}
+
+int launch_probe(int state, const void *U, int ilv, int nshots, int ncomp, int64_t N, int npts,
+                 int nd, const int32_t *gid, const double *w, double *out, void *stream) {
+  const int64_t tot = (int64_t)npts * nshots * ncomp;
+  if (tot <= 0) return 0;
+  const dim3 grid((unsigned)((tot + 255) / 256)), block(256);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_probe<double>, grid, block, 0, st, static_cast<const double *>(U), ilv,
+                       nshots, ncomp, N, npts, nd, gid, w, out);
+  else
+    hipLaunchKernelGGL(k_probe<float>, grid, block, 0, st, static_cast<const float *>(U), ilv,
+                       nshots, ncomp, N, npts, nd, gid, w, out);
+  return (int)hipGetLastError();
+}
+
 // lumped mass of one colour: first toucher writes, others add (deterministic)
 __global__ void k_mass(const int32_t *__restrict__ map, const double *__restrict__ Mloc,
                        int64_t E, int n, int dim, int64_t e0, int64_t e1, double *__restrict__ M) {

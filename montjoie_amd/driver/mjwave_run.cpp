@@ -22,7 +22,7 @@
 // (ISOTROPE), PhysicalMedia (rho / mu, SINUS, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
 // TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
-// ParametersOutputPoint.  Other keywords are reported and ignored; a value
+// ParametersOutputPoint, SismoPlane, FileOutputPlane, ParametersOutputPlane.  Other keywords are reported and ignored; a value
 // this path does not implement (another scheme, source or condition) stops
 // the run with a message, as Montjoie's own parser does (cout + abort).
 //
@@ -565,6 +565,109 @@ NodalMedia nodal_media(const Deck &d, const Q1Mesh &m, const QrMesh &qm, const M
   return nm;
 }
 
+// SismoPlane snapshots (Output/GridInterpolation.cxx):
+//   2-D: SismoPlane = xmin xmax ymin ymax Nx Ny | AUTO Nx Ny (:1780-1815; AUTO:
+//        the mesh box shrunk by 5 machine epsilons of its size, :1244-1262),
+//        points (xmin + i hx, ymin + j hy) (:2297-2345);
+//   3-D: SismoPlane = xmin ymin zmin xmax ymax zmax x0 y0 z0 Nx Ny (:4732-4756),
+//        points A + i/(Nx-1) (B - A) + j/(Ny-1) (C - A) (:5119-5166);
+// value index j Nx + i; points outside the mesh hold 0.  Written at the
+// ParametersOutputPlane times as <name>%04d.dat, numbered from 0, in
+// Montjoie's binary double layout (WriteMatlab, :6060-6104: number of grids,
+// dimension, data type, grid type, the grid's parameters, then the values
+// with their count).
+struct Plane {
+  int dim = 2, nx = 0, ny = 0;
+  double p[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // 2-D: xmin xmax ymin ymax; 3-D: A, B, C
+  std::vector<double> pts;                     // nx*ny points, index j*nx + i
+};
+
+Plane plane_grid(const std::vector<std::string> &v, int dim, const QrMesh &qm) {
+  Plane g;
+  g.dim = dim;
+  if (dim == 2) {
+    size_t nb = 0;
+    if (!v.empty() && v[0] == "AUTO") {
+      if (v.size() < 3) die("SismoPlane = AUTO Nx Ny");
+      double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+      for (int64_t q = 0; q < qm.N; ++q)
+        for (int a = 0; a < 2; ++a) {
+          lo[a] = std::min(lo[a], qm.xyz[q * 2 + a]);
+          hi[a] = std::max(hi[a], qm.xyz[q * 2 + a]);
+        }
+      const double eps = 2.220446049250313e-16;
+      for (int a = 0; a < 2; ++a) {
+        const double w = hi[a] - lo[a];
+        g.p[2 * a] = lo[a] + 5.0 * w * eps;
+        g.p[2 * a + 1] = hi[a] - 5.0 * (hi[a] - g.p[2 * a]) * eps;
+      }
+      nb = 1;
+    } else {
+      if (v.size() < 6) die("SismoPlane = xmin xmax ymin ymax Nx Ny");
+      for (int k = 0; k < 4; ++k) g.p[k] = num(v[k]);
+      nb = 4;
+    }
+    g.nx = (int)num(v[nb]);
+    g.ny = (int)num(v[nb + 1]);
+    const double hx = g.nx > 1 ? (g.p[1] - g.p[0]) / (g.nx - 1) : 0.0;
+    const double hy = g.ny > 1 ? (g.p[3] - g.p[2]) / (g.ny - 1) : 0.0;
+    g.pts.resize((size_t)g.nx * g.ny * 2);
+    for (int i = 0; i < g.nx; ++i)
+      for (int j = 0; j < g.ny; ++j) {
+        const size_t k = (size_t)j * g.nx + i;
+        g.pts[k * 2] = g.p[0] + i * hx;
+        g.pts[k * 2 + 1] = g.p[2] + j * hy;
+      }
+  } else {
+    if (v.size() < 11) die("SismoPlane = xmin ymin zmin xmax ymax zmax x0 y0 z0 Nx Ny");
+    double a[3], b[3], c[3];
+    for (int k = 0; k < 3; ++k) {
+      a[k] = num(v[k]);
+      b[k] = num(v[3 + k]);
+      c[k] = num(v[6 + k]);
+    }
+    // header order: xmin xmax ymin ymax zmin zmax x0 y0 z0
+    for (int k = 0; k < 3; ++k) {
+      g.p[2 * k] = a[k];
+      g.p[2 * k + 1] = b[k];
+      g.p[6 + k] = c[k];
+    }
+    g.nx = (int)num(v[9]);
+    g.ny = (int)num(v[10]);
+    const double sx = g.nx > 1 ? 1.0 / (g.nx - 1) : 0.0, sy = g.ny > 1 ? 1.0 / (g.ny - 1) : 0.0;
+    g.pts.resize((size_t)g.nx * g.ny * 3);
+    for (int i = 0; i < g.nx; ++i)
+      for (int j = 0; j < g.ny; ++j) {
+        const size_t k = (size_t)j * g.nx + i;
+        for (int d = 0; d < 3; ++d)
+          g.pts[k * 3 + d] = a[d] + i * sx * (b[d] - a[d]) + j * sy * (c[d] - a[d]);
+      }
+  }
+  if (g.nx < 1 || g.ny < 1) die("SismoPlane: Nx and Ny must be positive");
+  return g;
+}
+
+void write_plane(const std::string &name, const Plane &g, const std::vector<double> &vals) {
+  std::ofstream o(name, std::ios::binary);
+  if (!o) die("cannot write " + name);
+  auto wi = [&](int32_t x) { o.write(reinterpret_cast<const char *>(&x), 4); };
+  auto wd = [&](double x) { o.write(reinterpret_cast<const char *>(&x), 8); };
+  wi(1);                        // one grid
+  wi(g.dim);                    // dimension
+  wi(1);                        // real double values
+  wi(g.dim == 2 ? 0 : 2);       // grid type: plane
+  if (g.dim == 2) {
+    wd(g.p[0]); wd(g.p[1]); wi(g.nx);
+    wd(g.p[2]); wd(g.p[3]); wi(g.ny);
+  } else {
+    for (int k = 0; k < 9; ++k) wd(g.p[k]);
+    wi(g.nx); wi(g.ny);
+  }
+  wi((int32_t)vals.size());
+  o.write(reinterpret_cast<const char *>(vals.data()), (std::streamsize)(vals.size() * 8));
+  if (!o) die("cannot write " + name);
+}
+
 template <class T>
 void wr(std::ofstream &o, const std::vector<T> &v) {
   o.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
@@ -600,7 +703,8 @@ int main(int argc, char **argv) {
       "TypeElement", "TypeEquation", "MeshPath", "FileMesh", "TypeMesh", "OrderDiscretization",
       "ConditionReference", "MateriauDielec", "PhysicalMedia", "TypeSource", "OriginePhase", "Polarization",
       "Frequency", "TemporalSource", "TimeStep", "TimeInterval", "OrderTimeScheme", "SismoPoint",
-      "SismoCircle", "FileOutputPoint", "ParametersOutputPoint", "OutputFormat",
+      "SismoCircle", "FileOutputPoint", "ParametersOutputPoint", "SismoPlane",
+      "FileOutputPlane", "ParametersOutputPlane", "OutputFormat",
       "ElectricOrMagnetic", "PrintLevel"};
   for (const auto &kv : d.kv)
     if (!known.count(kv.first))
@@ -741,11 +845,68 @@ int main(int argc, char **argv) {
   const double t_setup =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
 
+  // snapshot planes: probe points and the steps they are written at
+  std::vector<Plane> planes;
+  for (const auto &v : d.all("SismoPlane")) planes.push_back(plane_grid(v, dim, qm));
+  if (planes.size() > 1) die("SismoPlane: one plane per deck");
+  std::vector<int64_t> snap_steps;
+  std::string plane_name;
+  std::vector<double> probe_pts;
+  if (!planes.empty()) {
+    if (elastic) die("SismoPlane: snapshots are written for ACOUSTIC decks only");
+    const auto *of = d.get("OutputFormat");
+    if (!of || of->size() < 2 || (*of)[0] != "BINARY" || (*of)[1] != "DOUBLE")
+      die("SismoPlane: only OutputFormat = BINARY DOUBLE is written");
+    const auto &fo = d.need("FileOutputPlane");
+    plane_name = fo.size() >= 2 ? fo[1] : fo[0];
+    double ps0 = t0, ps1 = t0 + nsteps * dt, psd = dt;
+    if (const auto *po = d.get("ParametersOutputPlane"))
+      if (po->size() >= 3) { ps0 = num((*po)[0]); ps1 = num((*po)[1]); psd = num((*po)[2]); }
+    const int64_t every = std::max<int64_t>(1, std::llround(psd / dt));
+    for (int64_t s = 0; s <= nsteps; ++s) {
+      const double t = t0 + s * dt;
+      if (t < ps0 - 1e-9 * dt || t > ps1 + 1e-9 * dt) continue;
+      if (std::llround((t - ps0) / dt) % every) continue;
+      snap_steps.push_back(s);
+    }
+    for (const auto &g : planes) probe_pts.insert(probe_pts.end(), g.pts.begin(), g.pts.end());
+    check(mjw_set_probes(p, (int)(probe_pts.size() / dim), probe_pts.data(), nullptr), p,
+          "SismoPlane");
+  }
+  int snap_no = 0;
+  std::vector<double> pv(probe_pts.size() / dim);
+  auto snapshot = [&]() {
+    check(mjw_probe(p, pv.data()), p, "SismoPlane");
+    size_t off = 0;
+    for (const auto &g : planes) {
+      const size_t n = (size_t)g.nx * g.ny;
+      std::vector<double> vals(pv.begin() + off, pv.begin() + off + n);
+      off += n;
+      char suf[32];
+      std::snprintf(suf, sizeof suf, "%04d.dat", snap_no);
+      write_plane(plane_name + suf, g, vals);
+    }
+    ++snap_no;
+  };
+
   // RunSecondOrderScheme: U^0 = U^1 = 0, rows n = 0..nsteps
   check(mjw_leapfrog_init(p, t0, dt, nullptr, nullptr), p, "leapfrog_init");
   std::vector<double> tr((size_t)(nsteps + 1) * (nrec ? nrec : 1));
   auto ts = std::chrono::steady_clock::now();
-  check(mjw_leapfrog_advance(p, (int)nsteps, nrec ? tr.data() : nullptr, 1), p, "advance");
+  int64_t done = 0;
+  for (const int64_t s : snap_steps) {
+    if (s > done) {
+      check(mjw_leapfrog_advance(p, (int)(s - done), nrec ? tr.data() + (size_t)done * nrec : nullptr,
+                                 1),
+            p, "advance");
+      done = s;
+    }
+    snapshot();
+  }
+  if (nsteps > done)
+    check(mjw_leapfrog_advance(p, (int)(nsteps - done),
+                               nrec ? tr.data() + (size_t)done * nrec : nullptr, 1),
+          p, "advance");
   if (nrec) check(mjw_record(p, tr.data() + (size_t)nsteps * nrec), p, "record");
   const double t_loop = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
 

@@ -240,6 +240,23 @@ class WaveProblem:
             self._chk(_lib.lib().mjw_record(self._h, dptr(tr)), "record")
         return tr
 
+    def set_probes(self, pts) -> np.ndarray:
+        """Points where probe() samples the current iterate; returns the
+        inside-the-mesh flags (mjw_set_probes)."""
+        pts = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, self.mesh.dim))
+        inside = np.zeros(pts.shape[0], np.uint8)
+        self._chk(_lib.lib().mjw_set_probes(self._h, pts.shape[0], dptr(pts),
+                                            inside.ctypes.data_as(C.c_void_p)), "set_probes")
+        self.nprobe = pts.shape[0]
+        return inside.astype(bool)
+
+    def probe(self) -> np.ndarray:
+        """[nshots, npts, ncomp] values of the current iterate (0 outside)."""
+        out = np.zeros((self.nshots, getattr(self, "nprobe", 0), self.ncomp))
+        if out.size:
+            self._chk(_lib.lib().mjw_probe(self._h, dptr(out)), "probe")
+        return out
+
     def run(self, nsteps: int, graph: bool = True):
         """Receiver traces for rows n = 0..nsteps (RunSecondOrderScheme)."""
         tr = self.advance(nsteps, traces=True, graph=graph)

@@ -325,6 +325,44 @@ def receivers(m: Mesh, pts):
     return elems, W
 
 
+def plane_points(dim: int, plane):
+    """SismoPlane grid points, index j*Nx + i.  2-D plane = (xmin, xmax, ymin,
+    ymax, Nx, Ny): (xmin + i hx, ymin + j hy) (Output/GridInterpolation.cxx:
+    2297-2345).  3-D plane = (A, B, C, Nx, Ny): A + i/(Nx-1) (B - A) +
+    j/(Ny-1) (C - A) (GridInterpolation.cxx:5119-5166)."""
+    plane = [float(v) for v in plane]
+    nx, ny = int(plane[-2]), int(plane[-1])
+    out = np.zeros((ny, nx, dim))
+    if dim == 2:
+        x0, x1, y0, y1 = plane[:4]
+        hx = (x1 - x0) / (nx - 1) if nx > 1 else 0.0
+        hy = (y1 - y0) / (ny - 1) if ny > 1 else 0.0
+        for i in range(nx):
+            for j in range(ny):
+                out[j, i] = (x0 + i * hx, y0 + j * hy)
+    else:
+        a, b, c = (np.array(plane[3 * k:3 * k + 3]) for k in range(3))
+        sx = 1.0 / (nx - 1) if nx > 1 else 0.0
+        sy = 1.0 / (ny - 1) if ny > 1 else 0.0
+        for i in range(nx):
+            for j in range(ny):
+                out[j, i] = a + i * sx * (b - a) + j * sy * (c - a)
+    return out.reshape(-1, dim)
+
+
+def snapshot(m: Mesh, u, pts):
+    """u interpolated at pts (first element holding the point); 0 outside the
+    mesh, as Montjoie leaves snapshot points it cannot place."""
+    pts = np.asarray(pts, np.float64).reshape(-1, m.dim)
+    out = np.zeros(len(pts))
+    w = np.zeros(m.nd)
+    for i, p in enumerate(pts):
+        e = lib().or_receiver(m.dim, m.nx, m.r, m.vert, np.ascontiguousarray(p), w)
+        if e >= 0:
+            out[i] = w @ u[m.elem_nodes[e]]
+    return out
+
+
 @dataclass
 class Problem:
     """A structured acoustic (ncomp=1) or elastic (ncomp=3) problem."""

@@ -108,6 +108,21 @@ def reference_traces(c: Case) -> np.ndarray:
     return tr
 
 
+def plane_file(raw: bytes, npts: int):
+    """(header bytes, values) of a SismoPlane binary double file."""
+    raw = bytes(raw)
+    return raw[:len(raw) - 8 * npts], np.frombuffer(raw[len(raw) - 8 * npts:], np.float64)
+
+
+def plane_steps(c: Case, times) -> list:
+    """Steps of the ParametersOutputPlane snapshots t0, t0 + dt_out, ... <= t1."""
+    t0, t1, dto = (float(v) for v in times)
+    every = max(1, int(round(dto / c.dt)))
+    return [s for s in range(c.nsteps + 1)
+            if t0 - 1e-9 * c.dt <= s * c.dt <= t1 + 1e-9 * c.dt
+            and int(round((s * c.dt - t0) / c.dt)) % every == 0]
+
+
 def rel_l2(a, b) -> float:
     a = np.asarray(a, float)
     b = np.asarray(b, float)

@@ -82,7 +82,7 @@ def sinus_line(name, ref, p):
 
 def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
-               sinus_mu=None):
+               sinus_mu=None, plane=None, plane_times=None):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1."""
     os.makedirs(d, exist_ok=True)
@@ -129,6 +129,11 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     ini.append("FileOutputPoint = diffracPoint.dat totalPoint.dat")
     for p in rec:
         ini.append("SismoPoint = " + " ".join(_f(c) for c in p))
+    if plane is not None and len(plane):
+        ini.append("SismoPlane = " + " ".join(_f(v) if i < len(plane) - 2 else str(int(v))
+                                              for i, v in enumerate(plane)))
+        ini.append("FileOutputPlane = diffracPlane totalPlane")
+        ini.append("ParametersOutputPlane = " + " ".join(_f(v) for v in plane_times))
     ini.append("OutputFormat = BINARY DOUBLE")
     ini.append("ElectricOrMagnetic = " + ("-1" if elastic else "0"))
     ini.append(f"TimeStep = {_f(dt)}")

@@ -34,21 +34,27 @@ GOLD = os.path.dirname(os.path.abspath(__file__))
 
 def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
-             sinus_mu=None):
+             sinus_mu=None, plane=None, plane_times=None):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
-               sinus_rho=sinus_rho, sinus_mu=sinus_mu)
+               sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):
         if f.startswith("total") or f.startswith("diffrac"):
             os.remove(os.path.join(d, f))
+    planes = np.zeros((0, 0), np.uint8)
     r_ = subprocess.run(args, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r_.returncode != 0:
         raise RuntimeError(f"{name}: reference run failed\n{r_.stdout[-3000:]}")
     out = np.loadtxt(os.path.join(d, "totalPoint.dat"), ndmin=2)
     times, traces = out[:, 0], out[:, 1:]
+    if plane is not None:
+        # SismoPlane snapshots totalPlane0000.dat, ... (raw bytes, one row each)
+        files = sorted(f for f in os.listdir(d) if f.startswith("totalPlane"))
+        planes = np.stack([np.frombuffer(open(os.path.join(d, f), "rb").read(), np.uint8)
+                           for f in files])
     np.savez_compressed(
         os.path.join(GOLD, name + ".npz"), dim=dim, nx=nx, r=r, pert=pert, seed=seed,
         layers=np.asarray(layers, float), f0=f0, dt=dt, nsteps=nsteps, src=np.asarray(src, float),
@@ -57,7 +63,10 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         rho_layers=np.asarray(rho_layers if rho_layers is not None else [], float),
         pol=np.asarray(pol if pol is not None else [], float), regular=regular,
         sinus_rho=np.asarray(sinus_rho if sinus_rho is not None else [], float),
-        sinus_mu=np.asarray(sinus_mu if sinus_mu is not None else [], float))
+        sinus_mu=np.asarray(sinus_mu if sinus_mu is not None else [], float),
+        plane=np.asarray(plane if plane is not None else [], float),
+        plane_times=np.asarray(plane_times if plane_times is not None else [], float),
+        plane_files=planes)
     print(f"{name}: {traces.shape} rows, times {times[0]}..{times[-1]}")
 
 
@@ -143,6 +152,25 @@ def ref3d_q3_sinus():
     rho = [0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 1.2, 0.3, 1.7, 1.3, 2.2]
     run_case("ref3d_q3_sinus", 3, 4, 3, 0.1, 31, [1.0], 8.0, 4e-3, 150, (0.46, 0.52, 0.49), rec,
              sinus_rho=rho, sinus_mu=mu)
+
+
+@case
+def ref2d_q4_plane():
+    """8x8 Q4 unit square with a SismoPlane snapshot grid that reaches outside
+    the mesh (zeros there), every 50 steps."""
+    circ = O.circle_points(0.5, 0.5, 0.3, 4)
+    run_case("ref2d_q4_plane", 2, 8, 4, 0.0, 0, [1.0], 10.0, 1e-3, 200, (0.47, 0.52), circ,
+             regular=True, plane=[-0.1, 0.9, 0.2, 1.15, 7, 6], plane_times=[0.0, 0.2, 0.05])
+
+
+@case
+def ref3d_q3_plane():
+    """4^3 Q3 perturbed hexahedra, a tilted SismoPlane A, B, C partly outside
+    the cube, every 30 steps."""
+    rec = seeded_points(3, 4, 81)
+    run_case("ref3d_q3_plane", 3, 4, 3, 0.1, 99, [1.0, 2.5], 8.0, 5e-3, 150, (0.43, 0.52, 0.47),
+             rec, plane=[0.05, 0.1, 0.5, 1.2, 0.1, 0.55, 0.05, 0.95, 0.6, 6, 5],
+             plane_times=[0.0, 0.6, 0.15])
 
 
 if __name__ == "__main__":

@@ -91,3 +91,28 @@ def test_nodal_media_refusals():
     Pel = M.WaveProblem(m, 1.0, 1.0, lam=1.0, ncomp=3)
     with pytest.raises(MjwError):
         Pel.set_media_nodal(rho, mu)
+
+
+@pytest.mark.parametrize("dim,prec,nshots", [(2, "fp64", 1), (3, "fp64", 3), (3, "fp32", 2)])
+def test_probes_match_oracle_interpolation(dim, prec, nshots):
+    """mjw_probe: the current iterate at arbitrary points (0 outside the
+    mesh), every shot, against the oracle's interpolation of the same state."""
+    m = M.regular_mesh(dim, 3, 4, 0.1, seed=3)
+    o = O.mesh(dim, 3, 4, 0.1, 3)
+    P = M.WaveProblem(m, 1.0, 1.0 + np.arange(m.E) % 2, precision=prec, nshots=nshots)
+    rng = np.random.default_rng(dim)
+    pts = rng.uniform(-0.2, 1.2, size=(40, dim))
+    pts[:4] = 0.0  # a vertex, on the boundary
+    inside = P.set_probes(pts)
+    want_in = np.all((pts >= 0) & (pts <= 1), axis=1)
+    assert np.array_equal(inside, want_in)
+    u0 = rng.standard_normal((nshots, m.N))
+    u0[:, o.dirichlet.astype(bool)] = 0.0
+    P.leapfrog_init(1e-3, u0=u0 if nshots > 1 else u0[0])
+    got = P.probe()
+    assert got.shape == (nshots, len(pts), 1)
+    tol = 1e-13 if prec == "fp64" else 1e-6
+    for s in range(nshots):
+        ref = O.snapshot(o, u0[s], pts)
+        assert np.array_equal(got[s, :, 0] == 0, ref == 0)
+        assert CS.rel_l2(got[s, :, 0], ref) < tol

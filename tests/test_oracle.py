@@ -287,3 +287,31 @@ def test_sinus_media_closed_forms():
                                * np.sin(2 * np.pi * 0.7 / 1.3) * cut, rtol=1e-14)
     assert g[1] == 2.0 and g[2] == 2.0  # outside the box
     assert g[3] == 2.0  # on the face x = 0.25, where the cut-off vanishes
+
+
+@pytest.mark.parametrize("name", ["ref2d_q4_plane", "ref3d_q3_plane"])
+def test_oracle_snapshots_match_montjoie(name):
+    """SismoPlane files written by Montjoie: the oracle's state at each
+    snapshot step, interpolated at the oracle's plane points (0 outside the
+    mesh), equals the stored values; the header carries the grid."""
+    c = CS.load(name)
+    z = np.load(os.path.join(CS.GOLD, name + ".npz"))
+    plane = z["plane"]
+    pts = O.plane_points(c.dim, plane)
+    steps = CS.plane_steps(c, z["plane_times"])
+    assert len(steps) == z["plane_files"].shape[0]
+    m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    outside = 0
+    for k, s in enumerate(steps):
+        head, ref = CS.plane_file(z["plane_files"][k], len(pts))
+        ints = np.frombuffer(head[:16], np.int32)
+        assert list(ints) == [1, c.dim, 1, 0 if c.dim == 2 else 2]
+        if s == 0:
+            assert not ref.any()
+            continue
+        uN, _, _ = CS.oracle_run(c, s)
+        got = O.snapshot(m, uN, pts)
+        outside = int((got == 0).sum())
+        assert np.array_equal(got == 0, ref == 0)
+        assert CS.rel_l2(got, ref) < 1e-10
+    assert outside > 0  # the grids reach outside the mesh
