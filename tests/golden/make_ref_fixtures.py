@@ -155,6 +155,17 @@ def ref3d_q3_sinus():
 
 
 @case
+def ref3d_q5_c2s():
+    """The 48^3 headline problem scaled to 12^3: Q5, vertices perturbed by
+    0.1 h (seed 2024), three layers c = 1 + 2 U (seed 2025), Dirac at the centre,
+    Ricker f0 = 8, 400 steps; 16 seeded receivers."""
+    c = [1.0 + 2.0 * O.uniform(2025, k) for k in range(3)]
+    mu = [v * v for v in c]
+    rec = seeded_points(3, 16, 82)
+    run_case("ref3d_q5_c2s", 3, 12, 5, 0.1, 2024, mu, 8.0, 1e-3, 400, (0.5, 0.5, 0.5), rec)
+
+
+@case
 def ref2d_q4_plane():
     """8x8 Q4 unit square with a SismoPlane snapshot grid that reaches outside
     the mesh (zeros there), every 50 steps."""
