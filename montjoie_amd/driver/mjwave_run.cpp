@@ -22,7 +22,8 @@
 // (ISOTROPE), PhysicalMedia (rho / mu, SINUS, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
 // TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
-// ParametersOutputPoint, SismoPlane, FileOutputPlane, ParametersOutputPlane.  Other keywords are reported and ignored; a value
+// ParametersOutputPoint, SismoPlane, FileOutputPlane, ParametersOutputPlane,
+// SismoLine, FileOutputLine, ParametersOutputLine.  Other keywords are reported and ignored; a value
 // this path does not implement (another scheme, source or condition) stops
 // the run with a message, as Montjoie's own parser does (cout + abort).
 //
@@ -565,7 +566,8 @@ NodalMedia nodal_media(const Deck &d, const Q1Mesh &m, const QrMesh &qm, const M
   return nm;
 }
 
-// SismoPlane snapshots (Output/GridInterpolation.cxx):
+// Snapshot grids (Output/GridInterpolation.cxx); SismoLine below.
+// SismoPlane:
 //   2-D: SismoPlane = xmin xmax ymin ymax Nx Ny | AUTO Nx Ny (:1780-1815; AUTO:
 //        the mesh box shrunk by 5 machine epsilons of its size, :1244-1262),
 //        points (xmin + i hx, ymin + j hy) (:2297-2345);
@@ -578,9 +580,33 @@ NodalMedia nodal_media(const Deck &d, const Q1Mesh &m, const QrMesh &qm, const M
 // with their count).
 struct Plane {
   int dim = 2, nx = 0, ny = 0;
-  double p[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // 2-D: xmin xmax ymin ymax; 3-D: A, B, C
+  bool line = false;                           // SismoLine: ny = 1
+  double p[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // plane 2-D: xmin xmax ymin ymax; 3-D: A, B, C
+                                               // line: xmin xmax ymin ymax [zmin zmax]
   std::vector<double> pts;                     // nx*ny points, index j*nx + i
 };
+
+// SismoLine = xmin xmax ymin ymax [zmin zmax] N (GridInterpolation.cxx:1926-1943,
+// :4782-4800): N points from (xmin, ymin[, zmin]) to (xmax, ymax[, zmax])
+Plane line_grid(const std::vector<std::string> &v, int dim) {
+  Plane g;
+  g.dim = dim;
+  g.line = true;
+  const size_t nv = 2 * dim;
+  if (v.size() < nv + 1) die(dim == 2 ? "SismoLine = xmin xmax ymin ymax N"
+                                      : "SismoLine = xmin xmax ymin ymax zmin zmax N");
+  for (size_t k = 0; k < nv; ++k) g.p[k] = num(v[k]);
+  g.nx = (int)num(v[nv]);
+  g.ny = 1;
+  if (g.nx < 1) die("SismoLine: N must be positive");
+  g.pts.resize((size_t)g.nx * dim);
+  for (int i = 0; i < g.nx; ++i)
+    for (int a = 0; a < dim; ++a) {
+      const double h = g.nx > 1 ? (g.p[2 * a + 1] - g.p[2 * a]) / (g.nx - 1) : 0.0;
+      g.pts[(size_t)i * dim + a] = g.p[2 * a] + i * h;
+    }
+  return g;
+}
 
 Plane plane_grid(const std::vector<std::string> &v, int dim, const QrMesh &qm) {
   Plane g;
@@ -655,8 +681,12 @@ void write_plane(const std::string &name, const Plane &g, const std::vector<doub
   wi(1);                        // one grid
   wi(g.dim);                    // dimension
   wi(1);                        // real double values
-  wi(g.dim == 2 ? 0 : 2);       // grid type: plane
-  if (g.dim == 2) {
+  // grid type: plane 0 (2-D) / 2 (3-D), line 1 (2-D) / 3 (3-D)
+  wi(g.line ? (g.dim == 2 ? 1 : 3) : (g.dim == 2 ? 0 : 2));
+  if (g.line) {
+    for (int k = 0; k < 2 * g.dim; ++k) wd(g.p[k]);
+    wi(g.nx);
+  } else if (g.dim == 2) {
     wd(g.p[0]); wd(g.p[1]); wi(g.nx);
     wd(g.p[2]); wd(g.p[3]); wi(g.ny);
   } else {
@@ -704,7 +734,8 @@ int main(int argc, char **argv) {
       "ConditionReference", "MateriauDielec", "PhysicalMedia", "TypeSource", "OriginePhase", "Polarization",
       "Frequency", "TemporalSource", "TimeStep", "TimeInterval", "OrderTimeScheme", "SismoPoint",
       "SismoCircle", "FileOutputPoint", "ParametersOutputPoint", "SismoPlane",
-      "FileOutputPlane", "ParametersOutputPlane", "OutputFormat",
+      "FileOutputPlane", "ParametersOutputPlane", "SismoLine", "FileOutputLine",
+      "ParametersOutputLine", "OutputFormat",
       "ElectricOrMagnetic", "PrintLevel"};
   for (const auto &kv : d.kv)
     if (!known.count(kv.first))
@@ -845,48 +876,60 @@ int main(int argc, char **argv) {
   const double t_setup =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
 
-  // snapshot planes: probe points and the steps they are written at
-  std::vector<Plane> planes;
-  for (const auto &v : d.all("SismoPlane")) planes.push_back(plane_grid(v, dim, qm));
-  if (planes.size() > 1) die("SismoPlane: one plane per deck");
-  std::vector<int64_t> snap_steps;
-  std::string plane_name;
+  // snapshot grids (SismoPlane, SismoLine): probe points, file names and the
+  // steps each is written at (its own ParametersOutput* times and counter)
+  struct Snap {
+    Plane g;
+    std::string name;
+    std::vector<int64_t> steps;
+    size_t off = 0;
+    int count = 0;
+  };
+  std::vector<Snap> snaps;
   std::vector<double> probe_pts;
-  if (!planes.empty()) {
-    if (elastic) die("SismoPlane: snapshots are written for ACOUSTIC decks only");
+  for (const char *kind : {"Plane", "Line"}) {
+    const std::string key = std::string("Sismo") + kind;
+    const auto lines = d.all(key);
+    if (lines.empty()) continue;
+    if (lines.size() > 1) die(key + ": one per deck");
+    if (elastic) die(key + ": snapshots are written for ACOUSTIC decks only");
     const auto *of = d.get("OutputFormat");
     if (!of || of->size() < 2 || (*of)[0] != "BINARY" || (*of)[1] != "DOUBLE")
-      die("SismoPlane: only OutputFormat = BINARY DOUBLE is written");
-    const auto &fo = d.need("FileOutputPlane");
-    plane_name = fo.size() >= 2 ? fo[1] : fo[0];
+      die(key + ": only OutputFormat = BINARY DOUBLE is written");
+    Snap sn;
+    sn.g = std::string(kind) == "Plane" ? plane_grid(lines[0], dim, qm) : line_grid(lines[0], dim);
+    const auto &fo = d.need(std::string("FileOutput") + kind);
+    sn.name = fo.size() >= 2 ? fo[1] : fo[0];
     double ps0 = t0, ps1 = t0 + nsteps * dt, psd = dt;
-    if (const auto *po = d.get("ParametersOutputPlane"))
+    if (const auto *po = d.get(std::string("ParametersOutput") + kind))
       if (po->size() >= 3) { ps0 = num((*po)[0]); ps1 = num((*po)[1]); psd = num((*po)[2]); }
     const int64_t every = std::max<int64_t>(1, std::llround(psd / dt));
-    for (int64_t s = 0; s <= nsteps; ++s) {
-      const double t = t0 + s * dt;
+    for (int64_t st = 0; st <= nsteps; ++st) {
+      const double t = t0 + st * dt;
       if (t < ps0 - 1e-9 * dt || t > ps1 + 1e-9 * dt) continue;
       if (std::llround((t - ps0) / dt) % every) continue;
-      snap_steps.push_back(s);
+      sn.steps.push_back(st);
     }
-    for (const auto &g : planes) probe_pts.insert(probe_pts.end(), g.pts.begin(), g.pts.end());
-    check(mjw_set_probes(p, (int)(probe_pts.size() / dim), probe_pts.data(), nullptr), p,
-          "SismoPlane");
+    sn.off = probe_pts.size() / dim;
+    probe_pts.insert(probe_pts.end(), sn.g.pts.begin(), sn.g.pts.end());
+    snaps.push_back(sn);
   }
-  int snap_no = 0;
+  std::set<int64_t> snap_steps;
+  for (const auto &sn : snaps) snap_steps.insert(sn.steps.begin(), sn.steps.end());
+  if (!probe_pts.empty())
+    check(mjw_set_probes(p, (int)(probe_pts.size() / dim), probe_pts.data(), nullptr), p,
+          "snapshot grid");
   std::vector<double> pv(probe_pts.size() / dim);
-  auto snapshot = [&]() {
-    check(mjw_probe(p, pv.data()), p, "SismoPlane");
-    size_t off = 0;
-    for (const auto &g : planes) {
-      const size_t n = (size_t)g.nx * g.ny;
-      std::vector<double> vals(pv.begin() + off, pv.begin() + off + n);
-      off += n;
+  auto snapshot = [&](int64_t st) {
+    check(mjw_probe(p, pv.data()), p, "snapshot grid");
+    for (auto &sn : snaps) {
+      if (!std::binary_search(sn.steps.begin(), sn.steps.end(), st)) continue;
+      const size_t n = (size_t)sn.g.nx * sn.g.ny;
+      std::vector<double> vals(pv.begin() + sn.off, pv.begin() + sn.off + n);
       char suf[32];
-      std::snprintf(suf, sizeof suf, "%04d.dat", snap_no);
-      write_plane(plane_name + suf, g, vals);
+      std::snprintf(suf, sizeof suf, "%04d.dat", sn.count++);
+      write_plane(sn.name + suf, sn.g, vals);
     }
-    ++snap_no;
   };
 
   // RunSecondOrderScheme: U^0 = U^1 = 0, rows n = 0..nsteps
@@ -901,7 +944,7 @@ int main(int argc, char **argv) {
             p, "advance");
       done = s;
     }
-    snapshot();
+    snapshot(s);
   }
   if (nsteps > done)
     check(mjw_leapfrog_advance(p, (int)(nsteps - done),

@@ -350,6 +350,20 @@ def plane_points(dim: int, plane):
     return out.reshape(-1, dim)
 
 
+def line_points(dim: int, line):
+    """SismoLine points: N points from (xmin, ymin[, zmin]) to (xmax, ymax[,
+    zmax]); line = (xmin, xmax, ymin, ymax, [zmin, zmax,] N)
+    (Output/GridInterpolation.cxx:1926-1943, :4782-4800)."""
+    line = [float(v) for v in line]
+    n = int(line[-1])
+    out = np.zeros((n, dim))
+    for a in range(dim):
+        lo, hi = line[2 * a], line[2 * a + 1]
+        h = (hi - lo) / (n - 1) if n > 1 else 0.0
+        out[:, a] = lo + np.arange(n) * h
+    return out
+
+
 def snapshot(m: Mesh, u, pts):
     """u interpolated at pts (first element holding the point); 0 outside the
     mesh, as Montjoie leaves snapshot points it cannot place."""

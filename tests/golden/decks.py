@@ -82,7 +82,7 @@ def sinus_line(name, ref, p):
 
 def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
-               sinus_mu=None, plane=None, plane_times=None):
+               sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1."""
     os.makedirs(d, exist_ok=True)
@@ -134,6 +134,10 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
                                               for i, v in enumerate(plane)))
         ini.append("FileOutputPlane = diffracPlane totalPlane")
         ini.append("ParametersOutputPlane = " + " ".join(_f(v) for v in plane_times))
+    if line is not None and len(line):
+        ini.append("SismoLine = " + " ".join(_f(v) for v in line[:-1]) + f" {int(line[-1])}")
+        ini.append("FileOutputLine = diffracLine totalLine")
+        ini.append("ParametersOutputLine = " + " ".join(_f(v) for v in line_times))
     ini.append("OutputFormat = BINARY DOUBLE")
     ini.append("ElectricOrMagnetic = " + ("-1" if elastic else "0"))
     ini.append(f"TimeStep = {_f(dt)}")

@@ -34,27 +34,33 @@ GOLD = os.path.dirname(os.path.abspath(__file__))
 
 def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
-             sinus_mu=None, plane=None, plane_times=None):
+             sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
-               sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times)
+               sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times,
+               line=line, line_times=line_times)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):
         if f.startswith("total") or f.startswith("diffrac"):
             os.remove(os.path.join(d, f))
     planes = np.zeros((0, 0), np.uint8)
+    lines = np.zeros((0, 0), np.uint8)
     r_ = subprocess.run(args, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r_.returncode != 0:
         raise RuntimeError(f"{name}: reference run failed\n{r_.stdout[-3000:]}")
     out = np.loadtxt(os.path.join(d, "totalPoint.dat"), ndmin=2)
     times, traces = out[:, 0], out[:, 1:]
+    def snap_files(prefix):
+        # snapshot files <prefix>0000.dat, ... (raw bytes, one row each)
+        files = sorted(f for f in os.listdir(d) if f.startswith(prefix))
+        return np.stack([np.frombuffer(open(os.path.join(d, f), "rb").read(), np.uint8)
+                         for f in files])
     if plane is not None:
-        # SismoPlane snapshots totalPlane0000.dat, ... (raw bytes, one row each)
-        files = sorted(f for f in os.listdir(d) if f.startswith("totalPlane"))
-        planes = np.stack([np.frombuffer(open(os.path.join(d, f), "rb").read(), np.uint8)
-                           for f in files])
+        planes = snap_files("totalPlane")
+    if line is not None:
+        lines = snap_files("totalLine")
     np.savez_compressed(
         os.path.join(GOLD, name + ".npz"), dim=dim, nx=nx, r=r, pert=pert, seed=seed,
         layers=np.asarray(layers, float), f0=f0, dt=dt, nsteps=nsteps, src=np.asarray(src, float),
@@ -66,7 +72,10 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         sinus_mu=np.asarray(sinus_mu if sinus_mu is not None else [], float),
         plane=np.asarray(plane if plane is not None else [], float),
         plane_times=np.asarray(plane_times if plane_times is not None else [], float),
-        plane_files=planes)
+        plane_files=planes,
+        line=np.asarray(line if line is not None else [], float),
+        line_times=np.asarray(line_times if line_times is not None else [], float),
+        line_files=lines)
     print(f"{name}: {traces.shape} rows, times {times[0]}..{times[-1]}")
 
 
@@ -168,20 +177,22 @@ def ref3d_q5_c2s():
 @case
 def ref2d_q4_plane():
     """8x8 Q4 unit square with a SismoPlane snapshot grid that reaches outside
-    the mesh (zeros there), every 50 steps."""
+    the mesh (zeros there), every 50 steps, and a SismoLine every 40 steps."""
     circ = O.circle_points(0.5, 0.5, 0.3, 4)
     run_case("ref2d_q4_plane", 2, 8, 4, 0.0, 0, [1.0], 10.0, 1e-3, 200, (0.47, 0.52), circ,
-             regular=True, plane=[-0.1, 0.9, 0.2, 1.15, 7, 6], plane_times=[0.0, 0.2, 0.05])
+             regular=True, plane=[-0.1, 0.9, 0.2, 1.15, 7, 6], plane_times=[0.0, 0.2, 0.05],
+             line=[0.1, 1.3, 0.2, 0.7, 7], line_times=[0.0, 0.2, 0.04])
 
 
 @case
 def ref3d_q3_plane():
     """4^3 Q3 perturbed hexahedra, a tilted SismoPlane A, B, C partly outside
-    the cube, every 30 steps."""
+    the cube, every 30 steps, and a SismoLine leaving the cube, every 50 steps."""
     rec = seeded_points(3, 4, 81)
     run_case("ref3d_q3_plane", 3, 4, 3, 0.1, 99, [1.0, 2.5], 8.0, 5e-3, 150, (0.43, 0.52, 0.47),
              rec, plane=[0.05, 0.1, 0.5, 1.2, 0.1, 0.55, 0.05, 0.95, 0.6, 6, 5],
-             plane_times=[0.0, 0.6, 0.15])
+             plane_times=[0.0, 0.6, 0.15], line=[0.1, 1.3, 0.2, 0.7, 0.3, 0.6, 9],
+             line_times=[0.0, 0.75, 0.25])
 
 
 if __name__ == "__main__":

@@ -29,7 +29,9 @@ def _write(c: CS.Case, d):
                      sinus_rho=z["sinus_rho"] if "sinus_rho" in z.files else None,
                      sinus_mu=z["sinus_mu"] if "sinus_mu" in z.files else None,
                      plane=z["plane"] if "plane" in z.files else None,
-                     plane_times=z["plane_times"] if "plane_times" in z.files else None)
+                     plane_times=z["plane_times"] if "plane_times" in z.files else None,
+                     line=z["line"] if "line" in z.files else None,
+                     line_times=z["line_times"] if "line_times" in z.files else None)
 
 
 def _dump(path):
@@ -163,27 +165,33 @@ def test_driver_time_step_auto(tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["ref2d_q4_plane", "ref3d_q3_plane"])
 def test_driver_snapshots_match_montjoie(name, tmp_path):
-    """SismoPlane: the driver's totalPlane%04d.dat files against Montjoie's,
-    byte for byte in the header, values to rel-L2 1e-10 with the same zeros
-    (points outside the mesh)."""
+    """SismoPlane and SismoLine: the driver's totalPlane%04d.dat and
+    totalLine%04d.dat files against Montjoie's, byte for byte in the header,
+    values to rel-L2 1e-10 with the same zeros (points outside the mesh)."""
     c = CS.load(name)
     z = np.load(os.path.join(CS.GOLD, name + ".npz"))
     _write(c, tmp_path)
-    print(f"deck {name} SismoPlane", flush=True)
+    print(f"deck {name} SismoPlane + SismoLine", flush=True)
     r = subprocess.run([RUNNER, "case.ini"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
-    files = sorted(f.name for f in tmp_path.iterdir() if f.name.startswith("totalPlane"))
-    ref = z["plane_files"]
-    assert len(files) == ref.shape[0]
-    npts = int(z["plane"][-1] * z["plane"][-2])
-    for k, f in enumerate(files):
-        assert f == f"totalPlane{k:04d}.dat"
-        raw = (tmp_path / f).read_bytes()
-        assert len(raw) == ref.shape[1]
-        h, v = CS.plane_file(raw, npts)
-        hr, vr = CS.plane_file(ref[k], npts)
-        assert h == hr
-        assert np.array_equal(v == 0, vr == 0)
-        if vr.any():
-            assert CS.rel_l2(v, vr) < 1e-10
+    for kind, prefix in (("plane", "totalPlane"), ("line", "totalLine")):
+        files = sorted(f.name for f in tmp_path.iterdir() if f.name.startswith(prefix))
+        ref = z[kind + "_files"]
+        assert len(files) == ref.shape[0]
+        g = z[kind]
+        npts = int(g[-1] * g[-2]) if kind == "plane" else int(g[-1])
+        for k, f in enumerate(files):
+            assert f == f"{prefix}{k:04d}.dat"
+            raw = (tmp_path / f).read_bytes()
+            assert len(raw) == ref.shape[1]
+            h, v = CS.plane_file(raw, npts)
+            hr, vr = CS.plane_file(ref[k], npts)
+            assert h == hr
+            if vr.any():
+                # zeros agree up to boundary points of rounding-size value
+                tiny = 1e-12 * np.abs(vr).max()
+                assert np.all(np.abs(vr[v == 0]) <= tiny) and np.all(np.abs(v[vr == 0]) <= tiny)
+                assert CS.rel_l2(v, vr) < 1e-10
+            else:
+                assert not v.any()

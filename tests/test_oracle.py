@@ -290,28 +290,36 @@ def test_sinus_media_closed_forms():
 
 
 @pytest.mark.parametrize("name", ["ref2d_q4_plane", "ref3d_q3_plane"])
-def test_oracle_snapshots_match_montjoie(name):
-    """SismoPlane files written by Montjoie: the oracle's state at each
-    snapshot step, interpolated at the oracle's plane points (0 outside the
-    mesh), equals the stored values; the header carries the grid."""
+@pytest.mark.parametrize("kind", ["plane", "line"])
+def test_oracle_snapshots_match_montjoie(name, kind):
+    """SismoPlane / SismoLine files written by Montjoie: the oracle's state at
+    each snapshot step, interpolated at the oracle's grid points (0 outside the
+    mesh), equals the stored values; the header carries the grid type."""
     c = CS.load(name)
     z = np.load(os.path.join(CS.GOLD, name + ".npz"))
-    plane = z["plane"]
-    pts = O.plane_points(c.dim, plane)
-    steps = CS.plane_steps(c, z["plane_times"])
-    assert len(steps) == z["plane_files"].shape[0]
+    if kind == "plane":
+        pts = O.plane_points(c.dim, z["plane"])
+        gtype = 0 if c.dim == 2 else 2
+    else:
+        pts = O.line_points(c.dim, z["line"])
+        gtype = 1 if c.dim == 2 else 3
+    steps = CS.plane_steps(c, z[kind + "_times"])
+    files = z[kind + "_files"]
+    assert len(steps) == files.shape[0]
     m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     outside = 0
     for k, s in enumerate(steps):
-        head, ref = CS.plane_file(z["plane_files"][k], len(pts))
-        ints = np.frombuffer(head[:16], np.int32)
-        assert list(ints) == [1, c.dim, 1, 0 if c.dim == 2 else 2]
+        head, ref = CS.plane_file(files[k], len(pts))
+        assert list(np.frombuffer(head[:16], np.int32)) == [1, c.dim, 1, gtype]
         if s == 0:
             assert not ref.any()
             continue
         uN, _, _ = CS.oracle_run(c, s)
         got = O.snapshot(m, uN, pts)
         outside = int((got == 0).sum())
-        assert np.array_equal(got == 0, ref == 0)
+        # zeros agree, up to points on the boundary that one side places
+        # inside (a Dirichlet value of rounding size) and the other outside
+        tiny = 1e-12 * np.abs(ref).max()
+        assert np.all(np.abs(ref[got == 0]) <= tiny) and np.all(np.abs(got[ref == 0]) <= tiny)
         assert CS.rel_l2(got, ref) < 1e-10
     assert outside > 0  # the grids reach outside the mesh
