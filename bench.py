@@ -128,12 +128,13 @@ def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10, 
     wall = time.perf_counter() - t0
     dev_ms = P.last_timing()[0]
     uN, _ = P.state()
+    energy = P.energy()  # FP64 accumulation, also for the FP32 state
     prof = P.profile(profile_steps)
     info = P.info()
     geo = P.geometry
     P.close()
     return dict(wall=wall, dev_ms_step=dev_ms, traces=tr, uN=uN, prof=prof, info=info, geometry=geo,
-                tile=tuned["tile"], nxcd=tuned["nxcd"])
+                tile=tuned["tile"], nxcd=tuned["nxcd"], energy=energy)
 
 
 def shot_sources(nshots, seed=2027):
@@ -143,12 +144,15 @@ def shot_sources(nshots, seed=2027):
 
 def bench_shots(m, mu, dt, steps, warmup, nshots=16):
     """FP32 batch of `nshots` Dirac shots in one problem (geometry read once per
-    element for all shots) against the FP64 path run shot by shot."""
+    element for all shots) against the FP64 path, run shot by shot and as the
+    same interleaved batch (time-to-solution ratios, and the FP32 error against
+    the FP64 batch)."""
     import montjoie_amd as M
     srcs = shot_sources(nshots)
     out = {}
     from montjoie_amd import tuner
-    for prec, ns in (("fp32", nshots), ("fp32_g64", nshots), ("fp64", 1)):
+    for key, prec, ns in (("fp32", "fp32", nshots), ("fp32_g64", "fp32_g64", nshots),
+                          ("fp64_batch", "fp64", nshots), ("fp64", "fp64", 1)):
         P = M.WaveProblem(m, 1.0, mu, precision=prec, nshots=ns)
         P.set_sources(srcs[:ns], shot=np.arange(ns))
         P.set_receivers(receivers())
@@ -159,12 +163,15 @@ def bench_shots(m, mu, dt, steps, warmup, nshots=16):
             P.advance(warmup, traces=False)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        P.advance(steps, traces=True)
+        tr = P.advance(steps, traces=True)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        out[prec] = dict(wall=wall, shots=ns, dev_ms=P.last_timing()[0])
+        out[key] = dict(wall=wall, shots=ns, dev_ms=P.last_timing()[0], traces=tr,
+                        uN=P.state()[0] if key in ("fp32", "fp64_batch") else None)
         P.close()
     t64_batch = out["fp64"]["wall"] * nshots           # FP64 path, one shot at a time
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+    ref = out["fp64_batch"]
     return {
         "shots": nshots,
         "fp32_batch_s": out["fp32"]["wall"],
@@ -175,6 +182,11 @@ def bench_shots(m, mu, dt, steps, warmup, nshots=16):
         "fp32_g64_batch_gdof_steps_s": m.N * steps * nshots / out["fp32_g64"]["wall"] / 1e9,
         "time_to_solution_ratio_fp64_over_fp32_batch": t64_batch / out["fp32"]["wall"],
         "time_to_solution_ratio_fp64_over_fp32_g64_batch": t64_batch / out["fp32_g64"]["wall"],
+        "fp64_batch_s": ref["wall"],
+        "time_to_solution_ratio_fp64_batch_over_fp32_batch": ref["wall"] / out["fp32"]["wall"],
+        "fp32_batch_rel_l2_state_vs_fp64_batch": rel(out["fp32"]["uN"], ref["uN"]),
+        "fp32_batch_rel_l2_traces_vs_fp64_batch": rel(out["fp32"]["traces"], ref["traces"]),
+        "fp32_g64_batch_rel_l2_traces_vs_fp64_batch": rel(out["fp32_g64"]["traces"], ref["traces"]),
     }
 
 
@@ -388,7 +400,14 @@ def bench_elastic(nx=32, order=4, steps=800, nshots=16, warmup=10):
         t0 = time.perf_counter()
         tr = P.advance(steps, traces=True)
         torch.cuda.synchronize()
-        res[prec] = dict(wall=time.perf_counter() - t0, traces=tr, uN=P.state()[1])
+        wall = time.perf_counter() - t0
+        # FP64 energy of the state after the timed steps (the pulse ends at
+        # t = 2.5/f0, long before), and 100 untimed steps later: conserved
+        uN = P.state()[1]
+        e1 = P.energy()
+        P.advance(100, traces=False)
+        e2 = P.energy()
+        res[prec] = dict(wall=wall, traces=tr, uN=uN, e1=e1, e2=e2)
         P.close()
         print(f"# elastic {prec}: {res[prec]['wall']:.3f} s", file=sys.stderr, flush=True)
     ndof = 3 * m.N
@@ -404,6 +423,14 @@ def bench_elastic(nx=32, order=4, steps=800, nshots=16, warmup=10):
         "time_to_solution_ratio_fp64_shot_by_shot_over_fp32_batch": t64_path / res["fp32"]["wall"],
         "fp32_rel_l2_state_vs_fp64": rel(res["fp32"]["uN"], res["fp64"]["uN"]),
         "fp32_rel_l2_traces_vs_fp64": rel(res["fp32"]["traces"], res["fp64"]["traces"]),
+        "energy_fp64_check": {
+            "fp32_vs_fp64_max_rel": float(np.max(np.abs(res["fp32"]["e1"] - res["fp64"]["e1"])
+                                                 / res["fp64"]["e1"])),
+            "fp32_drift_100_steps_max_rel": float(np.max(np.abs(res["fp32"]["e2"] - res["fp32"]["e1"])
+                                                         / res["fp32"]["e1"])),
+            "fp64_drift_100_steps_max_rel": float(np.max(np.abs(res["fp64"]["e2"] - res["fp64"]["e1"])
+                                                         / res["fp64"]["e1"])),
+        },
     }
 
 
@@ -552,6 +579,7 @@ def main():
         r["err_state"] = float(np.linalg.norm(r["uN"] - ref["uN"]) / np.linalg.norm(ref["uN"]))
         r["err_traces"] = float(np.linalg.norm(r["traces"] - ref["traces"]) /
                                 np.linalg.norm(ref["traces"]))
+        r["err_energy"] = float(abs(r["energy"][0] - ref["energy"][0]) / abs(ref["energy"][0]))
     gdof = lambda r: m.N * a.steps / r["wall"] / 1e9  # noqa: E731
     head = res["fp32"]
     nd = 216
@@ -587,6 +615,9 @@ def main():
                      "rel_l2_traces_vs_fp64": res["fp32_g64"]["err_traces"]},
         "fp32_rel_l2_state_vs_fp64": head["err_state"],
         "fp32_rel_l2_traces_vs_fp64": head["err_traces"],
+        "energy_fp64_check": {"fp64": float(ref["energy"][0]), "fp32": float(head["energy"][0]),
+                              "fp32_rel_diff": head["err_energy"],
+                              "fp32_g64_rel_diff": res["fp32_g64"]["err_energy"]},
         "fp32_over_fp64_speedup": gdof(head) / gdof(ref),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
