@@ -19,7 +19,8 @@
 // Keywords read: TypeElement (QUADRANGLE_LOBATTO, HEXAHEDRON_LOBATTO),
 // TypeEquation (ACOUSTIC, TIME_ELASTIC), MeshPath, FileMesh, TypeMesh,
 // OrderDiscretization, ConditionReference (DIRICHLET), MateriauDielec
-// (ISOTROPE), PhysicalMedia (rho / mu, SINUS, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
+// (ISOTROPE), PhysicalMedia (rho / mu, SINUS or MESH / SAME_MESH data files on the
+// computational mesh, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
 // TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
 // ParametersOutputPoint, SismoPlane, FileOutputPlane, ParametersOutputPlane,
@@ -529,6 +530,141 @@ double sinus_at(const Sinus &s, const double *x) {
   return s.offset + s.amp * c;
 }
 
+// Montjoie's local number of the nodal point (x_i, x_j[, x_k]) of a Q_r
+// quadrilateral / hexahedron, stored at i + (r+1) (j + (r+1) k)
+// (QuadrangleQuadrature.cxx:42-100, HexahedronQuadrature.cxx:43-182):
+// vertices, edges (square: bottom, right, top and left reversed; cube: the 12
+// edges of GetEdgeCube from first to second vertex), faces x=0, y=0, z=0,
+// z=r, y=r, x=r (i outer, j inner), then the interior (i outer, k inner).
+std::vector<int> nodal_numbering(int dim, int r) {
+  const int n = r + 1;
+  auto at = [&](int i, int j, int k) { return i + n * (j + n * k); };
+  std::vector<int> num(dim == 3 ? n * n * n : n * n, -1);
+  int nb = 0;
+  if (dim == 2) {
+    num[at(0, 0, 0)] = 0; num[at(r, 0, 0)] = 1; num[at(r, r, 0)] = 2; num[at(0, r, 0)] = 3;
+    nb = 4;
+    for (int i = 1; i < r; ++i) num[at(i, 0, 0)] = nb++;
+    for (int i = 1; i < r; ++i) num[at(r, i, 0)] = nb++;
+    for (int i = 1; i < r; ++i) num[at(r - i, r, 0)] = nb++;
+    for (int i = 1; i < r; ++i) num[at(0, r - i, 0)] = nb++;
+    for (int i = 1; i < r; ++i)
+      for (int j = 1; j < r; ++j) num[at(i, j, 0)] = nb++;
+    return num;
+  }
+  static const int cor[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  for (int c = 0; c < 8; ++c) num[at(cor[c][0] * r, cor[c][1] * r, cor[c][2] * r)] = nb++;
+  static const int edge[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {0, 4}, {1, 5},
+                                  {2, 6}, {3, 7}, {4, 5}, {5, 6}, {7, 6}, {4, 7}};
+  for (const auto &ed : edge)
+    for (int l = 1; l < r; ++l) {
+      int ijk[3];
+      for (int a = 0; a < 3; ++a) ijk[a] = (l * cor[ed[1]][a] * r + (r - l) * cor[ed[0]][a] * r) / r;
+      num[at(ijk[0], ijk[1], ijk[2])] = nb++;
+    }
+  for (int f = 0; f < 6; ++f)
+    for (int i = 1; i < r; ++i)
+      for (int j = 1; j < r; ++j) {
+        const int ijk[6][3] = {{0, i, j}, {i, 0, j}, {i, j, 0}, {i, j, r}, {i, r, j}, {r, i, j}};
+        num[at(ijk[f][0], ijk[f][1], ijk[f][2])] = nb++;
+      }
+  for (int i = 1; i < r; ++i)
+    for (int j = 1; j < r; ++j)
+      for (int k = 1; k < r; ++k) num[at(i, j, k)] = nb++;
+  return num;
+}
+
+// PhysicalMedia = rho|mu ref ISOTROPE MESH|SAME_MESH mesh_file offset amplitude
+//                 data_file order [cte] on the computational mesh
+// (PhysicalProperty.cxx:3674-3704): the data file (ReadMeshData,
+// Output/CommonInputOutput.cxx:325-352: int 0, int E, int offsets[E+1],
+// doubles) gives every element's values at its order-`order` nodal points;
+// offset + amplitude nu (:990-1000) is interpolated with the tensor Lagrange
+// basis on the GL points of that order at the element's quadrature points
+// (ProjectScalar, Harmonic/VarGeometryProblem.cxx:1428-1447).
+void mesh_media(const std::vector<std::string> &v, const Deck &d, const Q1Mesh &m,
+                const QrMesh &qm, int dim, int ref, std::vector<double> &dst) {
+  const std::vector<std::string> p(v.begin() + 2, v.end());
+  if (p.size() < 7) die("PhysicalMedia " + v[0] + " MESH: mesh_file offset amplitude data_file order");
+  if (v[0] == "mu" && p[0] != "ISOTROPE") die("PhysicalMedia mu: only ISOTROPE");
+  const std::string &mesh_file = p[2];
+  const double offset = num(p[3]), amplitude = num(p[4]);
+  std::string data_file = p[5];
+  const int rm = (int)num(p[6]);
+  const auto &fm = d.need("FileMesh");
+  if (p[1] == "MESH" && mesh_file != fm[0])
+    die("PhysicalMedia MESH: the field must live on the computational mesh (" + fm[0] + ")");
+  if (rm < 1 || rm > 30) die("PhysicalMedia MESH: order out of range");
+  std::ifstream in(data_file, std::ios::binary);
+  if (!in && !data_file.empty() && data_file[0] != '/') in.open(d.dir + "/" + data_file, std::ios::binary);
+  if (!in) die("cannot read " + data_file);
+  int32_t head[2];
+  in.read(reinterpret_cast<char *>(head), 8);
+  if (!in || head[0] != 0) die(data_file + ": not a real-valued mesh data file");
+  if (head[1] != qm.E) die(data_file + ": element count differs from the mesh");
+  std::vector<int32_t> off(head[1] + 1);
+  in.read(reinterpret_cast<char *>(off.data()), (std::streamsize)(off.size() * 4));
+  const int npn = dim == 3 ? (rm + 1) * (rm + 1) * (rm + 1) : (rm + 1) * (rm + 1);
+  for (int64_t e = 0; e < qm.E; ++e)
+    if (off[e + 1] - off[e] != npn) die(data_file + ": nodal count differs from order " + p[6]);
+  std::vector<double> nu((size_t)qm.E * npn);
+  in.read(reinterpret_cast<char *>(nu.data()), (std::streamsize)(nu.size() * 8));
+  if (!in) die(data_file + ": truncated");
+  // L[q][a]: Lagrange basis of the order-rm GL points at the order-r points
+  const int n = qm.order + 1, nm1 = rm + 1;
+  double xm[32], wm[32], xq[32], wq[32];
+  check(mjw_gauss_lobatto(rm, xm, wm), nullptr, "mjw_gauss_lobatto");
+  check(mjw_gauss_lobatto(qm.order, xq, wq), nullptr, "mjw_gauss_lobatto");
+  std::vector<double> L((size_t)n * nm1);
+  for (int q = 0; q < n; ++q)
+    for (int a = 0; a < nm1; ++a) {
+      double v = 1.0;
+      for (int b = 0; b < nm1; ++b)
+        if (b != a) v *= (xq[q] - xm[b]) / (xm[a] - xm[b]);
+      L[(size_t)q * nm1 + a] = v;
+    }
+  const std::vector<int> num_loc = nodal_numbering(dim, rm);
+  const int K = dim == 3 ? nm1 : 1, Kq = dim == 3 ? n : 1;
+  const int64_t nd = (int64_t)qm.en.size() / qm.E;
+  std::vector<double> T((size_t)nm1 * nm1 * K), A1((size_t)n * nm1 * K), A2((size_t)n * n * K);
+  for (int64_t e = 0; e < qm.E; ++e) {
+    if (m.elem_ref[e] != ref) continue;
+    for (int c = 0; c < K; ++c)
+      for (int b = 0; b < nm1; ++b)
+        for (int a = 0; a < nm1; ++a) {
+          const size_t t = a + nm1 * (b + (size_t)nm1 * c);
+          T[t] = offset + amplitude * nu[(size_t)e * npn + num_loc[t]];
+        }
+    // separable contractions: a -> i, b -> j, c -> k
+    for (int c = 0; c < K; ++c)
+      for (int b = 0; b < nm1; ++b)
+        for (int i = 0; i < n; ++i) {
+          double s = 0;
+          for (int a = 0; a < nm1; ++a) s += L[(size_t)i * nm1 + a] * T[a + nm1 * (b + (size_t)nm1 * c)];
+          A1[i + n * (b + (size_t)nm1 * c)] = s;
+        }
+    for (int c = 0; c < K; ++c)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          double s = 0;
+          for (int b = 0; b < nm1; ++b) s += L[(size_t)j * nm1 + b] * A1[i + n * (b + (size_t)nm1 * c)];
+          A2[i + n * (j + (size_t)n * c)] = s;
+        }
+    for (int k = 0; k < Kq; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          double s = 0;
+          if (dim == 3) {
+            for (int c = 0; c < K; ++c) s += L[(size_t)k * nm1 + c] * A2[i + n * (j + (size_t)n * c)];
+          } else {
+            s = A2[i + n * j];
+          }
+          dst[e * nd + i + n * (j + (int64_t)n * k)] = s;
+        }
+  }
+}
+
 // rho and mu at every quadrature point (the GL nodes of each element) when a
 // PhysicalMedia line makes them vary; empty otherwise
 struct NodalMedia {
@@ -550,11 +686,15 @@ NodalMedia nodal_media(const Deck &d, const Q1Mesh &m, const QrMesh &qm, const M
       nm.mu[e * nd + q] = md.mu[e];
     }
   for (const auto &v : lines) {
-    if (v.size() < 4) die("PhysicalMedia = name ref ISOTROPE SINUS ...");
+    if (v.size() < 4) die("PhysicalMedia = name ref ISOTROPE SINUS|MESH|SAME_MESH ...");
     if (v[0] != "rho" && v[0] != "mu") die("PhysicalMedia " + v[0] + ": only rho and mu");
     const int ref = (int)num(v[1]);
-    const Sinus s = parse_sinus(v, dim);
     std::vector<double> &dst = v[0] == "rho" ? nm.rho : nm.mu;
+    if (v[3] == "MESH" || v[3] == "SAME_MESH") {
+      mesh_media(v, d, m, qm, dim, ref, dst);
+      continue;
+    }
+    const Sinus s = parse_sinus(v, dim);
     for (int64_t e = 0; e < qm.E; ++e) {
       if (m.elem_ref[e] != ref) continue;
       for (int64_t q = 0; q < nd; ++q) {

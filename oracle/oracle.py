@@ -155,6 +155,127 @@ def sinus_media(pts, xmin, xmax, coef, offset, amplitude, period):
     return np.where(inside, offset + amplitude * val, offset)
 
 
+def nodal_numbering(dim: int, r: int) -> np.ndarray:
+    """num[i, j(, k)]: Montjoie's local number of the nodal point (x_i, x_j(,
+    x_k)) of a Q_r quadrilateral / hexahedron (Quadrature/
+    QuadrangleQuadrature.cxx:42-100, HexahedronQuadrature.cxx:43-182):
+    vertices, then edges (square: bottom, right, top reversed, left reversed;
+    cube: the 12 edges of GetEdgeCube from their first to their second
+    vertex), then faces x=0, y=0, z=0, z=r, y=r, x=r (i outer, j inner), then
+    the interior (i outer, k inner)."""
+    if dim == 2:
+        num = -np.ones((r + 1, r + 1), int)
+        num[0, 0], num[r, 0], num[r, r], num[0, r] = 0, 1, 2, 3
+        nb = 4
+        for i in range(1, r):
+            num[i, 0] = nb; nb += 1
+        for i in range(1, r):
+            num[r, i] = nb; nb += 1
+        for i in range(1, r):
+            num[r - i, r] = nb; nb += 1
+        for i in range(1, r):
+            num[0, r - i] = nb; nb += 1
+        for i in range(1, r):
+            for j in range(1, r):
+                num[i, j] = nb; nb += 1
+        return num
+    num = -np.ones((r + 1,) * 3, int)
+    corners = [(0, 0, 0), (r, 0, 0), (r, r, 0), (0, r, 0), (0, 0, r), (r, 0, r), (r, r, r),
+               (0, r, r)]
+    for n, c in enumerate(corners):
+        num[c] = n
+    nb = 8
+    edges = [(0, 1), (1, 2), (3, 2), (0, 3), (0, 4), (1, 5), (2, 6), (3, 7), (4, 5), (5, 6),
+             (7, 6), (4, 7)]
+    for a, b in edges:
+        p1, p2 = np.array(corners[a]), np.array(corners[b])
+        for l in range(1, r):
+            num[tuple((l * p2 + (r - l) * p1) // r)] = nb
+            nb += 1
+    faces = [lambda i, j: (0, i, j), lambda i, j: (i, 0, j), lambda i, j: (i, j, 0),
+             lambda i, j: (i, j, r), lambda i, j: (i, r, j), lambda i, j: (r, i, j)]
+    for f in faces:
+        for i in range(1, r):
+            for j in range(1, r):
+                num[f(i, j)] = nb
+                nb += 1
+    for i in range(1, r):
+        for j in range(1, r):
+            for k in range(1, r):
+                num[i, j, k] = nb
+                nb += 1
+    return num
+
+
+def mesh_media_points(m: "Mesh", rm: int) -> np.ndarray:
+    """(E, (rm+1)^dim, dim) positions of the order-rm nodal points of every
+    element in Montjoie's local numbering (GL points of order rm mapped by the
+    element's multilinear map, HexahedronGeomReference.cxx:76-112), the points
+    a PhysicalMedia MESH data file gives values at."""
+    x, _ = gl(rm)
+    num = nodal_numbering(m.dim, rm)
+    V = m.vert.reshape(-1, m.dim)
+    nv1 = m.nx + 1
+    npn = (rm + 1) ** m.dim
+    out = np.zeros((m.E, npn, m.dim))
+    for e in range(m.E):
+        if m.dim == 2:
+            a, b = e % m.nx, e // m.nx
+            cv = [a + nv1 * b, a + 1 + nv1 * b, a + 1 + nv1 * (b + 1), a + nv1 * (b + 1)]
+            for i in range(rm + 1):
+                for j in range(rm + 1):
+                    s, t = x[i], x[j]
+                    w = [(1 - s) * (1 - t), s * (1 - t), s * t, (1 - s) * t]
+                    out[e, num[i, j]] = sum(w[c] * V[cv[c]] for c in range(4))
+        else:
+            a, b, c0 = e % m.nx, (e // m.nx) % m.nx, e // (m.nx * m.nx)
+            vid = lambda p, q, z: p + nv1 * (q + nv1 * z)  # noqa: E731
+            cv = [vid(a, b, c0), vid(a + 1, b, c0), vid(a + 1, b + 1, c0), vid(a, b + 1, c0),
+                  vid(a, b, c0 + 1), vid(a + 1, b, c0 + 1), vid(a + 1, b + 1, c0 + 1),
+                  vid(a, b + 1, c0 + 1)]
+            ca = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1),
+                  (0, 1, 1)]
+            for i in range(rm + 1):
+                for j in range(rm + 1):
+                    for k in range(rm + 1):
+                        xi = (x[i], x[j], x[k])
+                        p = np.zeros(3)
+                        for c in range(8):
+                            w = 1.0
+                            for d in range(3):
+                                w *= xi[d] if ca[c][d] else 1 - xi[d]
+                            p += w * V[cv[c]]
+                        out[e, num[i, j, k]] = p
+    return out
+
+
+def mesh_media(m: "Mesh", nu, rm: int, offset: float = 0.0, amplitude: float = 1.0):
+    """PhysicalMedia ... MESH|SAME_MESH on the computational mesh: nu[e] holds
+    the element's values at its order-rm nodal points (Montjoie's local
+    numbering); the field offset + amplitude nu (PhysicalMeshMedia::Init,
+    Elliptic/PhysicalProperty.cxx:990-1000) is interpolated with the
+    tensor Lagrange basis on the GL points of order rm at the quadrature
+    points of order r (ProjectScalar, Harmonic/VarGeometryProblem.cxx:
+    1428-1447).  Returns (E, nd) in the local order i + n j (+ n^2 k)."""
+    xm, _ = gl(rm)
+    xq, _ = gl(m.r)
+    num = nodal_numbering(m.dim, rm)
+    # L[q, a] = phi_a(xq_q) of the order-rm 1-D Lagrange basis
+    L = np.array([[phi(rm, a, t) for a in range(rm + 1)] for t in xq])
+    n = m.r + 1
+    out = np.zeros((m.E, m.nd))
+    nu = np.asarray(nu, float).reshape(m.E, -1)
+    for e in range(m.E):
+        v = offset + amplitude * nu[e]
+        if m.dim == 2:
+            T = v[num]                                    # T[a, b]
+            out[e] = np.einsum("ia,jb,ab->ji", L, L, T).reshape(-1)
+        else:
+            T = v[num]                                    # T[a, b, c]
+            out[e] = np.einsum("ia,jb,kc,abc->kji", L, L, L, T).reshape(-1)
+    return out
+
+
 def quadrature_points(m: "Mesh"):
     """(E, nd, dim) positions of the element-local GL nodes (the quadrature
     points of the Lobatto elements)."""

@@ -50,6 +50,18 @@ def load(name: str) -> Case:
     mu_e = layers[lay]
     rho_e = z["rho_layers"][lay] if elastic else np.ones(E)
     lam_e = z["lam_layers"][lay] if elastic else None
+    mm = {k.split("_")[1]: k for k in z.files if k.startswith("meshmedia_") and k.endswith("_nu")}
+    if mm:
+        # PhysicalMedia ... SAME_MESH data files of medium 1 (every element)
+        m = O.mesh(dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]))
+        fields = {}
+        for nm in mm:
+            g = lambda k: z[f"meshmedia_{nm}_{k}"]  # noqa: E731
+            fields[nm] = O.mesh_media(m, g("nu"), int(g("order")), float(g("offset")),
+                                      float(g("amplitude")))
+        nd = m.nd
+        rho_e = fields.get("rho", np.broadcast_to(rho_e[:, None], (E, nd)).copy())
+        mu_e = fields.get("mu", np.broadcast_to(mu_e[:, None], (E, nd)).copy())
     srho = z["sinus_rho"] if "sinus_rho" in z.files else np.zeros(0)
     smu = z["sinus_mu"] if "sinus_mu" in z.files else np.zeros(0)
     if len(srho) or len(smu):

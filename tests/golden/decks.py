@@ -80,9 +80,22 @@ def sinus_line(name, ref, p):
     return f"PhysicalMedia = {name} {ref} ISOTROPE SINUS " + " ".join(_f(v) for v in vals)
 
 
+def write_don(path, nu):
+    """Binary data file of a PhysicalMedia MESH field (ReadMeshData,
+    Output/CommonInputOutput.cxx:325-352): int 0 (real), int E, int offsets[E+1],
+    then every element's values (double)."""
+    nu = np.asarray(nu, np.float64)
+    E, npn = nu.shape
+    with open(path, "wb") as fh:
+        fh.write(np.array([0, E], np.int32).tobytes())
+        fh.write((np.arange(E + 1, dtype=np.int32) * npn).tobytes())
+        fh.write(nu.tobytes())
+
+
 def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
-               sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None):
+               sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
+               mesh_media=None):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1."""
     os.makedirs(d, exist_ok=True)
@@ -116,6 +129,11 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
                        f"{_f(lam_layers[k])} {_f(layers[k])}")
         else:
             ini.append(f"MateriauDielec = {k + 1} ISOTROPE 1.0 {_f(layers[k])} 0.0")
+    for name, rm, off, amp, nu in (mesh_media or []):
+        # PhysicalMedia ... SAME_MESH on the deck's Medit mesh (medium 1)
+        write_don(os.path.join(d, name + ".don"), nu)
+        ini.append(f"PhysicalMedia = {name} 1 ISOTROPE SAME_MESH mesh.mesh {_f(off)} {_f(amp)} "
+                   f"{name}.don {int(rm)}")
     if sinus_rho is not None and len(sinus_rho):
         ini.append(sinus_line("rho", 1, sinus_rho))
     if sinus_mu is not None and len(sinus_mu):

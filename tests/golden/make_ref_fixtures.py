@@ -34,12 +34,13 @@ GOLD = os.path.dirname(os.path.abspath(__file__))
 
 def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
-             sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None):
+             sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
+             mesh_media=None):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
                sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times,
-               line=line, line_times=line_times)
+               line=line, line_times=line_times, mesh_media=mesh_media)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):
@@ -75,7 +76,9 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         plane_files=planes,
         line=np.asarray(line if line is not None else [], float),
         line_times=np.asarray(line_times if line_times is not None else [], float),
-        line_files=lines)
+        line_files=lines,
+        **{f"meshmedia_{nm}_{k}": v for nm, rm, off, amp, nu in (mesh_media or [])
+           for k, v in (("order", rm), ("offset", off), ("amplitude", amp), ("nu", nu))})
     print(f"{name}: {traces.shape} rows, times {times[0]}..{times[-1]}")
 
 
@@ -172,6 +175,40 @@ def ref3d_q5_c2s():
     mu = [v * v for v in c]
     rec = seeded_points(3, 16, 82)
     run_case("ref3d_q5_c2s", 3, 12, 5, 0.1, 2024, mu, 8.0, 1e-3, 400, (0.5, 0.5, 0.5), rec)
+
+
+def smooth_nodal(m, rm, seed, lo, hi):
+    """Values in [lo, hi] of a seeded smooth field at the order-rm nodal points
+    of every element (Montjoie's local numbering)."""
+    X = O.mesh_media_points(m, rm)
+    a = [1.0 + 2.0 * O.uniform(seed, k) for k in range(m.dim)]
+    g = np.prod([np.sin(np.pi * a[k] * X[..., k] + 0.3 * k) for k in range(m.dim)], axis=0)
+    return lo + (hi - lo) * 0.5 * (1.0 + g)
+
+
+@case
+def ref2d_q4_meshmedia():
+    """8x8 Q4 perturbed quadrilaterals (Medit file), rho and mu from
+    PhysicalMedia SAME_MESH data files: mu of order 3 (interpolated to the
+    Q4 quadrature points), rho of order 4."""
+    m = O.mesh(2, 8, 4, 0.1, 41)
+    mu = smooth_nodal(m, 3, 3031, 0.5, 1.5)
+    rho = smooth_nodal(m, 4, 3032, 0.6, 1.2)
+    circ = O.circle_points(0.5, 0.5, 0.3, 8)
+    run_case("ref2d_q4_meshmedia", 2, 8, 4, 0.1, 41, [1.0], 10.0, 1e-3, 200, (0.48, 0.52), circ,
+             mesh_media=[("mu", 3, 1.0, 2.0, mu), ("rho", 4, 0.5, 1.0, rho)])
+
+
+@case
+def ref3d_q3_meshmedia():
+    """4^3 Q3 perturbed hexahedra, mu of order 2 and rho of order 4 (above
+    the discretisation order) from SAME_MESH data files."""
+    m = O.mesh(3, 4, 3, 0.1, 42)
+    mu = smooth_nodal(m, 2, 3033, 0.5, 1.5)
+    rho = smooth_nodal(m, 4, 3034, 0.6, 1.2)
+    rec = seeded_points(3, 8, 83)
+    run_case("ref3d_q3_meshmedia", 3, 4, 3, 0.1, 42, [1.0], 8.0, 4e-3, 150, (0.47, 0.51, 0.53),
+             rec, mesh_media=[("mu", 2, 1.0, 2.0, mu), ("rho", 4, 0.5, 1.0, rho)])
 
 
 @case

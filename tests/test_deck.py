@@ -31,7 +31,11 @@ def _write(c: CS.Case, d):
                      plane=z["plane"] if "plane" in z.files else None,
                      plane_times=z["plane_times"] if "plane_times" in z.files else None,
                      line=z["line"] if "line" in z.files else None,
-                     line_times=z["line_times"] if "line_times" in z.files else None)
+                     line_times=z["line_times"] if "line_times" in z.files else None,
+                     mesh_media=[(nm, int(z[f"meshmedia_{nm}_order"]),
+                                  float(z[f"meshmedia_{nm}_offset"]),
+                                  float(z[f"meshmedia_{nm}_amplitude"]), z[f"meshmedia_{nm}_nu"])
+                                 for nm in ("mu", "rho") if f"meshmedia_{nm}_nu" in z.files])
 
 
 def _dump(path):
@@ -99,6 +103,20 @@ def test_driver_rejects_unsupported_decks(tmp_path):
     r = subprocess.run([RUNNER, "bad.ini", "--dump-mesh", "x.bin"], cwd=tmp_path,
                        capture_output=True, text=True)
     assert r.returncode != 0 and "LEAP_FROG" in r.stderr
+
+
+def test_driver_rejects_media_on_another_mesh(tmp_path):
+    c = CS.load("ref3d_q3_meshmedia")
+    _write(c, tmp_path)
+    ini = (tmp_path / "case.ini").read_text()
+    (tmp_path / "bad.ini").write_text(ini.replace("SAME_MESH mesh.mesh", "MESH other.mesh"))
+    r = subprocess.run([RUNNER, "bad.ini", "--dump-mesh", "x.bin"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "computational mesh" in r.stderr
+    (tmp_path / "mu.don").write_bytes((tmp_path / "mu.don").read_bytes()[:-8])
+    r = subprocess.run([RUNNER, "case.ini", "--dump-mesh", "x.bin"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "truncated" in r.stderr
 
 
 @pytest.mark.gpu
