@@ -24,7 +24,7 @@
 // TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
 // ParametersOutputPoint, SismoPlane, FileOutputPlane, ParametersOutputPlane,
-// SismoLine, FileOutputLine, ParametersOutputLine.  Other keywords are reported and ignored; a value
+// SismoLine, FileOutputLine, ParametersOutputLine, SaveReprise, PathReprise.  Other keywords are reported and ignored; a value
 // this path does not implement (another scheme, source or condition) stops
 // the run with a message, as Montjoie's own parser does (cout + abort).
 //
@@ -36,6 +36,7 @@
 // node map in local order q = i + n j + n^2 k, Dirichlet flags, media) as raw
 // little-endian arrays and exits without touching the GPU.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -43,6 +44,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <set>
 #include <sstream>
@@ -434,6 +436,174 @@ QrMesh number_nodes(const Q1Mesh &m, int order, const std::set<int> &dirichlet_r
   }
   q.dir = dflag;
   return q;
+}
+
+// ------------------------------------------------- Montjoie DOF numbering
+// Montjoie's global number of every node (the order of its solution vectors,
+// hence of SaveReprise files):
+//   - referenced boundaries sorted first: 2-D edges as (min, max)
+//     (Mesh/Mesh2DBoundaries.cxx:885-910); 3-D faces rotated to their lowest
+//     vertex, reversed when the next vertex exceeds the previous one
+//     (Mesh/Mesh3DBoundaries.cxx:650-700);
+//   - edges numbered as first met on the referenced boundaries (local edges
+//     (v_k, v_k+1)), then on the elements (square (0,1) (1,2) (2,3) (3,0);
+//     cube: Mesh/Mesh3D.cxx:5201-5300 with the edge table of
+//     Quadrature/HexahedronQuadrature.cxx:41-47); faces (3-D)
+//     from the referenced faces, then each element's six faces rotated to their
+//     lowest vertex (Mesh/Mesh3D.cxx:4928-5170);
+//   - DOFs: vertices, then r-1 per edge from its lower to its higher vertex,
+//     then (r-1)^2 per face, (s-1)(r-1) + (t-1) with s along v0 -> v1 and
+//     t along v0 -> v3 of the stored face, then (r-1)^dim per element with the
+//     first index outermost (Mesh/NumberMesh.cxx:695-900,
+//     face table Mesh/NumberMesh.cxx:2609).
+// Returns dof[g] for the driver's node g.
+std::vector<int64_t> montjoie_dofs(const Q1Mesh &m, const QrMesh &q) {
+  const int dim = m.dim, r = q.order, n = r + 1, nc = 1 << dim;
+  const int nd = dim == 3 ? n * n * n : n * n;
+  int64_t nv = 0;
+  for (int64_t v : m.elem) nv = std::max(nv, v + 1);
+  std::unordered_map<Key, int64_t, KeyHash> edges;
+  auto add_edge = [&](int64_t a, int64_t b) {
+    if (a > b) std::swap(a, b);
+    const Key k{(uint64_t)a, (uint64_t)b};
+    if (!edges.count(k)) edges.emplace(k, (int64_t)edges.size());
+  };
+  // faces: key = sorted corners; value = number and stored corner order
+  struct Face {
+    int64_t num;
+    int64_t v[4];
+  };
+  auto face_key = [](const int64_t *f) {
+    int64_t s[4] = {f[0], f[1], f[2], f[3]};
+    std::sort(s, s + 4);
+    return Key{((uint64_t)s[0] << 32) | (uint64_t)s[1], ((uint64_t)s[2] << 32) | (uint64_t)s[3]};
+  };
+  std::unordered_map<Key, Face, KeyHash> faces;
+  auto add_face = [&](const int64_t *f) {
+    const Key k = face_key(f);
+    if (faces.count(k)) return;
+    Face F;
+    F.num = (int64_t)faces.size();
+    std::copy(f, f + 4, F.v);
+    faces.emplace(k, F);
+  };
+  // Medit corner c -> (a, b, c) in {0,1}^dim
+  static const int ca[8] = {0, 1, 1, 0, 0, 1, 1, 0}, cb[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                   cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  if (dim == 2) {
+    for (const auto &b : m.bnd) add_edge(b[0], b[1]);
+    for (size_t e = 0; e * nc < m.elem.size(); ++e)
+      for (int k = 0; k < 4; ++k) add_edge(m.elem[e * nc + k], m.elem[e * nc + (k + 1) % 4]);
+  } else {
+    std::vector<std::array<int64_t, 4>> ref;
+    for (const auto &b : m.bnd) {
+      if (b.size() != 4) die("SaveReprise: boundary faces must be quadrilaterals");
+      int k0 = 0;
+      for (int k = 1; k < 4; ++k)
+        if (b[k] < b[k0]) k0 = k;
+      std::array<int64_t, 4> f;
+      const bool rev = b[(k0 + 1) % 4] > b[(k0 + 3) % 4];
+      for (int j = 0; j < 4; ++j) f[j] = rev ? b[(k0 + 4 - j) % 4] : b[(k0 + j) % 4];
+      ref.push_back(f);
+    }
+    for (const auto &f : ref)
+      for (int k = 0; k < 4; ++k) add_edge(f[k], f[(k + 1) % 4]);
+    static const int ce[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {0, 4}, {1, 5},
+                                  {2, 6}, {3, 7}, {4, 5}, {5, 6}, {7, 6}, {4, 7}};
+    for (size_t e = 0; e * nc < m.elem.size(); ++e)
+      for (const auto &ed : ce) add_edge(m.elem[e * nc + ed[0]], m.elem[e * nc + ed[1]]);
+    for (const auto &f : ref) add_face(f.data());
+    static const int cf[6][4] = {{0, 3, 7, 4}, {0, 1, 5, 4}, {0, 1, 2, 3},
+                                 {4, 5, 6, 7}, {3, 2, 6, 7}, {1, 2, 6, 5}};
+    for (size_t e = 0; e * nc < m.elem.size(); ++e)
+      for (const auto &fl : cf) {
+        int64_t v[4];
+        for (int j = 0; j < 4; ++j) v[j] = m.elem[e * nc + fl[j]];
+        int k0 = 0;
+        for (int k = 1; k < 4; ++k)
+          if (v[k] < v[k0]) k0 = k;
+        int64_t f[4];
+        for (int j = 0; j < 4; ++j) f[j] = v[(k0 + j) % 4];
+        add_face(f);
+      }
+  }
+  const int64_t ne = (int64_t)edges.size(), nf = (int64_t)faces.size();
+  const int64_t off_e = nv, off_f = nv + ne * (r - 1), off_i = off_f + nf * (r - 1) * (r - 1);
+  const int64_t ni = (int64_t)(dim == 3 ? (r - 1) * (r - 1) * (r - 1) : (r - 1) * (r - 1));
+  std::vector<int64_t> dof(q.N, -1);
+  for (int64_t e = 0; e < q.E; ++e) {
+    const int64_t *V = &m.elem[e * nc];
+    auto corner_of = [&](const int *cc) {  // coordinates in {0,1}^dim -> Medit corner
+      for (int c = 0; c < nc; ++c)
+        if (ca[c] == cc[0] && cb[c] == cc[1] && (dim == 2 || cz[c] == cc[2])) return c;
+      return -1;
+    };
+    const int K = dim == 3 ? n : 1;
+    for (int k = 0; k < K; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          const int idx[3] = {i, j, k};
+          int nb = 0;
+          for (int a = 0; a < dim; ++a) nb += idx[a] == 0 || idx[a] == r;
+          int64_t d;
+          if (nb == dim) {
+            const int cc[3] = {i / r, j / r, k / r};
+            d = V[corner_of(cc)];
+          } else if (nb == dim - 1) {
+            int ax = 0;
+            while (idx[ax] == 0 || idx[ax] == r) ++ax;
+            int s0[3], s1[3];
+            for (int a = 0; a < 3; ++a) s0[a] = s1[a] = idx[a] / std::max(r, 1);
+            s0[ax] = 0;
+            s1[ax] = 1;
+            int64_t a = V[corner_of(s0)], b = V[corner_of(s1)];
+            int t = idx[ax];
+            if (a > b) { std::swap(a, b); t = r - t; }
+            d = off_e + edges.at(Key{(uint64_t)a, (uint64_t)b}) * (r - 1) + t - 1;
+          } else if (dim == 3 && nb == 1) {
+            int ax = 0;
+            while (idx[ax] != 0 && idx[ax] != r) ++ax;
+            int64_t fv[4];
+            int pos[4][3], np = 0;
+            for (int c = 0; c < 8; ++c) {
+              const int cc[3] = {ca[c], cb[c], cz[c]};
+              if (cc[ax] * r != idx[ax]) continue;
+              fv[np] = V[c];
+              std::copy(cc, cc + 3, pos[np]);
+              ++np;
+            }
+            const Face &F = faces.at(face_key(fv));
+            auto where = [&](int64_t v) {
+              for (int c = 0; c < 4; ++c)
+                if (fv[c] == v) return c;
+              return -1;
+            };
+            const int *p0 = pos[where(F.v[0])], *p1 = pos[where(F.v[1])], *p3 = pos[where(F.v[3])];
+            int s = 0, t = 0;
+            for (int a = 0; a < 3; ++a) {
+              s += (idx[a] - r * p0[a]) * (p1[a] - p0[a]);
+              t += (idx[a] - r * p0[a]) * (p3[a] - p0[a]);
+            }
+            d = off_f + F.num * (r - 1) * (r - 1) + (int64_t)(s - 1) * (r - 1) + t - 1;
+          } else {
+            int64_t li = 0;
+            for (int a = 0; a < dim; ++a) li = li * (r - 1) + idx[a] - 1;
+            d = off_i + e * ni + li;
+          }
+          dof[q.en[e * nd + i + n * j + n * n * k]] = d;
+        }
+  }
+  return dof;
+}
+
+// Seldon's Vector::Write: int32 size, then the values
+template <class T>
+void write_vector(const std::string &path, const std::vector<T> &v) {
+  std::ofstream o(path, std::ios::binary);
+  const int32_t n = (int32_t)v.size();
+  o.write((const char *)&n, sizeof n);
+  o.write((const char *)v.data(), (std::streamsize)(v.size() * sizeof(T)));
+  if (!o) die("cannot write " + path);
 }
 
 // ------------------------------------------------------------------ media
@@ -875,7 +1045,7 @@ int main(int argc, char **argv) {
       "Frequency", "TemporalSource", "TimeStep", "TimeInterval", "OrderTimeScheme", "SismoPoint",
       "SismoCircle", "FileOutputPoint", "ParametersOutputPoint", "SismoPlane",
       "FileOutputPlane", "ParametersOutputPlane", "SismoLine", "FileOutputLine",
-      "ParametersOutputLine", "OutputFormat",
+      "ParametersOutputLine", "OutputFormat", "SaveReprise", "PathReprise", "LoadReprise",
       "ElectricOrMagnetic", "PrintLevel"};
   for (const auto &kv : d.kv)
     if (!known.count(kv.first))
@@ -899,6 +1069,16 @@ int main(int argc, char **argv) {
     if ((*s)[0] != "SRC_DIRAC") die("TypeSource " + (*s)[0] + ": only SRC_DIRAC");
   if (const auto *s = d.get("TemporalSource"))
     if ((*s)[0] != "RICKER") die("TemporalSource " + (*s)[0] + ": only RICKER");
+  if (const auto *s = d.get("LoadReprise"))
+    if (!s->empty() && (*s)[0] == "YES") die("LoadReprise: restarting is not implemented");
+  // SaveReprise = n: checkpoint every n steps (VarInstationary.cxx:321-333)
+  int64_t save_every = 0;
+  if (const auto *s = d.get("SaveReprise")) {
+    if (s->empty()) die("SaveReprise needs a number of steps");
+    save_every = (int64_t)num((*s)[0]);
+    if (save_every <= 0) die("SaveReprise: the number of steps must be positive");
+  }
+  const std::string path_reprise = d.get("PathReprise") ? d.need("PathReprise")[0] : "./";
 
   // mesh
   const auto &fm = d.need("FileMesh");
@@ -931,6 +1111,7 @@ int main(int argc, char **argv) {
     wr(o, md.lam);
     wr(o, nm.rho);  // empty unless PhysicalMedia: E * nd each
     wr(o, nm.mu);
+    wr(o, montjoie_dofs(q1, qm));
     return o ? 0 : 1;
   }
 
@@ -1072,19 +1253,55 @@ int main(int argc, char **argv) {
     }
   };
 
+  // SaveReprise (SaveDataReprise, VarInstationary.cxx:917-972): at every step
+  // nt with nt % n == 0, the final one included, U^nt in Montjoie's numbering
+  // goes to SolAcous<b>_P0N1.dat, b alternating 0, 1, 0, ..., and
+  // IterationAcousSaveNew0N1.dat = [nt, -1, b] after the previous one has been
+  // copied to IterationAcousSave0N1.dat: a restart reads IterationAcousSave
+  // (a complete SolAcous file) and the New file only when that one is damaged.
+  std::vector<int64_t> mj_dof;
+  if (save_every) mj_dof = montjoie_dofs(q1, qm);
+  int bascule = -1;
+  std::vector<double> un, ux;
+  auto save_reprise = [&](int64_t st) {
+    const std::string suffix = "0N1.dat";
+    const std::vector<int32_t> it = {(int32_t)st, -1, bascule < 0 ? 0 : bascule};
+    if (bascule < 0) {
+      bascule = 0;
+    } else {
+      std::ifstream in(path_reprise + "IterationAcousSaveNew" + suffix, std::ios::binary);
+      std::vector<char> old((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+      std::ofstream o(path_reprise + "IterationAcousSave" + suffix, std::ios::binary);
+      o.write(old.data(), (std::streamsize)old.size());
+      if (!o) die("cannot write " + path_reprise + "IterationAcousSave" + suffix);
+    }
+    write_vector(path_reprise + "IterationAcousSaveNew" + suffix, it);
+    un.resize((size_t)ncomp * qm.N);
+    ux.resize(un.size());
+    check(mjw_get_state(p, un.data(), nullptr), p, "SaveReprise");
+    for (int m = 0; m < ncomp; ++m)
+      for (int64_t g = 0; g < qm.N; ++g) ux[(size_t)m * qm.N + mj_dof[g]] = un[(size_t)m * qm.N + g];
+    write_vector(path_reprise + "SolAcous" + std::to_string(bascule) + "_P" + suffix, ux);
+    bascule = 1 - bascule;
+  };
+  std::set<int64_t> stops = snap_steps;
+  if (save_every)
+    for (int64_t st = 0; st <= nsteps; st += save_every) stops.insert(st);
+
   // RunSecondOrderScheme: U^0 = U^1 = 0, rows n = 0..nsteps
   check(mjw_leapfrog_init(p, t0, dt, nullptr, nullptr), p, "leapfrog_init");
   std::vector<double> tr((size_t)(nsteps + 1) * (nrec ? nrec : 1));
   auto ts = std::chrono::steady_clock::now();
   int64_t done = 0;
-  for (const int64_t s : snap_steps) {
+  for (const int64_t s : stops) {
     if (s > done) {
       check(mjw_leapfrog_advance(p, (int)(s - done), nrec ? tr.data() + (size_t)done * nrec : nullptr,
                                  1),
             p, "advance");
       done = s;
     }
-    snapshot(s);
+    if (save_every && s % save_every == 0) save_reprise(s);
+    if (snap_steps.count(s)) snapshot(s);
   }
   if (nsteps > done)
     check(mjw_leapfrog_advance(p, (int)(nsteps - done),

@@ -12,6 +12,7 @@ tests (regular / perturbed meshes of the unit square and cube).
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import os
 import subprocess
 from dataclasses import dataclass, field
@@ -274,6 +275,149 @@ def mesh_media(m: "Mesh", nu, rm: int, offset: float = 0.0, amplitude: float = 1
             T = v[num]                                    # T[a, b, c]
             out[e] = np.einsum("ia,jb,kc,abc->kji", L, L, L, T).reshape(-1)
     return out
+
+
+def montjoie_dofs(dim: int, r: int, corners, boundary, elem_nodes, n_nodes: int) -> np.ndarray:
+    """Montjoie's global DOF number of every node of a continuous Q_r mesh.
+
+    corners[e]: the element's vertices (Medit order), boundary: the referenced
+    boundary edges (2-D) or faces (3-D) in file order, elem_nodes[e][i + n j
+    (+ n^2 k)]: the caller's node numbers.  Restates
+      - referenced boundaries sorted: 2-D edges (min, max)
+        (Mesh2DBoundaries.cxx:885-910); 3-D faces rotated to their lowest
+        vertex, reversed when the next vertex exceeds the previous one
+        (Mesh3DBoundaries.cxx:650-700);
+      - edges numbered as first met on the referenced boundaries (local edges
+        (v_k, v_k+1)), then on the elements (square (0,1) (1,2) (2,3) (3,0);
+        cube: Quadrature/HexahedronQuadrature.cxx:41-47) (Mesh2D.cxx:2435-2454,
+        Mesh3D.cxx:5201-5300,
+        MeshBase.cxx:844-881); faces (3-D) from the referenced faces, then the
+        elements' local faces (NumberMesh.cxx:2609) rotated to their lowest
+        vertex (Mesh3D.cxx:4928-5170);
+      - DOFs: vertices, then r-1 per edge from its lower to its higher vertex,
+        then (r-1)^2 per face, index (s-1)(r-1) + (t-1) with s along
+        v0 -> v1 and t along v0 -> v3 of the stored face, then (r-1)^d per
+        element, interior nodes with the first index outermost
+        (NumberMesh.cxx:695-900, face table NumberMesh.cxx:2609)."""
+    n = r + 1
+    nv = int(max(max(c) for c in corners)) + 1
+    edges, eorder = {}, []
+
+    def add_edge(a, b):
+        k = (min(a, b), max(a, b))
+        if k not in edges:
+            edges[k] = len(eorder)
+            eorder.append(k)
+    if dim == 2:
+        for a, b in boundary:
+            add_edge(a, b)
+        for c in corners:
+            for a, b in ((0, 1), (1, 2), (2, 3), (3, 0)):
+                add_edge(c[a], c[b])
+        faces, nf = {}, 0
+    else:
+        def canon(f):
+            f = list(f)
+            k = int(np.argmin(f))
+            g = f[k:] + f[:k]
+            return [g[0], g[3], g[2], g[1]] if g[1] > g[3] else g
+        ref = [canon(f) for f in boundary]
+        for f in ref:
+            for t in range(4):
+                add_edge(f[t], f[(t + 1) % 4])
+        cube_edges = [(0, 1), (1, 2), (3, 2), (0, 3), (0, 4), (1, 5), (2, 6), (3, 7), (4, 5),
+                      (5, 6), (7, 6), (4, 7)]
+        for c in corners:
+            for a, b in cube_edges:
+                add_edge(c[a], c[b])
+        faces = {}
+        for f in ref:
+            faces.setdefault(frozenset(f), (len(faces), f))
+        cube_faces = [(0, 3, 7, 4), (0, 1, 5, 4), (0, 1, 2, 3), (4, 5, 6, 7), (3, 2, 6, 7),
+                      (1, 2, 6, 5)]
+        for c in corners:
+            for fl in cube_faces:
+                f = [c[x] for x in fl]
+                k = int(np.argmin(f))
+                f = f[k:] + f[:k]
+                faces.setdefault(frozenset(f), (len(faces), f))
+        nf = len(faces)
+    ne = len(eorder)
+    off_e, off_f = nv, nv + ne * (r - 1)
+    off_i = off_f + nf * (r - 1) ** 2
+    cc = ([(0, 0), (r, 0), (r, r), (0, r)] if dim == 2 else
+          [(0, 0, 0), (r, 0, 0), (r, r, 0), (0, r, 0), (0, 0, r), (r, 0, r), (r, r, r), (0, r, r)])
+    dof = -np.ones(n_nodes, np.int64)
+    for e, c in enumerate(corners):
+        ranges = [range(n)] * dim
+        for idx in itertools.product(*ranges):
+            q = sum(idx[a] * n ** a for a in range(dim))
+            bnd = [x in (0, r) for x in idx]
+            nb = sum(bnd)
+            if nb == dim:
+                d = c[cc.index(idx)]
+            elif nb == dim - 1:
+                ax = bnd.index(False)
+                s0 = list(idx); s0[ax] = 0
+                s1 = list(idx); s1[ax] = r
+                a, b, t = c[cc.index(tuple(s0))], c[cc.index(tuple(s1))], idx[ax]
+                if a > b:
+                    a, b, t = b, a, r - t
+                d = off_e + edges[(a, b)] * (r - 1) + t - 1
+            elif dim == 3 and nb == 1:
+                ax = bnd.index(True)
+                pos = {c[v]: cc[v] for v in range(8) if cc[v][ax] == idx[ax]}
+                fnum, f = faces[frozenset(pos)]
+                p0 = np.array(pos[f[0]])
+                s = int((np.array(idx) - p0) @ ((np.array(pos[f[1]]) - p0) // r))
+                t = int((np.array(idx) - p0) @ ((np.array(pos[f[3]]) - p0) // r))
+                d = off_f + fnum * (r - 1) ** 2 + (s - 1) * (r - 1) + t - 1
+            else:
+                li = 0
+                for a in range(dim):
+                    li = li * (r - 1) + idx[a] - 1
+                d = off_i + e * (r - 1) ** dim + li
+            dof[elem_nodes[e][q]] = d
+    return dof
+
+
+def regular_boundary(dim: int, nx: int):
+    """Boundary edges (2-D) or faces (3-D) of FileMesh = REGULAR on an nx^dim
+    grid of elements, 0-based vertex ids (vertex a + (nx+1)(b + (nx+1)c)), in
+    the order the reference creates them: 2-D y = ymin and y = ymax edge pairs,
+    then x = xmax and x = xmin (Mesh2D.cxx:4856-4873); 3-D x = xmin / x = xmax
+    and y = ymin / y = ymax pairs row by row, then z = zmin / z = zmax
+    (Mesh3D.cxx:8700-8840)."""
+    v1 = nx + 1
+    vid = lambda a, b, c=0: a + v1 * (b + v1 * c)  # noqa: E731
+    out = []
+    if dim == 2:
+        for i in range(nx):
+            out += [[vid(i, 0), vid(i + 1, 0)], [vid(i, nx), vid(i + 1, nx)]]
+        for j in range(nx):
+            out += [[vid(nx, j), vid(nx, j + 1)], [vid(0, j), vid(0, j + 1)]]
+        return out
+    for b in range(nx):
+        for a in range(nx):
+            out += [[vid(0, a, b), vid(0, a + 1, b), vid(0, a + 1, b + 1), vid(0, a, b + 1)],
+                    [vid(nx, a, b), vid(nx, a + 1, b), vid(nx, a + 1, b + 1), vid(nx, a, b + 1)],
+                    [vid(a, 0, b), vid(a + 1, 0, b), vid(a + 1, 0, b + 1), vid(a, 0, b + 1)],
+                    [vid(a, nx, b), vid(a + 1, nx, b), vid(a + 1, nx, b + 1), vid(a, nx, b + 1)]]
+    for b in range(nx):
+        for a in range(nx):
+            out += [[vid(a, b, 0), vid(a + 1, b, 0), vid(a + 1, b + 1, 0), vid(a, b + 1, 0)],
+                    [vid(a, b, nx), vid(a + 1, b, nx), vid(a + 1, b + 1, nx), vid(a, b + 1, nx)]]
+    return out
+
+
+def reprise_saves(nsteps: int, every: int):
+    """Checkpoints of SaveReprise = every over a run of nsteps steps: (nt, b)
+    for each nt = 0, every, 2 every, ... <= nsteps (GiveIterate and
+    GiveFinalIterate both reach SaveDataReprise), b alternating 0, 1, ...
+    (VarInstationary.cxx:917-972).  Each writes SolAcous<b> and
+    IterationAcousSaveNew = [nt, -1, b] after moving the previous New file to
+    IterationAcousSave."""
+    return [(nt, k % 2) for k, nt in enumerate(range(0, nsteps + 1, every))]
 
 
 def quadrature_points(m: "Mesh"):

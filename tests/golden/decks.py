@@ -15,27 +15,16 @@ def _f(x: float) -> str:
     return repr(float(x))
 
 
-def write_medit(path, m: O.Mesh, elem_ref, bnd_ref=1):
-    """Medit mesh of the structured (perturbed) mesh: vertices, hexahedra
-    (quadrilaterals in 2-D) with their media reference, boundary faces."""
-    dim, nx = m.dim, m.nx
+def medit_topology(dim, nx):
+    """Elements (Medit corner order) and boundary faces (edges in 2-D) of the
+    structured mesh, 0-based vertex ids, in the order write_medit lists them."""
     nv1 = nx + 1
-    V = m.vert.reshape(-1, dim)
-    lines = ["MeshVersionFormatted 1", "Dimension", str(dim), "Vertices", str(len(V))]
-    for v in V:
-        lines.append(" ".join(_f(c) for c in v) + " 0")
-    vid = lambda a, b, c=0: 1 + a + nv1 * (b + nv1 * c)  # noqa: E731
+    vid = lambda a, b, c=0: a + nv1 * (b + nv1 * c)  # noqa: E731
     if dim == 3:
-        lines += ["Hexahedra", str(nx ** 3)]
-        e = 0
-        for c in range(nx):
-            for b in range(nx):
-                for a in range(nx):
-                    cs = [vid(a, b, c), vid(a + 1, b, c), vid(a + 1, b + 1, c), vid(a, b + 1, c),
-                          vid(a, b, c + 1), vid(a + 1, b, c + 1), vid(a + 1, b + 1, c + 1),
-                          vid(a, b + 1, c + 1)]
-                    lines.append(" ".join(map(str, cs)) + f" {int(elem_ref[e])}")
-                    e += 1
+        corners = [[vid(a, b, c), vid(a + 1, b, c), vid(a + 1, b + 1, c), vid(a, b + 1, c),
+                    vid(a, b, c + 1), vid(a + 1, b, c + 1), vid(a + 1, b + 1, c + 1),
+                    vid(a, b + 1, c + 1)]
+                   for c in range(nx) for b in range(nx) for a in range(nx)]
         faces = []
         for i in range(nx):
             for j in range(nx):
@@ -45,22 +34,30 @@ def write_medit(path, m: O.Mesh, elem_ref, bnd_ref=1):
                 faces.append([vid(i, nx, j), vid(i, nx, j + 1), vid(i + 1, nx, j + 1), vid(i + 1, nx, j)])
                 faces.append([vid(0, i, j), vid(0, i, j + 1), vid(0, i + 1, j + 1), vid(0, i + 1, j)])
                 faces.append([vid(nx, i, j), vid(nx, i + 1, j), vid(nx, i + 1, j + 1), vid(nx, i, j + 1)])
-        lines += ["Quadrilaterals", str(len(faces))]
-        lines += [" ".join(map(str, f)) + f" {bnd_ref}" for f in faces]
     else:
-        lines += ["Quadrilaterals", str(nx * nx)]
-        e = 0
-        for b in range(nx):
-            for a in range(nx):
-                cs = [vid(a, b), vid(a + 1, b), vid(a + 1, b + 1), vid(a, b + 1)]
-                lines.append(" ".join(map(str, cs)) + f" {int(elem_ref[e])}")
-                e += 1
-        edges = []
+        corners = [[vid(a, b), vid(a + 1, b), vid(a + 1, b + 1), vid(a, b + 1)]
+                   for b in range(nx) for a in range(nx)]
+        faces = []
         for i in range(nx):
-            edges += [[vid(i, 0), vid(i + 1, 0)], [vid(nx, i), vid(nx, i + 1)],
+            faces += [[vid(i, 0), vid(i + 1, 0)], [vid(nx, i), vid(nx, i + 1)],
                       [vid(i + 1, nx), vid(i, nx)], [vid(0, i + 1), vid(0, i)]]
-        lines += ["Edges", str(len(edges))]
-        lines += [" ".join(map(str, f)) + f" {bnd_ref}" for f in edges]
+    return corners, faces
+
+
+def write_medit(path, m: O.Mesh, elem_ref, bnd_ref=1):
+    """Medit mesh of the structured (perturbed) mesh: vertices, hexahedra
+    (quadrilaterals in 2-D) with their media reference, boundary faces."""
+    dim, nx = m.dim, m.nx
+    V = m.vert.reshape(-1, dim)
+    lines = ["MeshVersionFormatted 1", "Dimension", str(dim), "Vertices", str(len(V))]
+    for v in V:
+        lines.append(" ".join(_f(c) for c in v) + " 0")
+    corners, faces = medit_topology(dim, nx)
+    lines += ["Hexahedra" if dim == 3 else "Quadrilaterals", str(len(corners))]
+    lines += [" ".join(str(v + 1) for v in cs) + f" {int(elem_ref[e])}"
+              for e, cs in enumerate(corners)]
+    lines += ["Quadrilaterals" if dim == 3 else "Edges", str(len(faces))]
+    lines += [" ".join(str(v + 1) for v in f) + f" {bnd_ref}" for f in faces]
     lines.append("End")
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
@@ -95,9 +92,10 @@ def write_don(path, nu):
 def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
-               mesh_media=None):
+               mesh_media=None, save_reprise=0):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  sinus_rho /
-    sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1."""
+    sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1.
+    save_reprise > 0: checkpoint files every save_reprise steps."""
     os.makedirs(d, exist_ok=True)
     m = O.mesh(dim, nx, r, pert, seed)
     nl = len(layers)
@@ -157,6 +155,8 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
         ini.append("FileOutputLine = diffracLine totalLine")
         ini.append("ParametersOutputLine = " + " ".join(_f(v) for v in line_times))
     ini.append("OutputFormat = BINARY DOUBLE")
+    if save_reprise:
+        ini.append(f"SaveReprise = {int(save_reprise)}")
     ini.append("ElectricOrMagnetic = " + ("-1" if elastic else "0"))
     ini.append(f"TimeStep = {_f(dt)}")
     ini.append(f"TimeInterval = 0.0 {_f(T)}")

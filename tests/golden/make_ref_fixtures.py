@@ -35,16 +35,17 @@ GOLD = os.path.dirname(os.path.abspath(__file__))
 def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
-             mesh_media=None):
+             mesh_media=None, save_reprise=0):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
                sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times,
-               line=line, line_times=line_times, mesh_media=mesh_media)
+               line=line, line_times=line_times, mesh_media=mesh_media,
+               save_reprise=save_reprise)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):
-        if f.startswith("total") or f.startswith("diffrac"):
+        if f.startswith(("total", "diffrac", "SolAcous", "IterationAcous")):
             os.remove(os.path.join(d, f))
     planes = np.zeros((0, 0), np.uint8)
     lines = np.zeros((0, 0), np.uint8)
@@ -62,6 +63,14 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         planes = snap_files("totalPlane")
     if line is not None:
         lines = snap_files("totalLine")
+    reprise = {}
+    if save_reprise:
+        # checkpoint files, raw bytes (SolAcous<b>_P0N1.dat, IterationAcousSave[New]0N1.dat)
+        for f in ("SolAcous0_P0N1.dat", "SolAcous1_P0N1.dat", "IterationAcousSaveNew0N1.dat",
+                  "IterationAcousSave0N1.dat"):
+            if os.path.exists(os.path.join(d, f)):
+                reprise["reprise_" + f.split(".")[0]] = np.frombuffer(
+                    open(os.path.join(d, f), "rb").read(), np.uint8)
     np.savez_compressed(
         os.path.join(GOLD, name + ".npz"), dim=dim, nx=nx, r=r, pert=pert, seed=seed,
         layers=np.asarray(layers, float), f0=f0, dt=dt, nsteps=nsteps, src=np.asarray(src, float),
@@ -78,7 +87,8 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         line_times=np.asarray(line_times if line_times is not None else [], float),
         line_files=lines,
         **{f"meshmedia_{nm}_{k}": v for nm, rm, off, amp, nu in (mesh_media or [])
-           for k, v in (("order", rm), ("offset", off), ("amplitude", amp), ("nu", nu))})
+           for k, v in (("order", rm), ("offset", off), ("amplitude", amp), ("nu", nu))},
+        save_reprise=save_reprise, **reprise)
     print(f"{name}: {traces.shape} rows, times {times[0]}..{times[-1]}")
 
 
@@ -230,6 +240,42 @@ def ref3d_q3_plane():
              rec, plane=[0.05, 0.1, 0.5, 1.2, 0.1, 0.55, 0.05, 0.95, 0.6, 6, 5],
              plane_times=[0.0, 0.6, 0.15], line=[0.1, 1.3, 0.2, 0.7, 0.3, 0.6, 9],
              line_times=[0.0, 0.75, 0.25])
+
+
+@case
+def ref2d_q4_reprise():
+    """8x8 Q4 perturbed quadrilaterals, two layers, SaveReprise every 40 of 160
+    steps: checkpoints at 0, 40, ..., 160 (the final iterate included)."""
+    circ = O.circle_points(0.5, 0.5, 0.3, 4)
+    run_case("ref2d_q4_reprise", 2, 8, 4, 0.1, 41, [1.0, 2.0], 10.0, 1e-3, 160, (0.47, 0.52),
+             circ, save_reprise=40)
+
+
+@case
+def ref3d_q3_reprise():
+    """4^3 Q3 perturbed hexahedra, two layers, SaveReprise every 50 of 120 steps
+    (last checkpoint at 100, before the final step)."""
+    rec = seeded_points(3, 4, 85)
+    run_case("ref3d_q3_reprise", 3, 4, 3, 0.1, 99, [1.0, 2.5], 8.0, 5e-3, 120, (0.43, 0.52, 0.47),
+             rec, save_reprise=50)
+
+
+@case
+def ref3d_q4_reprise_regular():
+    """3^3 Q4 REGULAR cube, SaveReprise every 30 of 90 steps."""
+    rec = seeded_points(3, 4, 86)
+    run_case("ref3d_q4_reprise_regular", 3, 3, 4, 0.0, 0, [1.0], 8.0, 5e-3, 90,
+             (0.43, 0.52, 0.47), rec, regular=True, save_reprise=30)
+
+
+@case
+def ref3d_el_q3_reprise():
+    """4^3 Q3 perturbed hexahedra, isotropic elastic, SaveReprise every 25 of
+    100 steps."""
+    rec = seeded_points(3, 4, 87)
+    run_case("ref3d_el_q3_reprise", 3, 4, 3, 0.1, 7, [1.0], 6.0, 4e-3, 100, (0.46, 0.53, 0.49),
+             rec, elastic=True, lam_layers=[2.0], rho_layers=[1.0], pol=(0.3, -0.5, 0.8),
+             save_reprise=25)
 
 
 if __name__ == "__main__":

@@ -35,7 +35,8 @@ def _write(c: CS.Case, d):
                      mesh_media=[(nm, int(z[f"meshmedia_{nm}_order"]),
                                   float(z[f"meshmedia_{nm}_offset"]),
                                   float(z[f"meshmedia_{nm}_amplitude"]), z[f"meshmedia_{nm}_nu"])
-                                 for nm in ("mu", "rho") if f"meshmedia_{nm}_nu" in z.files])
+                                 for nm in ("mu", "rho") if f"meshmedia_{nm}_nu" in z.files],
+                     save_reprise=int(z["save_reprise"]) if "save_reprise" in z.files else 0)
 
 
 def _dump(path):
@@ -51,11 +52,13 @@ def _dump(path):
         size = np.dtype(dt).itemsize * n
         out[name] = np.frombuffer(raw[off:off + size], dt)
         off += size
-    # media varying inside the elements (PhysicalMedia): rho, mu per quadrature point
-    nq = (len(raw) - off) // 16
-    assert nq in (0, E * nd) and off + 16 * nq == len(raw)
+    # media varying inside the elements (PhysicalMedia): rho, mu per quadrature
+    # point; then Montjoie's DOF number of every node (int64)
+    nq = (len(raw) - off - 8 * N) // 16
+    assert nq in (0, E * nd) and off + 16 * nq + 8 * N == len(raw)
     out["rho_q"] = np.frombuffer(raw[off:off + 8 * nq], np.float64).reshape(-1, nd)
-    out["mu_q"] = np.frombuffer(raw[off + 8 * nq:], np.float64).reshape(-1, nd)
+    out["mu_q"] = np.frombuffer(raw[off + 8 * nq:off + 16 * nq], np.float64).reshape(-1, nd)
+    out["mj_dof"] = np.frombuffer(raw[off + 16 * nq:], np.int64)
     out["xyz"] = out["xyz"].reshape(N, dim)
     out["en"] = out["en"].reshape(E, nd)
     return out
@@ -93,6 +96,13 @@ def test_driver_mesh_is_the_oracle_mesh(name, tmp_path):
         np.testing.assert_array_equal(g["rho"], c.rho_e)
     if c.elastic:
         np.testing.assert_array_equal(g["lam"], c.lam_e)
+    # Montjoie's DOF numbering (SaveReprise files) equals the oracle's
+    z = np.load(os.path.join(CS.GOLD, name + ".npz"))
+    corners, bnd = decks.medit_topology(c.dim, c.nx)
+    if bool(z["regular"]):
+        bnd = O.regular_boundary(c.dim, c.nx)
+    od = O.montjoie_dofs(c.dim, c.r, corners, bnd, oen, o.N)
+    assert np.array_equal(g["mj_dof"][perm], od)
 
 
 def test_driver_rejects_unsupported_decks(tmp_path):
@@ -213,3 +223,27 @@ def test_driver_snapshots_match_montjoie(name, tmp_path):
                 assert CS.rel_l2(v, vr) < 1e-10
             else:
                 assert not v.any()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", [n for n in CS.golden_names() if n.endswith(
+    ("_reprise", "_reprise_regular"))])
+def test_driver_checkpoints_match_montjoie(name, tmp_path):
+    """SaveReprise: the driver's IterationAcousSave[New]0N1.dat equal
+    Montjoie's byte for byte, its SolAcous<b>_P0N1.dat carry the same size and
+    the same iterates in Montjoie's DOF order, to rel-L2 1e-10."""
+    c = CS.load(name)
+    z = np.load(os.path.join(CS.GOLD, name + ".npz"))
+    _write(c, tmp_path)
+    print(f"deck {name} SaveReprise = {int(z['save_reprise'])}", flush=True)
+    r = subprocess.run([RUNNER, "case.ini"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    for f in ("IterationAcousSaveNew0N1", "IterationAcousSave0N1"):
+        assert (tmp_path / (f + ".dat")).read_bytes() == z["reprise_" + f].tobytes()
+    for b in (0, 1):
+        raw = (tmp_path / f"SolAcous{b}_P0N1.dat").read_bytes()
+        ref = z[f"reprise_SolAcous{b}_P0N1"].tobytes()
+        assert len(raw) == len(ref) and raw[:4] == ref[:4]
+        v, vr = np.frombuffer(raw[4:], np.float64), np.frombuffer(ref[4:], np.float64)
+        assert CS.rel_l2(v, vr) < 1e-10, CS.rel_l2(v, vr)

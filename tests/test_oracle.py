@@ -3,12 +3,16 @@ solutions (tests/golden, made from Montjoie's sources), and against what the
 mathematics fixes.  CPU only."""
 import json
 import os
+import sys
 
 import numpy as np
 import pytest
 
 import cases as CS
 from oracle import oracle as O
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+import decks  # noqa: E402
 
 GOLD = CS.GOLD
 
@@ -323,3 +327,60 @@ def test_oracle_snapshots_match_montjoie(name, kind):
         assert np.all(np.abs(ref[got == 0]) <= tiny) and np.all(np.abs(got[ref == 0]) <= tiny)
         assert CS.rel_l2(got, ref) < 1e-10
     assert outside > 0  # the grids reach outside the mesh
+
+
+def _montjoie_dofs(c: CS.Case, regular: bool):
+    """The oracle's Montjoie DOF number of each oracle node for a golden case."""
+    m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    corners, bnd = decks.medit_topology(c.dim, c.nx)
+    if regular:
+        bnd = O.regular_boundary(c.dim, c.nx)
+    return m, O.montjoie_dofs(c.dim, c.r, corners, bnd, m.elem_nodes.reshape(m.E, -1), m.N)
+
+
+@pytest.mark.parametrize("name", [n for n in CS.golden_names() if n.endswith(
+    ("_reprise", "_reprise_regular"))])
+def test_oracle_checkpoints_match_montjoie(name):
+    """SaveReprise files written by Montjoie: IterationAcousSave[New] hold the
+    last two checkpoints (int32 size 3, then nt, -1, b), and SolAcous<b> the
+    oracle's U^nt at those steps, put in Montjoie's DOF order by the oracle's
+    numbering (component-blocked for elastic), to rel-L2 1e-10."""
+    c = CS.load(name)
+    z = np.load(os.path.join(CS.GOLD, name + ".npz"))
+    saves = O.reprise_saves(c.nsteps, int(z["save_reprise"]))
+    assert len(saves) >= 2
+    ints = lambda k: list(np.frombuffer(z["reprise_" + k].tobytes(), np.int32))  # noqa: E731
+    assert ints("IterationAcousSaveNew0N1") == [3, saves[-1][0], -1, saves[-1][1]]
+    assert ints("IterationAcousSave0N1") == [3, saves[-2][0], -1, saves[-2][1]]
+    m, dof = _montjoie_dofs(c, bool(z["regular"]))
+    assert np.array_equal(np.sort(dof), np.arange(m.N))
+    ncomp = 3 if c.elastic else 1
+    for nt, b in saves[-2:]:
+        raw = z[f"reprise_SolAcous{b}_P0N1"].tobytes()
+        assert np.frombuffer(raw[:4], np.int32)[0] == ncomp * m.N
+        ref = np.frombuffer(raw[4:], np.float64)
+        uN, _, _ = CS.oracle_run(c, nt)
+        got = np.zeros(ncomp * m.N)
+        for k in range(ncomp):
+            got[k * m.N + dof] = uN[k * m.N:(k + 1) * m.N]
+        assert np.abs(ref).max() > 0
+        assert CS.rel_l2(got, ref) < 1e-10, (nt, CS.rel_l2(got, ref))
+
+
+def test_oracle_montjoie_dofs_tiny():
+    """The numbering on the smallest meshes, against hand-derived values: one
+    Q2 square (vertices, edges (0,1) (1,2) (3,2)->(2,3) (0,3), interior) and
+    one Q2 cube's face and interior nodes."""
+    en = np.arange(9).reshape(1, 9)
+    d = O.montjoie_dofs(2, 2, [[0, 1, 2, 3]], [], en, 9)
+    # local q = i + 3 j: corners 0 2 8 6 -> vertices 0 1 2 3; edges (0,1)=4,
+    # (1,2)=5, (2,3)=6 walked from 2 to 3 (i = 1 on the top edge is its middle),
+    # (3,0) = 7 stored as (0,3); interior 8
+    assert list(d) == [0, 4, 1, 7, 8, 5, 3, 6, 2]
+    en3 = np.arange(27).reshape(1, 27)
+    d3 = O.montjoie_dofs(3, 2, [[0, 1, 2, 3, 4, 5, 6, 7]], [], en3, 27)
+    # 8 vertices, 12 edges (8..19), 6 faces (20..25) in the cube's face order
+    # x=0, y=0, z=0, z=1, y=1, x=1, interior 26
+    assert d3[13] == 26
+    assert [d3[q] for q in (12, 10, 4, 22, 16, 14)] == [20, 21, 22, 23, 24, 25]
+    assert sorted(d3) == list(range(27))
