@@ -192,6 +192,18 @@ int mjw_ricker_pulse_table(double f0, double t_begin, double dt, int nsteps, dou
  * :2721-2761).  Traces are recorded for every shot.
  */
 int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts, const int32_t *comp);
+/*
+ * Receivers of values or first derivatives: deriv[r] = -1 records u(x_r),
+ * deriv[r] = a records d u / d x_a (x_r) = sum_i (DF^{-T} grad phi_i)_a u_i in
+ * the first element holding x_r.  The elastic SismoPoint output interpolates
+ * grad u at the point and writes the stress C grad u
+ * (Output/GridInterpolation.cxx :836-850 with compute_grad;
+ * Elliptic/Elastic/VarElastic.cxx:257-282).  elem (may be NULL) receives the
+ * element of each receiver, in the caller's element numbering, so the caller
+ * can apply that element's medium.
+ */
+int mjw_set_receivers_grad(mjw_problem *p, int nrec, const double *pts, const int32_t *comp,
+                           const int32_t *deriv, int64_t *elem);
 
 /* ---------------------------------------------------------------- leapfrog */
 /*

@@ -777,14 +777,22 @@ static bool in_bbox(const mjw_problem *p, int64_t e, const double *x) {
   return true;
 }
 
-static bool locate_in(const mjw_problem *p, int64_t e, const double *x, double *phi) {
+// deriv < 0: phi = basis values at x; deriv = a: d phi / d x_a
+static bool locate_in(const mjw_problem *p, int64_t e, const double *x, double *phi,
+                      int deriv = -1) {
   std::vector<double> xq((size_t)p->nd * p->dim);
   for (int q = 0; q < p->nd; ++q)
     for (int a = 0; a < p->dim; ++a)
       xq[(size_t)q * p->dim + a] = p->xyz[(int64_t)p->en[e * p->nd + q] * p->dim + a];
   double xi[3];
   if (!locate_point(p->dim, p->order, xq.data(), x, xi)) return false;
-  tensor_basis(p->dim, p->order, xi, phi);
+  if (deriv < 0) {
+    tensor_basis(p->dim, p->order, xi, phi);
+  } else {
+    std::vector<double> g((size_t)p->dim * p->nd);
+    tensor_grad(p->dim, p->order, xq.data(), xi, g.data());
+    std::copy(g.begin() + (size_t)deriv * p->nd, g.begin() + (size_t)(deriv + 1) * p->nd, phi);
+  }
   return true;
 }
 
@@ -933,8 +941,8 @@ extern "C" int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table) {
   return MJW_OK;
 }
 
-extern "C" int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts,
-                                 const int32_t *comp) {
+static int set_receivers(mjw_problem *p, int nrec, const double *pts, const int32_t *comp,
+                         const int32_t *deriv, int64_t *elem) {
   if (!p || nrec < 0 || (nrec > 0 && !pts)) FAIL(p, MJW_ERR_ARG, "mjw_set_receivers: bad args");
   HIPCHK(p, hipSetDevice(p->device));
   HIPCHK(p, hipStreamSynchronize(p->stream));
@@ -944,13 +952,16 @@ extern "C" int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts,
     const double *x = pts + (size_t)r * p->dim;
     cm[r] = comp ? comp[r] : 0;
     if (cm[r] < 0 || cm[r] >= p->ncomp) FAIL(p, MJW_ERR_ARG, "receiver component out of range");
+    const int dv = deriv ? deriv[r] : -1;
+    if (dv < -1 || dv >= p->dim) FAIL(p, MJW_ERR_ARG, "receiver derivative out of range");
     int64_t found = -1;
     const auto cand = candidates(p, x);
     for (const int32_t *ce = cand.first; ce != cand.second && found < 0; ++ce) {
       if (!in_bbox(p, *ce, x)) continue;
-      if (locate_in(p, *ce, x, &w[(size_t)r * p->nd])) found = *ce;
+      if (locate_in(p, *ce, x, &w[(size_t)r * p->nd], dv)) found = *ce;
     }
     if (found < 0) FAIL(p, MJW_ERR_ARG, "receiver " + std::to_string(r) + " outside the mesh");
+    if (elem) elem[r] = found;
     for (int q = 0; q < p->nd; ++q)
       gid[(size_t)r * p->nd + q] = p->nperm[p->en[found * p->nd + q]];
   }
@@ -1005,6 +1016,16 @@ extern "C" int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts,
   }
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   return MJW_OK;
+}
+
+extern "C" int mjw_set_receivers(mjw_problem *p, int nrec, const double *pts,
+                                 const int32_t *comp) {
+  return set_receivers(p, nrec, pts, comp, nullptr, nullptr);
+}
+
+extern "C" int mjw_set_receivers_grad(mjw_problem *p, int nrec, const double *pts,
+                                      const int32_t *comp, const int32_t *deriv, int64_t *elem) {
+  return set_receivers(p, nrec, pts, comp, deriv, elem);
 }
 
 // ---------------------------------------------------------------- leapfrog

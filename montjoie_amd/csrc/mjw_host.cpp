@@ -352,6 +352,49 @@ bool trilinear_coefficients(int order, int64_t E, const int32_t *elem_nodes, con
 // FiniteElement/PointsReference.cxx:277-360, and the projection of points found
 // slightly outside, DefineSourceElliptic.cxx:2539-2543).
 // ---------------------------------------------------------------------------
+// gradient weights at xi: g[a*nd + q] = d phi_q / d x_a = sum_b DF^{-1}[b][a]
+// d phi_q / d xi_b, DF = sum_q x_q (grad_xi phi_q)^T the Jacobian of the
+// element's nodal map (exact for the multilinear map of straight elements)
+void tensor_grad(int dim, int order, const double *xq, const double *xi, double *g) {
+  const int n = order + 1;
+  const int nd = dim == 2 ? n * n : n * n * n;
+  double xg[kMaxN + 1], wg[kMaxN + 1], ph[3][kMaxN + 1], dph[3][kMaxN + 1];
+  gl_rule(order, xg, wg);
+  for (int a = 0; a < dim; ++a) lagrange(order, xg, xi[a], ph[a], dph[a]);
+  std::vector<double> gr((size_t)nd * 3);
+  double J[9] = {0};
+  for (int q = 0; q < nd; ++q) {
+    const int ii[3] = {q % n, (q / n) % n, q / (n * n)};
+    for (int b = 0; b < dim; ++b) {
+      double t = dph[b][ii[b]];
+      for (int c = 0; c < dim; ++c)
+        if (c != b) t *= ph[c][ii[c]];
+      gr[(size_t)q * 3 + b] = t;
+    }
+    for (int a = 0; a < dim; ++a)
+      for (int b = 0; b < dim; ++b) J[a * dim + b] += gr[(size_t)q * 3 + b] * xq[q * dim + a];
+  }
+  double inv[9];
+  if (dim == 2) {
+    const double det = J[0] * J[3] - J[1] * J[2];
+    inv[0] = J[3] / det; inv[1] = -J[1] / det; inv[2] = -J[2] / det; inv[3] = J[0] / det;
+  } else {
+    const double c0 = J[4] * J[8] - J[5] * J[7], c1 = J[5] * J[6] - J[3] * J[8],
+                 c2 = J[3] * J[7] - J[4] * J[6];
+    const double det = J[0] * c0 + J[1] * c1 + J[2] * c2;
+    const double adj[9] = {c0, J[2] * J[7] - J[1] * J[8], J[1] * J[5] - J[2] * J[4],
+                           c1, J[0] * J[8] - J[2] * J[6], J[2] * J[3] - J[0] * J[5],
+                           c2, J[1] * J[6] - J[0] * J[7], J[0] * J[4] - J[1] * J[3]};
+    for (int k = 0; k < 9; ++k) inv[k] = adj[k] / det;
+  }
+  for (int q = 0; q < nd; ++q)
+    for (int a = 0; a < dim; ++a) {
+      double v = 0.0;
+      for (int b = 0; b < dim; ++b) v += inv[b * dim + a] * gr[(size_t)q * 3 + b];
+      g[(size_t)a * nd + q] = v;
+    }
+}
+
 void tensor_basis(int dim, int order, const double *xi, double *phi) {
   const int n = order + 1;
   double xg[kMaxN + 1], wg[kMaxN + 1], p[3][kMaxN + 1];

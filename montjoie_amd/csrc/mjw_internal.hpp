@@ -37,6 +37,8 @@ bool locate_point(int dim, int order, const double *xq /*nd*dim*/, const double 
                   double *xi);
 // tensor basis values phi_q(xi) (nd values)
 void tensor_basis(int dim, int order, const double *xi, double *phi);
+void tensor_grad(int dim, int order, const double *xq /*nd*dim*/, const double *xi,
+                 double *g /*dim*nd*/);
 
 // ------------------------------------------------------------ device launch
 enum StateType { kF64 = 0, kF32 = 1 };

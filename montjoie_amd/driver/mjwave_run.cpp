@@ -29,8 +29,9 @@
 // the run with a message, as Montjoie's own parser does (cout + abort).
 //
 // Elastic decks: Montjoie's rows hold the displacement of every point (3
-// values per point, point-major) followed by 9 further fields per point;
-// this driver writes the displacement part.
+// values per point, point-major) followed by the stress tensor of every point
+// (9 values, row-major; DisplayStress = YES, its default); this driver writes
+// both.
 //
 // --dump-mesh writes the assembled mesh (sizes, node positions, element ->
 // node map in local order q = i + n j + n^2 k, Dirichlet flags, media) as raw
@@ -1045,7 +1046,7 @@ int main(int argc, char **argv) {
       "Frequency", "TemporalSource", "TimeStep", "TimeInterval", "OrderTimeScheme", "SismoPoint",
       "SismoCircle", "FileOutputPoint", "ParametersOutputPoint", "SismoPlane",
       "FileOutputPlane", "ParametersOutputPlane", "SismoLine", "FileOutputLine",
-      "ParametersOutputLine", "OutputFormat", "SaveReprise", "PathReprise", "LoadReprise",
+      "ParametersOutputLine", "OutputFormat", "SaveReprise", "PathReprise", "LoadReprise", "DisplayStress",
       "ElectricOrMagnetic", "PrintLevel"};
   for (const auto &kv : d.kv)
     if (!known.count(kv.first))
@@ -1069,6 +1070,8 @@ int main(int argc, char **argv) {
     if ((*s)[0] != "SRC_DIRAC") die("TypeSource " + (*s)[0] + ": only SRC_DIRAC");
   if (const auto *s = d.get("TemporalSource"))
     if ((*s)[0] != "RICKER") die("TemporalSource " + (*s)[0] + ": only RICKER");
+  if (const auto *s = d.get("DisplayStress"))
+    if (!s->empty() && (*s)[0] == "NO") die("DisplayStress = NO: only the stress output is written");
   if (const auto *s = d.get("LoadReprise"))
     if (!s->empty() && (*s)[0] == "YES") die("LoadReprise: restarting is not implemented");
   // SaveReprise = n: checkpoint every n steps (VarInstationary.cxx:321-333)
@@ -1185,15 +1188,31 @@ int main(int argc, char **argv) {
       ++npt;
     }
   }
+  // receivers: every component of u at every point (component-major); for
+  // elastic decks also d u_i / d x_a at every point (point-major, i outer),
+  // from which the stress columns are formed
   std::vector<double> rpts;
-  std::vector<int32_t> comp;
+  std::vector<int32_t> comp, deriv;
   for (int m = 0; m < ncomp; ++m)
     for (int r = 0; r < npt; ++r) {
       for (int a = 0; a < dim; ++a) rpts.push_back(pts[(size_t)r * dim + a]);
       comp.push_back(m);
+      deriv.push_back(-1);
     }
-  const int nrec = npt * ncomp;
-  if (nrec) check(mjw_set_receivers(p, nrec, rpts.data(), comp.data()), p, "receivers");
+  if (elastic)
+    for (int r = 0; r < npt; ++r)
+      for (int m = 0; m < 3; ++m)
+        for (int a = 0; a < 3; ++a) {
+          for (int b = 0; b < dim; ++b) rpts.push_back(pts[(size_t)r * dim + b]);
+          comp.push_back(m);
+          deriv.push_back(a);
+        }
+  const int nrec = (int)comp.size();
+  std::vector<int64_t> rec_elem(nrec, -1);
+  if (nrec)
+    check(mjw_set_receivers_grad(p, nrec, rpts.data(), comp.data(), deriv.data(),
+                                 rec_elem.data()),
+          p, "receivers");
   const double t_setup =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
 
@@ -1330,6 +1349,20 @@ int main(int argc, char **argv) {
       for (int r = 0; r < npt; ++r)
         for (int m = 0; m < ncomp; ++m)
           std::fprintf(f, "\t%.15e", tr[(size_t)s * nrec + (size_t)m * npt + r]);
+      // elastic: then the stress C grad u of each point, 9 values row-major
+      // (sigma_ij = lambda tr(grad u) delta_ij + mu (du_i/dx_j + du_j/dx_i),
+      // ModifyOutputUnknown, Elliptic/Elastic/VarElastic.cxx:257-282), with
+      // the medium of the element holding the point
+      if (elastic)
+        for (int r = 0; r < npt; ++r) {
+          const double *g = &tr[(size_t)s * nrec + (size_t)3 * npt + (size_t)r * 9];
+          const int64_t e = rec_elem[(size_t)3 * npt + (size_t)r * 9];
+          const double lam = md.lam[e], mu = md.mu[e], div = g[0] + g[4] + g[8];
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+              std::fprintf(f, "\t%.15e",
+                           (i == j ? lam * div : 0.0) + mu * (g[i * 3 + j] + g[j * 3 + i]));
+        }
       std::fprintf(f, "\n");
     }
     std::fclose(f);

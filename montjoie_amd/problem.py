@@ -212,12 +212,19 @@ class WaveProblem:
         t = np.ascontiguousarray(table, np.float64)
         self._chk(_lib.lib().mjw_set_pulse(self._h, len(t), dptr(t)), "set_pulse")
 
-    def set_receivers(self, pts, comp=None):
+    def set_receivers(self, pts, comp=None, deriv=None):
+        """Receivers of u (deriv None or -1) or of d u / d x_deriv at pts on
+        component comp; returns the element holding each point."""
         pts = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, self.mesh.dim))
         cm = None if comp is None else np.ascontiguousarray(np.asarray(comp, np.int32))
-        self._chk(_lib.lib().mjw_set_receivers(self._h, pts.shape[0], dptr(pts), iptr(cm)),
-                  "set_receivers")
+        dv = None if deriv is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(deriv, np.int32), (pts.shape[0],)))
+        elem = np.zeros(pts.shape[0], np.int64)
+        self._chk(_lib.lib().mjw_set_receivers_grad(
+            self._h, pts.shape[0], dptr(pts), iptr(cm), iptr(dv),
+            elem.ctypes.data_as(_lib.C.POINTER(_lib.C.c_int64))), "set_receivers")
         self.nrec = pts.shape[0]
+        return elem
 
     # ------------------------------------------------------------ leapfrog
     def leapfrog_init(self, dt: float, t_begin: float = 0.0, u0=None, du0=None):

@@ -826,6 +826,74 @@ long or_receiver(int dim, int nx, int r, const double *vert, const double *p, do
   return -1;
 }
 
+/* derivative of the Lagrange basis function i of the points x at t */
+static double or_dphi(int r, const double *x, int i, double t) {
+  double c[OR_MAXN1D];
+  cte_phi(r, x, c);
+  double s = 0.0;
+  for (int l = 0; l <= r; l++) {
+    if (l == i) continue;
+    double p = 1.0;
+    for (int k = 0; k <= r; k++)
+      if (k != i && k != l) p *= t - x[k];
+    s += p;
+  }
+  return s * c[i];
+}
+
+/*
+ * Receiver value and gradient weights at p (the elastic SismoPoint output
+ * interpolates u and grad u at the point, Output/GridInterpolation.cxx
+ * :836-850 with compute_grad): wq[q] = phi_q(xi), wg[a*nd + q] =
+ * d phi_q / d x_a = sum_b DF^{-1}[b][a] d phi_q / d xi_b with DF the Jacobian
+ * of the element's multilinear map at xi.  Returns the element or -1.
+ */
+long or_receiver_grad(int dim, int nx, int r, const double *vert, const double *p, double *wq,
+                      double *wg) {
+  long E = (dim == 2) ? (long)nx * nx : (long)nx * nx * nx;
+  int n = r + 1, nd = (dim == 2) ? n * n : n * n * n;
+  double x[OR_MAXN1D], w[OR_MAXN1D];
+  or_gl(r, x, w);
+  for (long e = 0; e < E; e++) {
+    double xi[3];
+    if (!locate_in_elem(dim, nx, vert, e, p, xi)) continue;
+    tensor_phi(dim, r, x, xi, wq);
+    long cv[8];
+    elem_corners(dim, nx, e, cv);
+    double N8[8], dN[24], DF[9] = {0}, Di[9];
+    multilinear(dim, xi, N8, dN);
+    for (int a = 0; a < dim; a++)
+      for (int b = 0; b < dim; b++) {
+        double d = 0.0;
+        for (int c = 0; c < (1 << dim); c++) d += dN[c * dim + b] * vert[cv[c] * dim + a];
+        DF[a * dim + b] = d;
+      }
+    det_inv(dim, DF, Di);
+    double v1[3][OR_MAXN1D], d1[3][OR_MAXN1D];
+    for (int a = 0; a < dim; a++)
+      for (int i = 0; i < n; i++) {
+        v1[a][i] = or_phi(r, x, i, xi[a]);
+        d1[a][i] = or_dphi(r, x, i, xi[a]);
+      }
+    for (int q = 0; q < nd; q++) {
+      int idx[3] = {q % n, (q / n) % n, q / (n * n)};
+      double gref[3];
+      for (int b = 0; b < dim; b++) {
+        double v = 1.0;
+        for (int a = 0; a < dim; a++) v *= (a == b) ? d1[a][idx[a]] : v1[a][idx[a]];
+        gref[b] = v;
+      }
+      for (int a = 0; a < dim; a++) {
+        double g = 0.0;
+        for (int b = 0; b < dim; b++) g += Di[b * dim + a] * gref[b];
+        wg[a * nd + q] = g;
+      }
+    }
+    return e;
+  }
+  return -1;
+}
+
 /* SismoCircle points (GridInterpolation.cxx:1966-1983 bounds, :2721-2740) */
 void or_circle_points(double cx, double cy, double radius, int npts, double *pts) {
   const double pi = 3.14159265358979323846;

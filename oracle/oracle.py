@@ -71,6 +71,8 @@ def lib():
     L.or_dirac.restype = C.c_int
     L.or_receiver.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
     L.or_receiver.restype = C.c_long
+    L.or_receiver_grad.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+    L.or_receiver_grad.restype = C.c_long
     L.or_circle_points.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp]
     L.or_leapfrog.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, _ip,
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -576,6 +578,22 @@ def dirac(m: Mesh, p, pol=(1.0,)):
     return b
 
 
+def receiver_gradients(m: Mesh, pts):
+    """(elements, value weights [npts, nd], gradient weights [npts, dim, nd]):
+    u(p) = w . u_e and d u / d x_a (p) = g[a] . u_e on the first element
+    containing each point (-1 outside)."""
+    pts = np.asarray(pts, float).reshape(-1, m.dim)
+    el = np.full(len(pts), -1, np.int64)
+    W = np.zeros((len(pts), m.nd))
+    Gw = np.zeros((len(pts), m.dim, m.nd))
+    for k, p in enumerate(pts):
+        w = np.zeros(m.nd)
+        g = np.zeros(m.dim * m.nd)
+        el[k] = lib().or_receiver_grad(m.dim, m.nx, m.r, m.vert, np.ascontiguousarray(p), w, g)
+        W[k], Gw[k] = w, g.reshape(m.dim, m.nd)
+    return el, W, Gw
+
+
 def receivers(m: Mesh, pts):
     pts = np.asarray(pts, np.float64).reshape(-1, m.dim)
     elems = np.zeros(len(pts), dtype=np.int64)
@@ -692,12 +710,18 @@ def lambda_max(P: Problem, m_iter: int = 120, seed: int = 7) -> float:
                                m.dirichlet, m_iter, seed)
 
 
-def leapfrog(P: Problem, b, pulse, dt, nsteps, rec_pts=None, rec_comp=None, u0=None, du0=None):
-    """Returns (U^N, U^{N+1}, traces[(N+1), nrec])."""
+def leapfrog(P: Problem, b, pulse, dt, nsteps, rec_pts=None, rec_comp=None, u0=None, du0=None,
+             rec_weights=None):
+    """Returns (U^N, U^{N+1}, traces[(N+1), nrec]).  rec_weights = (elements,
+    weights [nrec, nd]) replaces the value weights of rec_pts (e.g. gradient
+    weights from receiver_gradients)."""
     m = P.mesh
     c, nc = P.colors
     NT = m.N * P.ncomp
-    if rec_pts is not None and len(rec_pts):
+    if rec_weights is not None:
+        re = np.ascontiguousarray(rec_weights[0], np.int64)
+        rw = np.ascontiguousarray(rec_weights[1], np.float64)
+    elif rec_pts is not None and len(rec_pts):
         re, rw = receivers(m, rec_pts)
     else:
         re, rw = np.zeros(0, np.int64), np.zeros(0)

@@ -108,6 +108,33 @@ def oracle_run(c: Case, nsteps: int | None = None):
     return O.leapfrog(P, b, pulse, c.dt, nsteps, c.rec, rc)
 
 
+def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
+    """Stress traces of an elastic case from the oracle: d u_i / d x_a at each
+    receiver recorded by the leapfrog (gradient weights), then
+    sigma = lambda tr(grad u) I + mu (grad u + grad u^T) with the medium of
+    the element holding the point; rows n = 0..nsteps, 9 values per point
+    (row-major), point-major."""
+    nsteps = c.nsteps if nsteps is None else nsteps
+    m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    P = O.elastic_problem(m, c.lam_e, c.mu_e, c.rho_e)
+    b = O.dirac(m, c.src, c.pol)
+    el, _, Gw = O.receiver_gradients(m, c.rec)
+    assert (el >= 0).all()
+    npt = len(c.rec)
+    # receiver (point r, component i, direction a), point-major
+    re = np.repeat(el, 9)
+    rw = np.stack([Gw[r, a] for r in range(npt) for i in range(3) for a in range(3)])
+    comp = np.array([i for r in range(npt) for i in range(3) for a in range(3)], np.int32)
+    pulse = O.pulse_table(c.f0, 0.0, c.dt, nsteps)
+    _, _, g = O.leapfrog(P, b, pulse, c.dt, nsteps, rec_comp=comp, rec_weights=(re, rw))
+    g = g.reshape(nsteps + 1, npt, 3, 3)
+    lam = c.lam_e[el][None, :, None, None]
+    mu = c.mu_e[el][None, :, None, None]
+    div = np.trace(g, axis1=2, axis2=3)[:, :, None, None]
+    sig = lam * div * np.eye(3) + mu * (g + g.transpose(0, 1, 3, 2))
+    return sig.reshape(nsteps + 1, npt * 9)
+
+
 def reference_traces(c: Case) -> np.ndarray:
     """Montjoie's traces re-ordered like oracle_run's (rows n = 0..nsteps)."""
     tr = c.traces

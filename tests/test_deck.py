@@ -132,8 +132,9 @@ def test_driver_rejects_media_on_another_mesh(tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", CS.golden_names())
 def test_driver_seismograms_match_montjoie(name, tmp_path):
-    """FP64 run of the deck: the written totalPoint.dat against Montjoie's
-    (rel-L2 <= 1e-10, the requirement; times exact)."""
+    """FP64 run of the deck: the written totalPoint.dat against Montjoie's,
+    every column (elastic: displacement and stress; rel-L2 <= 1e-10, the
+    requirement; times exact)."""
     c = CS.load(name)
     _write(c, tmp_path)
     print(f"deck {name}", flush=True)
@@ -142,9 +143,15 @@ def test_driver_seismograms_match_montjoie(name, tmp_path):
     assert r.returncode == 0, r.stderr + r.stdout
     out = np.loadtxt(tmp_path / "totalPoint.dat", ndmin=2)
     np.testing.assert_allclose(out[:, 0], c.times, rtol=1e-15, atol=1e-18)
-    ref = c.traces[:, :out.shape[1] - 1]
-    assert out.shape[0] == ref.shape[0]
-    assert CS.rel_l2(out[:, 1:], ref) < 1e-10
+    ref = c.traces
+    assert out.shape == (ref.shape[0], ref.shape[1] + 1)
+    if c.elastic:
+        # displacement columns, then the stress tensor of each point
+        n3 = 3 * len(c.rec)
+        assert CS.rel_l2(out[:, 1:n3 + 1], ref[:, :n3]) < 1e-10
+        assert CS.rel_l2(out[:, n3 + 1:], ref[:, n3:]) < 1e-10
+    else:
+        assert CS.rel_l2(out[:, 1:], ref) < 1e-10
 
 
 def _dense_lambda_max(c: CS.Case) -> float:

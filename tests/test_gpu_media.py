@@ -116,3 +116,57 @@ def test_probes_match_oracle_interpolation(dim, prec, nshots):
         ref = O.snapshot(o, u0[s], pts)
         assert np.array_equal(got[s, :, 0] == 0, ref == 0)
         assert CS.rel_l2(got[s, :, 0], ref) < tol
+
+
+@pytest.mark.parametrize("name", [n for n in CS.golden_names() if "_el_" in n])
+def test_gradient_receivers_give_montjoie_stress(name):
+    """mjw_set_receivers_grad on an elastic golden case: d u_i / d x_a at each
+    receiver over the whole FP64 run, turned into lambda tr(grad u) I +
+    mu (grad u + grad u^T) with the medium of the reported element, against
+    the stress columns of Montjoie's SismoPoint rows (rel-L2 1e-10)."""
+    c = CS.load(name)
+    P = CS.gpu_problem(c, "fp64")
+    npt = len(c.rec)
+    pts = np.repeat(c.rec, 9, axis=0)
+    comp = np.tile(np.repeat(np.arange(3), 3), npt)
+    deriv = np.tile(np.arange(3), 3 * npt)
+    elem = P.set_receivers(pts, comp, deriv)
+    o = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    el, _, _ = O.receiver_gradients(o, c.rec)
+    assert np.array_equal(elem[::9], el)
+    P.set_pulse(M.ricker_pulse_table(c.f0, 0.0, c.dt, c.nsteps))
+    P.leapfrog_init(c.dt)
+    g = P.run(c.nsteps)[:, 0, :].reshape(c.nsteps + 1, npt, 3, 3)
+    lam = c.lam_e[el][None, :, None, None]
+    mu = c.mu_e[el][None, :, None, None]
+    div = np.trace(g, axis1=2, axis2=3)[:, :, None, None]
+    sig = (lam * div * np.eye(3) + mu * (g + g.transpose(0, 1, 3, 2))).reshape(c.nsteps + 1, -1)
+    err = CS.rel_l2(sig, c.traces[:, 3 * npt:])
+    print(f"{name}: stress rel-L2 vs Montjoie {err:.2e}")
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gradient_receivers_match_oracle(dim):
+    """Acoustic gradient receivers on a perturbed mesh: the recorded
+    d u / d x_a equals the oracle's gradient weights applied to the same
+    initial state (FP64 1e-13), and a receiver outside the mesh is refused."""
+    m = M.regular_mesh(dim, 3, 4, 0.1, seed=3)
+    o = O.mesh(dim, 3, 4, 0.1, 3)
+    P = M.WaveProblem(m, 1.0, 1.0 + np.arange(m.E) % 2)
+    rng = np.random.default_rng(7 + dim)
+    pts = rng.uniform(0.05, 0.95, size=(6, dim))
+    rp = np.repeat(pts, dim, axis=0)
+    deriv = np.tile(np.arange(dim), len(pts))
+    elem = P.set_receivers(rp, None, deriv)
+    el, _, Gw = O.receiver_gradients(o, pts)
+    assert np.array_equal(elem[::dim], el)
+    u0 = rng.standard_normal(m.N)
+    u0[o.dirichlet.astype(bool)] = 0.0
+    P.leapfrog_init(1e-3, u0=u0)
+    got = P.record()[0]
+    en = o.elem_nodes.reshape(o.E, -1)
+    want = np.array([Gw[k, a] @ u0[en[el[k]]] for k in range(len(pts)) for a in range(dim)])
+    assert CS.rel_l2(got, want) < 1e-13
+    with pytest.raises(MjwError):
+        P.set_receivers(np.full((1, dim), 1.5), None, [0])

@@ -384,3 +384,36 @@ def test_oracle_montjoie_dofs_tiny():
     assert d3[13] == 26
     assert [d3[q] for q in (12, 10, 4, 22, 16, 14)] == [20, 21, 22, 23, 24, 25]
     assert sorted(d3) == list(range(27))
+
+
+@pytest.mark.parametrize("name", [n for n in CS.golden_names() if "_el_" in n])
+def test_oracle_elastic_stress_matches_montjoie(name):
+    """Elastic SismoPoint rows end with the stress tensor of each point (9
+    values, row-major): the oracle's gradient receivers turned into
+    lambda tr(grad u) I + mu (grad u + grad u^T) match them to rel-L2 1e-10
+    on every row."""
+    c = CS.load(name)
+    npt = len(c.rec)
+    ref = c.traces[:, 3 * npt:]
+    assert ref.shape[1] == 9 * npt
+    got = CS.oracle_stress(c)
+    assert got.shape == ref.shape
+    assert np.abs(ref).max() > 0
+    assert CS.rel_l2(got, ref) < 1e-10, CS.rel_l2(got, ref)
+
+
+def test_oracle_gradient_weights_exact_on_linear_fields():
+    """Gradient weights reproduce the gradient of a linear field exactly in a
+    perturbed element (straight-sided hexahedra contain the linear fields)
+    and sum to zero."""
+    m = O.mesh(3, 3, 4, 0.15, 5)
+    pts = np.array([[0.31, 0.52, 0.47], [0.77, 0.12, 0.66]])
+    el, W, Gw = O.receiver_gradients(m, pts)
+    X = m.xyz.reshape(m.N, 3)
+    en = m.elem_nodes.reshape(m.E, -1)
+    g = np.array([0.3, -1.2, 2.5])
+    for k in range(len(pts)):
+        u = X[en[el[k]]] @ g + 0.7
+        np.testing.assert_allclose(Gw[k] @ u, g, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(Gw[k].sum(axis=1), 0.0, atol=1e-12)
+        np.testing.assert_allclose(W[k] @ u, pts[k] @ g + 0.7, rtol=0, atol=1e-12)
