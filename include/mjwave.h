@@ -234,6 +234,13 @@ int mjw_record(mjw_problem *p, double *traces);
  * mjw_probe writes values[(s*npts + i)*ncomp + m] in FP64.
  */
 int mjw_set_probes(mjw_problem *p, int npts, const double *pts, uint8_t *inside);
+/* probes of values (deriv[i] = -1) or first derivatives (deriv[i] = a:
+ * d u / d x_a, as mjw_set_receivers_grad) for the elastic snapshot files,
+ * which hold each displacement component and each stress component
+ * (<name>%04d_U<m>.dat, <name>%04d_dU<k>.dat); elem (may be NULL) receives
+ * the element holding each point, -1 outside the mesh. */
+int mjw_set_probes_grad(mjw_problem *p, int npts, const double *pts, const int32_t *deriv,
+                        uint8_t *inside, int64_t *elem);
 int mjw_probe(mjw_problem *p, double *values);
 /* current pair (U^n, U^{n+1}) as FP64 host vectors [nshots][n_nodes*ncomp];
  * either may be NULL (GetIterate / GetIterate(k)). */

@@ -1201,7 +1201,8 @@ extern "C" int mjw_record(mjw_problem *p, double *traces) {
   return MJW_OK;
 }
 
-extern "C" int mjw_set_probes(mjw_problem *p, int npts, const double *pts, uint8_t *inside) {
+static int set_probes(mjw_problem *p, int npts, const double *pts, const int32_t *deriv,
+                      uint8_t *inside, int64_t *elem) {
   if (!p || npts < 0 || (npts > 0 && !pts)) FAIL(p, MJW_ERR_ARG, "mjw_set_probes: bad args");
   HIPCHK(p, hipSetDevice(p->device));
   HIPCHK(p, hipStreamSynchronize(p->stream));
@@ -1209,12 +1210,15 @@ extern "C" int mjw_set_probes(mjw_problem *p, int npts, const double *pts, uint8
   std::vector<double> w((size_t)npts * p->nd, 0.0);
   for (int r = 0; r < npts; ++r) {
     const double *x = pts + (size_t)r * p->dim;
+    const int dv = deriv ? deriv[r] : -1;
+    if (dv < -1 || dv >= p->dim) FAIL(p, MJW_ERR_ARG, "probe derivative out of range");
     int64_t found = -1;
     const auto cand = candidates(p, x);
     for (const int32_t *ce = cand.first; ce != cand.second && found < 0; ++ce) {
       if (!in_bbox(p, *ce, x)) continue;
-      if (locate_in(p, *ce, x, &w[(size_t)r * p->nd])) found = *ce;
+      if (locate_in(p, *ce, x, &w[(size_t)r * p->nd], dv)) found = *ce;
     }
+    if (elem) elem[r] = found;
     if (found >= 0) {
       for (int q = 0; q < p->nd; ++q)
         gid[(size_t)r * p->nd + q] = p->nperm[p->en[found * p->nd + q]];
@@ -1233,6 +1237,15 @@ extern "C" int mjw_set_probes(mjw_problem *p, int npts, const double *pts, uint8
   HIPCHK(p, hipMemcpy(p->d_pr_gid, gid.data(), gid.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(p, hipMemcpy(p->d_pr_w, w.data(), w.size() * 8, hipMemcpyHostToDevice));
   return MJW_OK;
+}
+
+extern "C" int mjw_set_probes(mjw_problem *p, int npts, const double *pts, uint8_t *inside) {
+  return set_probes(p, npts, pts, nullptr, inside, nullptr);
+}
+
+extern "C" int mjw_set_probes_grad(mjw_problem *p, int npts, const double *pts,
+                                   const int32_t *deriv, uint8_t *inside, int64_t *elem) {
+  return set_probes(p, npts, pts, deriv, inside, elem);
 }
 
 extern "C" int mjw_probe(mjw_problem *p, double *values) {

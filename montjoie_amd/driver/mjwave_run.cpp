@@ -1227,12 +1227,12 @@ int main(int argc, char **argv) {
   };
   std::vector<Snap> snaps;
   std::vector<double> probe_pts;
+  std::vector<int32_t> probe_deriv;
   for (const char *kind : {"Plane", "Line"}) {
     const std::string key = std::string("Sismo") + kind;
     const auto lines = d.all(key);
     if (lines.empty()) continue;
     if (lines.size() > 1) die(key + ": one per deck");
-    if (elastic) die(key + ": snapshots are written for ACOUSTIC decks only");
     const auto *of = d.get("OutputFormat");
     if (!of || of->size() < 2 || (*of)[0] != "BINARY" || (*of)[1] != "DOUBLE")
       die(key + ": only OutputFormat = BINARY DOUBLE is written");
@@ -1250,25 +1250,55 @@ int main(int argc, char **argv) {
       if (std::llround((t - ps0) / dt) % every) continue;
       sn.steps.push_back(st);
     }
+    // elastic: the points again for d / d x_0, d / d x_1, d / d x_2 (stress)
     sn.off = probe_pts.size() / dim;
-    probe_pts.insert(probe_pts.end(), sn.g.pts.begin(), sn.g.pts.end());
+    for (int dv = -1; dv < (elastic ? 3 : 0); ++dv) {
+      probe_pts.insert(probe_pts.end(), sn.g.pts.begin(), sn.g.pts.end());
+      probe_deriv.insert(probe_deriv.end(), sn.g.pts.size() / dim, dv);
+    }
     snaps.push_back(sn);
   }
   std::set<int64_t> snap_steps;
   for (const auto &sn : snaps) snap_steps.insert(sn.steps.begin(), sn.steps.end());
+  std::vector<int64_t> probe_elem(probe_deriv.size(), -1);
   if (!probe_pts.empty())
-    check(mjw_set_probes(p, (int)(probe_pts.size() / dim), probe_pts.data(), nullptr), p,
-          "snapshot grid");
-  std::vector<double> pv(probe_pts.size() / dim);
+    check(mjw_set_probes_grad(p, (int)(probe_pts.size() / dim), probe_pts.data(),
+                              probe_deriv.data(), nullptr, probe_elem.data()),
+          p, "snapshot grid");
+  std::vector<double> pv(probe_deriv.size() * ncomp);
+  // acoustic: <name>%04d.dat; elastic (OutputFormat's multi-field layout):
+  // <name>%04d_U<m>.dat per displacement component and <name>%04d_dU<k>.dat
+  // per stress component sigma_ij, k = 3 i + j, with the medium of the
+  // element holding the point (0 outside the mesh)
   auto snapshot = [&](int64_t st) {
     check(mjw_probe(p, pv.data()), p, "snapshot grid");
     for (auto &sn : snaps) {
       if (!std::binary_search(sn.steps.begin(), sn.steps.end(), st)) continue;
       const size_t n = (size_t)sn.g.nx * sn.g.ny;
-      std::vector<double> vals(pv.begin() + sn.off, pv.begin() + sn.off + n);
-      char suf[32];
-      std::snprintf(suf, sizeof suf, "%04d.dat", sn.count++);
-      write_plane(sn.name + suf, sn.g, vals);
+      char num[16];
+      std::snprintf(num, sizeof num, "%04d", sn.count++);
+      std::vector<double> vals(n);
+      if (!elastic) {
+        for (size_t i = 0; i < n; ++i) vals[i] = pv[sn.off + i];
+        write_plane(sn.name + num + ".dat", sn.g, vals);
+        continue;
+      }
+      for (int m = 0; m < 3; ++m) {
+        for (size_t i = 0; i < n; ++i) vals[i] = pv[(sn.off + i) * 3 + m];
+        write_plane(sn.name + num + "_U" + std::to_string(m) + ".dat", sn.g, vals);
+      }
+      // gradient probes: sn.off + (1 + a) n + i holds d u_m / d x_a at point i
+      auto grad = [&](size_t i, int m, int a) { return pv[(sn.off + (1 + a) * n + i) * 3 + m]; };
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          for (size_t i = 0; i < n; ++i) {
+            const int64_t e = probe_elem[sn.off + n + i];
+            if (e < 0) { vals[i] = 0.0; continue; }
+            const double div = grad(i, 0, 0) + grad(i, 1, 1) + grad(i, 2, 2);
+            vals[i] = (a == b ? md.lam[e] * div : 0.0) + md.mu[e] * (grad(i, a, b) + grad(i, b, a));
+          }
+          write_plane(sn.name + num + "_dU" + std::to_string(3 * a + b) + ".dat", sn.g, vals);
+        }
     }
   };
 

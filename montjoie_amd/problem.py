@@ -247,13 +247,20 @@ class WaveProblem:
             self._chk(_lib.lib().mjw_record(self._h, dptr(tr)), "record")
         return tr
 
-    def set_probes(self, pts) -> np.ndarray:
-        """Points where probe() samples the current iterate; returns the
-        inside-the-mesh flags (mjw_set_probes)."""
+    def set_probes(self, pts, deriv=None) -> np.ndarray:
+        """Points where probe() samples the current iterate (deriv None or -1)
+        or its derivative d / d x_deriv; returns the inside-the-mesh flags
+        (mjw_set_probes, mjw_set_probes_grad)."""
         pts = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, self.mesh.dim))
         inside = np.zeros(pts.shape[0], np.uint8)
-        self._chk(_lib.lib().mjw_set_probes(self._h, pts.shape[0], dptr(pts),
-                                            inside.ctypes.data_as(C.c_void_p)), "set_probes")
+        if deriv is None:
+            self._chk(_lib.lib().mjw_set_probes(self._h, pts.shape[0], dptr(pts),
+                                                inside.ctypes.data_as(C.c_void_p)), "set_probes")
+        else:
+            dv = np.ascontiguousarray(np.broadcast_to(np.asarray(deriv, np.int32), (pts.shape[0],)))
+            self._chk(_lib.lib().mjw_set_probes_grad(
+                self._h, pts.shape[0], dptr(pts), iptr(dv),
+                inside.ctypes.data_as(C.POINTER(C.c_uint8)), None), "set_probes")
         self.nprobe = pts.shape[0]
         return inside.astype(bool)
 

@@ -660,6 +660,27 @@ def snapshot(m: Mesh, u, pts):
     return out
 
 
+def snapshot_elastic(m: Mesh, u, pts, lam_e, mu_e):
+    """Elastic snapshot fields at pts: [3, npts] displacement and [9, npts]
+    stress sigma_ij = lambda tr(grad u) delta_ij + mu (du_i/dx_j + du_j/dx_i)
+    (row-major, k = 3 i + j) with the medium of the first element holding each
+    point; 0 outside the mesh."""
+    el, W, Gw = receiver_gradients(m, pts)
+    npts = len(el)
+    disp = np.zeros((3, npts))
+    sig = np.zeros((9, npts))
+    en = m.elem_nodes.reshape(m.E, -1)
+    for k in range(npts):
+        if el[k] < 0:
+            continue
+        ue = np.stack([u[c * m.N + en[el[k]]] for c in range(3)])
+        disp[:, k] = ue @ W[k]
+        g = ue @ Gw[k].T  # g[i, a] = d u_i / d x_a
+        s = lam_e[el[k]] * np.trace(g) * np.eye(3) + mu_e[el[k]] * (g + g.T)
+        sig[:, k] = s.ravel()
+    return disp, sig
+
+
 @dataclass
 class Problem:
     """A structured acoustic (ncomp=1) or elastic (ncomp=3) problem."""

@@ -54,9 +54,13 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         raise RuntimeError(f"{name}: reference run failed\n{r_.stdout[-3000:]}")
     out = np.loadtxt(os.path.join(d, "totalPoint.dat"), ndmin=2)
     times, traces = out[:, 0], out[:, 1:]
+    names = {}
+
     def snap_files(prefix):
-        # snapshot files <prefix>0000.dat, ... (raw bytes, one row each)
+        # snapshot files <prefix>0000.dat, ... (elastic: <prefix>0000_U0.dat,
+        # ..., <prefix>0000_dU8.dat), sorted by name, raw bytes, one row each
         files = sorted(f for f in os.listdir(d) if f.startswith(prefix))
+        names[prefix] = np.array(files)
         return np.stack([np.frombuffer(open(os.path.join(d, f), "rb").read(), np.uint8)
                          for f in files])
     if plane is not None:
@@ -88,7 +92,9 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         line_files=lines,
         **{f"meshmedia_{nm}_{k}": v for nm, rm, off, amp, nu in (mesh_media or [])
            for k, v in (("order", rm), ("offset", off), ("amplitude", amp), ("nu", nu))},
-        save_reprise=save_reprise, **reprise)
+        save_reprise=save_reprise, **reprise,
+        plane_names=names.get("totalPlane", np.zeros(0, "U1")),
+        line_names=names.get("totalLine", np.zeros(0, "U1")))
     print(f"{name}: {traces.shape} rows, times {times[0]}..{times[-1]}")
 
 
@@ -276,6 +282,19 @@ def ref3d_el_q3_reprise():
     run_case("ref3d_el_q3_reprise", 3, 4, 3, 0.1, 7, [1.0], 6.0, 4e-3, 100, (0.46, 0.53, 0.49),
              rec, elastic=True, lam_layers=[2.0], rho_layers=[1.0], pol=(0.3, -0.5, 0.8),
              save_reprise=25)
+
+
+@case
+def ref3d_el_q3_plane():
+    """Elastic 4^3 Q3 perturbed hexahedra with a tilted SismoPlane partly
+    outside the cube every 30 steps and a SismoLine every 45 steps: one file
+    per displacement component and per stress component."""
+    rec = seeded_points(3, 3, 88)
+    run_case("ref3d_el_q3_plane", 3, 4, 3, 0.1, 11, [1.5], 6.0, 4e-3, 90, (0.44, 0.53, 0.5),
+             rec, elastic=True, lam_layers=[2.0], rho_layers=[1.2], pol=(0.6, 0.2, -0.7),
+             plane=[0.05, 0.1, 0.5, 1.2, 0.1, 0.55, 0.05, 0.95, 0.6, 5, 4],
+             plane_times=[0.0, 0.36, 0.12], line=[0.1, 1.3, 0.2, 0.7, 0.3, 0.6, 7],
+             line_times=[0.0, 0.36, 0.18])
 
 
 if __name__ == "__main__":

@@ -198,11 +198,13 @@ def test_driver_time_step_auto(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["ref2d_q4_plane", "ref3d_q3_plane"])
+@pytest.mark.parametrize("name", ["ref2d_q4_plane", "ref3d_q3_plane", "ref3d_el_q3_plane"])
 def test_driver_snapshots_match_montjoie(name, tmp_path):
     """SismoPlane and SismoLine: the driver's totalPlane%04d.dat and
-    totalLine%04d.dat files against Montjoie's, byte for byte in the header,
-    values to rel-L2 1e-10 with the same zeros (points outside the mesh)."""
+    totalLine%04d.dat files (elastic: one per displacement and stress
+    component, totalPlane%04d_U<m>.dat / _dU<k>.dat) against Montjoie's, the
+    same names, byte for byte in the header, values to rel-L2 1e-10 with the
+    same zeros (points outside the mesh)."""
     c = CS.load(name)
     z = np.load(os.path.join(CS.GOLD, name + ".npz"))
     _write(c, tmp_path)
@@ -216,8 +218,10 @@ def test_driver_snapshots_match_montjoie(name, tmp_path):
         assert len(files) == ref.shape[0]
         g = z[kind]
         npts = int(g[-1] * g[-2]) if kind == "plane" else int(g[-1])
+        names = (list(z[kind + "_names"]) if kind + "_names" in z.files
+                 else [f"{prefix}{k:04d}.dat" for k in range(len(files))])
         for k, f in enumerate(files):
-            assert f == f"{prefix}{k:04d}.dat"
+            assert f == names[k]
             raw = (tmp_path / f).read_bytes()
             assert len(raw) == ref.shape[1]
             h, v = CS.plane_file(raw, npts)

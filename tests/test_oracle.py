@@ -417,3 +417,36 @@ def test_oracle_gradient_weights_exact_on_linear_fields():
         np.testing.assert_allclose(Gw[k] @ u, g, rtol=0, atol=1e-12)
         np.testing.assert_allclose(Gw[k].sum(axis=1), 0.0, atol=1e-12)
         np.testing.assert_allclose(W[k] @ u, pts[k] @ g + 0.7, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["plane", "line"])
+def test_oracle_elastic_snapshots_match_montjoie(kind):
+    """Elastic SismoPlane / SismoLine files written by Montjoie: one file per
+    displacement component (_U<m>) and per stress component (_dU<k>) at each
+    snapshot step; the oracle's fields at the grid points match them to
+    rel-L2 1e-10 with the same zeros outside the mesh."""
+    name = "ref3d_el_q3_plane"
+    c = CS.load(name)
+    z = np.load(os.path.join(CS.GOLD, name + ".npz"))
+    pts = O.plane_points(3, z["plane"]) if kind == "plane" else O.line_points(3, z["line"])
+    gtype = 2 if kind == "plane" else 3
+    steps = CS.plane_steps(c, z[kind + "_times"])
+    files, names = z[kind + "_files"], list(z[kind + "_names"])
+    prefix = "totalPlane" if kind == "plane" else "totalLine"
+    suffixes = [f"_U{m}" for m in range(3)] + [f"_dU{k}" for k in range(9)]
+    assert names == [f"{prefix}{k:04d}{sf}.dat" for k in range(len(steps)) for sf in suffixes]
+    m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    for k, s in enumerate(steps):
+        uN, _, _ = CS.oracle_run(c, s)
+        disp, sig = O.snapshot_elastic(m, uN, pts, c.lam_e, c.mu_e)
+        fields = np.vstack([disp, sig])
+        for f in range(12):
+            head, ref = CS.plane_file(files[12 * k + f], len(pts))
+            assert list(np.frombuffer(head[:16], np.int32)) == [1, 3, 1, gtype]
+            got = fields[f]
+            if not ref.any():
+                assert not got.any()
+                continue
+            tiny = 1e-12 * np.abs(ref).max()
+            assert np.all(np.abs(ref[got == 0]) <= tiny) and np.all(np.abs(got[ref == 0]) <= tiny)
+            assert CS.rel_l2(got, ref) < 1e-10, (s, names[12 * k + f], CS.rel_l2(got, ref))
