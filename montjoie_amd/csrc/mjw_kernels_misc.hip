@@ -3,6 +3,7 @@
 // step), vector helpers and deterministic FP64 reductions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "mjw_internal.hpp"
@@ -228,11 +229,47 @@ int launch_mass_assemble(const int32_t *map, const double *Mloc, int64_t E, int 
 // ---------------------------------------------------------------------------
 constexpr int kTailThreads = 1024;
 
+// receiver values of the iterate U: one wavefront per (receiver, shot) pair
+// it = wave0, wave0 + stride, ..., lanes over the element's nodes, reduced in
+// a fixed order (the result does not depend on the launch shape)
+template <typename T>
+__device__ __forceinline__ void record_waves(const TailArgs &a, int ncomp, int64_t step,
+                                             int64_t base, int64_t wave0, int64_t stride) {
+  const T *U = static_cast<const T *>(a.U);
+  const int lane = threadIdx.x & 63;
+  const int64_t NT = (int64_t)ncomp * a.N;
+  const int64_t tot = (int64_t)a.nrec * a.nshots;
+  for (int64_t it = wave0; it < tot; it += stride) {
+    const int r = (int)(it % a.nrec), s = (int)(it / a.nrec);
+    const int64_t cbase = (int64_t)a.rec_comp[r] * a.N;
+    double acc = 0.0;
+    for (int q = lane; q < a.nd; q += 64) {
+      const int64_t d = cbase + a.rec_gid[(int64_t)r * a.nd + q];
+      acc += a.rec_w[(int64_t)r * a.nd + q] *
+             (double)U[a.ilv ? d * a.nshots + s : (int64_t)s * NT + d];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) a.traces[((step - base + a.row_offset) * a.nshots + s) * a.nrec + r] = acc;
+  }
+}
+
+// many receivers: the recording spread over the chip, before the one-block
+// tail (which then only injects the sources and advances the step counter;
+// it runs after this kernel on the same stream, so the step read here is
+// the one the tail advances)
+constexpr int kRecordThreads = 256;
+template <typename T>
+__global__ __launch_bounds__(kRecordThreads) void k_record(TailArgs a, int ncomp) {
+  const int64_t nw = kRecordThreads / 64;
+  record_waves<T>(a, ncomp, a.ctl[0], a.ctl[1], (int64_t)blockIdx.x * nw + (threadIdx.x >> 6),
+                  (int64_t)gridDim.x * nw);
+}
+
 template <typename T>
 __device__ __forceinline__ void tail_body(const TailArgs &a, int ncomp) {
   const int64_t step = a.ctl[0];
   const int64_t base = a.ctl[1];
-  const T *U = static_cast<const T *>(a.U);
   T *P = static_cast<T *>(a.P);
   const T *sd = static_cast<const T *>(a.sdiag);
   const double pulse = (a.pulse && step < a.npulse) ? a.pulse[step] : 0.0;
@@ -247,22 +284,8 @@ __device__ __forceinline__ void tail_body(const TailArgs &a, int ncomp) {
     }
   }
   if (a.traces) {
-    // one wavefront per (receiver, shot), lanes over the element's nodes
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int tot = a.nrec * a.nshots;
-    for (int it = wave; it < tot; it += nw) {
-      const int r = it % a.nrec, s = it / a.nrec;
-      const int64_t cbase = (int64_t)a.rec_comp[r] * a.N;
-      double acc = 0.0;
-      for (int q = lane; q < a.nd; q += 64) {
-        const int64_t d = cbase + a.rec_gid[(int64_t)r * a.nd + q];
-        acc += a.rec_w[(int64_t)r * a.nd + q] *
-               (double)U[a.ilv ? d * a.nshots + s : (int64_t)s * NT + d];
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-      if (lane == 0) a.traces[((step - base + a.row_offset) * a.nshots + s) * a.nrec + r] = acc;
-    }
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    record_waves<T>(a, ncomp, step, base, wave, nw);
   }
   if (a.nadj && P) {
     // adjoint source: receiver transpose of dJ/dtrace at this step, unique
@@ -301,10 +324,23 @@ int launch_tail_batch(int state, const TailArgs *args, int nprob, void *stream) 
 
 int launch_tail_n(const TailArgs &a, int ncomp, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
+  // more (receiver, shot) pairs than four per wave of the one-block tail:
+  // record them with a grid of their own first
+  const int64_t tot = (int64_t)a.nrec * a.nshots;
+  TailArgs t = a;
+  if (a.traces && tot > 4 * (kTailThreads / 64)) {
+    const int64_t per = kRecordThreads / 64;
+    const unsigned grid = (unsigned)std::min<int64_t>((tot + per - 1) / per, 2048);
+    if (a.state == kF64)
+      hipLaunchKernelGGL(k_record<double>, dim3(grid), dim3(kRecordThreads), 0, st, a, ncomp);
+    else
+      hipLaunchKernelGGL(k_record<float>, dim3(grid), dim3(kRecordThreads), 0, st, a, ncomp);
+    t.traces = nullptr;
+  }
   if (a.state == kF64)
-    hipLaunchKernelGGL(k_tail<double>, dim3(1), dim3(kTailThreads), 0, st, a, ncomp);
+    hipLaunchKernelGGL(k_tail<double>, dim3(1), dim3(kTailThreads), 0, st, t, ncomp);
   else
-    hipLaunchKernelGGL(k_tail<float>, dim3(1), dim3(kTailThreads), 0, st, a, ncomp);
+    hipLaunchKernelGGL(k_tail<float>, dim3(1), dim3(kTailThreads), 0, st, t, ncomp);
   return (int)hipGetLastError();
 }
 
