@@ -115,6 +115,22 @@ def test_driver_rejects_unsupported_decks(tmp_path):
     assert r.returncode != 0 and "LEAP_FROG" in r.stderr
 
 
+@pytest.mark.parametrize("line,word", [("LoadReprise = YES", "LoadReprise"),
+                                       ("DisplayStress = NO", "DisplayStress"),
+                                       ("SaveReprise = 0", "SaveReprise")])
+def test_driver_rejects_unsupported_outputs(tmp_path, line, word):
+    """Restarts, raw-gradient elastic output and a non-positive checkpoint
+    period stop the run with a message naming the keyword (before any GPU
+    call: --dump-mesh runs on the CPU)."""
+    c = CS.load("ref2d_q4_n8")
+    _write(c, tmp_path)
+    ini = (tmp_path / "case.ini").read_text()
+    (tmp_path / "bad.ini").write_text(ini + line + "\n")
+    r = subprocess.run([RUNNER, "bad.ini", "--dump-mesh", "x.bin"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and word in r.stderr
+
+
 def test_driver_rejects_media_on_another_mesh(tmp_path):
     c = CS.load("ref3d_q3_meshmedia")
     _write(c, tmp_path)
