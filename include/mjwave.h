@@ -237,8 +237,10 @@ int mjw_set_probes(mjw_problem *p, int npts, const double *pts, uint8_t *inside)
 /* probes of values (deriv[i] = -1) or first derivatives (deriv[i] = a:
  * d u / d x_a, as mjw_set_receivers_grad) for the elastic snapshot files,
  * which hold each displacement component and each stress component
- * (<name>%04d_U<m>.dat, <name>%04d_dU<k>.dat); elem (may be NULL) receives
- * the element holding each point, -1 outside the mesh. */
+ * (<name>%04d_U<m>.dat, <name>%04d_dU<k>.dat; the grid output of
+ * Output/OutputHarmonic.cxx:3009-3027 with the gradient turned into the
+ * stress by Elliptic/Elastic/VarElastic.cxx:257-282); elem (may be NULL)
+ * receives the element holding each point, -1 outside the mesh. */
 int mjw_set_probes_grad(mjw_problem *p, int npts, const double *pts, const int32_t *deriv,
                         uint8_t *inside, int64_t *elem);
 int mjw_probe(mjw_problem *p, double *values);
