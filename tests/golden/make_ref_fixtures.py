@@ -297,6 +297,36 @@ def ref3d_el_q3_plane():
              line_times=[0.0, 0.36, 0.18])
 
 
+@case
+def ref3d_q7_pert():
+    """3^3 Q7 perturbed hexahedra (the highest order), two layers."""
+    rec = seeded_points(3, 6, 89)
+    run_case("ref3d_q7_pert", 3, 3, 7, 0.1, 31, [1.0, 2.0], 6.0, 4e-3, 150, (0.48, 0.51, 0.46),
+             rec)
+
+
+@case
+def ref3d_q2_pert():
+    """5^3 Q2 perturbed hexahedra, three layers."""
+    rec = seeded_points(3, 6, 90)
+    run_case("ref3d_q2_pert", 3, 5, 2, 0.1, 33, [1.0, 2.0, 1.5], 4.0, 1.5e-2, 120,
+             (0.52, 0.47, 0.5), rec)
+
+
+@case
+def ref2d_q1_pert():
+    """16x16 Q1 perturbed quadrilaterals (the lowest order), two layers."""
+    circ = O.circle_points(0.5, 0.5, 0.3, 6)
+    run_case("ref2d_q1_pert", 2, 16, 1, 0.1, 32, [1.0, 3.0], 4.0, 1e-2, 150, (0.47, 0.52), circ)
+
+
+@case
+def ref2d_q6_pert():
+    """6x6 Q6 perturbed quadrilaterals, two layers."""
+    circ = O.circle_points(0.5, 0.5, 0.3, 6)
+    run_case("ref2d_q6_pert", 2, 6, 6, 0.1, 34, [1.0, 2.0], 8.0, 2.5e-3, 200, (0.47, 0.52), circ)
+
+
 if __name__ == "__main__":
     if not os.path.exists(os.path.join(REF, "mjref_acous.x")):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
