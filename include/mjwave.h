@@ -307,6 +307,19 @@ int mjw_set_mu(mjw_problem *p, const double *mu);
  * mjw_set_mu and the adjoint are refused, and mjw_leapfrog_init must be called
  * again. */
 int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const double *mu_q);
+/*
+ * Damping sigma >= 0 per element (acoustic: rho u'' + sigma u' - div(mu grad u)
+ * = f, the sigma of MateriauDielec = ref ISOTROPE rho mu sigma,
+ * Elliptic/Helmholtz/VarHelmholtz.cxx:64-66,108-109).  S_h = sigma J w per
+ * node, lumped like the mass; the leapfrog then solves
+ * (D + dt/2 S) U^{n+1} = 2 D U^n - (D - dt/2 S) U^{n-1} + dt^2 (F - A U^n)
+ * (Advance, ModifiedEquation.cxx:295-345, with DhMinusdtSh / DhPlusdtSh,
+ * Hyperbolic/Acoustic/TimeAcoustic.cxx:970-1004) and starts from
+ * U^1 = U^0 + dt du0 + dt^2/2 D^{-1}(-A U^0 - S du0) (:99-124).  Call before
+ * mjw_leapfrog_init; all-zero sigma removes the damping.  Adjoint, batch and
+ * mjw_profile refuse damped problems.
+ */
+int mjw_set_damping(mjw_problem *p, const double *sigma);
 
 /* --------------------------------------------------------- synthetic meshes */
 /*

@@ -63,6 +63,12 @@ struct mjw_problem {
   bool tri_ok = false;
   int geo = kGeoStored;
   bool nodal_media = false;           // rho, mu given per quadrature point
+  // damping S_h = sigma J w per node (mjw_set_damping), solver order
+  std::vector<double> damp_S;
+  int64_t ndamp = 0;                  // damped, non-Dirichlet nodes
+  double *d_Meff = nullptr, *d_damp_k = nullptr;
+  int32_t *d_damp_idx = nullptr;
+  void *d_damp_save = nullptr;
   std::vector<double> hcoef;          // E*kCoef, sorted order, FP64
   void *d_coef = nullptr, *d_glxw = nullptr;
   double *d_M = nullptr, *d_Minv = nullptr, *d_isqM = nullptr;
@@ -208,7 +214,8 @@ static void release(mjw_problem *p) {
                   p->d_lam, p->d_mu, p->d_rho, p->d_D, p->d_U[0], p->d_U[1], p->d_tmp,
                   p->d_tmp2, p->d_part, p->d_red, p->d_ctl, p->d_pulse, p->d_src_dof,
                   p->d_src_shot, p->d_src_val, p->d_rec_gid, p->d_rec_comp, p->d_rec_w,
-                  p->d_traces, p->d_pr_gid, p->d_pr_w, p->d_pr_out};
+                  p->d_traces, p->d_pr_gid, p->d_pr_w, p->d_pr_out, p->d_Meff, p->d_damp_k,
+                  p->d_damp_idx, p->d_damp_save};
   for (void *q : ptrs) dfree(q);
   for (void *q : p->snaps) dfree(q);
   void *aptrs[] = {p->d_pair[0], p->d_pair[1], p->d_grad, p->d_adjg, p->d_adj_node,
@@ -1040,7 +1047,41 @@ extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, cons
   p->adj_ready = false;
   const int64_t ns = nstate(p);
   const int64_t n1 = (int64_t)p->ncomp * p->N;
-  KCHK(p, launch_make_sdiag(p->state, p->N, p->d_M, p->d_dir, dt * dt, p->d_sdiag, st));
+  if (!p->damp_S.empty()) {
+    // the step's kernel list depends on the damped lists rebuilt here
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+    // damped: s = dt^2 / (D + dt/2 S), k = dt S / (D + dt/2 S) on the damped
+    // nodes, -dt/2 S / D for the first step's -S du0 term
+    std::vector<double> meff(p->N), kk, c0;
+    std::vector<int32_t> idx;
+    for (int64_t g = 0; g < p->N; ++g) {
+      const int64_t i = p->nperm[g];
+      const double S = p->damp_S[i], M = p->mass[g];
+      meff[i] = M + 0.5 * dt * S;
+      if (S > 0.0 && !p->dir[g]) {
+        idx.push_back((int32_t)i);
+        kk.push_back(dt * S / meff[i]);
+        c0.push_back(-0.5 * dt * S / M);
+      }
+    }
+    dfree(p->d_Meff); dfree(p->d_damp_k); dfree(p->d_damp_idx); dfree(p->d_damp_save);
+    p->d_Meff = nullptr; p->d_damp_k = nullptr; p->d_damp_idx = nullptr; p->d_damp_save = nullptr;
+    p->ndamp = (int64_t)idx.size();
+    HIPCHK(p, dalloc(p, &p->d_Meff, (size_t)p->N * 8));
+    HIPCHK(p, hipMemcpy(p->d_Meff, meff.data(), (size_t)p->N * 8, hipMemcpyHostToDevice));
+    KCHK(p, launch_make_sdiag(p->state, p->N, p->d_Meff, p->d_dir, dt * dt, p->d_sdiag, st));
+    if (p->ndamp) {
+      HIPCHK(p, dalloc(p, &p->d_damp_idx, idx.size() * 4));
+      HIPCHK(p, dalloc(p, &p->d_damp_k, 2 * kk.size() * 8));
+      HIPCHK(p, dalloc(p, &p->d_damp_save,
+                       (size_t)p->ndamp * p->ncomp * p->nshots * ssize(p)));
+      kk.insert(kk.end(), c0.begin(), c0.end());
+      HIPCHK(p, hipMemcpy(p->d_damp_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+      HIPCHK(p, hipMemcpy(p->d_damp_k, kk.data(), kk.size() * 8, hipMemcpyHostToDevice));
+    }
+  } else {
+    KCHK(p, launch_make_sdiag(p->state, p->N, p->d_M, p->d_dir, dt * dt, p->d_sdiag, st));
+  }
   p->cur = 0;
   void *U0 = p->d_U[0], *U1 = p->d_U[1];
   if (u0) {
@@ -1056,6 +1097,11 @@ extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, cons
   if (du0) {
     KCHK(p, upload_vec(p, du0, p->d_tmp2, p->nshots));
     KCHK(p, launch_axpby(p->state, ns, 1.0, p->d_tmp2, 1.0, p->d_tmp, st));
+    // damping: D U'' = -A U - S U' at t_begin (ModifiedEquation.cxx:111-113)
+    if (p->ndamp)
+      KCHK(p, launch_damp(p->state, 2, p->ndamp, p->ncomp, p->nshots, p->ilv, p->N,
+                          p->d_damp_idx, p->d_damp_k + p->ndamp, p->d_tmp, nullptr, p->d_tmp2,
+                          st));
   }
   HIPCHK(p, hipMemcpyAsync(U1, U0, ns * ssize(p), hipMemcpyDeviceToDevice, st));
   KCHK(p, launch_axpby(p->state, ns, dt, p->d_tmp, 1.0, U1, st));
@@ -1074,6 +1120,12 @@ extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, cons
 // source and records the traces of U^{n+1} (row n+1).
 static int enqueue_step(mjw_problem *p, int parity, double *traces) {
   void *Ucur = p->d_U[parity ^ 1], *Y = p->d_U[parity];
+  if (p->ndamp) {
+    // keep U^n on the damped nodes before the passes overwrite it
+    int rc = launch_damp(p->state, 0, p->ndamp, p->ncomp, p->nshots, p->ilv, p->N,
+                         p->d_damp_idx, p->d_damp_k, Y, p->d_damp_save, nullptr, p->stream);
+    if (rc) return rc;
+  }
   for (int c = 0; c < p->ncolor; ++c) {
     StiffArgs a = stiff_args(p, kLeap, Ucur, Y, p->nshots);
     a.e0 = p->coff[c];
@@ -1089,7 +1141,10 @@ static int enqueue_step(mjw_problem *p, int parity, double *traces) {
   t.traces = (traces && p->nrec) ? traces : nullptr; t.N = p->N; t.nshots = p->nshots; t.ilv = p->ilv;
   t.advance = 1;
   t.row_offset = 1;
-  return launch_tail_n(t, p->ncomp, p->stream);
+  int rc = launch_tail_n(t, p->ncomp, p->stream);
+  if (rc || !p->ndamp) return rc;
+  return launch_damp(p->state, 1, p->ndamp, p->ncomp, p->nshots, p->ilv, p->N, p->d_damp_idx,
+                     p->d_damp_k, Y, p->d_damp_save, Ucur, p->stream);
 }
 
 // traces of U^n (d_U[cur]) into row step - base (no source, no step change)
@@ -1248,6 +1303,74 @@ extern "C" int mjw_set_probes_grad(mjw_problem *p, int npts, const double *pts,
   return set_probes(p, npts, pts, deriv, inside, elem);
 }
 
+// damping matrix S_h = sum_e sigma_e J w phi_i (lumped like the mass; the
+// damping term of rho u'' + sigma u' - div(mu grad u) = f, VarHelmholtz.cxx
+// :64-66,108-109 "Glob_matMass_DhSigma", assembled by TimeAcoustic.cxx:970-1004)
+extern "C" int mjw_set_damping(mjw_problem *p, const double *sigma) {
+  if (!p || !sigma) FAIL(p, MJW_ERR_ARG, "mjw_set_damping: NULL");
+  if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_damping: acoustic problems only");
+  HIPCHK(p, hipSetDevice(p->device));
+  const int64_t E = p->E, N = p->N;
+  std::vector<double> ss(E);
+  bool any = false;
+  for (int64_t es = 0; es < E; ++es) {
+    const double v = sigma[p->perm[es]];
+    if (!(v >= 0.0) || !std::isfinite(v))
+      FAIL(p, MJW_ERR_ARG, "mjw_set_damping: sigma must be finite, >= 0");
+    ss[es] = v;
+    any = any || v > 0.0;
+  }
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  p->inited = false;
+  p->ndamp = 0;
+  if (!any) {
+    p->damp_S.clear();
+    return MJW_OK;
+  }
+  double xg[kMaxN], wg[kMaxN], Dg[kMaxN * kMaxN];
+  gl_rule(p->order, xg, wg);
+  gl_deriv(p->order, xg, Dg);
+  std::vector<double> xyz_s((size_t)N * p->dim);
+  for (int64_t g = 0; g < N; ++g)
+    for (int a = 0; a < p->dim; ++a) xyz_s[(size_t)p->nperm[g] * p->dim + a] = p->xyz[g * p->dim + a];
+  double *d_xyz = nullptr, *d_Dg = nullptr, *d_w = nullptr, *d_Mloc = nullptr, *d_sig = nullptr,
+         *d_S = nullptr;
+  int *d_bad = nullptr;
+  hipStream_t st = p->stream;
+  hipError_t e = hipSuccess;
+  auto ok = [&](hipError_t x) { if (e == hipSuccess) e = x; return e == hipSuccess; };
+  ok(hipMalloc(&d_xyz, xyz_s.size() * 8));
+  ok(hipMalloc(&d_Dg, p->n * p->n * 8));
+  ok(hipMalloc(&d_w, p->n * 8));
+  ok(hipMalloc(&d_Mloc, (size_t)E * p->nd * 8));
+  ok(hipMalloc(&d_sig, (size_t)E * 8));
+  ok(hipMalloc(&d_S, (size_t)N * 8));
+  ok(hipMalloc(&d_bad, 4));
+  ok(hipMemsetAsync(d_bad, 0, 4, st));
+  ok(hipMemcpyAsync(d_xyz, xyz_s.data(), xyz_s.size() * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_Dg, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_w, wg, p->n * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_sig, ss.data(), (size_t)E * 8, hipMemcpyHostToDevice, st));
+  // sigma in the place of rho: the kernel's "mass" output is sigma J w
+  if (e == hipSuccess &&
+      launch_geometry_full(p->dim, p->order, 1, E, p->d_map, d_xyz, d_Dg, d_w, d_sig, p->d_mu,
+                           nullptr, nullptr, nullptr, nullptr, d_Mloc, d_bad, st))
+    e = hipErrorLaunchFailure;
+  for (int c = 0; c < p->ncolor && e == hipSuccess; ++c)
+    if (launch_mass_assemble(p->d_map, d_Mloc, E, p->dim, p->order, p->coff[c], p->coff[c + 1],
+                             d_S, st))
+      e = hipErrorLaunchFailure;
+  p->damp_S.assign(N, 0.0);
+  ok(hipMemcpyAsync(p->damp_S.data(), d_S, (size_t)N * 8, hipMemcpyDeviceToHost, st));
+  ok(hipStreamSynchronize(st));
+  dfree(d_xyz); dfree(d_Dg); dfree(d_w); dfree(d_Mloc); dfree(d_sig); dfree(d_S); dfree(d_bad);
+  if (e != hipSuccess) {
+    p->damp_S.clear();
+    FAIL(p, MJW_ERR_HIP, std::string("mjw_set_damping: ") + hipGetErrorString(e));
+  }
+  return MJW_OK;
+}
+
 extern "C" int mjw_probe(mjw_problem *p, double *values) {
   if (!p || !values) FAIL(p, MJW_ERR_ARG, "mjw_probe: NULL");
   if (p->nprobe == 0) return MJW_OK;
@@ -1292,6 +1415,7 @@ extern "C" int mjw_last_timing(const mjw_problem *p, double *ms2) {
 extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
   if (!p || !out || nsteps < 1) FAIL(p, MJW_ERR_ARG, "mjw_profile: bad args");
   if (!p->inited) FAIL(p, MJW_ERR_STATE, "not initialised");
+  if (!p->damp_S.empty()) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_profile: undamped problems only");
   HIPCHK(p, hipSetDevice(p->device));
   hipStream_t st = p->stream;
   // events only around each step's colour passes and its tail: the colour
@@ -1569,6 +1693,7 @@ extern "C" int mjw_adjoint_forward(mjw_problem *p, int nsteps, int nsnap, double
     FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: single-shot acoustic problems only");
   if (p->nodal_media)
     FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: per-element mu only (the problem has nodal media)");
+  if (!p->damp_S.empty()) FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: undamped problems only");
   if (!p->inited || p->step != 0 || !p->zero_ic)
     FAIL(p, MJW_ERR_STATE, "adjoint: call mjw_leapfrog_init with zero initial state first");
   if (p->nrec == 0) FAIL(p, MJW_ERR_STATE, "adjoint: no receivers");
@@ -1837,6 +1962,8 @@ extern "C" int mjw_batch_create(mjw_batch **out, int nprob, mjw_problem *const *
     if (!p) BFAIL((mjw_batch *)nullptr, MJW_ERR_ARG, "mjw_batch_create: NULL problem");
     if (p->dim != 3 || p->ncomp != 1 || p->nshots != 1)
       BFAIL((mjw_batch *)nullptr, MJW_ERR_UNSUPPORTED, "batch: single-shot 3-D acoustic problems");
+    if (!p->damp_S.empty())
+      BFAIL((mjw_batch *)nullptr, MJW_ERR_UNSUPPORTED, "batch: undamped problems only");
     if (p->device != probs[0]->device || p->state != probs[0]->state ||
         p->geom != probs[0]->geom)
       BFAIL((mjw_batch *)nullptr, MJW_ERR_ARG, "batch: problems must share device and precision");

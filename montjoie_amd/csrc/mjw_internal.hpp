@@ -169,6 +169,10 @@ int launch_mask(int state, int64_t n, int S, int ilv, int64_t N, const uint8_t *
                 void *stream);
 int launch_make_sdiag(int state, int64_t N, const double *M, const uint8_t *dir, double dt2,
                       void *s, void *stream);
+// damped nodes: mode 0 save = y, 1 y += coef (save - u), 2 y += coef u
+int launch_damp(int state, int mode, int64_t ndamp, int ncomp, int S, int ilv, int64_t N,
+                const int32_t *idx, const double *coef, void *y, void *save, const void *u,
+                void *stream);
 // per shot: out[s] = sum_i w[i%N] x_i y_i  /  sum_i w[i%N] (a_i-b_i)^2, FP64,
 // deterministic; part = scratch of reduction_scratch_doubles(S) doubles
 int launch_dot_part(int state, int64_t N, int ncomp, int S, int ilv, const double *w,

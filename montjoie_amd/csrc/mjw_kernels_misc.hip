@@ -517,6 +517,50 @@ int launch_make_sdiag(int state, int64_t N, const double *M, const uint8_t *dir,
   return (int)hipGetLastError();
 }
 
+// Damped leapfrog (S_h != 0, Advance, ModifiedEquation.cxx:295-345):
+//   (D + dt/2 S) U^{n+1} = 2 D U^n - (D - dt/2 S) U^{n-1} + dt^2 (F - A U^n).
+// The colour passes form Y* = 2 U^n - U^{n-1} + s (F - A U^n) with
+// s = dt^2 / (D + dt/2 S); the exact update is Y* + k (U^{n-1} - U^n) with
+// k = dt S / (D + dt/2 S), nonzero only on damped nodes.  Over the damped
+// nodes idx (every component and shot):
+//   mode 0: save = Y (U^{n-1}, before the passes overwrite it)
+//   mode 1: Y += k (save - U)
+//   mode 2: Y += coef * U
+template <typename T>
+__global__ void k_damp(int mode, int64_t ndamp, int ncomp, int S, int ilv, int64_t N,
+                       const int32_t *__restrict__ idx, const double *__restrict__ coef,
+                       T *__restrict__ y, T *__restrict__ save, const T *__restrict__ u) {
+  const int64_t tot = ndamp * ncomp * S;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < tot;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = j % ndamp, rest = j / ndamp;
+    const int64_t m = rest % ncomp, s = rest / ncomp;
+    const int64_t g = idx[i];
+    const int64_t o = ilv ? ((m * N + g) * S + s) : (s * ncomp * N + m * N + g);
+    if (mode == 0)
+      save[j] = y[o];
+    else if (mode == 1)
+      y[o] += (T)coef[i] * (save[j] - u[o]);
+    else
+      y[o] += (T)coef[i] * u[o];
+  }
+}
+
+int launch_damp(int state, int mode, int64_t ndamp, int ncomp, int S, int ilv, int64_t N,
+                const int32_t *idx, const double *coef, void *y, void *save, const void *u,
+                void *stream) {
+  const int64_t tot = ndamp * ncomp * S;
+  if (tot <= 0) return 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_damp<double>, dim3(grid_for(tot)), dim3(256), 0, st, mode, ndamp, ncomp,
+                       S, ilv, N, idx, coef, (double *)y, (double *)save, (const double *)u);
+  else
+    hipLaunchKernelGGL(k_damp<float>, dim3(grid_for(tot)), dim3(256), 0, st, mode, ndamp, ncomp,
+                       S, ilv, N, idx, coef, (float *)y, (float *)save, (const float *)u);
+  return (int)hipGetLastError();
+}
+
 // G[(plane*E + e)*P + q] *= sc[e] over all planes (media update)
 template <typename T>
 __global__ void k_scale_planes(T *__restrict__ G, int64_t n, int64_t E, int P,

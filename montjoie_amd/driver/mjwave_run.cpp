@@ -609,7 +609,7 @@ void write_vector(const std::string &path, const std::vector<T> &v) {
 
 // ------------------------------------------------------------------ media
 struct Media {
-  std::vector<double> rho, mu, lam;
+  std::vector<double> rho, mu, lam, sigma;  // sigma: acoustic damping (empty if none)
 };
 
 Media media(const Deck &d, const Q1Mesh &m, bool elastic) {
@@ -627,12 +627,16 @@ Media media(const Deck &d, const Q1Mesh &m, bool elastic) {
       md.rho.push_back(p[0]);
       md.lam.push_back(p[1]);
       md.mu.push_back(p[2]);
-    } else {  // ISOTROPE rho mu sigma
-      if (p[2] != 0.0) die("MateriauDielec: damping (sigma != 0) is not on the explicit path");
+    } else {  // ISOTROPE rho mu sigma (sigma: damping, rho u'' + sigma u' - div(mu grad u))
+      if (!(p[2] >= 0.0)) die("MateriauDielec: the damping sigma must be >= 0");
       md.rho.push_back(p[0]);
       md.mu.push_back(p[1]);
+      md.sigma.push_back(p[2]);
     }
   }
+  bool damped = false;
+  for (double v : md.sigma) damped = damped || v > 0.0;
+  if (!damped) md.sigma.clear();
   return md;
 }
 
@@ -1132,6 +1136,7 @@ int main(int argc, char **argv) {
         nullptr, "mjw_create");
   if (!nm.rho.empty())
     check(mjw_set_media_nodal(p, nm.rho.data(), nm.mu.data()), p, "PhysicalMedia");
+  if (!md.sigma.empty()) check(mjw_set_damping(p, md.sigma.data()), p, "MateriauDielec sigma");
   // time step: given, or TimeStep = AUTO: 0.999 times the leapfrog limit
   // 2 / sqrt(lambda_max(D^-1 K)) (VarInstationary.cxx:91-103,1229-1240,
   // EvaluateCFL_Generic, EvaluationCflWave.cxx:717-760, P(0) = 2), lambda_max

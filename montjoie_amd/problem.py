@@ -212,6 +212,13 @@ class WaveProblem:
         t = np.ascontiguousarray(table, np.float64)
         self._chk(_lib.lib().mjw_set_pulse(self._h, len(t), dptr(t)), "set_pulse")
 
+    def set_damping(self, sigma):
+        """Damping sigma per element (rho u'' + sigma u' - div(mu grad u) = f;
+        mjw_set_damping); call leapfrog_init afterwards."""
+        sg = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma, np.float64),
+                                                  (self.mesh.E,)))
+        self._chk(_lib.lib().mjw_set_damping(self._h, dptr(sg)), "set_damping")
+
     def set_receivers(self, pts, comp=None, deriv=None):
         """Receivers of u (deriv None or -1) or of d u / d x_deriv at pts on
         component comp; returns the element holding each point."""

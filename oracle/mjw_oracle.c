@@ -911,14 +911,15 @@ void or_circle_points(double cx, double cy, double radius, int npts, double *pts
 /* ------------------------------------------------------------------------ */
 
 /*
- * Second-order leapfrog on D u'' + A u = f(t) b with homogeneous Dirichlet
+ * Second-order leapfrog on D u'' + S u' + A u = f(t) b (S diagonal, NULL =
+ * undamped) with homogeneous Dirichlet
  * nodes, restating RunSecondOrderScheme (Instationary/ModifiedEquation.cxx:
  * 16-53), SetInitialConditionS (:99-124, order 2) and Advance (:295-345,
  * order 2, S_h = 0):
- *   U0 = u0, U1 = U0 + dt du0 + dt^2/2 D^{-1}(-A U0);  Dirichlet on both
+ *   U0 = u0, U1 = U0 + dt du0 + dt^2/2 D^{-1}(-A U0 - S du0);  Dirichlet on both
  *   each step n (t = t_begin + n dt):  record R U0;
- *     Y = -A U1 + pulse[n] b;  Y = dt^2 Y - D U0;  Y += 2 D U1;
- *     U0 = U1; U1 = Y; zero Dirichlet; U1 *= D^{-1}
+ *     Y = -A U1 + pulse[n] b;  Y = dt^2 Y - (D - dt/2 S) U0;  Y += 2 D U1;
+ *     U0 = U1; U1 = Y; zero Dirichlet; U1 *= (D + dt/2 S)^{-1}
  *   after the loop record R U0  (GiveFinalIterate).
  * ncomp = 1 (acoustic, G) or 3 (elastic, jw/dfinv/lam/mu).
  * Inputs u0/du0 may be NULL (zero).  Outputs: uN = U^nsteps, uN1 = U^{nsteps+1}
@@ -927,7 +928,7 @@ void or_circle_points(double cx, double cy, double radius, int npts, double *pts
  */
 void or_leapfrog(int dim, int r, int ncomp, long E, long N, const int *elem_nodes,
                  const double *G, const double *jw, const double *dfinv, const double *lam,
-                 const double *mu, const int *color, int ncolor, const double *Dm,
+                 const double *mu, const int *color, int ncolor, const double *Dm, const double *Sd,
                  const unsigned char *dirichlet, const double *b, const double *pulse,
                  double dt, int nsteps, const double *u0, const double *du0, int nrec,
                  const long *rec_elem, const double *rec_w, const int *rec_comp,
@@ -938,7 +939,15 @@ void or_leapfrog(int dim, int r, int ncomp, long E, long N, const int *elem_node
   double *U1 = (double *)calloc((size_t)NT, sizeof(double));
   double *Y = (double *)calloc((size_t)NT, sizeof(double));
   double *invD = (double *)malloc(sizeof(double) * NT);
+  double *invDp = (double *)malloc(sizeof(double) * NT);
+  double *Dmin = (double *)malloc(sizeof(double) * NT);
   for (long g = 0; g < NT; g++) invD[g] = 1.0 / Dm[g % N];
+  /* damping: D - dt/2 S and (D + dt/2 S)^{-1} (TimeAcoustic.cxx:977-1004) */
+  for (long g = 0; g < NT; g++) {
+    double sg = Sd ? Sd[g % N] : 0.0;
+    Dmin[g] = Dm[g % N] - 0.5 * dt * sg;
+    invDp[g] = 1.0 / (Dm[g % N] + 0.5 * dt * sg);
+  }
   double dt2 = dt * dt;
   long nb = 0;
   for (long g = 0; g < NT; g++)
@@ -974,8 +983,10 @@ void or_leapfrog(int dim, int r, int ncomp, long E, long N, const int *elem_node
   if (u0) memcpy(U0, u0, sizeof(double) * NT);
   memcpy(U1, U0, sizeof(double) * NT);
   OR_DIRICHLET(U0);
-  /* U1 = D^{-1}(K U0 - S du0) with K = -A, S = 0 */
+  /* U1 = D^{-1}(K U0 - S du0) with K = -A */
   OR_APPLY(-1.0, U0, Y);
+  if (Sd && du0)
+    for (long g = 0; g < NT; g++) Y[g] -= Sd[g % N] * du0[g];
   for (long g = 0; g < NT; g++) U1[g] = Y[g] * invD[g];
   for (long g = 0; g < NT; g++) {
     double v = U1[g] * (0.5 * dt);
@@ -989,11 +1000,11 @@ void or_leapfrog(int dim, int r, int ncomp, long E, long N, const int *elem_node
     OR_APPLY(-1.0, U1, Y);
     if (pulse && pulse[it] != 0.0)
       for (long s = 0; s < nb; s++) Y[bi[s]] += pulse[it] * b[bi[s]];
-    for (long g = 0; g < NT; g++) Y[g] = dt2 * Y[g] + (-1.0) * (Dm[g % N] * U0[g]);
+    for (long g = 0; g < NT; g++) Y[g] = dt2 * Y[g] + (-1.0) * (Dmin[g] * U0[g]);
     for (long g = 0; g < NT; g++) Y[g] = 1.0 * Y[g] + 2.0 * (Dm[g % N] * U1[g]);
     double *tmp = U0; U0 = U1; U1 = Y; Y = tmp;
     OR_DIRICHLET(U1);
-    for (long g = 0; g < NT; g++) U1[g] *= invD[g];
+    for (long g = 0; g < NT; g++) U1[g] *= invDp[g];
   }
   OR_RECORD(nsteps);
   if (uN) memcpy(uN, U0, sizeof(double) * NT);
@@ -1001,7 +1012,7 @@ void or_leapfrog(int dim, int r, int ncomp, long E, long N, const int *elem_node
 #undef OR_APPLY
 #undef OR_DIRICHLET
 #undef OR_RECORD
-  free(U0); free(U1); free(Y); free(invD); free(bi);
+  free(U0); free(U1); free(Y); free(invD); free(invDp); free(Dmin); free(bi);
 }
 
 /* ------------------------------------------------------------------------ */

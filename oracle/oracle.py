@@ -76,9 +76,9 @@ def lib():
     L.or_circle_points.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp]
     L.or_leapfrog.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, _ip,
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                              _ip, C.c_int, _dp, _bp, C.c_void_p, C.c_void_p, C.c_double,
-                              C.c_int, C.c_void_p, C.c_void_p, C.c_int, _lp, _dp, C.c_void_p,
-                              _dp, _dp, _dp]
+                              _ip, C.c_int, _dp, C.c_void_p, _bp, C.c_void_p, C.c_void_p,
+                              C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_int, _lp, _dp,
+                              C.c_void_p, _dp, _dp, _dp]
     L.or_lambda_max.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, _ip,
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 _ip, C.c_int, _dp, _bp, C.c_int, C.c_uint64]
@@ -732,7 +732,7 @@ def lambda_max(P: Problem, m_iter: int = 120, seed: int = 7) -> float:
 
 
 def leapfrog(P: Problem, b, pulse, dt, nsteps, rec_pts=None, rec_comp=None, u0=None, du0=None,
-             rec_weights=None):
+             rec_weights=None, S=None):
     """Returns (U^N, U^{N+1}, traces[(N+1), nrec]).  rec_weights = (elements,
     weights [nrec, nd]) replaces the value weights of rec_pts (e.g. gradient
     weights from receiver_gradients)."""
@@ -753,11 +753,20 @@ def leapfrog(P: Problem, b, pulse, dt, nsteps, rec_pts=None, rec_comp=None, u0=N
     tr = np.zeros(((nsteps + 1) * max(nrec, 1)))
     pulse = None if pulse is None else np.ascontiguousarray(pulse, np.float64)
     lib().or_leapfrog(m.dim, m.r, P.ncomp, m.E, m.N, m.elem_nodes.reshape(-1), _ptr(P.G),
-                      _ptr(P.jw), _ptr(P.dfi), _ptr(P.lam), _ptr(P.mu), c, nc, P.M, m.dirichlet,
-                      _ptr(b), _ptr(pulse), dt, nsteps, _ptr(u0), _ptr(du0), nrec,
+                      _ptr(P.jw), _ptr(P.dfi), _ptr(P.lam), _ptr(P.mu), c, nc, P.M,
+                      _ptr(None if S is None else np.ascontiguousarray(S, np.float64)),
+                      m.dirichlet, _ptr(b), _ptr(pulse), dt, nsteps, _ptr(u0), _ptr(du0), nrec,
                       np.ascontiguousarray(re), np.ascontiguousarray(rw.reshape(-1)), _ptr(rc),
                       uN, uN1, tr)
     return uN, uN1, tr[: (nsteps + 1) * nrec].reshape(nsteps + 1, nrec)
+
+
+def damping(m: Mesh, sigma_e):
+    """Damping diagonal S = sum_e sigma_e J w phi_i (lumped like the mass,
+    VarHelmholtz.cxx:64-66,108-109), from the oracle's geometry with sigma in
+    the place of rho."""
+    _, Mloc = geometry(m, np.ones(m.E), np.asarray(sigma_e, float))
+    return mass(m, Mloc, dirichlet=False)[0]
 
 
 def adjoint_gradient(P: Problem, b, pulse, dt, nsteps, rec_pts, dJ):

@@ -33,6 +33,7 @@ class Case:
     rec: np.ndarray
     times: np.ndarray | None = None
     traces: np.ndarray | None = None
+    sigma_e: np.ndarray | None = None
 
 
 def golden_names():
@@ -73,9 +74,11 @@ def load(name: str) -> Case:
                                                                          pts.shape[:2]).copy()
         mu_e = sinus_field(smu, pts) if len(smu) else np.broadcast_to(mu_e[:, None],
                                                                       pts.shape[:2]).copy()
+    sg = z["sigma_layers"] if "sigma_layers" in z.files else np.zeros(0)
+    sigma_e = sg[lay] if len(sg) else None
     return Case(name, dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]), elastic, mu_e,
                 rho_e, lam_e, float(z["f0"]), float(z["dt"]), int(z["nsteps"]), z["src"],
-                z["pol"] if elastic else None, z["rec"], z["times"], z["traces"])
+                z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e)
 
 
 def sinus_field(p, pts):
@@ -100,12 +103,13 @@ def oracle_run(c: Case, nsteps: int | None = None):
         b = O.dirac(m, c.src)
         rc = None
     pulse = O.pulse_table(c.f0, 0.0, c.dt, nsteps)
+    S = O.damping(m, c.sigma_e) if c.sigma_e is not None and np.any(c.sigma_e) else None
     if c.elastic:
         # every receiver on the three components, component-major
         pts = np.vstack([c.rec] * 3)
         comp = np.repeat(np.arange(3, dtype=np.int32), len(c.rec))
         return O.leapfrog(P, b, pulse, c.dt, nsteps, pts, comp)
-    return O.leapfrog(P, b, pulse, c.dt, nsteps, c.rec, rc)
+    return O.leapfrog(P, b, pulse, c.dt, nsteps, c.rec, rc, S=S)
 
 
 def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
@@ -182,6 +186,8 @@ def gpu_problem(c: Case, precision: str = "fp64", nshots: int = 1, srcs=None):
         P.set_media_nodal(c.rho_e, c.mu_e)
     else:
         P = M.WaveProblem(m, c.rho_e, c.mu_e, nshots=nshots, precision=precision)
+    if c.sigma_e is not None and np.any(c.sigma_e):
+        P.set_damping(c.sigma_e)
     if srcs is None:
         srcs = [c.src] * nshots
     pol = None if not c.elastic else np.vstack([c.pol] * len(srcs))

@@ -92,10 +92,11 @@ def write_don(path, nu):
 def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
-               mesh_media=None, save_reprise=0):
+               mesh_media=None, save_reprise=0, sigma_layers=None):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1.
-    save_reprise > 0: checkpoint files every save_reprise steps."""
+    save_reprise > 0: checkpoint files every save_reprise steps.  sigma_layers:
+    acoustic damping of each layer (MateriauDielec's third value)."""
     os.makedirs(d, exist_ok=True)
     m = O.mesh(dim, nx, r, pert, seed)
     nl = len(layers)
@@ -126,7 +127,8 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
             ini.append(f"MateriauDielec = {k + 1} ISOTROPE {_f(rho_layers[k])} "
                        f"{_f(lam_layers[k])} {_f(layers[k])}")
         else:
-            ini.append(f"MateriauDielec = {k + 1} ISOTROPE 1.0 {_f(layers[k])} 0.0")
+            sg = 0.0 if sigma_layers is None or not len(sigma_layers) else sigma_layers[k]
+            ini.append(f"MateriauDielec = {k + 1} ISOTROPE 1.0 {_f(layers[k])} {_f(sg)}")
     for name, rm, off, amp, nu in (mesh_media or []):
         # PhysicalMedia ... SAME_MESH on the deck's Medit mesh (medium 1)
         write_don(os.path.join(d, name + ".don"), nu)

@@ -35,13 +35,13 @@ GOLD = os.path.dirname(os.path.abspath(__file__))
 def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
-             mesh_media=None, save_reprise=0):
+             mesh_media=None, save_reprise=0, sigma_layers=None):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
                sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times,
                line=line, line_times=line_times, mesh_media=mesh_media,
-               save_reprise=save_reprise)
+               save_reprise=save_reprise, sigma_layers=sigma_layers)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):
@@ -93,6 +93,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         **{f"meshmedia_{nm}_{k}": v for nm, rm, off, amp, nu in (mesh_media or [])
            for k, v in (("order", rm), ("offset", off), ("amplitude", amp), ("nu", nu))},
         save_reprise=save_reprise, **reprise,
+        sigma_layers=np.asarray(sigma_layers if sigma_layers is not None else [], float),
         plane_names=names.get("totalPlane", np.zeros(0, "U1")),
         line_names=names.get("totalLine", np.zeros(0, "U1")))
     print(f"{name}: {traces.shape} rows, times {times[0]}..{times[-1]}")
@@ -325,6 +326,23 @@ def ref2d_q6_pert():
     """6x6 Q6 perturbed quadrilaterals, two layers."""
     circ = O.circle_points(0.5, 0.5, 0.3, 6)
     run_case("ref2d_q6_pert", 2, 6, 6, 0.1, 34, [1.0, 2.0], 8.0, 2.5e-3, 200, (0.47, 0.52), circ)
+
+
+@case
+def ref2d_q4_damped():
+    """8x8 Q4 perturbed quadrilaterals, three layers with damping sigma 0, 3, 12
+    (MateriauDielec's third value)."""
+    circ = O.circle_points(0.5, 0.5, 0.3, 6)
+    run_case("ref2d_q4_damped", 2, 8, 4, 0.1, 43, [1.0, 2.0, 1.5], 10.0, 1e-3, 300, (0.47, 0.52),
+             circ, sigma_layers=[0.0, 3.0, 12.0])
+
+
+@case
+def ref3d_q3_damped():
+    """4^3 Q3 perturbed hexahedra, two layers with damping sigma 5 and 20."""
+    rec = seeded_points(3, 6, 91)
+    run_case("ref3d_q3_damped", 3, 4, 3, 0.1, 99, [1.0, 2.5], 8.0, 5e-3, 150, (0.43, 0.52, 0.47),
+             rec, sigma_layers=[5.0, 20.0])
 
 
 if __name__ == "__main__":
