@@ -320,6 +320,9 @@ int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const double *mu_q)
  * mjw_profile refuse damped problems.
  */
 int mjw_set_damping(mjw_problem *p, const double *sigma);
+/* damping sigma at every quadrature point, [n_elem][nd] in local order (a
+ * PhysicalMedia sigma field, Elliptic/Helmholtz/VarHelmholtz.cxx:1219-1226) */
+int mjw_set_damping_nodal(mjw_problem *p, const double *sigma_q);
 
 /* --------------------------------------------------------- synthetic meshes */
 /*

@@ -1306,20 +1306,22 @@ extern "C" int mjw_set_probes_grad(mjw_problem *p, int npts, const double *pts,
 // damping matrix S_h = sum_e sigma_e J w phi_i (lumped like the mass; the
 // damping term of rho u'' + sigma u' - div(mu grad u) = f, VarHelmholtz.cxx
 // :64-66,108-109 "Glob_matMass_DhSigma", assembled by TimeAcoustic.cxx:970-1004)
-extern "C" int mjw_set_damping(mjw_problem *p, const double *sigma) {
+// sigma per element (nodal = false) or per quadrature point [E][nd] (nodal)
+static int set_damping(mjw_problem *p, const double *sigma, bool nodal) {
   if (!p || !sigma) FAIL(p, MJW_ERR_ARG, "mjw_set_damping: NULL");
   if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_damping: acoustic problems only");
   HIPCHK(p, hipSetDevice(p->device));
-  const int64_t E = p->E, N = p->N;
-  std::vector<double> ss(E);
+  const int64_t E = p->E, N = p->N, per = nodal ? p->nd : 1;
+  std::vector<double> ss((size_t)E * per);
   bool any = false;
-  for (int64_t es = 0; es < E; ++es) {
-    const double v = sigma[p->perm[es]];
-    if (!(v >= 0.0) || !std::isfinite(v))
-      FAIL(p, MJW_ERR_ARG, "mjw_set_damping: sigma must be finite, >= 0");
-    ss[es] = v;
-    any = any || v > 0.0;
-  }
+  for (int64_t es = 0; es < E; ++es)
+    for (int64_t q = 0; q < per; ++q) {
+      const double v = sigma[p->perm[es] * per + q];
+      if (!(v >= 0.0) || !std::isfinite(v))
+        FAIL(p, MJW_ERR_ARG, "mjw_set_damping: sigma must be finite, >= 0");
+      ss[(size_t)es * per + q] = v;
+      any = any || v > 0.0;
+    }
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   p->inited = false;
   p->ndamp = 0;
@@ -1343,18 +1345,19 @@ extern "C" int mjw_set_damping(mjw_problem *p, const double *sigma) {
   ok(hipMalloc(&d_Dg, p->n * p->n * 8));
   ok(hipMalloc(&d_w, p->n * 8));
   ok(hipMalloc(&d_Mloc, (size_t)E * p->nd * 8));
-  ok(hipMalloc(&d_sig, (size_t)E * 8));
+  ok(hipMalloc(&d_sig, ss.size() * 8));
   ok(hipMalloc(&d_S, (size_t)N * 8));
   ok(hipMalloc(&d_bad, 4));
   ok(hipMemsetAsync(d_bad, 0, 4, st));
   ok(hipMemcpyAsync(d_xyz, xyz_s.data(), xyz_s.size() * 8, hipMemcpyHostToDevice, st));
   ok(hipMemcpyAsync(d_Dg, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
   ok(hipMemcpyAsync(d_w, wg, p->n * 8, hipMemcpyHostToDevice, st));
-  ok(hipMemcpyAsync(d_sig, ss.data(), (size_t)E * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_sig, ss.data(), ss.size() * 8, hipMemcpyHostToDevice, st));
   // sigma in the place of rho: the kernel's "mass" output is sigma J w
   if (e == hipSuccess &&
-      launch_geometry_full(p->dim, p->order, 1, E, p->d_map, d_xyz, d_Dg, d_w, d_sig, p->d_mu,
-                           nullptr, nullptr, nullptr, nullptr, d_Mloc, d_bad, st))
+      launch_geometry_full(p->dim, p->order, 1, E, p->d_map, d_xyz, d_Dg, d_w,
+                           nodal ? p->d_rho : d_sig, p->d_mu, nodal ? d_sig : nullptr, nullptr,
+                           nullptr, nullptr, d_Mloc, d_bad, st))
     e = hipErrorLaunchFailure;
   for (int c = 0; c < p->ncolor && e == hipSuccess; ++c)
     if (launch_mass_assemble(p->d_map, d_Mloc, E, p->dim, p->order, p->coff[c], p->coff[c + 1],
@@ -1369,6 +1372,14 @@ extern "C" int mjw_set_damping(mjw_problem *p, const double *sigma) {
     FAIL(p, MJW_ERR_HIP, std::string("mjw_set_damping: ") + hipGetErrorString(e));
   }
   return MJW_OK;
+}
+
+extern "C" int mjw_set_damping(mjw_problem *p, const double *sigma) {
+  return set_damping(p, sigma, false);
+}
+
+extern "C" int mjw_set_damping_nodal(mjw_problem *p, const double *sigma_q) {
+  return set_damping(p, sigma_q, true);
 }
 
 extern "C" int mjw_probe(mjw_problem *p, double *values) {

@@ -19,7 +19,7 @@
 // Keywords read: TypeElement (QUADRANGLE_LOBATTO, HEXAHEDRON_LOBATTO),
 // TypeEquation (ACOUSTIC, TIME_ELASTIC), MeshPath, FileMesh, TypeMesh,
 // OrderDiscretization, ConditionReference (DIRICHLET), MateriauDielec
-// (ISOTROPE), PhysicalMedia (rho / mu, SINUS or MESH / SAME_MESH data files on the
+// (ISOTROPE, acoustic damping sigma), PhysicalMedia (rho / mu / sigma, SINUS or MESH / SAME_MESH data files on the
 // computational mesh, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
 // TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
@@ -843,7 +843,8 @@ void mesh_media(const std::vector<std::string> &v, const Deck &d, const Q1Mesh &
 // rho and mu at every quadrature point (the GL nodes of each element) when a
 // PhysicalMedia line makes them vary; empty otherwise
 struct NodalMedia {
-  std::vector<double> rho, mu;
+  std::vector<double> rho, mu;  // empty unless a rho or mu field is given: E * nd each
+  std::vector<double> sigma;    // empty unless a sigma (damping) field is given
 };
 
 NodalMedia nodal_media(const Deck &d, const Q1Mesh &m, const QrMesh &qm, const Media &md, int dim,
@@ -853,18 +854,29 @@ NodalMedia nodal_media(const Deck &d, const Q1Mesh &m, const QrMesh &qm, const M
   if (lines.empty()) return nm;
   if (elastic) die("PhysicalMedia: varying media are supported for ACOUSTIC decks only");
   const int64_t nd = (int64_t)qm.en.size() / qm.E;
-  nm.rho.resize(qm.en.size());
-  nm.mu.resize(qm.en.size());
-  for (int64_t e = 0; e < qm.E; ++e)
-    for (int64_t q = 0; q < nd; ++q) {
-      nm.rho[e * nd + q] = md.rho[e];
-      nm.mu[e * nd + q] = md.mu[e];
-    }
+  bool rhomu = false, sig = false;
   for (const auto &v : lines) {
     if (v.size() < 4) die("PhysicalMedia = name ref ISOTROPE SINUS|MESH|SAME_MESH ...");
-    if (v[0] != "rho" && v[0] != "mu") die("PhysicalMedia " + v[0] + ": only rho and mu");
+    if (v[0] != "rho" && v[0] != "mu" && v[0] != "sigma")
+      die("PhysicalMedia " + v[0] + ": only rho, mu and sigma");
+    (v[0] == "sigma" ? sig : rhomu) = true;
+  }
+  if (rhomu) {
+    nm.rho.resize(qm.en.size());
+    nm.mu.resize(qm.en.size());
+  }
+  if (sig) nm.sigma.resize(qm.en.size());
+  for (int64_t e = 0; e < qm.E; ++e)
+    for (int64_t q = 0; q < nd; ++q) {
+      if (rhomu) {
+        nm.rho[e * nd + q] = md.rho[e];
+        nm.mu[e * nd + q] = md.mu[e];
+      }
+      if (sig) nm.sigma[e * nd + q] = md.sigma.empty() ? 0.0 : md.sigma[e];
+    }
+  for (const auto &v : lines) {
     const int ref = (int)num(v[1]);
-    std::vector<double> &dst = v[0] == "rho" ? nm.rho : nm.mu;
+    std::vector<double> &dst = v[0] == "rho" ? nm.rho : v[0] == "mu" ? nm.mu : nm.sigma;
     if (v[3] == "MESH" || v[3] == "SAME_MESH") {
       mesh_media(v, d, m, qm, dim, ref, dst);
       continue;
@@ -1136,7 +1148,10 @@ int main(int argc, char **argv) {
         nullptr, "mjw_create");
   if (!nm.rho.empty())
     check(mjw_set_media_nodal(p, nm.rho.data(), nm.mu.data()), p, "PhysicalMedia");
-  if (!md.sigma.empty()) check(mjw_set_damping(p, md.sigma.data()), p, "MateriauDielec sigma");
+  if (!nm.sigma.empty())
+    check(mjw_set_damping_nodal(p, nm.sigma.data()), p, "PhysicalMedia sigma");
+  else if (!md.sigma.empty())
+    check(mjw_set_damping(p, md.sigma.data()), p, "MateriauDielec sigma");
   // time step: given, or TimeStep = AUTO: 0.999 times the leapfrog limit
   // 2 / sqrt(lambda_max(D^-1 K)) (VarInstationary.cxx:91-103,1229-1240,
   // EvaluateCFL_Generic, EvaluationCflWave.cxx:717-760, P(0) = 2), lambda_max

@@ -213,10 +213,15 @@ class WaveProblem:
         self._chk(_lib.lib().mjw_set_pulse(self._h, len(t), dptr(t)), "set_pulse")
 
     def set_damping(self, sigma):
-        """Damping sigma per element (rho u'' + sigma u' - div(mu grad u) = f;
-        mjw_set_damping); call leapfrog_init afterwards."""
-        sg = np.ascontiguousarray(np.broadcast_to(np.asarray(sigma, np.float64),
-                                                  (self.mesh.E,)))
+        """Damping sigma per element, or [E, nd] at the quadrature points
+        (rho u'' + sigma u' - div(mu grad u) = f; mjw_set_damping[_nodal]);
+        call leapfrog_init afterwards."""
+        sg = np.asarray(sigma, np.float64)
+        if sg.ndim == 2:
+            sg = np.ascontiguousarray(sg)
+            self._chk(_lib.lib().mjw_set_damping_nodal(self._h, dptr(sg)), "set_damping")
+            return
+        sg = np.ascontiguousarray(np.broadcast_to(sg, (self.mesh.E,)))
         self._chk(_lib.lib().mjw_set_damping(self._h, dptr(sg)), "set_damping")
 
     def set_receivers(self, pts, comp=None, deriv=None):

@@ -762,10 +762,11 @@ def leapfrog(P: Problem, b, pulse, dt, nsteps, rec_pts=None, rec_comp=None, u0=N
 
 
 def damping(m: Mesh, sigma_e):
-    """Damping diagonal S = sum_e sigma_e J w phi_i (lumped like the mass,
+    """Damping diagonal S = sum_e sigma J w phi_i (lumped like the mass,
     VarHelmholtz.cxx:64-66,108-109), from the oracle's geometry with sigma in
-    the place of rho."""
-    _, Mloc = geometry(m, np.ones(m.E), np.asarray(sigma_e, float))
+    the place of rho; sigma per element or [E, nd] at the quadrature points."""
+    sg = np.asarray(sigma_e, float)
+    _, Mloc = geometry(m, np.ones(sg.shape), sg)
     return mass(m, Mloc, dirichlet=False)[0]
 
 

@@ -76,6 +76,14 @@ def load(name: str) -> Case:
                                                                       pts.shape[:2]).copy()
     sg = z["sigma_layers"] if "sigma_layers" in z.files else np.zeros(0)
     sigma_e = sg[lay] if len(sg) else None
+    ssg = z["sinus_sigma"] if "sinus_sigma" in z.files else np.zeros(0)
+    if len(ssg):
+        # PhysicalMedia sigma SINUS of medium 1 (the first layer): damping at
+        # the quadrature points; the other layers keep MateriauDielec's sigma
+        m = O.mesh(dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]))
+        f = sinus_field(ssg, O.quadrature_points(m))
+        base = sg[lay] if len(sg) else np.zeros(E)
+        sigma_e = np.where((lay == 0)[:, None], f, base[:, None])
     return Case(name, dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]), elastic, mu_e,
                 rho_e, lam_e, float(z["f0"]), float(z["dt"]), int(z["nsteps"]), z["src"],
                 z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e)

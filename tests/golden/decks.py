@@ -92,7 +92,7 @@ def write_don(path, nu):
 def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
-               mesh_media=None, save_reprise=0, sigma_layers=None):
+               mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1.
     save_reprise > 0: checkpoint files every save_reprise steps.  sigma_layers:
@@ -138,6 +138,8 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
         ini.append(sinus_line("rho", 1, sinus_rho))
     if sinus_mu is not None and len(sinus_mu):
         ini.append(sinus_line("mu", 1, sinus_mu))
+    if sinus_sigma is not None and len(sinus_sigma):
+        ini.append(sinus_line("sigma", 1, sinus_sigma))
     ini.append("TypeSource = SRC_DIRAC")
     ini.append("OriginePhase = " + " ".join(_f(c) for c in src))
     if elastic:

@@ -37,7 +37,8 @@ def _write(c: CS.Case, d):
                                   float(z[f"meshmedia_{nm}_amplitude"]), z[f"meshmedia_{nm}_nu"])
                                  for nm in ("mu", "rho") if f"meshmedia_{nm}_nu" in z.files],
                      save_reprise=int(z["save_reprise"]) if "save_reprise" in z.files else 0,
-                     sigma_layers=z["sigma_layers"] if "sigma_layers" in z.files else None)
+                     sigma_layers=z["sigma_layers"] if "sigma_layers" in z.files else None,
+                     sinus_sigma=z["sinus_sigma"] if "sinus_sigma" in z.files else None)
 
 
 def _dump(path):
