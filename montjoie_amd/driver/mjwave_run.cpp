@@ -1000,14 +1000,24 @@ Plane plane_grid(const std::vector<std::string> &v, int dim, const QrMesh &qm) {
   return g;
 }
 
-void write_plane(const std::string &name, const Plane &g, const std::vector<double> &vals) {
+// dbl: OutputFormat = BINARY DOUBLE (data type 1, grid parameters and values
+// in double), else BINARY FLOAT (data type 0, floats)
+void write_plane(const std::string &name, const Plane &g, const std::vector<double> &vals,
+                 bool dbl) {
   std::ofstream o(name, std::ios::binary);
   if (!o) die("cannot write " + name);
   auto wi = [&](int32_t x) { o.write(reinterpret_cast<const char *>(&x), 4); };
-  auto wd = [&](double x) { o.write(reinterpret_cast<const char *>(&x), 8); };
+  auto wd = [&](double x) {
+    if (dbl) {
+      o.write(reinterpret_cast<const char *>(&x), 8);
+    } else {
+      const float f = (float)x;
+      o.write(reinterpret_cast<const char *>(&f), 4);
+    }
+  };
   wi(1);                        // one grid
   wi(g.dim);                    // dimension
-  wi(1);                        // real double values
+  wi(dbl ? 1 : 0);              // real double / float values
   // grid type: plane 0 (2-D) / 2 (3-D), line 1 (2-D) / 3 (3-D)
   wi(g.line ? (g.dim == 2 ? 1 : 3) : (g.dim == 2 ? 0 : 2));
   if (g.line) {
@@ -1021,7 +1031,7 @@ void write_plane(const std::string &name, const Plane &g, const std::vector<doub
     wi(g.nx); wi(g.ny);
   }
   wi((int32_t)vals.size());
-  o.write(reinterpret_cast<const char *>(vals.data()), (std::streamsize)(vals.size() * 8));
+  for (double v : vals) wd(v);
   if (!o) die("cannot write " + name);
 }
 
@@ -1098,6 +1108,16 @@ int main(int argc, char **argv) {
     if (save_every <= 0) die("SaveReprise: the number of steps must be positive");
   }
   const std::string path_reprise = d.get("PathReprise") ? d.need("PathReprise")[0] : "./";
+  // OutputFormat = BINARY|ASCII|... DOUBLE|FLOAT (Output/OutputHarmonic.cxx:321-380): the
+  // precision sets the snapshot data type and the seismogram digits (15 or 7,
+  // Output/GridInterpolation.cxx:3150-3152)
+  bool out_binary = true, out_double = true;
+  if (const auto *of = d.get("OutputFormat")) {
+    if (of->size() < 2) die("OutputFormat needs a format and a precision");
+    out_binary = (*of)[0] == "BINARY";
+    if ((*of)[1] == "FLOAT") out_double = false;
+    else if ((*of)[1] != "DOUBLE") die("OutputFormat: only DOUBLE or FLOAT precision");
+  }
 
   // mesh
   const auto &fm = d.need("FileMesh");
@@ -1253,9 +1273,7 @@ int main(int argc, char **argv) {
     const auto lines = d.all(key);
     if (lines.empty()) continue;
     if (lines.size() > 1) die(key + ": one per deck");
-    const auto *of = d.get("OutputFormat");
-    if (!of || of->size() < 2 || (*of)[0] != "BINARY" || (*of)[1] != "DOUBLE")
-      die(key + ": only OutputFormat = BINARY DOUBLE is written");
+    if (!out_binary) die(key + ": only OutputFormat = BINARY DOUBLE or BINARY FLOAT is written");
     Snap sn;
     sn.g = std::string(kind) == "Plane" ? plane_grid(lines[0], dim, qm) : line_grid(lines[0], dim);
     const auto &fo = d.need(std::string("FileOutput") + kind);
@@ -1300,12 +1318,12 @@ int main(int argc, char **argv) {
       std::vector<double> vals(n);
       if (!elastic) {
         for (size_t i = 0; i < n; ++i) vals[i] = pv[sn.off + i];
-        write_plane(sn.name + num + ".dat", sn.g, vals);
+        write_plane(sn.name + num + ".dat", sn.g, vals, out_double);
         continue;
       }
       for (int m = 0; m < 3; ++m) {
         for (size_t i = 0; i < n; ++i) vals[i] = pv[(sn.off + i) * 3 + m];
-        write_plane(sn.name + num + "_U" + std::to_string(m) + ".dat", sn.g, vals);
+        write_plane(sn.name + num + "_U" + std::to_string(m) + ".dat", sn.g, vals, out_double);
       }
       // gradient probes: sn.off + (1 + a) n + i holds d u_m / d x_a at point i
       auto grad = [&](size_t i, int m, int a) { return pv[(sn.off + (1 + a) * n + i) * 3 + m]; };
@@ -1317,7 +1335,8 @@ int main(int argc, char **argv) {
             const double div = grad(i, 0, 0) + grad(i, 1, 1) + grad(i, 2, 2);
             vals[i] = (a == b ? md.lam[e] * div : 0.0) + md.mu[e] * (grad(i, a, b) + grad(i, b, a));
           }
-          write_plane(sn.name + num + "_dU" + std::to_string(3 * a + b) + ".dat", sn.g, vals);
+          write_plane(sn.name + num + "_dU" + std::to_string(3 * a + b) + ".dat", sn.g, vals,
+                      out_double);
         }
     }
   };
@@ -1394,11 +1413,12 @@ int main(int argc, char **argv) {
       const double t = t0 + s * dt;
       if (t < ts0 - 1e-9 * dt || t > ts1 + 1e-9 * dt) continue;
       if (std::llround((t - ts0) / dt) % every) continue;
-      std::fprintf(f, "%.15e", t);
+      const char *fmt = out_double ? "\t%.15e" : "\t%.7e";
+      std::fprintf(f, fmt + 1, t);
       // point-major: the components of point 0, then of point 1, ...
       for (int r = 0; r < npt; ++r)
         for (int m = 0; m < ncomp; ++m)
-          std::fprintf(f, "\t%.15e", tr[(size_t)s * nrec + (size_t)m * npt + r]);
+          std::fprintf(f, fmt, tr[(size_t)s * nrec + (size_t)m * npt + r]);
       // elastic: then the stress C grad u of each point, 9 values row-major
       // (sigma_ij = lambda tr(grad u) delta_ij + mu (du_i/dx_j + du_j/dx_i),
       // ModifyOutputUnknown, Elliptic/Elastic/VarElastic.cxx:257-282), with
@@ -1410,7 +1430,7 @@ int main(int argc, char **argv) {
           const double lam = md.lam[e], mu = md.mu[e], div = g[0] + g[4] + g[8];
           for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j)
-              std::fprintf(f, "\t%.15e",
+              std::fprintf(f, fmt,
                            (i == j ? lam * div : 0.0) + mu * (g[i * 3 + j] + g[j * 3 + i]));
         }
       std::fprintf(f, "\n");

@@ -92,7 +92,8 @@ def write_don(path, nu):
 def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
-               mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None):
+               mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
+               output_format="BINARY DOUBLE"):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1.
     save_reprise > 0: checkpoint files every save_reprise steps.  sigma_layers:
@@ -158,7 +159,7 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
         ini.append("SismoLine = " + " ".join(_f(v) for v in line[:-1]) + f" {int(line[-1])}")
         ini.append("FileOutputLine = diffracLine totalLine")
         ini.append("ParametersOutputLine = " + " ".join(_f(v) for v in line_times))
-    ini.append("OutputFormat = BINARY DOUBLE")
+    ini.append(f"OutputFormat = {output_format}")
     if save_reprise:
         ini.append(f"SaveReprise = {int(save_reprise)}")
     ini.append("ElectricOrMagnetic = " + ("-1" if elastic else "0"))

@@ -35,13 +35,15 @@ GOLD = os.path.dirname(os.path.abspath(__file__))
 def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
-             mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None):
+             mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
+             output_format="BINARY DOUBLE"):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
                sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times,
                line=line, line_times=line_times, mesh_media=mesh_media,
-               save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma)
+               save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma,
+               output_format=output_format)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):
@@ -95,6 +97,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         save_reprise=save_reprise, **reprise,
         sigma_layers=np.asarray(sigma_layers if sigma_layers is not None else [], float),
         sinus_sigma=np.asarray(sinus_sigma if sinus_sigma is not None else [], float),
+        output_format=output_format,
         plane_names=names.get("totalPlane", np.zeros(0, "U1")),
         line_names=names.get("totalLine", np.zeros(0, "U1")))
     print(f"{name}: {traces.shape} rows, times {times[0]}..{times[-1]}")
@@ -355,6 +358,18 @@ def ref2d_q4_sinus_sigma():
     run_case("ref2d_q4_sinus_sigma", 2, 8, 4, 0.1, 44, [1.0, 2.0], 10.0, 1e-3, 300, (0.47, 0.52),
              circ, sigma_layers=[1.0, 2.0],
              sinus_sigma=[0.05, 0.1, 0.95, 0.9, 0.01, 0.02, 4.0, 3.0, 0.5, 0.7])
+
+
+@case
+def fmt3d_q3_plane_float():
+    """The ref3d_q3_plane deck with OutputFormat = BINARY FLOAT: float snapshot
+    files (data type 0) and 8-digit seismograms.  Named outside the ref*
+    family: its 7-digit traces are not a 1e-10 parity fixture."""
+    rec = seeded_points(3, 4, 81)
+    run_case("fmt3d_q3_plane_float", 3, 4, 3, 0.1, 99, [1.0, 2.5], 8.0, 5e-3, 150,
+             (0.43, 0.52, 0.47), rec, plane=[0.05, 0.1, 0.5, 1.2, 0.1, 0.55, 0.05, 0.95, 0.6, 6, 5],
+             plane_times=[0.0, 0.6, 0.15], line=[0.1, 1.3, 0.2, 0.7, 0.3, 0.6, 9],
+             line_times=[0.0, 0.75, 0.25], output_format="BINARY FLOAT")
 
 
 if __name__ == "__main__":

@@ -38,7 +38,9 @@ def _write(c: CS.Case, d):
                                  for nm in ("mu", "rho") if f"meshmedia_{nm}_nu" in z.files],
                      save_reprise=int(z["save_reprise"]) if "save_reprise" in z.files else 0,
                      sigma_layers=z["sigma_layers"] if "sigma_layers" in z.files else None,
-                     sinus_sigma=z["sinus_sigma"] if "sinus_sigma" in z.files else None)
+                     sinus_sigma=z["sinus_sigma"] if "sinus_sigma" in z.files else None,
+                     output_format=str(z["output_format"]) if "output_format" in z.files
+                     else "BINARY DOUBLE")
 
 
 def _dump(path):
@@ -276,3 +278,38 @@ def test_driver_checkpoints_match_montjoie(name, tmp_path):
         assert len(raw) == len(ref) and raw[:4] == ref[:4]
         v, vr = np.frombuffer(raw[4:], np.float64), np.frombuffer(ref[4:], np.float64)
         assert CS.rel_l2(v, vr) < 1e-10, CS.rel_l2(v, vr)
+
+
+@pytest.mark.gpu
+def test_driver_float_output_matches_montjoie(tmp_path):
+    """OutputFormat = BINARY FLOAT: snapshot files with data type 0 and float
+    grid parameters and values, and seismograms with 8 significant digits,
+    as Montjoie writes them.  Headers and names exact; values to float
+    rounding (1e-6 relative to the largest value)."""
+    c = CS.load("fmt3d_q3_plane_float")
+    z = np.load(os.path.join(CS.GOLD, "fmt3d_q3_plane_float.npz"))
+    _write(c, tmp_path)
+    print("deck fmt3d_q3_plane_float", flush=True)
+    r = subprocess.run([RUNNER, "case.ini"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    row = (tmp_path / "totalPoint.dat").read_text().splitlines()[5].split("\t")
+    assert all(len(v.split("e")[0].lstrip("-")) == 9 for v in row)  # d.ddddddd
+    out = np.loadtxt(tmp_path / "totalPoint.dat", ndmin=2)
+    assert out.shape == (c.traces.shape[0], c.traces.shape[1] + 1)
+    assert np.abs(out[:, 1:] - c.traces).max() <= 1e-6 * np.abs(c.traces).max()
+    for kind, prefix in (("plane", "totalPlane"), ("line", "totalLine")):
+        files = sorted(f.name for f in tmp_path.iterdir() if f.name.startswith(prefix))
+        ref = z[kind + "_files"]
+        assert files == list(z[kind + "_names"])
+        g = z[kind]
+        npts = int(g[-1] * g[-2]) if kind == "plane" else int(g[-1])
+        for k, f in enumerate(files):
+            raw = (tmp_path / f).read_bytes()
+            assert len(raw) == ref.shape[1]
+            n = len(raw) - 4 * npts
+            assert np.frombuffer(raw[8:12], np.int32)[0] == 0
+            assert raw[:n] == bytes(ref[k][:n])
+            v = np.frombuffer(raw[n:], np.float32)
+            vr = np.frombuffer(bytes(ref[k][n:]), np.float32)
+            assert np.abs(v - vr).max() <= 1e-6 * max(np.abs(vr).max(), 1e-30)
