@@ -24,7 +24,8 @@
 // TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
 // ParametersOutputPoint, SismoPlane, FileOutputPlane, ParametersOutputPlane,
-// SismoLine, FileOutputLine, ParametersOutputLine, SaveReprise, PathReprise.  Other keywords are reported and ignored; a value
+// SismoLine, FileOutputLine, ParametersOutputLine, OutputFormat (BINARY DOUBLE or FLOAT),
+// DisplayStress (YES), SaveReprise, PathReprise.  Other keywords are reported and ignored; a value
 // this path does not implement (another scheme, source or condition) stops
 // the run with a message, as Montjoie's own parser does (cout + abort).
 //
