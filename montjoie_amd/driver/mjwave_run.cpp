@@ -24,7 +24,7 @@
 // TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
 // ParametersOutputPoint, SismoPlane, FileOutputPlane, ParametersOutputPlane,
-// SismoLine, FileOutputLine, ParametersOutputLine, OutputFormat (BINARY DOUBLE or FLOAT),
+// SismoLine, FileOutputLine, ParametersOutputLine, OutputFormat (BINARY or ASCII, DOUBLE or FLOAT),
 // DisplayStress (YES), SaveReprise, PathReprise.  Other keywords are reported and ignored; a value
 // this path does not implement (another scheme, source or condition) stops
 // the run with a message, as Montjoie's own parser does (cout + abort).
@@ -1001,10 +1001,38 @@ Plane plane_grid(const std::vector<std::string> &v, int dim, const QrMesh &qm) {
   return g;
 }
 
+// OutputFormat = ASCII DOUBLE|FLOAT: the same fields as text, one per line
+// (WriteMatlab's ascii branch, Output/GridInterpolation.cxx:6108-6130, with
+// the grid header of WriteText, :3146-3175 (2-D) and :5597-5640 (3-D)):
+// integers as such, reals at 15 (DOUBLE) or 7 (FLOAT) significant digits in
+// the default stream notation (%g)
+void write_plane_text(const std::string &name, const Plane &g, const std::vector<double> &vals,
+                      bool dbl) {
+  FILE *f = std::fopen(name.c_str(), "w");
+  if (!f) die("cannot write " + name);
+  const int prec = dbl ? 15 : 7;
+  auto wr_d = [&](double x) { std::fprintf(f, "%.*g\n", prec, x); };
+  std::fprintf(f, "1\n%d\n%d\n%d\n", g.dim, dbl ? 1 : 0,
+               g.line ? (g.dim == 2 ? 1 : 3) : (g.dim == 2 ? 0 : 2));
+  if (g.line) {
+    for (int k = 0; k < 2 * g.dim; ++k) wr_d(g.p[k]);
+    std::fprintf(f, "%d\n", g.nx);
+  } else if (g.dim == 2) {
+    wr_d(g.p[0]); wr_d(g.p[1]); std::fprintf(f, "%d\n", g.nx);
+    wr_d(g.p[2]); wr_d(g.p[3]); std::fprintf(f, "%d\n", g.ny);
+  } else {
+    for (int k = 0; k < 9; ++k) wr_d(g.p[k]);
+    std::fprintf(f, "%d\n%d\n", g.nx, g.ny);
+  }
+  for (double v : vals) wr_d(v);
+  if (std::fclose(f) != 0) die("cannot write " + name);
+}
+
 // dbl: OutputFormat = BINARY DOUBLE (data type 1, grid parameters and values
 // in double), else BINARY FLOAT (data type 0, floats)
 void write_plane(const std::string &name, const Plane &g, const std::vector<double> &vals,
-                 bool dbl) {
+                 bool dbl, bool binary = true) {
+  if (!binary) return write_plane_text(name, g, vals, dbl);
   std::ofstream o(name, std::ios::binary);
   if (!o) die("cannot write " + name);
   auto wi = [&](int32_t x) { o.write(reinterpret_cast<const char *>(&x), 4); };
@@ -1112,10 +1140,11 @@ int main(int argc, char **argv) {
   // OutputFormat = BINARY|ASCII|... DOUBLE|FLOAT (Output/OutputHarmonic.cxx:321-380): the
   // precision sets the snapshot data type and the seismogram digits (15 or 7,
   // Output/GridInterpolation.cxx:3150-3152)
-  bool out_binary = true, out_double = true;
+  bool out_binary = true, out_ascii = false, out_double = true;
   if (const auto *of = d.get("OutputFormat")) {
     if (of->size() < 2) die("OutputFormat needs a format and a precision");
     out_binary = (*of)[0] == "BINARY";
+    out_ascii = (*of)[0] == "ASCII";
     if ((*of)[1] == "FLOAT") out_double = false;
     else if ((*of)[1] != "DOUBLE") die("OutputFormat: only DOUBLE or FLOAT precision");
   }
@@ -1274,7 +1303,8 @@ int main(int argc, char **argv) {
     const auto lines = d.all(key);
     if (lines.empty()) continue;
     if (lines.size() > 1) die(key + ": one per deck");
-    if (!out_binary) die(key + ": only OutputFormat = BINARY DOUBLE or BINARY FLOAT is written");
+    if (!out_binary && !out_ascii)
+      die(key + ": only OutputFormat = BINARY or ASCII (DOUBLE or FLOAT) is written");
     Snap sn;
     sn.g = std::string(kind) == "Plane" ? plane_grid(lines[0], dim, qm) : line_grid(lines[0], dim);
     const auto &fo = d.need(std::string("FileOutput") + kind);
@@ -1319,12 +1349,13 @@ int main(int argc, char **argv) {
       std::vector<double> vals(n);
       if (!elastic) {
         for (size_t i = 0; i < n; ++i) vals[i] = pv[sn.off + i];
-        write_plane(sn.name + num + ".dat", sn.g, vals, out_double);
+        write_plane(sn.name + num + ".dat", sn.g, vals, out_double, out_binary);
         continue;
       }
       for (int m = 0; m < 3; ++m) {
         for (size_t i = 0; i < n; ++i) vals[i] = pv[(sn.off + i) * 3 + m];
-        write_plane(sn.name + num + "_U" + std::to_string(m) + ".dat", sn.g, vals, out_double);
+        write_plane(sn.name + num + "_U" + std::to_string(m) + ".dat", sn.g, vals, out_double,
+                    out_binary);
       }
       // gradient probes: sn.off + (1 + a) n + i holds d u_m / d x_a at point i
       auto grad = [&](size_t i, int m, int a) { return pv[(sn.off + (1 + a) * n + i) * 3 + m]; };
@@ -1337,7 +1368,7 @@ int main(int argc, char **argv) {
             vals[i] = (a == b ? md.lam[e] * div : 0.0) + md.mu[e] * (grad(i, a, b) + grad(i, b, a));
           }
           write_plane(sn.name + num + "_dU" + std::to_string(3 * a + b) + ".dat", sn.g, vals,
-                      out_double);
+                      out_double, out_binary);
         }
     }
   };

@@ -63,8 +63,11 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         # ..., <prefix>0000_dU8.dat), sorted by name, raw bytes, one row each
         files = sorted(f for f in os.listdir(d) if f.startswith(prefix))
         names[prefix] = np.array(files)
-        return np.stack([np.frombuffer(open(os.path.join(d, f), "rb").read(), np.uint8)
-                         for f in files])
+        raw = [np.frombuffer(open(os.path.join(d, f), "rb").read(), np.uint8) for f in files]
+        # text files (ASCII formats) differ in length: zero-padded rows + lengths
+        names[prefix + "_len"] = np.array([len(r) for r in raw])
+        width = max(len(r) for r in raw)
+        return np.stack([np.pad(r, (0, width - len(r))) for r in raw])
     if plane is not None:
         planes = snap_files("totalPlane")
     if line is not None:
@@ -99,7 +102,9 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         sinus_sigma=np.asarray(sinus_sigma if sinus_sigma is not None else [], float),
         output_format=output_format,
         plane_names=names.get("totalPlane", np.zeros(0, "U1")),
-        line_names=names.get("totalLine", np.zeros(0, "U1")))
+        line_names=names.get("totalLine", np.zeros(0, "U1")),
+        plane_lens=names.get("totalPlane_len", np.zeros(0, int)),
+        line_lens=names.get("totalLine_len", np.zeros(0, int)))
     print(f"{name}: {traces.shape} rows, times {times[0]}..{times[-1]}")
 
 
@@ -370,6 +375,27 @@ def fmt3d_q3_plane_float():
              (0.43, 0.52, 0.47), rec, plane=[0.05, 0.1, 0.5, 1.2, 0.1, 0.55, 0.05, 0.95, 0.6, 6, 5],
              plane_times=[0.0, 0.6, 0.15], line=[0.1, 1.3, 0.2, 0.7, 0.3, 0.6, 9],
              line_times=[0.0, 0.75, 0.25], output_format="BINARY FLOAT")
+
+
+@case
+def fmt2d_q4_plane_ascii():
+    """The ref2d_q4_plane deck with OutputFormat = ASCII DOUBLE (text snapshot
+    files)."""
+    circ = O.circle_points(0.5, 0.5, 0.3, 4)
+    run_case("fmt2d_q4_plane_ascii", 2, 8, 4, 0.0, 0, [1.0], 10.0, 1e-3, 200, (0.47, 0.52), circ,
+             regular=True, plane=[-0.1, 0.9, 0.2, 1.15, 7, 6], plane_times=[0.0, 0.2, 0.05],
+             line=[0.1, 1.3, 0.2, 0.7, 7], line_times=[0.0, 0.2, 0.04],
+             output_format="ASCII DOUBLE")
+
+
+@case
+def fmt3d_q3_plane_ascii_float():
+    """The ref3d_q3_plane deck with OutputFormat = ASCII FLOAT."""
+    rec = seeded_points(3, 4, 81)
+    run_case("fmt3d_q3_plane_ascii_float", 3, 4, 3, 0.1, 99, [1.0, 2.5], 8.0, 5e-3, 150,
+             (0.43, 0.52, 0.47), rec, plane=[0.05, 0.1, 0.5, 1.2, 0.1, 0.55, 0.05, 0.95, 0.6, 6, 5],
+             plane_times=[0.0, 0.6, 0.15], line=[0.1, 1.3, 0.2, 0.7, 0.3, 0.6, 9],
+             line_times=[0.0, 0.75, 0.25], output_format="ASCII FLOAT")
 
 
 if __name__ == "__main__":

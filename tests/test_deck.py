@@ -313,3 +313,42 @@ def test_driver_float_output_matches_montjoie(tmp_path):
             v = np.frombuffer(raw[n:], np.float32)
             vr = np.frombuffer(bytes(ref[k][n:]), np.float32)
             assert np.abs(v - vr).max() <= 1e-6 * max(np.abs(vr).max(), 1e-30)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["fmt2d_q4_plane_ascii", "fmt3d_q3_plane_ascii_float"])
+def test_driver_ascii_output_matches_montjoie(name, tmp_path):
+    """OutputFormat = ASCII DOUBLE / FLOAT: text snapshot files, one field per
+    line, header lines (grid count, dimension, data type, grid type, grid
+    parameters at 15 or 7 significant digits) identical to Montjoie's, values
+    to 1e-10 (DOUBLE) or float rounding (FLOAT) with the same zeros."""
+    c = CS.load(name)
+    z = np.load(os.path.join(CS.GOLD, name + ".npz"))
+    dbl = str(z["output_format"]).endswith("DOUBLE")
+    _write(c, tmp_path)
+    print(f"deck {name}", flush=True)
+    r = subprocess.run([RUNNER, "case.ini"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    for kind, prefix in (("plane", "totalPlane"), ("line", "totalLine")):
+        files = sorted(f.name for f in tmp_path.iterdir() if f.name.startswith(prefix))
+        assert files == list(z[kind + "_names"])
+        g = z[kind]
+        npts = int(g[-1] * g[-2]) if kind == "plane" else int(g[-1])
+        for k, f in enumerate(files):
+            mine = (tmp_path / f).read_text().split("\n")
+            ref = bytes(z[kind + "_files"][k][:z[kind + "_lens"][k]]).decode().split("\n")
+            assert len(mine) == len(ref)
+            nh = len(ref) - 1 - npts
+            assert mine[:nh] == ref[:nh]
+            v = np.array(mine[nh:-1], float)
+            vr = np.array(ref[nh:-1], float)
+            if not vr.any():
+                assert not v.any()
+                continue
+            tiny = 1e-12 * np.abs(vr).max()
+            assert np.all(np.abs(vr[v == 0]) <= tiny) and np.all(np.abs(v[vr == 0]) <= tiny)
+            if dbl:
+                assert CS.rel_l2(v, vr) < 1e-10
+            else:
+                assert np.abs(v - vr).max() <= 1e-6 * np.abs(vr).max()
