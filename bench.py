@@ -9,9 +9,10 @@ the stiffness kernel + source/receiver tail), inputs resident in HBM.
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--no-cpu]
 
-Prints one JSON line.  `value` is the FP32 (state and geometry) path's
-GDoF-steps/s; the FP64 path, the FP32-with-FP64-geometry path and their
-rel-L2 errors against FP64 are reported beside it.
+Prints one JSON line.  `value` is the GDoF-steps/s of the FP32 state path
+with FP64 geometric factors (the workload's precision); the FP64 path and the
+all-FP32 path are reported beside it, with the errors of the FP32 paths
+against FP64 over the workload's 500 steps (untimed runs).
 """
 from __future__ import annotations
 
@@ -102,14 +103,33 @@ def flops_per_step(E, n):
     return E * nd * (12 * n + 18 + 6)
 
 
+def timed_advance(P, dt, steps, warmup):
+    """Untimed warm-up (graph capture and instantiation, traces buffer sized
+    for `steps`), then `steps` steps from a fresh zero state timed with the
+    device synchronised on both sides.  Returns (wall seconds, traces)."""
+    P.leapfrog_init(dt)
+    P.advance(max(warmup, steps, 16), traces=True, graph=True)
+    P.leapfrog_init(dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tr = P.advance(steps, traces=True, graph=True)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, tr
+
+
 def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10, tile=None,
-            nxcd=None):
+            nxcd=None, acc_steps=500):
+    """One precision on the headline workload: `steps` timed steps (HIP graph,
+    traces recorded), after an untimed warm-up of at least as many steps that
+    captures and instantiates the graph and sizes the traces buffer; then, from
+    a fresh start and untimed, the `acc_steps` run whose traces, state and FP64
+    energy give the accuracy figures."""
     import montjoie_amd as M
     P = M.WaveProblem(m, 1.0, mu, precision=prec, nshots=nshots)
     srcs = [(0.5, 0.5, 0.5)] * nshots
     P.set_sources(srcs, shot=np.arange(nshots))
     P.set_receivers(receivers())
-    total = warmup + steps + profile_steps
+    total = max(warmup, steps, acc_steps, 16) + profile_steps
     P.set_pulse(M.ricker_pulse_table(f0, 0.0, dt, total))
     if tile is None:
         from montjoie_amd import tuner
@@ -118,15 +138,11 @@ def run_gpu(m, mu, dt, prec, steps, warmup, f0=8.0, nshots=1, profile_steps=10, 
         P.set_tile(tile)
         P.set_xcd_order(nxcd or 8)
         tuned = {"tile": tile, "nxcd": nxcd or 8}
-    P.leapfrog_init(dt)
-    if warmup:
-        P.advance(warmup, traces=False, graph=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    tr = P.advance(steps, traces=True, graph=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    wall, tr = timed_advance(P, dt, steps, warmup)
     dev_ms = P.last_timing()[0]
+    if steps != acc_steps:
+        P.leapfrog_init(dt)
+        tr = P.advance(acc_steps, traces=True, graph=True)
     uN, _ = P.state()
     energy = P.energy()  # FP64 accumulation, also for the FP32 state
     prof = P.profile(profile_steps)
@@ -142,7 +158,7 @@ def shot_sources(nshots, seed=2027):
                      for s in range(nshots)])
 
 
-def bench_shots(m, mu, dt, steps, warmup, nshots=16):
+def bench_shots(m, mu, dt, steps, warmup, nshots=16, acc_steps=500):
     """FP32 batch of `nshots` Dirac shots in one problem (geometry read once per
     element for all shots) against the FP64 path, run shot by shot and as the
     same interleaved batch (time-to-solution ratios, and the FP32 error against
@@ -156,20 +172,22 @@ def bench_shots(m, mu, dt, steps, warmup, nshots=16):
         P = M.WaveProblem(m, 1.0, mu, precision=prec, nshots=ns)
         P.set_sources(srcs[:ns], shot=np.arange(ns))
         P.set_receivers(receivers())
-        P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, warmup + steps + 64))
+        P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, max(warmup, steps, acc_steps, 16) + 64))
         tuner.tune_problem(P, steps=4, dt=dt)  # launch shape only: result-neutral
-        P.leapfrog_init(dt)
-        if warmup:
-            P.advance(warmup, traces=False)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        tr = P.advance(steps, traces=True)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        out[key] = dict(wall=wall, shots=ns, dev_ms=P.last_timing()[0], traces=tr,
-                        uN=P.state()[0] if key in ("fp32", "fp64_batch") else None)
+        wall, tr = timed_advance(P, dt, steps, warmup)
+        dev_ms = P.last_timing()[0]
+        if key != "fp64" and steps != acc_steps:
+            P.leapfrog_init(dt)
+            tr = P.advance(acc_steps, traces=True)
+        prof = P.profile(4) if key == "fp32_g64" else None
+        out[key] = dict(wall=wall, shots=ns, dev_ms=dev_ms, traces=tr, prof=prof,
+                        uN=P.state()[0] if key != "fp64" else None)
         P.close()
     t64_batch = out["fp64"]["wall"] * nshots           # FP64 path, one shot at a time
+    # stored-geometry bytes per step: FP64 tensor and map per element node, three
+    # FP32 state streams per node and shot, FP32 dt^2/M per node
+    sb = m.E * m.nd * (6 * 8 + 4) + m.N * (3 * nshots * 4 + 4)
+    g64_ms = out["fp32_g64"]["prof"]["stiffness_launch_ms"]
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
     ref = out["fp64_batch"]
     return {
@@ -184,9 +202,19 @@ def bench_shots(m, mu, dt, steps, warmup, nshots=16):
         "time_to_solution_ratio_fp64_over_fp32_g64_batch": t64_batch / out["fp32_g64"]["wall"],
         "fp64_batch_s": ref["wall"],
         "time_to_solution_ratio_fp64_batch_over_fp32_batch": ref["wall"] / out["fp32"]["wall"],
+        "accuracy_steps": acc_steps,
         "fp32_batch_rel_l2_state_vs_fp64_batch": rel(out["fp32"]["uN"], ref["uN"]),
         "fp32_batch_rel_l2_traces_vs_fp64_batch": rel(out["fp32"]["traces"], ref["traces"]),
+        "fp32_g64_batch_rel_l2_state_vs_fp64_batch": rel(out["fp32_g64"]["uN"], ref["uN"]),
         "fp32_g64_batch_rel_l2_traces_vs_fp64_batch": rel(out["fp32_g64"]["traces"], ref["traces"]),
+        "time_to_solution_ratio_fp64_batch_over_fp32_g64_batch":
+            ref["wall"] / out["fp32_g64"]["wall"],
+        "roofline": {"bound": "hbm", "achieved": sb / 8 / (g64_ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": sb / 8 / (g64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_shots3d, FP32 state, FP64 geometry, one colour pass of "
+                               f"{nshots} interleaved shots",
+                     "algorithmic_bytes_per_launch": sb / 8, "launch_ms": g64_ms},
     }
 
 
@@ -204,8 +232,10 @@ def smooth_field(m, seed, ng=4):
 
 def bench_adjoint(nx=40, order=4, steps=2000, nsnap=20, ndir=8):
     """Gradient of an L2 receiver misfit w.r.t. per-element c^2 on a 40^3 Q4
-    cube: cost of forward + adjoint over a plain forward solve (FP32), FD check
-    of the FP64 gradient and FP32 vs FP64 gradient error."""
+    cube: cost of forward + adjoint over a plain forward solve (FP32 state with
+    FP64 geometric factors and FP64 gradient accumulation), central
+    finite-difference check of the FP64 gradient on 8 random directions, and
+    the FP32 vs FP64 gradient error."""
     import montjoie_amd as M
     from montjoie_amd.adjoint import AdjointSolver
     m = M.regular_mesh(3, nx, order, 0.0, 0)
@@ -219,14 +249,14 @@ def bench_adjoint(nx=40, order=4, steps=2000, nsnap=20, ndir=8):
                        f"perturbed 10% (seed 2030), {steps} steps, 64 receivers (8x8 at z=0.95), "
                        f"{nsnap} checkpoints", "dt": dt, "n_dof": m.N}
     solvers = {}
-    for prec in ("fp32", "fp64"):
+    for prec in ("fp32_g64", "fp64"):
         P = P64 if prec == "fp64" else M.WaveProblem(m, 1.0, m0, precision=prec)
         P.set_sources([(0.5, 0.5, 0.5)])
         P.set_receivers(rec)
         P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, steps + 4))
         solvers[prec] = (P, AdjointSolver(P, dt, steps, nsnap))
     grads = {}
-    for prec in ("fp32", "fp64"):
+    for prec in ("fp32_g64", "fp64"):
         P, S = solvers[prec]
         dobs = S.forward(mtrue)
         S.set_mu(m0)
@@ -254,40 +284,50 @@ def bench_adjoint(nx=40, order=4, steps=2000, nsnap=20, ndir=8):
             errs2, errs4 = [], []
             for _ in range(ndir):
                 d = rng.standard_normal(m.E)
-                h = 1e-4 * np.linalg.norm(m0) / np.linalg.norm(d)
-                jp, jm = J(m0 + h * d), J(m0 - h * d)
-                jp2, jm2 = J(m0 + 2 * h * d), J(m0 - 2 * h * d)
-                fd2 = (jp - jm) / (2 * h)
-                fd4 = (-jp2 + 8 * jp - 8 * jm + jm2) / (12 * h)
-                errs2.append(abs(fd2 - g @ d) / abs(fd4))
-                errs4.append(abs(fd4 - g @ d) / abs(fd4))
-            res["fd_check_fp64_max_rel_err"] = float(max(errs4))
-            res["fd_stencil"] = "central, 4th order, h = 1e-4 ||m|| / ||d||"
-            res["fd2_check_fp64_max_rel_err"] = float(max(errs2))
+                gd = g @ d
+                # central difference, step small enough that its O(h^2)
+                # truncation stays below the 1e-6 target
+                h = 1e-5 * np.linalg.norm(m0) / np.linalg.norm(d)
+                fd2 = (J(m0 + h * d) - J(m0 - h * d)) / (2 * h)
+                errs2.append(abs(fd2 - gd) / abs(gd))
+                # fourth-order stencil at a 10x larger step, as a cross-check
+                h4 = 1e-4 * np.linalg.norm(m0) / np.linalg.norm(d)
+                jp, jm = J(m0 + h4 * d), J(m0 - h4 * d)
+                jp2, jm2 = J(m0 + 2 * h4 * d), J(m0 - 2 * h4 * d)
+                fd4 = (-jp2 + 8 * jp - 8 * jm + jm2) / (12 * h4)
+                errs4.append(abs(fd4 - gd) / abs(gd))
+            res["fd_check_fp64_max_rel_err"] = float(max(errs2))
+            res["fd_stencil"] = "central (J(m+hd) - J(m-hd)) / 2h, h = 1e-5 ||m|| / ||d||"
+            res["fd4_check_fp64_max_rel_err"] = float(max(errs4))
+            res["fd4_stencil"] = "4th-order central, h = 1e-4 ||m|| / ||d||"
             res["fd_directions"] = ndir
-    res["fp32_vs_fp64_gradient_rel_l2"] = float(np.linalg.norm(grads["fp32"] - grads["fp64"]) /
-                                               np.linalg.norm(grads["fp64"]))
+    res["fp32_vs_fp64_gradient_rel_l2"] = float(np.linalg.norm(grads["fp32_g64"] - grads["fp64"])
+                                               / np.linalg.norm(grads["fp64"]))
+    res["precision"] = ("fp32_g64: FP32 state, FP64 geometric factors, FP64 gradient "
+                        "accumulation; fp64: all FP64")
     for P, _ in solvers.values():
         P.close()
     return res
 
 
-def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028, tune=True):
+def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028, tune=True,
+                prec="fp32_g64"):
     """64 independent 3-D acoustic problems of uneven size and order (edge
     k ~ U{kmin..kmax}, order ~ U{omin..omax}, smooth velocity 2 + sum of 4
-    Gaussians), FP32, one batch: per-colour launches over all problems, step
-    loop in a HIP graph, launch order picked by the autotuner."""
+    Gaussians), FP32 state with FP64 geometric factors, one batch: per-colour
+    launches over all problems, step loop in a HIP graph, launch order picked
+    by the autotuner."""
     import montjoie_amd as M
     from montjoie_amd import tuner
     from montjoie_amd.batch import Batch
     t_setup = time.perf_counter()
-    probs, dts, ndof = [], [], 0
+    probs, dts, ndof, nbytes = [], [], 0, 0
     for i in range(nprob):
         k = kmin + int(_unif(seed, 2 * i) * (kmax - kmin + 1))
         r = omin + int(_unif(seed, 2 * i + 1) * (omax - omin + 1))
         m = M.regular_mesh(3, k, r, 0.0, 0)
         c = 2.0 + 1.5 * (2 * smooth_field(m, seed * 1000 + i) - 1) * 0.5
-        P = M.WaveProblem(m, 1.0, c * c, precision="fp32")
+        P = M.WaveProblem(m, 1.0, c * c, precision=prec)
         P.set_sources([(0.5, 0.5, 0.5)])
         P.set_receivers(receivers(4, seed + i))
         dt = 0.5 * 2.0 / np.sqrt(P.lambda_max(30, 3))
@@ -295,6 +335,7 @@ def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028,
         probs.append(P)
         dts.append(dt)
         ndof += m.N
+        nbytes += algorithmic_bytes_per_step(m.E, m.nd, m.N, 4, 8)
         del m
         if i % 16 == 15:
             print(f"# batch setup: {i + 1}/{nprob} problems, {ndof} dof", file=sys.stderr, flush=True)
@@ -305,7 +346,7 @@ def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028,
     tuned = tuner.tune_batch(B, steps=4) if tune else None
     for P, dt in zip(probs, dts):
         P.leapfrog_init(dt)
-    B.advance(16)  # warm-up (graph capture)
+    B.advance(steps)  # warm-up: graph capture and instantiation, traces buffers
     for P, dt in zip(probs, dts):
         P.leapfrog_init(dt)
     torch.cuda.synchronize()
@@ -313,6 +354,7 @@ def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028,
     B.advance(steps, graph=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    dev_ms = B.ms_per_step()
     # the same problems one after the other (per-problem graphs)
     t1 = time.perf_counter()
     for P, dt in zip(probs, dts):
@@ -324,10 +366,17 @@ def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028,
     B.close()
     for P in probs:
         P.close()
+    achieved = nbytes / (dev_ms * 1e-3) / 1e9
     return {"problems": nprob, "total_dof": ndof, "steps": steps, "orders": orders,
+            "precision": "FP32 state, FP64 geometric factors",
             "batch_s": wall, "batch_gdof_steps_s": ndof * steps / wall / 1e9,
             "one_by_one_s": wall_seq, "one_by_one_gdof_steps_s": ndof * steps / wall_seq / 1e9,
-            "setup_s": t_setup, "autotuned": tuned and {"order_mode": tuned["order_mode"]}}
+            "setup_s": t_setup, "autotuned": tuned and {"order_mode": tuned["order_mode"]},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "note": "whole step (8 colour launches over all problems + tails), "
+                                 "stored-geometry algorithmic bytes summed over the problems",
+                         "algorithmic_bytes_per_step": nbytes, "device_ms_per_step": dev_ms}}
 
 
 def bench_host_apply(m, mu, reps=5):
@@ -367,12 +416,20 @@ def sphere_dirs(n, seed=2027):
     return np.array(out)
 
 
+def elastic_bytes_per_step(E, nd, N, nshots, state_bytes, geom_bytes=8):
+    """Bytes per step of the elastic 16-shot pass, no cache reuse credited: per
+    element node the stored geometry (J w and J DF^{-1}, 10 values) and the map;
+    per node and shot three state streams of 3 components, and M^{-1}."""
+    return E * nd * (10 * geom_bytes + 4) + N * (3 * 3 * nshots * state_bytes + 8)
+
+
 def bench_elastic(nx=32, order=4, steps=800, nshots=16, warmup=10):
     """Isotropic elastodynamics, Q4 hexahedra nx^3 (vertices perturbed <= 0.1h,
     seed 2024), 3 layers lambda, mu ~ U[1,4], rho ~ U[1,2] (seed 2026);
     16 Dirac shots at U[0.25,0.75]^3 with directions uniform on the sphere
-    (seed 2027), Ricker f0 = 8, dt = 0.5 CFL.  FP32 16-shot batch against
-    the FP64 path (batch for the error, one shot for the shot-by-shot time)."""
+    (seed 2027), Ricker f0 = 8, dt = 0.5 CFL, 800 steps.  FP32 16-shot batch
+    (FP32 state, FP64 geometric factors; all-FP32 beside it) against the FP64
+    path (batch for the error, one shot for the shot-by-shot time)."""
     import montjoie_amd as M
     from montjoie_amd import tuner
     m = M.regular_mesh(3, nx, order, 0.1, 2024)
@@ -386,51 +443,61 @@ def bench_elastic(nx=32, order=4, steps=800, nshots=16, warmup=10):
     dt = 0.5 * 2.0 / np.sqrt(P.lambda_max(60, 1))
     P.close()
     res = {}
-    for prec, ns in (("fp32", nshots), ("fp64", nshots), ("fp64_1", 1)):
-        P = M.WaveProblem(m, rho, mu, lam=lam, ncomp=3, nshots=ns,
-                          precision=prec.split("_")[0])
+    for key, prec, ns in (("fp32_g64", "fp32_g64", nshots), ("fp32", "fp32", nshots),
+                          ("fp64", "fp64", nshots), ("fp64_1", "fp64", 1)):
+        P = M.WaveProblem(m, rho, mu, lam=lam, ncomp=3, nshots=ns, precision=prec)
         P.set_sources(srcs[:ns], pol=pol[:ns], shot=np.arange(ns))
         P.set_receivers(rec, comp)
-        P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, warmup + steps + 64))
-        tuner.tune_problem(P, steps=4, dt=dt)  # launch shape only: result-neutral
-        P.leapfrog_init(dt)
-        if warmup:
-            P.advance(warmup, traces=False)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        tr = P.advance(steps, traces=True)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+        P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, max(warmup, steps, 16) + 64))
+        tuned = tuner.tune_problem(P, steps=4, dt=dt)  # launch shape only: result-neutral
+        wall, tr = timed_advance(P, dt, steps, warmup)
         # FP64 energy of the state after the timed steps (the pulse ends at
         # t = 2.5/f0, long before), and 100 untimed steps later: conserved
         uN = P.state()[1]
         e1 = P.energy()
         P.advance(100, traces=False)
         e2 = P.energy()
-        res[prec] = dict(wall=wall, traces=tr, uN=uN, e1=e1, e2=e2)
+        prof = P.profile(4) if key == "fp32_g64" else None
+        res[key] = dict(wall=wall, traces=tr, uN=uN, e1=e1, e2=e2, prof=prof, tile=tuned["tile"],
+                        ncolors=P.info()["ncolors"])
         P.close()
-        print(f"# elastic {prec}: {res[prec]['wall']:.3f} s", file=sys.stderr, flush=True)
+        print(f"# elastic {key}: {res[key]['wall']:.3f} s", file=sys.stderr, flush=True)
     ndof = 3 * m.N
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
     t64_path = res["fp64_1"]["wall"] * nshots
+    head = res["fp32_g64"]
+    launch_ms = head["prof"]["stiffness_launch_ms"]
+    launch_bytes = elastic_bytes_per_step(m.E, m.nd, m.N, nshots, 4) / head["ncolors"]
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
     return {
         "workload": f"elastic Q{order} {nx}^3, 3 layers, {nshots} shots, {steps} steps",
         "n_dof": ndof, "dt": dt,
-        "fp32_batch_s": res["fp32"]["wall"],
+        "precision": "FP32 state, FP64 geometric factors; energy in FP64",
+        "fp32_g64_batch_s": head["wall"],
+        "fp32_g64_batch_gdof_steps_s": ndof * steps * nshots / head["wall"] / 1e9,
         "fp32_batch_gdof_steps_s": ndof * steps * nshots / res["fp32"]["wall"] / 1e9,
         "fp64_batch_gdof_steps_s": ndof * steps * nshots / res["fp64"]["wall"] / 1e9,
         "fp64_one_shot_gdof_steps_s": ndof * steps / res["fp64_1"]["wall"] / 1e9,
-        "time_to_solution_ratio_fp64_shot_by_shot_over_fp32_batch": t64_path / res["fp32"]["wall"],
-        "fp32_rel_l2_state_vs_fp64": rel(res["fp32"]["uN"], res["fp64"]["uN"]),
+        "time_to_solution_ratio_fp64_shot_by_shot_over_fp32_g64_batch": t64_path / head["wall"],
+        "time_to_solution_ratio_fp64_shot_by_shot_over_fp32_batch":
+            t64_path / res["fp32"]["wall"],
+        "fp32_g64_rel_l2_state_vs_fp64": rel(head["uN"], res["fp64"]["uN"]),
+        "fp32_g64_rel_l2_traces_vs_fp64": rel(head["traces"], res["fp64"]["traces"]),
         "fp32_rel_l2_traces_vs_fp64": rel(res["fp32"]["traces"], res["fp64"]["traces"]),
         "energy_fp64_check": {
-            "fp32_vs_fp64_max_rel": float(np.max(np.abs(res["fp32"]["e1"] - res["fp64"]["e1"])
-                                                 / res["fp64"]["e1"])),
-            "fp32_drift_100_steps_max_rel": float(np.max(np.abs(res["fp32"]["e2"] - res["fp32"]["e1"])
-                                                         / res["fp32"]["e1"])),
-            "fp64_drift_100_steps_max_rel": float(np.max(np.abs(res["fp64"]["e2"] - res["fp64"]["e1"])
+            "fp32_g64_vs_fp64_max_rel": float(np.max(np.abs(head["e1"] - res["fp64"]["e1"])
+                                                     / res["fp64"]["e1"])),
+            "fp32_g64_drift_100_steps_max_rel": float(np.max(np.abs(head["e2"] - head["e1"])
+                                                             / head["e1"])),
+            "fp64_drift_100_steps_max_rel": float(np.max(np.abs(res["fp64"]["e2"] -
+                                                                res["fp64"]["e1"])
                                                          / res["fp64"]["e1"])),
         },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": f"k_shots_el (tile {head['tile']}, one colour pass, FP32 state, "
+                               "FP64 geometry)",
+                     "algorithmic_bytes_per_launch": launch_bytes, "launch_ms": launch_ms},
     }
 
 
@@ -512,6 +579,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=48)
+    ap.add_argument("--acc-steps", type=int, default=500,
+                    help="step count of the untimed accuracy runs (the workload's 500)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32_g64", "fp32"], default=None,
                     help="run one precision only (profiling runs)")
@@ -565,7 +634,8 @@ def main():
         print("# calibration: 3 x 1 GiB device copies", file=sys.stderr, flush=True)
     res = {}
     for prec in ((a.only,) if a.only else ("fp64", "fp32_g64", "fp32")):
-        res[prec] = run_gpu(m, mu, dt, prec, a.steps, a.warmup, tile=a.tile, nxcd=a.xcd)
+        res[prec] = run_gpu(m, mu, dt, prec, a.steps, a.warmup, tile=a.tile, nxcd=a.xcd,
+                            acc_steps=a.acc_steps)
         r = res[prec]
         print(f"# {prec}: {r['wall'] / a.steps * 1e3:.4f} ms/step wall, "
               f"{r['dev_ms_step']:.4f} ms/step device, stiffness launch "
@@ -581,15 +651,23 @@ def main():
                                 np.linalg.norm(ref["traces"]))
         r["err_energy"] = float(abs(r["energy"][0] - ref["energy"][0]) / abs(ref["energy"][0]))
     gdof = lambda r: m.N * a.steps / r["wall"] / 1e9  # noqa: E731
-    head = res["fp32"]
+    ms = lambda r: r["wall"] / a.steps * 1e3  # noqa: E731
+    head = res["fp32_g64"]
     nd = 216
     ncol = head["info"]["ncolors"]
-    bytes_step = algorithmic_bytes_per_step(m.E, nd, m.N, 4, 4)
-    launch_bytes = bytes_step / ncol
-    achieved = launch_bytes / (head["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9
     tri = head["geometry"] == "trilinear"
-    kernel_bytes = (trilinear_bytes_per_step(m.E, nd, m.N, 4, 4) if tri else bytes_step) / ncol
-    traffic = measured_traffic() if (a.nx == 48 and tri) else None
+    launch_ms = head["prof"]["stiffness_launch_ms"]
+    # bytes per step, no cache reuse credited: the stored-geometry algorithm
+    # (per-node FP64 tensor + map + 4 FP32 state streams) and this kernel's own
+    # (24 FP64 trilinear coefficients per element in place of the tensor)
+    bytes_step = algorithmic_bytes_per_step(m.E, nd, m.N, 4, 8)
+    launch_bytes = bytes_step / ncol
+    achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    kernel_bytes = (trilinear_bytes_per_step(m.E, nd, m.N, 4, 8) if tri else bytes_step) / ncol
+    traffic = (measured_traffic("stiffness_q5_48cube_g64_counters.md")
+               if (a.nx == 48 and tri) else None)
+    flops = flops_per_step(m.E, 6)
+    tflops = lambda r: flops / (r["wall"] / a.steps) / 1e12  # noqa: E731
     out = {
         "metric": "GDoF-steps/s, Q5 hex acoustic, FP32 vs FP64 at rel-L2 err; gradient/forward cost",
         "value": gdof(head),
@@ -597,40 +675,55 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": head["wall"] / a.steps * 1e3,
+        "ms_per_step": ms(head),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32 state, fp64 geometric factors",
         "data": "synthetic",
         "config": {"workload": f"3-D acoustic Q5 hexahedra {a.nx}^3 unit cube, vertices "
                                "perturbed <=0.1h (seed 2024), 3 layers c~U[1,3] (seed 2025), "
-                               "Dirac centre, Ricker f0=8, dt=0.5 CFL",
+                               "Dirac centre, Ricker f0=8, dt=0.5 CFL, 16 receivers",
                    "n_dof": m.N, "n_elem": m.E, "order": 5, "dt": dt, "shots": 1,
+                   "precision": "FP32 state with FP64 geometric factors (MJW_FP32_G64); "
+                                "energy and traces accumulated in FP64",
                    "parallelism": "single GPU"},
-        "fp64": {"value": gdof(ref), "ms_per_step": ref["wall"] / a.steps * 1e3},
-        "fp32_g64": {"value": gdof(res["fp32_g64"]),
-                     "ms_per_step": res["fp32_g64"]["wall"] / a.steps * 1e3,
-                     "rel_l2_state_vs_fp64": res["fp32_g64"]["err_state"],
-                     "rel_l2_traces_vs_fp64": res["fp32_g64"]["err_traces"]},
-        "fp32_rel_l2_state_vs_fp64": head["err_state"],
-        "fp32_rel_l2_traces_vs_fp64": head["err_traces"],
-        "energy_fp64_check": {"fp64": float(ref["energy"][0]), "fp32": float(head["energy"][0]),
-                              "fp32_rel_diff": head["err_energy"],
-                              "fp32_g64_rel_diff": res["fp32_g64"]["err_energy"]},
-        "fp32_over_fp64_speedup": gdof(head) / gdof(ref),
+        "accuracy": {"steps": a.acc_steps,
+                     "note": "untimed runs of the full step count from a zero state, each "
+                             "precision against the FP64 GPU path (which matches Montjoie at "
+                             "<= 1e-10 on the golden fixtures)",
+                     "rel_l2_traces_vs_fp64": head["err_traces"],
+                     "rel_l2_state_vs_fp64": head["err_state"],
+                     "energy_rel_diff_vs_fp64": head["err_energy"],
+                     "fp64_energy": float(ref["energy"][0])},
+        "flops": {"per_step": flops, "note": "algorithmic count: 2*3*n per node and "
+                  "direction for the gradient and the divergence, 18 for the tensor, 6 for "
+                  "the update (no geometry rebuild counted)",
+                  "achieved_TFLOPs": tflops(head), "peak_TFLOPs_fp32": 157.3,
+                  "frac_fp32": tflops(head) / 157.3,
+                  "fp64_achieved_TFLOPs": tflops(ref), "peak_TFLOPs_fp64": 78.6,
+                  "fp64_frac": tflops(ref) / 78.6},
+        "fp64": {"value": gdof(ref), "ms_per_step": ms(ref)},
+        "fp32": {"value": gdof(res["fp32"]), "ms_per_step": ms(res["fp32"]),
+                 "note": "FP32 geometric factors too (narrower than the workload asks)",
+                 "rel_l2_state_vs_fp64": res["fp32"]["err_state"],
+                 "rel_l2_traces_vs_fp64": res["fp32"]["err_traces"]},
+        "fp32_g64_over_fp64_speedup": gdof(head) / gdof(ref),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "geometry": head["geometry"],
-                     "kernel_bytes_per_launch": kernel_bytes,
-                     "kernel_bytes_frac": kernel_bytes / (head["prof"]["stiffness_launch_ms"] * 1e-3)
-                     / 1e9 / HBM_PEAK_GBS,
-                     "traffic_GBs": None if traffic is None else
-                     traffic / (head["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9,
-                     "kernel": kernel_name(head["tile"], tri) + " (one colour pass)",
-                     "tile": head["tile"], "xcd_order": head["nxcd"],
                      "algorithmic_bytes_per_launch": launch_bytes,
-                     "launch_ms": head["prof"]["stiffness_launch_ms"],
+                     "algorithmic_bytes_note": "per colour pass: E/8 elements x 216 nodes x "
+                                               "(6 x 8 B tensor + 4 B map) + N/8 nodes x 4 x 4 B "
+                                               "state (stored-geometry algorithm)",
+                     "own_bytes_per_launch": kernel_bytes,
+                     "own_bytes_frac": kernel_bytes / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic_GBs": None if traffic is None else
+                     traffic / (launch_ms * 1e-3) / 1e9,
+                     "kernel": kernel_name(head["tile"], tri, t="float").replace(
+                         "float, float", "float, double") + " (one colour pass)",
+                     "tile": head["tile"], "xcd_order": head["nxcd"],
+                     "launch_ms": launch_ms,
                      "fp64_launch_ms": ref["prof"]["stiffness_launch_ms"],
                      "fp64_achieved_GBs": algorithmic_bytes_per_step(m.E, nd, m.N, 8, 8) / ncol /
                      (ref["prof"]["stiffness_launch_ms"] * 1e-3) / 1e9},

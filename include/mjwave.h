@@ -150,22 +150,43 @@ int mjw_apply_kh_dev(mjw_problem *p, double alpha, const void *U, double beta, v
  */
 int mjw_node_order(const mjw_problem *p, int32_t *solver_index);
 
-/* Y = beta*Y + alpha*D*U, D = lumped mass with Dirichlet rows 1
- * (ApplyOperatorDh, ContinuousUnsteadyMassMatrix.cxx:1329-1346). */
+/*
+ * Diagonal (lumped) mass operators on one host vector of n_nodes*ncomp FP64
+ * values, caller's numbering (ContinuousUnsteadyMassMatrix.cxx:1309-1420,
+ * matrices built by ContinuousMassMatrixAcoustic::Init,
+ * Hyperbolic/Acoustic/TimeAcoustic.cxx:929-1004).  D = lumped mass with
+ * Dirichlet rows 1; S = lumped damping sigma J w (mjw_set_damping; 0 without
+ * damping and on Dirichlet rows).  beta = 0 overwrites Y.
+ *   mjw_apply_dh:             Y = beta*Y + alpha*D*U          (ApplyOperatorDh)
+ *   mjw_apply_sh:             Y = beta*Y + alpha*S*U          (ApplyOperatorSh)
+ *   mjw_apply_dh_minus_dt_sh: Y = beta*Y + alpha*(D - dt/2 S)*U (ApplyOperatorDhMinusdtSh)
+ *   mjw_solve_dh:             Y = D^{-1} Y                    (SolveOperatorDh)
+ *   mjw_solve_dh_plus_dt_sh:  Y = (D + dt/2 S)^{-1} Y         (SolveOperatorDhPlusdtSh)
+ * dt is the time step the reference bakes into DhMinusdtSh / DhPlusdtSh.
+ */
 int mjw_apply_dh(mjw_problem *p, double alpha, const double *U, double beta, double *Y);
-/* Y = D^{-1} Y (SolveOperatorDh / SolveOperatorDhPlusdtSh with S_h = 0,
- * ContinuousUnsteadyMassMatrix.cxx:1376-1420). */
+int mjw_apply_sh(mjw_problem *p, double alpha, const double *U, double beta, double *Y);
+int mjw_apply_dh_minus_dt_sh(mjw_problem *p, double dt, double alpha, const double *U,
+                             double beta, double *Y);
 int mjw_solve_dh(mjw_problem *p, double *Y);
+int mjw_solve_dh_plus_dt_sh(mjw_problem *p, double dt, double *Y);
+/* Homogeneous Dirichlet condition: Y = 0 on the Dirichlet rows of every
+ * component (SetDirichletCondition / SetDirichletConditionSource with no
+ * inhomogeneous Dirichlet data, VarInstationary.cxx:2466-2475,
+ * VarInstationaryInline / ImposeNullDirichletCondition). */
+int mjw_set_dirichlet_condition(mjw_problem *p, double *Y);
 /* lumped mass diagonal (n_nodes doubles, Dirichlet rows 1) */
 int mjw_get_mass(mjw_problem *p, double *D);
 
 /* ------------------------------------------------------ sources, receivers */
 /*
  * Point (Dirac) sources: for source k, shot index shot[k] (NULL: all 0),
- * position pts[k*dim], polarisation pol[k*ncomp] (NULL: 1).  b_i = pol
- * phi_i(F^{-1}(x)) averaged over the elements containing x, Dirichlet rows
- * dropped, |b_i| <= eps ||b|| dropped (AddDiracSourceGen,
- * Source/DefineSourceElliptic.cxx:2439-2725; VarInstationary.cxx:620-675).
+ * position pts[k*dim], polarisation pol[k*ncomp] (NULL: 1).  Each point adds
+ * pol phi_i(F^{-1}(x)) averaged over the elements containing x to the
+ * source vector F of its shot; then F's Dirichlet rows and its entries with
+ * |F_i| <= eps ||F|| are dropped (AddDiracSourceGen,
+ * Source/DefineSourceElliptic.cxx:2439-2725; ComputeRightHandSide,
+ * VarInstationary.cxx:620-675).  Points of one shot may share nodes.
  */
 int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, const double *pol,
                     const int32_t *shot);
@@ -175,6 +196,30 @@ int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, const double *p
  * VarInstationary.cxx:2394).  table[nsteps] FP64, same for all shots.
  */
 int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table);
+/*
+ * Time function of the sources for the operator-level entry below
+ * (GetNewTimeSource + InitSource, VarInstationary.cxx:409-420,2359-2365):
+ * MJW_TIME_RICKER is the Ricker primitive interpolated by the degree-24
+ * Gauss-Lobatto polynomial on windows of length 0.5/f0 from t_init, moving
+ * forward with the evaluation time (TimeSourceHyperbolic, Source/
+ * TimeSource.cxx:776-880); eps_time is the window tolerance, Montjoie's
+ * (final_time - initial_time)*1e-15 (VarInstationary.cxx:993); the source is
+ * cut off from t = 2.5/f0 on.  MJW_TIME_NONE removes it.
+ */
+#define MJW_TIME_NONE 0
+#define MJW_TIME_RICKER 1
+int mjw_set_time_source(mjw_problem *p, int kind, double f0, double t_init, double eps_time);
+/*
+ * Y += alpha * f^(nb_deriv)(t) * F_shot on one host vector (n_nodes*ncomp,
+ * caller's numbering), F_shot = the sparse source vector of that shot
+ * (mjw_set_sources); nothing for t >= 2.5/f0 (AddScalarTimeSource ->
+ * AddScalarSourceAtTime, VarInstationary.cxx:2391-2420,2509-2513).
+ * nb_deriv = 0 is the primitive, 1 the pulse (what Advance adds), up to 9.
+ * Calls must come with non-decreasing t, as in the reference (the windows
+ * move forward only).
+ */
+int mjw_add_scalar_time_source(mjw_problem *p, int shot, double alpha, double t, int nb_deriv,
+                               double *Y);
 /* Gauss-Lobatto rule on [0,1] of `order` (order+1 points, weights sum 1):
  * Globatto::ConstructQuadrature(r, QUADRATURE_LOBATTO)
  * (Quadrature/GaussLobatto.cxx:29-75).  Host only, FP64, order 1..24. */

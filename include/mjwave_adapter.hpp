@@ -3,9 +3,10 @@
 //
 //  * the operator system VirtualOdeSystem<T> (Instationary/TimeSchemes.hxx:35-151,
 //    implemented by VarInstationary_Base, VarInstationaryInline.cxx:174-242):
-//    ApplyOperatorKh / ApplyOperatorDh / ApplyOperatorDhMinusdtSh /
-//    SolveOperatorDh / SolveOperatorDhPlusdtSh / AddScalarTimeSource /
-//    SetDirichletCondition;
+//    ApplyOperatorKh / ApplyOperatorDh / ApplyOperatorSh /
+//    ApplyOperatorDhMinusdtSh / SolveOperatorDh / SolveOperatorDhPlusdtSh /
+//    AddScalarTimeSource / SetDirichletCondition(Source), each on one host
+//    vector (damping included: S = sigma J w after SetDamping);
 //  * the leapfrog time scheme ModifiedEquationIterator<T>
 //    (Instationary/ModifiedEquation.hxx:78-114, .cxx:99-124,295-345):
 //    SetInitialConditionS / Advance / GetIterate / GetCFL / GetNumberOfIterates.
@@ -70,34 +71,62 @@ class HipWaveSystem {
   mjw_problem *handle() const { return p_; }
   int64_t GetNbDof() const { return ndof_; }
 
+  // Time step baked into D - dt/2 S and (D + dt/2 S)^{-1}, as Montjoie's
+  // mass matrices take it at Init (TimeAcoustic.cxx:977-1004).
+  void SetTimeStep(double dt) { dt_ = dt; }
+  double GetTimeStep() const { return dt_; }
+  // Damping sigma per element (MateriauDielec ... sigma): S_h = sigma J w
+  void SetDamping(const std::vector<double> &sigma) {
+    check(mjw_set_damping(p_, sigma.data()), p_, "SetDamping");
+  }
+
   // Y = beta Y + alpha K U, K = -A  (ApplyOperatorKh)
   template <class V>
   void ApplyOperatorKh(double alpha, double /*t*/, const V &U, double beta, V &Y) {
     check(mjw_apply_kh(p_, alpha, U.data(), beta, Y.data()), p_, "ApplyOperatorKh");
   }
-  // Y = beta Y + alpha D U  (ApplyOperatorDh; D - dt/2 S = D since S = 0)
+  // Y = beta Y + alpha D U  (ApplyOperatorDh)
   template <class V>
   void ApplyOperatorDh(double alpha, double /*t*/, const V &U, double beta, V &Y) {
     check(mjw_apply_dh(p_, alpha, U.data(), beta, Y.data()), p_, "ApplyOperatorDh");
   }
+  // Y = beta Y + alpha S U  (ApplyOperatorSh)
   template <class V>
-  void ApplyOperatorDhMinusdtSh(double alpha, double t, const V &U, double beta, V &Y) {
-    ApplyOperatorDh(alpha, t, U, beta, Y);
+  void ApplyOperatorSh(double alpha, double /*t*/, const V &U, double beta, V &Y) {
+    check(mjw_apply_sh(p_, alpha, U.data(), beta, Y.data()), p_, "ApplyOperatorSh");
   }
-  // Y = D^{-1} Y  (SolveOperatorDh / SolveOperatorDhPlusdtSh with S = 0)
+  // Y = beta Y + alpha (D - dt/2 S) U  (ApplyOperatorDhMinusdtSh)
+  template <class V>
+  void ApplyOperatorDhMinusdtSh(double alpha, double /*t*/, const V &U, double beta, V &Y) {
+    check(mjw_apply_dh_minus_dt_sh(p_, dt_, alpha, U.data(), beta, Y.data()), p_,
+          "ApplyOperatorDhMinusdtSh (SetTimeStep first)");
+  }
+  // Y = D^{-1} Y  (SolveOperatorDh)
   template <class V>
   void SolveOperatorDh(V &Y) {
     check(mjw_solve_dh(p_, Y.data()), p_, "SolveOperatorDh");
   }
+  // Y = (D + dt/2 S)^{-1} Y  (SolveOperatorDhPlusdtSh)
   template <class V>
   void SolveOperatorDhPlusdtSh(V &Y) {
-    SolveOperatorDh(Y);
+    check(mjw_solve_dh_plus_dt_sh(p_, dt_, Y.data()), p_,
+          "SolveOperatorDhPlusdtSh (SetTimeStep first)");
   }
-  // homogeneous Dirichlet: zero the Dirichlet rows (SetDirichletCondition)
+  // Y += alpha f^(nb_deriv)(t) F  (AddScalarTimeSource; shot 0)
   template <class V>
-  void SetDirichletCondition(double /*t*/, int /*n*/, V &Y, const std::vector<uint8_t> &dir) {
-    for (size_t i = 0; i < Y.size(); ++i)
-      if (dir[i % dir.size()]) Y[i] = 0.0;
+  void AddScalarTimeSource(double alpha, double t, int nb_deriv, V &Y) {
+    check(mjw_add_scalar_time_source(p_, 0, alpha, t, nb_deriv, Y.data()), p_,
+          "AddScalarTimeSource");
+  }
+  // homogeneous Dirichlet: zero the Dirichlet rows (SetDirichletCondition,
+  // SetDirichletConditionSource)
+  template <class V>
+  void SetDirichletCondition(double /*t*/, int /*nb_deriv*/, V &Y) {
+    check(mjw_set_dirichlet_condition(p_, Y.data()), p_, "SetDirichletCondition");
+  }
+  template <class V>
+  void SetDirichletConditionSource(double /*t*/, int /*nb_deriv*/, V &Y) {
+    check(mjw_set_dirichlet_condition(p_, Y.data()), p_, "SetDirichletConditionSource");
   }
 
   // Dirac sources with a Ricker pulse (SRC_DIRAC + TemporalSource = RICKER)
@@ -108,10 +137,15 @@ class HipWaveSystem {
                           shot.empty() ? nullptr : shot.data()),
           p_, "SetDiracSources");
   }
+  // TemporalSource = RICKER for both paths: the pulse table of the fused
+  // device loop and the time function of AddScalarTimeSource (window
+  // tolerance (final - initial time) * 1e-15, VarInstationary.cxx:993)
   void SetRickerPulse(double f0, double t_begin, double dt, int nsteps) {
     std::vector<double> tab(nsteps);
     check(mjw_ricker_pulse_table(f0, t_begin, dt, nsteps, tab.data()), nullptr, "Ricker");
     check(mjw_set_pulse(p_, nsteps, tab.data()), p_, "SetRickerPulse");
+    check(mjw_set_time_source(p_, MJW_TIME_RICKER, f0, t_begin, (nsteps * dt) * 1e-15), p_,
+          "SetRickerPulse");
   }
   void SetReceivers(const std::vector<double> &pts, int dim, const std::vector<int32_t> &comp = {}) {
     const int nrec = (int)(pts.size() / dim);
@@ -134,6 +168,7 @@ class HipWaveSystem {
   int ncomp_, nshots_;
   int64_t ndof_;
   int nrec_ = 0;
+  double dt_ = 0.0;
 };
 
 // Order-2 leapfrog (ModifiedEquationIterator) whose iterates live on the
@@ -150,11 +185,13 @@ class HipLeapFrogScheme {
                             du0.size() ? du0.data() : nullptr),
           sys_.handle(), "SetInitialConditionS");
     dt_ = dt;
+    sys_.SetTimeStep(dt);
   }
   void SetInitialConditionZero(double t, double dt) {
     check(mjw_leapfrog_init(sys_.handle(), t, dt, nullptr, nullptr), sys_.handle(),
           "SetInitialConditionS");
     dt_ = dt;
+    sys_.SetTimeStep(dt);
   }
   // one step (Advance); traces of the iterate before the step appended
   void Advance(double /*t*/, int /*n*/) { AdvanceN(1, nullptr); }

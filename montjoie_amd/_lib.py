@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libmjwave.so")
 MJW_OK = 0
 MJW_FP64, MJW_FP32, MJW_FP32_G64 = 0, 1, 2
 MJW_GEOM_STORED, MJW_GEOM_TRILINEAR = 0, 1
+MJW_TIME_NONE, MJW_TIME_RICKER = 0, 1
 GEOMETRY_MODES = {"stored": MJW_GEOM_STORED, "trilinear": MJW_GEOM_TRILINEAR}
 PRECISIONS = {"fp64": MJW_FP64, "fp32": MJW_FP32, "fp32_g64": MJW_FP32_G64}
 
@@ -41,6 +42,12 @@ SIGNATURES = {
     "mjw_apply_kh_dev": (C.c_int, [_vp, C.c_double, _vp, C.c_double, _vp]),
     "mjw_apply_dh": (C.c_int, [_vp, C.c_double, _dp, C.c_double, _dp]),
     "mjw_solve_dh": (C.c_int, [_vp, _dp]),
+    "mjw_apply_sh": (C.c_int, [_vp, C.c_double, _dp, C.c_double, _dp]),
+    "mjw_apply_dh_minus_dt_sh": (C.c_int, [_vp, C.c_double, C.c_double, _dp, C.c_double, _dp]),
+    "mjw_solve_dh_plus_dt_sh": (C.c_int, [_vp, C.c_double, _dp]),
+    "mjw_set_dirichlet_condition": (C.c_int, [_vp, _dp]),
+    "mjw_set_time_source": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double]),
+    "mjw_add_scalar_time_source": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, _dp]),
     "mjw_get_mass": (C.c_int, [_vp, _dp]),
     "mjw_set_sources": (C.c_int, [_vp, C.c_int, _dp, _dp, _i32p]),
     "mjw_set_pulse": (C.c_int, [_vp, C.c_int, _dp]),

@@ -38,20 +38,28 @@ def _run(cmd):
     return r.stdout
 
 
-def build(verbose: bool = False, jobs: int = 0) -> str:
+def build(verbose: bool = False, jobs: int = 0, incremental: bool = False) -> str:
+    """Compile and link; with incremental=True an object newer than its source
+    and than every header of csrc/ and include/ is kept."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
+    hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    hdrs += [os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE)]
+    newest_hdr = max(os.path.getmtime(h) for h in hdrs)
     cmds = []
     objs = []
     for f in srcs:
         src = os.path.join(CSRC, f)
         obj = os.path.join(OBJ, f + ".o")
         objs.append(obj)
+        if (incremental and os.path.exists(obj) and
+                os.path.getmtime(obj) > max(os.path.getmtime(src), newest_hdr)):
+            continue
         if f.endswith(".hip"):
             cmds.append([HIPCC, *HIP_FLAGS, "-c", src, "-o", obj])
         else:
             cmds.append(["g++", *CXX_FLAGS, "-c", src, "-o", obj])
-    jobs = jobs or min(len(cmds), max(1, (os.cpu_count() or 4)))
+    jobs = jobs or min(max(len(cmds), 1), max(1, (os.cpu_count() or 4)))
     with ThreadPoolExecutor(jobs) as ex:
         outs = list(ex.map(_run, cmds))
     if verbose:
@@ -70,4 +78,4 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="-i" in sys.argv))

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <numeric>
 #include <string>
 #include <unordered_map>
@@ -88,6 +89,14 @@ struct mjw_problem {
   double *d_pulse = nullptr;
   int64_t npulse = 0;
   int nsrc = 0;
+  // host copy of the sparse source vectors (caller's numbering m*N + g)
+  std::vector<int64_t> h_src_dof;
+  std::vector<int32_t> h_src_shot;
+  std::vector<double> h_src_val;
+  // time function of the operator-level source entry (mjw_set_time_source)
+  GlTimeSource tsrc;
+  bool has_tsrc = false;
+  double tlimit = 0.0;
   int64_t *d_src_dof = nullptr;
   int32_t *d_src_shot = nullptr;
   double *d_src_val = nullptr;
@@ -748,14 +757,44 @@ extern "C" int mjw_apply_kh(mjw_problem *p, double alpha, const double *U, doubl
   return MJW_OK;
 }
 
+// S_h of node g in the caller's order; 0 on Dirichlet rows (S_h is assembled
+// with Dirichlet coefficient 0, TimeAcoustic.cxx:966-971)
+static double damp_at(const mjw_problem *p, int64_t g) {
+  if (p->damp_S.empty() || p->dir[g]) return 0.0;
+  return p->damp_S[p->nperm[g]];
+}
+
+// Y = beta*Y + alpha*diag(c)*U over the ncomp*N entries (beta = 0 overwrites)
+template <class F>
+static void diag_apply(const mjw_problem *p, double alpha, const double *U, double beta,
+                       double *Y, F coef) {
+  const int64_t n = (int64_t)p->ncomp * p->N;
+  for (int64_t i = 0; i < n; ++i) {
+    const double d = alpha * (coef(i % p->N) * U[i]);
+    Y[i] = beta == 0.0 ? d : beta * Y[i] + d;
+  }
+}
+
 extern "C" int mjw_apply_dh(mjw_problem *p, double alpha, const double *U, double beta,
                             double *Y) {
   if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_dh: NULL");
-  const int64_t n = (int64_t)p->ncomp * p->N;
-  for (int64_t i = 0; i < n; ++i) {
-    const double d = alpha * (p->mass[i % p->N] * U[i]);
-    Y[i] = beta == 0.0 ? d : beta * Y[i] + d;
-  }
+  diag_apply(p, alpha, U, beta, Y, [p](int64_t g) { return p->mass[g]; });
+  return MJW_OK;
+}
+
+extern "C" int mjw_apply_sh(mjw_problem *p, double alpha, const double *U, double beta,
+                            double *Y) {
+  if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_sh: NULL");
+  diag_apply(p, alpha, U, beta, Y, [p](int64_t g) { return damp_at(p, g); });
+  return MJW_OK;
+}
+
+extern "C" int mjw_apply_dh_minus_dt_sh(mjw_problem *p, double dt, double alpha, const double *U,
+                                        double beta, double *Y) {
+  if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_dh_minus_dt_sh: NULL");
+  if (!(dt > 0.0) || !std::isfinite(dt)) FAIL(p, MJW_ERR_ARG, "mjw_apply_dh_minus_dt_sh: bad dt");
+  diag_apply(p, alpha, U, beta, Y,
+             [p, dt](int64_t g) { return p->mass[g] - 0.5 * dt * damp_at(p, g); });
   return MJW_OK;
 }
 
@@ -763,6 +802,25 @@ extern "C" int mjw_solve_dh(mjw_problem *p, double *Y) {
   if (!p || !Y) FAIL(p, MJW_ERR_ARG, "mjw_solve_dh: NULL");
   const int64_t n = (int64_t)p->ncomp * p->N;
   for (int64_t i = 0; i < n; ++i) Y[i] *= 1.0 / p->mass[i % p->N];
+  return MJW_OK;
+}
+
+extern "C" int mjw_solve_dh_plus_dt_sh(mjw_problem *p, double dt, double *Y) {
+  if (!p || !Y) FAIL(p, MJW_ERR_ARG, "mjw_solve_dh_plus_dt_sh: NULL");
+  if (!(dt > 0.0) || !std::isfinite(dt)) FAIL(p, MJW_ERR_ARG, "mjw_solve_dh_plus_dt_sh: bad dt");
+  const int64_t n = (int64_t)p->ncomp * p->N;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t g = i % p->N;
+    Y[i] *= 1.0 / (p->mass[g] + 0.5 * dt * damp_at(p, g));
+  }
+  return MJW_OK;
+}
+
+extern "C" int mjw_set_dirichlet_condition(mjw_problem *p, double *Y) {
+  if (!p || !Y) FAIL(p, MJW_ERR_ARG, "mjw_set_dirichlet_condition: NULL");
+  for (int m = 0; m < p->ncomp; ++m)
+    for (int64_t g = 0; g < p->N; ++g)
+      if (p->dir[g]) Y[(int64_t)m * p->N + g] = 0.0;
   return MJW_OK;
 }
 
@@ -872,15 +930,17 @@ extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, cons
   if (!p || nsrc < 0 || (nsrc > 0 && !pts)) FAIL(p, MJW_ERR_ARG, "mjw_set_sources: bad args");
   HIPCHK(p, hipSetDevice(p->device));
   HIPCHK(p, hipStreamSynchronize(p->stream));
-  std::vector<int64_t> dof;
-  std::vector<int32_t> sh;
-  std::vector<double> val;
+  // one source vector per shot, the sum over its points in the order given
+  // (ComputeRightHandSide, VarInstationary.cxx:620-675, forms one vector F
+  // for all points); each entry of the device list is then a distinct
+  // (shot, dof), so the injection has no write conflicts
+  std::vector<std::map<int64_t, double>> acc(p->nshots);
   std::vector<double> phi(p->nd);
   for (int k = 0; k < nsrc; ++k) {
     const int s = shot ? shot[k] : 0;
     if (s < 0 || s >= p->nshots) FAIL(p, MJW_ERR_ARG, "source shot index out of range");
     const double *x = pts + (size_t)k * p->dim;
-    std::unordered_map<int64_t, double> b;  // node -> sum of phi
+    std::map<int64_t, double> b;  // node -> sum of phi over the elements holding x
     int cnt = 0;
     const auto cand = candidates(p, x);
     for (const int32_t *ce = cand.first; ce != cand.second; ++ce) {
@@ -891,33 +951,38 @@ extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, cons
       ++cnt;
     }
     if (cnt == 0) FAIL(p, MJW_ERR_ARG, "source point " + std::to_string(k) + " outside the mesh");
-    // average over the elements holding the point (DefineSourceElliptic.cxx:2712-2725),
-    // drop Dirichlet rows and values below eps*||b|| (VarInstationary.cxx:637-658)
-    std::vector<std::pair<int64_t, double>> ent(b.begin(), b.end());
-    std::sort(ent.begin(), ent.end());
-    double nrm = 0.0;
+    // average over the elements holding the point (DefineSourceElliptic.cxx:2712-2725)
     for (int m = 0; m < p->ncomp; ++m) {
       const double pm = pol ? pol[(size_t)k * p->ncomp + m] : 1.0;
-      for (auto &kv : ent) {
+      for (auto &kv : b) {
         double v = pm * kv.second;
         if (cnt > 1) v *= 1.0 / cnt;
-        if (p->dir[kv.first]) v = 0.0;
-        nrm += v * v;
-      }
-    }
-    const double thr = 2.220446049250313e-16 * std::sqrt(nrm);
-    for (int m = 0; m < p->ncomp; ++m) {
-      const double pm = pol ? pol[(size_t)k * p->ncomp + m] : 1.0;
-      for (auto &kv : ent) {
-        double v = pm * kv.second;
-        if (cnt > 1) v *= 1.0 / cnt;
-        if (p->dir[kv.first] || std::fabs(v) <= thr) continue;
-        dof.push_back((int64_t)m * p->N + p->nperm[kv.first]);
-        sh.push_back(s);
-        val.push_back(v);
+        acc[s][(int64_t)m * p->N + kv.first] += v;
       }
     }
   }
+  // per shot: drop Dirichlet rows, then values below eps*||F||
+  // (VarInstationary.cxx:637-658)
+  std::vector<int64_t> dof, hdof;
+  std::vector<int32_t> sh;
+  std::vector<double> val;
+  for (int s = 0; s < p->nshots; ++s) {
+    double nrm = 0.0;
+    for (auto &kv : acc[s])
+      if (!p->dir[kv.first % p->N]) nrm += kv.second * kv.second;
+    const double thr = 2.220446049250313e-16 * std::sqrt(nrm);
+    for (auto &kv : acc[s]) {
+      const int64_t g = kv.first % p->N, m = kv.first / p->N;
+      if (p->dir[g] || std::fabs(kv.second) <= thr) continue;
+      dof.push_back(m * p->N + p->nperm[g]);
+      hdof.push_back(kv.first);
+      sh.push_back(s);
+      val.push_back(kv.second);
+    }
+  }
+  p->h_src_dof = hdof;
+  p->h_src_shot = sh;
+  p->h_src_val = val;
   dfree(p->d_src_dof); dfree(p->d_src_shot); dfree(p->d_src_val);
   p->d_src_dof = nullptr; p->d_src_shot = nullptr; p->d_src_val = nullptr;
   p->nsrc = (int)dof.size();
@@ -930,6 +995,36 @@ extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, cons
     HIPCHK(p, hipMemcpy(p->d_src_val, val.data(), val.size() * 8, hipMemcpyHostToDevice));
   }
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  return MJW_OK;
+}
+
+extern "C" int mjw_set_time_source(mjw_problem *p, int kind, double f0, double t_init,
+                                   double eps_time) {
+  if (!p) FAIL(p, MJW_ERR_ARG, "mjw_set_time_source: NULL");
+  if (kind == MJW_TIME_NONE) {
+    p->has_tsrc = false;
+    return MJW_OK;
+  }
+  if (kind != MJW_TIME_RICKER) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_time_source: unknown kind");
+  if (!(f0 > 0.0) || !std::isfinite(f0) || !std::isfinite(t_init) || !(eps_time >= 0.0))
+    FAIL(p, MJW_ERR_ARG, "mjw_set_time_source: bad f0, t_init or eps");
+  p->tsrc.init(f0, t_init, eps_time);
+  p->tlimit = 2.5 / f0;
+  p->has_tsrc = true;
+  return MJW_OK;
+}
+
+extern "C" int mjw_add_scalar_time_source(mjw_problem *p, int shot, double alpha, double t,
+                                          int nb_deriv, double *Y) {
+  if (!p || !Y) FAIL(p, MJW_ERR_ARG, "mjw_add_scalar_time_source: NULL");
+  if (shot < 0 || shot >= p->nshots) FAIL(p, MJW_ERR_ARG, "shot index out of range");
+  if (nb_deriv < 0 || nb_deriv >= GlTimeSource::kNDer)
+    FAIL(p, MJW_ERR_ARG, "mjw_add_scalar_time_source: nb_deriv out of range");
+  if (!p->has_tsrc) FAIL(p, MJW_ERR_STATE, "mjw_set_time_source has not been called");
+  if (t >= p->tlimit) return MJW_OK;
+  const double a = alpha * p->tsrc.eval(t, nb_deriv);
+  for (size_t i = 0; i < p->h_src_dof.size(); ++i)
+    if (p->h_src_shot[i] == shot) Y[p->h_src_dof[i]] += a * p->h_src_val[i];
   return MJW_OK;
 }
 

@@ -122,65 +122,83 @@ void lagrange(int order, const double *x, double t, double *phi, double *dphi) {
 }
 
 // ---------------------------------------------------------------------------
-// Ricker pulse through the degree-24 GL interpolator of its primitive
+// Time function of the sources: the Ricker primitive (TimeRickerSource::
+// Evaluate, Source/TimeSource.cxx:33-51) interpolated by the degree-24 GL
+// Lagrange polynomial on windows of length 0.5/f0 that move forward with t,
+// derivatives by the GL differentiation matrix applied nb_deriv times
 // (TimeSourceHyperbolic::Init / GenerateValues / EvaluateDerivative,
-// Source/TimeSource.cxx:818-880; primitive TimeRickerSource::Evaluate,
-// :33-51; window tolerance VarInstationary.cxx:993; cut-off at 2.5/f0,
-// VarInstationary.cxx:416-420, 2394).  Value at step n is the first derivative
-// of the interpolant at t_begin + n*dt + dt (ModifiedEquation.cxx:316).
+// Source/TimeSource.cxx:818-880).  nb_deriv = 1 is the pulse itself.
 // ---------------------------------------------------------------------------
-void ricker_pulse_table(double f0, double t_begin, double dt, int nsteps, double *out) {
-  constexpr int R = 24, NP = R + 1;
-  double xg[NP], wg[NP], Dg[NP * NP], cinv[NP];
-  gl_rule(R, xg, wg);
-  gl_deriv(R, xg, Dg);
-  for (int i = 0; i < NP; ++i) {
-    double p = 1.0;
-    for (int k = 0; k < NP; ++k)
-      if (k != i) p *= (xg[i] - xg[k]);
-    cinv[i] = 1.0 / p;
-  }
-  const double pi = 3.14159265358979323846;
-  auto primitive = [&](double t) {
-    const double T0 = 1.0 / f0;
-    if (t <= -0.9 * T0 || t >= 2.9 * T0) return 0.0;
-    const double a = f0 * t - 1.0;
-    return -a * std::exp(-(a * a) * (pi * pi)) / f0;
-  };
-  const double eps = (nsteps * dt) * 1e-15;
-  const double tcut = 2.5 / f0;
-  double w0 = t_begin, w1 = t_begin + 0.5 / f0;
-  double prim[NP], dval[NP];
-  auto regenerate = [&]() {
-    for (int i = 0; i < NP; ++i) prim[i] = primitive((1.0 - xg[i]) * w0 + xg[i] * w1);
-    const double inv = 1.0 / (w1 - w0);
-    for (int a = 0; a < NP; ++a) {
-      double s = 0.0;
-      for (int k = 0; k < NP; ++k) s += Dg[a * NP + k] * prim[k];
-      dval[a] = s * inv;
+void GlTimeSource::init(double freq, double t_init, double eps_time) {
+  f0 = freq;
+  eps = eps_time;
+  t0 = t_init;
+  t1 = t_init + 0.5 / freq;
+  if (!ready) {
+    double wg[kNP];
+    gl_rule(kR, xg, wg);
+    gl_deriv(kR, xg, Dg);
+    for (int i = 0; i < kNP; ++i) {
+      double p = 1.0;
+      for (int k = 0; k < kNP; ++k)
+        if (k != i) p *= (xg[i] - xg[k]);
+      cinv[i] = 1.0 / p;
     }
-  };
-  regenerate();
+    ready = true;
+  }
+  generate();
+}
+
+double GlTimeSource::primitive(double t) const {
+  const double pi = 3.14159265358979323846;
+  const double T0 = 1.0 / f0;
+  if (t <= -0.9 * T0 || t >= 2.9 * T0) return 0.0;
+  const double a = f0 * t - 1.0;
+  return -a * std::exp(-(a * a) * (pi * pi)) / f0;
+}
+
+void GlTimeSource::generate() {
+  for (int i = 0; i < kNP; ++i) val[0][i] = primitive((1.0 - xg[i]) * t0 + xg[i] * t1);
+  const double inv = 1.0 / (t1 - t0);
+  for (int d = 1; d < kNDer; ++d)
+    for (int a = 0; a < kNP; ++a) {
+      double s = 0.0;
+      for (int k = 0; k < kNP; ++k) s += Dg[a * kNP + k] * val[d - 1][k];
+      val[d][a] = s * inv;
+    }
+}
+
+double GlTimeSource::eval(double t, int nb_deriv) {
+  if (t > t1 + eps && t1 > t0) {
+    while (t > t1 + eps) {
+      const double len = t1 - t0;
+      t0 = t1;
+      t1 = t1 + len;
+    }
+    generate();
+  }
+  const double lam = (t - t0) / (t1 - t0);
+  double acc = 0.0;
+  for (int i = 0; i < kNP; ++i) {
+    double p = 1.0;
+    for (int k = 0; k < kNP; ++k)
+      if (k != i) p *= (lam - xg[k]);
+    acc += (p * cinv[i]) * val[nb_deriv][i];
+  }
+  return acc;
+}
+
+// Pulse table of the leapfrog: the value at step n is the first derivative at
+// t_begin + n*dt + dt (ModifiedEquation.cxx:316), window tolerance
+// (final - initial time) * 1e-15 (VarInstationary.cxx:993), zero from the
+// cut-off 2.5/f0 on (VarInstationary.cxx:416-420, 2394).
+void ricker_pulse_table(double f0, double t_begin, double dt, int nsteps, double *out) {
+  GlTimeSource src;
+  src.init(f0, t_begin, (nsteps * dt) * 1e-15);
+  const double tcut = 2.5 / f0;
   for (int n = 0; n < nsteps; ++n) {
     const double t = (t_begin + n * dt) + dt;
-    if (t >= tcut) { out[n] = 0.0; continue; }
-    if (t > w1 + eps && w1 > w0) {
-      while (t > w1 + eps) {
-        const double len = w1 - w0;
-        w0 = w1;
-        w1 = w1 + len;
-      }
-      regenerate();
-    }
-    const double lam = (t - w0) / (w1 - w0);
-    double acc = 0.0;
-    for (int i = 0; i < NP; ++i) {
-      double p = 1.0;
-      for (int k = 0; k < NP; ++k)
-        if (k != i) p *= (lam - xg[k]);
-      acc += (p * cinv[i]) * dval[i];
-    }
-    out[n] = acc;
+    out[n] = t >= tcut ? 0.0 : src.eval(t, 1);
   }
 }
 

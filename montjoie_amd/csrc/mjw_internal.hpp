@@ -19,6 +19,21 @@ void gl_deriv(int order, const double *x, double *D);
 void lagrange(int order, const double *x, double t, double *phi, double *dphi);
 // Ricker pulse table through the degree-24 GL interpolator.
 void ricker_pulse_table(double f0, double t_begin, double dt, int nsteps, double *out);
+// Stateful Ricker time source: its degree-24 GL interpolant on windows of
+// length 0.5/f0 starting at t_init, which move forward with the evaluation
+// time; eval(t, k) = k-th derivative of the interpolated primitive (k = 1:
+// the pulse), 0 <= k < kNDer.
+struct GlTimeSource {
+  static constexpr int kR = 24, kNP = kR + 1, kNDer = 10;
+  double f0 = 1.0, t0 = 0.0, t1 = -1.0, eps = 0.0;
+  bool ready = false;
+  double xg[kNP], Dg[kNP * kNP], cinv[kNP];
+  double val[kNDer][kNP];
+  void init(double freq, double t_init, double eps_time);
+  double primitive(double t) const;
+  void generate();
+  double eval(double t, int nb_deriv);
+};
 
 // ----------------------------------------------------------------- host mesh
 void mesh_regular_size(int dim, int nx, int order, int64_t *N, int64_t *E);

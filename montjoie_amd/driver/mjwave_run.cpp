@@ -22,8 +22,8 @@
 // (ISOTROPE, acoustic damping sigma), PhysicalMedia (rho / mu / sigma, SINUS or MESH / SAME_MESH data files on the
 // computational mesh, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
 // TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
-// (2 LEAP_FROG), SismoPoint, SismoCircle, FileOutputPoint,
-// ParametersOutputPoint, SismoPlane, FileOutputPlane, ParametersOutputPlane,
+// (2 LEAP_FROG), SismoPoint, FileOutputPoint, ParametersOutputPoint, SismoCircle,
+// FileOutputCircle, ParametersOutputCircle, SismoPlane, FileOutputPlane, ParametersOutputPlane,
 // SismoLine, FileOutputLine, ParametersOutputLine, OutputFormat (BINARY or ASCII, DOUBLE or FLOAT),
 // DisplayStress (YES), SaveReprise, PathReprise.  Other keywords are reported and ignored; a value
 // this path does not implement (another scheme, source or condition) stops
@@ -909,6 +909,7 @@ NodalMedia nodal_media(const Deck &d, const Q1Mesh &m, const QrMesh &qm, const M
 struct Plane {
   int dim = 2, nx = 0, ny = 0;
   bool line = false;                           // SismoLine: ny = 1
+  bool circle = false;                         // SismoCircle: ny = 1, p = x0 y0 radius
   double p[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // plane 2-D: xmin xmax ymin ymax; 3-D: A, B, C
                                                // line: xmin xmax ymin ymax [zmin zmax]
   std::vector<double> pts;                     // nx*ny points, index j*nx + i
@@ -933,6 +934,36 @@ Plane line_grid(const std::vector<std::string> &v, int dim) {
       const double h = g.nx > 1 ? (g.p[2 * a + 1] - g.p[2 * a]) / (g.nx - 1) : 0.0;
       g.pts[(size_t)i * dim + a] = g.p[2 * a] + i * h;
     }
+  return g;
+}
+
+// SismoCircle = x0 y0 radius N (2-D; GridInterpolation.cxx:1966-1983): the box
+// [x0 - r, x0 + r] x [y0 - r, y0 + r] is kept, and the points are
+// c + rho (cos i dtheta, sin i dtheta) with c = the box centre, rho = half its
+// width and dtheta = 2 pi / N (:2721-2740); points outside the mesh hold 0.
+// Written at the ParametersOutputCircle times like the planes, grid type 3,
+// header c, rho, N (:3076-3079, :3203-3205).
+Plane circle_grid(const std::vector<std::string> &v, int dim) {
+  if (dim != 2 || v.size() < 4) die("SismoCircle = x0 y0 radius N (2-D)");
+  Plane g;
+  g.dim = 2;
+  g.circle = true;
+  const double cx = num(v[0]), cy = num(v[1]), rad = num(v[2]);
+  g.nx = (int)num(v[3]);
+  g.ny = 1;
+  if (g.nx < 1) die("SismoCircle: N must be positive");
+  const double xmin = cx - rad, xmax = cx + rad, ymin = cy - rad, ymax = cy + rad;
+  const double pi = 3.14159265358979323846;
+  const double step = (2.0 * pi) / g.nx;
+  const double c0 = (xmin + xmax) / 2.0, c1 = (ymin + ymax) / 2.0, r = (xmax - xmin) / 2.0;
+  g.p[0] = 0.5 * (xmin + xmax);
+  g.p[1] = 0.5 * (ymin + ymax);
+  g.p[2] = 0.5 * (xmax - xmin);
+  g.pts.resize((size_t)g.nx * 2);
+  for (int i = 0; i < g.nx; ++i) {
+    g.pts[(size_t)i * 2] = c0 + r * std::cos(step * i);
+    g.pts[(size_t)i * 2 + 1] = c1 + r * std::sin(step * i);
+  }
   return g;
 }
 
@@ -1013,8 +1044,10 @@ void write_plane_text(const std::string &name, const Plane &g, const std::vector
   const int prec = dbl ? 15 : 7;
   auto wr_d = [&](double x) { std::fprintf(f, "%.*g\n", prec, x); };
   std::fprintf(f, "1\n%d\n%d\n%d\n", g.dim, dbl ? 1 : 0,
-               g.line ? (g.dim == 2 ? 1 : 3) : (g.dim == 2 ? 0 : 2));
-  if (g.line) {
+               g.circle ? 3 : g.line ? (g.dim == 2 ? 1 : 3) : (g.dim == 2 ? 0 : 2));
+  if (g.circle) {
+    wr_d(g.p[0]); wr_d(g.p[1]); wr_d(g.p[2]); std::fprintf(f, "%d\n", g.nx);
+  } else if (g.line) {
     for (int k = 0; k < 2 * g.dim; ++k) wr_d(g.p[k]);
     std::fprintf(f, "%d\n", g.nx);
   } else if (g.dim == 2) {
@@ -1047,9 +1080,11 @@ void write_plane(const std::string &name, const Plane &g, const std::vector<doub
   wi(1);                        // one grid
   wi(g.dim);                    // dimension
   wi(dbl ? 1 : 0);              // real double / float values
-  // grid type: plane 0 (2-D) / 2 (3-D), line 1 (2-D) / 3 (3-D)
-  wi(g.line ? (g.dim == 2 ? 1 : 3) : (g.dim == 2 ? 0 : 2));
-  if (g.line) {
+  // grid type: plane 0 (2-D) / 2 (3-D), line 1 (2-D) / 3 (3-D), circle 3 (2-D)
+  wi(g.circle ? 3 : g.line ? (g.dim == 2 ? 1 : 3) : (g.dim == 2 ? 0 : 2));
+  if (g.circle) {
+    wd(g.p[0]); wd(g.p[1]); wd(g.p[2]); wi(g.nx);
+  } else if (g.line) {
     for (int k = 0; k < 2 * g.dim; ++k) wd(g.p[k]);
     wi(g.nx);
   } else if (g.dim == 2) {
@@ -1099,7 +1134,8 @@ int main(int argc, char **argv) {
       "TypeElement", "TypeEquation", "MeshPath", "FileMesh", "TypeMesh", "OrderDiscretization",
       "ConditionReference", "MateriauDielec", "PhysicalMedia", "TypeSource", "OriginePhase", "Polarization",
       "Frequency", "TemporalSource", "TimeStep", "TimeInterval", "OrderTimeScheme", "SismoPoint",
-      "SismoCircle", "FileOutputPoint", "ParametersOutputPoint", "SismoPlane",
+      "SismoCircle", "FileOutputCircle", "ParametersOutputCircle", "FileOutputPoint",
+      "ParametersOutputPoint", "SismoPlane",
       "FileOutputPlane", "ParametersOutputPlane", "SismoLine", "FileOutputLine",
       "ParametersOutputLine", "OutputFormat", "SaveReprise", "PathReprise", "LoadReprise", "DisplayStress",
       "ElectricOrMagnetic", "PrintLevel"};
@@ -1239,24 +1275,13 @@ int main(int argc, char **argv) {
   std::vector<double> tab(nsteps);
   check(mjw_ricker_pulse_table(f0, t0, dt, (int)nsteps, tab.data()), nullptr, "Ricker");
   check(mjw_set_pulse(p, (int)nsteps, tab.data()), p, "pulse");
-  // receivers: SismoPoint, SismoCircle (x0 y0 radius N in 2-D)
+  // receivers: SismoPoint (SismoCircle is a snapshot grid, below)
   std::vector<double> pts;
   int npt = 0;
   for (const auto &v : d.all("SismoPoint")) {
     if ((int)v.size() < dim) die("SismoPoint needs " + std::to_string(dim) + " coordinates");
     for (int a = 0; a < dim; ++a) pts.push_back(num(v[a]));
     ++npt;
-  }
-  for (const auto &v : d.all("SismoCircle")) {
-    if (dim != 2 || v.size() < 4) die("SismoCircle = x0 y0 radius N (2-D)");
-    const double cx = num(v[0]), cy = num(v[1]), rad = num(v[2]);
-    const int nc = (int)num(v[3]);
-    const double pi = 3.14159265358979323846;
-    for (int k = 0; k < nc; ++k) {
-      pts.push_back(cx + rad * std::cos(2 * pi * k / nc));
-      pts.push_back(cy + rad * std::sin(2 * pi * k / nc));
-      ++npt;
-    }
   }
   // receivers: every component of u at every point (component-major); for
   // elastic decks also d u_i / d x_a at every point (point-major, i outer),
@@ -1286,7 +1311,7 @@ int main(int argc, char **argv) {
   const double t_setup =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - tc).count();
 
-  // snapshot grids (SismoPlane, SismoLine): probe points, file names and the
+  // snapshot grids (SismoPlane, SismoLine, SismoCircle): probe points, file names and the
   // steps each is written at (its own ParametersOutput* times and counter)
   struct Snap {
     Plane g;
@@ -1298,15 +1323,17 @@ int main(int argc, char **argv) {
   std::vector<Snap> snaps;
   std::vector<double> probe_pts;
   std::vector<int32_t> probe_deriv;
-  for (const char *kind : {"Plane", "Line"}) {
+  for (const char *kind : {"Plane", "Line", "Circle"}) {
     const std::string key = std::string("Sismo") + kind;
+    const std::string k = kind;
     const auto lines = d.all(key);
     if (lines.empty()) continue;
     if (lines.size() > 1) die(key + ": one per deck");
     if (!out_binary && !out_ascii)
       die(key + ": only OutputFormat = BINARY or ASCII (DOUBLE or FLOAT) is written");
     Snap sn;
-    sn.g = std::string(kind) == "Plane" ? plane_grid(lines[0], dim, qm) : line_grid(lines[0], dim);
+    sn.g = k == "Plane" ? plane_grid(lines[0], dim, qm)
+           : k == "Line" ? line_grid(lines[0], dim) : circle_grid(lines[0], dim);
     const auto &fo = d.need(std::string("FileOutput") + kind);
     sn.name = fo.size() >= 2 ? fo[1] : fo[0];
     double ps0 = t0, ps1 = t0 + nsteps * dt, psd = dt;

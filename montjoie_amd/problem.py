@@ -193,6 +193,58 @@ class WaveProblem:
         self._chk(_lib.lib().mjw_apply_kh_dev(self._h, alpha, C.c_void_p(U_ptr), beta,
                                               C.c_void_p(Y_ptr)), "apply_kh_dev")
 
+    def _vec(self, Y):
+        return np.zeros(self.ndof) if Y is None else np.ascontiguousarray(Y, np.float64).copy()
+
+    def apply_dh(self, U, alpha=1.0, beta=0.0, Y=None) -> np.ndarray:
+        """Y = beta Y + alpha D U (ApplyOperatorDh)."""
+        U, Y = np.ascontiguousarray(U, np.float64), self._vec(Y)
+        self._chk(_lib.lib().mjw_apply_dh(self._h, alpha, dptr(U), beta, dptr(Y)), "apply_dh")
+        return Y
+
+    def apply_sh(self, U, alpha=1.0, beta=0.0, Y=None) -> np.ndarray:
+        """Y = beta Y + alpha S U, S = lumped damping (ApplyOperatorSh)."""
+        U, Y = np.ascontiguousarray(U, np.float64), self._vec(Y)
+        self._chk(_lib.lib().mjw_apply_sh(self._h, alpha, dptr(U), beta, dptr(Y)), "apply_sh")
+        return Y
+
+    def apply_dh_minus_dt_sh(self, dt, U, alpha=1.0, beta=0.0, Y=None) -> np.ndarray:
+        """Y = beta Y + alpha (D - dt/2 S) U (ApplyOperatorDhMinusdtSh)."""
+        U, Y = np.ascontiguousarray(U, np.float64), self._vec(Y)
+        self._chk(_lib.lib().mjw_apply_dh_minus_dt_sh(self._h, dt, alpha, dptr(U), beta, dptr(Y)),
+                  "apply_dh_minus_dt_sh")
+        return Y
+
+    def solve_dh(self, Y) -> np.ndarray:
+        """D^{-1} Y (SolveOperatorDh)."""
+        Y = self._vec(Y)
+        self._chk(_lib.lib().mjw_solve_dh(self._h, dptr(Y)), "solve_dh")
+        return Y
+
+    def solve_dh_plus_dt_sh(self, dt, Y) -> np.ndarray:
+        """(D + dt/2 S)^{-1} Y (SolveOperatorDhPlusdtSh)."""
+        Y = self._vec(Y)
+        self._chk(_lib.lib().mjw_solve_dh_plus_dt_sh(self._h, dt, dptr(Y)), "solve_dh_plus_dt_sh")
+        return Y
+
+    def set_dirichlet_condition(self, Y) -> np.ndarray:
+        """Y with its Dirichlet rows set to 0 (SetDirichletCondition)."""
+        Y = self._vec(Y)
+        self._chk(_lib.lib().mjw_set_dirichlet_condition(self._h, dptr(Y)), "set_dirichlet")
+        return Y
+
+    def set_time_source(self, f0: float, t_init: float = 0.0, eps_time: float = 0.0):
+        """Ricker time function of add_scalar_time_source (mjw_set_time_source)."""
+        self._chk(_lib.lib().mjw_set_time_source(self._h, _lib.MJW_TIME_RICKER, f0, t_init,
+                                                 eps_time), "set_time_source")
+
+    def add_scalar_time_source(self, t: float, Y, alpha=1.0, nb_deriv=1, shot=0) -> np.ndarray:
+        """Y + alpha f^(nb_deriv)(t) F_shot (AddScalarTimeSource)."""
+        Y = self._vec(Y)
+        self._chk(_lib.lib().mjw_add_scalar_time_source(self._h, shot, alpha, t, nb_deriv,
+                                                        dptr(Y)), "add_scalar_time_source")
+        return Y
+
     def mass(self) -> np.ndarray:
         D = np.zeros(self.mesh.N)
         self._chk(_lib.lib().mjw_get_mass(self._h, dptr(D)), "get_mass")

@@ -93,8 +93,9 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
                mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
-               output_format="BINARY DOUBLE"):
-    """case.ini (+ mesh.mesh unless REGULAR) in directory d.  sinus_rho /
+               output_format="BINARY DOUBLE", circle=None):
+    """case.ini (+ mesh.mesh unless REGULAR) in directory d.  circle = (x0, y0,
+    radius, n): a SismoCircle line after the SismoPoint lines.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1.
     save_reprise > 0: checkpoint files every save_reprise steps.  sigma_layers:
     acoustic damping of each layer (MateriauDielec's third value)."""
@@ -150,6 +151,10 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     ini.append("FileOutputPoint = diffracPoint.dat totalPoint.dat")
     for p in rec:
         ini.append("SismoPoint = " + " ".join(_f(c) for c in p))
+    if circle is not None and len(circle):
+        ini.append("SismoCircle = " + " ".join(_f(v) for v in circle[:3]) + f" {int(circle[3])}")
+        ini.append("FileOutputCircle = diffracCircle.dat totalCircle.dat")
+        ini.append(f"ParametersOutputCircle = 0.0 {_f(nsteps * dt + 0.5 * dt)} {_f(dt)}")
     if plane is not None and len(plane):
         ini.append("SismoPlane = " + " ".join(_f(v) if i < len(plane) - 2 else str(int(v))
                                               for i, v in enumerate(plane)))

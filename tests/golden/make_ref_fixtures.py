@@ -36,14 +36,14 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
              mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
-             output_format="BINARY DOUBLE"):
+             output_format="BINARY DOUBLE", circle=None):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
                sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times,
                line=line, line_times=line_times, mesh_media=mesh_media,
                save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma,
-               output_format=output_format)
+               output_format=output_format, circle=circle)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):
@@ -56,6 +56,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         raise RuntimeError(f"{name}: reference run failed\n{r_.stdout[-3000:]}")
     out = np.loadtxt(os.path.join(d, "totalPoint.dat"), ndmin=2)
     times, traces = out[:, 0], out[:, 1:]
+
     names = {}
 
     def snap_files(prefix):
@@ -72,6 +73,11 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         planes = snap_files("totalPlane")
     if line is not None:
         lines = snap_files("totalLine")
+    circles = np.zeros((0, 0), np.uint8)
+    if circle is not None:
+        # SismoCircle output (FileOutputCircle): one snapshot file per output
+        # time in the grid layout of WriteMatlab, 0 at points outside the mesh
+        circles = snap_files("totalCircle")
     reprise = {}
     if save_reprise:
         # checkpoint files, raw bytes (SolAcous<b>_P0N1.dat, IterationAcousSave[New]0N1.dat)
@@ -101,6 +107,9 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         sigma_layers=np.asarray(sigma_layers if sigma_layers is not None else [], float),
         sinus_sigma=np.asarray(sinus_sigma if sinus_sigma is not None else [], float),
         output_format=output_format,
+        circle=np.asarray(circle if circle is not None else [], float),
+        circle_files=circles,
+        circle_names=names.get("totalCircle", np.zeros(0, "U1")),
         plane_names=names.get("totalPlane", np.zeros(0, "U1")),
         line_names=names.get("totalLine", np.zeros(0, "U1")),
         plane_lens=names.get("totalPlane_len", np.zeros(0, int)),
@@ -126,6 +135,40 @@ def ref2d_q4_n8():
     circ = O.circle_points(0.5, 0.5, 0.3, 16)
     run_case("ref2d_q4_n8", 2, 8, 4, 0.0, 0, [1.0], 10.0, 1e-3, 200, (0.5, 0.5),
              np.vstack([circ, [[0.3, 0.4], [0.71, 0.52]]]), regular=True)
+
+
+@case
+def ref3d_el_q4():
+    """isotropic elastic at the order of the 32^3 16-shot workload: 3^3 Q4
+    perturbed hexahedra, layered lambda, mu, rho drawn like the workload's
+    (lambda, mu in [1, 4], rho in [1, 2])."""
+    lam = [1.0 + 3.0 * O.uniform(2033, k) for k in range(3)]
+    mu = [1.0 + 3.0 * O.uniform(2034, k) for k in range(3)]
+    rho = [1.0 + O.uniform(2035, k) for k in range(3)]
+    rec = seeded_points(3, 6, 92)
+    run_case("ref3d_el_q4", 3, 3, 4, 0.1, 17, mu, 8.0, 3e-3, 150, (0.47, 0.54, 0.49), rec,
+             elastic=True, lam_layers=lam, rho_layers=rho, pol=(-0.4, 0.7, 0.5))
+
+
+@case
+def ref2d_c1_circle():
+    """The 64x64 Q4 / 1000-step / 0.5-CFL case with its 16 receivers given as
+    Montjoie's own SismoCircle = 0.5 0.5 0.3 16 (no SismoPoint)."""
+    m = O.mesh(2, 64, 4)
+    P = O.acoustic_problem(m, 1.0, 1.0)
+    lam = O.lambda_max(P, 200, 11)
+    dt = 0.5 * 2.0 / np.sqrt(lam)
+    run_case("ref2d_c1_circle", 2, 64, 4, 0.0, 0, [1.0], 10.0, dt, 1000, (0.5, 0.5),
+             np.zeros((0, 2)), regular=True, circle=(0.5, 0.5, 0.3, 16))
+
+
+@case
+def ref2d_q4_circle_out():
+    """8x8 Q4 perturbed quadrilaterals, two layers, a SismoCircle that crosses
+    the boundary of the square (points outside the mesh), after two
+    SismoPoint receivers."""
+    run_case("ref2d_q4_circle_out", 2, 8, 4, 0.1, 45, [1.0, 2.0], 10.0, 1e-3, 200, (0.46, 0.53),
+             np.array([[0.3, 0.4], [0.71, 0.52]]), circle=(0.75, 0.45, 0.4, 12))
 
 
 @case

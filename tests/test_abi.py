@@ -100,3 +100,26 @@ def test_cpp_adapter_builds_and_links(tmp_path):
                         "-Wl,-rpath," + os.path.join(ROOT, "montjoie_amd")],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+REF_SRC = "/root/reference/MONTJOIE/src"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SRC), reason="Montjoie's sources are not present")
+def test_montjoie_binding_compiles_against_montjoie_headers(tmp_path):
+    """include/mjwave_montjoie.hpp (HipOdeSystem : VirtualOdeSystem<Real_wp>)
+    compiles against Montjoie's own headers: every override matches a virtual
+    of Instationary/TimeSchemes.hxx:35-151.  Syntax only; the GPU tests run
+    Montjoie's leapfrog over it (oracle/_ref/mjref_drive_hip.x)."""
+    import subprocess
+    src = tmp_path / "binding.cc"
+    src.write_text('#include "MontjoieFlag.hxx"\n#include "Montjoie.hxx"\n'
+                   "using namespace Montjoie;\n#include \"mjwave_montjoie.hpp\"\n"
+                   "int main() { ModifiedEquationIterator<Real_wp> s; s.SetOrder(2);\n"
+                   "  mjw::HipWaveSystem *h = nullptr; (void)h; return 0; }\n")
+    r = subprocess.run(["g++", "-std=c++11", "-w", "-ftemplate-depth-5000", "-fsyntax-only",
+                        "-I", REF_SRC,
+                        "-I", os.path.join(os.path.dirname(REF_SRC), "..", "SELDON"),
+                        "-I", os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]

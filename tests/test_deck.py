@@ -40,7 +40,8 @@ def _write(c: CS.Case, d):
                      sigma_layers=z["sigma_layers"] if "sigma_layers" in z.files else None,
                      sinus_sigma=z["sinus_sigma"] if "sinus_sigma" in z.files else None,
                      output_format=str(z["output_format"]) if "output_format" in z.files
-                     else "BINARY DOUBLE")
+                     else "BINARY DOUBLE",
+                     circle=z["circle"] if "circle" in z.files and len(z["circle"]) else None)
 
 
 def _dump(path):
@@ -161,6 +162,9 @@ def test_driver_seismograms_match_montjoie(name, tmp_path):
     r = subprocess.run([RUNNER, "case.ini"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
+    if not len(c.rec):
+        pytest.skip("no SismoPoint: the deck's SismoCircle files are checked by "
+                    "test_driver_circle_matches_montjoie")
     out = np.loadtxt(tmp_path / "totalPoint.dat", ndmin=2)
     np.testing.assert_allclose(out[:, 0], c.times, rtol=1e-15, atol=1e-18)
     ref = c.traces
@@ -352,3 +356,37 @@ def test_driver_ascii_output_matches_montjoie(name, tmp_path):
                 assert CS.rel_l2(v, vr) < 1e-10
             else:
                 assert np.abs(v - vr).max() <= 1e-6 * np.abs(vr).max()
+
+
+CIRCLES = [n for n in CS.golden_names() if "circle" in n]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CIRCLES)
+def test_driver_circle_matches_montjoie(name, tmp_path):
+    """SismoCircle = x0 y0 r N with FileOutputCircle: the driver's
+    totalCircle%04d.dat files against Montjoie's (one per output step, grid
+    type 3), header byte for byte, values to rel-L2 1e-10, zeros at the points
+    outside the mesh."""
+    c = CS.load(name)
+    z = np.load(os.path.join(CS.GOLD, name + ".npz"))
+    _write(c, tmp_path)
+    print(f"deck {name} SismoCircle", flush=True)
+    r = subprocess.run([RUNNER, "case.ini"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    files = sorted(f.name for f in tmp_path.iterdir() if f.name.startswith("totalCircle"))
+    ref = z["circle_files"]
+    assert files == list(z["circle_names"])
+    npts = int(z["circle"][3])
+    for k, f in enumerate(files):
+        raw = (tmp_path / f).read_bytes()
+        h, v = CS.plane_file(raw, npts)
+        hr, vr = CS.plane_file(ref[k], npts)
+        assert h == hr
+        if vr.any():
+            tiny = 1e-12 * np.abs(vr).max()
+            assert np.all(np.abs(vr[v == 0]) <= tiny) and np.all(np.abs(v[vr == 0]) <= tiny)
+            assert CS.rel_l2(v, vr) < 1e-10
+        else:
+            assert not v.any()

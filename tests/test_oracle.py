@@ -450,3 +450,35 @@ def test_oracle_elastic_snapshots_match_montjoie(kind):
             tiny = 1e-12 * np.abs(ref).max()
             assert np.all(np.abs(ref[got == 0]) <= tiny) and np.all(np.abs(got[ref == 0]) <= tiny)
             assert CS.rel_l2(got, ref) < 1e-10, (s, names[12 * k + f], CS.rel_l2(got, ref))
+
+
+@pytest.mark.parametrize("name", [n for n in CS.golden_names() if "circle" in n])
+def test_oracle_circle_matches_montjoie(name):
+    """SismoCircle files written by Montjoie (every step): the oracle's
+    iterate at or_circle_points equals the stored values (0 at the points
+    outside the mesh, which the one fixture has), so the point placement is
+    pinned to the interpolation's sensitivity; the header carries grid type 3
+    and N."""
+    c = CS.load(name)
+    z = np.load(os.path.join(CS.GOLD, name + ".npz"))
+    cx, cy, rad, n = (float(v) for v in z["circle"])
+    n = int(n)
+    pts = O.circle_points(cx, cy, rad, n)
+    assert np.allclose(np.hypot(pts[:, 0] - cx, pts[:, 1] - cy), rad, rtol=1e-15, atol=1e-15)
+    m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
+    el, _, _ = O.receiver_gradients(m, pts)
+    inside = el >= 0
+    c.rec = pts[inside]
+    _, _, tr = CS.oracle_run(c)
+    files = z["circle_files"]
+    assert files.shape[0] == c.nsteps + 1
+    got = np.zeros((c.nsteps + 1, n))
+    got[:, inside] = tr
+    ref = np.stack([CS.plane_file(f, n)[1] for f in files])
+    head = CS.plane_file(files[0], n)[0]
+    assert list(np.frombuffer(head[:16], np.int32)) == [1, 2, 1, 3]
+    assert list(np.frombuffer(head[-8:], np.int32)) == [n, n]
+    assert not ref[:, ~inside].any()
+    err = CS.rel_l2(got, ref)
+    print(f"{name}: {int(inside.sum())}/{n} points inside, rel-L2 {err:.2e}")
+    assert err < 1e-10
