@@ -1336,6 +1336,11 @@ int main(int argc, char **argv) {
            : k == "Line" ? line_grid(lines[0], dim) : circle_grid(lines[0], dim);
     const auto &fo = d.need(std::string("FileOutput") + kind);
     sn.name = fo.size() >= 2 ? fo[1] : fo[0];
+    // the name's extension is dropped (GetBaseString, SELDON/share/Common.cxx:43-51,
+    // applied by Output/OutputHarmonic.cxx:1254-1257); the step number and
+    // ".dat" follow
+    if (const size_t dot = sn.name.find_last_of('.'); dot != std::string::npos)
+      sn.name = sn.name.substr(0, dot);
     double ps0 = t0, ps1 = t0 + nsteps * dt, psd = dt;
     if (const auto *po = d.get(std::string("ParametersOutput") + kind))
       if (po->size() >= 3) { ps0 = num((*po)[0]); ps1 = num((*po)[1]); psd = num((*po)[2]); }

@@ -377,16 +377,22 @@ def test_driver_circle_matches_montjoie(name, tmp_path):
     assert r.returncode == 0, r.stderr + r.stdout
     files = sorted(f.name for f in tmp_path.iterdir() if f.name.startswith("totalCircle"))
     ref = z["circle_files"]
-    assert files == list(z["circle_names"])
+    names = list(z["circle_names"])
+    assert len(files) == len(names) and all(a == b for a, b in zip(files, names)), \
+        (files[:3], names[:3])
     npts = int(z["circle"][3])
+    got, want = [], []
     for k, f in enumerate(files):
-        raw = (tmp_path / f).read_bytes()
-        h, v = CS.plane_file(raw, npts)
+        h, v = CS.plane_file((tmp_path / f).read_bytes(), npts)
         hr, vr = CS.plane_file(ref[k], npts)
         assert h == hr
-        if vr.any():
-            tiny = 1e-12 * np.abs(vr).max()
-            assert np.all(np.abs(vr[v == 0]) <= tiny) and np.all(np.abs(v[vr == 0]) <= tiny)
-            assert CS.rel_l2(v, vr) < 1e-10
-        else:
-            assert not v.any()
+        got.append(v)
+        want.append(vr)
+    got, want = np.stack(got), np.stack(want)
+    # the same zeros (points outside the mesh; steps before the wave arrives,
+    # where either side may hold values of underflow size)
+    tiny = 1e-12 * np.abs(want).max()
+    assert np.all(np.abs(want[got == 0]) <= tiny) and np.all(np.abs(got[want == 0]) <= tiny)
+    err = CS.rel_l2(got, want)
+    print(f"{name}: {len(files)} circle files, rel-L2 {err:.2e}")
+    assert err < 1e-10

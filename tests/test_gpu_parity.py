@@ -64,6 +64,8 @@ def test_elastic_apply_matches_oracle(r, nx):
 @pytest.mark.parametrize("name", CS.golden_names())
 def test_leapfrog_fp64_matches_montjoie(name):
     c = CS.load(name)
+    if not len(c.rec):
+        pytest.skip("no SismoPoint receivers (SismoCircle files: test_deck.py)")
     P, tr = CS.gpu_run(c, "fp64")
     ref = CS.reference_traces(c)
     err_ref = CS.rel_l2(tr[:, 0, :], ref)
