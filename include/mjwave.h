@@ -267,6 +267,18 @@ int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, const double *u
  * in a HIP graph when use_graph != 0.
  */
 int mjw_leapfrog_advance(mjw_problem *p, int nsteps, double *traces, int use_graph);
+/*
+ * mjw_leapfrog_advance without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same steps, kernels and
+ * results as mjw_leapfrog_advance(p, nsteps, traces, 0); the trace rows are
+ * staged in one static device buffer per GPU of 2^20 doubles (8 MiB, held
+ * until the process ends), so (nsteps + 1) * nshots * nrec must not exceed
+ * 1048576 (MJW_ERR_ARG otherwise: advance in shorter calls).  Every problem
+ * on one GPU shares that buffer: calls that overlap on one GPU (several host
+ * threads) race.  No graph and no timing (mjw_last_timing is not updated).
+ */
+int mjw_leapfrog_advance_static(mjw_problem *p, int nsteps, double *traces);
 /* traces of the current iterate only (GiveFinalIterate): [nshots][nrec] */
 int mjw_record(mjw_problem *p, double *traces);
 /*

@@ -59,6 +59,7 @@ SIGNATURES = {
     "mjw_set_probes_grad": (C.c_int, [_vp, C.c_int, _dp, _i32p, _u8p, _i64p]),
     "mjw_leapfrog_init": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _dp]),
     "mjw_leapfrog_advance": (C.c_int, [_vp, C.c_int, _dp, C.c_int]),
+    "mjw_leapfrog_advance_static": (C.c_int, [_vp, C.c_int, _dp]),
     "mjw_record": (C.c_int, [_vp, _dp]),
     "mjw_get_state": (C.c_int, [_vp, _dp, _dp]),
     "mjw_state_dev": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),

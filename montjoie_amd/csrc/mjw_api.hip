@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <unordered_map>
@@ -151,6 +152,35 @@ struct mjw_problem {
       FAIL(p, MJW_ERR_HIP, std::string("kernel launch failed (") + std::to_string(_e) + \
                                ") at " + __FILE__ + ":" + std::to_string(__LINE__)); \
   } while (0)
+
+// ------------------------------------------------ step loop without allocation
+// mjw_leapfrog_advance_static stages the trace rows in `scratch`, a file-scope
+// device array of kStaticTraceDoubles doubles (8 MiB) that every GPU the
+// library runs on holds until the process ends.  Host code never uses the
+// array's name as an address: the device address comes from
+// hipGetSymbolAddress, once per device with that device current (in
+// mjw_create), and is kept per device in g_scratch.
+constexpr int64_t kStaticTraceDoubles = int64_t(1) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kStaticTraceDoubles];
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices] = {};
+
+// the calling thread's current device must be `device`
+static hipError_t setup_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch[device] = static_cast<double *>(ptr);
+  return e;
+}
+
+static double *scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
+}
 
 static size_t ssize(const mjw_problem *p) { return p->state == kF64 ? 8 : 4; }
 static int64_t nstate(const mjw_problem *p) { return (int64_t)p->nshots * p->ncomp * p->N; }
@@ -357,6 +387,8 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   if (hipSetDevice(device) != hipSuccess) return fail(MJW_ERR_HIP, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(MJW_ERR_HIP, "hipStreamCreate failed");
+  if (setup_scratch(device) != hipSuccess)
+    return fail(MJW_ERR_HIP, "hipGetSymbolAddress of the static trace buffer failed");
 
   // host copies
   p->xyz.assign(node_xyz, node_xyz + n_nodes * dim);
@@ -1333,6 +1365,38 @@ extern "C" int mjw_leapfrog_advance(mjw_problem *p, int nsteps, double *traces, 
   if (want)
     HIPCHK(p, hipMemcpy(traces, p->d_traces, (size_t)nsteps * p->nshots * p->nrec * 8,
                         hipMemcpyDeviceToHost));
+  return MJW_OK;
+}
+
+extern "C" int mjw_leapfrog_advance_static(mjw_problem *p, int nsteps, double *traces) {
+  if (!p || nsteps < 0) FAIL(p, MJW_ERR_ARG, "mjw_leapfrog_advance_static: bad args");
+  if (!p->inited) FAIL(p, MJW_ERR_STATE, "mjw_leapfrog_init has not been called");
+  if (nsteps == 0) return MJW_OK;
+  const bool want = traces && p->nrec > 0;
+  const int64_t need = ((int64_t)nsteps + 1) * p->nshots * std::max(p->nrec, 1);
+  if (want && need > kStaticTraceDoubles)
+    FAIL(p, MJW_ERR_ARG,
+         "mjw_leapfrog_advance_static: (nsteps + 1) * nshots * nrec = " + std::to_string(need) +
+             " trace values exceed the static staging buffer of " +
+             std::to_string(kStaticTraceDoubles) + " (advance in shorter calls)");
+  HIPCHK(p, hipSetDevice(p->device));
+  double *buf = nullptr;
+  if (want) {
+    buf = scratch_of(p->device);
+    if (!buf) FAIL(p, MJW_ERR_STATE, "static trace buffer not set up on this device");
+  }
+  hipStream_t st = p->stream;
+  KCHK(p, launch_set_ctl(p->d_ctl, p->step, p->step, st));
+  if (want) KCHK(p, enqueue_record(p, buf));  // row 0 = U^n
+  for (int k = 0; k < nsteps; ++k) {
+    KCHK(p, enqueue_step(p, p->cur, buf));
+    p->cur ^= 1;
+  }
+  p->step += nsteps;
+  if (want)
+    HIPCHK(p, hipMemcpyAsync(traces, buf, (size_t)nsteps * p->nshots * p->nrec * 8,
+                             hipMemcpyDeviceToHost, st));
+  HIPCHK(p, hipStreamSynchronize(st));
   return MJW_OK;
 }
 

@@ -184,6 +184,8 @@ int launch_mask(int state, int64_t n, int S, int ilv, int64_t N, const uint8_t *
                 void *stream);
 int launch_make_sdiag(int state, int64_t N, const double *M, const uint8_t *dir, double dt2,
                       void *s, void *stream);
+// ctl[0] = step, ctl[1] = base on the device (a kernel, no host copy)
+int launch_set_ctl(int64_t *ctl, int64_t step, int64_t base, void *stream);
 // damped nodes: mode 0 save = y, 1 y += coef (save - u), 2 y += coef u
 int launch_damp(int state, int mode, int64_t ndamp, int ncomp, int S, int ilv, int64_t N,
                 const int32_t *idx, const double *coef, void *y, void *save, const void *u,

@@ -505,6 +505,18 @@ __global__ void k_sdiag(int64_t N, const double *__restrict__ M, const uint8_t *
     s[i] = (dir && dir[i]) ? T(0) : (T)(dt2 / M[i]);
 }
 
+// step counter and trace base row without a host copy: two lanes, vector stores
+__global__ void k_set_ctl(int64_t *__restrict__ ctl, int64_t step, int64_t base) {
+  const int i = threadIdx.x;
+  if (i < 2) ctl[i] = i == 0 ? step : base;
+}
+
+int launch_set_ctl(int64_t *ctl, int64_t step, int64_t base, void *stream) {
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), ctl,
+                     step, base);
+  return (int)hipGetLastError();
+}
+
 int launch_make_sdiag(int state, int64_t N, const double *M, const uint8_t *dir, double dt2,
                       void *s, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
