@@ -1180,3 +1180,289 @@ void or_adjoint_acoustic(int dim, int r, long E, long N, const int *elem_nodes, 
 #undef OR_HVEC
   free(U); free(S); free(t1); free(nu0); free(nu1); free(h);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Time table of the source, any derivative and offset                      */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * out[n] = f^(nb_deriv)(t_begin + n*dt + offset) for n < nsteps: the GL-24
+ * interpolant of the Ricker primitive and its derivatives by repeated
+ * application of the GL differentiation matrix (TimeSourceHyperbolic::
+ * GenerateValues / EvaluateDerivative, Source/TimeSource.cxx:826-880), zero
+ * from t = 2.5/f0 on (AddScalarSourceAtTime, VarInstationary.cxx:2394).  The
+ * second-order leapfrog adds nb_deriv = 1 at offset dt (ModifiedEquation.cxx:
+ * 316, or_pulse_table); the first-order system leapfrog adds nb_deriv = 0 at
+ * offset dt/2 (ModifiedEquation.cxx:784-785).  nb_deriv <= 3.
+ */
+void or_time_table(double f0, double t_begin, double dt, int nsteps, int nb_deriv,
+                   double offset, double *out) {
+  const int R = 24, NP = 25;
+  double x[OR_MAXN1D], w[OR_MAXN1D], D[OR_MAXN1D * OR_MAXN1D], c[OR_MAXN1D];
+  or_gl(R, x, w);
+  or_gl_deriv(R, x, D);
+  cte_phi(R, x, c);
+  double eps = (nsteps * dt) * 1e-15;
+  double tlimit = 2.5 / f0;
+  double t0 = t_begin, t1 = t0 + 0.5 / f0;
+  double V[4][OR_MAXN1D];
+  if (nb_deriv < 0 || nb_deriv > 3) return;
+#define OR_GEN_DERIVS()                                                      \
+  do {                                                                       \
+    for (int i = 0; i < NP; i++)                                             \
+      V[0][i] = ricker_primitive(f0, (1.0 - x[i]) * t0 + x[i] * t1);         \
+    for (int d = 1; d <= nb_deriv; d++)                                      \
+      for (int j = 0; j < NP; j++) {                                         \
+        double s = 0.0;                                                      \
+        for (int k = 0; k < NP; k++) s += D[j * NP + k] * V[d - 1][k];       \
+        V[d][j] = s * (1.0 / (t1 - t0));                                     \
+      }                                                                      \
+  } while (0)
+  OR_GEN_DERIVS();
+  for (int n = 0; n < nsteps; n++) {
+    double ts = (t_begin + n * dt) + offset;
+    if (ts >= tlimit) { out[n] = 0.0; continue; }
+    if ((ts > t1 + eps) && (t1 > t0)) {
+      while (ts > t1 + eps) {
+        double tnext = t1 + (t1 - t0);
+        t0 = t1; t1 = tnext;
+      }
+      OR_GEN_DERIVS();
+    }
+    double p = 0.0, lambda = (ts - t0) / (t1 - t0);
+    for (int i = 0; i < NP; i++) {
+      double ph = 1.0;
+      for (int k = 0; k < NP; k++)
+        if (k != i) ph *= lambda - x[k];
+      p += (ph * c[i]) * V[nb_deriv][i];
+    }
+    out[n] = p;
+  }
+#undef OR_GEN_DERIVS
+}
+
+/* ------------------------------------------------------------------------ */
+/* First-order split system with PML layers (LEAP_FROG_PML)                 */
+/* ------------------------------------------------------------------------ */
+
+/* x = P^{-1} y for a dense k x k block (k <= 3) by Gaussian elimination
+ * with partial pivoting */
+static void or_block_solve(int k, const double *P, const double *y, double *x) {
+  double A[3][4];
+  for (int i = 0; i < k; i++) {
+    for (int j = 0; j < k; j++) A[i][j] = P[i * k + j];
+    A[i][k] = y[i];
+  }
+  for (int c = 0; c < k; c++) {
+    int piv = c;
+    for (int i = c + 1; i < k; i++)
+      if (fabs(A[i][c]) > fabs(A[piv][c])) piv = i;
+    if (piv != c)
+      for (int j = 0; j <= k; j++) { double t = A[c][j]; A[c][j] = A[piv][j]; A[piv][j] = t; }
+    for (int i = c + 1; i < k; i++) {
+      double f = A[i][c] / A[c][c];
+      for (int j = c; j <= k; j++) A[i][j] -= f * A[c][j];
+    }
+  }
+  for (int i = k - 1; i >= 0; i--) {
+    double s = A[i][k];
+    for (int j = i + 1; j < k; j++) s -= A[i][j] * x[j];
+    x[i] = s / A[i][i];
+  }
+}
+
+/*
+ * Montjoie's first-order split acoustic system with PML layers, restated
+ * for the structured (extended) mesh (dim, nx, r, vert):
+ *   D dU/dt = R^S V - S U + F(t) b,     B dV/dt = R^V U - S^V V
+ * with U = (u, u*, u**) -- u on every node, the split copies u* = u_x - u_y
+ * (and u** = u_x - u_z in 3-D) on the nodes of PML elements -- and V (dim
+ * values) at the quadrature points of every element (element matrices of
+ * VarHelmholtz_Base::ComputeElementaryMatrixHelm, first-order branch,
+ * Elliptic/Helmholtz/VarHelmholtz.cxx:296-378,620-660 with
+ * HelmholtzElementaryMatrixClass::ComputeMatrixV / ComputeMatrixVdiag,
+ * :3277-3440,3444-3720, coefficients :133-230; extracted into R^S, R^V by
+ * VarInstationary_Base::ExtractStiffnessMatrix, Instationary/
+ * VarInstationary.cxx:1423-1470; mass and damping blocks
+ * Hyperbolic/Acoustic/TimeAcoustic.cxx:1079-1235):
+ *   (R^S V)_i   = -sum_k w_k (J DF^{-T} grad phi_i)(xi_k) . v_k,
+ *   (R^S V)*_i  = same with v_k replaced by (v_x, -v_y[, 0]),
+ *   (R^S V)**_i = same with (v_x, 0, -v_z);
+ *   (R^V U)_k   =  w_k J DF^{-T} grad u(xi_k)  (main u only);
+ *   D = rho J w lumped (u: every element, u*, u**: PML elements only;
+ *       Dirichlet rows of u = 1);
+ *   S: u: sigma J w outside the PML; in a PML element, per node
+ *       2-D [[s+, s-], [s-, s+]], s+ = rho (sx+sy)/2 Jw, s- = rho (sx-sy)/2 Jw;
+ *       3-D rows (u, u*, u**): [S, d1 - d3, d2 + d3], [d1, S + d3, d1],
+ *       [d2, d2, S - d3] with S = rho (sx+sy+sz)/3 Jw, d1 = rho (sx-sy)/3 Jw,
+ *       d2 = rho (sx-sz)/3 Jw, d3 = rho (sy-sz)/3 Jw; Dirichlet rows 0;
+ *   B = J w / mu, S^V_a = tau_a J w / mu (per component).
+ * Scheme (ModifiedEquationSystemIterator, order 2, ModifiedEquation.cxx:
+ * 511-560 with zero initial data, Advance :773-812, loop RunFirstOrderScheme
+ * :441-481): V^{1/2} = 0; for n: record u^n;
+ *   (D + dt/2 S) U^{n+1} = (D - dt/2 S) U^n + dt (R^S V^{n+1/2} + f_n b),
+ *   u^{n+1} = 0 on Dirichlet nodes,
+ *   (B + dt/2 S^V) V^{n+3/2} = (B - dt/2 S^V) V^{n+1/2} + dt R^V U^{n+1};
+ * record u^nsteps.  f_n = or_time_table(nb_deriv 0, offset dt/2).
+ * pml[e] != 0 marks PML elements, tau[(e*nd + q)*dim + a] the damping at
+ * their quadrature points (GL nodes); sigma (per element, NULL = 0) the
+ * physical damping outside the PML.  Outputs: uN[N] = u^nsteps,
+ * traces[(nsteps+1)*nrec] of u as in or_leapfrog.
+ */
+void or_leapfrog_pml(int dim, int nx, int r, const double *vert, long N, const int *elem_nodes,
+                     const double *rho, const double *mu, const double *sigma,
+                     const unsigned char *pml, const double *tau,
+                     const unsigned char *dirichlet, const double *b, const double *f, double dt,
+                     int nsteps, int nrec, const long *rec_elem, const double *rec_w,
+                     double *uN, double *traces) {
+  int n = r + 1, nd = (dim == 2) ? n * n : n * n * n, d = dim;
+  long E = (dim == 2) ? (long)nx * nx : (long)nx * nx * nx;
+  double xg[OR_MAXN], wg[OR_MAXN], Dm[OR_MAXN * OR_MAXN];
+  or_gl(r, xg, wg);
+  or_gl_deriv(r, xg, Dm);
+  /* per element node: Gk = w J DF^{-1} (row-major dim x dim), Jw */
+  double *Gk = (double *)malloc(sizeof(double) * E * nd * d * d);
+  double *Jw = (double *)malloc(sizeof(double) * E * nd);
+  for (long e = 0; e < E; e++) {
+    long cv[8];
+    elem_corners(dim, nx, e, cv);
+    for (int q = 0; q < nd; q++) {
+      int loc[3] = {q % n, (q / n) % n, q / (n * n)};
+      double xi[3], N8[8], dN[24], DF[9], Di[9], wq = 1.0;
+      for (int a = 0; a < dim; a++) { xi[a] = xg[loc[a]]; wq *= wg[loc[a]]; }
+      multilinear(dim, xi, N8, dN);
+      for (int a = 0; a < dim; a++)
+        for (int c = 0; c < dim; c++) {
+          double s = 0.0;
+          for (int k = 0; k < (1 << dim); k++) s += dN[k * dim + c] * vert[cv[k] * dim + a];
+          DF[a * dim + c] = s;
+        }
+      double J = det_inv(dim, DF, Di);
+      long o = e * nd + q;
+      Jw[o] = J * wq;
+      for (int a = 0; a < d * d; a++) Gk[o * d * d + a] = wq * J * Di[a];
+    }
+  }
+  /* assembled mass and damping blocks per node: D[c][g] for copy c (0 = u),
+   * S[g][(c1*d + c2)] */
+  double *Dn = (double *)calloc((size_t)d * N, sizeof(double));
+  double *Sn = (double *)calloc((size_t)N * d * d, sizeof(double));
+  unsigned char *inpml = (unsigned char *)calloc((size_t)N, 1);
+  for (long e = 0; e < E; e++)
+    for (int q = 0; q < nd; q++) {
+      long o = e * nd + q;
+      int g = elem_nodes[o];
+      double m = rho[e] * Jw[o];
+      Dn[g] += m;
+      double *S = Sn + (long)g * d * d;
+      if (!pml[e]) {
+        if (sigma) S[0] += sigma[e] * Jw[o];
+        continue;
+      }
+      inpml[g] = 1;
+      for (int c = 1; c < d; c++) Dn[(long)c * N + g] += m;
+      const double *t = tau + o * d;
+      if (dim == 2) {
+        double sp = rho[e] * (0.5 * (t[0] + t[1])) * Jw[o];
+        double sm = rho[e] * (0.5 * (t[0] - t[1])) * Jw[o];
+        S[0] += sp; S[3] += sp; S[1] += sm; S[2] += sm;
+      } else {
+        double s0 = rho[e] * (t[0] + t[1] + t[2]) * Jw[o] / 3.0;
+        double d1 = rho[e] * (t[0] - t[1]) * Jw[o] / 3.0;
+        double d2 = rho[e] * (t[0] - t[2]) * Jw[o] / 3.0;
+        double d3 = rho[e] * (t[1] - t[2]) * Jw[o] / 3.0;
+        S[0] += s0; S[1] += d1 - d3; S[2] += d2 + d3;
+        S[3] += d1; S[4] += s0 + d3; S[5] += d1;
+        S[6] += d2; S[7] += d2; S[8] += s0 - d3;
+      }
+    }
+  for (long g = 0; g < N; g++)
+    if (dirichlet[g]) {
+      Dn[g] = 1.0;
+      for (int c = 0; c < d; c++) { Sn[g * d * d + c] = 0.0; Sn[g * d * d + c * d] = 0.0; }
+    }
+  double *U = (double *)calloc((size_t)d * N, sizeof(double));
+  double *Pr = (double *)calloc((size_t)d * N, sizeof(double));
+  double *V = (double *)calloc((size_t)E * nd * d, sizeof(double));
+  for (int it = 0; it <= nsteps; it++) {
+    for (int s = 0; s < nrec; s++) {
+      double acc = 0.0;
+      long e = rec_elem[s];
+      for (int q = 0; q < nd; q++) acc += rec_w[(long)s * nd + q] * U[elem_nodes[e * nd + q]];
+      traces[(long)it * nrec + s] = acc;
+    }
+    if (it == nsteps) break;
+    /* U^{n+1}: Pr holds R^S V^{n+1/2} */
+    for (long g = 0; g < N; g++) {
+      if (b) Pr[g] += f[it] * b[g];
+      int k = inpml[g] ? d : 1;
+      double A[9], P[9], y[3], x[3];
+      for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++) {
+          double sv = Sn[g * d * d + i * d + j];
+          double dg = (i == j) ? Dn[(long)i * N + g] : 0.0;
+          A[i * k + j] = dg - 0.5 * dt * sv;
+          P[i * k + j] = dg + 0.5 * dt * sv;
+        }
+      for (int i = 0; i < k; i++) {
+        double s = dt * Pr[(long)i * N + g];
+        for (int j = 0; j < k; j++) s += A[i * k + j] * U[(long)j * N + g];
+        y[i] = s;
+      }
+      or_block_solve(k, P, y, x);
+      for (int i = 0; i < k; i++) U[(long)i * N + g] = x[i];
+      if (dirichlet[g]) U[g] = 0.0;
+    }
+    /* V^{n+3/2} and the next R^S V */
+    memset(Pr, 0, sizeof(double) * d * N);
+    for (long e = 0; e < E; e++) {
+      double ue[OR_MAXN * OR_MAXN * OR_MAXN];
+      double fl[3][3 * OR_MAXN * OR_MAXN * OR_MAXN];
+      for (int q = 0; q < nd; q++) ue[q] = U[elem_nodes[e * nd + q]];
+      for (int q = 0; q < nd; q++) {
+        int i = q % n, j = (q / n) % n, kk = q / (n * n);
+        double gr[3] = {0, 0, 0};
+        for (int m = 0; m < n; m++) {
+          gr[0] += Dm[i * n + m] * ue[m + n * j + n * n * kk];
+          gr[1] += Dm[j * n + m] * ue[i + n * m + n * n * kk];
+          if (dim == 3) gr[2] += Dm[kk * n + m] * ue[i + n * j + n * n * m];
+        }
+        long o = e * nd + q;
+        const double *G = Gk + o * d * d;
+        double *v = V + o * d;
+        double Bq = Jw[o] / mu[e];
+        for (int a = 0; a < d; a++) {
+          double rv = 0.0; /* (w J DF^{-T} grad u)_a = sum_c G[c][a] gr_c */
+          for (int c = 0; c < d; c++) rv += G[c * d + a] * gr[c];
+          double sv = pml[e] ? tau[o * d + a] * Jw[o] / mu[e] : 0.0;
+          double pv = dt * rv + (Bq - 0.5 * dt * sv) * v[a];
+          v[a] = pv * (1.0 / (Bq + 0.5 * dt * sv));
+        }
+        /* fluxes -G P v for the copies */
+        for (int c = 0; c < (pml[e] ? d : 1); c++) {
+          double pvv[3] = {v[0], v[1], dim == 3 ? v[2] : 0.0};
+          if (c == 1) { pvv[1] = -v[1]; if (dim == 3) pvv[2] = 0.0; }
+          if (c == 2) { pvv[1] = 0.0; pvv[2] = -v[2]; }
+          for (int a = 0; a < d; a++) {
+            double s = 0.0;
+            for (int cc = 0; cc < d; cc++) s += G[a * d + cc] * pvv[cc];
+            fl[c][q * d + a] = -s;
+          }
+        }
+      }
+      for (int c = 0; c < (pml[e] ? d : 1); c++)
+        for (int q = 0; q < nd; q++) {
+          int i = q % n, j = (q / n) % n, kk = q / (n * n);
+          double s = 0.0;
+          for (int m = 0; m < n; m++) {
+            s += Dm[m * n + i] * fl[c][(m + n * j + n * n * kk) * d + 0];
+            s += Dm[m * n + j] * fl[c][(i + n * m + n * n * kk) * d + 1];
+            if (dim == 3) s += Dm[m * n + kk] * fl[c][(i + n * j + n * n * m) * d + 2];
+          }
+          Pr[(long)c * N + elem_nodes[e * nd + q]] += s;
+        }
+    }
+  }
+  memcpy(uN, U, sizeof(double) * N);
+  free(Gk); free(Jw); free(Dn); free(Sn); free(inpml); free(U); free(Pr); free(V);
+}

@@ -86,6 +86,11 @@ def lib():
     L.or_adjoint_acoustic.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, _ip, _dp, _dp, _ip,
                                       C.c_int, _dp, _bp, C.c_void_p, C.c_void_p, C.c_double,
                                       C.c_int, C.c_int, _lp, _dp, _dp, _dp]
+    L.or_time_table.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                C.c_double, _dp]
+    L.or_leapfrog_pml.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_long, _ip, _dp, _dp,
+                                  C.c_void_p, _bp, _dp, _bp, C.c_void_p, _dp, C.c_double,
+                                  C.c_int, C.c_int, _lp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -788,3 +793,132 @@ def forward_traces(P: Problem, b, pulse, dt, nsteps, rec_pts):
     """Traces rows 0..nsteps of the zero-initial-state leapfrog."""
     _, _, tr = leapfrog(P, b, pulse, dt, nsteps, rec_pts)
     return tr
+
+
+# --------------------------------------------------------------------------
+# PML layers and the first-order split scheme (LEAP_FROG_PML)
+# --------------------------------------------------------------------------
+
+def time_table(f0: float, t_begin: float, dt: float, nsteps: int, nb_deriv: int,
+               offset: float) -> np.ndarray:
+    """f^(nb_deriv)(t_begin + n dt + offset), n < nsteps, of the GL-24
+    interpolated Ricker primitive (or_time_table)."""
+    out = np.zeros(nsteps)
+    lib().or_time_table(f0, t_begin, dt, nsteps, nb_deriv, offset, out)
+    return out
+
+
+@dataclass
+class PmlMesh:
+    """A structured mesh of [0,1]^dim with PML layers on every side, and the
+    PML data of its elements."""
+    mesh: Mesh
+    nx: int                   # elements per direction of the physical mesh
+    layers: int
+    thickness: float
+    pml: np.ndarray           # (E,) uint8: element inside the layers
+    kind: np.ndarray          # (E, dim) int: -1 / +1 layer below / above the box, 0 none
+    phys: np.ndarray          # (E,) physical element whose medium the element takes
+
+
+def pml_mesh(dim: int, nx: int, r: int, pert: float, seed: int, thickness: float,
+             layers: int) -> PmlMesh:
+    """The structured mesh of [0,1]^dim (vertices perturbed like mesh())
+    extended by `layers` layers of elements of width thickness/layers on
+    every side, the way AddPML = YES XY(Z) thickness layers extrudes a box
+    mesh (PmlRegionParameter::AddPMLElements, Mesh/Mesh2DBoundaries.cxx:
+    104-192, Mesh3DBoundaries.cxx:59-; Mesh<>::ExtrudeCoordinate, Mesh/
+    Mesh2D.cxx:4898-4990: new vertex coordinate pos + (i+1) delta / layers,
+    the other coordinates copied from the boundary vertex; axes in the order
+    x, y, z).  A PML element takes the medium of the physical element it was
+    extruded from (its indices clamped into the physical block) and the
+    layer kind of its centre relative to the box [0,1]^dim."""
+    L = lib()
+    nv1 = nx + 1
+    vphys = np.zeros(nv1 ** dim * dim)
+    L.or_mesh_vertices(dim, nx, pert, seed, vphys)
+    vphys = vphys.reshape(-1, dim)
+    nt = nx + 2 * layers
+    nt1 = nt + 1
+
+    def coord(i):
+        # layer coordinates of one axis, or None inside the physical range
+        if i < layers:
+            return 0.0 + ((layers - i) * (-thickness)) / layers
+        if i > layers + nx:
+            return 1.0 + ((i - layers - nx) * thickness) / layers
+        return None
+
+    vert = np.zeros((nt1 ** dim, dim))
+    for v in range(nt1 ** dim):
+        I = [(v // nt1 ** a) % nt1 for a in range(dim)]
+        inside = all(layers <= I[a] <= layers + nx for a in range(dim))
+        if inside:
+            pv = sum((I[a] - layers) * nv1 ** a for a in range(dim))
+            vert[v] = vphys[pv]
+            continue
+        for a in range(dim):
+            c = coord(I[a])
+            if c is None:
+                # copy of a boundary vertex of the box: regular coordinate
+                ia = I[a] - layers
+                c = 1.0 if ia == nx else ia * (1.0 / nx)
+            vert[v, a] = c
+    vert = np.ascontiguousarray(vert.reshape(-1))
+    N = (nt * r + 1) ** dim
+    E = nt ** dim
+    nd = (r + 1) ** dim
+    xyz = np.zeros(N * dim)
+    en = np.zeros(E * nd, dtype=np.int32)
+    dr = np.zeros(N, dtype=np.uint8)
+    if L.or_mesh_nodes(dim, nt, r, vert, xyz, en, dr):
+        raise ValueError("mesh")
+    m = Mesh(dim, nt, r, pert, seed, vert, xyz.reshape(N, dim), en.reshape(E, nd), dr)
+    e = np.arange(E)
+    idx = np.stack([(e // nt ** a) % nt for a in range(dim)], axis=1)
+    kind = np.where(idx < layers, -1, np.where(idx >= layers + nx, 1, 0))
+    cl = np.clip(idx - layers, 0, nx - 1)
+    phys = sum(cl[:, a] * nx ** a for a in range(dim))
+    pml = (kind != 0).any(axis=1).astype(np.uint8)
+    return PmlMesh(m, nx, layers, thickness, pml, kind, phys)
+
+
+def pml_tau(pm: PmlMesh, vsigma: float, vmax: float) -> np.ndarray:
+    """Damping tau[e, q, a] at the quadrature points (GL nodes) of every
+    element: parabolic profile dsig (x_a - box_a)^2 in the layers beyond
+    the box face of the element's kind, dsig = log(1000) 3 vsigma vmax /
+    (2 a^3), a = thickness (VarBoundaryCondition_Base::GetDampingTauPML and
+    EvaluateFunctionTauPML (PML_PARABOLE), Harmonic/BoundaryConditionHarmonic.
+    cxx:1639-1700,1548-1570; first-order formulation keeps tau itself,
+    GetDampingFactorPML :1588-1636); vmax = max sqrt(mu / rho) over the
+    media of the PML elements (:1486-1522)."""
+    m = pm.mesh
+    a = pm.thickness
+    dsig = np.log(1000e0) * 3e0 * vsigma * vmax / (2 * a ** 3)
+    X = m.xyz[m.elem_nodes]                       # (E, nd, dim)
+    tau = np.zeros(X.shape)
+    for ax in range(m.dim):
+        k = pm.kind[:, ax][:, None]
+        dlo = X[:, :, ax] - 0.0
+        dhi = X[:, :, ax] - 1.0
+        tau[:, :, ax] = np.where(k == -1, dsig * dlo * dlo, np.where(k == 1, dsig * dhi * dhi, 0.0))
+    return tau
+
+
+def leapfrog_pml(pm: PmlMesh, rho_e, mu_e, tau, b, table, dt, nsteps, rec_pts, sigma_e=None):
+    """First-order split scheme with PML (or_leapfrog_pml): returns
+    (u^nsteps, traces[nsteps+1, nrec]); rho_e, mu_e per element of the
+    extended mesh, table = time_table(..., 0, dt/2)."""
+    m = pm.mesh
+    re, rw = receivers(m, rec_pts)
+    uN = np.zeros(m.N)
+    tr = np.zeros((nsteps + 1) * max(len(re), 1))
+    sg = None if sigma_e is None else np.ascontiguousarray(sigma_e, np.float64)
+    lib().or_leapfrog_pml(m.dim, m.nx, m.r, m.vert, m.N, m.elem_nodes.reshape(-1),
+                          np.ascontiguousarray(rho_e, np.float64),
+                          np.ascontiguousarray(mu_e, np.float64), _ptr(sg), pm.pml,
+                          np.ascontiguousarray(tau, np.float64).reshape(-1), m.dirichlet,
+                          _ptr(np.ascontiguousarray(b, np.float64)),
+                          np.ascontiguousarray(table, np.float64), dt, nsteps, len(re),
+                          np.ascontiguousarray(re), np.ascontiguousarray(rw.reshape(-1)), uN, tr)
+    return uN, tr[: (nsteps + 1) * len(re)].reshape(nsteps + 1, len(re))

@@ -34,6 +34,7 @@ class Case:
     times: np.ndarray | None = None
     traces: np.ndarray | None = None
     sigma_e: np.ndarray | None = None
+    pml: tuple | None = None       # (thickness, layers, vsigma): AddPML + LEAP_FROG_PML
 
 
 def golden_names():
@@ -84,9 +85,23 @@ def load(name: str) -> Case:
         f = sinus_field(ssg, O.quadrature_points(m))
         base = sg[lay] if len(sg) else np.zeros(E)
         sigma_e = np.where((lay == 0)[:, None], f, base[:, None])
+    pml = None
+    if "pml" in z.files and len(z["pml"]):
+        pml = (float(z["pml"][0]), int(z["pml"][1]), float(z["pml"][2]))
     return Case(name, dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]), elastic, mu_e,
                 rho_e, lam_e, float(z["f0"]), float(z["dt"]), int(z["nsteps"]), z["src"],
-                z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e)
+                z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e, pml)
+
+
+def pml_setup(c: Case):
+    """(extended mesh, rho, mu per extended element, tau) of a PML case: the
+    layers extruded as AddPML does, media of the element each was extruded
+    from, parabolic damping with vmax over the PML media."""
+    thick, layers, vsigma = c.pml
+    pm = O.pml_mesh(c.dim, c.nx, c.r, c.pert, c.seed, thick, layers)
+    mu, rho = c.mu_e[pm.phys], c.rho_e[pm.phys]
+    vmax = float(np.sqrt(mu[pm.pml == 1] / rho[pm.pml == 1]).max())
+    return pm, rho, mu, O.pml_tau(pm, vsigma, vmax)
 
 
 def sinus_field(p, pts):
@@ -101,6 +116,12 @@ def sinus_field(p, pts):
 def oracle_run(c: Case, nsteps: int | None = None):
     """Run the case through the C oracle: returns (uN, uN1, traces[n, rec])."""
     nsteps = c.nsteps if nsteps is None else nsteps
+    if c.pml is not None:
+        pm, rho, mu, tau = pml_setup(c)
+        tab = O.time_table(c.f0, 0.0, c.dt, nsteps, 0, 0.5 * c.dt)
+        uN, tr = O.leapfrog_pml(pm, rho, mu, tau, O.dirac(pm.mesh, c.src), tab, c.dt, nsteps,
+                                c.rec)
+        return uN, None, tr
     m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     if c.elastic:
         P = O.elastic_problem(m, c.lam_e, c.mu_e, c.rho_e)

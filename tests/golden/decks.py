@@ -93,9 +93,11 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
                mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
-               output_format="BINARY DOUBLE", circle=None):
+               output_format="BINARY DOUBLE", circle=None, pml=None):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  circle = (x0, y0,
-    radius, n): a SismoCircle line after the SismoPoint lines.  sinus_rho /
+    radius, n): a SismoCircle line after the SismoPoint lines.  pml = (axes,
+    thickness, layers, vsigma): AddPML = YES axes thickness layers, DampingPML =
+    vsigma and the first-order scheme OrderTimeScheme = 2 LEAP_FROG_PML.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1.
     save_reprise > 0: checkpoint files every save_reprise steps.  sigma_layers:
     acoustic damping of each layer (MateriauDielec's third value)."""
@@ -124,6 +126,10 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     ini.append(f"OrderDiscretization = {r}")
     bref = 1 if regular else nl + 1
     ini.append(f"ConditionReference = {bref} DIRICHLET")
+    if pml is not None and len(pml):
+        axes = {2: "XY", 3: "XYZ"}[dim] if isinstance(pml[0], (int, float, np.floating)) else pml[0]
+        ini.append(f"AddPML = YES {axes} {_f(pml[1])} {int(pml[2])}")
+        ini.append(f"DampingPML = {_f(pml[3])}")
     for k in range(nl):
         if elastic:
             ini.append(f"MateriauDielec = {k + 1} ISOTROPE {_f(rho_layers[k])} "
@@ -170,7 +176,8 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     ini.append("ElectricOrMagnetic = " + ("-1" if elastic else "0"))
     ini.append(f"TimeStep = {_f(dt)}")
     ini.append(f"TimeInterval = 0.0 {_f(T)}")
-    ini.append("OrderTimeScheme = 2 LEAP_FROG")
+    ini.append("OrderTimeScheme = 2 " + ("LEAP_FROG_PML" if pml is not None and len(pml)
+                                           else "LEAP_FROG"))
     ini.append("TemporalSource = RICKER")
     ini.append("PrintLevel = 0")
     with open(os.path.join(d, "case.ini"), "w") as fh:

@@ -36,14 +36,14 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
              mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
-             output_format="BINARY DOUBLE", circle=None):
+             output_format="BINARY DOUBLE", circle=None, pml=None):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
                sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times,
                line=line, line_times=line_times, mesh_media=mesh_media,
                save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma,
-               output_format=output_format, circle=circle)
+               output_format=output_format, circle=circle, pml=pml)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
     for f in os.listdir(d):
@@ -108,6 +108,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         sinus_sigma=np.asarray(sinus_sigma if sinus_sigma is not None else [], float),
         output_format=output_format,
         circle=np.asarray(circle if circle is not None else [], float),
+        pml=np.asarray(pml[1:] if pml is not None else [], float),
         circle_files=circles,
         circle_names=names.get("totalCircle", np.zeros(0, "U1")),
         plane_names=names.get("totalPlane", np.zeros(0, "U1")),
@@ -169,6 +170,27 @@ def ref2d_q4_circle_out():
     SismoPoint receivers."""
     run_case("ref2d_q4_circle_out", 2, 8, 4, 0.1, 45, [1.0, 2.0], 10.0, 1e-3, 200, (0.46, 0.53),
              np.array([[0.3, 0.4], [0.71, 0.52]]), circle=(0.75, 0.45, 0.4, 12))
+
+
+@case
+def ref2d_q4_pml():
+    """8x8 Q4 perturbed quadrilaterals, two layers, PML of two layers of
+    thickness 0.125 on every side (AddPML = YES XY 0.25 2, parabolic
+    DampingPML = 3), first-order scheme LEAP_FROG_PML; two receivers inside
+    the PML."""
+    rec = np.array([[0.3, 0.4], [0.71, 0.52], [0.86, 0.18], [1.12, 0.5], [-0.1, -0.15]])
+    run_case("ref2d_q4_pml", 2, 8, 4, 0.1, 46, [1.0, 2.0], 10.0, 1e-3, 400, (0.47, 0.52), rec,
+             pml=(2, 0.25, 2, 3.0))
+
+
+@case
+def ref3d_q3_pml():
+    """4^3 Q3 perturbed hexahedra, two layers, PML of two layers of thickness
+    0.25 on every face (AddPML = YES XYZ 0.5 2, DampingPML = 2), LEAP_FROG_PML;
+    one receiver inside the PML corner region."""
+    rec = np.vstack([seeded_points(3, 4, 93), [[1.2, -0.15, 0.5], [-0.3, 1.3, 1.2]]])
+    run_case("ref3d_q3_pml", 3, 4, 3, 0.1, 47, [1.0, 2.5], 8.0, 4e-3, 150, (0.46, 0.53, 0.49),
+             rec, pml=(3, 0.5, 2, 2.0))
 
 
 @case

@@ -482,3 +482,60 @@ def test_oracle_circle_matches_montjoie(name):
     err = CS.rel_l2(got, ref)
     print(f"{name}: {int(inside.sum())}/{n} points inside, rel-L2 {err:.2e}")
     assert err < 1e-10
+
+
+PMLS = [n for n in CS.golden_names() if n.endswith("_pml")]
+
+
+@pytest.mark.parametrize("name", PMLS)
+def test_oracle_pml_matches_montjoie(name):
+    """AddPML layers + DampingPML + OrderTimeScheme = 2 LEAP_FROG_PML run by
+    Montjoie: the oracle's first-order split scheme on its extruded mesh
+    matches the seismograms (receivers inside the layers included) tighter
+    than the generic 1e-10 golden test."""
+    c = CS.load(name)
+    pm, _, _, _ = CS.pml_setup(c)
+    assert int(pm.pml.sum()) == pm.mesh.E - c.nx ** c.dim
+    _, _, tr = CS.oracle_run(c)
+    err = CS.rel_l2(tr, c.traces)
+    print(f"{name}: {pm.mesh.E} elements ({int(pm.pml.sum())} in the PML), rel-L2 {err:.2e}")
+    assert err < 1e-12
+
+
+def test_time_table_derivatives():
+    """or_time_table: nb_deriv 1 at offset dt is the leapfrog pulse table;
+    nb_deriv 0 and 1 follow the closed-form Ricker primitive and pulse to the
+    interpolation error; zero from 2.5/f0 on."""
+    f0, dt, n = 8.0, 1e-3, 500
+    assert np.array_equal(O.time_table(f0, 0.0, dt, n, 1, dt), O.pulse_table(f0, 0.0, dt, n))
+    t = np.arange(n) * dt + 0.5 * dt
+    a = f0 * t - 1.0
+    g = np.exp(-(np.pi * a) ** 2)
+    live = t < 2.5 / f0
+    p0 = O.time_table(f0, 0.0, dt, n, 0, 0.5 * dt)
+    p1 = O.time_table(f0, 0.0, dt, n, 1, 0.5 * dt)
+    assert np.abs(p0[live] - (-a * g / f0)[live]).max() < 1e-11
+    assert np.abs(p1[live] - ((2 * np.pi ** 2 * a * a - 1) * g)[live]).max() < 1e-9
+    assert not p0[~live].any() and not p1[~live].any()
+
+
+def test_pml_mesh_geometry():
+    """The extruded layers: vertex coordinates of every layer, kinds and
+    media of the corner elements, profile zero on the box faces and
+    dsig a^2 on the outer faces."""
+    pm = O.pml_mesh(2, 4, 3, 0.1, 5, 0.3, 3)
+    m = pm.mesh
+    assert m.nx == 10 and pm.pml.sum() == 100 - 16
+    xs = np.unique(np.round(m.vert.reshape(-1, 2)[:, 0], 12))
+    for k in range(1, 4):
+        assert np.any(np.isclose(xs, -0.1 * k)) and np.any(np.isclose(xs, 1 + 0.1 * k))
+    assert tuple(pm.kind[0]) == (-1, -1) and pm.phys[0] == 0
+    assert tuple(pm.kind[99]) == (1, 1) and pm.phys[99] == 15
+    tau = O.pml_tau(pm, 2.0, 1.5)
+    dsig = np.log(1000.0) * 3 * 2.0 * 1.5 / (2 * 0.3 ** 3)
+    X = m.xyz[m.elem_nodes]
+    outer = np.isclose(X[:, :, 0], -0.3) & (pm.kind[:, 0] == -1)[:, None]
+    assert np.allclose(tau[:, :, 0][outer], dsig * 0.09)
+    face = np.isclose(X[:, :, 0], 0.0)
+    assert np.all(tau[:, :, 0][face] == 0.0)
+    assert np.all(tau[pm.pml == 0] == 0.0)
