@@ -381,6 +381,40 @@ int mjw_set_damping(mjw_problem *p, const double *sigma);
  * PhysicalMedia sigma field, Elliptic/Helmholtz/VarHelmholtz.cxx:1219-1226) */
 int mjw_set_damping_nodal(mjw_problem *p, const double *sigma_q);
 
+/* ------------------------------------------------- PML layers (LEAP_FROG_PML) */
+/*
+ * Perfectly matched layers with Montjoie's first-order split formulation and
+ * its leapfrog (OrderTimeScheme = 2 LEAP_FROG_PML: ModifiedEquationSystemIterator,
+ * Instationary/ModifiedEquation.hxx:123-220, .cxx:441-481,511-560,773-812).
+ * The mesh given to mjw_create already holds the layers (AddPML extrudes them,
+ * Mesh/MeshBoundaries.cxx:114-120 -> Mesh2DBoundaries.cxx:104-192); pml[e] != 0
+ * marks the elements inside them and tau[(e*nd + q)*dim + a] >= 0 is the
+ * damping sigma_a at their quadrature points (the GL nodes, caller's element
+ * order; GetDampingTauPML, Harmonic/BoundaryConditionHarmonic.cxx:1639-1760;
+ * ignored outside the layers).  The unknowns become u on every node, its split
+ * copies u* = u_x - u_y (and u** = u_x - u_z in 3-D) on the nodes of PML
+ * elements, and the vector V at every quadrature point:
+ *   D dU/dt = R^S V - S U + f(t) b,   B dV/dt = R^V U - S^V V
+ * (element matrices Elliptic/Helmholtz/VarHelmholtz.cxx:133-230,296-378,
+ * 620-660,3277-3720; block-diagonal D - dt/2 S, (D + dt/2 S)^{-1},
+ * Hyperbolic/Acoustic/TimeAcoustic.cxx:1079-1235).  mjw_leapfrog_init then
+ * starts this scheme from zero data (u0 = du0 = NULL) and mjw_leapfrog_advance
+ * runs it: traces row n = R u^n; the pulse table of mjw_set_pulse holds the
+ * source's primitive at t_n + dt/2 (mjw_time_source_table with nb_deriv = 0,
+ * offset = dt/2, ModifiedEquation.cxx:784-785).  Acoustic, one shot, per-element
+ * media, no physical damping; energy, adjoint, batches and mjw_profile refuse
+ * such problems, and mjw_get_state returns u^n only.  Call once.
+ */
+int mjw_set_pml(mjw_problem *p, const uint8_t *pml, const double *tau);
+/*
+ * out[n] = f^(nb_deriv)(t_begin + n*dt + offset), n < nsteps, for the Ricker
+ * source through Montjoie's degree-24 Gauss-Lobatto interpolator (window
+ * tolerance nsteps*dt*1e-15, zero from 2.5/f0 on); nb_deriv 0 (primitive) to 9.
+ * mjw_ricker_pulse_table is the case nb_deriv = 1, offset = dt.  Host only.
+ */
+int mjw_time_source_table(double f0, double t_begin, double dt, int nsteps, int nb_deriv,
+                          double offset, double *out);
+
 /* --------------------------------------------------------- synthetic meshes */
 /*
  * Regular Q_r mesh of [0,1]^dim with nx elements per direction (Montjoie's

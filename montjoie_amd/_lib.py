@@ -55,6 +55,9 @@ SIGNATURES = {
     "mjw_set_receivers": (C.c_int, [_vp, C.c_int, _dp, _i32p]),
     "mjw_set_receivers_grad": (C.c_int, [_vp, C.c_int, _dp, _i32p, _i32p, _i64p]),
     "mjw_set_damping": (C.c_int, [_vp, _dp]),
+    "mjw_set_pml": (C.c_int, [_vp, _u8p, _dp]),
+    "mjw_time_source_table": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                        C.c_double, _dp]),
     "mjw_set_damping_nodal": (C.c_int, [_vp, _dp]),
     "mjw_set_probes_grad": (C.c_int, [_vp, C.c_int, _dp, _i32p, _u8p, _i64p]),
     "mjw_leapfrog_init": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _dp]),

@@ -128,6 +128,16 @@ struct mjw_problem {
   double *d_adj_w = nullptr;
   hipGraphExec_t bexec = nullptr;     // two backward steps
   double ms_adj_fwd = 0.0, ms_adj_bwd = 0.0;
+  // PML layers: first-order split system (mjw_set_pml)
+  bool pml = false;
+  std::vector<double> pml_D;          // [d][N] lumped mass of u, u*, u** (solver order)
+  std::vector<double> pml_S;          // [npn][d][d] damping blocks of the PML nodes
+  std::vector<int32_t> pml_node;      // [npn] their solver indices
+  double *d_pml_G = nullptr, *d_pml_B = nullptr, *d_pml_tau = nullptr, *d_pml_Dm = nullptr;
+  uint8_t *d_pml_flag = nullptr, *d_pml_blk = nullptr;
+  void *d_pml_V = nullptr, *d_pml_Pr = nullptr;
+  int32_t *d_pml_node = nullptr;
+  double *d_pml_A = nullptr, *d_pml_Pinv = nullptr, *d_pml_sa = nullptr, *d_pml_sinv = nullptr;
 };
 
 #define FAIL(p, code, msg)                  \
@@ -239,6 +249,21 @@ extern "C" int mjw_ricker_pulse_table(double f0, double t_begin, double dt, int 
   return MJW_OK;
 }
 
+extern "C" int mjw_time_source_table(double f0, double t_begin, double dt, int nsteps,
+                                     int nb_deriv, double offset, double *out) {
+  if (!(f0 > 0) || !(dt > 0) || nsteps < 0 || (!out && nsteps > 0) || nb_deriv < 0 ||
+      nb_deriv >= GlTimeSource::kNDer || !std::isfinite(offset))
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_time_source_table: bad arguments");
+  GlTimeSource src;
+  src.init(f0, t_begin, (nsteps * dt) * 1e-15);
+  const double tcut = 2.5 / f0;
+  for (int n = 0; n < nsteps; ++n) {
+    const double t = (t_begin + n * dt) + offset;
+    out[n] = t >= tcut ? 0.0 : src.eval(t, nb_deriv);
+  }
+  return MJW_OK;
+}
+
 // --------------------------------------------------------------- lifecycle
 static void release(mjw_problem *p) {
   if (!p) return;
@@ -261,6 +286,10 @@ static void release(mjw_problem *p) {
                    p->d_adj_ptr, p->d_adj_rec, p->d_adj_w};
   for (void *q : aptrs) dfree(q);
   if (p->bexec) (void)hipGraphExecDestroy(p->bexec);
+  void *pptrs[] = {p->d_pml_G, p->d_pml_B, p->d_pml_tau, p->d_pml_Dm, p->d_pml_flag,
+                   p->d_pml_blk, p->d_pml_V, p->d_pml_Pr, p->d_pml_node, p->d_pml_A,
+                   p->d_pml_Pinv, p->d_pml_sa, p->d_pml_sinv};
+  for (void *q : pptrs) dfree(q);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
 }
@@ -1163,9 +1192,13 @@ extern "C" int mjw_set_receivers_grad(mjw_problem *p, int nrec, const double *pt
 }
 
 // ---------------------------------------------------------------- leapfrog
+static int pml_init(mjw_problem *p, double t_begin, double dt, const double *u0,
+                    const double *du0);
+
 extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, const double *u0,
                                  const double *du0) {
   if (!p || !(dt > 0.0) || !std::isfinite(dt)) FAIL(p, MJW_ERR_ARG, "mjw_leapfrog_init: bad dt");
+  if (p->pml) return pml_init(p, t_begin, dt, u0, du0);
   HIPCHK(p, hipSetDevice(p->device));
   hipStream_t st = p->stream;
   p->dt = dt;
@@ -1245,7 +1278,10 @@ extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, cons
 // U^n in d_U[parity] and U^{n+1} in d_U[parity^1], the colour passes apply
 // the stiffness to U^{n+1} and overwrite U^n with U^{n+2}; the tail adds the
 // source and records the traces of U^{n+1} (row n+1).
+static int enqueue_step_pml(mjw_problem *p, double *traces);
+
 static int enqueue_step(mjw_problem *p, int parity, double *traces) {
+  if (p->pml) return enqueue_step_pml(p, traces);
   void *Ucur = p->d_U[parity ^ 1], *Y = p->d_U[parity];
   if (p->ndamp) {
     // keep U^n on the damped nodes before the passes overwrite it
@@ -1331,7 +1367,8 @@ extern "C" int mjw_leapfrog_advance(mjw_problem *p, int nsteps, double *traces, 
   hipEvent_t ev0, ev1;
   HIPCHK(p, hipEventCreate(&ev0));
   HIPCHK(p, hipEventCreate(&ev1));
-  if (want) KCHK(p, enqueue_record(p, p->d_traces));  // row 0 = U^n
+  // row 0 = U^n (the first-order system records U^n in each step's tail)
+  if (want && !p->pml) KCHK(p, enqueue_record(p, p->d_traces));
   HIPCHK(p, hipEventRecord(ev0, st));
   int done = 0;
   if (use_graph && nsteps >= kGraphSteps) {
@@ -1354,6 +1391,7 @@ extern "C" int mjw_leapfrog_advance(mjw_problem *p, int nsteps, double *traces, 
     KCHK(p, enqueue_step(p, p->cur, want ? p->d_traces : nullptr));
     p->cur ^= 1;
   }
+  if (p->pml) p->cur = 0;  // one state buffer
   HIPCHK(p, hipEventRecord(ev1, st));
   HIPCHK(p, hipEventSynchronize(ev1));
   float ms = 0.f;
@@ -1387,11 +1425,12 @@ extern "C" int mjw_leapfrog_advance_static(mjw_problem *p, int nsteps, double *t
   }
   hipStream_t st = p->stream;
   KCHK(p, launch_set_ctl(p->d_ctl, p->step, p->step, st));
-  if (want) KCHK(p, enqueue_record(p, buf));  // row 0 = U^n
+  if (want && !p->pml) KCHK(p, enqueue_record(p, buf));  // row 0 = U^n
   for (int k = 0; k < nsteps; ++k) {
     KCHK(p, enqueue_step(p, p->cur, buf));
     p->cur ^= 1;
   }
+  if (p->pml) p->cur = 0;
   p->step += nsteps;
   if (want)
     HIPCHK(p, hipMemcpyAsync(traces, buf, (size_t)nsteps * p->nshots * p->nrec * 8,
@@ -1467,6 +1506,7 @@ extern "C" int mjw_set_probes_grad(mjw_problem *p, int npts, const double *pts,
 // :64-66,108-109 "Glob_matMass_DhSigma", assembled by TimeAcoustic.cxx:970-1004)
 // sigma per element (nodal = false) or per quadrature point [E][nd] (nodal)
 static int set_damping(mjw_problem *p, const double *sigma, bool nodal) {
+  if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_damping: physical damping with PML layers is not supported");
   if (!p || !sigma) FAIL(p, MJW_ERR_ARG, "mjw_set_damping: NULL");
   if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_damping: acoustic problems only");
   HIPCHK(p, hipSetDevice(p->device));
@@ -1556,7 +1596,9 @@ extern "C" int mjw_probe(mjw_problem *p, double *values) {
 extern "C" int mjw_get_state(mjw_problem *p, double *Un, double *Un1) {
   if (!p) FAIL(p, MJW_ERR_ARG, "NULL problem");
   HIPCHK(p, hipSetDevice(p->device));
-  const int64_t ns = nstate(p);
+  if (Un1 && p->pml)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_get_state: the first-order system (PML layers) holds "
+                                 "u^n only (pass Un1 = NULL)");
   if (Un) KCHK(p, download_vec(p, p->d_U[p->cur], Un, p->nshots));
   if (Un1) KCHK(p, download_vec(p, p->d_U[p->cur ^ 1], Un1, p->nshots));
   return MJW_OK;
@@ -1583,6 +1625,7 @@ extern "C" int mjw_last_timing(const mjw_problem *p, double *ms2) {
 }
 
 extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
+  if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_profile: not available with PML layers");
   if (!p || !out || nsteps < 1) FAIL(p, MJW_ERR_ARG, "mjw_profile: bad args");
   if (!p->inited) FAIL(p, MJW_ERR_STATE, "not initialised");
   if (!p->damp_S.empty()) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_profile: undamped problems only");
@@ -1637,6 +1680,7 @@ extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
 }
 
 extern "C" int mjw_energy(mjw_problem *p, double *energy) {
+  if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_energy: not available for the first-order system (PML layers)");
   if (!p || !energy) FAIL(p, MJW_ERR_ARG, "mjw_energy: NULL");
   if (!p->inited) FAIL(p, MJW_ERR_STATE, "not initialised");
   HIPCHK(p, hipSetDevice(p->device));
@@ -1751,6 +1795,7 @@ extern "C" int mjw_lambda_max(mjw_problem *p, int iters, uint64_t seed, double *
 // re-anchors U on the checkpoint pairs stored by the forward pass.
 
 extern "C" int mjw_set_mu(mjw_problem *p, const double *mu) {
+  if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_mu: not available with PML layers");
   if (!p || !mu) FAIL(p, MJW_ERR_ARG, "mjw_set_mu: NULL");
   if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_mu: acoustic problems only");
   if (p->nodal_media)
@@ -1789,6 +1834,7 @@ extern "C" int mjw_set_mu(mjw_problem *p, const double *mu) {
 // lumped mass from the vertices; the per-element trilinear form no longer
 // applies, so the problem switches to stored geometry for good.
 extern "C" int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const double *mu_q) {
+  if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_media_nodal: not available with PML layers");
   if (!p || !rho_q || !mu_q) FAIL(p, MJW_ERR_ARG, "mjw_set_media_nodal: NULL");
   if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_media_nodal: acoustic problems only");
   HIPCHK(p, hipSetDevice(p->device));
@@ -1858,6 +1904,7 @@ extern "C" int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const do
 }
 
 extern "C" int mjw_adjoint_forward(mjw_problem *p, int nsteps, int nsnap, double *traces) {
+  if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: not available with PML layers");
   if (!p || nsteps < 2 || nsnap < 1 || !traces) FAIL(p, MJW_ERR_ARG, "mjw_adjoint_forward: bad args");
   if (p->ncomp != 1 || p->nshots != 1)
     FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: single-shot acoustic problems only");
@@ -2060,6 +2107,250 @@ extern "C" int mjw_adjoint_timing(const mjw_problem *p, double *ms2) {
 // items of every problem having that colour (k_batch3d dispatches on the
 // element order), then one tail block per problem; the step loop is captured
 // in a HIP graph.
+// ---------------------------------------------------------- PML layers
+// Montjoie's first-order split system (ModifiedEquationSystemIterator, order
+// 2, Instationary/ModifiedEquation.cxx:511-560,773-812) with the element
+// matrices of the first-order Helmholtz formulation (Elliptic/Helmholtz/
+// VarHelmholtz.cxx:133-230,296-378,620-660,3277-3720) and the block-diagonal
+// mass / damping operators of Hyperbolic/Acoustic/TimeAcoustic.cxx:1079-1235.
+
+// 2x2 / 3x3 inverse (adjugate over determinant)
+static void inv_small(int d, const double *P, double *Q) {
+  if (d == 1) { Q[0] = 1.0 / P[0]; return; }
+  if (d == 2) {
+    const double det = P[0] * P[3] - P[1] * P[2];
+    Q[0] = P[3] / det; Q[1] = -P[1] / det; Q[2] = -P[2] / det; Q[3] = P[0] / det;
+    return;
+  }
+  const double c00 = P[4] * P[8] - P[5] * P[7], c01 = P[5] * P[6] - P[3] * P[8],
+               c02 = P[3] * P[7] - P[4] * P[6];
+  const double det = P[0] * c00 + P[1] * c01 + P[2] * c02;
+  Q[0] = c00 / det; Q[1] = (P[2] * P[7] - P[1] * P[8]) / det; Q[2] = (P[1] * P[5] - P[2] * P[4]) / det;
+  Q[3] = c01 / det; Q[4] = (P[0] * P[8] - P[2] * P[6]) / det; Q[5] = (P[2] * P[3] - P[0] * P[5]) / det;
+  Q[6] = c02 / det; Q[7] = (P[1] * P[6] - P[0] * P[7]) / det; Q[8] = (P[0] * P[4] - P[1] * P[3]) / det;
+}
+
+extern "C" int mjw_set_pml(mjw_problem *p, const uint8_t *pml, const double *tau) {
+  if (!p || !pml || !tau) FAIL(p, MJW_ERR_ARG, "mjw_set_pml: NULL");
+  if (p->ncomp != 1 || p->nshots != 1)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_pml: single-shot acoustic problems only");
+  if (!p->damp_S.empty())
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_pml: physical damping with PML layers is not supported "
+                                 "(VarHelmholtz.cxx:411-422 refuses it inside the layers)");
+  if (p->nodal_media)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_pml: per-element media only");
+  if (p->pml) FAIL(p, MJW_ERR_STATE, "mjw_set_pml: the layers are already set");
+  HIPCHK(p, hipSetDevice(p->device));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  const int d = p->dim, n = p->n, nd = p->nd, P = p->P;
+  const int64_t E = p->E, N = p->N, EN = E * nd;
+  for (int64_t i = 0; i < EN * d; ++i)
+    if (!(tau[i] >= 0.0) || !std::isfinite(tau[i]))
+      FAIL(p, MJW_ERR_ARG, "mjw_set_pml: tau must be finite and >= 0");
+  double xg[kMaxN], wg[kMaxN], Dg[kMaxN * kMaxN];
+  gl_rule(p->order, xg, wg);
+  gl_deriv(p->order, xg, Dg);
+  std::vector<double> hG((size_t)d * d * EN), hB(EN), ht((size_t)d * EN, 0.0);
+  std::vector<uint8_t> hflag(E), blk(N, 0);
+  std::vector<double> Dc((size_t)d * N, 0.0);
+  std::vector<int32_t> bidx(N, -1);
+  std::vector<int32_t> pnode;
+  std::vector<double> Sb;
+  std::vector<double> rho_e(E), mu_e(E);
+  HIPCHK(p, hipMemcpy(rho_e.data(), p->d_rho, E * 8, hipMemcpyDeviceToHost));
+  HIPCHK(p, hipMemcpy(mu_e.data(), p->d_mu, E * 8, hipMemcpyDeviceToHost));
+  for (int64_t es = 0; es < E; ++es) {
+    const int64_t e = p->perm[es];
+    hflag[es] = pml[e] ? 1 : 0;
+    const int32_t *en = &p->en[e * nd];
+    for (int q = 0; q < nd; ++q) {
+      const int i = q % n, j = (q / n) % n, k = q / P;
+      const int ix[3] = {i, j, k};
+      // DF[a][b] = d x_a / d xi_b through the GL derivative matrix
+      double DF[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int b = 0; b < d; ++b)
+        for (int m = 0; m < n; ++m) {
+          int jx[3] = {i, j, k};
+          jx[b] = m;
+          const int qm = jx[0] + n * jx[1] + P * jx[2];
+          const double c = Dg[ix[b] * n + m];
+          for (int a = 0; a < d; ++a) DF[a * d + b] += c * p->xyz[(int64_t)en[qm] * d + a];
+        }
+      double adj[9], J;
+      if (d == 2) {
+        J = DF[0] * DF[3] - DF[1] * DF[2];
+        adj[0] = DF[3]; adj[1] = -DF[1]; adj[2] = -DF[2]; adj[3] = DF[0];
+      } else {
+        adj[0] = DF[4] * DF[8] - DF[5] * DF[7]; adj[1] = DF[2] * DF[7] - DF[1] * DF[8];
+        adj[2] = DF[1] * DF[5] - DF[2] * DF[4]; adj[3] = DF[5] * DF[6] - DF[3] * DF[8];
+        adj[4] = DF[0] * DF[8] - DF[2] * DF[6]; adj[5] = DF[2] * DF[3] - DF[0] * DF[5];
+        adj[6] = DF[3] * DF[7] - DF[4] * DF[6]; adj[7] = DF[1] * DF[6] - DF[0] * DF[7];
+        adj[8] = DF[0] * DF[4] - DF[1] * DF[3];
+        J = DF[0] * adj[0] + DF[1] * adj[3] + DF[2] * adj[6];
+      }
+      if (!(J > 0.0)) FAIL(p, MJW_ERR_GEOMETRY, "mjw_set_pml: degenerate element");
+      double w = 1.0;
+      for (int a = 0; a < d; ++a) w *= wg[ix[a]];
+      const int64_t pidx = ((int64_t)k * E + es) * P + (q % P);
+      for (int a = 0; a < d * d; ++a) hG[(size_t)a * EN + pidx] = w * adj[a];
+      const double Jw = J * w;
+      hB[pidx] = Jw / mu_e[es];
+      if (!pml[e]) continue;
+      const double *t = tau + (e * nd + q) * d;
+      for (int a = 0; a < d; ++a) ht[(size_t)a * EN + pidx] = t[a];
+      const int32_t g = p->nperm[en[q]];
+      if (bidx[g] < 0) {
+        bidx[g] = (int32_t)pnode.size();
+        pnode.push_back(g);
+        Sb.resize(Sb.size() + d * d, 0.0);
+      }
+      blk[g] = 1;
+      const double m = rho_e[es] * Jw;
+      for (int c = 1; c < d; ++c) Dc[(size_t)c * N + g] += m;
+      double *S = &Sb[(size_t)bidx[g] * d * d];
+      if (d == 2) {
+        const double sp = rho_e[es] * (0.5 * (t[0] + t[1])) * Jw;
+        const double sm = rho_e[es] * (0.5 * (t[0] - t[1])) * Jw;
+        S[0] += sp; S[3] += sp; S[1] += sm; S[2] += sm;
+      } else {
+        const double s0 = rho_e[es] * (t[0] + t[1] + t[2]) * Jw / 3.0;
+        const double d1 = rho_e[es] * (t[0] - t[1]) * Jw / 3.0;
+        const double d2 = rho_e[es] * (t[0] - t[2]) * Jw / 3.0;
+        const double d3 = rho_e[es] * (t[1] - t[2]) * Jw / 3.0;
+        S[0] += s0; S[1] += d1 - d3; S[2] += d2 + d3;
+        S[3] += d1; S[4] += s0 + d3; S[5] += d1;
+        S[6] += d2; S[7] += d2; S[8] += s0 - d3;
+      }
+    }
+  }
+  for (int64_t g = 0; g < N; ++g) Dc[p->nperm[g]] = p->mass[g];  // Dirichlet rows 1
+  for (size_t t = 0; t < pnode.size(); ++t) {
+    const int32_t g = pnode[t];
+    if (p->dir[p->ninv[g]])
+      for (int c = 0; c < d; ++c) { Sb[t * d * d + c] = 0.0; Sb[t * d * d + c * d] = 0.0; }
+  }
+  p->pml_D = Dc;
+  p->pml_S = Sb;
+  p->pml_node = pnode;
+  // device arrays
+  const size_t sz = ssize(p);
+  HIPCHK(p, dalloc(p, &p->d_pml_G, hG.size() * 8));
+  HIPCHK(p, dalloc(p, &p->d_pml_B, hB.size() * 8));
+  HIPCHK(p, dalloc(p, &p->d_pml_tau, ht.size() * 8));
+  HIPCHK(p, dalloc(p, &p->d_pml_Dm, (size_t)n * n * 8));
+  HIPCHK(p, dalloc(p, &p->d_pml_flag, (size_t)E));
+  HIPCHK(p, dalloc(p, &p->d_pml_blk, (size_t)N));
+  HIPCHK(p, dalloc(p, &p->d_pml_node, std::max<size_t>(pnode.size(), 1) * 4));
+  HIPCHK(p, dalloc(p, &p->d_pml_V, (size_t)d * EN * sz));
+  HIPCHK(p, dalloc(p, &p->d_pml_Pr, (size_t)d * N * sz));
+  HIPCHK(p, dalloc(p, &p->d_pml_sa, (size_t)N * 8));
+  HIPCHK(p, dalloc(p, &p->d_pml_sinv, (size_t)N * 8));
+  HIPCHK(p, dalloc(p, &p->d_pml_A, std::max<size_t>(Sb.size(), 1) * 8));
+  HIPCHK(p, dalloc(p, &p->d_pml_Pinv, std::max<size_t>(Sb.size(), 1) * 8));
+  HIPCHK(p, hipMemcpy(p->d_pml_G, hG.data(), hG.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(p, hipMemcpy(p->d_pml_B, hB.data(), hB.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(p, hipMemcpy(p->d_pml_tau, ht.data(), ht.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(p, hipMemcpy(p->d_pml_Dm, Dg, (size_t)n * n * 8, hipMemcpyHostToDevice));
+  HIPCHK(p, hipMemcpy(p->d_pml_flag, hflag.data(), (size_t)E, hipMemcpyHostToDevice));
+  HIPCHK(p, hipMemcpy(p->d_pml_blk, blk.data(), (size_t)N, hipMemcpyHostToDevice));
+  if (!pnode.empty())
+    HIPCHK(p, hipMemcpy(p->d_pml_node, pnode.data(), pnode.size() * 4, hipMemcpyHostToDevice));
+  // the state: u and its split copies in one buffer, u first
+  void *U = nullptr;
+  HIPCHK(p, dalloc(p, &U, (size_t)d * N * sz));
+  dfree(p->d_U[0]);
+  p->d_U[0] = U;
+  p->cur = 0;
+  p->pml = true;
+  p->inited = false;
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  return MJW_OK;
+}
+
+static int pml_init(mjw_problem *p, double t_begin, double dt, const double *u0,
+                    const double *du0) {
+  if (u0 || du0)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "PML layers: the first-order system starts from zero data "
+                                 "(u0 = du0 = NULL)");
+  HIPCHK(p, hipSetDevice(p->device));
+  hipStream_t st = p->stream;
+  const int d = p->dim;
+  const int64_t N = p->N, EN = p->E * p->nd;
+  p->dt = dt;
+  p->t_begin = t_begin;
+  p->zero_ic = true;
+  p->adj_ready = false;
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // dt is baked in
+  std::vector<double> sa(N), sinv(N);
+  for (int64_t i = 0; i < N; ++i) {
+    sa[i] = p->pml_D[i];
+    sinv[i] = p->dir[p->ninv[i]] ? 0.0 : 1.0 / p->pml_D[i];
+  }
+  const size_t nb = p->pml_node.size();
+  std::vector<double> A(nb * d * d), Pinv(nb * d * d);
+  for (size_t t = 0; t < nb; ++t) {
+    const int32_t g = p->pml_node[t];
+    double Pm[9];
+    for (int r = 0; r < d; ++r)
+      for (int c = 0; c < d; ++c) {
+        const double dg = r == c ? p->pml_D[(size_t)r * N + g] : 0.0;
+        const double sv = p->pml_S[(t * d + r) * d + c];
+        A[(t * d + r) * d + c] = dg - 0.5 * dt * sv;
+        Pm[r * d + c] = dg + 0.5 * dt * sv;
+      }
+    inv_small(d, Pm, &Pinv[t * d * d]);
+  }
+  HIPCHK(p, hipMemcpy(p->d_pml_sa, sa.data(), N * 8, hipMemcpyHostToDevice));
+  HIPCHK(p, hipMemcpy(p->d_pml_sinv, sinv.data(), N * 8, hipMemcpyHostToDevice));
+  if (nb) {
+    HIPCHK(p, hipMemcpy(p->d_pml_A, A.data(), A.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(p, hipMemcpy(p->d_pml_Pinv, Pinv.data(), Pinv.size() * 8, hipMemcpyHostToDevice));
+  }
+  const size_t sz = ssize(p);
+  // U^0 = 0, V^{1/2} = 0 (zero data), P = R^S V^{1/2} = 0
+  HIPCHK(p, hipMemsetAsync(p->d_U[0], 0, (size_t)d * N * sz, st));
+  HIPCHK(p, hipMemsetAsync(p->d_pml_V, 0, (size_t)d * EN * sz, st));
+  HIPCHK(p, hipMemsetAsync(p->d_pml_Pr, 0, (size_t)d * N * sz, st));
+  p->cur = 0;
+  p->step = 0;
+  int64_t ctl[2] = {0, 0};
+  HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, st));
+  HIPCHK(p, hipStreamSynchronize(st));
+  p->inited = true;
+  return MJW_OK;
+}
+
+// one step of the first-order system: traces of u^n and F_n b into P, the
+// node update to U^{n+1}, then per colour V^{n+3/2} and P = R^S V^{n+3/2}
+static int enqueue_step_pml(mjw_problem *p, double *traces) {
+  TailArgs t{};
+  t.state = p->state; t.ctl = p->d_ctl; t.pulse = p->d_pulse; t.npulse = p->npulse;
+  t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot; t.src_val = p->d_src_val;
+  t.sdiag = nullptr; t.nrec = traces ? p->nrec : 0; t.nd = p->nd; t.rec_gid = p->d_rec_gid;
+  t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp; t.U = p->d_U[0]; t.P = p->d_pml_Pr;
+  t.traces = (traces && p->nrec) ? traces : nullptr; t.N = p->N; t.nshots = 1; t.ilv = 0;
+  t.advance = 1;
+  t.row_offset = 0;
+  int rc = launch_tail_n(t, 1, p->stream);
+  if (rc) return rc;
+  PmlArgs a{};
+  a.dim = p->dim; a.order = p->order; a.state = p->state; a.E = p->E; a.N = p->N;
+  a.map = p->d_map; a.G = p->d_pml_G; a.B = p->d_pml_B; a.tau = p->d_pml_tau;
+  a.pml = p->d_pml_flag; a.Dm = p->d_pml_Dm; a.U = p->d_U[0]; a.V = p->d_pml_V;
+  a.Pr = p->d_pml_Pr; a.dt = p->dt; a.npn = (int64_t)p->pml_node.size(); a.pnode = p->d_pml_node;
+  a.A = p->d_pml_A; a.Pinv = p->d_pml_Pinv; a.dir = p->d_dir; a.blk = p->d_pml_blk;
+  a.sa = p->d_pml_sa; a.sinv = p->d_pml_sinv;
+  rc = launch_pml_nodes(a, p->stream);
+  if (rc) return rc;
+  for (int c = 0; c < p->ncolor; ++c) {
+    a.e0 = p->coff[c];
+    a.e1 = p->coff[c + 1];
+    rc = launch_pml_elem(a, p->stream);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 struct mjw_batch {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -2134,6 +2425,8 @@ extern "C" int mjw_batch_create(mjw_batch **out, int nprob, mjw_problem *const *
       BFAIL((mjw_batch *)nullptr, MJW_ERR_UNSUPPORTED, "batch: single-shot 3-D acoustic problems");
     if (!p->damp_S.empty())
       BFAIL((mjw_batch *)nullptr, MJW_ERR_UNSUPPORTED, "batch: undamped problems only");
+    if (p->pml)
+      BFAIL((mjw_batch *)nullptr, MJW_ERR_UNSUPPORTED, "batch: problems without PML layers only");
     if (p->device != probs[0]->device || p->state != probs[0]->state ||
         p->geom != probs[0]->geom)
       BFAIL((mjw_batch *)nullptr, MJW_ERR_ARG, "batch: problems must share device and precision");

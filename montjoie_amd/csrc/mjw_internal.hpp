@@ -201,6 +201,27 @@ int reduction_scratch_doubles(int S);
 int launch_scale_planes(int g32, void *G, int64_t nplanes, int64_t E, int P, const double *sc,
                         void *stream);
 
+// ------------------------------------------------- PML layers (LEAP_FROG_PML)
+struct PmlArgs {
+  int dim, order, state;
+  int64_t E, e0, e1, N;                // elements (plane stride), colour range, nodes
+  const int32_t *map;                  // element map (plane-major, bit 31 ignored)
+  const double *G;                     // [d*d][E*nd] w J DF^{-1}, row-major
+  const double *B;                     // [E*nd] J w / mu
+  const double *tau;                   // [d][E*nd] PML damping (PML elements)
+  const uint8_t *pml;                  // [E] element inside the layers (sorted order)
+  const double *Dm;                    // n*n GL derivative matrix
+  void *U, *V, *Pr;                    // [d][N], [d][E*nd], [d][N] (state precision)
+  double dt;
+  int64_t npn;                         // nodes of PML elements
+  const int32_t *pnode;                // [npn] their solver indices
+  const double *A, *Pinv;              // [npn][d][d] D - dt/2 S, (D + dt/2 S)^{-1}
+  const uint8_t *dir, *blk;            // [N] Dirichlet node, PML node
+  const double *sa, *sinv;             // [N] D - dt/2 S, 1/(D + dt/2 S) (0 on Dirichlet)
+};
+int launch_pml_elem(const PmlArgs &a, void *stream);
+int launch_pml_nodes(const PmlArgs &a, void *stream);
+
 // ---------------------------------------------------------------- batching
 struct BatchProb {          // device descriptor of one problem of a batch
   const int32_t *map;

@@ -280,7 +280,8 @@ __device__ __forceinline__ void tail_body(const TailArgs &a, int ncomp) {
       const int s = a.src_shot[idx];
       const int64_t g = dof % a.N;
       T *y = P + (a.ilv ? dof * a.nshots + s : (int64_t)s * NT + dof);
-      *y += sd[g] * (T)(pulse * a.src_val[idx]);
+      // sdiag = null: the source enters unscaled (first-order system)
+      *y += sd ? sd[g] * (T)(pulse * a.src_val[idx]) : (T)(pulse * a.src_val[idx]);
     }
   }
   if (a.traces) {

@@ -76,6 +76,18 @@ def ricker_pulse_table(f0: float, t_begin: float, dt: float, nsteps: int) -> np.
     return out
 
 
+def time_source_table(f0: float, t_begin: float, dt: float, nsteps: int, nb_deriv: int = 1,
+                      offset: float | None = None) -> np.ndarray:
+    """f^(nb_deriv)(t_begin + n dt + offset) of the Ricker source (offset
+    defaults to dt: the second-order leapfrog's table; the first-order PML
+    scheme uses nb_deriv = 0, offset = dt/2)."""
+    out = np.zeros(nsteps)
+    off = dt if offset is None else offset
+    check(_lib.lib().mjw_time_source_table(f0, t_begin, dt, nsteps, nb_deriv, off, dptr(out)),
+          None, "time_source_table")
+    return out
+
+
 class WaveProblem:
     """One acoustic (ncomp=1) or elastic (ncomp=3) problem on one device."""
 
@@ -276,6 +288,18 @@ class WaveProblem:
         sg = np.ascontiguousarray(np.broadcast_to(sg, (self.mesh.E,)))
         self._chk(_lib.lib().mjw_set_damping(self._h, dptr(sg)), "set_damping")
 
+    def set_pml(self, pml, tau):
+        """PML layers (mjw_set_pml): pml[e] marks the elements of the layers,
+        tau[e, q, a] the damping at their quadrature points; the problem then
+        runs Montjoie's first-order LEAP_FROG_PML scheme (pulse table:
+        time_source_table(..., nb_deriv=0, offset=dt/2))."""
+        fl = np.ascontiguousarray(np.asarray(pml, np.uint8).reshape(self.mesh.E))
+        tv = np.ascontiguousarray(np.asarray(tau, np.float64).reshape(-1))
+        if tv.size != self.mesh.E * self.mesh.nd * self.mesh.dim:
+            raise ValueError("tau must hold E * nd * dim values")
+        self._chk(_lib.lib().mjw_set_pml(self._h, u8ptr(fl), dptr(tv)), "set_pml")
+        self.pml = True
+
     def set_receivers(self, pts, comp=None, deriv=None):
         """Receivers of u (deriv None or -1) or of d u / d x_deriv at pts on
         component comp; returns the element holding each point."""
@@ -344,8 +368,12 @@ class WaveProblem:
         return np.concatenate([tr, last[None]], axis=0)
 
     def state(self):
+        """(U^n, U^{n+1}) [nshots, ndof]; with PML layers (u^n, None)."""
         n = self.nshots * self.ndof
         a = np.zeros(n)
+        if getattr(self, "pml", False):
+            self._chk(_lib.lib().mjw_get_state(self._h, dptr(a), None), "get_state")
+            return a.reshape(self.nshots, self.ndof), None
         b = np.zeros(n)
         self._chk(_lib.lib().mjw_get_state(self._h, dptr(a), dptr(b)), "get_state")
         return a.reshape(self.nshots, self.ndof), b.reshape(self.nshots, self.ndof)

@@ -203,8 +203,19 @@ def rel_l2(a, b) -> float:
 
 def gpu_problem(c: Case, precision: str = "fp64", nshots: int = 1, srcs=None):
     """The same case on the HIP path (montjoie_amd); sources default to the
-    case's single source replicated for every shot."""
+    case's single source replicated for every shot.  PML cases: the extended
+    mesh handed over as a caller's mesh, with its layers (mjw_set_pml)."""
     import montjoie_amd as M
+    if c.pml is not None:
+        pm, rho, mu, tau = pml_setup(c)
+        o = pm.mesh
+        m = M.Mesh(c.dim, c.r, np.ascontiguousarray(o.xyz), np.ascontiguousarray(o.elem_nodes),
+                   np.ascontiguousarray(o.dirichlet), o.nx)
+        P = M.WaveProblem(m, rho, mu, precision=precision)
+        P.set_pml(pm.pml, tau)
+        P.set_sources(c.src)
+        P.set_receivers(c.rec)
+        return P
     m = M.regular_mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     if c.elastic:
         P = M.WaveProblem(m, c.rho_e, c.mu_e, lam=c.lam_e, ncomp=3, nshots=nshots,
@@ -232,7 +243,11 @@ def gpu_run(c: Case, precision="fp64", nsteps=None, graph=True, nshots=1, srcs=N
     import montjoie_amd as M
     nsteps = c.nsteps if nsteps is None else nsteps
     P = gpu_problem(c, precision, nshots, srcs)
-    P.set_pulse(M.ricker_pulse_table(c.f0, 0.0, c.dt, nsteps))
+    if c.pml is not None:
+        # first-order system: the primitive at t_n + dt/2
+        P.set_pulse(M.time_source_table(c.f0, 0.0, c.dt, nsteps, 0, 0.5 * c.dt))
+    else:
+        P.set_pulse(M.ricker_pulse_table(c.f0, 0.0, c.dt, nsteps))
     P.leapfrog_init(c.dt)
     tr = P.run(nsteps, graph=graph)
     return P, tr
