@@ -284,6 +284,323 @@ Q1Mesh medit_mesh(int dim, const std::string &path) {
   return m;
 }
 
+// ------------------------------------------------------------ PML layers
+// AddPML = YES <axes> thickness layers|AUTO (PmlRegionParameter_Base::
+// SetParameters, Mesh/MeshBoundaries.cxx:32-102): X, Y, Z extrude both sides of
+// the box along that axis, X- / X+ (Y-, Z-, ...) one side.
+struct PmlSpec {
+  bool on = false;
+  int side[3] = {0, 0, 0};  // bit 0: negative side, bit 1: positive side
+  double thickness = 0.0;
+  int layers = 0;           // 0: AUTO
+};
+
+PmlSpec parse_pml(const Deck &d, int dim) {
+  PmlSpec s;
+  const auto lines = d.all("AddPML");
+  if (lines.empty()) return s;
+  if (lines.size() > 1) die("AddPML: one PML region per deck");
+  const auto &v = lines[0];
+  if (v.size() < 4) die("AddPML needs YES axes thickness layers");
+  if (v[0] != "YES") die("AddPML: only AddPML = YES <axes> thickness layers (no reference-based layers)");
+  const std::string &ax = v[1];
+  if (ax.find('R') != std::string::npos) die("AddPML: radial layers are not supported");
+  for (int a = 0; a < dim; ++a) {
+    const char c = "XYZ"[a];
+    if (ax.find(c) == std::string::npos) continue;
+    if (ax.find(std::string(1, c) + "-") != std::string::npos) s.side[a] = 1;
+    else if (ax.find(std::string(1, c) + "+") != std::string::npos) s.side[a] = 2;
+    else s.side[a] = 3;
+  }
+  s.thickness = num(v[2]);
+  if (!(s.thickness > 0)) die("AddPML: the thickness must be positive");
+  s.layers = v[3] == "AUTO" ? 0 : (int)num(v[3]);
+  if (v[3] != "AUTO" && s.layers < 1) die("AddPML: the number of layers must be positive");
+  s.on = s.side[0] || s.side[1] || s.side[2];
+  return s;
+}
+
+// vertex-set key of a face (sorted vertex ids)
+std::vector<int64_t> face_key(std::vector<int64_t> f) {
+  std::sort(f.begin(), f.end());
+  return f;
+}
+
+// faces of a Q1 element in Medit corner order (quadrilateral: 4 edges,
+// hexahedron: 6 faces)
+std::vector<std::vector<int64_t>> elem_faces(const Q1Mesh &m, int64_t e) {
+  const int nc = 1 << m.dim;
+  const int64_t *c = &m.elem[(size_t)e * nc];
+  if (m.dim == 2) return {{c[0], c[1]}, {c[1], c[2]}, {c[2], c[3]}, {c[3], c[0]}};
+  return {{c[0], c[1], c[2], c[3]}, {c[4], c[5], c[6], c[7]}, {c[0], c[1], c[5], c[4]},
+          {c[1], c[2], c[6], c[5]}, {c[2], c[3], c[7], c[6]}, {c[3], c[0], c[4], c[7]}};
+}
+
+// Layers along axis a beyond the plane x_a = pos (delta < 0: below it), the
+// way Mesh<>::ExtrudeCoordinate does (Mesh/Mesh2D.cxx:4901-5073,
+// Mesh/Mesh3D.cxx:9414-9660): the vertices within 1e-10 of the plane (R_N::
+// threshold, Mesh/NumberMesh.cxx:1621,2727) are copied nl times with x_a
+// moved to the layer planes; every referenced boundary face lying on the plane
+// grows one element per layer, of the medium (reference) of the element it
+// bounds, and is itself moved to the outer plane with its reference; the
+// rim of those faces gets lateral faces, referenced like the non-extruded
+// boundary face they continue (the extruded face's reference otherwise).
+void extrude(Q1Mesh &m, int a, int nl, double pos, double delta) {
+  const int dim = m.dim, nc = 1 << dim;
+  const int64_t nv0 = (int64_t)(m.vert.size() / dim);
+  std::vector<int64_t> to(nv0, -1);
+  int64_t nn = 0;
+  for (int64_t i = 0; i < nv0; ++i)
+    if (std::fabs(pos - m.vert[(size_t)i * dim + a]) <= 1e-10) to[i] = nv0 + nn++;
+  if (!nn) return;
+  m.vert.resize((size_t)(nv0 + nn * nl) * dim);
+  for (int k = 0; k < nl; ++k)
+    for (int64_t i = 0; i < nv0; ++i)
+      if (to[i] >= 0) {
+        double *x = &m.vert[(size_t)(to[i] + k * nn) * dim];
+        const double *x0 = &m.vert[(size_t)i * dim];
+        for (int b = 0; b < dim; ++b) x[b] = x0[b];
+        // 2-D: pos + (k+1) delta / nl; 3-D: x_a + delta (k+1) / nl
+        x[a] = dim == 2 ? pos + (double)(k + 1) * delta / nl : x0[a] + delta * (double)(k + 1) / nl;
+      }
+  auto layer = [&](int64_t v, int k) { return k == 0 ? v : to[v] + (int64_t)(k - 1) * nn; };
+  const size_t nb0 = m.bnd.size();
+  std::vector<char> top(nb0, 0);
+  std::map<std::vector<int64_t>, int64_t> owner;  // top face -> element it bounds
+  for (size_t f = 0; f < nb0; ++f) {
+    bool on = true;
+    for (int64_t v : m.bnd[f]) on = on && to[v] >= 0;
+    top[f] = on;
+    if (on) owner[face_key(m.bnd[f])] = -1;
+  }
+  const int64_t E0 = (int64_t)m.elem_ref.size();
+  for (int64_t e = 0; e < E0; ++e)
+    for (const auto &fc : elem_faces(m, e)) {
+      auto it = owner.find(face_key(fc));
+      if (it != owner.end()) it->second = e;
+    }
+  // rim of the extruded patch: vertices (2-D) / edges (3-D) on one top face only
+  std::map<std::vector<int64_t>, std::pair<int, size_t>> rim;  // key -> (count, face)
+  for (size_t f = 0; f < nb0; ++f) {
+    if (!top[f]) continue;
+    const auto &fv = m.bnd[f];
+    const int n = (int)fv.size();
+    if (dim == 2) {
+      for (int64_t v : fv) {
+        auto &r = rim[{v}];
+        r.first++;
+        r.second = f;
+      }
+    } else {
+      for (int j = 0; j < n; ++j) {
+        auto &r = rim[face_key({fv[j], fv[(j + 1) % n]})];
+        r.first++;
+        r.second = f;
+      }
+    }
+  }
+  std::vector<std::vector<int64_t>> new_bnd;
+  std::vector<int> new_ref;
+  for (const auto &kv : rim) {
+    if (kv.second.first != 1) continue;
+    const size_t f = kv.second.second;
+    int ref = m.bnd_ref[f];
+    if (dim == 2) {
+      // another boundary edge at this vertex running along axis a
+      // (GetPointWithOtherReference, Mesh2D.cxx:5043-5058)
+      const int64_t v = kv.first[0];
+      for (size_t g = 0; g < nb0; ++g) {
+        if (top[g] || m.bnd_ref[g] == ref) continue;
+        const auto &ev = m.bnd[g];
+        if (ev[0] != v && ev[1] != v) continue;
+        const int64_t w = ev[0] == v ? ev[1] : ev[0];
+        if (std::fabs(m.vert[(size_t)v * 2 + (1 - a)] - m.vert[(size_t)w * 2 + (1 - a)]) <= 1e-9 &&
+            m.bnd_ref[g] != 0)
+          ref = m.bnd_ref[g];
+      }
+      for (int k = 0; k < nl; ++k) {
+        new_bnd.push_back({layer(v, k), layer(v, k + 1)});
+        new_ref.push_back(ref);
+      }
+    } else {
+      // a non-extruded boundary face sharing the edge (Mesh3D.cxx:9606-9619)
+      const int64_t v0 = kv.first[0], v1 = kv.first[1];
+      for (size_t g = 0; g < nb0; ++g) {
+        if (top[g]) continue;
+        const auto &fv = m.bnd[g];
+        const int n = (int)fv.size();
+        bool has = false;
+        for (int j = 0; j < n && !has; ++j)
+          has = face_key({fv[j], fv[(j + 1) % n]}) == std::vector<int64_t>{v0, v1};
+        if (has && m.bnd_ref[g] > 0) ref = m.bnd_ref[g];
+      }
+      // the edge in the order the top face runs it
+      const auto &fv = m.bnd[f];
+      const int n = (int)fv.size();
+      int64_t p0 = v0, p1 = v1;
+      for (int j = 0; j < n; ++j)
+        if (face_key({fv[j], fv[(j + 1) % n]}) == std::vector<int64_t>{v0, v1}) {
+          p0 = fv[j];
+          p1 = fv[(j + 1) % n];
+        }
+      for (int k = 0; k < nl; ++k) {
+        new_bnd.push_back({layer(p0, k), layer(p1, k), layer(p1, k + 1), layer(p0, k + 1)});
+        new_ref.push_back(ref);
+      }
+    }
+  }
+  // the new elements, positively oriented
+  const double *X = nullptr;
+  auto sub = [&](int64_t i, int64_t j, double *o) {
+    for (int b = 0; b < dim; ++b) o[b] = X[(size_t)j * dim + b] - X[(size_t)i * dim + b];
+  };
+  for (size_t f = 0; f < nb0; ++f) {
+    if (!top[f]) continue;
+    const int64_t owner_e = owner[face_key(m.bnd[f])];
+    if (owner_e < 0) die("AddPML: a boundary face on the extruded plane bounds no element");
+    std::vector<int64_t> fv = m.bnd[f];
+    for (int k = 0; k < nl; ++k) {
+      std::vector<int64_t> el(nc);
+      for (int j = 0; j < nc / 2; ++j) {
+        el[j] = layer(fv[j], k);
+        el[dim == 2 ? 3 - j : j + 4] = layer(fv[j], k + 1);
+      }
+      X = m.vert.data();
+      double u[3], w[3], z[3] = {0, 0, 0};
+      sub(el[0], el[1], u);
+      sub(el[0], el[3], w);
+      double vol;
+      if (dim == 2) {
+        vol = u[0] * w[1] - u[1] * w[0];
+      } else {
+        sub(el[0], el[4], z);
+        vol = (u[1] * w[2] - u[2] * w[1]) * z[0] + (u[2] * w[0] - u[0] * w[2]) * z[1] +
+              (u[0] * w[1] - u[1] * w[0]) * z[2];
+      }
+      if (vol < 0) {
+        if (dim == 2) {
+          std::swap(el[0], el[1]);
+          std::swap(el[2], el[3]);
+        } else {
+          std::swap(el[1], el[3]);
+          std::swap(el[5], el[7]);
+        }
+      }
+      m.elem.insert(m.elem.end(), el.begin(), el.end());
+      m.elem_ref.push_back(m.elem_ref[owner_e]);
+    }
+    for (auto &v : m.bnd[f]) v = layer(v, nl);
+  }
+  m.bnd.insert(m.bnd.end(), new_bnd.begin(), new_bnd.end());
+  m.bnd_ref.insert(m.bnd_ref.end(), new_ref.begin(), new_ref.end());
+}
+
+// PML region of a Q1 mesh: the layers extruded axis by axis, x- then x+,
+// y-, y+ (z-, z+) (PmlRegionParameter<>::AddPMLElements, Mesh/
+// Mesh2DBoundaries.cxx:104-192, Mesh3DBoundaries.cxx:59-140); returns the
+// kind of every element: bit 2a set below the box along axis a, bit 2a+1
+// above it, from its centre and the box of the mesh before extrusion (0 for
+// the physical elements).  layers = AUTO: ceil(thickness / mean length of the
+// boundary edges on the box), at least 1 (AddPML, MeshBoundaries.cxx:108-118,
+// GetAverageLengthOnBoundary, Mesh2DBoundaries.cxx:16-95,
+// Mesh3DBoundaries.cxx:14-50).
+std::vector<int> add_pml(Q1Mesh &m, PmlSpec &s) {
+  const int dim = m.dim, nc = 1 << dim;
+  const int64_t nv = (int64_t)(m.vert.size() / dim);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t i = 0; i < nv; ++i)
+    for (int a = 0; a < dim; ++a) {
+      lo[a] = std::min(lo[a], m.vert[(size_t)i * dim + a]);
+      hi[a] = std::max(hi[a], m.vert[(size_t)i * dim + a]);
+    }
+  if (s.layers == 0) {
+    std::vector<char> onb(nv, 0);
+    for (int64_t i = 0; i < nv; ++i)
+      for (int a = 0; a < dim; ++a) {
+        const double x = m.vert[(size_t)i * dim + a];
+        if (std::fabs(x - lo[a]) <= 1e-10 || std::fabs(x - hi[a]) <= 1e-10) onb[i] = 1;
+      }
+    double sum = 0.0;
+    int64_t cnt = 0;
+    for (const auto &f : m.bnd) {
+      const int n = (int)f.size();
+      for (int j = 0; j < (dim == 2 ? 1 : n); ++j) {
+        const int64_t v0 = f[j], v1 = f[(j + 1) % n];
+        if (!onb[v0] || !onb[v1]) continue;
+        double d2 = 0.0;
+        for (int a = 0; a < dim; ++a) {
+          const double dx = m.vert[(size_t)v0 * dim + a] - m.vert[(size_t)v1 * dim + a];
+          d2 += dx * dx;
+        }
+        sum += std::sqrt(d2);
+        ++cnt;
+      }
+    }
+    const double h = cnt ? sum / cnt : 0.0;
+    s.layers = h > 0 ? std::max(1, (int)std::ceil(s.thickness / h)) : 1;
+  }
+  const int64_t E0 = (int64_t)m.elem_ref.size();
+  for (int a = 0; a < dim; ++a) {
+    // the current extent along a (the earlier axes' layers do not move it)
+    double mn = 1e300, mx = -1e300;
+    for (size_t i = 0; i < m.vert.size() / dim; ++i) {
+      mn = std::min(mn, m.vert[i * dim + a]);
+      mx = std::max(mx, m.vert[i * dim + a]);
+    }
+    if (s.side[a] & 1) extrude(m, a, s.layers, mn, -s.thickness);
+    if (s.side[a] & 2) extrude(m, a, s.layers, mx, s.thickness);
+  }
+  const int64_t E = (int64_t)m.elem_ref.size();
+  std::vector<int> kind(E, 0);
+  for (int64_t e = E0; e < E; ++e) {
+    double c[3] = {0, 0, 0};
+    for (int j = 0; j < nc; ++j)
+      for (int a = 0; a < dim; ++a) c[a] += m.vert[(size_t)m.elem[(size_t)e * nc + j] * dim + a];
+    for (int a = 0; a < dim; ++a) {
+      c[a] *= 1.0 / nc;
+      if (c[a] < lo[a]) kind[e] |= 1 << (2 * a);
+      if (c[a] > hi[a]) kind[e] |= 2 << (2 * a);
+    }
+  }
+  return kind;
+}
+
+// DampingPML = [PARABOLE|LINEAR|CONSTANT|SHIFTED_PARABOLE] vsigma [offset]
+// (VarBoundaryCondition_Base::SetInputData, Harmonic/
+// BoundaryConditionHarmonic.cxx:130-199) and TransverseDampingPML = coef
+// (:117-129)
+struct PmlDamping {
+  int profile = 0;  // 0 parabola, 1 constant, 2 linear, 3 shifted parabola
+  double vsigma = 0.0, offset = 0.0, transverse = 0.0;
+};
+
+PmlDamping parse_pml_damping(const Deck &d) {
+  PmlDamping p;
+  if (const auto *v = d.get("DampingPML")) {
+    if (v->empty()) die("DampingPML needs vsigma");
+    const std::string &w = (*v)[0];
+    auto need = [&](size_t n) {
+      if (v->size() < n) die("DampingPML = " + w + " needs more parameters");
+    };
+    if (w == "PARABOLE") { need(2); p.vsigma = num((*v)[1]); }
+    else if (w == "CONSTANT") { need(2); p.profile = 1; p.vsigma = num((*v)[1]); }
+    else if (w == "LINEAR") { need(2); p.profile = 2; p.vsigma = num((*v)[1]); }
+    else if (w == "SHIFTED_PARABOLE") {
+      need(3);
+      p.profile = 3;
+      p.vsigma = num((*v)[1]);
+      p.offset = num((*v)[2]);
+    } else {
+      p.vsigma = num(w);
+    }
+  }
+  if (const auto *v = d.get("TransverseDampingPML")) {
+    if (v->empty()) die("TransverseDampingPML needs a coefficient");
+    p.transverse = num((*v)[0]);
+  }
+  return p;
+}
+
 // ------------------------------------------------------- Q_r numbering
 // Global GL nodes shared by neighbouring elements are identified
 // topologically: a node on an edge by the edge's vertex ids and its index
@@ -609,6 +926,70 @@ void write_vector(const std::string &path, const std::vector<T> &v) {
 }
 
 // ------------------------------------------------------------------ media
+// sigma_a at every quadrature point of the layers (GetDampingTauPML and
+// EvaluateFunctionTauPML, Harmonic/BoundaryConditionHarmonic.cxx:1548-1570,
+// 1639-1760; the first-order formulation keeps sigma itself, :1588-1636):
+// dx = x_a minus the box face the element lies beyond, dsig = log(1000) 3
+// vsigma vmax / (2 a^3), a the thickness, vmax = max sqrt(|mu / rho|) over the
+// media of the layers (FindElementsInsidePML, :1424-1535); the box is the
+// bounding box of the vertices of the physical elements.
+std::vector<double> pml_tau(const Q1Mesh &m, const std::vector<int> &kind, const QrMesh &q,
+                            const std::vector<double> &rho, const std::vector<double> &mu,
+                            const PmlSpec &s, const PmlDamping &pd) {
+  const int dim = m.dim, nc = 1 << dim, nd = (int)(q.en.size() / q.E);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  double vmax = 0.0;
+  for (int64_t e = 0; e < q.E; ++e) {
+    if (kind[e]) {
+      vmax = std::max(vmax, std::sqrt(std::fabs(mu[e] / rho[e])));
+      continue;
+    }
+    for (int j = 0; j < nc; ++j)
+      for (int a = 0; a < dim; ++a) {
+        const double x = m.vert[(size_t)m.elem[(size_t)e * nc + j] * dim + a];
+        lo[a] = std::min(lo[a], x);
+        hi[a] = std::max(hi[a], x);
+      }
+  }
+  const double A = s.thickness;
+  const double dsig = std::log(1000e0) * 3e0 * pd.vsigma * vmax / (2 * std::pow(A, 3));
+  auto f = [&](double dx) {
+    switch (pd.profile) {
+      case 1: return dsig * A * A;
+      case 2: return dsig * std::fabs(dx) * A;
+      case 3: return dsig * (dx * dx + pd.offset * A * A);
+      default: return dsig * dx * dx;
+    }
+  };
+  std::vector<double> tau((size_t)q.E * nd * dim, 0.0);
+  for (int64_t e = 0; e < q.E; ++e) {
+    if (!kind[e]) continue;
+    for (int i = 0; i < nd; ++i) {
+      const double *x = &q.xyz[(size_t)q.en[(size_t)e * nd + i] * dim];
+      double z[3] = {0, 0, 0};
+      for (int a = 0; a < dim; ++a) {
+        if (kind[e] & (1 << (2 * a))) z[a] = f(x[a] - lo[a]);
+        else if (kind[e] & (2 << (2 * a))) z[a] = f(x[a] - hi[a]);
+      }
+      if (pd.transverse != 0) {
+        const double c = pd.transverse;
+        double o[3];
+        if (dim == 2) {
+          o[0] = std::max(z[0], c * z[1]);
+          o[1] = std::max(c * z[0], z[1]);
+        } else {
+          o[0] = std::max(z[0], c * std::max(z[1], z[2]));
+          o[1] = std::max(c * std::max(z[0], z[2]), z[1]);
+          o[2] = std::max(c * std::max(z[0], z[1]), z[2]);
+        }
+        for (int a = 0; a < dim; ++a) z[a] = o[a];
+      }
+      for (int a = 0; a < dim; ++a) tau[((size_t)e * nd + i) * dim + a] = z[a];
+    }
+  }
+  return tau;
+}
+
 struct Media {
   std::vector<double> rho, mu, lam, sigma;  // sigma: acoustic damping (empty if none)
 };
@@ -1138,7 +1519,7 @@ int main(int argc, char **argv) {
       "ParametersOutputPoint", "SismoPlane",
       "FileOutputPlane", "ParametersOutputPlane", "SismoLine", "FileOutputLine",
       "ParametersOutputLine", "OutputFormat", "SaveReprise", "PathReprise", "LoadReprise", "DisplayStress",
-      "ElectricOrMagnetic", "PrintLevel"};
+      "ElectricOrMagnetic", "PrintLevel", "AddPML", "DampingPML", "TransverseDampingPML"};
   for (const auto &kv : d.kv)
     if (!known.count(kv.first))
       std::fprintf(stderr, "mjwave_run: keyword %s ignored\n", kv.first.c_str());
@@ -1154,9 +1535,25 @@ int main(int argc, char **argv) {
   if (elastic && dim != 3) die("TIME_ELASTIC needs hexahedra");
   const int ncomp = elastic ? 3 : 1;
   const int order = (int)num(d.need("OrderDiscretization")[0]);
-  if (const auto *o = d.get("OrderTimeScheme"))
-    if (o->size() < 2 || num((*o)[0]) != 2 || (*o)[1] != "LEAP_FROG")
-      die("OrderTimeScheme: only 2 LEAP_FROG");
+  // OrderTimeScheme = 2 LEAP_FROG (second-order scheme, ModifiedEquationIterator)
+  // or 2 LEAP_FROG_PML (the first-order split system, ModifiedEquationSystemIterator)
+  bool first_order = false;
+  if (const auto *o = d.get("OrderTimeScheme")) {
+    if (o->size() < 2 || num((*o)[0]) != 2 || ((*o)[1] != "LEAP_FROG" && (*o)[1] != "LEAP_FROG_PML"))
+      die("OrderTimeScheme: only 2 LEAP_FROG or 2 LEAP_FROG_PML");
+    first_order = (*o)[1] == "LEAP_FROG_PML";
+  }
+  PmlSpec pml_spec = parse_pml(d, dim);
+  const PmlDamping pml_damp = parse_pml_damping(d);
+  if (pml_spec.on && !first_order)
+    die("AddPML: only with OrderTimeScheme = 2 LEAP_FROG_PML (the second-order PML formulation "
+        "is not implemented)");
+  if (first_order) {
+    if (elastic) die("LEAP_FROG_PML: acoustic decks only");
+    if (d.get("SaveReprise")) die("SaveReprise: not available with LEAP_FROG_PML");
+    if (d.get("PhysicalMedia")) die("PhysicalMedia: not available with LEAP_FROG_PML (per-reference media only)");
+    if (d.need("TimeStep")[0] == "AUTO") die("TimeStep = AUTO: not available with LEAP_FROG_PML");
+  }
   if (const auto *s = d.get("TypeSource"))
     if ((*s)[0] != "SRC_DIRAC") die("TypeSource " + (*s)[0] + ": only SRC_DIRAC");
   if (const auto *s = d.get("TemporalSource"))
@@ -1201,8 +1598,18 @@ int main(int argc, char **argv) {
     if (v.back() != "DIRICHLET") die("ConditionReference " + v.back() + ": only DIRICHLET");
     for (size_t k = 0; k + 1 < v.size(); ++k) drefs.insert((int)num(v[k]));
   }
+  std::vector<int> pml_kind;
+  if (pml_spec.on) pml_kind = add_pml(q1, pml_spec);
+  pml_kind.resize(q1.elem_ref.size(), 0);
   const QrMesh qm = number_nodes(q1, order, drefs);
   const Media md = media(d, q1, elastic);
+  if (first_order && !md.sigma.empty())
+    for (double sg : md.sigma)
+      if (sg != 0) die("MateriauDielec damping: not available with LEAP_FROG_PML");
+  std::vector<uint8_t> pml_flag(pml_kind.size());
+  for (size_t e = 0; e < pml_kind.size(); ++e) pml_flag[e] = pml_kind[e] != 0;
+  std::vector<double> pml_tau_q;
+  if (first_order) pml_tau_q = pml_tau(q1, pml_kind, qm, md.rho, md.mu, pml_spec, pml_damp);
   const NodalMedia nm = nodal_media(d, q1, qm, md, dim, elastic);
   if (!dump.empty()) {
     std::ofstream o(dump, std::ios::binary);
@@ -1217,6 +1624,13 @@ int main(int argc, char **argv) {
     wr(o, nm.rho);  // empty unless PhysicalMedia: E * nd each
     wr(o, nm.mu);
     wr(o, montjoie_dofs(q1, qm));
+    if (first_order) {
+      // the layers: element flags, then sigma at every quadrature point
+      std::ofstream op(dump + ".pml", std::ios::binary);
+      wr(op, pml_flag);
+      wr(op, pml_tau_q);
+      if (!op) return 1;
+    }
     return o ? 0 : 1;
   }
 
@@ -1232,6 +1646,8 @@ int main(int argc, char **argv) {
                    md.rho.data(), md.mu.data(), elastic ? md.lam.data() : nullptr, qm.dir.data(),
                    1, precision),
         nullptr, "mjw_create");
+  if (first_order)
+    check(mjw_set_pml(p, pml_flag.data(), pml_tau_q.data()), p, "AddPML");
   if (!nm.rho.empty())
     check(mjw_set_media_nodal(p, nm.rho.data(), nm.mu.data()), p, "PhysicalMedia");
   if (!nm.sigma.empty())
@@ -1273,7 +1689,11 @@ int main(int argc, char **argv) {
     check(mjw_set_sources(p, 1, x, pol.data(), nullptr), p, "source");
   }
   std::vector<double> tab(nsteps);
-  check(mjw_ricker_pulse_table(f0, t0, dt, (int)nsteps, tab.data()), nullptr, "Ricker");
+  if (first_order)  // the primitive at t_n + dt/2 (ModifiedEquation.cxx:784-785)
+    check(mjw_time_source_table(f0, t0, dt, (int)nsteps, 0, 0.5 * dt, tab.data()), nullptr,
+          "Ricker");
+  else
+    check(mjw_ricker_pulse_table(f0, t0, dt, (int)nsteps, tab.data()), nullptr, "Ricker");
   check(mjw_set_pulse(p, (int)nsteps, tab.data()), p, "pulse");
   // receivers: SismoPoint (SismoCircle is a snapshot grid, below)
   std::vector<double> pts;

@@ -41,7 +41,8 @@ def _write(c: CS.Case, d):
                      sinus_sigma=z["sinus_sigma"] if "sinus_sigma" in z.files else None,
                      output_format=str(z["output_format"]) if "output_format" in z.files
                      else "BINARY DOUBLE",
-                     circle=z["circle"] if "circle" in z.files and len(z["circle"]) else None)
+                     circle=z["circle"] if "circle" in z.files and len(z["circle"]) else None,
+                     pml=(c.dim,) + tuple(c.pml) if c.pml is not None else None)
 
 
 def _dump(path):
@@ -69,6 +70,41 @@ def _dump(path):
     return out
 
 
+def _check_pml_mesh(c: CS.Case, g, pml_path):
+    """AddPML deck: the driver's extruded mesh (its elements in its own order)
+    against the oracle's extended structured mesh: the same elements (matched
+    by centre), each holding the same GL nodes (matched by position), the same
+    Dirichlet nodes, media of the element each layer was extruded from, the
+    same PML elements and sigma at every quadrature point."""
+    pm, rho, mu, tau = CS.pml_setup(c)
+    o = pm.mesh
+    nd = (c.r + 1) ** c.dim
+    assert (g["N"], g["E"]) == (o.N, o.E)
+    raw = open(pml_path, "rb").read()
+    flag = np.frombuffer(raw[:o.E], np.uint8)
+    gtau = np.frombuffer(raw[o.E:], np.float64).reshape(o.E, nd, c.dim)
+    oen = o.elem_nodes.reshape(o.E, nd)
+    oxyz = o.xyz.reshape(o.N, c.dim)
+    key = lambda X: tuple(np.round(X.mean(axis=0), 9))  # noqa: E731
+    oi = {key(oxyz[oen[e]]): e for e in range(o.E)}
+    assert len(oi) == o.E
+    for e in range(g["E"]):
+        Xg = g["xyz"][g["en"][e]]
+        eo = oi[key(Xg)]
+        Xo = oxyz[oen[eo]]
+        # local node q of the driver's element -> local node of the oracle's
+        d2 = ((Xg[:, None, :] - Xo[None, :, :]) ** 2).sum(-1)
+        lq = d2.argmin(axis=1)
+        assert np.unique(lq).size == nd and d2[np.arange(nd), lq].max() < 1e-26, e
+        assert g["rho"][e] == rho[eo] and g["mu"][e] == mu[eo], e
+        assert flag[e] == pm.pml[eo], e
+        np.testing.assert_allclose(gtau[e], tau[eo][lq], rtol=1e-13, atol=1e-13)
+    # Dirichlet nodes: the outer boundary of the extended box
+    gd = {tuple(np.round(x, 9)) for x in g["xyz"][g["dir"] == 1]}
+    od = {tuple(np.round(x, 9)) for x in oxyz[o.dirichlet.astype(bool)]}
+    assert gd == od
+
+
 @pytest.mark.parametrize("name", CS.golden_names())
 def test_driver_mesh_is_the_oracle_mesh(name, tmp_path):
     if not os.path.exists(RUNNER):
@@ -79,6 +115,9 @@ def test_driver_mesh_is_the_oracle_mesh(name, tmp_path):
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     g = _dump(tmp_path / "mesh.bin")
+    if c.pml is not None:
+        _check_pml_mesh(c, g, tmp_path / "mesh.bin.pml")
+        return
     o = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     assert (g["N"], g["E"]) == (o.N, o.E)
     oen = o.elem_nodes.reshape(o.E, -1)
@@ -396,3 +435,69 @@ def test_driver_circle_matches_montjoie(name, tmp_path):
     err = CS.rel_l2(got, want)
     print(f"{name}: {len(files)} circle files, rel-L2 {err:.2e}")
     assert err < 1e-10
+
+
+def _pml_dump(tmp_path, ini_text, name="p.ini"):
+    (tmp_path / name).write_text(ini_text)
+    r = subprocess.run([RUNNER, name, "--dump-mesh", name + ".bin"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    g = _dump(tmp_path / (name + ".bin"))
+    raw = open(tmp_path / (name + ".bin.pml"), "rb").read()
+    flag = np.frombuffer(raw[:g["E"]], np.uint8)
+    return g, flag, np.frombuffer(raw[g["E"]:], np.float64)
+
+
+def test_driver_pml_auto_layers_and_one_sided_axes(tmp_path):
+    """AddPML = ... AUTO: ceil(thickness / mean boundary edge length) layers
+    (0.25 over edges of 1/8: 2, the fixture's count, so the same mesh and
+    sigma); X+ Y-: layers on those two sides only, the corner element once,
+    sigma zero along the axes an element does not extend beyond."""
+    c = CS.load("ref2d_q4_pml")
+    _write(c, tmp_path)
+    ini = (tmp_path / "case.ini").read_text()
+    assert "AddPML = YES XY 0.25 2" in ini
+    g2, f2, t2 = _pml_dump(tmp_path, ini, "two.ini")
+    ga, fa, ta = _pml_dump(tmp_path, ini.replace("AddPML = YES XY 0.25 2",
+                                                 "AddPML = YES XY 0.25 AUTO"), "auto.ini")
+    assert (ga["N"], ga["E"]) == (g2["N"], g2["E"])
+    assert np.array_equal(ga["xyz"], g2["xyz"]) and np.array_equal(ga["en"], g2["en"])
+    assert np.array_equal(fa, f2) and np.array_equal(ta, t2)
+    g1, f1, t1 = _pml_dump(tmp_path, ini.replace("AddPML = YES XY 0.25 2",
+                                                 "AddPML = YES X+Y- 0.25 2"), "one.ini")
+    nx, L, nd = c.nx, 2, (c.r + 1) ** 2
+    assert g1["E"] == (nx + L) ** 2 and int(f1.sum()) == (nx + L) ** 2 - nx ** 2
+    X = g1["xyz"][g1["en"]]                      # (E, nd, 2)
+    tau = t1.reshape(g1["E"], nd, 2)
+    cen = X.mean(axis=1)
+    assert cen[:, 0].min() > 0 and cen[:, 1].max() < 1      # no layer at x < 0, y > 1
+    in_x = cen[:, 0] > 1
+    in_y = cen[:, 1] < 0
+    assert np.array_equal(f1.astype(bool), in_x | in_y)
+    assert np.all(tau[~in_x, :, 0] == 0) and np.all(tau[~in_y, :, 1] == 0)
+    assert np.all(tau[in_x, :, 0] >= 0) and tau[in_x, :, 0].max() > 0
+    # sigma grows away from the box: parabola dsig (x - 1)^2 in the x+ layers
+    x = X[in_x, :, 0].ravel()
+    s = tau[in_x, :, 0].ravel()
+    k = np.argmax(x)
+    np.testing.assert_allclose(s / s[k], (x - 1) ** 2 / (x[k] - 1) ** 2, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("edit,word", [
+    (lambda s: s.replace("2 LEAP_FROG_PML", "2 LEAP_FROG"), "LEAP_FROG_PML"),
+    (lambda s: s.replace("AddPML = YES XY 0.25 2", "AddPML = YES R 0.25 2"), "radial"),
+    (lambda s: s.replace("AddPML = YES XY 0.25 2", "AddPML = 1 2 0.25 2"), "AddPML"),
+    (lambda s: s + "SaveReprise = 10\n", "SaveReprise"),
+    (lambda s: s.replace("TimeStep = ", "TimeStep = AUTO #"), "AUTO"),
+])
+def test_driver_rejects_unsupported_pml_decks(tmp_path, edit, word):
+    """The second-order PML formulation, radial and reference-based layers,
+    checkpoints and TimeStep = AUTO with the first-order scheme stop the run
+    with a message naming what is missing (before any GPU call)."""
+    c = CS.load("ref2d_q4_pml")
+    _write(c, tmp_path)
+    ini = (tmp_path / "case.ini").read_text()
+    (tmp_path / "bad.ini").write_text(edit(ini))
+    r = subprocess.run([RUNNER, "bad.ini", "--dump-mesh", "x.bin"], cwd=tmp_path,
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and word in r.stderr, r.stderr
