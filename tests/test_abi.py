@@ -57,6 +57,18 @@ def test_ricker_table_matches_oracle(f0, dt, n):
     np.testing.assert_allclose(a, b, rtol=0, atol=1e-13 * np.abs(b).max())
 
 
+@pytest.mark.parametrize("nb_deriv", [0, 1, 2, 3])
+@pytest.mark.parametrize("t_begin,offset", [(0.0, 0.0), (0.0, 0.5e-3), (0.013, 1e-3)])
+def test_time_source_table_matches_oracle(nb_deriv, t_begin, offset):
+    """The host table of the source or a derivative (mjw_time_source_table,
+    AddScalarTimeSource's evaluation) against the oracle's restatement."""
+    f0, dt, n = 8.0, 1e-3, 400
+    a = M.time_source_table(f0, t_begin, dt, n, nb_deriv, offset)
+    b = O.time_table(f0, t_begin, dt, n, nb_deriv, offset)
+    scale = np.abs(b).max()
+    assert scale > 0 and np.abs(a - b).max() <= 1e-14 * scale
+
+
 def test_create_rejects_bad_arguments_without_device():
     L = _lib.lib()
     h = C.c_void_p()

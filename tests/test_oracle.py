@@ -517,6 +517,16 @@ def test_time_table_derivatives():
     assert np.abs(p0[live] - (-a * g / f0)[live]).max() < 1e-11
     assert np.abs(p1[live] - ((2 * np.pi ** 2 * a * a - 1) * g)[live]).max() < 1e-9
     assert not p0[~live].any() and not p1[~live].any()
+    # second and third derivatives: f0 2 pi^2 a (3 - 2 pi^2 a^2) g and
+    # f0^2 2 pi^2 (3 - 12 pi^2 a^2 + 4 pi^4 a^4) g, to the interpolation error
+    pi2 = np.pi ** 2
+    p2 = O.time_table(f0, 0.0, dt, n, 2, 0.5 * dt)
+    p3 = O.time_table(f0, 0.0, dt, n, 3, 0.5 * dt)
+    e2 = f0 * 2 * pi2 * a * (3 - 2 * pi2 * a * a) * g
+    e3 = f0 ** 2 * 2 * pi2 * (3 - 12 * pi2 * a * a + 4 * pi2 * pi2 * a ** 4) * g
+    assert np.abs(p2[live] - e2[live]).max() < 1e-9 * np.abs(e2).max()
+    assert np.abs(p3[live] - e3[live]).max() < 1e-8 * np.abs(e3).max()
+    assert not p2[~live].any() and not p3[~live].any()
 
 
 def test_pml_mesh_geometry():
