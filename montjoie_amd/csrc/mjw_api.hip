@@ -546,7 +546,16 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
     CK(hipMemcpyAsync(p->d_D, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
   else
     CK(hipMemcpyAsync(p->d_D, Df.data(), p->n * p->n * 4, hipMemcpyHostToDevice, st));
-  CK(dalloc(p, &p->d_G, gsz));
+  // straight-sided hexahedra: per-element trilinear coefficients (the
+  // default); the stored per-node factors are then built only when the
+  // stored mode is asked for (build_stored_geometry)
+  const bool tri =
+      dim == 3 && trilinear_coefficients(order, n_elem, elem_nodes, node_xyz, p->perm.data(),
+                                         ms.data(), 1e-11, p->hcoef);
+  if (!tri) {
+    p->hcoef.clear();
+    CK(dalloc(p, &p->d_G, gsz));
+  }
   CK(hipMalloc(&d_xyz, n_nodes * dim * 8));
   CK(hipMalloc(&d_Dg, p->n * p->n * 8));
   CK(hipMalloc(&d_w, p->n * 8));
@@ -577,18 +586,13 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   d_xyz = d_Dg = d_w = d_Mloc = nullptr;
   d_bad = nullptr;
   if (bad) return fail(MJW_ERR_GEOMETRY, "non-positive Jacobian: degenerate or inverted element");
-  // straight-sided hexahedra: per-element trilinear coefficients (default)
-  if (dim == 3 &&
-      trilinear_coefficients(order, n_elem, elem_nodes, node_xyz, p->perm.data(), ms.data(),
-                             1e-11, p->hcoef)) {
+  if (tri) {
     const size_t gb = p->geom == kG64 ? 8 : 4;
     CK(dalloc(p, &p->d_coef, p->hcoef.size() * gb));
     CK(dalloc(p, &p->d_glxw, 2 * p->n * gb));
     CK(upload_trilinear(p, xg, wg));
     p->tri_ok = true;
     p->geo = kGeoTrilinear;
-  } else {
-    p->hcoef.clear();
   }
   CK(dalloc(p, &p->d_Minv, n_nodes * 8));
   CK(dalloc(p, &p->d_isqM, n_nodes * 8));
@@ -669,6 +673,50 @@ extern "C" int mjw_set_xcd_order(mjw_problem *p, int nxcd) {
   return MJW_OK;
 }
 
+// the stored per-node factors of an acoustic or elastic problem with
+// per-element media, built from the node positions when a problem created in
+// trilinear mode is switched to the stored mode
+static hipError_t build_stored_geometry(mjw_problem *p) {
+  if (p->d_G) return hipSuccess;
+  const int dim = p->dim, nd = p->nd;
+  const int ngc = p->ncomp == 3 ? 9 : (dim == 2 ? 3 : 6);
+  const int64_t N = p->N, E = p->E;
+  double xg[kMaxN], wg[kMaxN], Dg[kMaxN * kMaxN];
+  gl_rule(p->order, xg, wg);
+  gl_deriv(p->order, xg, Dg);
+  std::vector<double> xyz_s((size_t)N * dim);
+  for (int64_t g = 0; g < N; ++g)
+    for (int a = 0; a < dim; ++a) xyz_s[(size_t)p->nperm[g] * dim + a] = p->xyz[g * dim + a];
+  void *G = nullptr;
+  double *d_xyz = nullptr, *d_Dg = nullptr, *d_w = nullptr;
+  int *d_bad = nullptr;
+  hipStream_t st = p->stream;
+  hipError_t e = hipSuccess;
+  auto ok = [&](hipError_t x) { if (e == hipSuccess) e = x; return e == hipSuccess; };
+  ok(dalloc(p, &G, (size_t)E * nd * ngc * (p->geom == kG64 ? 8 : 4)));
+  ok(hipMalloc(&d_xyz, xyz_s.size() * 8));
+  ok(hipMalloc(&d_Dg, p->n * p->n * 8));
+  ok(hipMalloc(&d_w, p->n * 8));
+  ok(hipMalloc(&d_bad, 4));
+  ok(hipMemsetAsync(d_bad, 0, 4, st));
+  ok(hipMemcpyAsync(d_xyz, xyz_s.data(), xyz_s.size() * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_Dg, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_w, wg, p->n * 8, hipMemcpyHostToDevice, st));
+  if (e == hipSuccess &&
+      launch_geometry_full(dim, p->order, p->ncomp, E, p->d_map, d_xyz, d_Dg, d_w, p->d_rho,
+                           p->d_mu, nullptr, nullptr, p->geom == kG64 ? (double *)G : nullptr,
+                           p->geom == kG32 ? (float *)G : nullptr, nullptr, d_bad, st))
+    e = hipErrorLaunchFailure;
+  ok(hipStreamSynchronize(st));
+  dfree(d_xyz); dfree(d_Dg); dfree(d_w); dfree(d_bad);
+  if (e != hipSuccess) {
+    dfree(G);
+    return e;
+  }
+  p->d_G = G;
+  return hipSuccess;
+}
+
 extern "C" int mjw_set_geometry_mode(mjw_problem *p, int mode) {
   if (!p) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "NULL problem");
   if (mode != MJW_GEOM_STORED && mode != MJW_GEOM_TRILINEAR)
@@ -677,6 +725,10 @@ extern "C" int mjw_set_geometry_mode(mjw_problem *p, int mode) {
     FAIL(p, MJW_ERR_UNSUPPORTED,
          "trilinear geometry needs a 3-D mesh of straight-sided hexahedra");
   const int g = mode == MJW_GEOM_TRILINEAR ? kGeoTrilinear : kGeoStored;
+  if (g == kGeoStored && !p->d_G) {
+    HIPCHK(p, hipSetDevice(p->device));
+    HIPCHK(p, build_stored_geometry(p));
+  }
   if (g != p->geo) {
     if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
     if (p->bexec) { (void)hipGraphExecDestroy(p->bexec); p->bexec = nullptr; }
@@ -1813,8 +1865,10 @@ extern "C" int mjw_set_mu(mjw_problem *p, const double *mu) {
   HIPCHK(p, hipMemcpy(d_sc, sc.data(), p->E * 8, hipMemcpyHostToDevice));
   const int ngc = p->dim == 2 ? 3 : 6;
   const int K = p->dim == 2 ? 1 : p->n;
-  int rc = launch_scale_planes(p->geom == kG64 ? 0 : 1, p->d_G, (int64_t)K * ngc, p->E, p->P, d_sc,
-                               p->stream);
+  // the stored factors, when they are built, scale with mu
+  int rc = p->d_G ? launch_scale_planes(p->geom == kG64 ? 0 : 1, p->d_G, (int64_t)K * ngc, p->E,
+                                        p->P, d_sc, p->stream)
+                  : 0;
   if (!rc) rc = (int)hipMemcpyAsync(p->d_mu, ms.data(), p->E * 8, hipMemcpyHostToDevice, p->stream);
   if (!rc) rc = (int)hipStreamSynchronize(p->stream);
   if (!rc && p->tri_ok) {
@@ -1869,6 +1923,7 @@ extern "C" int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const do
   ok(hipMalloc(&d_Mloc, bq));
   ok(hipMalloc(&d_rq, bq));
   ok(hipMalloc(&d_mq, bq));
+  if (!p->d_G) ok(dalloc(p, &p->d_G, (size_t)E * nd * (p->dim == 2 ? 3 : 6) * (p->geom == kG64 ? 8 : 4)));
   ok(hipMalloc(&d_bad, 4));
   ok(hipMemsetAsync(d_bad, 0, 4, st));
   ok(hipMemcpyAsync(d_xyz, xyz_s.data(), xyz_s.size() * 8, hipMemcpyHostToDevice, st));

@@ -99,7 +99,7 @@ __global__ void k_geometry(int dim, int n, int ncomp, int64_t E, const int32_t *
   const double Jw = J * w;
   // media at this quadrature point: per-node fields when given, else the
   // element's constants
-  Mloc[id] = (rho_q ? rho_q[id] : rho[e]) * Jw;
+  if (Mloc) Mloc[id] = (rho_q ? rho_q[id] : rho[e]) * Jw;
   const int ij = i + n * j;
   if (ncomp == 1) {
     const double c = Jw * (mu_q ? mu_q[id] : mu[e]);

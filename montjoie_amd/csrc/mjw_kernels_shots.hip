@@ -46,10 +46,14 @@ __global__ __launch_bounds__(N *N *ST) void k_shots3d(
     sD[q] = Dm[q];
     sDt[q] = Dm[(q % N) * N + q / N];
   }
-  // this block's run of R consecutive elements of the colour
-  const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * R;
+  // this block's run of R consecutive elements of the colour and its shot
+  // tile; the tiles of one run are consecutive logical blocks (same XCD, same
+  // time), so they share the L2 lines of the interleaved state
+  const unsigned ntile = (unsigned)((nshots + ST - 1) / ST);
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x, nxcd);
+  const int64_t eb = e0 + (int64_t)(lb / ntile) * R;
   const int64_t ee = eb + R < e1 ? eb + R : e1;
-  const int s = blockIdx.y * ST + lane;
+  const int s = (int)(lb % ntile) * ST + lane;
   const bool sok = s < nshots;
   const rsrc_t ru = mkres(U), ry = mkres(Y), rs = mkres(sdiag);
   int32_t gid[N];
@@ -213,7 +217,7 @@ template <int N, int ST, int R, typename T, typename TG>
 static int run_shots_st(const StiffArgs &a, hipStream_t st) {
   const int64_t ne = a.e1 - a.e0;
   if (ne <= 0) return 0;
-  const dim3 grid((unsigned)((ne + R - 1) / R), (unsigned)((a.nshots + ST - 1) / ST));
+  const dim3 grid((unsigned)(((ne + R - 1) / R) * ((a.nshots + ST - 1) / ST)));
   const dim3 block(N * N * ST);
   const T *D = static_cast<const T *>(a.Dm);
   const T *U = static_cast<const T *>(a.U);
@@ -340,8 +344,9 @@ namespace mjw {
 // MltAdd_ElementH1, Computation/ProdMatVectH1.cxx:674-944) with one element
 // and ST shot lanes per block.  The factor M = sqrt(J w) DF^{-1} of each node
 // is built once per block into LDS (trilinear map or stored planes).  LDS
-// planes [9][k][i + n j][lane]: u_m (0..2), xi_1 fluxes (3..5), xi_2 fluxes
-// (6..8).
+// planes [6][k][i + n j][lane]: u_m (0..2); slab k of them is overwritten by
+// its xi_1 fluxes once every thread has read it (one barrier per slab); xi_2
+// fluxes in planes 3..5.
 template <int N, int ST, typename T, typename TG, int MODE, bool TRI>
 __global__ __launch_bounds__(N *N *ST) void k_shots_el(
     const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
@@ -351,14 +356,19 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
     const TG *__restrict__ glxw) {
   constexpr int P = N * N, NN = P * N, PL = NN * ST;
   using A = TG;
-  __shared__ T lds[9 * PL];
+  __shared__ T lds[6 * PL];
   __shared__ A lm[P * N * 9];  // [i + n j][k][9]
   __shared__ TG lc[TRI ? kCoef : 1];
   const int t = threadIdx.x;
   const int lane = t % ST, ij = t / ST;
   const int i = ij % N, j = ij / N;
-  const int64_t e = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd);
-  const int s = blockIdx.y * ST + lane;
+  // element and shot tile; the tiles of one element are consecutive logical
+  // blocks (same XCD, same time), so they share the L2 lines of the
+  // interleaved state
+  const unsigned ntile = (unsigned)((nshots + ST - 1) / ST);
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x, nxcd);
+  const int64_t e = e0 + (int64_t)(lb / ntile);
+  const int s = (int)(lb % ntile) * ST + lane;
   const bool act = e < e1, sact = act && s < nshots;
   if constexpr (TRI) {
     for (int q = t; q < kCoef; q += P * ST)
@@ -406,17 +416,21 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
       }
   }
   __syncthreads();
+  // per layer k: the gradients of slab k, a barrier (every read of slab k of
+  // the state planes is done), then the slab's xi_1 fluxes overwrite it
   T v2[3][N];
+  T Di[N], Dj[N];
   if (sact) {
-    T Di[N], Dj[N];
 #pragma unroll
     for (int l = 0; l < N; ++l) {
       Di[l] = Dm[i * N + l];
       Dj[l] = Dm[j * N + l];
     }
+  }
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      A g[3][3];
+  for (int k = 0; k < N; ++k) {
+    A g[3][3];
+    if (sact) {
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         T a0 = 0, a1 = 0, a2 = 0;
@@ -430,6 +444,9 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
         g[m][1] = a1;
         g[m][2] = a2;
       }
+    }
+    __syncthreads();
+    if (sact) {
       const A *M = lm + (ij * N + k) * 9;
       A du[3][3];
 #pragma unroll
@@ -446,9 +463,9 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
           sg[m][a] = mu * (du[m][a] + du[a][m]) + (m == a ? lam * tr : A(0));
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        lds[(3 + m) * PL + (k * P + ij) * ST + lane] =
+        lds[m * PL + (k * P + ij) * ST + lane] =
             (T)(M[0] * sg[m][0] + M[1] * sg[m][1] + M[2] * sg[m][2]);
-        lds[(6 + m) * PL + (k * P + ij) * ST + lane] =
+        lds[(3 + m) * PL + (k * P + ij) * ST + lane] =
             (T)(M[3] * sg[m][0] + M[4] * sg[m][1] + M[5] * sg[m][2]);
         v2[m][k] = (T)(M[6] * sg[m][0] + M[7] * sg[m][1] + M[8] * sg[m][2]);
       }
@@ -485,8 +502,8 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
         T o = 0;
 #pragma unroll
         for (int l = 0; l < N; ++l) {
-          o += Dti[l] * lds[(3 + m) * PL + (k * P + j * N + l) * ST + lane];
-          o += Dtj[l] * lds[(6 + m) * PL + (k * P + l * N + i) * ST + lane];
+          o += Dti[l] * lds[m * PL + (k * P + j * N + l) * ST + lane];
+          o += Dtj[l] * lds[(3 + m) * PL + (k * P + l * N + i) * ST + lane];
           o += Dm[l * N + k] * v2[m][l];
         }
         const uint32_t off = slot(m, mm);
@@ -502,12 +519,188 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
   }
 }
 
+// The same pass with one thread per (node, shot): thread (q, lane), q = i +
+// n j + n^2 k.  A block is n^3 ST threads, so for n = 5 and ST = 8 (1000
+// threads) 98 % of the lanes carry work, where the column mapping (25 ST
+// threads) leaves a quarter of its last wave idle.  All three contractions
+// read their neighbours from LDS; after a barrier the nine flux planes
+// overwrite the three state planes.  The sums run in the column kernel's
+// order (a0, a1, a2 per l; xi_1, xi_2, xi_3 terms per l in the divergence), so
+// both mappings give the same bits.  The component offset of an interleaved
+// value is a scalar (the buffer's SGPR offset): one VGPR offset per thread.
+template <int N, int ST, typename T, typename TG, int MODE, bool TRI>
+__global__ __launch_bounds__(N *N *N *ST) void k_shots_el_node(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    const double *__restrict__ lam_e, const double *__restrict__ mu_e, int64_t E, int64_t e0,
+    int64_t e1, const T *__restrict__ U, T *__restrict__ Y, const T *__restrict__ sdiag,
+    T alpha, T beta, int64_t N_nodes, int nshots, int nxcd, const TG *__restrict__ coef,
+    const TG *__restrict__ glxw) {
+  constexpr int P = N * N, NN = P * N, PL = NN * ST;
+  using A = TG;
+  __shared__ T lds[9 * PL];     // u_m (0..2); then fluxes xi_1 (0..2), xi_2 (3..5), xi_3 (6..8)
+  __shared__ A lm[NN * 9];      // [q][9]
+  __shared__ TG lc[TRI ? kCoef : 1];
+  const int t = threadIdx.x;
+  const int lane = t % ST, q = t / ST;
+  const int i = q % N, j = (q / N) % N, k = q / P;
+  // element and shot tile; the tiles of one element are consecutive logical
+  // blocks (same XCD, same time), so they share the L2 lines of the
+  // interleaved state
+  const unsigned ntile = (unsigned)((nshots + ST - 1) / ST);
+  const unsigned lb = xcd_block(blockIdx.x, gridDim.x, nxcd);
+  const int64_t e = e0 + (int64_t)(lb / ntile);
+  const int s = (int)(lb % ntile) * ST + lane;
+  const bool act = e < e1, sact = act && s < nshots;
+  // the element's factor M = sqrt(J w) DF^{-1} per node, built by the first
+  // n^3 threads (whole waves, the others wait at the barrier)
+  if constexpr (TRI) {
+    for (int c = t; c < kCoef; c += NN * ST)
+      if (act) lc[c] = coef[e * kCoef + c];
+    __syncthreads();
+    if (act && t < NN) {
+      const int gij = t % P, gk = t / P, gi = gij % N, gj = gij / N;
+      TriCol<A> tc;
+      tri_setup<A, TG>(lc, (A)glxw[gi], (A)glxw[gj], (A)glxw[N + gi] * (A)glxw[N + gj], false, tc);
+      tri_m<A>(tc, (A)glxw[gk], (A)glxw[N + gk], lm + t * 9);
+    }
+  } else {
+    if (act && t < NN) {
+      const int gij = t % P, gk = t / P;
+#pragma unroll
+      for (int c = 0; c < 9; ++c) lm[t * 9 + c] = (A)G[((int64_t)(gk * 9 + c) * E + e) * P + gij];
+    }
+  }
+  int32_t gq = 0;
+  A lam = 0, mu = 0;
+  if (act) {
+    gq = map[((int64_t)k * E + e) * P + i + N * j];
+    lam = (A)lam_e[e];
+    mu = (A)mu_e[e];
+  }
+  const rsrc_t ru = mkres(U), ry = mkres(Y), rs = mkres(sdiag);
+  const uint32_t vo = ((uint32_t)(gq & 0x7fffffff) * (uint32_t)nshots + (uint32_t)s) *
+                      (uint32_t)sizeof(T);
+  const uint32_t cs = (uint32_t)N_nodes * (uint32_t)nshots * (uint32_t)sizeof(T);
+  T u[3], yk[3], sk = 0;
+  if (sact) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      u[m] = bld<T>(ru, vo, (uint32_t)m * cs);
+      lds[m * PL + t] = u[m];
+    }
+    // previous values and dt^2/M, in flight across the barriers
+#pragma unroll
+    for (int m = 0; m < 3; ++m) yk[m] = bld<T>(ry, vo, (uint32_t)m * cs);
+    if constexpr (MODE == kLeap)
+      sk = bld<T>(rs, (uint32_t)(gq & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
+  }
+  __syncthreads();
+  T f1[3], f2[3], f3[3];
+  if (sact) {
+    T Di[N], Dj[N], Dk[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      Di[l] = Dm[i * N + l];
+      Dj[l] = Dm[j * N + l];
+      Dk[l] = Dm[k * N + l];
+    }
+    A g[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        a0 += Di[l] * lds[m * PL + (l + N * j + P * k) * ST + lane];
+        a1 += Dj[l] * lds[m * PL + (i + N * l + P * k) * ST + lane];
+        a2 += Dk[l] * lds[m * PL + (i + N * j + P * l) * ST + lane];
+      }
+      g[m][0] = a0;
+      g[m][1] = a1;
+      g[m][2] = a2;
+    }
+    const A *M = lm + q * 9;
+    A du[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        du[m][a] = M[0 * 3 + a] * g[m][0] + M[1 * 3 + a] * g[m][1] + M[2 * 3 + a] * g[m][2];
+    const A tr = du[0][0] + du[1][1] + du[2][2];
+    A sg[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        sg[m][a] = mu * (du[m][a] + du[a][m]) + (m == a ? lam * tr : A(0));
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      f1[m] = (T)(M[0] * sg[m][0] + M[1] * sg[m][1] + M[2] * sg[m][2]);
+      f2[m] = (T)(M[3] * sg[m][0] + M[4] * sg[m][1] + M[5] * sg[m][2]);
+      f3[m] = (T)(M[6] * sg[m][0] + M[7] * sg[m][1] + M[8] * sg[m][2]);
+    }
+  }
+  __syncthreads();  // every read of the state planes is done
+  if (sact) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      lds[m * PL + t] = f1[m];
+      lds[(3 + m) * PL + t] = f2[m];
+      lds[(6 + m) * PL + t] = f3[m];
+    }
+  }
+  __syncthreads();
+  if (sact) {
+    T Dti[N], Dtj[N], Dtk[N];
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      Dti[l] = Dm[l * N + i];
+      Dtj[l] = Dm[l * N + j];
+      Dtk[l] = Dm[l * N + k];
+    }
+    const bool first = gq < 0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      T o = 0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        o += Dti[l] * lds[m * PL + (l + N * j + P * k) * ST + lane];
+        o += Dtj[l] * lds[(3 + m) * PL + (i + N * l + P * k) * ST + lane];
+        o += Dtk[l] * lds[(6 + m) * PL + (i + N * j + P * l) * ST + lane];
+      }
+      const T y = yk[m];
+      T r;
+      if constexpr (MODE == kLeap)
+        r = first ? (T(2) * u[m] - y - sk * o) : (y - sk * o);
+      else
+        r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
+      bst<T>(r, ry, vo, (uint32_t)m * cs);
+    }
+  }
+}
+
+// thread-per-node tile: the largest power of two ST <= 16 with n^3 ST <= 1024
+// threads and LDS = 9 n^3 ST sizeof(T) + 9 n^3 sizeof(TG) + kCoef sizeof(TG)
+// <= 64 KiB
+template <int N, typename T, typename TG>
+struct ShotTileElNode {
+  static constexpr int lds(int st) {
+    return 9 * N * N * N * st * (int)sizeof(T) + 9 * N * N * N * (int)sizeof(TG) +
+           kCoef * (int)sizeof(TG);
+  }
+  static constexpr int pick() {
+    int st = 16;
+    while (st > 1 && (lds(st) > 65536 || N * N * N * st > 1024)) st /= 2;
+    return st;
+  }
+  static constexpr int value = (lds(1) <= 65536 && N * N * N <= 1024) ? pick() : 0;
+};
+
 // lanes: largest power of two <= 16 with n^2*ST <= 1024 threads and
-// LDS = 9*n^3*ST*sizeof(T) + 9*n^3*sizeof(TG) + kCoef*sizeof(TG) <= 64 KiB
+// LDS = 6*n^3*ST*sizeof(T) + 9*n^3*sizeof(TG) + kCoef*sizeof(TG) <= 64 KiB
 template <int N, typename T, typename TG>
 struct ShotTileEl {
   static constexpr int lds(int st) {
-    return 9 * N * N * N * st * (int)sizeof(T) + 9 * N * N * N * (int)sizeof(TG) +
+    return 6 * N * N * N * st * (int)sizeof(T) + 9 * N * N * N * (int)sizeof(TG) +
            kCoef * (int)sizeof(TG);
   }
   static constexpr int pick() {
@@ -522,7 +715,7 @@ template <int N, int ST, typename T, typename TG>
 static int run_shots_el_st(const StiffArgs &a, hipStream_t st) {
   const int64_t ne = a.e1 - a.e0;
   if (ne <= 0) return 0;
-  const dim3 grid((unsigned)ne, (unsigned)((a.nshots + ST - 1) / ST));
+  const dim3 grid((unsigned)(ne * ((a.nshots + ST - 1) / ST)));
   const dim3 block(N * N * ST);
   const T *D = static_cast<const T *>(a.Dm);
   const T *U = static_cast<const T *>(a.U);
@@ -548,9 +741,61 @@ static int run_shots_el_st(const StiffArgs &a, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+template <int N, int ST, typename T, typename TG>
+static int run_shots_el_node_st(const StiffArgs &a, hipStream_t st) {
+  const int64_t ne = a.e1 - a.e0;
+  if (ne <= 0) return 0;
+  const dim3 grid((unsigned)(ne * ((a.nshots + ST - 1) / ST)));
+  const dim3 block(N * N * N * ST);
+  const T *D = static_cast<const T *>(a.Dm);
+  const T *U = static_cast<const T *>(a.U);
+  T *Y = static_cast<T *>(a.Y);
+  const T *s = static_cast<const T *>(a.sdiag);
+  const TG *G = static_cast<const TG *>(a.G);
+  const TG *cf = static_cast<const TG *>(a.coef);
+  const TG *gw = static_cast<const TG *>(a.glxw);
+  const bool tri = a.geo == kGeoTrilinear;
+  if (tri && (!cf || !gw)) return -1002;
+#define MJW_SHOTS_ELN(MODE, TRI)                                                             \
+  hipLaunchKernelGGL((k_shots_el_node<N, ST, T, TG, MODE, TRI>), grid, block, 0, st, a.map, \
+                     G, D, a.lam, a.mu, a.E, a.e0, a.e1, U, Y, s, (T)a.alpha, (T)a.beta,    \
+                     a.N, a.nshots, a.nxcd, cf, gw)
+  if (a.mode == kLeap) {
+    if (tri) MJW_SHOTS_ELN(kLeap, true);
+    else MJW_SHOTS_ELN(kLeap, false);
+  } else {
+    if (tri) MJW_SHOTS_ELN(kApply, true);
+    else MJW_SHOTS_ELN(kApply, false);
+  }
+#undef MJW_SHOTS_ELN
+  return (int)hipGetLastError();
+}
+
+// tile -ST: one thread per (node, shot) with ST shot lanes
+template <int N, typename T, typename TG>
+static int run_shots_el_node_t(const StiffArgs &a, hipStream_t st, int want) {
+  constexpr int SM = ShotTileElNode<N, T, TG>::value;
+  if constexpr (SM < 1) {
+    return -1005;
+  } else {
+    if (want > SM) return -1005;
+    if constexpr (SM >= 16)
+      if (want == 16) return run_shots_el_node_st<N, 16, T, TG>(a, st);
+    if constexpr (SM >= 8)
+      if (want == 8) return run_shots_el_node_st<N, 8, T, TG>(a, st);
+    if constexpr (SM >= 4)
+      if (want == 4) return run_shots_el_node_st<N, 4, T, TG>(a, st);
+    if constexpr (SM >= 2)
+      if (want == 2) return run_shots_el_node_st<N, 2, T, TG>(a, st);
+    if (want == 1) return run_shots_el_node_st<N, 1, T, TG>(a, st);
+    return -1005;
+  }
+}
+
 template <int N, typename T, typename TG>
 static int run_shots_el_t(const StiffArgs &a, hipStream_t st) {
   constexpr int SM = ShotTileEl<N, T, TG>::value;
+  if (a.epb < 0) return run_shots_el_node_t<N, T, TG>(a, st, -a.epb);
   if constexpr (SM < 1) {
     return -1005;
   } else {
@@ -596,11 +841,19 @@ int launch_stiffness_shots_elastic(const StiffArgs &a, void *stream) {
 
 template <int N>
 static void shot_tiles_el_n(int state, int geom, std::vector<int> &v) {
-  int sm;
-  if (state == kF64) sm = ShotTileEl<N, double, double>::value;
-  else if (geom == kG64) sm = ShotTileEl<N, float, double>::value;
-  else sm = ShotTileEl<N, float, float>::value;
+  int sm, sn;
+  if (state == kF64) {
+    sm = ShotTileEl<N, double, double>::value;
+    sn = ShotTileElNode<N, double, double>::value;
+  } else if (geom == kG64) {
+    sm = ShotTileEl<N, float, double>::value;
+    sn = ShotTileElNode<N, float, double>::value;
+  } else {
+    sm = ShotTileEl<N, float, float>::value;
+    sn = ShotTileElNode<N, float, float>::value;
+  }
   for (int t = sm; t >= 1; t /= 2) v.push_back(t);
+  for (int t = sn; t >= 1; t /= 2) v.push_back(-t);
 }
 
 std::vector<int> tile_choices_shots_elastic(int order, int state, int geom) {

@@ -159,6 +159,34 @@ def test_every_shot_tile_gives_identical_results(prec):
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32_g64"])
+def test_every_elastic_shot_tile_gives_identical_results(prec):
+    """Interleaved elastic shots: the column mapping (tile ST) and the
+    thread-per-node mapping (tile -ST) at every shot-lane count compute the
+    same numbers."""
+    from montjoie_amd import tuner
+    m = M.regular_mesh(3, 3, 4, 0.1, seed=12)
+    E = m.E
+    P = M.WaveProblem(m, 1.0 + (np.arange(E) % 2), 1.0 + (np.arange(E) % 3),
+                      lam=2.0 + (np.arange(E) % 4), ncomp=3, precision=prec, nshots=11)
+    rng = np.random.default_rng(6)
+    pol = rng.standard_normal((11, 3))
+    P.set_sources(0.25 + 0.5 * rng.random((11, 3)), pol=pol, shot=np.arange(11))
+    P.set_receivers(np.repeat(0.2 + 0.6 * rng.random((4, 3)), 3, axis=0),
+                    np.tile(np.arange(3, dtype=np.int32), 4))
+    P.set_pulse(M.ricker_pulse_table(8.0, 0.0, 1e-3, 40))
+    tiles = tuner.tile_choices(P)
+    assert any(t < 0 for t in tiles) and any(t > 0 for t in tiles)
+    outs = []
+    for t in tiles:
+        print(f"elastic shot tile {t}", flush=True)
+        P.set_tile(t)
+        P.leapfrog_init(1e-3)
+        outs.append(P.run(30))
+    assert np.abs(outs[0]).max() > 0
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
 def test_energy_conserved_fp64():
     m = M.regular_mesh(3, 3, 4, 0.1, seed=9)
     P = M.WaveProblem(m, 1.0, 1.0 + np.arange(m.E) % 2)
