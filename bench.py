@@ -173,7 +173,9 @@ def bench_shots(m, mu, dt, steps, warmup, nshots=16, acc_steps=500):
         P.set_sources(srcs[:ns], shot=np.arange(ns))
         P.set_receivers(receivers())
         P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, max(warmup, steps, acc_steps, 16) + 64))
-        tuner.tune_problem(P, steps=4, dt=dt)  # launch shape only: result-neutral
+        # launch shape only; every candidate but the matrix-core tile (FP32
+        # rounding) gives the same bits
+        tuner.tune_problem(P, steps=4, dt=dt)
         wall, tr = timed_advance(P, dt, steps, warmup)
         dev_ms = P.last_timing()[0]
         if key != "fp64" and steps != acc_steps:

@@ -105,6 +105,11 @@ int launch_stiffness(const StiffArgs &a, void *stream);
 // 3-D acoustic, several shots in the interleaved layout; StiffArgs::epb =
 // shot lanes per column (0: automatic)
 int launch_stiffness_shots(const StiffArgs &a, void *stream);
+// interleaved acoustic shots with the contractions on the matrix cores (FP32
+// state, orders 1..6): tile kTileMfma
+constexpr int kTileMfma = 1000;
+bool shots_mfma_supported(int order, int state);
+int launch_stiffness_shots_mfma(const StiffArgs &a, void *stream);
 // supported shot-lane counts, largest first
 std::vector<int> tile_choices_shots(int order, int state, int geom);
 // isotropic elastic, several shots interleaved

@@ -251,6 +251,7 @@ template <int N, typename T, typename TG>
 static int run_shots_t(const StiffArgs &a, hipStream_t st) {
   constexpr int SM = ShotTile<N, T, TG>::value;
   static_assert(SM >= 1, "shot tile");
+  if (a.epb == kTileMfma) return launch_stiffness_shots_mfma(a, st);
   int want = a.epb % 100, R = a.epb / 100;
   if (a.epb <= 0) {
     want = 1;
@@ -310,6 +311,7 @@ std::vector<int> tile_choices_shots(int order, int state, int geom) {
     case 8: shot_tiles_n<8>(state, geom, v); break;
     default: break;
   }
+  if (shots_mfma_supported(order, state)) v.push_back(kTileMfma);
   return v;
 }
 

@@ -152,11 +152,66 @@ def test_every_shot_tile_gives_identical_results(prec):
     P.set_pulse(M.ricker_pulse_table(8.0, 0.0, 1e-3, 40))
     outs = []
     for t in tuner.tile_choices(P):
+        if t == MFMA_TILE:  # sums the divergence in another order: test_mfma_shot_tile_*
+            continue
         P.set_tile(t)
         P.leapfrog_init(1e-3)
         outs.append(P.run(30))
     assert len(outs) >= 3
     assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+MFMA_TILE = 1000
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp32_g64"])
+@pytest.mark.parametrize("order", [2, 4, 5, 6])
+def test_mfma_shot_tile_matches_vector_tiles(prec, order):
+    """Interleaved acoustic shots with the contractions on the matrix cores
+    (v_mfma_f32_4x4x1f32): 19 shots (a full and a partial tile of 16), every
+    shot against the FP64 path (rel-L2 <= 1e-5, or no further than the vector
+    kernel's own FP32 error where that is larger) and against the vector
+    kernel; the interleaved operator application through the energy."""
+    from montjoie_amd import tuner
+    m = M.regular_mesh(3, 3, order, 0.1, seed=13)
+    mu = 1.0 + (np.arange(m.E) % 3)
+    rng = np.random.default_rng(8)
+    src = 0.25 + 0.5 * rng.random((19, 3))
+    rec = 0.2 + 0.6 * rng.random((6, 3))
+
+    def make(p):
+        Q = M.WaveProblem(m, 1.0, mu, precision=p, nshots=19)
+        Q.set_sources(src, shot=np.arange(19))
+        Q.set_receivers(rec)
+        Q.set_pulse(M.ricker_pulse_table(8.0, 0.0, 1e-3, 80))
+        return Q
+
+    P = make(prec)
+    tiles = tuner.tile_choices(P)
+    assert MFMA_TILE in tiles
+    vec = max(t for t in tiles if 0 < t <= 16)  # widest vector shot tile
+    P.set_tile(vec)
+    P.leapfrog_init(1e-3)
+    ref = P.run(60)
+    P.set_tile(MFMA_TILE)
+    P.leapfrog_init(1e-3)
+    got = P.run(60)
+    P64 = make("fp64")
+    P64.leapfrog_init(1e-3)
+    r64 = P64.run(60)
+    assert np.abs(ref).max() > 0
+    for s in range(19):
+        e_vec = CS.rel_l2(got[:, s], ref[:, s])
+        e_64 = CS.rel_l2(got[:, s], r64[:, s])
+        e_ref = CS.rel_l2(ref[:, s], r64[:, s])  # the vector kernel's own FP32 error
+        tol = max(1e-5, 1.5 * e_ref)
+        assert e_vec < 2 * tol and e_64 < tol, (s, e_vec, e_64, e_ref)
+    # the interleaved operator application (energy of every shot)
+    e_mfma = np.asarray(P.energy())
+    P.set_tile(vec)
+    e_vec = np.asarray(P.energy())
+    assert np.abs(e_vec).max() > 0
+    np.testing.assert_allclose(e_mfma, e_vec, rtol=1e-5, atol=1e-7 * np.abs(e_vec).max())
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32_g64"])
