@@ -323,7 +323,7 @@ def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028,
     from montjoie_amd import tuner
     from montjoie_amd.batch import Batch
     t_setup = time.perf_counter()
-    probs, dts, ndof, nbytes = [], [], 0, 0
+    probs, dts, ndof, nbytes, obytes = [], [], 0, 0, 0
     for i in range(nprob):
         k = kmin + int(_unif(seed, 2 * i) * (kmax - kmin + 1))
         r = omin + int(_unif(seed, 2 * i + 1) * (omax - omin + 1))
@@ -338,6 +338,7 @@ def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028,
         dts.append(dt)
         ndof += m.N
         nbytes += algorithmic_bytes_per_step(m.E, m.nd, m.N, 4, 8)
+        obytes += trilinear_bytes_per_step(m.E, m.nd, m.N, 4, 8)
         del m
         if i % 16 == 15:
             print(f"# batch setup: {i + 1}/{nprob} problems, {ndof} dof", file=sys.stderr, flush=True)
@@ -369,16 +370,26 @@ def bench_batch(nprob=64, kmin=8, kmax=40, omin=3, omax=7, steps=300, seed=2028,
     for P in probs:
         P.close()
     achieved = nbytes / (dev_ms * 1e-3) / 1e9
+    own = obytes / (dev_ms * 1e-3) / 1e9
     return {"problems": nprob, "total_dof": ndof, "steps": steps, "orders": orders,
             "precision": "FP32 state, FP64 geometric factors",
             "batch_s": wall, "batch_gdof_steps_s": ndof * steps / wall / 1e9,
             "one_by_one_s": wall_seq, "one_by_one_gdof_steps_s": ndof * steps / wall_seq / 1e9,
-            "setup_s": t_setup, "autotuned": tuned and {"order_mode": tuned["order_mode"]},
+            "setup_s": t_setup,
+            "autotuned": tuned and {"order_mode": tuned["order_mode"],
+                                    "candidates_ms_per_step": {str(m): ms for ms, m in
+                                                               tuned["candidates"]},
+                                    "modes": "0/1: one launch per colour over every order; "
+                                             "2/3/4: one launch per (colour, order), blocks of "
+                                             "about 256/512/128 threads"},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                          "note": "whole step (8 colour launches over all problems + tails), "
                                  "stored-geometry algorithmic bytes summed over the problems",
-                         "algorithmic_bytes_per_step": nbytes, "device_ms_per_step": dev_ms}}
+                         "algorithmic_bytes_per_step": nbytes, "device_ms_per_step": dev_ms,
+                         "own_bytes_per_step": obytes, "own_bytes_frac": own / HBM_PEAK_GBS,
+                         "own_bytes_note": "trilinear kernels: 24 coefficients per element, "
+                                           "map per element node, 4 state streams per node"}}
 
 
 def bench_host_apply(m, mu, reps=5):
@@ -537,42 +548,117 @@ def bench_2d(nx=64, order=4, steps=1000):
     return out
 
 
-def cpu_reference(nx=24, order=5, steps=150, dt=1e-3):
-    """Montjoie's own solver (oracle/_ref, built from the reference sources)
-    on a bounded sample of the same workload family, one core."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "mjref_acous.x")
-    if not os.path.exists(exe):
-        return None
+def _ref_acoustic_deck(d, nx, order, steps, dt):
+    """Montjoie data file + Medit mesh of the CPU sample (acoustic, the
+    headline's layered perturbed cube at a bounded size); returns the DoF count."""
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     from oracle import oracle as O
     import decks as F
+    os.makedirs(d, exist_ok=True)
+    m = O.mesh(3, nx, order, 0.1, 2024)
+    c = [1.0 + 2.0 * _unif(2025, k) for k in range(3)]
+    lay = ((np.arange(m.E) // nx ** 2) * 3) // nx
+    F.write_medit(os.path.join(d, "mesh.mesh"), m, lay + 1, bnd_ref=4)
+    ini = ["TypeElement = HEXAHEDRON_LOBATTO", "TypeEquation = ACOUSTIC", "MeshPath = ./",
+           "Frequency = 8.0 0.0", "FileMesh = mesh.mesh", "TypeMesh = HEXA",
+           f"OrderDiscretization = {order}", "ConditionReference = 4 DIRICHLET"]
+    ini += [f"MateriauDielec = {k + 1} ISOTROPE 1.0 {c[k] ** 2!r} 0.0" for k in range(3)]
+    ini += ["TypeSource = SRC_DIRAC", "OriginePhase = 0.5 0.5 0.5",
+            f"ParametersOutputPoint = 0.0 {steps * dt + dt / 2!r} {dt!r}",
+            "FileOutputPoint = diffracPoint.dat totalPoint.dat", "SismoPoint = 0.3 0.4 0.6",
+            "OutputFormat = BINARY DOUBLE", "ElectricOrMagnetic = 0", f"TimeStep = {dt!r}",
+            f"TimeInterval = 0.0 {(steps - 0.5) * dt!r}", "OrderTimeScheme = 2 LEAP_FROG",
+            "TemporalSource = RICKER", "PrintLevel = 1"]
+    open(os.path.join(d, "case.ini"), "w").write("\n".join(ini) + "\n")
+    return m.N
+
+
+def _ref_elastic_deck(d, nx, order, steps, dt, shot=0):
+    """Elastic sample (the C3 family: layered lambda, mu, rho, perturbed cube,
+    one of the 16 Dirac shots with its direction); returns the DoF count."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from oracle import oracle as O
+    import decks as F
+    lam = [1.0 + 3.0 * _unif(2026, k) for k in range(3)]
+    mu = [1.0 + 3.0 * _unif(2026, 3 + k) for k in range(3)]
+    rho = [1.0 + _unif(2026, 6 + k) for k in range(3)]
+    src = shot_sources(16)[shot]
+    pol = sphere_dirs(16)[shot]
+    F.write_deck(d, 3, nx, order, 0.1, 2024, mu, 8.0, dt, steps, tuple(src),
+                 np.array([[0.3, 0.4, 0.6]]), elastic=True, lam_layers=lam, rho_layers=rho,
+                 pol=tuple(pol))
+    ini = os.path.join(d, "case.ini")
+    txt = open(ini).read().replace("PrintLevel = 0", "PrintLevel = 1")  # prints the loop time
+    open(ini, "w").write(txt)
+    return 3 * O.mesh(3, nx, order, 0.1, 2024).N
+
+
+def _ref_run(d, elastic, timeout=900):
+    exe = os.path.join(ROOT, "oracle", "_ref", "mjref_elastic.x" if elastic else "mjref_acous.x")
+    args = [exe, "case.ini"] + ([] if elastic else ["3"])
+    return subprocess.Popen(args, cwd=d, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                            text=True)
+
+
+def _ref_loop_seconds(proc, timeout=900):
+    out, _ = proc.communicate(timeout=timeout)
+    mt = re.search(r"Time to complete time iterations : ([0-9.eE+-]+)", out)
+    if proc.returncode != 0 or not mt:
+        return None
+    return float(mt.group(1))
+
+
+def cpu_reference(nx=24, order=5, steps=150, dt=1e-3):
+    """Montjoie's own solver (oracle/_ref, built from the reference sources)
+    on a bounded sample of the same workload family, one core."""
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "mjref_acous.x")):
+        return None
     with tempfile.TemporaryDirectory(dir=ROOT, prefix=".mjref_") as d:
-        m = O.mesh(3, nx, order, 0.1, 2024)
-        c = [1.0 + 2.0 * _unif(2025, k) for k in range(3)]
-        lay = ((np.arange(m.E) // nx ** 2) * 3) // nx
-        F.write_medit(os.path.join(d, "mesh.mesh"), m, lay + 1, bnd_ref=4)
-        ini = ["TypeElement = HEXAHEDRON_LOBATTO", "TypeEquation = ACOUSTIC", "MeshPath = ./",
-               "Frequency = 8.0 0.0", "FileMesh = mesh.mesh", "TypeMesh = HEXA",
-               f"OrderDiscretization = {order}", "ConditionReference = 4 DIRICHLET"]
-        ini += [f"MateriauDielec = {k + 1} ISOTROPE 1.0 {c[k] ** 2!r} 0.0" for k in range(3)]
-        ini += ["TypeSource = SRC_DIRAC", "OriginePhase = 0.5 0.5 0.5",
-                f"ParametersOutputPoint = 0.0 {steps * dt + dt / 2!r} {dt!r}",
-                "FileOutputPoint = diffracPoint.dat totalPoint.dat", "SismoPoint = 0.3 0.4 0.6",
-                "OutputFormat = BINARY DOUBLE", "ElectricOrMagnetic = 0", f"TimeStep = {dt!r}",
-                f"TimeInterval = 0.0 {(steps - 0.5) * dt!r}", "OrderTimeScheme = 2 LEAP_FROG",
-                "TemporalSource = RICKER", "PrintLevel = 1"]
-        open(os.path.join(d, "case.ini"), "w").write("\n".join(ini) + "\n")
-        r = subprocess.run([exe, "case.ini", "3"], cwd=d, stdout=subprocess.PIPE,
-                           stderr=subprocess.STDOUT, text=True, timeout=600)
-        mt = re.search(r"Time to complete time iterations : ([0-9.eE+-]+)", r.stdout)
-        if r.returncode != 0 or not mt:
+        ndof = _ref_acoustic_deck(d, nx, order, steps, dt)
+        secs = _ref_loop_seconds(_ref_run(d, False))
+        if secs is None:
             return None
-        secs = float(mt.group(1))
-        return dict(value=m.N * steps / secs / 1e9, unit="GDoF-steps/s", cores=1,
+        return dict(value=ndof * steps / secs / 1e9, unit="GDoF-steps/s", cores=1,
                     kind="reference",
                     sample=f"Montjoie's own leapfrog (oracle/_ref, g++ -O2), {nx}^3 Q{order} "
-                           f"perturbed layered cube ({m.N} DoF), {steps} steps, time loop "
+                           f"perturbed layered cube ({ndof} DoF), {steps} steps, time loop "
                            f"{secs:.2f} s, one core")
+
+
+def cpu_reference_procs(elastic, nproc=8, nx=None, order=None, steps=None, dt=None):
+    """Batch baseline of the reference: `nproc` concurrent single-core
+    processes of Montjoie's own solver (the path has no threads), each on a
+    bounded sample of the batch's workload family (acoustic: the 24^3 Q5
+    layered cube; elastic: 12^3 Q4 layered cube, shots of the 16-shot set);
+    aggregate throughput = nproc * DoF * steps / slowest time loop."""
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref",
+                                       "mjref_elastic.x" if elastic else "mjref_acous.x")):
+        return None
+    nx = nx or (12 if elastic else 24)
+    order = order or (4 if elastic else 5)
+    steps = steps or (100 if elastic else 150)
+    dt = dt or (5e-4 if elastic else 1e-3)
+    with tempfile.TemporaryDirectory(dir=ROOT, prefix=".mjref_") as root:
+        dirs = [os.path.join(root, f"p{k}") for k in range(nproc)]
+        ndof = 0
+        for k, d in enumerate(dirs):
+            ndof = (_ref_elastic_deck(d, nx, order, steps, dt, shot=k) if elastic
+                    else _ref_acoustic_deck(d, nx, order, steps, dt))
+        t0 = time.perf_counter()
+        procs = [_ref_run(d, elastic) for d in dirs]
+        secs = [_ref_loop_seconds(p) for p in procs]
+        wall = time.perf_counter() - t0
+        if any(x is None for x in secs):
+            return None
+        slow = max(secs)
+        kind = "elastic" if elastic else "acoustic"
+        return dict(value=nproc * ndof * steps / slow / 1e9, unit="GDoF-steps/s", cores=nproc,
+                    kind="reference",
+                    sample=f"{nproc} concurrent processes of Montjoie's own {kind} leapfrog "
+                           f"(oracle/_ref, g++ -O2, one core each), {nx}^3 Q{order} layered "
+                           f"perturbed cube ({ndof} DoF each), {steps} steps; time loops "
+                           f"{min(secs):.2f}-{slow:.2f} s, wall {wall:.2f} s",
+                    one_process_value=ndof * steps / min(secs) / 1e9)
 
 
 def main():
@@ -666,6 +752,7 @@ def main():
     launch_bytes = bytes_step / ncol
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
     kernel_bytes = (trilinear_bytes_per_step(m.E, nd, m.N, 4, 8) if tri else bytes_step) / ncol
+    node_slot = m.E / ncol * (nd * (4 * 4 + 4) + (24 * 8 if tri else 6 * 8 * nd))
     traffic = (measured_traffic("stiffness_q5_48cube_g64_counters.md")
                if (a.nx == 48 and tri) else None)
     flops = flops_per_step(m.E, 6)
@@ -720,6 +807,12 @@ def main():
                                                "state (stored-geometry algorithm)",
                      "own_bytes_per_launch": kernel_bytes,
                      "own_bytes_frac": kernel_bytes / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     # what a colour pass must move when no node survives in cache between
+                     # passes (the 224 MB state exceeds the 32 MB of L2): every node slot
+                     # of its elements (4 state streams + map) plus the coefficients
+                     "node_slot_bytes_per_launch": node_slot,
+                     "traffic_over_node_slot_bytes": None if traffic is None else
+                     traffic / node_slot,
                      "traffic_GBs": None if traffic is None else
                      traffic / (launch_ms * 1e-3) / 1e9,
                      "kernel": kernel_name(head["tile"], tri, t="float").replace(
@@ -746,6 +839,12 @@ def main():
         print(f"# 2d: {out['acoustic_2d']}", file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_reference()
+        if not a.quick:
+            # batch baselines: 8 concurrent single-core reference processes
+            out["batch_64_problems"]["cpu_baseline_8proc"] = cpu_reference_procs(False)
+            out["elastic_16_shots"]["cpu_baseline_8proc"] = cpu_reference_procs(True)
+            print(f"# cpu 8 processes: {out['batch_64_problems']['cpu_baseline_8proc']}, "
+                  f"{out['elastic_16_shots']['cpu_baseline_8proc']}", file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 

@@ -444,8 +444,11 @@ int mjw_batch_create(mjw_batch **out, int nprob, mjw_problem *const *probs);
 int mjw_batch_advance(mjw_batch *b, int nsteps, int use_graph);
 /* traces [nsteps][nrec] of problem iprob from the last mjw_batch_advance */
 int mjw_batch_traces(mjw_batch *b, int iprob, double *traces);
-/* tuning: order_mode 0 = work items by decreasing element order (default),
- * 1 = problem by problem; nxcd reserved */
+/* tuning: order_mode 0 = one launch per colour, work items by decreasing
+ * element order (default), 1 = the same launch problem by problem; 2, 3, 4 =
+ * one launch per (colour, element order), blocks of about 256, 512 or 128
+ * threads with LDS sized for that order; every mode gives the same bits;
+ * nxcd reserved */
 int mjw_batch_config(mjw_batch *b, int nxcd, int order_mode);
 /* device ms per step of the last mjw_batch_advance */
 int mjw_batch_timing(const mjw_batch *b, double *ms);

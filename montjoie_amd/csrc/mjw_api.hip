@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <array>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -2412,10 +2413,15 @@ struct mjw_batch {
   std::vector<mjw_problem *> probs;
   int state = kF64, geom = kG64;
   int ncolor = 0;
-  int order_mode = 0;                 // 0: items by decreasing order, 1: problem-major
+  // work order: 0 items by decreasing order, 1 problem-major (one launch per
+  // colour over every order); 2, 3, 4 one launch per (colour, order) with
+  // blocks of about 256, 512, 128 threads
+  int order_mode = 0;
   BatchProb *d_probs = nullptr;
   std::vector<BatchItem *> d_items;   // per colour
   std::vector<int> nitems;
+  // grouped modes: per colour, (order, first item, item count) of each order
+  std::vector<std::vector<std::array<int, 3>>> segs;
   TailArgs *d_tail[2] = {nullptr, nullptr};
   hipGraphExec_t gexec = nullptr;
   bool g_traces = false;
@@ -2428,6 +2434,7 @@ static void batch_free_items(mjw_batch *b) {
   for (BatchItem *q : b->d_items) dfree(q);
   b->d_items.clear();
   b->nitems.clear();
+  b->segs.clear();
 }
 
 static int batch_build_items(mjw_batch *b) {
@@ -2439,19 +2446,27 @@ static int batch_build_items(mjw_batch *b) {
     for (size_t i = 0; i < b->probs.size(); ++i) {
       mjw_problem *p = b->probs[i];
       if (c >= p->ncolor) continue;
-      const int epb = batch_epb(p->order);
+      const int epb =
+          b->order_mode >= 2 ? batch_group_epb(p->order, b->order_mode - 2) : batch_epb(p->order);
       for (int64_t eb = p->coff[c]; eb < p->coff[c + 1]; eb += epb) {
         BatchItem it{};
         it.prob = (int32_t)i;
         it.eb = eb;
         it.e1 = p->coff[c + 1];
-        items.push_back({b->order_mode == 0 ? -p->order : (int)i, it});
+        items.push_back({b->order_mode == 1 ? (int)i : -p->order, it});
       }
     }
     std::stable_sort(items.begin(), items.end(),
                      [](const auto &x, const auto &y) { return x.first < y.first; });
     std::vector<BatchItem> h(items.size());
     for (size_t k = 0; k < items.size(); ++k) h[k] = items[k].second;
+    std::vector<std::array<int, 3>> sg;  // runs of one order (items sorted by order)
+    for (size_t k = 0; k < items.size(); ++k) {
+      const int ord = -items[k].first;
+      if (sg.empty() || sg.back()[0] != ord) sg.push_back({ord, (int)k, 0});
+      sg.back()[2]++;
+    }
+    b->segs.push_back(sg);
     BatchItem *d = nullptr;
     if (hipMalloc(&d, std::max<size_t>(1, h.size()) * sizeof(BatchItem)) != hipSuccess) return -1;
     if (!h.empty() &&
@@ -2519,7 +2534,7 @@ extern "C" int mjw_batch_create(mjw_batch **out, int nprob, mjw_problem *const *
 }
 
 extern "C" int mjw_batch_config(mjw_batch *b, int nxcd, int order_mode) {
-  if (!b || order_mode < 0 || order_mode > 1) BFAIL(b, MJW_ERR_ARG, "mjw_batch_config: bad args");
+  if (!b || order_mode < 0 || order_mode > 4) BFAIL(b, MJW_ERR_ARG, "mjw_batch_config: bad args");
   (void)nxcd;  // block order of the batch launches follows the work list
   if (order_mode != b->order_mode) {
     b->order_mode = order_mode;
@@ -2531,6 +2546,14 @@ extern "C" int mjw_batch_config(mjw_batch *b, int nxcd, int order_mode) {
 
 static int batch_enqueue_step(mjw_batch *b, int parity) {
   for (int c = 0; c < b->ncolor; ++c) {
+    if (b->order_mode >= 2) {
+      for (const auto &sg : b->segs[c]) {
+        int rc = launch_batch3d_order(b->state, b->geom, sg[0], b->order_mode - 2, b->d_probs,
+                                      b->d_items[c] + sg[1], sg[2], parity, b->stream);
+        if (rc) return rc;
+      }
+      continue;
+    }
     int rc = launch_batch3d(b->state, b->geom, b->d_probs, b->d_items[c], b->nitems[c], parity,
                             b->stream);
     if (rc) return rc;

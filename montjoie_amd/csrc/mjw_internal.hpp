@@ -243,6 +243,11 @@ struct BatchItem {          // one block of work: elements [eb, min(eb+EPB, e1))
 int batch_epb(int order);
 int launch_batch3d(int state, int geom, const BatchProb *probs, const BatchItem *items,
                    int nitems, int parity, void *stream);
+// grouped launches: one per (colour, order), tile 0 / 1 / 2 = blocks of about
+// 256 / 512 / 128 threads
+int batch_group_epb(int order, int tile);
+int launch_batch3d_order(int state, int geom, int order, int tile, const BatchProb *probs,
+                         const BatchItem *items, int nitems, int parity, void *stream);
 // one tail block per problem: args[blockIdx.x]
 int launch_tail_batch(int state, const TailArgs *args, int nprob, void *stream);
 

@@ -717,6 +717,87 @@ int launch_batch3d(int state, int geom, const BatchProb *probs, const BatchItem 
   return (int)hipGetLastError();
 }
 
+// Grouped batch launches: one launch per (colour, element order) over that
+// order's work items, with the block size of the tile (about 128, 256 or 512
+// threads) and LDS sized for that order alone, so small orders are not held to
+// the largest order's LDS and registers.
+template <int N, int EPB, typename T, typename TG>
+__global__ __launch_bounds__(EPB *N *N) void k_batch3d_n(const BatchProb *__restrict__ probs,
+                                                          const BatchItem *__restrict__ items,
+                                                          int parity) {
+  __shared__ T lu[EPB * N * N * N];
+  __shared__ T lv[EPB * N * N * N];
+  __shared__ TG lc[EPB * kCoef];
+  const BatchItem it = items[blockIdx.x];
+  const BatchProb &p = probs[it.prob];
+  const T *U = static_cast<const T *>(p.buf[parity ^ 1]);
+  T *Y = static_cast<T *>(p.buf[parity]);
+  const TG *G = static_cast<const TG *>(p.G);
+  const T *D = static_cast<const T *>(p.Dm);
+  const T *s = static_cast<const T *>(p.sdiag);
+  const TG *cf = static_cast<const TG *>(p.coef);
+  const TG *gw = static_cast<const TG *>(p.glxw);
+  if (p.geo == kGeoTrilinear)
+    scalar3d_body<N, EPB, T, TG, kLeap, false, false, true>(lu, lv, nullptr, lc, p.map, G, D, p.E,
+                                                            it.eb, it.e1, U, Y, s, T(0), T(0),
+                                                            p.N, 1, nullptr, nullptr, cf, gw);
+  else
+    scalar3d_body<N, EPB, T, TG, kLeap, false, false, false>(lu, lv, nullptr, lc, p.map, G, D,
+                                                             p.E, it.eb, it.e1, U, Y, s, T(0),
+                                                             T(0), p.N, 1, nullptr, nullptr, cf,
+                                                             gw);
+}
+
+// elements per block of a grouped batch tile: 0 -> 256 threads, 1 -> 512, 2 -> 128
+constexpr int batch_tile_epb(int n, int tile) {
+  return (tile == 1 ? 512 : tile == 2 ? 128 : 256) / (n * n) > 0
+             ? (tile == 1 ? 512 : tile == 2 ? 128 : 256) / (n * n)
+             : 1;
+}
+
+int batch_group_epb(int order, int tile) { return batch_tile_epb(order + 1, tile); }
+
+template <int N, int TILE, typename T, typename TG>
+static void launch_batch_n(const BatchProb *probs, const BatchItem *items, int nitems, int parity,
+                           hipStream_t st) {
+  constexpr int EPB = batch_tile_epb(N, TILE);
+  hipLaunchKernelGGL((k_batch3d_n<N, EPB, T, TG>), dim3(nitems), dim3(EPB * N * N), 0, st, probs,
+                     items, parity);
+}
+
+template <int N, typename T, typename TG>
+static void launch_batch_t(int tile, const BatchProb *probs, const BatchItem *items, int nitems,
+                           int parity, hipStream_t st) {
+  if (tile == 1) launch_batch_n<N, 1, T, TG>(probs, items, nitems, parity, st);
+  else if (tile == 2) launch_batch_n<N, 2, T, TG>(probs, items, nitems, parity, st);
+  else launch_batch_n<N, 0, T, TG>(probs, items, nitems, parity, st);
+}
+
+template <int N>
+static void launch_batch_g(int state, int geom, int tile, const BatchProb *probs,
+                           const BatchItem *items, int nitems, int parity, hipStream_t st) {
+  if (state == kF64) launch_batch_t<N, double, double>(tile, probs, items, nitems, parity, st);
+  else if (geom == kG64) launch_batch_t<N, float, double>(tile, probs, items, nitems, parity, st);
+  else launch_batch_t<N, float, float>(tile, probs, items, nitems, parity, st);
+}
+
+int launch_batch3d_order(int state, int geom, int order, int tile, const BatchProb *probs,
+                         const BatchItem *items, int nitems, int parity, void *stream) {
+  if (nitems <= 0) return 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (order + 1) {
+    case 2: launch_batch_g<2>(state, geom, tile, probs, items, nitems, parity, st); break;
+    case 3: launch_batch_g<3>(state, geom, tile, probs, items, nitems, parity, st); break;
+    case 4: launch_batch_g<4>(state, geom, tile, probs, items, nitems, parity, st); break;
+    case 5: launch_batch_g<5>(state, geom, tile, probs, items, nitems, parity, st); break;
+    case 6: launch_batch_g<6>(state, geom, tile, probs, items, nitems, parity, st); break;
+    case 7: launch_batch_g<7>(state, geom, tile, probs, items, nitems, parity, st); break;
+    case 8: launch_batch_g<8>(state, geom, tile, probs, items, nitems, parity, st); break;
+    default: return -1000;
+  }
+  return (int)hipGetLastError();
+}
+
 // --------------------------------------------------------------- dispatch
 template <int N, int EPB, typename T, typename TG, int MODE, bool HOIST, bool GRAD, bool TRI>
 static void launch3(const StiffArgs &a, hipStream_t st, unsigned blocks) {

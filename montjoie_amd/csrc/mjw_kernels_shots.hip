@@ -403,17 +403,19 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
     mu = (A)mu_e[e];
   }
   const rsrc_t ru = mkres(U), ry = mkres(Y), rs = mkres(sdiag);
-  auto slot = [&](int m, int32_t g) {
-    return (((uint32_t)m * (uint32_t)N_nodes + (uint32_t)(g & 0x7fffffff)) * (uint32_t)nshots +
-            (uint32_t)s) * (uint32_t)sizeof(T);
+  // byte offset of node g's shot s in component 0; component m adds the
+  // scalar m * cs (the buffer's SGPR offset)
+  auto slot = [&](int32_t g) {
+    return ((uint32_t)(g & 0x7fffffff) * (uint32_t)nshots + (uint32_t)s) * (uint32_t)sizeof(T);
   };
+  const uint32_t cs = (uint32_t)N_nodes * (uint32_t)nshots * (uint32_t)sizeof(T);
   T u[3][N];
   if (sact) {
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        u[m][k] = bld<T>(ru, slot(m, gid[k]), 0u);
+        u[m][k] = bld<T>(ru, slot(gid[k]), (uint32_t)m * cs);
         lds[m * PL + (k * P + ij) * ST + lane] = u[m][k];
       }
   }
@@ -480,7 +482,7 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
 #pragma unroll
     for (int k = 0; k < N; ++k) {
 #pragma unroll
-      for (int m = 0; m < 3; ++m) yk[m][k] = bld<T>(ry, slot(m, gid[k]), 0u);
+      for (int m = 0; m < 3; ++m) yk[m][k] = bld<T>(ry, slot(gid[k]), (uint32_t)m * cs);
       if constexpr (MODE == kLeap)
         sk[k] = bld<T>(rs, (uint32_t)(gid[k] & 0x7fffffff) * (uint32_t)sizeof(T), 0u);
     }
@@ -508,14 +510,14 @@ __global__ __launch_bounds__(N *N *ST) void k_shots_el(
           o += Dtj[l] * lds[(3 + m) * PL + (k * P + l * N + i) * ST + lane];
           o += Dm[l * N + k] * v2[m][l];
         }
-        const uint32_t off = slot(m, mm);
+        const uint32_t off = slot(mm);
         const T y = yk[m][k];
         T r;
         if constexpr (MODE == kLeap)
           r = first ? (T(2) * u[m][k] - y - sgd * o) : (y - sgd * o);
         else
           r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
-        bst<T>(r, ry, off, 0u);
+        bst<T>(r, ry, off, (uint32_t)m * cs);
       }
     }
   }

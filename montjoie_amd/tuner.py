@@ -56,7 +56,7 @@ def tune_problem(P, steps: int = 8, dt: float | None = None, use_cache: bool = T
 
 def tune_batch(B, steps: int = 6, dt: float | None = None) -> dict:
     results = []
-    for mode in (0, 1):
+    for mode in (0, 1, 2, 3, 4):
         B.config(mode)
         if dt is not None:
             for P in B.problems:

@@ -42,9 +42,16 @@ def test_batch_equals_individual_runs(prec):
         alone.append(P.advance(40, graph=False)[:, 0, :])
         P.leapfrog_init(dt)
     B = Batch([P for P, _ in probs])
-    B.advance(40, graph=True)
-    for i, (P, dt) in enumerate(probs):
-        assert np.array_equal(B.traces(i), alone[i]), i
+    # every work order: one launch per colour (0, 1) or per (colour, order)
+    # with blocks of about 256, 512, 128 threads (2, 3, 4)
+    for mode in (0, 1, 2, 3, 4):
+        print(f"batch order_mode {mode}", flush=True)
+        B.config(mode)
+        for P, dt in probs:
+            P.leapfrog_init(dt)
+        B.advance(40, graph=True)
+        for i, (P, dt) in enumerate(probs):
+            assert np.array_equal(B.traces(i), alone[i]), (mode, i)
     B.close()
 
 
@@ -112,5 +119,5 @@ def test_tuners_are_result_neutral():
     B.advance(20)
     for i in range(3):
         assert np.array_equal(a[i], B.traces(i))
-    assert res["order_mode"] in (0, 1)
+    assert res["order_mode"] in (0, 1, 2, 3, 4)
     B.close()
