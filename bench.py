@@ -181,9 +181,9 @@ def bench_shots(m, mu, dt, steps, warmup, nshots=16, acc_steps=500):
         if key != "fp64" and steps != acc_steps:
             P.leapfrog_init(dt)
             tr = P.advance(acc_steps, traces=True)
+        uN = P.state()[0] if key != "fp64" else None  # before mjw_profile advances it
         prof = P.profile(4) if key == "fp32_g64" else None
-        out[key] = dict(wall=wall, shots=ns, dev_ms=dev_ms, traces=tr, prof=prof,
-                        uN=P.state()[0] if key != "fp64" else None)
+        out[key] = dict(wall=wall, shots=ns, dev_ms=dev_ms, traces=tr, prof=prof, uN=uN)
         P.close()
     t64_batch = out["fp64"]["wall"] * nshots           # FP64 path, one shot at a time
     # stored-geometry bytes per step: FP64 tensor and map per element node, three
