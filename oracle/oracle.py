@@ -822,87 +822,135 @@ class PmlMesh:
 
 
 def pml_mesh(dim: int, nx: int, r: int, pert: float, seed: int, thickness: float,
-             layers: int) -> PmlMesh:
+             layers: int, sides=None) -> PmlMesh:
     """The structured mesh of [0,1]^dim (vertices perturbed like mesh())
-    extended by `layers` layers of elements of width thickness/layers on
-    every side, the way AddPML = YES XY(Z) thickness layers extrudes a box
-    mesh (PmlRegionParameter::AddPMLElements, Mesh/Mesh2DBoundaries.cxx:
-    104-192, Mesh3DBoundaries.cxx:59-; Mesh<>::ExtrudeCoordinate, Mesh/
-    Mesh2D.cxx:4898-4990: new vertex coordinate pos + (i+1) delta / layers,
-    the other coordinates copied from the boundary vertex; axes in the order
-    x, y, z).  A PML element takes the medium of the physical element it was
-    extruded from (its indices clamped into the physical block) and the
-    layer kind of its centre relative to the box [0,1]^dim."""
+    extended by `layers` layers of elements of width thickness/layers beyond
+    the box faces of `sides` (per axis (below, above), default all), the way
+    AddPML = YES <axes> thickness layers extrudes a box mesh
+    (PmlRegionParameter::AddPMLElements, Mesh/Mesh2DBoundaries.cxx:104-192,
+    Mesh3DBoundaries.cxx:59-; Mesh<>::ExtrudeCoordinate, Mesh/Mesh2D.cxx:
+    4898-4990: new vertex coordinate pos + (i+1) delta / layers, the other
+    coordinates copied from the boundary vertex; axes in the order x, y, z).
+    A PML element takes the medium of the physical element it was extruded
+    from (its indices clamped into the physical block) and the layer kind of
+    its centre relative to the box [0,1]^dim."""
     L = lib()
+    if sides is None:
+        sides = [(True, True)] * dim
+    lo = [layers if sides[a][0] else 0 for a in range(dim)]
+    hi = [layers if sides[a][1] else 0 for a in range(dim)]
     nv1 = nx + 1
     vphys = np.zeros(nv1 ** dim * dim)
     L.or_mesh_vertices(dim, nx, pert, seed, vphys)
     vphys = vphys.reshape(-1, dim)
-    nt = nx + 2 * layers
-    nt1 = nt + 1
+    nt = [nx + lo[a] + hi[a] for a in range(dim)]
+    nt1 = [n + 1 for n in nt]
+    if len(set(nt)) != 1:
+        raise ValueError("pml_mesh: the extended grid must stay cubic (or_mesh_nodes)")
+    ntc, ntc1 = nt[0], nt1[0]
 
-    def coord(i):
-        # layer coordinates of one axis, or None inside the physical range
-        if i < layers:
-            return 0.0 + ((layers - i) * (-thickness)) / layers
-        if i > layers + nx:
-            return 1.0 + ((i - layers - nx) * thickness) / layers
+    def coord(a, i):
+        # layer coordinates of axis a, or None inside the physical range
+        if i < lo[a]:
+            return 0.0 + ((lo[a] - i) * (-thickness)) / layers
+        if i > lo[a] + nx:
+            return 1.0 + ((i - lo[a] - nx) * thickness) / layers
         return None
 
-    vert = np.zeros((nt1 ** dim, dim))
-    for v in range(nt1 ** dim):
-        I = [(v // nt1 ** a) % nt1 for a in range(dim)]
-        inside = all(layers <= I[a] <= layers + nx for a in range(dim))
+    vert = np.zeros((ntc1 ** dim, dim))
+    for v in range(ntc1 ** dim):
+        I = [(v // ntc1 ** a) % ntc1 for a in range(dim)]
+        inside = all(lo[a] <= I[a] <= lo[a] + nx for a in range(dim))
         if inside:
-            pv = sum((I[a] - layers) * nv1 ** a for a in range(dim))
+            pv = sum((I[a] - lo[a]) * nv1 ** a for a in range(dim))
             vert[v] = vphys[pv]
             continue
         for a in range(dim):
-            c = coord(I[a])
+            c = coord(a, I[a])
             if c is None:
                 # copy of a boundary vertex of the box: regular coordinate
-                ia = I[a] - layers
+                ia = I[a] - lo[a]
                 c = 1.0 if ia == nx else ia * (1.0 / nx)
             vert[v, a] = c
     vert = np.ascontiguousarray(vert.reshape(-1))
-    N = (nt * r + 1) ** dim
-    E = nt ** dim
+    N = (ntc * r + 1) ** dim
+    E = ntc ** dim
     nd = (r + 1) ** dim
     xyz = np.zeros(N * dim)
     en = np.zeros(E * nd, dtype=np.int32)
     dr = np.zeros(N, dtype=np.uint8)
-    if L.or_mesh_nodes(dim, nt, r, vert, xyz, en, dr):
+    if L.or_mesh_nodes(dim, ntc, r, vert, xyz, en, dr):
         raise ValueError("mesh")
-    m = Mesh(dim, nt, r, pert, seed, vert, xyz.reshape(N, dim), en.reshape(E, nd), dr)
+    m = Mesh(dim, ntc, r, pert, seed, vert, xyz.reshape(N, dim), en.reshape(E, nd), dr)
     e = np.arange(E)
-    idx = np.stack([(e // nt ** a) % nt for a in range(dim)], axis=1)
-    kind = np.where(idx < layers, -1, np.where(idx >= layers + nx, 1, 0))
-    cl = np.clip(idx - layers, 0, nx - 1)
+    idx = np.stack([(e // ntc ** a) % ntc for a in range(dim)], axis=1)
+    lo_a, hi_a = np.array(lo), np.array(lo) + nx
+    kind = np.where(idx < lo_a, -1, np.where(idx >= hi_a, 1, 0))
+    cl = np.clip(idx - lo_a, 0, nx - 1)
     phys = sum(cl[:, a] * nx ** a for a in range(dim))
     pml = (kind != 0).any(axis=1).astype(np.uint8)
     return PmlMesh(m, nx, layers, thickness, pml, kind, phys)
 
 
-def pml_tau(pm: PmlMesh, vsigma: float, vmax: float) -> np.ndarray:
+PML_PARABOLE, PML_CONSTANT, PML_LINEAR, PML_SHIFTED_PARABOLE = 0, 1, 2, 3
+
+
+def pml_tau(pm: PmlMesh, vsigma: float, vmax: float, profile: int = PML_PARABOLE,
+            offset: float = 0.0, transverse: float = 0.0) -> np.ndarray:
     """Damping tau[e, q, a] at the quadrature points (GL nodes) of every
-    element: parabolic profile dsig (x_a - box_a)^2 in the layers beyond
+    element: the DampingPML profile of the distance dx = x_a - box_a beyond
     the box face of the element's kind, dsig = log(1000) 3 vsigma vmax /
     (2 a^3), a = thickness (VarBoundaryCondition_Base::GetDampingTauPML and
-    EvaluateFunctionTauPML (PML_PARABOLE), Harmonic/BoundaryConditionHarmonic.
-    cxx:1639-1700,1548-1570; first-order formulation keeps tau itself,
-    GetDampingFactorPML :1588-1636); vmax = max sqrt(mu / rho) over the
-    media of the PML elements (:1486-1522)."""
+    EvaluateFunctionTauPML, Harmonic/BoundaryConditionHarmonic.cxx:1548-1570,
+    1639-1760): parabola dsig dx^2, constant dsig a^2, linear dsig |dx| a,
+    shifted parabola dsig (dx^2 + offset a^2); TransverseDampingPML = c mixes
+    the axes (tau_a = max(tau_a, c max of the others)); the first-order
+    formulation keeps tau itself (GetDampingFactorPML :1588-1636); vmax =
+    max sqrt(mu / rho) over the media of the PML elements (:1486-1522)."""
     m = pm.mesh
     a = pm.thickness
     dsig = np.log(1000e0) * 3e0 * vsigma * vmax / (2 * a ** 3)
+
+    def f(dx):
+        if profile == PML_CONSTANT:
+            return np.full_like(dx, dsig * a * a)
+        if profile == PML_LINEAR:
+            return dsig * np.abs(dx) * a
+        if profile == PML_SHIFTED_PARABOLE:
+            return dsig * (dx * dx + offset * a * a)
+        return dsig * dx * dx
+
     X = m.xyz[m.elem_nodes]                       # (E, nd, dim)
     tau = np.zeros(X.shape)
     for ax in range(m.dim):
         k = pm.kind[:, ax][:, None]
-        dlo = X[:, :, ax] - 0.0
-        dhi = X[:, :, ax] - 1.0
-        tau[:, :, ax] = np.where(k == -1, dsig * dlo * dlo, np.where(k == 1, dsig * dhi * dhi, 0.0))
+        tau[:, :, ax] = np.where(k == -1, f(X[:, :, ax] - 0.0),
+                                 np.where(k == 1, f(X[:, :, ax] - 1.0), 0.0))
+    if transverse != 0:
+        c = transverse
+        z = tau.copy()
+        for ax in range(m.dim):
+            others = np.max(np.delete(z, ax, axis=2), axis=2)
+            tau[:, :, ax] = np.maximum(z[:, :, ax], c * others)
     return tau
+
+
+def pml_sides(axes: str, dim: int):
+    """Sides per axis of AddPML's axis string (PmlRegionParameter_Base::
+    SetParameters, Mesh/MeshBoundaries.cxx:44-70): 'X-' below only, 'X+'
+    above only, 'X' both."""
+    out = []
+    for a in range(dim):
+        c = "XYZ"[a]
+        if c not in axes:
+            out.append((False, False))
+        elif c + "-" in axes:
+            out.append((True, False))
+        elif c + "+" in axes:
+            out.append((False, True))
+        else:
+            out.append((True, True))
+    return out
 
 
 def leapfrog_pml(pm: PmlMesh, rho_e, mu_e, tau, b, table, dt, nsteps, rec_pts, sigma_e=None):

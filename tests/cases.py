@@ -34,7 +34,9 @@ class Case:
     times: np.ndarray | None = None
     traces: np.ndarray | None = None
     sigma_e: np.ndarray | None = None
-    pml: tuple | None = None       # (thickness, layers, vsigma): AddPML + LEAP_FROG_PML
+    # AddPML + LEAP_FROG_PML: (thickness, layers, vsigma, axes, profile, offset,
+    # transverse); layers = the count used (AUTO decks: the resolved count)
+    pml: tuple | None = None
 
 
 def golden_names():
@@ -87,7 +89,9 @@ def load(name: str) -> Case:
         sigma_e = np.where((lay == 0)[:, None], f, base[:, None])
     pml = None
     if "pml" in z.files and len(z["pml"]):
-        pml = (float(z["pml"][0]), int(z["pml"][1]), float(z["pml"][2]))
+        p = [float(v) for v in z["pml"]] + [0.0, 0.0, 0.0]
+        axes = str(z["pml_axes"]) if "pml_axes" in z.files else {2: "XY", 3: "XYZ"}[dim]
+        pml = (p[0], int(p[1]), p[2], axes, int(p[3]), p[4], p[5])
     return Case(name, dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]), elastic, mu_e,
                 rho_e, lam_e, float(z["f0"]), float(z["dt"]), int(z["nsteps"]), z["src"],
                 z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e, pml)
@@ -97,11 +101,22 @@ def pml_setup(c: Case):
     """(extended mesh, rho, mu per extended element, tau) of a PML case: the
     layers extruded as AddPML does, media of the element each was extruded
     from, parabolic damping with vmax over the PML media."""
-    thick, layers, vsigma = c.pml
-    pm = O.pml_mesh(c.dim, c.nx, c.r, c.pert, c.seed, thick, layers)
+    thick, layers, vsigma, axes, profile, offset, transverse = c.pml
+    pm = O.pml_mesh(c.dim, c.nx, c.r, c.pert, c.seed, thick, layers,
+                    O.pml_sides(axes, c.dim))
     mu, rho = c.mu_e[pm.phys], c.rho_e[pm.phys]
     vmax = float(np.sqrt(mu[pm.pml == 1] / rho[pm.pml == 1]).max())
-    return pm, rho, mu, O.pml_tau(pm, vsigma, vmax)
+    return pm, rho, mu, O.pml_tau(pm, vsigma, vmax, profile, offset, transverse)
+
+
+def pml_deck(c: Case):
+    """The deck-writer PML spec of a PML case (AUTO when the fixture's deck
+    let Montjoie count the layers)."""
+    z = np.load(os.path.join(GOLD, c.name + ".npz"))
+    thick, layers, vsigma, axes, profile, offset, transverse = c.pml
+    auto = "pml_auto" in z.files and int(z["pml_auto"]) == 1
+    return dict(axes=axes, thickness=thick, layers="AUTO" if auto else layers, vsigma=vsigma,
+                profile=profile, offset=offset, transverse=transverse)
 
 
 def sinus_field(p, pts):

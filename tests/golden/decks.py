@@ -89,15 +89,26 @@ def write_don(path, nu):
         fh.write(nu.tobytes())
 
 
+def pml_spec(pml, dim):
+    """PML parameters as a dict (axes, thickness, layers int or "AUTO",
+    vsigma, profile 0 parabola / 1 constant / 2 linear / 3 shifted parabola,
+    offset, transverse); a tuple (ignored, thickness, layers, vsigma) means
+    every side with the parabolic profile."""
+    if isinstance(pml, dict):
+        return pml
+    return dict(axes={2: "XY", 3: "XYZ"}[dim], thickness=float(pml[1]), layers=int(pml[2]),
+                vsigma=float(pml[3]), profile=0, offset=0.0, transverse=0.0)
+
+
 def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
                mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
                output_format="BINARY DOUBLE", circle=None, pml=None):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  circle = (x0, y0,
-    radius, n): a SismoCircle line after the SismoPoint lines.  pml = (axes,
-    thickness, layers, vsigma): AddPML = YES axes thickness layers, DampingPML =
-    vsigma and the first-order scheme OrderTimeScheme = 2 LEAP_FROG_PML.  sinus_rho /
+    radius, n): a SismoCircle line after the SismoPoint lines.  pml (pml_spec):
+    AddPML = YES axes thickness layers, DampingPML (and TransverseDampingPML)
+    and the first-order scheme OrderTimeScheme = 2 LEAP_FROG_PML.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1.
     save_reprise > 0: checkpoint files every save_reprise steps.  sigma_layers:
     acoustic damping of each layer (MateriauDielec's third value)."""
@@ -127,9 +138,18 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     bref = 1 if regular else nl + 1
     ini.append(f"ConditionReference = {bref} DIRICHLET")
     if pml is not None and len(pml):
-        axes = {2: "XY", 3: "XYZ"}[dim] if isinstance(pml[0], (int, float, np.floating)) else pml[0]
-        ini.append(f"AddPML = YES {axes} {_f(pml[1])} {int(pml[2])}")
-        ini.append(f"DampingPML = {_f(pml[3])}")
+        spec = pml_spec(pml, dim)
+        lay = spec["layers"] if spec["layers"] == "AUTO" else int(spec["layers"])
+        ini.append(f"AddPML = YES {spec['axes']} {_f(spec['thickness'])} {lay}")
+        prof, vs = spec.get("profile", 0), _f(spec["vsigma"])
+        if prof == 0:
+            ini.append(f"DampingPML = {vs}")
+        elif prof == 3:
+            ini.append(f"DampingPML = SHIFTED_PARABOLE {vs} {_f(spec['offset'])}")
+        else:
+            ini.append(f"DampingPML = {'CONSTANT' if prof == 1 else 'LINEAR'} {vs}")
+        if spec.get("transverse"):
+            ini.append(f"TransverseDampingPML = {_f(spec['transverse'])}")
     for k in range(nl):
         if elastic:
             ini.append(f"MateriauDielec = {k + 1} ISOTROPE {_f(rho_layers[k])} "

@@ -25,11 +25,26 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 from oracle import oracle as O  # noqa: E402
-from tests.golden.decks import write_deck  # noqa: E402
+from tests.golden.decks import pml_spec, write_deck  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
 RUN = os.path.join(REF, "run")
 GOLD = os.path.dirname(os.path.abspath(__file__))
+
+
+def _pml_fields(pml, dim, nx):
+    """npz fields of a PML case: pml = [thickness, layers, vsigma, profile,
+    offset, transverse] (layers = the count used: AUTO decks get Montjoie's
+    ceil(thickness / h), h = 1/nx the boundary edge length of the structured
+    mesh, thicknesses chosen away from multiples of h), pml_axes, pml_auto."""
+    if pml is None:
+        return dict(pml=np.zeros(0))
+    sp = pml_spec(pml, dim)
+    auto = sp["layers"] == "AUTO"
+    lay = max(1, int(np.ceil(sp["thickness"] * nx))) if auto else int(sp["layers"])
+    return dict(pml=np.array([sp["thickness"], lay, sp["vsigma"], sp.get("profile", 0),
+                              sp.get("offset", 0.0), sp.get("transverse", 0.0)], float),
+                pml_axes=np.array(sp["axes"]), pml_auto=np.array(int(auto)))
 
 
 def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=False,
@@ -108,7 +123,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         sinus_sigma=np.asarray(sinus_sigma if sinus_sigma is not None else [], float),
         output_format=output_format,
         circle=np.asarray(circle if circle is not None else [], float),
-        pml=np.asarray(pml[1:] if pml is not None else [], float),
+        **_pml_fields(pml, dim, nx),
         circle_files=circles,
         circle_names=names.get("totalCircle", np.zeros(0, "U1")),
         plane_names=names.get("totalPlane", np.zeros(0, "U1")),
@@ -191,6 +206,42 @@ def ref3d_q3_pml():
     rec = np.vstack([seeded_points(3, 4, 93), [[1.2, -0.15, 0.5], [-0.3, 1.3, 1.2]]])
     run_case("ref3d_q3_pml", 3, 4, 3, 0.1, 47, [1.0, 2.5], 8.0, 4e-3, 150, (0.46, 0.53, 0.49),
              rec, pml=(3, 0.5, 2, 2.0))
+
+
+@case
+def ref2d_q3_onesided_pml():
+    """6x6 Q3 perturbed quadrilaterals, two layers; PML only beyond x = 1 and
+    below y = 0 (AddPML = YES X+Y- 0.3 AUTO: ceil(0.3 / (1/6)) = 2 layers),
+    linear damping profile (DampingPML = LINEAR 2.5); receivers in both
+    layers, in their corner and near the undamped Dirichlet sides."""
+    rec = np.array([[0.35, 0.45], [0.7, 0.6], [1.15, 0.5], [0.5, -0.2], [1.2, -0.2],
+                    [0.05, 0.9]])
+    run_case("ref2d_q3_onesided_pml", 2, 6, 3, 0.1, 48, [1.0, 2.0], 10.0, 1e-3, 400,
+             (0.46, 0.53), rec,
+             pml=dict(axes="X+Y-", thickness=0.3, layers="AUTO", vsigma=2.5, profile=2))
+
+
+@case
+def ref2d_q4_shifted_pml():
+    """8x8 Q4 perturbed quadrilaterals, two layers, PML of 3 layers of 0.1 on
+    every side with the shifted parabola (DampingPML = SHIFTED_PARABOLE 2.0
+    0.2) and transverse damping (TransverseDampingPML = 0.5)."""
+    rec = np.array([[0.3, 0.4], [0.71, 0.52], [1.2, 0.5], [-0.2, -0.2], [0.5, 1.25]])
+    run_case("ref2d_q4_shifted_pml", 2, 8, 4, 0.1, 49, [1.0, 2.0], 10.0, 1e-3, 400,
+             (0.47, 0.52), rec,
+             pml=dict(axes="XY", thickness=0.3, layers=3, vsigma=2.0, profile=3, offset=0.2,
+                      transverse=0.5))
+
+
+@case
+def ref3d_q2_onesided_pml():
+    """4^3 Q2 perturbed hexahedra, two layers; PML below x = 0, above y = 1
+    and below z = 0 (AddPML = YES X-Y+Z- 0.5 2), constant damping
+    (DampingPML = CONSTANT 1.5)."""
+    rec = np.vstack([seeded_points(3, 3, 94), [[-0.3, 0.5, 0.5], [0.5, 1.3, -0.2]]])
+    run_case("ref3d_q2_onesided_pml", 3, 4, 2, 0.1, 50, [1.0, 2.5], 8.0, 4e-3, 150,
+             (0.46, 0.53, 0.49), rec,
+             pml=dict(axes="X-Y+Z-", thickness=0.5, layers=2, vsigma=1.5, profile=1))
 
 
 @case

@@ -42,7 +42,7 @@ def _write(c: CS.Case, d):
                      output_format=str(z["output_format"]) if "output_format" in z.files
                      else "BINARY DOUBLE",
                      circle=z["circle"] if "circle" in z.files and len(z["circle"]) else None,
-                     pml=(c.dim,) + tuple(c.pml) if c.pml is not None else None)
+                     pml=CS.pml_deck(c) if c.pml is not None else None)
 
 
 def _dump(path):
