@@ -12,6 +12,7 @@
 #include <map>
 #include <mutex>
 #include <numeric>
+#include <unordered_set>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -472,9 +473,18 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
     }
     for (int64_t g = 0; g < n_nodes; ++g)
       if (!seen[g]) by_rank.push_back((int32_t)g);  // nodes of no element
-    std::vector<uint64_t> masks(mask);
-    std::sort(masks.begin(), masks.end());
-    masks.erase(std::unique(masks.begin(), masks.end()), masks.end());
+    // the distinct colour sets (a few dozen), sorted; one pass over the nodes
+    std::vector<uint64_t> masks;
+    {
+      std::unordered_set<uint64_t> distinct;
+      uint64_t last = ~0ull;
+      for (uint64_t m : mask)
+        if (m != last) {
+          last = m;
+          if (distinct.insert(m).second) masks.push_back(m);
+        }
+      std::sort(masks.begin(), masks.end());
+    }
     std::vector<int64_t> off(masks.size() + 1, 0);
     std::vector<int32_t> cid(n_nodes);
     for (int64_t g = 0; g < n_nodes; ++g) {

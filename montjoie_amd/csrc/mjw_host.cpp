@@ -324,7 +324,14 @@ bool trilinear_coefficients(int order, int64_t E, const int32_t *elem_nodes, con
   double xg[kMaxN], wg[kMaxN];
   gl_rule(order, xg, wg);
   coef.assign((size_t)E * kCoef, 0.0);
+  // elements are independent: fitted in parallel, the verdict shared
+  int bad = 0;
+#pragma omp parallel for schedule(static)
   for (int64_t es = 0; es < E; ++es) {
+    int failed;
+#pragma omp atomic read
+    failed = bad;
+    if (failed) continue;
     const int32_t *en = elem_nodes + perm[es] * nd;
     auto X = [&](int i, int j, int k, int d) {
       return xyz[(int64_t)en[i + n * j + n * n * k] * 3 + d];
@@ -346,23 +353,29 @@ bool trilinear_coefficients(int order, int64_t E, const int32_t *elem_nodes, con
       diam2 += (x111 - x000) * (x111 - x000);
     }
     const double lim = tol * std::sqrt(diam2);
-    for (int k = 0; k < n; ++k)
-      for (int j = 0; j < n; ++j)
-        for (int i = 0; i < n; ++i) {
+    bool fits = true;
+    for (int k = 0; k < n && fits; ++k)
+      for (int j = 0; j < n && fits; ++j)
+        for (int i = 0; i < n && fits; ++i) {
           const double u = xg[i], v = xg[j], w = xg[k];
           for (int d = 0; d < 3; ++d) {
             const double x = a[0][d] + a[1][d] * u + a[2][d] * v + a[3][d] * w +
                              a[4][d] * u * v + a[5][d] * v * w + a[6][d] * u * w +
                              a[7][d] * u * v * w;
-            if (!(std::fabs(x - X(i, j, k, d)) <= lim)) return false;
+            if (!(std::fabs(x - X(i, j, k, d)) <= lim)) fits = false;
           }
         }
+    if (!fits) {
+#pragma omp atomic write
+      bad = 1;
+      continue;
+    }
     double *c = &coef[(size_t)es * kCoef];
     for (int m = 1; m < 8; ++m)
       for (int d = 0; d < 3; ++d) c[(m - 1) * 3 + d] = a[m][d];
     c[21] = mu_sorted[es];
   }
-  return true;
+  return !bad;
 }
 
 // ---------------------------------------------------------------------------
