@@ -182,7 +182,7 @@ def bench_shots(m, mu, dt, steps, warmup, nshots=16, acc_steps=500):
             P.leapfrog_init(dt)
             tr = P.advance(acc_steps, traces=True)
         uN = P.state()[0] if key != "fp64" else None  # before mjw_profile advances it
-        prof = P.profile(4) if key == "fp32_g64" else None
+        prof = P.profile(4) if key in ("fp32_g64", "fp32") else None
         out[key] = dict(wall=wall, shots=ns, dev_ms=dev_ms, traces=tr, prof=prof, uN=uN)
         P.close()
     t64_batch = out["fp64"]["wall"] * nshots           # FP64 path, one shot at a time
@@ -190,6 +190,7 @@ def bench_shots(m, mu, dt, steps, warmup, nshots=16, acc_steps=500):
     # FP32 state streams per node and shot, FP32 dt^2/M per node
     sb = m.E * m.nd * (6 * 8 + 4) + m.N * (3 * nshots * 4 + 4)
     g64_ms = out["fp32_g64"]["prof"]["stiffness_launch_ms"]
+    f32_ms = out["fp32"]["prof"]["stiffness_launch_ms"]
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
     ref = out["fp64_batch"]
     return {
@@ -217,6 +218,11 @@ def bench_shots(m, mu, dt, steps, warmup, nshots=16, acc_steps=500):
                      "kernel": "k_shots3d, FP32 state, FP64 geometry, one colour pass of "
                                f"{nshots} interleaved shots",
                      "algorithmic_bytes_per_launch": sb / 8, "launch_ms": g64_ms},
+        "roofline_fp32": {"bound": "hbm", "achieved": sb / 8 / (f32_ms * 1e-3) / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": sb / 8 / (f32_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "kernel": "k_shots3d, FP32 state and geometry, one colour pass",
+                          "launch_ms": f32_ms},
     }
 
 
@@ -436,6 +442,15 @@ def elastic_bytes_per_step(E, nd, N, nshots, state_bytes, geom_bytes=8):
     return E * nd * (10 * geom_bytes + 4) + N * (3 * 3 * nshots * state_bytes + 8)
 
 
+def elastic_flops_per_step(E, n, nshots):
+    """Algorithmic FLOPs per step of the elastic pass: per element and shot,
+    three gradients and three divergences of 3 directions (2 n flops per node
+    each) and, per node, M^T g (54), the isotropic stress (about 30) and M
+    sigma (54)."""
+    nd = n ** 3
+    return E * nshots * nd * (2 * 2 * 9 * n + 54 + 30 + 54)
+
+
 def bench_elastic(nx=32, order=4, steps=800, nshots=16, warmup=10):
     """Isotropic elastodynamics, Q4 hexahedra nx^3 (vertices perturbed <= 0.1h,
     seed 2024), 3 layers lambda, mu ~ U[1,4], rho ~ U[1,2] (seed 2026);
@@ -470,7 +485,7 @@ def bench_elastic(nx=32, order=4, steps=800, nshots=16, warmup=10):
         e1 = P.energy()
         P.advance(100, traces=False)
         e2 = P.energy()
-        prof = P.profile(4) if key == "fp32_g64" else None
+        prof = P.profile(4) if key in ("fp32_g64", "fp32") else None
         res[key] = dict(wall=wall, traces=tr, uN=uN, e1=e1, e2=e2, prof=prof, tile=tuned["tile"],
                         ncolors=P.info()["ncolors"])
         P.close()
@@ -482,6 +497,8 @@ def bench_elastic(nx=32, order=4, steps=800, nshots=16, warmup=10):
     launch_ms = head["prof"]["stiffness_launch_ms"]
     launch_bytes = elastic_bytes_per_step(m.E, m.nd, m.N, nshots, 4) / head["ncolors"]
     achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
+    f32_ms = res["fp32"]["prof"]["stiffness_launch_ms"]
+    launch_flops = elastic_flops_per_step(m.E, order + 1, nshots) / head["ncolors"]
     return {
         "workload": f"elastic Q{order} {nx}^3, 3 layers, {nshots} shots, {steps} steps",
         "n_dof": ndof, "dt": dt,
@@ -510,7 +527,19 @@ def bench_elastic(nx=32, order=4, steps=800, nshots=16, warmup=10):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": f"k_shots_el (tile {head['tile']}, one colour pass, FP32 state, "
                                "FP64 geometry)",
-                     "algorithmic_bytes_per_launch": launch_bytes, "launch_ms": launch_ms},
+                     "algorithmic_bytes_per_launch": launch_bytes, "launch_ms": launch_ms,
+                     "flops_per_launch": launch_flops,
+                     "achieved_TFLOPs": launch_flops / (launch_ms * 1e-3) / 1e12},
+        "roofline_fp32": {"bound": "hbm", "achieved": launch_bytes / (f32_ms * 1e-3) / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": launch_bytes / (f32_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "kernel": f"k_shots_el (tile {res['fp32']['tile']}, one colour pass, "
+                                    "FP32 state and geometry: the workload's stated precision)",
+                          "launch_ms": f32_ms,
+                          "achieved_TFLOPs": launch_flops / (f32_ms * 1e-3) / 1e12,
+                          "peak_TFLOPs_fp32_non_packed": 78.6,
+                          "note": "packed FP32 instructions are disabled in this build, so the "
+                                  "vector FP32 rate is half the 157.3 TFLOP/s packed peak"},
     }
 
 
