@@ -83,6 +83,7 @@ struct mjw_problem {
   void *d_U[2] = {nullptr, nullptr};
   int cur = 0;
   void *d_tmp = nullptr, *d_tmp2 = nullptr;
+  double *d_stage = nullptr;          // host-vector staging (one FP64 vector)
   double *d_part = nullptr, *d_red = nullptr;
   // leapfrog
   double dt = 0.0, t_begin = 0.0;
@@ -274,6 +275,7 @@ static void release(mjw_problem *p) {
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   dfree(p->d_coef);
   dfree(p->d_glxw);
+  dfree(p->d_stage);
   dfree(p->d_nperm);
   dfree(p->d_ninv);
   void *ptrs[] = {p->d_map, p->d_G, p->d_M, p->d_Minv, p->d_isqM, p->d_sdiag, p->d_dir,
@@ -818,51 +820,44 @@ extern "C" int mjw_node_order(const mjw_problem *p, int32_t *solver_index) {
 // host FP64 vectors in the caller's layout [Sv][ncomp][N] and numbering <->
 // device vectors in the solver's (state precision; Sv > 1 on a problem with
 // interleaved shots uses the interleaved layout)
-static int64_t solver_slot(const mjw_problem *p, int Sv, int64_t s, int64_t m, int64_t i) {
-  const int64_t per = (int64_t)p->ncomp * p->N, mi = m * p->N + i;
-  return (p->ilv && Sv > 1) ? mi * Sv + s : s * per + mi;
+// FP64 device buffer of one vector ([ncomp][N], caller's numbering) through
+// which host vectors cross the boundary; allocated on first use
+static double *stage_buf(mjw_problem *p) {
+  if (!p->d_stage &&
+      dalloc(p, &p->d_stage, (size_t)p->ncomp * p->N * sizeof(double)) != hipSuccess)
+    p->d_stage = nullptr;
+  return p->d_stage;
 }
 
+// host vectors [Sv][ncomp][N] in the caller's numbering (FP64) <-> device
+// vectors in the solver layout and precision; renumbered on the device, one
+// shot at a time
 static int upload_vec(mjw_problem *p, const double *h, void *dst, int Sv) {
-  const int64_t per = (int64_t)p->ncomp * p->N, n = per * Sv;
-  std::vector<double> hs((size_t)n);
-  for (int64_t s = 0; s < Sv; ++s)
-    for (int64_t m = 0; m < p->ncomp; ++m)
-      for (int64_t g = 0; g < p->N; ++g)
-        hs[solver_slot(p, Sv, s, m, p->nperm[g])] = h[s * per + m * p->N + g];
-  int rc = (int)hipStreamSynchronize(p->stream);
-  if (rc) return rc;
-  if (p->state == kF64)
-    return (int)hipMemcpy(dst, hs.data(), n * 8, hipMemcpyHostToDevice);
-  double *tmp = nullptr;
-  if (hipMalloc(&tmp, n * 8) != hipSuccess) return -1;
-  rc = (int)hipMemcpyAsync(tmp, hs.data(), n * 8, hipMemcpyHostToDevice, p->stream);
-  if (!rc) rc = launch_convert_from_f64(p->state, n, tmp, dst, p->stream);
-  if (!rc) rc = (int)hipStreamSynchronize(p->stream);
-  dfree(tmp);
-  return rc;
+  const int64_t per = (int64_t)p->ncomp * p->N;
+  double *stage = stage_buf(p);
+  if (!stage) return -1;
+  const int ilv = (p->ilv && Sv > 1) ? 1 : 0;
+  for (int s = 0; s < Sv; ++s) {
+    int rc = (int)hipMemcpyAsync(stage, h + s * per, per * 8, hipMemcpyHostToDevice, p->stream);
+    if (!rc) rc = launch_to_solver_f64(p->state, Sv, s, p->ncomp, p->N, ilv, p->d_ninv, stage, dst,
+                                       p->stream);
+    if (rc) return rc;
+  }
+  return (int)hipStreamSynchronize(p->stream);
 }
 
 static int download_vec(mjw_problem *p, const void *src, double *h, int Sv) {
-  const int64_t per = (int64_t)p->ncomp * p->N, n = per * Sv;
-  std::vector<double> hs((size_t)n);
-  int rc = 0;
-  if (p->state == kF64) {
-    rc = (int)hipMemcpyAsync(hs.data(), src, n * 8, hipMemcpyDeviceToHost, p->stream);
+  const int64_t per = (int64_t)p->ncomp * p->N;
+  double *stage = stage_buf(p);
+  if (!stage) return -1;
+  const int ilv = (p->ilv && Sv > 1) ? 1 : 0;
+  for (int s = 0; s < Sv; ++s) {
+    int rc = launch_from_solver_f64(p->state, Sv, s, p->ncomp, p->N, ilv, p->d_nperm, src, stage,
+                                    p->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(h + s * per, stage, per * 8, hipMemcpyDeviceToHost, p->stream);
     if (!rc) rc = (int)hipStreamSynchronize(p->stream);
-  } else {
-    double *tmp = nullptr;
-    if (hipMalloc(&tmp, n * 8) != hipSuccess) return -1;
-    rc = launch_convert_to_f64(p->state, n, src, tmp, p->stream);
-    if (!rc) rc = (int)hipMemcpyAsync(hs.data(), tmp, n * 8, hipMemcpyDeviceToHost, p->stream);
-    if (!rc) rc = (int)hipStreamSynchronize(p->stream);
-    dfree(tmp);
+    if (rc) return rc;
   }
-  if (rc) return rc;
-  for (int64_t s = 0; s < Sv; ++s)
-    for (int64_t m = 0; m < p->ncomp; ++m)
-      for (int64_t g = 0; g < p->N; ++g)
-        h[s * per + m * p->N + g] = hs[solver_slot(p, Sv, s, m, p->nperm[g])];
   return 0;
 }
 

@@ -181,6 +181,12 @@ int launch_to_solver(int state, int S, int ncomp, int64_t N, int ilv, const int3
                      const void *x, void *y, void *stream);
 int launch_from_solver(int state, int S, int ncomp, int64_t N, int ilv, const int32_t *nperm,
                        const void *x, void *y, void *stream);
+// one FP64 vector in the caller's numbering ([ncomp][N]) <-> shot s of the
+// solver layout (S shots, interleaved when ilv) in the state precision
+int launch_to_solver_f64(int state, int S, int s, int ncomp, int64_t N, int ilv,
+                         const int32_t *ninv, const double *x, void *y, void *stream);
+int launch_from_solver_f64(int state, int S, int s, int ncomp, int64_t N, int ilv,
+                           const int32_t *nperm, const void *x, double *y, void *stream);
 // y[i] *= c*d[node(i)], node(i) = (ilv ? i / S : i) % N
 int launch_scale_by(int state, int64_t n, int S, int ilv, int64_t N, const double *d, double c,
                     void *y, void *stream);

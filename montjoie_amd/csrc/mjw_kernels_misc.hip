@@ -471,6 +471,59 @@ __global__ void k_from_solver(int S, int ncomp, int64_t N, int ilv,
   }
 }
 
+// One caller vector (FP64, caller's node numbering, [ncomp][N]) into shot s
+// of the solver layout in the state precision, and back: the renumbering and
+// the precision change of the host-vector entry points, on the device.
+template <typename TO>
+__global__ void k_to_solver_f64(int S, int s, int ncomp, int64_t N, int ilv,
+                                const int32_t *__restrict__ ninv, const double *__restrict__ x,
+                                TO *__restrict__ y) {
+  const int64_t per = (int64_t)ncomp * N;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < per;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = u / N, i = u - m * N;
+    y[solver_index(s, m * N + i, S, per, ilv)] = (TO)x[m * N + ninv[i]];
+  }
+}
+
+template <typename TI>
+__global__ void k_from_solver_f64(int S, int s, int ncomp, int64_t N, int ilv,
+                                  const int32_t *__restrict__ nperm, const TI *__restrict__ x,
+                                  double *__restrict__ y) {
+  const int64_t per = (int64_t)ncomp * N;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < per;
+       u += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = u / N, g = u - m * N;
+    y[u] = (double)x[solver_index(s, m * N + nperm[g], S, per, ilv)];
+  }
+}
+
+int launch_to_solver_f64(int state, int S, int s, int ncomp, int64_t N, int ilv,
+                         const int32_t *ninv, const double *x, void *y, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t n = (int64_t)ncomp * N;
+  if (state == kF64)
+    hipLaunchKernelGGL(k_to_solver_f64<double>, dim3(grid_for(n)), dim3(256), 0, st, S, s, ncomp,
+                       N, ilv, ninv, x, (double *)y);
+  else
+    hipLaunchKernelGGL(k_to_solver_f64<float>, dim3(grid_for(n)), dim3(256), 0, st, S, s, ncomp,
+                       N, ilv, ninv, x, (float *)y);
+  return (int)hipGetLastError();
+}
+
+int launch_from_solver_f64(int state, int S, int s, int ncomp, int64_t N, int ilv,
+                           const int32_t *nperm, const void *x, double *y, void *stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t n = (int64_t)ncomp * N;
+  if (state == kF64)
+    hipLaunchKernelGGL(k_from_solver_f64<double>, dim3(grid_for(n)), dim3(256), 0, st, S, s,
+                       ncomp, N, ilv, nperm, (const double *)x, y);
+  else
+    hipLaunchKernelGGL(k_from_solver_f64<float>, dim3(grid_for(n)), dim3(256), 0, st, S, s,
+                       ncomp, N, ilv, nperm, (const float *)x, y);
+  return (int)hipGetLastError();
+}
+
 int launch_to_solver(int state, int S, int ncomp, int64_t N, int ilv, const int32_t *ninv,
                      const void *x, void *y, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
