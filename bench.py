@@ -419,9 +419,26 @@ def bench_host_apply(m, mu, reps=5):
         P.apply_kh_dev(U.data_ptr(), Y.data_ptr())
     torch.cuda.synchronize()
     dev = (time.perf_counter() - t0) / reps
+    # the diagonal operators of Montjoie's Advance on host vectors, through the
+    # C ABI on preallocated arrays: (D - dt/2 S) U and (D + dt/2 S)^{-1} Y
+    from montjoie_amd import _lib
+    L, h, dt = _lib.lib(), P.handle, 1e-3
+    y = np.zeros(m.N)
+    diag = {}
+    for name, call in (("apply_dh_minus_dt_sh",
+                        lambda: L.mjw_apply_dh_minus_dt_sh(h, dt, 1.0, _lib.dptr(u), 0.0,
+                                                           _lib.dptr(y))),
+                       ("solve_dh_plus_dt_sh", lambda: L.mjw_solve_dh_plus_dt_sh(h, dt,
+                                                                                 _lib.dptr(y)))):
+        call()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            assert call() == 0
+        diag[name + "_ms"] = (time.perf_counter() - t0) / reps * 1e3
     P.close()
     return {"precision": "fp64", "host_vectors_ms": host * 1e3, "device_vectors_ms": dev * 1e3,
-            "host_gdof_s": m.N / host / 1e9, "device_gdof_s": m.N / dev / 1e9}
+            "host_gdof_s": m.N / host / 1e9, "device_gdof_s": m.N / dev / 1e9,
+            "host_diagonal_operators": diag}
 
 
 def sphere_dirs(n, seed=2027):

@@ -69,6 +69,7 @@ struct mjw_problem {
   bool nodal_media = false;           // rho, mu given per quadrature point
   // damping S_h = sigma J w per node (mjw_set_damping), solver order
   std::vector<double> damp_S;
+  std::vector<double> damp_c;         // the same in the caller's numbering, 0 on Dirichlet rows
   int64_t ndamp = 0;                  // damped, non-Dirichlet nodes
   double *d_Meff = nullptr, *d_damp_k = nullptr;
   int32_t *d_damp_idx = nullptr;
@@ -878,33 +879,26 @@ extern "C" int mjw_apply_kh(mjw_problem *p, double alpha, const double *U, doubl
 
 // S_h of node g in the caller's order; 0 on Dirichlet rows (S_h is assembled
 // with Dirichlet coefficient 0, TimeAcoustic.cxx:966-971)
-static double damp_at(const mjw_problem *p, int64_t g) {
-  if (p->damp_S.empty() || p->dir[g]) return 0.0;
-  return p->damp_S[p->nperm[g]];
-}
-
-// Y = beta*Y + alpha*diag(c)*U over the ncomp*N entries (beta = 0 overwrites)
-template <class F>
-static void diag_apply(const mjw_problem *p, double alpha, const double *U, double beta,
-                       double *Y, F coef) {
-  const int64_t n = (int64_t)p->ncomp * p->N;
-  for (int64_t i = 0; i < n; ++i) {
-    const double d = alpha * (coef(i % p->N) * U[i]);
-    Y[i] = beta == 0.0 ? d : beta * Y[i] + d;
-  }
+// S_h in the caller's order (0 on Dirichlet rows, S_h being assembled with
+// Dirichlet coefficient 0, TimeAcoustic.cxx:966-971), or null when undamped
+static const double *damp_of(const mjw_problem *p) {
+  return p->damp_c.empty() ? nullptr : p->damp_c.data();
 }
 
 extern "C" int mjw_apply_dh(mjw_problem *p, double alpha, const double *U, double beta,
                             double *Y) {
   if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_dh: NULL");
-  diag_apply(p, alpha, U, beta, Y, [p](int64_t g) { return p->mass[g]; });
+  diag_axpby(p->N, p->ncomp, p->mass.data(), nullptr, 0.0, alpha, U, beta, Y);
   return MJW_OK;
 }
 
 extern "C" int mjw_apply_sh(mjw_problem *p, double alpha, const double *U, double beta,
                             double *Y) {
   if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_sh: NULL");
-  diag_apply(p, alpha, U, beta, Y, [p](int64_t g) { return damp_at(p, g); });
+  if (damp_of(p))
+    diag_axpby(p->N, p->ncomp, damp_of(p), nullptr, 0.0, alpha, U, beta, Y);
+  else  // S = 0
+    diag_axpby(p->N, p->ncomp, nullptr, nullptr, 0.0, alpha, U, beta, Y);
   return MJW_OK;
 }
 
@@ -912,34 +906,27 @@ extern "C" int mjw_apply_dh_minus_dt_sh(mjw_problem *p, double dt, double alpha,
                                         double beta, double *Y) {
   if (!p || !U || !Y) FAIL(p, MJW_ERR_ARG, "mjw_apply_dh_minus_dt_sh: NULL");
   if (!(dt > 0.0) || !std::isfinite(dt)) FAIL(p, MJW_ERR_ARG, "mjw_apply_dh_minus_dt_sh: bad dt");
-  diag_apply(p, alpha, U, beta, Y,
-             [p, dt](int64_t g) { return p->mass[g] - 0.5 * dt * damp_at(p, g); });
+  // coefficient mass - (dt/2) S per node
+  diag_axpby(p->N, p->ncomp, p->mass.data(), damp_of(p), -0.5 * dt, alpha, U, beta, Y);
   return MJW_OK;
 }
 
 extern "C" int mjw_solve_dh(mjw_problem *p, double *Y) {
   if (!p || !Y) FAIL(p, MJW_ERR_ARG, "mjw_solve_dh: NULL");
-  const int64_t n = (int64_t)p->ncomp * p->N;
-  for (int64_t i = 0; i < n; ++i) Y[i] *= 1.0 / p->mass[i % p->N];
+  diag_solve(p->N, p->ncomp, p->mass.data(), nullptr, 0.0, Y);
   return MJW_OK;
 }
 
 extern "C" int mjw_solve_dh_plus_dt_sh(mjw_problem *p, double dt, double *Y) {
   if (!p || !Y) FAIL(p, MJW_ERR_ARG, "mjw_solve_dh_plus_dt_sh: NULL");
   if (!(dt > 0.0) || !std::isfinite(dt)) FAIL(p, MJW_ERR_ARG, "mjw_solve_dh_plus_dt_sh: bad dt");
-  const int64_t n = (int64_t)p->ncomp * p->N;
-  for (int64_t i = 0; i < n; ++i) {
-    const int64_t g = i % p->N;
-    Y[i] *= 1.0 / (p->mass[g] + 0.5 * dt * damp_at(p, g));
-  }
+  diag_solve(p->N, p->ncomp, p->mass.data(), damp_of(p), 0.5 * dt, Y);
   return MJW_OK;
 }
 
 extern "C" int mjw_set_dirichlet_condition(mjw_problem *p, double *Y) {
   if (!p || !Y) FAIL(p, MJW_ERR_ARG, "mjw_set_dirichlet_condition: NULL");
-  for (int m = 0; m < p->ncomp; ++m)
-    for (int64_t g = 0; g < p->N; ++g)
-      if (p->dir[g]) Y[(int64_t)m * p->N + g] = 0.0;
+  diag_mask(p->N, p->ncomp, p->dir.data(), Y);
   return MJW_OK;
 }
 
@@ -1584,6 +1571,7 @@ static int set_damping(mjw_problem *p, const double *sigma, bool nodal) {
   p->ndamp = 0;
   if (!any) {
     p->damp_S.clear();
+    p->damp_c.clear();
     return MJW_OK;
   }
   double xg[kMaxN], wg[kMaxN], Dg[kMaxN * kMaxN];
@@ -1626,8 +1614,12 @@ static int set_damping(mjw_problem *p, const double *sigma, bool nodal) {
   dfree(d_xyz); dfree(d_Dg); dfree(d_w); dfree(d_Mloc); dfree(d_sig); dfree(d_S); dfree(d_bad);
   if (e != hipSuccess) {
     p->damp_S.clear();
+    p->damp_c.clear();
     FAIL(p, MJW_ERR_HIP, std::string("mjw_set_damping: ") + hipGetErrorString(e));
   }
+  p->damp_c.assign(N, 0.0);
+  for (int64_t g = 0; g < N; ++g)
+    if (!p->dir[g]) p->damp_c[g] = p->damp_S[p->nperm[g]];
   return MJW_OK;
 }
 

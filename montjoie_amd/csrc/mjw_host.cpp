@@ -309,6 +309,42 @@ int color_elements(int64_t E, int nd, int64_t N, const int32_t *elem_nodes,
 }
 
 // ---------------------------------------------------------------------------
+// Diagonal operators on host vectors [ncomp][N] (the lumped mass, damping and
+// Dirichlet entries of the operator-level boundary): per node coefficient
+// a[g] + cc * c[g] (a null: 0, c null: a alone), in parallel over the nodes.
+// ---------------------------------------------------------------------------
+void diag_axpby(int64_t N, int ncomp, const double *a, const double *c, double cc, double alpha,
+                const double *U, double beta, double *Y) {
+  for (int m = 0; m < ncomp; ++m) {
+    const double *u = U + (int64_t)m * N;
+    double *y = Y + (int64_t)m * N;
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < N; ++g) {
+      const double coef = a ? (c ? a[g] + cc * c[g] : a[g]) : 0.0;
+      const double d = alpha * (coef * u[g]);
+      y[g] = beta == 0.0 ? d : beta * y[g] + d;
+    }
+  }
+}
+
+void diag_solve(int64_t N, int ncomp, const double *a, const double *c, double cc, double *Y) {
+  for (int m = 0; m < ncomp; ++m) {
+    double *y = Y + (int64_t)m * N;
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < N; ++g) y[g] *= 1.0 / (c ? a[g] + cc * c[g] : a[g]);
+  }
+}
+
+void diag_mask(int64_t N, int ncomp, const uint8_t *dir, double *Y) {
+  for (int m = 0; m < ncomp; ++m) {
+    double *y = Y + (int64_t)m * N;
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < N; ++g)
+      if (dir[g]) y[g] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Straight-sided hexahedra: for a non-curved element Montjoie evaluates DF of
 // the transformation built from the element's vertices at the quadrature
 // points (ComputeLocalMassMatrix, Harmonic/VarGeometryProblem.cxx:671-690,

@@ -45,6 +45,14 @@ void mesh_regular(int dim, int nx, int order, double pert, uint64_t seed, double
 int color_elements(int64_t E, int nd, int64_t N, const int32_t *elem_nodes,
                    std::vector<int> &color);
 
+// Diagonal operators on host vectors [ncomp][N], parallel over the nodes:
+// Y = beta Y + alpha diag(a + cc c) U (a null: coefficient 0; c null: a),
+// Y /= a + cc c, and Y = 0 on the nodes with dir set.
+void diag_axpby(int64_t N, int ncomp, const double *a, const double *c, double cc, double alpha,
+                const double *U, double beta, double *Y);
+void diag_solve(int64_t N, int ncomp, const double *a, const double *c, double cc, double *Y);
+void diag_mask(int64_t N, int ncomp, const uint8_t *dir, double *Y);
+
 // Point location in an isoparametric Q_r element: Newton on
 // x(xi) = sum_q x_q phi_q(xi).  Returns true when the reference point lies in
 // [-tol, 1+tol]^dim (xi clamped to [0,1]).
