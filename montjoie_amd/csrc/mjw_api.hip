@@ -1100,7 +1100,9 @@ extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, cons
     HIPCHK(p, hipMemcpy(p->d_src_shot, sh.data(), sh.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(p, hipMemcpy(p->d_src_val, val.data(), val.size() * 8, hipMemcpyHostToDevice));
   }
+  // both step graphs hold the old source arrays
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (p->bexec) { (void)hipGraphExecDestroy(p->bexec); p->bexec = nullptr; }
   return MJW_OK;
 }
 
@@ -1145,7 +1147,9 @@ extern "C" int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table) {
     HIPCHK(p, dalloc(p, &p->d_pulse, (size_t)nsteps * 8));
     HIPCHK(p, hipMemcpy(p->d_pulse, table, (size_t)nsteps * 8, hipMemcpyHostToDevice));
   }
+  // both step graphs hold the old table
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (p->bexec) { (void)hipGraphExecDestroy(p->bexec); p->bexec = nullptr; }
   return MJW_OK;
 }
 
