@@ -72,7 +72,8 @@ const char *mjw_last_error(const mjw_problem *p);
  * (VarGeometryProblem.cxx:648-790 geometry, VarHelmholtz.cxx:53-124
  * coefficients, TimeAcoustic.cxx:889-1004 lumped mass) + ComputeStiffnessMatrix
  * (VarInstationary.cxx:1303-1420).
- *   dim 2|3, order 1..7, ncomp 1 (acoustic) | 3 (elastic, dim 3 only);
+ *   dim 2|3, order 1..7, ncomp 1 (acoustic) | dim (isotropic elastic; in 2-D
+ *   plane strain, Elliptic/Elastic/VarElastic.cxx:22-30);
  *   node_xyz[n_nodes*dim] positions of all global GL nodes;
  *   elem_nodes[n_elem*(order+1)^dim] element -> global node map;
  *   rho[n_elem], mu[n_elem] per element; lambda[n_elem] for elasticity
@@ -110,7 +111,7 @@ int mjw_tile_choices(const mjw_problem *p, int *out, int cap);
 /*
  * Geometry of the 3-D stiffness kernels.
  *   MJW_GEOM_STORED: the per-node factors (acoustic J w mu DF^{-1} DF^{-T},
- *     6 values; elastic sqrt(J w) DF^{-1}, 9 values) are computed once by
+ *     6 values; elastic sqrt(J w) DF^{-1}, dim^2 values) are computed once by
  *     mjw_create and read every step;
  *   MJW_GEOM_TRILINEAR: for meshes of straight-sided hexahedra (every node the
  *     image of the GL grid under the trilinear map of the element's 8

@@ -360,10 +360,11 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   if (dim != 2 && dim != 3) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "dim must be 2 or 3");
   if (order < 1 || order > kMaxOrder)
     FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED, "order must be in 1..7");
-  if (ncomp != 1 && !(ncomp == 3 && dim == 3))
-    FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED, "ncomp must be 1, or 3 in 3-D");
+  if (ncomp != 1 && ncomp != dim)
+    FAIL((mjw_problem *)nullptr, MJW_ERR_UNSUPPORTED,
+         "ncomp must be 1 (acoustic) or the dimension (elastic)");
   if (n_nodes <= 0 || n_nodes >= ((int64_t)1 << 31) || n_elem <= 0 || !node_xyz || !elem_nodes ||
-      !rho || !mu || (ncomp == 3 && !lambda))
+      !rho || !mu || (ncomp > 1 && !lambda))
     FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_create: bad sizes or NULL arrays");
   if (nshots < 1 || nshots > 4096) FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "bad nshots");
   if (precision != MJW_FP64 && precision != MJW_FP32 && precision != MJW_FP32_G64)
@@ -374,7 +375,7 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
     const int64_t P = (int64_t)(order + 1) * (order + 1);
     const int64_t K = dim == 2 ? 1 : order + 1;
     const int64_t sb = precision == MJW_FP64 ? 8 : 4, gb = precision == MJW_FP32 ? 4 : 8;
-    const int64_t ngc = ncomp == 3 ? 9 : (dim == 2 ? 3 : 6);
+    const int64_t ngc = ncomp > 1 ? dim * dim : (dim == 2 ? 3 : 6);
     const int64_t lim = ((int64_t)1 << 31) - 4096;
     // several 3-D shots are interleaved: one array of S*ncomp*N values
     const int64_t sv = (dim == 3 && nshots > 1) ? nshots : 1;
@@ -415,7 +416,7 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
       return fail(MJW_ERR_ARG, "elem_nodes entry out of range");
   for (int64_t e = 0; e < n_elem; ++e)
     if (!(rho[e] > 0) || !std::isfinite(mu[e]) || !(mu[e] > 0) ||
-        (ncomp == 3 && !std::isfinite(lambda[e])))
+        (ncomp > 1 && !std::isfinite(lambda[e])))
       return fail(MJW_ERR_ARG, "media must be finite with rho, mu > 0");
 
   if (hipSetDevice(device) != hipSuccess) return fail(MJW_ERR_HIP, "hipSetDevice failed");
@@ -519,11 +520,11 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
         }
     }
   }
-  std::vector<double> rs(n_elem), ms(n_elem), ls(ncomp == 3 ? n_elem : 1);
+  std::vector<double> rs(n_elem), ms(n_elem), ls(ncomp > 1 ? n_elem : 1);
   for (int64_t es = 0; es < n_elem; ++es) {
     rs[es] = rho[p->perm[es]];
     ms[es] = mu[p->perm[es]];
-    if (ncomp == 3) ls[es] = lambda[p->perm[es]];
+    if (ncomp > 1) ls[es] = lambda[p->perm[es]];
   }
   // 1-D rule
   double xg[kMaxN], wg[kMaxN], Dg[kMaxN * kMaxN];
@@ -532,7 +533,7 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   std::vector<float> Df(p->n * p->n);
   for (int i = 0; i < p->n * p->n; ++i) Df[i] = (float)Dg[i];
 
-  const int ngc = ncomp == 3 ? 9 : (dim == 2 ? 3 : 6);
+  const int ngc = ncomp > 1 ? dim * dim : (dim == 2 ? 3 : 6);
   const size_t gsz = (size_t)n_elem * nd * ngc * (p->geom == kG64 ? 8 : 4);
   double *d_xyz = nullptr, *d_Dg = nullptr, *d_w = nullptr, *d_Mloc = nullptr;
   int *d_bad = nullptr;
@@ -551,7 +552,7 @@ extern "C" int mjw_create(mjw_problem **out, int device, int dim, int order, int
   CK(dalloc(p, &p->d_mu, n_elem * 8));
   CK(hipMemcpyAsync(p->d_rho, rs.data(), n_elem * 8, hipMemcpyHostToDevice, st));
   CK(hipMemcpyAsync(p->d_mu, ms.data(), n_elem * 8, hipMemcpyHostToDevice, st));
-  if (ncomp == 3) {
+  if (ncomp > 1) {
     CK(dalloc(p, &p->d_lam, n_elem * 8));
     CK(hipMemcpyAsync(p->d_lam, ls.data(), n_elem * 8, hipMemcpyHostToDevice, st));
   }
@@ -693,7 +694,7 @@ extern "C" int mjw_set_xcd_order(mjw_problem *p, int nxcd) {
 static hipError_t build_stored_geometry(mjw_problem *p) {
   if (p->d_G) return hipSuccess;
   const int dim = p->dim, nd = p->nd;
-  const int ngc = p->ncomp == 3 ? 9 : (dim == 2 ? 3 : 6);
+  const int ngc = p->ncomp > 1 ? dim * dim : (dim == 2 ? 3 : 6);
   const int64_t N = p->N, E = p->E;
   double xg[kMaxN], wg[kMaxN], Dg[kMaxN * kMaxN];
   gl_rule(p->order, xg, wg);

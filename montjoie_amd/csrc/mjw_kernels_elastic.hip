@@ -1,5 +1,5 @@
 // Sum-factorised stiffness kernels for isotropic elastodynamics on
-// Gauss-Lobatto Q_r hexahedra, gfx950.
+// Gauss-Lobatto Q_r hexahedra (and quadrilaterals, k_elastic2d), gfx950.
 //
 // Restates MltAdd_ElementH1 (Computation/ProdMatVectH1.cxx:674-944) for the
 // continuous second-order case: three reference gradients (Rh^T, :745-751),
@@ -233,9 +233,164 @@ static int run_elastic_prec(const StiffArgs &a, hipStream_t st) {
   X(2, 16) X(2, 32) X(3, 7) X(3, 14) X(4, 4) X(4, 8) X(5, 5) X(5, 3) X(6, 2) X(6, 3) X(7, 1) \
   X(7, 2) X(8, 1)
 
+// ------------------------------------------------------------------ 2-D
+// Quadrilaterals, two components: the 2-D instance of the same product
+// (ProdMatVectH1.cxx:674-944 on QUADRANGLE_LOBATTO, plane strain
+// sigma = lambda tr(e) I + 2 mu e).  One thread per (element, node), EPB
+// elements per block; M = sqrt(J w) DF^{-1} (4 values, planes [4][E][n*n]);
+// the two state planes of a block are staged in LDS, then overwritten by the
+// four flux planes once every thread has read them.  Shots are consecutive
+// vectors (u[(s*2 + m)*N + g]) and loop inside the block.
+template <int N, int EPB, typename T, typename TG, int MODE>
+__global__ __launch_bounds__(EPB *N *N) void k_elastic2d(
+    const int32_t *__restrict__ map, const TG *__restrict__ G, const T *__restrict__ Dm,
+    const double *__restrict__ lam_e, const double *__restrict__ mu_e, int64_t E, int64_t e0,
+    int64_t e1, const T *__restrict__ U, T *__restrict__ Y, const T *__restrict__ sdiag,
+    T alpha, T beta, int64_t N_nodes, int nshots, int nxcd) {
+  constexpr int P = N * N, PL = EPB * P;
+  using A = TG;
+  __shared__ T lds[4 * PL];  // u_0 u_1, then the fluxes f_m,b at plane 2m + b
+  const int t = threadIdx.x;
+  const int el = t / P, q = t - el * P;
+  const int i = q % N, j = q / N;
+  const int64_t eb = e0 + (int64_t)xcd_block(blockIdx.x, gridDim.x, nxcd) * EPB;
+  const int64_t e = eb + el;
+  const bool act = e < e1;
+  const int64_t plane = E * P;
+  A M[4] = {0, 0, 0, 0};
+  A lam = 0, mu = 0;
+  int32_t gid = 0;
+  if (act) {
+    gid = map[e * P + q];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) M[c] = (A)G[c * plane + e * P + q];
+    lam = (A)lam_e[e];
+    mu = (A)mu_e[e];
+  }
+  T Di[N], Dj[N], Dti[N], Dtj[N];
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    Di[l] = Dm[i * N + l];
+    Dj[l] = Dm[j * N + l];
+    Dti[l] = Dm[l * N + i];
+    Dtj[l] = Dm[l * N + j];
+  }
+  const int64_t g = (int64_t)(gid & 0x7fffffff);
+  const bool first = gid < 0;
+  const int base = el * P;
+  for (int s = 0; s < nshots; ++s) {
+    const int64_t o0 = (int64_t)s * 2 * N_nodes + g, o1 = o0 + N_nodes;
+    T u0 = 0, u1 = 0, y0 = 0, y1 = 0, sg = 0;
+    if (act) {
+      u0 = U[o0];
+      u1 = U[o1];
+      y0 = Y[o0];
+      y1 = Y[o1];
+      if (MODE == kLeap) sg = sdiag[g];
+      lds[base + q] = u0;
+      lds[PL + base + q] = u1;
+    }
+    __syncthreads();
+    A f[2][2] = {{0, 0}, {0, 0}};
+    if (act) {
+      T gr[2][2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        T a0 = 0, a1 = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          a0 += Di[l] * lds[m * PL + base + j * N + l];
+          a1 += Dj[l] * lds[m * PL + base + l * N + i];
+        }
+        gr[m][0] = a0;
+        gr[m][1] = a1;
+      }
+      // du = M^T g (= sqrt(Jw) grad u), sigma' = C : du, f = M sigma'
+      A du[2][2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) du[m][a] = M[0 * 2 + a] * (A)gr[m][0] + M[1 * 2 + a] * (A)gr[m][1];
+      const A tr = du[0][0] + du[1][1];
+      A sgm[2][2];
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) sgm[m][a] = mu * (du[m][a] + du[a][m]) + (m == a ? lam * tr : A(0));
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) f[m][b] = M[b * 2 + 0] * sgm[m][0] + M[b * 2 + 1] * sgm[m][1];
+    }
+    __syncthreads();  // every read of the staged state is done
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) lds[(2 * m + b) * PL + base + q] = (T)f[m][b];
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        T o = 0;
+#pragma unroll
+        for (int l = 0; l < N; ++l) {
+          o += Dti[l] * lds[(2 * m) * PL + base + j * N + l];
+          o += Dtj[l] * lds[(2 * m + 1) * PL + base + l * N + i];
+        }
+        const T u = m ? u1 : u0, y = m ? y1 : y0;
+        T r;
+        if constexpr (MODE == kLeap)
+          r = first ? (T(2) * u - y - sg * o) : (y - sg * o);
+        else
+          r = first ? (beta == T(0) ? alpha * o : beta * y + alpha * o) : (y + alpha * o);
+        Y[m ? o1 : o0] = r;
+      }
+    }
+    __syncthreads();  // the fluxes are read before the next shot is staged
+  }
+}
+
+template <int N, int EPB, typename T, typename TG>
+static int run_elastic2d(const StiffArgs &a, hipStream_t st) {
+  const int64_t ne = a.e1 - a.e0;
+  if (ne <= 0) return 0;
+  const unsigned blocks = (unsigned)((ne + EPB - 1) / EPB);
+#define MJW_EL2(MODE)                                                                           \
+  hipLaunchKernelGGL((k_elastic2d<N, EPB, T, TG, MODE>), dim3(blocks), dim3(EPB * N * N), 0, st, \
+                     a.map, static_cast<const TG *>(a.G), static_cast<const T *>(a.Dm), a.lam,  \
+                     a.mu, a.E, a.e0, a.e1, static_cast<const T *>(a.U), static_cast<T *>(a.Y), \
+                     static_cast<const T *>(a.sdiag), (T)a.alpha, (T)a.beta, a.N, a.nshots,      \
+                     a.nxcd)
+  if (a.mode == kLeap) MJW_EL2(kLeap);
+  else MJW_EL2(kApply);
+#undef MJW_EL2
+  return (int)hipGetLastError();
+}
+
+template <int N, int EPB>
+static int run_elastic2d_prec(const StiffArgs &a, hipStream_t st) {
+  if (a.state == kF64) return run_elastic2d<N, EPB, double, double>(a, st);
+  if (a.geom == kG64) return run_elastic2d<N, EPB, float, double>(a, st);
+  return run_elastic2d<N, EPB, float, float>(a, st);
+}
+
+// about 256 and 512 threads per block
+#define MJW_TILES_EL2(X)                                                                       \
+  X(2, 64) X(2, 128) X(3, 28) X(3, 56) X(4, 16) X(4, 32) X(5, 10) X(5, 20) X(6, 7) X(6, 14)    \
+  X(7, 5) X(7, 10) X(8, 4) X(8, 8)
+
 int launch_stiffness_elastic(const StiffArgs &a, void *stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int n = a.order + 1;
+  if (a.dim == 2) {
+#define X(NN, EE) \
+  if (n == NN && a.epb == EE) return run_elastic2d_prec<NN, EE>(a, st);
+    MJW_TILES_EL2(X)
+#undef X
+    return -1000;
+  }
 #define X(NN, EE) \
   if (n == NN && a.epb == EE) return run_elastic_prec<NN, EE>(a, st);
   MJW_TILES_EL(X)
@@ -243,12 +398,16 @@ int launch_stiffness_elastic(const StiffArgs &a, void *stream) {
   return -1000;
 }
 
-std::vector<int> tile_choices_elastic(int order) {
+std::vector<int> tile_choices_elastic(int dim, int order) {
   std::vector<int> v;
   const int n = order + 1;
 #define X(NN, EE) \
   if (n == NN) v.push_back(EE);
-  MJW_TILES_EL(X)
+  if (dim == 2) {
+    MJW_TILES_EL2(X)
+  } else {
+    MJW_TILES_EL(X)
+  }
 #undef X
   return v;
 }

@@ -13,18 +13,18 @@ namespace mjw {
 int launch_stiffness_scalar(const StiffArgs &a, void *stream);
 int launch_stiffness_elastic(const StiffArgs &a, void *stream);
 std::vector<int> tile_choices_scalar(int dim, int order);
-std::vector<int> tile_choices_elastic(int order);
+std::vector<int> tile_choices_elastic(int dim, int order);
 
 int launch_stiffness(const StiffArgs &a, void *stream) {
   if (a.ilv && a.dim == 3 && !a.grad)
     return a.ncomp == 3 ? launch_stiffness_shots_elastic(a, stream)
                         : launch_stiffness_shots(a, stream);
-  if (a.ncomp == 3) return launch_stiffness_elastic(a, stream);
+  if (a.ncomp > 1) return launch_stiffness_elastic(a, stream);
   return launch_stiffness_scalar(a, stream);
 }
 
 std::vector<int> tile_choices(int dim, int order, int ncomp) {
-  if (ncomp == 3) return tile_choices_elastic(order);
+  if (ncomp > 1) return tile_choices_elastic(dim, order);
   return tile_choices_scalar(dim, order);
 }
 
@@ -127,9 +127,11 @@ __global__ void k_geometry(int dim, int n, int ncomp, int64_t E, const int32_t *
       }
     }
   } else {
+    // elastic: M = sqrt(J w) DF^{-1}, dim*dim planes per layer k
     const double sq = sqrt(Jw);
-    for (int cc = 0; cc < 9; ++cc) {
-      const int64_t o = (((int64_t)k * 9 + cc) * E + e) * P + ij;
+    const int dd = dim * dim;
+    for (int cc = 0; cc < dd; ++cc) {
+      const int64_t o = (((int64_t)k * dd + cc) * E + e) * P + ij;
       const double v = sq * Di[cc];
       if (G64) G64[o] = v;
       if (G32) G32[o] = (float)v;

@@ -29,10 +29,10 @@
 // this path does not implement (another scheme, source or condition) stops
 // the run with a message, as Montjoie's own parser does (cout + abort).
 //
-// Elastic decks: Montjoie's rows hold the displacement of every point (3
-// values per point, point-major) followed by the stress tensor of every point
-// (9 values, row-major; DisplayStress = YES, its default); this driver writes
-// both.
+// Elastic decks (quadrilaterals or hexahedra): Montjoie's rows hold the
+// displacement of every point (d values per point, point-major) followed by
+// the stress tensor of every point (d*d values, row-major; DisplayStress =
+// YES, its default); this driver writes both.
 //
 // --dump-mesh writes the assembled mesh (sizes, node positions, element ->
 // node map in local order q = i + n j + n^2 k, Dirichlet flags, media) as raw
@@ -1532,8 +1532,7 @@ int main(int argc, char **argv) {
   const std::string eq = d.need("TypeEquation")[0];
   const bool elastic = eq == "TIME_ELASTIC";
   if (!elastic && eq != "ACOUSTIC") die("TypeEquation " + eq + ": only ACOUSTIC / TIME_ELASTIC");
-  if (elastic && dim != 3) die("TIME_ELASTIC needs hexahedra");
-  const int ncomp = elastic ? 3 : 1;
+  const int ncomp = elastic ? dim : 1;
   const int order = (int)num(d.need("OrderDiscretization")[0]);
   // OrderTimeScheme = 2 LEAP_FROG (second-order scheme, ModifiedEquationIterator)
   // or 2 LEAP_FROG_PML (the first-order split system, ModifiedEquationSystemIterator)
@@ -1684,7 +1683,7 @@ int main(int argc, char **argv) {
     std::vector<double> pol(ncomp, 1.0);
     if (elastic) {
       const auto &pv = d.need("Polarization");
-      for (int a = 0; a < 3; ++a) pol[a] = num(pv.at(a));
+      for (int a = 0; a < dim; ++a) pol[a] = num(pv.at(a));
     }
     check(mjw_set_sources(p, 1, x, pol.data(), nullptr), p, "source");
   }
@@ -1716,8 +1715,8 @@ int main(int argc, char **argv) {
     }
   if (elastic)
     for (int r = 0; r < npt; ++r)
-      for (int m = 0; m < 3; ++m)
-        for (int a = 0; a < 3; ++a) {
+      for (int m = 0; m < dim; ++m)
+        for (int a = 0; a < dim; ++a) {
           for (int b = 0; b < dim; ++b) rpts.push_back(pts[(size_t)r * dim + b]);
           comp.push_back(m);
           deriv.push_back(a);
@@ -1771,9 +1770,9 @@ int main(int argc, char **argv) {
       if (std::llround((t - ps0) / dt) % every) continue;
       sn.steps.push_back(st);
     }
-    // elastic: the points again for d / d x_0, d / d x_1, d / d x_2 (stress)
+    // elastic: the points again for d / d x_a, a < dim (stress)
     sn.off = probe_pts.size() / dim;
-    for (int dv = -1; dv < (elastic ? 3 : 0); ++dv) {
+    for (int dv = -1; dv < (elastic ? dim : 0); ++dv) {
       probe_pts.insert(probe_pts.end(), sn.g.pts.begin(), sn.g.pts.end());
       probe_deriv.insert(probe_deriv.end(), sn.g.pts.size() / dim, dv);
     }
@@ -1789,7 +1788,7 @@ int main(int argc, char **argv) {
   std::vector<double> pv(probe_deriv.size() * ncomp);
   // acoustic: <name>%04d.dat; elastic (OutputFormat's multi-field layout):
   // <name>%04d_U<m>.dat per displacement component and <name>%04d_dU<k>.dat
-  // per stress component sigma_ij, k = 3 i + j, with the medium of the
+  // per stress component sigma_ij, k = d i + j, with the medium of the
   // element holding the point (0 outside the mesh)
   auto snapshot = [&](int64_t st) {
     check(mjw_probe(p, pv.data()), p, "snapshot grid");
@@ -1804,22 +1803,25 @@ int main(int argc, char **argv) {
         write_plane(sn.name + num + ".dat", sn.g, vals, out_double, out_binary);
         continue;
       }
-      for (int m = 0; m < 3; ++m) {
-        for (size_t i = 0; i < n; ++i) vals[i] = pv[(sn.off + i) * 3 + m];
+      for (int m = 0; m < dim; ++m) {
+        for (size_t i = 0; i < n; ++i) vals[i] = pv[(sn.off + i) * dim + m];
         write_plane(sn.name + num + "_U" + std::to_string(m) + ".dat", sn.g, vals, out_double,
                     out_binary);
       }
       // gradient probes: sn.off + (1 + a) n + i holds d u_m / d x_a at point i
-      auto grad = [&](size_t i, int m, int a) { return pv[(sn.off + (1 + a) * n + i) * 3 + m]; };
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
+      auto grad = [&](size_t i, int m, int a) {
+        return pv[(sn.off + (1 + a) * n + i) * dim + m];
+      };
+      for (int a = 0; a < dim; ++a)
+        for (int b = 0; b < dim; ++b) {
           for (size_t i = 0; i < n; ++i) {
             const int64_t e = probe_elem[sn.off + n + i];
             if (e < 0) { vals[i] = 0.0; continue; }
-            const double div = grad(i, 0, 0) + grad(i, 1, 1) + grad(i, 2, 2);
+            double div = 0.0;
+            for (int c = 0; c < dim; ++c) div += grad(i, c, c);
             vals[i] = (a == b ? md.lam[e] * div : 0.0) + md.mu[e] * (grad(i, a, b) + grad(i, b, a));
           }
-          write_plane(sn.name + num + "_dU" + std::to_string(3 * a + b) + ".dat", sn.g, vals,
+          write_plane(sn.name + num + "_dU" + std::to_string(dim * a + b) + ".dat", sn.g, vals,
                       out_double, out_binary);
         }
     }
@@ -1903,19 +1905,22 @@ int main(int argc, char **argv) {
       for (int r = 0; r < npt; ++r)
         for (int m = 0; m < ncomp; ++m)
           std::fprintf(f, fmt, tr[(size_t)s * nrec + (size_t)m * npt + r]);
-      // elastic: then the stress C grad u of each point, 9 values row-major
+      // elastic: then the stress C grad u of each point, d*d values row-major
       // (sigma_ij = lambda tr(grad u) delta_ij + mu (du_i/dx_j + du_j/dx_i),
       // ModifyOutputUnknown, Elliptic/Elastic/VarElastic.cxx:257-282), with
       // the medium of the element holding the point
       if (elastic)
         for (int r = 0; r < npt; ++r) {
-          const double *g = &tr[(size_t)s * nrec + (size_t)3 * npt + (size_t)r * 9];
-          const int64_t e = rec_elem[(size_t)3 * npt + (size_t)r * 9];
-          const double lam = md.lam[e], mu = md.mu[e], div = g[0] + g[4] + g[8];
-          for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j)
+          const size_t dd = (size_t)dim * dim;
+          const double *g = &tr[(size_t)s * nrec + (size_t)dim * npt + (size_t)r * dd];
+          const int64_t e = rec_elem[(size_t)dim * npt + (size_t)r * dd];
+          const double lam = md.lam[e], mu = md.mu[e];
+          double div = 0.0;
+          for (int c = 0; c < dim; ++c) div += g[c * dim + c];
+          for (int i = 0; i < dim; ++i)
+            for (int j = 0; j < dim; ++j)
               std::fprintf(f, fmt,
-                           (i == j ? lam * div : 0.0) + mu * (g[i * 3 + j] + g[j * 3 + i]));
+                           (i == j ? lam * div : 0.0) + mu * (g[i * dim + j] + g[j * dim + i]));
         }
       std::fprintf(f, "\n");
     }

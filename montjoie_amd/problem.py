@@ -89,7 +89,8 @@ def time_source_table(f0: float, t_begin: float, dt: float, nsteps: int, nb_deri
 
 
 class WaveProblem:
-    """One acoustic (ncomp=1) or elastic (ncomp=3) problem on one device."""
+    """One acoustic (ncomp=1) or elastic (ncomp = dimension) problem on one
+    device."""
 
     def __init__(self, mesh: Mesh, rho, mu, lam=None, ncomp: int = 1, nshots: int = 1,
                  precision: str = "fp64", device: int = 0, dirichlet=True):
@@ -101,7 +102,7 @@ class WaveProblem:
         rho = np.ascontiguousarray(np.broadcast_to(np.asarray(rho, np.float64), (E,)))
         mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, np.float64), (E,)))
         lamv = None
-        if ncomp == 3:
+        if ncomp > 1:
             lamv = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float64), (E,)))
         xyz = np.ascontiguousarray(mesh.xyz, np.float64)
         en = np.ascontiguousarray(mesh.elem_nodes, np.int32)

@@ -1,6 +1,6 @@
 /*
  * mjw_oracle.c -- CPU FP64 restatement of Montjoie's explicit spectral-element
- * leapfrog path (acoustic 2-D/3-D and isotropic elastic 3-D).
+ * leapfrog path (acoustic 2-D/3-D and isotropic elastic 2-D/3-D).
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP path is
  * compared against; it is loaded by tests/, by __graft_entry__.smoke() and by
@@ -517,7 +517,7 @@ void or_stiffness(int dim, int r, long E, long N, const int *elem_nodes, const d
 }
 
 /* ------------------------------------------------------------------------ */
-/* Matrix-free stiffness (isotropic elastic, 3 components)                  */
+/* Matrix-free stiffness (isotropic elastic, d components)                  */
 /* ------------------------------------------------------------------------ */
 
 /*
@@ -578,30 +578,83 @@ static void local_apply_elastic(int n, const double *D, const double *jw, const 
   free(v);
 }
 
+/*
+ * The same product on quadrilaterals (the 2-D instance of MltAdd_ElementH1,
+ * Computation/ProdMatVectH1.cxx:674-944, with ApplyTensorStiffness of
+ * Elliptic/Elastic/VarElastic.cxx:22-30 in 2-D): two components, reference
+ * gradients along xi_1 and xi_2, DF^{-1} 2x2 (row-major), plane strain
+ * sigma = lambda tr(e) I + 2 mu e.
+ */
+static void local_apply_elastic2d(int n, const double *D, const double *jw, const double *dfi,
+                                  double lam, double mu, const double *ue, double *out) {
+  int nd = n * n;
+  double *v = (double *)malloc(sizeof(double) * 4 * nd);
+  for (int q = 0; q < nd; q++) {
+    int i = q % n, j = q / n;
+    double gr[2][2]; /* gr[m][b] = d u_m / d xi_b */
+    for (int m = 0; m < 2; m++) {
+      const double *um = ue + m * nd;
+      double g0 = 0, g1 = 0;
+      for (int l = 0; l < n; l++) {
+        g0 += D[i * n + l] * um[l + n * j];
+        g1 += D[j * n + l] * um[i + n * l];
+      }
+      gr[m][0] = g0; gr[m][1] = g1;
+    }
+    const double *Di = dfi + 4 * q; /* Di[a*2+b] = (DF^{-1})_{ab} */
+    double du[2][2];
+    for (int m = 0; m < 2; m++)
+      for (int a = 0; a < 2; a++) du[m][a] = Di[0 * 2 + a] * gr[m][0] + Di[1 * 2 + a] * gr[m][1];
+    double tr = du[0][0] + du[1][1];
+    double sg[2][2];
+    for (int m = 0; m < 2; m++)
+      for (int a = 0; a < 2; a++) sg[m][a] = mu * (du[m][a] + du[a][m]) + (m == a ? lam * tr : 0.0);
+    double s = jw[q];
+    for (int m = 0; m < 2; m++)
+      for (int b = 0; b < 2; b++)
+        v[4 * q + 2 * m + b] = s * (Di[b * 2 + 0] * sg[m][0] + Di[b * 2 + 1] * sg[m][1]);
+  }
+  for (int q = 0; q < nd; q++) {
+    int i = q % n, j = q / n;
+    for (int m = 0; m < 2; m++) {
+      double s = 0.0;
+      for (int l = 0; l < n; l++) {
+        s += D[l * n + i] * v[4 * (l + n * j) + 2 * m + 0];
+        s += D[l * n + j] * v[4 * (i + n * l) + 2 * m + 1];
+      }
+      out[m * nd + q] = s;
+    }
+  }
+  free(v);
+}
+
 /* y = beta*y + alpha*A u for elasticity (MltAddVectorH1,
- * Computation/ProdMatVectH1.cxx:1228-1276).  u, y: 3N component-blocked. */
-void or_stiffness_elastic(int r, long E, long N, const int *elem_nodes, const double *jw,
+ * Computation/ProdMatVectH1.cxx:1228-1276).  u, y: dim*N component-blocked. */
+void or_stiffness_elastic(int dim, int r, long E, long N, const int *elem_nodes, const double *jw,
                           const double *dfinv, const double *lam, const double *mu,
                           const int *color, int ncolor, double alpha, const double *u,
                           double beta, double *y) {
-  int n = r + 1, nd = n * n * n;
+  int n = r + 1, nd = dim == 2 ? n * n : n * n * n, dd = dim * dim;
   double x[OR_MAXN], w[OR_MAXN], D[OR_MAXN * OR_MAXN];
   or_gl(r, x, w);
   or_gl_deriv(r, x, D);
   if (beta == 0.0)
-    for (long g = 0; g < 3 * N; g++) y[g] = 0.0;
+    for (long g = 0; g < dim * N; g++) y[g] = 0.0;
   else if (beta != 1.0)
-    for (long g = 0; g < 3 * N; g++) y[g] *= beta;
+    for (long g = 0; g < dim * N; g++) y[g] *= beta;
   for (int c = 0; c < ncolor; c++) {
 #pragma omp parallel for schedule(static)
     for (long e = 0; e < E; e++) {
       if (color[e] != c) continue;
-      double *ue = (double *)malloc(sizeof(double) * 6 * nd);
-      double *out = ue + 3 * nd;
-      for (int m = 0; m < 3; m++)
+      double *ue = (double *)malloc(sizeof(double) * 2 * dim * nd);
+      double *out = ue + dim * nd;
+      for (int m = 0; m < dim; m++)
         for (int q = 0; q < nd; q++) ue[m * nd + q] = u[m * N + elem_nodes[e * nd + q]];
-      local_apply_elastic(n, D, jw + e * nd, dfinv + e * nd * 9, lam[e], mu[e], ue, out);
-      for (int m = 0; m < 3; m++)
+      if (dim == 2)
+        local_apply_elastic2d(n, D, jw + e * nd, dfinv + e * nd * dd, lam[e], mu[e], ue, out);
+      else
+        local_apply_elastic(n, D, jw + e * nd, dfinv + e * nd * dd, lam[e], mu[e], ue, out);
+      for (int m = 0; m < dim; m++)
         for (int q = 0; q < nd; q++) y[m * N + elem_nodes[e * nd + q]] += alpha * out[m * nd + q];
       free(ue);
     }
@@ -921,7 +974,7 @@ void or_circle_points(double cx, double cy, double radius, int npts, double *pts
  *     Y = -A U1 + pulse[n] b;  Y = dt^2 Y - (D - dt/2 S) U0;  Y += 2 D U1;
  *     U0 = U1; U1 = Y; zero Dirichlet; U1 *= (D + dt/2 S)^{-1}
  *   after the loop record R U0  (GiveFinalIterate).
- * ncomp = 1 (acoustic, G) or 3 (elastic, jw/dfinv/lam/mu).
+ * ncomp = 1 (acoustic, G) or dim (elastic, jw/dfinv/lam/mu).
  * Inputs u0/du0 may be NULL (zero).  Outputs: uN = U^nsteps, uN1 = U^{nsteps+1}
  * (the scheme's U1 at exit), traces[(nsteps+1)*nrec] = sum_q wr_q U0[node_q]
  * on component rcomp[rec].
@@ -961,7 +1014,7 @@ void or_leapfrog(int dim, int r, int ncomp, long E, long N, const int *elem_node
     if (ncomp == 1)                                                                        \
       or_stiffness(dim, r, E, N, elem_nodes, G, color, ncolor, (alpha), (uu), 0.0, (yy));  \
     else                                                                                   \
-      or_stiffness_elastic(r, E, N, elem_nodes, jw, dfinv, lam, mu, color, ncolor, (alpha), \
+      or_stiffness_elastic(dim, r, E, N, elem_nodes, jw, dfinv, lam, mu, color, ncolor, (alpha), \
                            (uu), 0.0, (yy));                                               \
   } while (0)
 #define OR_DIRICHLET(vv)                                  \
@@ -1074,7 +1127,7 @@ double or_lambda_max(int dim, int r, int ncomp, long E, long N, const int *elem_
     if (ncomp == 1)
       or_stiffness(dim, r, E, N, elem_nodes, G, color, ncolor, 1.0, t1, 0.0, wv);
     else
-      or_stiffness_elastic(r, E, N, elem_nodes, jw, dfinv, lam, mu, color, ncolor, 1.0, t1, 0.0, wv);
+      or_stiffness_elastic(dim, r, E, N, elem_nodes, jw, dfinv, lam, mu, color, ncolor, 1.0, t1, 0.0, wv);
     for (long g = 0; g < NT; g++) wv[g] *= sq[g];
     double a = 0.0;
     for (long g = 0; g < NT; g++) a += wv[g] * q[g];

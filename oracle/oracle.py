@@ -61,7 +61,7 @@ def lib():
     L.or_color.restype = C.c_int
     L.or_stiffness.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, _ip, _dp, C.c_void_p,
                                C.c_int, C.c_double, _dp, C.c_double, _dp]
-    L.or_stiffness_elastic.argtypes = [C.c_int, C.c_long, C.c_long, _ip, _dp, _dp, _dp, _dp,
+    L.or_stiffness_elastic.argtypes = [C.c_int, C.c_int, C.c_long, C.c_long, _ip, _dp, _dp, _dp, _dp,
                                        _ip, C.c_int, C.c_double, _dp, C.c_double, _dp]
     L.or_ricker.argtypes = [C.c_double, C.c_double]
     L.or_ricker.restype = C.c_double
@@ -565,10 +565,11 @@ def stiffness(m: Mesh, G, u, colors=None, alpha=1.0):
 
 def stiffness_elastic(m: Mesh, jw, dfi, lam, mu, u, colors, alpha=1.0):
     c, nc = colors
-    y = np.zeros(3 * m.N)
+    y = np.zeros(m.dim * m.N)
     lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float64), (m.E,)))
     mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, np.float64), (m.E,)))
-    lib().or_stiffness_elastic(m.r, m.E, m.N, m.elem_nodes.reshape(-1), jw, dfi, lam, mu, c, nc,
+    lib().or_stiffness_elastic(m.dim, m.r, m.E, m.N, m.elem_nodes.reshape(-1), jw, dfi, lam, mu,
+                               c, nc,
                                alpha, np.ascontiguousarray(u, dtype=np.float64), 0.0, y)
     return y
 
@@ -666,29 +667,30 @@ def snapshot(m: Mesh, u, pts):
 
 
 def snapshot_elastic(m: Mesh, u, pts, lam_e, mu_e):
-    """Elastic snapshot fields at pts: [3, npts] displacement and [9, npts]
+    """Elastic snapshot fields at pts: [d, npts] displacement and [d*d, npts]
     stress sigma_ij = lambda tr(grad u) delta_ij + mu (du_i/dx_j + du_j/dx_i)
-    (row-major, k = 3 i + j) with the medium of the first element holding each
+    (row-major, k = d i + j) with the medium of the first element holding each
     point; 0 outside the mesh."""
+    d = m.dim
     el, W, Gw = receiver_gradients(m, pts)
     npts = len(el)
-    disp = np.zeros((3, npts))
-    sig = np.zeros((9, npts))
+    disp = np.zeros((d, npts))
+    sig = np.zeros((d * d, npts))
     en = m.elem_nodes.reshape(m.E, -1)
     for k in range(npts):
         if el[k] < 0:
             continue
-        ue = np.stack([u[c * m.N + en[el[k]]] for c in range(3)])
+        ue = np.stack([u[c * m.N + en[el[k]]] for c in range(d)])
         disp[:, k] = ue @ W[k]
         g = ue @ Gw[k].T  # g[i, a] = d u_i / d x_a
-        s = lam_e[el[k]] * np.trace(g) * np.eye(3) + mu_e[el[k]] * (g + g.T)
+        s = lam_e[el[k]] * np.trace(g) * np.eye(d) + mu_e[el[k]] * (g + g.T)
         sig[:, k] = s.ravel()
     return disp, sig
 
 
 @dataclass
 class Problem:
-    """A structured acoustic (ncomp=1) or elastic (ncomp=3) problem."""
+    """A structured acoustic (ncomp=1) or elastic (ncomp = dim) problem."""
     mesh: Mesh
     ncomp: int
     G: np.ndarray = None
@@ -715,7 +717,7 @@ def acoustic_problem(m: Mesh, mu_e, rho_e) -> Problem:
 def elastic_problem(m: Mesh, lam_e, mu_e, rho_e) -> Problem:
     jw, dfi, Mloc = geometry(m, mu_e, rho_e, elastic=True)
     M, Mi = mass(m, Mloc)
-    return Problem(m, 3, jw=jw, dfi=dfi,
+    return Problem(m, m.dim, jw=jw, dfi=dfi,
                    lam=np.ascontiguousarray(np.broadcast_to(np.asarray(lam_e, float), (m.E,))),
                    mu=np.ascontiguousarray(np.broadcast_to(np.asarray(mu_e, float), (m.E,))),
                    rho=np.broadcast_to(np.asarray(rho_e, float), (m.E,)).copy(), M=M, Minv=Mi,

@@ -150,8 +150,8 @@ def oracle_run(c: Case, nsteps: int | None = None):
     S = O.damping(m, c.sigma_e) if c.sigma_e is not None and np.any(c.sigma_e) else None
     if c.elastic:
         # every receiver on the three components, component-major
-        pts = np.vstack([c.rec] * 3)
-        comp = np.repeat(np.arange(3, dtype=np.int32), len(c.rec))
+        pts = np.vstack([c.rec] * c.dim)
+        comp = np.repeat(np.arange(c.dim, dtype=np.int32), len(c.rec))
         return O.leapfrog(P, b, pulse, c.dt, nsteps, pts, comp)
     return O.leapfrog(P, b, pulse, c.dt, nsteps, c.rec, rc, S=S)
 
@@ -160,9 +160,10 @@ def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
     """Stress traces of an elastic case from the oracle: d u_i / d x_a at each
     receiver recorded by the leapfrog (gradient weights), then
     sigma = lambda tr(grad u) I + mu (grad u + grad u^T) with the medium of
-    the element holding the point; rows n = 0..nsteps, 9 values per point
+    the element holding the point; rows n = 0..nsteps, d*d values per point
     (row-major), point-major."""
     nsteps = c.nsteps if nsteps is None else nsteps
+    d = c.dim
     m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     P = O.elastic_problem(m, c.lam_e, c.mu_e, c.rho_e)
     b = O.dirac(m, c.src, c.pol)
@@ -170,28 +171,28 @@ def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
     assert (el >= 0).all()
     npt = len(c.rec)
     # receiver (point r, component i, direction a), point-major
-    re = np.repeat(el, 9)
-    rw = np.stack([Gw[r, a] for r in range(npt) for i in range(3) for a in range(3)])
-    comp = np.array([i for r in range(npt) for i in range(3) for a in range(3)], np.int32)
+    re = np.repeat(el, d * d)
+    rw = np.stack([Gw[r, a] for r in range(npt) for i in range(d) for a in range(d)])
+    comp = np.array([i for r in range(npt) for i in range(d) for a in range(d)], np.int32)
     pulse = O.pulse_table(c.f0, 0.0, c.dt, nsteps)
     _, _, g = O.leapfrog(P, b, pulse, c.dt, nsteps, rec_comp=comp, rec_weights=(re, rw))
-    g = g.reshape(nsteps + 1, npt, 3, 3)
+    g = g.reshape(nsteps + 1, npt, d, d)
     lam = c.lam_e[el][None, :, None, None]
     mu = c.mu_e[el][None, :, None, None]
     div = np.trace(g, axis1=2, axis2=3)[:, :, None, None]
-    sig = lam * div * np.eye(3) + mu * (g + g.transpose(0, 1, 3, 2))
-    return sig.reshape(nsteps + 1, npt * 9)
+    sig = lam * div * np.eye(d) + mu * (g + g.transpose(0, 1, 3, 2))
+    return sig.reshape(nsteps + 1, npt * d * d)
 
 
 def reference_traces(c: Case) -> np.ndarray:
     """Montjoie's traces re-ordered like oracle_run's (rows n = 0..nsteps)."""
     tr = c.traces
     if c.elastic:
-        nr = len(c.rec)
+        nr, d = len(c.rec), c.dim
         # elastic SismoPoint rows start with the displacement of each point
-        # (3 components per point), followed by 9 further fields per point
-        tr = tr[:, :3 * nr].reshape(tr.shape[0], nr, 3).transpose(0, 2, 1)
-        tr = tr.reshape(tr.shape[0], 3 * nr)
+        # (d components per point), followed by d*d further fields per point
+        tr = tr[:, :d * nr].reshape(tr.shape[0], nr, d).transpose(0, 2, 1)
+        tr = tr.reshape(tr.shape[0], d * nr)
     return tr
 
 
@@ -233,7 +234,7 @@ def gpu_problem(c: Case, precision: str = "fp64", nshots: int = 1, srcs=None):
         return P
     m = M.regular_mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     if c.elastic:
-        P = M.WaveProblem(m, c.rho_e, c.mu_e, lam=c.lam_e, ncomp=3, nshots=nshots,
+        P = M.WaveProblem(m, c.rho_e, c.mu_e, lam=c.lam_e, ncomp=c.dim, nshots=nshots,
                           precision=precision)
     elif c.mu_e.ndim == 2:
         # media varying inside the elements
@@ -248,7 +249,8 @@ def gpu_problem(c: Case, precision: str = "fp64", nshots: int = 1, srcs=None):
     pol = None if not c.elastic else np.vstack([c.pol] * len(srcs))
     P.set_sources(np.asarray(srcs), pol=pol, shot=np.arange(len(srcs)) % nshots)
     if c.elastic:
-        P.set_receivers(np.vstack([c.rec] * 3), comp=np.repeat(np.arange(3), len(c.rec)))
+        P.set_receivers(np.vstack([c.rec] * c.dim),
+                        comp=np.repeat(np.arange(c.dim), len(c.rec)))
     else:
         P.set_receivers(c.rec)
     return P

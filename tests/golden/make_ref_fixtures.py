@@ -60,7 +60,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
                save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma,
                output_format=output_format, circle=circle, pml=pml)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
-    args = [exe, "case.ini"] + ([] if elastic else [str(dim)])
+    args = [exe, "case.ini", str(dim)]
     for f in os.listdir(d):
         if f.startswith(("total", "diffrac", "SolAcous", "IterationAcous")):
             os.remove(os.path.join(d, f))
@@ -164,6 +164,30 @@ def ref3d_el_q4():
     rec = seeded_points(3, 6, 92)
     run_case("ref3d_el_q4", 3, 3, 4, 0.1, 17, mu, 8.0, 3e-3, 150, (0.47, 0.54, 0.49), rec,
              elastic=True, lam_layers=lam, rho_layers=rho, pol=(-0.4, 0.7, 0.5))
+
+
+@case
+def ref2d_el_q4():
+    """isotropic elastic on quadrilaterals: 8x8 Q4 perturbed unit square,
+    three layers of lambda, mu in [1, 4] and rho in [1, 2]."""
+    lam = [1.0 + 3.0 * O.uniform(2036, k) for k in range(3)]
+    mu = [1.0 + 3.0 * O.uniform(2037, k) for k in range(3)]
+    rho = [1.0 + O.uniform(2038, k) for k in range(3)]
+    rec = seeded_points(2, 8, 93)
+    run_case("ref2d_el_q4", 2, 8, 4, 0.12, 21, mu, 10.0, 1e-3, 300, (0.46, 0.53), rec,
+             elastic=True, lam_layers=lam, rho_layers=rho, pol=(0.6, -0.8))
+
+
+@case
+def ref2d_el_q7():
+    """isotropic elastic on quadrilaterals at the highest order: 4x4 Q7
+    perturbed, two layers."""
+    lam = [1.0 + 3.0 * O.uniform(2039, k) for k in range(2)]
+    mu = [1.0 + 3.0 * O.uniform(2040, k) for k in range(2)]
+    rho = [1.0 + O.uniform(2041, k) for k in range(2)]
+    rec = seeded_points(2, 6, 94)
+    run_case("ref2d_el_q7", 2, 4, 7, 0.1, 22, mu, 8.0, 1e-3, 300, (0.52, 0.44), rec,
+             elastic=True, lam_layers=lam, rho_layers=rho, pol=(-0.3, 0.9))
 
 
 @case
@@ -421,6 +445,18 @@ def ref3d_el_q3_plane():
              plane=[0.05, 0.1, 0.5, 1.2, 0.1, 0.55, 0.05, 0.95, 0.6, 5, 4],
              plane_times=[0.0, 0.36, 0.12], line=[0.1, 1.3, 0.2, 0.7, 0.3, 0.6, 7],
              line_times=[0.0, 0.36, 0.18])
+
+
+@case
+def ref2d_el_q4_plane():
+    """Elastic 8x8 Q4 perturbed quadrilaterals with a SismoPlane box reaching
+    outside the square every 50 steps and a SismoLine every 40 steps: one
+    file per displacement component and per stress component."""
+    rec = seeded_points(2, 3, 95)
+    run_case("ref2d_el_q4_plane", 2, 8, 4, 0.1, 23, [1.5, 2.5], 10.0, 1e-3, 200, (0.47, 0.52),
+             rec, elastic=True, lam_layers=[2.0, 1.2], rho_layers=[1.2, 1.0], pol=(0.6, -0.8),
+             plane=[-0.1, 0.9, 0.2, 1.15, 7, 6], plane_times=[0.0, 0.2, 0.05],
+             line=[0.1, 1.3, 0.2, 0.7, 7], line_times=[0.0, 0.2, 0.04])
 
 
 @case

@@ -195,7 +195,7 @@ def test_two_sources_in_one_element_of_one_shot():
     assert CS.rel_l2(runs[0], otr) < 1e-12
 
 
-@pytest.mark.parametrize("dim,ncomp", [(2, 1), (3, 1), (3, 3)])
+@pytest.mark.parametrize("dim,ncomp", [(2, 1), (3, 1), (2, 2), (3, 3)])
 def test_energy_matches_oracle(dim, ncomp):
     """mjw_energy = 1/2 |(U^{n+1} - U^n)/dt|_M^2 + 1/2 (U^{n+1})^T A U^n with the
     oracle's M and A, after 60 steps of a source run (FP64)."""
@@ -210,9 +210,9 @@ def test_energy_matches_oracle(dim, ncomp):
         P.set_sources(src)
     else:
         lam, rho = 2.0 + np.arange(m.E) % 2, 1.0 + 0.5 * (np.arange(m.E) % 3)
-        P = M.WaveProblem(m, rho, mu, lam=lam, ncomp=3)
+        P = M.WaveProblem(m, rho, mu, lam=lam, ncomp=dim)
         OP = O.elastic_problem(o, lam, mu, rho)
-        P.set_sources(src, pol=[0.3, -0.5, 0.8])
+        P.set_sources(src, pol=[0.3, -0.5, 0.8][:dim])
     P.set_pulse(M.ricker_pulse_table(8.0, 0.0, dt, n))
     P.leapfrog_init(dt)
     P.advance(n, traces=False)

@@ -126,22 +126,22 @@ def test_gradient_receivers_give_montjoie_stress(name):
     the stress columns of Montjoie's SismoPoint rows (rel-L2 1e-10)."""
     c = CS.load(name)
     P = CS.gpu_problem(c, "fp64")
-    npt = len(c.rec)
-    pts = np.repeat(c.rec, 9, axis=0)
-    comp = np.tile(np.repeat(np.arange(3), 3), npt)
-    deriv = np.tile(np.arange(3), 3 * npt)
+    npt, d = len(c.rec), c.dim
+    pts = np.repeat(c.rec, d * d, axis=0)
+    comp = np.tile(np.repeat(np.arange(d), d), npt)
+    deriv = np.tile(np.arange(d), d * npt)
     elem = P.set_receivers(pts, comp, deriv)
     o = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     el, _, _ = O.receiver_gradients(o, c.rec)
-    assert np.array_equal(elem[::9], el)
+    assert np.array_equal(elem[::d * d], el)
     P.set_pulse(M.ricker_pulse_table(c.f0, 0.0, c.dt, c.nsteps))
     P.leapfrog_init(c.dt)
-    g = P.run(c.nsteps)[:, 0, :].reshape(c.nsteps + 1, npt, 3, 3)
+    g = P.run(c.nsteps)[:, 0, :].reshape(c.nsteps + 1, npt, d, d)
     lam = c.lam_e[el][None, :, None, None]
     mu = c.mu_e[el][None, :, None, None]
     div = np.trace(g, axis1=2, axis2=3)[:, :, None, None]
-    sig = (lam * div * np.eye(3) + mu * (g + g.transpose(0, 1, 3, 2))).reshape(c.nsteps + 1, -1)
-    err = CS.rel_l2(sig, c.traces[:, 3 * npt:])
+    sig = (lam * div * np.eye(d) + mu * (g + g.transpose(0, 1, 3, 2))).reshape(c.nsteps + 1, -1)
+    err = CS.rel_l2(sig, c.traces[:, d * npt:])
     print(f"{name}: stress rel-L2 vs Montjoie {err:.2e}")
     assert err < 1e-10
 

@@ -245,6 +245,34 @@ def test_elastic_rigid_motions_and_symmetry():
     assert abs(v @ O.apply(P, u) - u @ O.apply(P, v)) < 1e-10
 
 
+@pytest.mark.parametrize("r", [1, 2, 4, 6])
+def test_elastic2d_kernel_invariants(r):
+    """Quadrilaterals, two components: rigid translations and the
+    infinitesimal rotation are in the kernel, A is symmetric, and for the
+    linear field u = G x the energy u^T A u equals the area times
+    lambda tr(e)^2 + 2 mu e:e with e = sym(G) (GL quadrature is exact for
+    the bilinear Jacobian).  A transposed DF^{-1}, a dropped symmetrisation
+    or a missing lambda term fails one of them."""
+    m = O.mesh(2, 3, r, 0.12, 5)
+    lam, mu = 2.0, 1.5
+    P = O.elastic_problem(m, lam, mu, 1.0)
+    assert P.ncomp == 2
+    X = m.xyz
+    t = np.concatenate([np.full(m.N, 1.0), np.full(m.N, -2.0)])
+    rot = np.concatenate([-X[:, 1], X[:, 0]])
+    for v in (t, rot):
+        assert np.abs(O.apply(P, v)).max() < 1e-11
+    rng = np.random.default_rng(r)
+    u, v = rng.standard_normal((2, 2 * m.N))
+    assert abs(v @ O.apply(P, u) - u @ O.apply(P, v)) < 1e-10 * np.abs(u @ O.apply(P, u))
+    G = np.array([[0.7, -0.3], [0.45, 1.1]])
+    ul = np.concatenate([X @ G[0], X @ G[1]])
+    e = 0.5 * (G + G.T)
+    area = 1.0  # the unit square: perturbation moves interior vertices only
+    ref = area * (lam * np.trace(e) ** 2 + 2 * mu * np.sum(e * e))
+    assert abs(ul @ O.apply(P, ul) - ref) < 1e-12 * ref
+
+
 def test_oracle_adjoint_gradient_matches_finite_differences():
     """The FP64 reference gradient is pinned by central differences of the
     oracle's own misfit (the reference has no adjoint to compare with)."""
@@ -388,14 +416,14 @@ def test_oracle_montjoie_dofs_tiny():
 
 @pytest.mark.parametrize("name", [n for n in CS.golden_names() if "_el_" in n])
 def test_oracle_elastic_stress_matches_montjoie(name):
-    """Elastic SismoPoint rows end with the stress tensor of each point (9
+    """Elastic SismoPoint rows end with the stress tensor of each point (d*d
     values, row-major): the oracle's gradient receivers turned into
     lambda tr(grad u) I + mu (grad u + grad u^T) match them to rel-L2 1e-10
-    on every row."""
+    on every row, in 2-D and 3-D."""
     c = CS.load(name)
-    npt = len(c.rec)
-    ref = c.traces[:, 3 * npt:]
-    assert ref.shape[1] == 9 * npt
+    npt, d = len(c.rec), c.dim
+    ref = c.traces[:, d * npt:]
+    assert ref.shape[1] == d * d * npt
     got = CS.oracle_stress(c)
     assert got.shape == ref.shape
     assert np.abs(ref).max() > 0
@@ -419,37 +447,40 @@ def test_oracle_gradient_weights_exact_on_linear_fields():
         np.testing.assert_allclose(W[k] @ u, pts[k] @ g + 0.7, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("name", ["ref3d_el_q3_plane", "ref2d_el_q4_plane"])
 @pytest.mark.parametrize("kind", ["plane", "line"])
-def test_oracle_elastic_snapshots_match_montjoie(kind):
+def test_oracle_elastic_snapshots_match_montjoie(kind, name):
     """Elastic SismoPlane / SismoLine files written by Montjoie: one file per
-    displacement component (_U<m>) and per stress component (_dU<k>) at each
-    snapshot step; the oracle's fields at the grid points match them to
-    rel-L2 1e-10 with the same zeros outside the mesh."""
-    name = "ref3d_el_q3_plane"
+    displacement component (_U<m>) and per stress component (_dU<k>, k = d i
+    + j) at each snapshot step; the oracle's fields at the grid points match
+    them to rel-L2 1e-10 with the same zeros outside the mesh (hexahedra and
+    quadrilaterals)."""
     c = CS.load(name)
+    d = c.dim
     z = np.load(os.path.join(CS.GOLD, name + ".npz"))
-    pts = O.plane_points(3, z["plane"]) if kind == "plane" else O.line_points(3, z["line"])
-    gtype = 2 if kind == "plane" else 3
+    pts = O.plane_points(d, z["plane"]) if kind == "plane" else O.line_points(d, z["line"])
+    gtype = (2 if kind == "plane" else 3) if d == 3 else (0 if kind == "plane" else 1)
     steps = CS.plane_steps(c, z[kind + "_times"])
     files, names = z[kind + "_files"], list(z[kind + "_names"])
     prefix = "totalPlane" if kind == "plane" else "totalLine"
-    suffixes = [f"_U{m}" for m in range(3)] + [f"_dU{k}" for k in range(9)]
+    suffixes = [f"_U{m}" for m in range(d)] + [f"_dU{k}" for k in range(d * d)]
+    nf = len(suffixes)
     assert names == [f"{prefix}{k:04d}{sf}.dat" for k in range(len(steps)) for sf in suffixes]
     m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     for k, s in enumerate(steps):
         uN, _, _ = CS.oracle_run(c, s)
         disp, sig = O.snapshot_elastic(m, uN, pts, c.lam_e, c.mu_e)
         fields = np.vstack([disp, sig])
-        for f in range(12):
-            head, ref = CS.plane_file(files[12 * k + f], len(pts))
-            assert list(np.frombuffer(head[:16], np.int32)) == [1, 3, 1, gtype]
+        for f in range(nf):
+            head, ref = CS.plane_file(files[nf * k + f], len(pts))
+            assert list(np.frombuffer(head[:16], np.int32)) == [1, d, 1, gtype]
             got = fields[f]
             if not ref.any():
                 assert not got.any()
                 continue
             tiny = 1e-12 * np.abs(ref).max()
             assert np.all(np.abs(ref[got == 0]) <= tiny) and np.all(np.abs(got[ref == 0]) <= tiny)
-            assert CS.rel_l2(got, ref) < 1e-10, (s, names[12 * k + f], CS.rel_l2(got, ref))
+            assert CS.rel_l2(got, ref) < 1e-10, (s, names[nf * k + f], CS.rel_l2(got, ref))
 
 
 @pytest.mark.parametrize("name", [n for n in CS.golden_names() if "circle" in n])
