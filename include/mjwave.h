@@ -368,8 +368,11 @@ int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const double *mu_q)
 /*
  * Damping sigma >= 0 per element (acoustic: rho u'' + sigma u' - div(mu grad u)
  * = f, the sigma of MateriauDielec = ref ISOTROPE rho mu sigma,
- * Elliptic/Helmholtz/VarHelmholtz.cxx:64-66,108-109).  S_h = sigma J w per
- * node, lumped like the mass; the leapfrog then solves
+ * Elliptic/Helmholtz/VarHelmholtz.cxx:64-66,108-109; elastic:
+ * rho u'' + sigma u' - div(C : grad u) = f, the fourth value of
+ * MateriauDielec = ref ISOTROPE rho lambda mu sigma,
+ * Elliptic/Elastic/VarElastic.cxx:59-71).  S_h = sigma J w per node (the same
+ * for every component), lumped like the mass; the leapfrog then solves
  * (D + dt/2 S) U^{n+1} = 2 D U^n - (D - dt/2 S) U^{n-1} + dt^2 (F - A U^n)
  * (Advance, ModifiedEquation.cxx:295-345, with DhMinusdtSh / DhPlusdtSh,
  * Hyperbolic/Acoustic/TimeAcoustic.cxx:970-1004) and starts from

@@ -1553,12 +1553,13 @@ extern "C" int mjw_set_probes_grad(mjw_problem *p, int npts, const double *pts,
 
 // damping matrix S_h = sum_e sigma_e J w phi_i (lumped like the mass; the
 // damping term of rho u'' + sigma u' - div(mu grad u) = f, VarHelmholtz.cxx
-// :64-66,108-109 "Glob_matMass_DhSigma", assembled by TimeAcoustic.cxx:970-1004)
+// :64-66,108-109 "Glob_matMass_DhSigma", assembled by TimeAcoustic.cxx:970-1004;
+// elastic: sigma I in the mass tensor, VarElastic.cxx:59-71, so every
+// component of a node takes the same S, assembled by TimeElastic.cxx:296-330)
 // sigma per element (nodal = false) or per quadrature point [E][nd] (nodal)
 static int set_damping(mjw_problem *p, const double *sigma, bool nodal) {
   if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_damping: physical damping with PML layers is not supported");
   if (!p || !sigma) FAIL(p, MJW_ERR_ARG, "mjw_set_damping: NULL");
-  if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_damping: acoustic problems only");
   HIPCHK(p, hipSetDevice(p->device));
   const int64_t E = p->E, N = p->N, per = nodal ? p->nd : 1;
   std::vector<double> ss((size_t)E * per);

@@ -19,7 +19,8 @@
 // Keywords read: TypeElement (QUADRANGLE_LOBATTO, HEXAHEDRON_LOBATTO),
 // TypeEquation (ACOUSTIC, TIME_ELASTIC), MeshPath, FileMesh, TypeMesh,
 // OrderDiscretization, ConditionReference (DIRICHLET), MateriauDielec
-// (ISOTROPE, acoustic damping sigma), PhysicalMedia (rho / mu / sigma, SINUS or MESH / SAME_MESH data files on the
+// (ISOTROPE, elastic also YOUNG_POISSON; damping sigma), PhysicalMedia
+// (rho / mu / sigma, SINUS or MESH / SAME_MESH data files on the
 // computational mesh, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
 // TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, FileOutputPoint, ParametersOutputPoint, SismoCircle,
@@ -991,30 +992,55 @@ std::vector<double> pml_tau(const Q1Mesh &m, const std::vector<int> &kind, const
 }
 
 struct Media {
-  std::vector<double> rho, mu, lam, sigma;  // sigma: acoustic damping (empty if none)
+  std::vector<double> rho, mu, lam, sigma;  // sigma: damping (empty if none)
 };
 
+// MateriauDielec = ref ISOTROPE rho mu [sigma]          (acoustic)
+//                  ref ISOTROPE rho lambda mu [sigma]   (elastic)
+//                  ref YOUNG_POISSON rho E nu [sigma]   (elastic)
+// read value after value as SetIndices does (VarHelmholtz / VarElastic,
+// Elliptic/Elastic/VarElastic.cxx:1692-1708): rho, then the stiffness, then
+// the damping sigma (rho u'' + sigma u' - div(...) = f), which is 0 when the
+// line ends before it (GenericPhysicalIndice::SetInputVaryingMedia,
+// Elliptic/PhysicalProperty.cxx:3614-3620).  YOUNG_POISSON gives mu =
+// E / (2 (1 + nu)) and lambda = E nu / ((1 + nu)(1 - 2 nu)) in 3-D, but
+// E nu / (1 - nu^2) in 2-D (ElasticPhysicalIndice::SetInputData,
+// PhysicalProperty.cxx:3321-3345).
 Media media(const Deck &d, const Q1Mesh &m, bool elastic) {
-  std::map<int, std::vector<double>> mat;
+  std::map<int, std::vector<double>> mat;  // rho, lambda (elastic), mu, sigma
   for (const auto &v : d.all("MateriauDielec")) {
-    if (v.size() < 5 || v[1] != "ISOTROPE") die("MateriauDielec: only ISOTROPE media are supported");
-    mat[(int)num(v[0])] = {num(v[2]), num(v[3]), num(v[4])};
+    const bool young = elastic && v.size() > 1 && v[1] == "YOUNG_POISSON";
+    if (v.size() < 2 || (v[1] != "ISOTROPE" && !young))
+      die(std::string("MateriauDielec: only ISOTROPE") + (elastic ? " and YOUNG_POISSON" : "") +
+          " media are supported");
+    const size_t nv = elastic ? 3 : 2;  // values before sigma
+    if (v.size() < 2 + nv)
+      die("MateriauDielec " + v[0] + ": " + (elastic ? (young ? "rho E nu" : "rho lambda mu")
+                                                     : "rho mu") + " expected");
+    std::vector<double> p;
+    p.push_back(num(v[2]));
+    if (young) {
+      const double E = num(v[3]), nu = num(v[4]);
+      p.push_back(m.dim == 2 ? E * nu / (1.0 - nu * nu) : E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+      p.push_back(0.5 * E / (1.0 + nu));
+    } else {
+      if (elastic) p.push_back(num(v[3]));
+      p.push_back(num(v[2 + nv - 1]));
+    }
+    const double sg = v.size() > 2 + nv ? num(v[2 + nv]) : 0.0;
+    if (!(sg >= 0.0)) die("MateriauDielec: the damping sigma must be >= 0");
+    p.push_back(sg);
+    mat[(int)num(v[0])] = p;
   }
   Media md;
   for (int ref : m.elem_ref) {
     auto it = mat.find(ref);
     if (it == mat.end()) die("no MateriauDielec for element reference " + std::to_string(ref));
     const auto &p = it->second;
-    if (elastic) {  // ISOTROPE rho lambda mu
-      md.rho.push_back(p[0]);
-      md.lam.push_back(p[1]);
-      md.mu.push_back(p[2]);
-    } else {  // ISOTROPE rho mu sigma (sigma: damping, rho u'' + sigma u' - div(mu grad u))
-      if (!(p[2] >= 0.0)) die("MateriauDielec: the damping sigma must be >= 0");
-      md.rho.push_back(p[0]);
-      md.mu.push_back(p[1]);
-      md.sigma.push_back(p[2]);
-    }
+    md.rho.push_back(p[0]);
+    if (elastic) md.lam.push_back(p[1]);
+    md.mu.push_back(p[elastic ? 2 : 1]);
+    md.sigma.push_back(p.back());
   }
   bool damped = false;
   for (double v : md.sigma) damped = damped || v > 0.0;
@@ -1620,6 +1646,8 @@ int main(int argc, char **argv) {
     wr(o, md.rho);
     wr(o, md.mu);
     wr(o, md.lam);
+    // MateriauDielec's damping sigma per element (zeros when undamped)
+    wr(o, md.sigma.empty() ? std::vector<double>((size_t)qm.E, 0.0) : md.sigma);
     wr(o, nm.rho);  // empty unless PhysicalMedia: E * nd each
     wr(o, nm.mu);
     wr(o, montjoie_dofs(q1, qm));

@@ -279,8 +279,9 @@ class WaveProblem:
 
     def set_damping(self, sigma):
         """Damping sigma per element, or [E, nd] at the quadrature points
-        (rho u'' + sigma u' - div(mu grad u) = f; mjw_set_damping[_nodal]);
-        call leapfrog_init afterwards."""
+        (rho u'' + sigma u' - div(mu grad u) = f, elastic: rho u'' + sigma u'
+        - div(C : grad u) = f; mjw_set_damping[_nodal]); call leapfrog_init
+        afterwards."""
         sg = np.asarray(sigma, np.float64)
         if sg.ndim == 2:
             sg = np.ascontiguousarray(sg)

@@ -152,7 +152,7 @@ def oracle_run(c: Case, nsteps: int | None = None):
         # every receiver on the three components, component-major
         pts = np.vstack([c.rec] * c.dim)
         comp = np.repeat(np.arange(c.dim, dtype=np.int32), len(c.rec))
-        return O.leapfrog(P, b, pulse, c.dt, nsteps, pts, comp)
+        return O.leapfrog(P, b, pulse, c.dt, nsteps, pts, comp, S=S)
     return O.leapfrog(P, b, pulse, c.dt, nsteps, c.rec, rc, S=S)
 
 
@@ -175,7 +175,8 @@ def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
     rw = np.stack([Gw[r, a] for r in range(npt) for i in range(d) for a in range(d)])
     comp = np.array([i for r in range(npt) for i in range(d) for a in range(d)], np.int32)
     pulse = O.pulse_table(c.f0, 0.0, c.dt, nsteps)
-    _, _, g = O.leapfrog(P, b, pulse, c.dt, nsteps, rec_comp=comp, rec_weights=(re, rw))
+    S = O.damping(m, c.sigma_e) if c.sigma_e is not None and np.any(c.sigma_e) else None
+    _, _, g = O.leapfrog(P, b, pulse, c.dt, nsteps, rec_comp=comp, rec_weights=(re, rw), S=S)
     g = g.reshape(nsteps + 1, npt, d, d)
     lam = c.lam_e[el][None, :, None, None]
     mu = c.mu_e[el][None, :, None, None]

@@ -104,14 +104,16 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
                mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
-               output_format="BINARY DOUBLE", circle=None, pml=None):
+               output_format="BINARY DOUBLE", circle=None, pml=None, young=None):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  circle = (x0, y0,
     radius, n): a SismoCircle line after the SismoPoint lines.  pml (pml_spec):
     AddPML = YES axes thickness layers, DampingPML (and TransverseDampingPML)
     and the first-order scheme OrderTimeScheme = 2 LEAP_FROG_PML.  sinus_rho /
     sinus_mu: SINUS fields (sinus_line layout) overriding rho / mu of medium 1.
     save_reprise > 0: checkpoint files every save_reprise steps.  sigma_layers:
-    acoustic damping of each layer (MateriauDielec's third value)."""
+    damping of each layer (MateriauDielec's value after the stiffness).
+    young = (E_layers, nu_layers): elastic media written as YOUNG_POISSON
+    rho E nu instead of ISOTROPE rho lambda mu."""
     os.makedirs(d, exist_ok=True)
     m = O.mesh(dim, nx, r, pert, seed)
     nl = len(layers)
@@ -152,8 +154,15 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
             ini.append(f"TransverseDampingPML = {_f(spec['transverse'])}")
     for k in range(nl):
         if elastic:
-            ini.append(f"MateriauDielec = {k + 1} ISOTROPE {_f(rho_layers[k])} "
+            if young is not None:
+                mat = (f"MateriauDielec = {k + 1} YOUNG_POISSON {_f(rho_layers[k])} "
+                       f"{_f(young[0][k])} {_f(young[1][k])}")
+            else:
+                mat = (f"MateriauDielec = {k + 1} ISOTROPE {_f(rho_layers[k])} "
                        f"{_f(lam_layers[k])} {_f(layers[k])}")
+            if sigma_layers is not None and len(sigma_layers):
+                mat += f" {_f(sigma_layers[k])}"
+            ini.append(mat)
         else:
             sg = 0.0 if sigma_layers is None or not len(sigma_layers) else sigma_layers[k]
             ini.append(f"MateriauDielec = {k + 1} ISOTROPE 1.0 {_f(layers[k])} {_f(sg)}")

@@ -51,14 +51,14 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
              mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
-             output_format="BINARY DOUBLE", circle=None, pml=None):
+             output_format="BINARY DOUBLE", circle=None, pml=None, young=None):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
                sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times,
                line=line, line_times=line_times, mesh_media=mesh_media,
                save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma,
-               output_format=output_format, circle=circle, pml=pml)
+               output_format=output_format, circle=circle, pml=pml, young=young)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini", str(dim)]
     for f in os.listdir(d):
@@ -123,6 +123,8 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         sinus_sigma=np.asarray(sinus_sigma if sinus_sigma is not None else [], float),
         output_format=output_format,
         circle=np.asarray(circle if circle is not None else [], float),
+        young_E=np.asarray(young[0] if young is not None else [], float),
+        young_nu=np.asarray(young[1] if young is not None else [], float),
         **_pml_fields(pml, dim, nx),
         circle_files=circles,
         circle_names=names.get("totalCircle", np.zeros(0, "U1")),
@@ -188,6 +190,43 @@ def ref2d_el_q7():
     rec = seeded_points(2, 6, 94)
     run_case("ref2d_el_q7", 2, 4, 7, 0.1, 22, mu, 8.0, 1e-3, 300, (0.52, 0.44), rec,
              elastic=True, lam_layers=lam, rho_layers=rho, pol=(-0.3, 0.9))
+
+
+def young_poisson(dim, E, nu):
+    """Lame coefficients (lambda, mu) of the YOUNG_POISSON media the deck
+    states, as Montjoie converts them (ElasticPhysicalIndice::SetInputData,
+    Elliptic/PhysicalProperty.cxx:3321-3345): plane stress in 2-D."""
+    E, nu = np.asarray(E, float), np.asarray(nu, float)
+    lam = E * nu / (1.0 - nu * nu) if dim == 2 else E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return lam, 0.5 * E / (1.0 + nu)
+
+
+@case
+def ref3d_el_q3_damped():
+    """damped isotropic elastic: 4^3 Q3 perturbed hexahedra, two layers of
+    ISOTROPE rho lambda mu sigma with sigma 2 and 9."""
+    lam = [1.0 + 3.0 * O.uniform(2042, k) for k in range(2)]
+    mu = [1.0 + 3.0 * O.uniform(2043, k) for k in range(2)]
+    rho = [1.0 + O.uniform(2044, k) for k in range(2)]
+    rec = seeded_points(3, 5, 95)
+    run_case("ref3d_el_q3_damped", 3, 4, 3, 0.1, 23, mu, 8.0, 4e-3, 150, (0.46, 0.53, 0.5), rec,
+             elastic=True, lam_layers=lam, rho_layers=rho, pol=(0.3, -0.5, 0.8),
+             sigma_layers=[2.0, 9.0])
+
+
+@case
+def ref2d_el_q4_young_damped():
+    """2-D elastic media given as YOUNG_POISSON rho E nu sigma (Montjoie's
+    plane-stress conversion to lambda, mu): 8x8 Q4 perturbed, three layers,
+    sigma 0, 4 and 15."""
+    E = [2.0 + 4.0 * O.uniform(2045, k) for k in range(3)]
+    nu = [0.1 + 0.3 * O.uniform(2046, k) for k in range(3)]
+    rho = [1.0 + O.uniform(2047, k) for k in range(3)]
+    lam, mu = young_poisson(2, E, nu)
+    rec = seeded_points(2, 6, 96)
+    run_case("ref2d_el_q4_young_damped", 2, 8, 4, 0.1, 24, list(mu), 10.0, 1e-3, 300,
+             (0.48, 0.51), rec, elastic=True, lam_layers=list(lam), rho_layers=rho,
+             pol=(0.8, 0.6), sigma_layers=[0.0, 4.0, 15.0], young=(E, nu))
 
 
 @case

@@ -42,7 +42,9 @@ def _write(c: CS.Case, d):
                      output_format=str(z["output_format"]) if "output_format" in z.files
                      else "BINARY DOUBLE",
                      circle=z["circle"] if "circle" in z.files and len(z["circle"]) else None,
-                     pml=CS.pml_deck(c) if c.pml is not None else None)
+                     pml=CS.pml_deck(c) if c.pml is not None else None,
+                     young=(z["young_E"], z["young_nu"])
+                     if "young_E" in z.files and len(z["young_E"]) else None)
 
 
 def _dump(path):
@@ -54,7 +56,8 @@ def _dump(path):
     out = {"dim": dim, "order": order, "ncomp": ncomp, "N": N, "E": E}
     for name, dt, n in (("xyz", np.float64, N * dim), ("en", np.int32, E * nd),
                         ("dir", np.uint8, N), ("rho", np.float64, E), ("mu", np.float64, E),
-                        ("lam", np.float64, E if ncomp > 1 else 0)):
+                        ("lam", np.float64, E if ncomp > 1 else 0),
+                        ("sigma", np.float64, E)):
         size = np.dtype(dt).itemsize * n
         out[name] = np.frombuffer(raw[off:off + size], dt)
         off += size
@@ -140,6 +143,12 @@ def test_driver_mesh_is_the_oracle_mesh(name, tmp_path):
         np.testing.assert_array_equal(g["rho"], c.rho_e)
     if c.elastic:
         np.testing.assert_array_equal(g["lam"], c.lam_e)
+    # MateriauDielec's damping (a PhysicalMedia sigma field overrides it at
+    # the quadrature points of its medium)
+    if c.sigma_e is None:
+        assert not g["sigma"].any()
+    elif c.sigma_e.ndim == 1:
+        np.testing.assert_array_equal(g["sigma"], c.sigma_e)
     # Montjoie's DOF numbering (SaveReprise files) equals the oracle's
     z = np.load(os.path.join(CS.GOLD, name + ".npz"))
     corners, bnd = decks.medit_topology(c.dim, c.nx)

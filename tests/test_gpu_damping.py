@@ -1,5 +1,5 @@
 """Damped leapfrog (mjw_set_damping): rho u'' + sigma u' - div(mu grad u) = f
-with S_h = sigma J w lumped like the mass.  Whole runs of the damped golden
+(elastic: - div(C : grad u)) with S_h = sigma J w lumped like the mass.  Whole runs of the damped golden
 cases against Montjoie's traces are in test_gpu_parity.py and test_deck.py
 (ref*_damped); here: non-zero initial velocity (the -S du0 term of the first
 step) against the oracle, FP32, interleaved shots, graph replay, and the
@@ -61,6 +61,52 @@ def test_damped_fp32_and_shots():
     for s, x in enumerate(srcs):
         _, ts = CS.gpu_run(c, "fp64", srcs=[x])
         assert CS.rel_l2(tm[:, s, :], ts[:, 0, :]) < 1e-13
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_g64"])
+def test_damped_elastic_shots_match_oracle(prec):
+    """Damped elastic media (sigma I in the mass tensor): 4 interleaved shots
+    of the 3-D fixture ref3d_el_q3_damped with their own sources and
+    polarisations, each against the oracle's damped run (FP64 1e-12, FP32
+    state 1e-5), after the initial velocity's -S du0 term is exercised by a
+    single-shot run from (u0, du0)."""
+    c = CS.load("ref3d_el_q3_damped")
+    rng = np.random.default_rng(31)
+    srcs = 0.3 + 0.4 * rng.random((4, 3))
+    pol = rng.standard_normal((4, 3))
+    m = M.regular_mesh(3, c.nx, c.r, c.pert, c.seed)
+    o = O.mesh(3, c.nx, c.r, c.pert, c.seed)
+    P = M.WaveProblem(m, c.rho_e, c.mu_e, lam=c.lam_e, ncomp=3, nshots=4, precision=prec)
+    P.set_damping(c.sigma_e)
+    P.set_sources(srcs, pol=pol, shot=np.arange(4))
+    rec = np.vstack([c.rec] * 3)
+    comp = np.repeat(np.arange(3, dtype=np.int32), len(c.rec))
+    P.set_receivers(rec, comp=comp)
+    n = 100
+    P.set_pulse(M.ricker_pulse_table(c.f0, 0.0, c.dt, n))
+    P.leapfrog_init(c.dt)
+    tr = P.run(n)
+    OP = O.elastic_problem(o, c.lam_e, c.mu_e, c.rho_e)
+    S = O.damping(o, c.sigma_e)
+    pulse = O.pulse_table(c.f0, 0.0, c.dt, n)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    for s in range(4):
+        _, _, otr = O.leapfrog(OP, O.dirac(o, srcs[s], pol[s]), pulse, c.dt, n, rec, comp, S=S)
+        err = CS.rel_l2(tr[:, s, :], otr)
+        print(f"damped elastic shot {s} {prec}: rel-L2 vs oracle {err:.2e}")
+        assert err < tol
+    if prec != "fp64":
+        return
+    Q = M.WaveProblem(m, c.rho_e, c.mu_e, lam=c.lam_e, ncomp=3, precision=prec)
+    Q.set_damping(c.sigma_e)
+    Q.set_receivers(rec, comp=comp)
+    u0 = 1e-2 * rng.standard_normal(3 * m.N)
+    du0 = rng.standard_normal(3 * m.N)
+    u0[np.tile(o.dirichlet.astype(bool), 3)] = 0.0
+    Q.leapfrog_init(c.dt, u0=u0, du0=du0)
+    tq = Q.run(40)[:, 0, :]
+    _, _, oq = O.leapfrog(OP, np.zeros(3 * o.N), None, c.dt, 40, rec, comp, u0=u0, du0=du0, S=S)
+    assert CS.rel_l2(tq, oq) < 1e-12
 
 
 def test_damped_graph_equals_eager():

@@ -1,6 +1,7 @@
 """The oracle pinned: against the reference's own known-answer tables and
 solutions (tests/golden, made from Montjoie's sources), and against what the
 mathematics fixes.  CPU only."""
+import dataclasses
 import json
 import os
 import sys
@@ -215,6 +216,28 @@ def test_oracle_matches_montjoie(name):
     ref = CS.reference_traces(c)
     assert tr.shape == ref.shape
     assert CS.rel_l2(tr, ref) < 1e-12
+
+
+def test_young_poisson_2d_is_plane_stress():
+    """YOUNG_POISSON media of a 2-D deck: Montjoie's traces are matched by the
+    plane-stress Lame coefficient lambda = E nu / (1 - nu^2) the fixture
+    stores, and missed by the 3-D (plane-strain) E nu / ((1 + nu)(1 - 2 nu));
+    undamping the layers (sigma 0, 4, 15) misses them too."""
+    c = CS.load("ref2d_el_q4_young_damped")
+    z = np.load(os.path.join(CS.GOLD, c.name + ".npz"))
+    E, nu = z["young_E"], z["young_nu"]
+    np.testing.assert_allclose(z["lam_layers"], E * nu / (1 - nu * nu), rtol=1e-15)
+    np.testing.assert_allclose(z["layers"], 0.5 * E / (1 + nu), rtol=1e-15)
+    ref = CS.reference_traces(c)
+    _, _, tr = CS.oracle_run(c, 150)
+    assert CS.rel_l2(tr, ref[:151]) < 1e-12
+    e = np.arange(c.nx ** c.dim)
+    lay = ((e // c.nx ** (c.dim - 1)) * len(E)) // c.nx
+    np.testing.assert_array_equal(c.lam_e, z["lam_layers"][lay])
+    strain = (E * nu / ((1 + nu) * (1 - 2 * nu)))[lay]
+    _, _, tr3 = CS.oracle_run(dataclasses.replace(c, lam_e=strain), 150)
+    _, _, tr0 = CS.oracle_run(dataclasses.replace(c, sigma_e=None), 150)
+    assert CS.rel_l2(tr3, ref[:151]) > 1e-3 and CS.rel_l2(tr0, ref[:151]) > 1e-3
 
 
 def test_lambda_max_bounds_cfl():
