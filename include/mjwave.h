@@ -211,9 +211,29 @@ int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table);
 #define MJW_TIME_RICKER 1
 int mjw_set_time_source(mjw_problem *p, int kind, double f0, double t_init, double eps_time);
 /*
+ * The same for every TemporalSource of the explicit path (GetNewTimeSource,
+ * VarInstationary.cxx:409-481; primitives Source/TimeSource.cxx), through the
+ * same interpolator and windows of 0.5/f0, with `param` the keyword's value:
+ *   MJW_TIME_RICKER            (no parameter)      cut-off 2.5/f0
+ *   MJW_TIME_DERIVATIVE_RICKER (no parameter)      cut-off 2.5/f0
+ *   MJW_TIME_MODIFIED_RICKER   centre time t0      cut-off (2.5 + t0)/(sqrt(2) f0)
+ *   MJW_TIME_GAUSSIAN          plateau width >= 0  cut-off 2.5/f0 + width
+ *   MJW_TIME_HARMONIC          final time tf       cut-off tf
+ *   MJW_TIME_SINUS_GAUSSIAN    alpha > 0           cut-off 2 sqrt(ln(1e6)/alpha)
+ * mjw_add_scalar_time_source then adds nothing from the cut-off on.
+ */
+#define MJW_TIME_DERIVATIVE_RICKER 2
+#define MJW_TIME_MODIFIED_RICKER 3
+#define MJW_TIME_GAUSSIAN 4
+#define MJW_TIME_HARMONIC 5
+#define MJW_TIME_SINUS_GAUSSIAN 6
+int mjw_set_temporal_source(mjw_problem *p, int kind, double f0, double param, double t_init,
+                            double eps_time);
+/*
  * Y += alpha * f^(nb_deriv)(t) * F_shot on one host vector (n_nodes*ncomp,
  * caller's numbering), F_shot = the sparse source vector of that shot
- * (mjw_set_sources); nothing for t >= 2.5/f0 (AddScalarTimeSource ->
+ * (mjw_set_sources); nothing from the kind's cut-off on (2.5/f0 for the
+ * Ricker; AddScalarTimeSource ->
  * AddScalarSourceAtTime, VarInstationary.cxx:2391-2420,2509-2513).
  * nb_deriv = 0 is the primitive, 1 the pulse (what Advance adds), up to 9.
  * Calls must come with non-decreasing t, as in the reference (the windows
@@ -418,6 +438,11 @@ int mjw_set_pml(mjw_problem *p, const uint8_t *pml, const double *tau);
  */
 int mjw_time_source_table(double f0, double t_begin, double dt, int nsteps, int nb_deriv,
                           double offset, double *out);
+/* The same table for any MJW_TIME_* kind and its parameter (see
+ * mjw_set_temporal_source), zero from the kind's cut-off on: the leapfrog's
+ * mjw_set_pulse table is nb_deriv = 1, offset = dt.  Host only. */
+int mjw_temporal_source_table(int kind, double f0, double param, double t_begin, double dt,
+                              int nsteps, int nb_deriv, double offset, double *out);
 
 /* --------------------------------------------------------- synthetic meshes */
 /*

@@ -141,11 +141,17 @@ class HipWaveSystem {
   // device loop and the time function of AddScalarTimeSource (window
   // tolerance (final - initial time) * 1e-15, VarInstationary.cxx:993)
   void SetRickerPulse(double f0, double t_begin, double dt, int nsteps) {
+    SetTemporalSource(MJW_TIME_RICKER, f0, 0.0, t_begin, dt, nsteps);
+  }
+  // any TemporalSource kind (MJW_TIME_*) with its value, for both
+  void SetTemporalSource(int kind, double f0, double param, double t_begin, double dt,
+                         int nsteps) {
     std::vector<double> tab(nsteps);
-    check(mjw_ricker_pulse_table(f0, t_begin, dt, nsteps, tab.data()), nullptr, "Ricker");
-    check(mjw_set_pulse(p_, nsteps, tab.data()), p_, "SetRickerPulse");
-    check(mjw_set_time_source(p_, MJW_TIME_RICKER, f0, t_begin, (nsteps * dt) * 1e-15), p_,
-          "SetRickerPulse");
+    check(mjw_temporal_source_table(kind, f0, param, t_begin, dt, nsteps, 1, dt, tab.data()),
+          nullptr, "TemporalSource");
+    check(mjw_set_pulse(p_, nsteps, tab.data()), p_, "SetTemporalSource");
+    check(mjw_set_temporal_source(p_, kind, f0, param, t_begin, (nsteps * dt) * 1e-15), p_,
+          "SetTemporalSource");
   }
   void SetReceivers(const std::vector<double> &pts, int dim, const std::vector<int32_t> &comp = {}) {
     const int nrec = (int)(pts.size() / dim);

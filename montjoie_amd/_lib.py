@@ -47,6 +47,8 @@ SIGNATURES = {
     "mjw_solve_dh_plus_dt_sh": (C.c_int, [_vp, C.c_double, _dp]),
     "mjw_set_dirichlet_condition": (C.c_int, [_vp, _dp]),
     "mjw_set_time_source": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double]),
+    "mjw_set_temporal_source": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                          C.c_double]),
     "mjw_add_scalar_time_source": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, _dp]),
     "mjw_get_mass": (C.c_int, [_vp, _dp]),
     "mjw_set_sources": (C.c_int, [_vp, C.c_int, _dp, _dp, _i32p]),
@@ -58,6 +60,8 @@ SIGNATURES = {
     "mjw_set_pml": (C.c_int, [_vp, _u8p, _dp]),
     "mjw_time_source_table": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                         C.c_double, _dp]),
+    "mjw_temporal_source_table": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, C.c_int, C.c_int, C.c_double, _dp]),
     "mjw_set_damping_nodal": (C.c_int, [_vp, _dp]),
     "mjw_set_probes_grad": (C.c_int, [_vp, C.c_int, _dp, _i32p, _u8p, _i64p]),
     "mjw_leapfrog_init": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _dp]),

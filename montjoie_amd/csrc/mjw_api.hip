@@ -253,19 +253,23 @@ extern "C" int mjw_ricker_pulse_table(double f0, double t_begin, double dt, int 
   return MJW_OK;
 }
 
+extern "C" int mjw_temporal_source_table(int kind, double f0, double param, double t_begin,
+                                         double dt, int nsteps, int nb_deriv, double offset,
+                                         double *out) {
+  if (!(f0 > 0) || !(dt > 0) || nsteps < 0 || (!out && nsteps > 0) || nb_deriv < 0 ||
+      nb_deriv >= GlTimeSource::kNDer || !std::isfinite(offset) || !std::isfinite(t_begin))
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_temporal_source_table: bad arguments");
+  if (std::isnan(time_source_limit(kind, f0, param)))
+    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG,
+         "mjw_temporal_source_table: unknown kind or a parameter it cannot take");
+  source_table(kind, f0, param, t_begin, dt, nsteps, nb_deriv, offset, out);
+  return MJW_OK;
+}
+
 extern "C" int mjw_time_source_table(double f0, double t_begin, double dt, int nsteps,
                                      int nb_deriv, double offset, double *out) {
-  if (!(f0 > 0) || !(dt > 0) || nsteps < 0 || (!out && nsteps > 0) || nb_deriv < 0 ||
-      nb_deriv >= GlTimeSource::kNDer || !std::isfinite(offset))
-    FAIL((mjw_problem *)nullptr, MJW_ERR_ARG, "mjw_time_source_table: bad arguments");
-  GlTimeSource src;
-  src.init(f0, t_begin, (nsteps * dt) * 1e-15);
-  const double tcut = 2.5 / f0;
-  for (int n = 0; n < nsteps; ++n) {
-    const double t = (t_begin + n * dt) + offset;
-    out[n] = t >= tcut ? 0.0 : src.eval(t, nb_deriv);
-  }
-  return MJW_OK;
+  return mjw_temporal_source_table(MJW_TIME_RICKER, f0, 0.0, t_begin, dt, nsteps, nb_deriv,
+                                   offset, out);
 }
 
 // --------------------------------------------------------------- lifecycle
@@ -1107,20 +1111,31 @@ extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, cons
   return MJW_OK;
 }
 
-extern "C" int mjw_set_time_source(mjw_problem *p, int kind, double f0, double t_init,
-                                   double eps_time) {
-  if (!p) FAIL(p, MJW_ERR_ARG, "mjw_set_time_source: NULL");
+extern "C" int mjw_set_temporal_source(mjw_problem *p, int kind, double f0, double param,
+                                       double t_init, double eps_time) {
+  if (!p) FAIL(p, MJW_ERR_ARG, "mjw_set_temporal_source: NULL");
   if (kind == MJW_TIME_NONE) {
     p->has_tsrc = false;
     return MJW_OK;
   }
-  if (kind != MJW_TIME_RICKER) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_time_source: unknown kind");
+  if (kind < MJW_TIME_RICKER || kind > MJW_TIME_SINUS_GAUSSIAN)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_temporal_source: unknown kind");
   if (!(f0 > 0.0) || !std::isfinite(f0) || !std::isfinite(t_init) || !(eps_time >= 0.0))
-    FAIL(p, MJW_ERR_ARG, "mjw_set_time_source: bad f0, t_init or eps");
-  p->tsrc.init(f0, t_init, eps_time);
-  p->tlimit = 2.5 / f0;
+    FAIL(p, MJW_ERR_ARG, "mjw_set_temporal_source: bad f0, t_init or eps");
+  const double tl = time_source_limit(kind, f0, param);
+  if (std::isnan(tl)) FAIL(p, MJW_ERR_ARG, "mjw_set_temporal_source: bad parameter for this kind");
+  p->tsrc.init(f0, t_init, eps_time, kind, param);
+  p->tlimit = tl;
   p->has_tsrc = true;
   return MJW_OK;
+}
+
+extern "C" int mjw_set_time_source(mjw_problem *p, int kind, double f0, double t_init,
+                                   double eps_time) {
+  if (p && kind != MJW_TIME_NONE && kind != MJW_TIME_RICKER)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_time_source: Ricker only (mjw_set_temporal_source "
+                                 "takes the other kinds and their parameter)");
+  return mjw_set_temporal_source(p, kind, f0, 0.0, t_init, eps_time);
 }
 
 extern "C" int mjw_add_scalar_time_source(mjw_problem *p, int shot, double alpha, double t,

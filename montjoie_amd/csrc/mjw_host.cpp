@@ -4,6 +4,7 @@
 // reference.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstring>
 
 #include "mjw_internal.hpp"
@@ -122,14 +123,17 @@ void lagrange(int order, const double *x, double t, double *phi, double *dphi) {
 }
 
 // ---------------------------------------------------------------------------
-// Time function of the sources: the Ricker primitive (TimeRickerSource::
-// Evaluate, Source/TimeSource.cxx:33-51) interpolated by the degree-24 GL
+// Time function of the sources: the primitive of the chosen pulse (the
+// Evaluate of the VirtualTimeSource that GetNewTimeSource builds,
+// Instationary/VarInstationary.cxx:409-481) interpolated by the degree-24 GL
 // Lagrange polynomial on windows of length 0.5/f0 that move forward with t,
 // derivatives by the GL differentiation matrix applied nb_deriv times
 // (TimeSourceHyperbolic::Init / GenerateValues / EvaluateDerivative,
 // Source/TimeSource.cxx:818-880).  nb_deriv = 1 is the pulse itself.
 // ---------------------------------------------------------------------------
-void GlTimeSource::init(double freq, double t_init, double eps_time) {
+void GlTimeSource::init(double freq, double t_init, double eps_time, int kind_, double par_) {
+  kind = kind_;
+  par = par_;
   f0 = freq;
   eps = eps_time;
   t0 = t_init;
@@ -149,12 +153,64 @@ void GlTimeSource::init(double freq, double t_init, double eps_time) {
   generate();
 }
 
+// The Evaluate of each pulse (Source/TimeSource.cxx): kind 1 RICKER, the
+// Ricker primitive (:33-51); 2 DERIVATIVE_RICKER, the Ricker itself (:55-67,
+// 121-124); 3 MODIFIED_RICKER centred at par (:152-160); 4 GAUSSIAN with a
+// plateau of width par after one period (:198-213); 5 HARMONIC ending at par
+// (:597-611); 6 SINUS_GAUSSIAN of parameter par centred half way to its
+// cut-off (:667-679, VarInstationary.cxx:479-481).
 double GlTimeSource::primitive(double t) const {
   const double pi = 3.14159265358979323846;
-  const double T0 = 1.0 / f0;
-  if (t <= -0.9 * T0 || t >= 2.9 * T0) return 0.0;
-  const double a = f0 * t - 1.0;
-  return -a * std::exp(-(a * a) * (pi * pi)) / f0;
+  switch (kind) {
+    case 1:
+    case 2: {
+      const double T0 = 1.0 / f0;
+      if (t <= -0.9 * T0 || t >= 2.9 * T0) return 0.0;
+      const double a = f0 * t - 1.0;
+      if (kind == 1) return -a * std::exp(-(a * a) * (pi * pi)) / f0;
+      const double q = (a * pi) * (a * pi);
+      return (2.0 * q - 1.0) * std::exp(-q);
+    }
+    case 3: {
+      const double s = t - par, a = f0 * s;
+      return -s * std::exp(-2.0 * ((a * pi) * (a * pi)));
+    }
+    case 4: {
+      const double T0 = 1.0 / f0;
+      double s;
+      if (t < T0) s = t;
+      else if (t >= T0 + par) s = t - par;
+      else return 1.0;
+      const double a = pi * (f0 * s - 1.0);
+      return std::exp(-(a * a));
+    }
+    case 5: {
+      if (t <= 0.0 || t >= par) return 0.0;
+      const double a = f0 * f0 * t * t, b = f0 * f0 * (t - par) * (t - par);
+      return std::exp(-1.0 / (pi * a)) * std::sin(2.0 * pi * f0 * t) * std::exp(-1.0 / (pi * b));
+    }
+    case 6: {
+      const double toff = 0.5 * time_source_limit(6, f0, par), s = t - toff;
+      return std::sin(2.0 * pi * f0 * t) * std::exp(-par * (s * s));
+    }
+    default:
+      return 0.0;
+  }
+}
+
+// tlimit_source of GetNewTimeSource (VarInstationary.cxx:418-481)
+double time_source_limit(int kind, double f0, double par) {
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  if (!(f0 > 0.0) || !std::isfinite(f0) || !std::isfinite(par)) return nan;
+  switch (kind) {
+    case 1:
+    case 2: return 2.5 / f0;
+    case 3: return (2.5 + par) / (std::sqrt(2.0) * f0);
+    case 4: return par >= 0.0 ? 2.5 / f0 + par : nan;
+    case 5: return par;
+    case 6: return par > 0.0 ? 2.0 * std::sqrt(std::log(1e6) / par) : nan;
+    default: return nan;
+  }
 }
 
 void GlTimeSource::generate() {
@@ -193,12 +249,17 @@ double GlTimeSource::eval(double t, int nb_deriv) {
 // (final - initial time) * 1e-15 (VarInstationary.cxx:993), zero from the
 // cut-off 2.5/f0 on (VarInstationary.cxx:416-420, 2394).
 void ricker_pulse_table(double f0, double t_begin, double dt, int nsteps, double *out) {
+  source_table(1, f0, 0.0, t_begin, dt, nsteps, 1, dt, out);
+}
+
+void source_table(int kind, double f0, double par, double t_begin, double dt, int nsteps,
+                  int nb_deriv, double offset, double *out) {
   GlTimeSource src;
-  src.init(f0, t_begin, (nsteps * dt) * 1e-15);
-  const double tcut = 2.5 / f0;
+  src.init(f0, t_begin, (nsteps * dt) * 1e-15, kind, par);
+  const double tcut = time_source_limit(kind, f0, par);
   for (int n = 0; n < nsteps; ++n) {
-    const double t = (t_begin + n * dt) + dt;
-    out[n] = t >= tcut ? 0.0 : src.eval(t, 1);
+    const double t = (t_begin + n * dt) + offset;
+    out[n] = t >= tcut ? 0.0 : src.eval(t, nb_deriv);
   }
 }
 

@@ -19,21 +19,29 @@ void gl_deriv(int order, const double *x, double *D);
 void lagrange(int order, const double *x, double t, double *phi, double *dphi);
 // Ricker pulse table through the degree-24 GL interpolator.
 void ricker_pulse_table(double f0, double t_begin, double dt, int nsteps, double *out);
+// out[n] = f^(nb_deriv)(t_begin + n dt + offset), 0 from the cut-off on
+void source_table(int kind, double f0, double par, double t_begin, double dt, int nsteps,
+                  int nb_deriv, double offset, double *out);
 // Stateful Ricker time source: its degree-24 GL interpolant on windows of
 // length 0.5/f0 starting at t_init, which move forward with the evaluation
 // time; eval(t, k) = k-th derivative of the interpolated primitive (k = 1:
 // the pulse), 0 <= k < kNDer.
 struct GlTimeSource {
   static constexpr int kR = 24, kNP = kR + 1, kNDer = 10;
+  int kind = 1;        // MJW_TIME_* of mjwave.h
+  double par = 0.0;    // the kind's parameter (centre time, width, final time, alpha)
   double f0 = 1.0, t0 = 0.0, t1 = -1.0, eps = 0.0;
   bool ready = false;
   double xg[kNP], Dg[kNP * kNP], cinv[kNP];
   double val[kNDer][kNP];
-  void init(double freq, double t_init, double eps_time);
+  void init(double freq, double t_init, double eps_time, int kind_ = 1, double par_ = 0.0);
   double primitive(double t) const;
   void generate();
   double eval(double t, int nb_deriv);
 };
+// the kind's cut-off time (the source is 0 from it on), or NaN for an
+// unknown kind or a parameter it cannot take
+double time_source_limit(int kind, double f0, double par);
 
 // ----------------------------------------------------------------- host mesh
 void mesh_regular_size(int dim, int nx, int order, int64_t *N, int64_t *E);

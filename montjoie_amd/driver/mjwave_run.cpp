@@ -22,7 +22,8 @@
 // (ISOTROPE, elastic also YOUNG_POISSON; damping sigma), PhysicalMedia
 // (rho / mu / sigma, SINUS or MESH / SAME_MESH data files on the
 // computational mesh, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
-// TemporalSource (RICKER), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
+// TemporalSource (RICKER, DERIVATIVE_RICKER, MODIFIED_RICKER, GAUSSIAN, HARMONIC,
+// SINUS_GAUSSIAN), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
 // (2 LEAP_FROG), SismoPoint, FileOutputPoint, ParametersOutputPoint, SismoCircle,
 // FileOutputCircle, ParametersOutputCircle, SismoPlane, FileOutputPlane, ParametersOutputPlane,
 // SismoLine, FileOutputLine, ParametersOutputLine, OutputFormat (BINARY or ASCII, DOUBLE or FLOAT),
@@ -1581,8 +1582,25 @@ int main(int argc, char **argv) {
   }
   if (const auto *s = d.get("TypeSource"))
     if ((*s)[0] != "SRC_DIRAC") die("TypeSource " + (*s)[0] + ": only SRC_DIRAC");
-  if (const auto *s = d.get("TemporalSource"))
-    if ((*s)[0] != "RICKER") die("TemporalSource " + (*s)[0] + ": only RICKER");
+  // TemporalSource = KIND [value] (GetNewTimeSource, VarInstationary.cxx:409-481)
+  int tkind = MJW_TIME_RICKER;
+  double tpar = 0.0;
+  if (const auto *s = d.get("TemporalSource")) {
+    const std::string k = s->empty() ? std::string() : (*s)[0];
+    auto par = [&](const char *what) {
+      if (s->size() < 2) die("TemporalSource = " + k + " " + what + ": the value is missing");
+      return num((*s)[1]);
+    };
+    if (k == "RICKER") tkind = MJW_TIME_RICKER;
+    else if (k == "DERIVATIVE_RICKER") tkind = MJW_TIME_DERIVATIVE_RICKER;
+    else if (k == "MODIFIED_RICKER") { tkind = MJW_TIME_MODIFIED_RICKER; tpar = par("t0"); }
+    else if (k == "GAUSSIAN") { tkind = MJW_TIME_GAUSSIAN; tpar = s->size() > 1 ? num((*s)[1]) : 0.0; }
+    else if (k == "HARMONIC") { tkind = MJW_TIME_HARMONIC; tpar = par("tf"); }
+    else if (k == "SINUS_GAUSSIAN") { tkind = MJW_TIME_SINUS_GAUSSIAN; tpar = par("alpha"); }
+    else
+      die("TemporalSource " + k + ": only RICKER, DERIVATIVE_RICKER, MODIFIED_RICKER, GAUSSIAN, "
+          "HARMONIC and SINUS_GAUSSIAN");
+  }
   if (const auto *s = d.get("DisplayStress"))
     if (!s->empty() && (*s)[0] == "NO") die("DisplayStress = NO: only the stress output is written");
   if (const auto *s = d.get("LoadReprise"))
@@ -1716,11 +1734,11 @@ int main(int argc, char **argv) {
     check(mjw_set_sources(p, 1, x, pol.data(), nullptr), p, "source");
   }
   std::vector<double> tab(nsteps);
-  if (first_order)  // the primitive at t_n + dt/2 (ModifiedEquation.cxx:784-785)
-    check(mjw_time_source_table(f0, t0, dt, (int)nsteps, 0, 0.5 * dt, tab.data()), nullptr,
-          "Ricker");
-  else
-    check(mjw_ricker_pulse_table(f0, t0, dt, (int)nsteps, tab.data()), nullptr, "Ricker");
+  // the pulse at t_n + dt (ModifiedEquation.cxx:316); the first-order scheme
+  // takes the primitive at t_n + dt/2 (ModifiedEquation.cxx:784-785)
+  check(mjw_temporal_source_table(tkind, f0, tpar, t0, dt, (int)nsteps, first_order ? 0 : 1,
+                                  first_order ? 0.5 * dt : dt, tab.data()),
+        nullptr, "TemporalSource");
   check(mjw_set_pulse(p, (int)nsteps, tab.data()), p, "pulse");
   // receivers: SismoPoint (SismoCircle is a snapshot grid, below)
   std::vector<double> pts;

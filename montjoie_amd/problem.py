@@ -88,6 +88,24 @@ def time_source_table(f0: float, t_begin: float, dt: float, nsteps: int, nb_deri
     return out
 
 
+# TemporalSource kinds (MJW_TIME_* of mjwave.h) by Montjoie keyword
+TEMPORAL_SOURCES = {"RICKER": 1, "DERIVATIVE_RICKER": 2, "MODIFIED_RICKER": 3, "GAUSSIAN": 4,
+                    "HARMONIC": 5, "SINUS_GAUSSIAN": 6}
+
+
+def temporal_source_table(kind, f0: float, param: float, t_begin: float, dt: float, nsteps: int,
+                          nb_deriv: int = 1, offset: float | None = None) -> np.ndarray:
+    """f^(nb_deriv)(t_begin + n dt + offset) of a TemporalSource (kind: a
+    keyword of TEMPORAL_SOURCES or its number; param: the keyword's value,
+    mjw_temporal_source_table); offset defaults to dt, the leapfrog's table."""
+    k = TEMPORAL_SOURCES[kind] if isinstance(kind, str) else int(kind)
+    out = np.zeros(nsteps)
+    off = dt if offset is None else offset
+    check(_lib.lib().mjw_temporal_source_table(k, f0, param, t_begin, dt, nsteps, nb_deriv, off,
+                                               dptr(out)), None, "temporal_source_table")
+    return out
+
+
 class WaveProblem:
     """One acoustic (ncomp=1) or elastic (ncomp = dimension) problem on one
     device."""
@@ -246,10 +264,13 @@ class WaveProblem:
         self._chk(_lib.lib().mjw_set_dirichlet_condition(self._h, dptr(Y)), "set_dirichlet")
         return Y
 
-    def set_time_source(self, f0: float, t_init: float = 0.0, eps_time: float = 0.0):
-        """Ricker time function of add_scalar_time_source (mjw_set_time_source)."""
-        self._chk(_lib.lib().mjw_set_time_source(self._h, _lib.MJW_TIME_RICKER, f0, t_init,
-                                                 eps_time), "set_time_source")
+    def set_time_source(self, f0: float, t_init: float = 0.0, eps_time: float = 0.0,
+                        kind="RICKER", param: float = 0.0):
+        """Time function of add_scalar_time_source: a TemporalSource keyword
+        (or MJW_TIME_* number) and its value (mjw_set_temporal_source)."""
+        k = TEMPORAL_SOURCES[kind] if isinstance(kind, str) else int(kind)
+        self._chk(_lib.lib().mjw_set_temporal_source(self._h, k, f0, param, t_init, eps_time),
+                  "set_time_source")
 
     def add_scalar_time_source(self, t: float, Y, alpha=1.0, nb_deriv=1, shot=0) -> np.ndarray:
         """Y + alpha f^(nb_deriv)(t) F_shot (AddScalarTimeSource)."""

@@ -1295,6 +1295,110 @@ void or_time_table(double f0, double t_begin, double dt, int nsteps, int nb_deri
 }
 
 /* ------------------------------------------------------------------------ */
+/* Every TemporalSource kind of the explicit path                           */
+/* ------------------------------------------------------------------------ */
+
+/* cut-off time of each kind (tlimit_source, GetNewTimeSource,
+ * Instationary/VarInstationary.cxx:409-481): 1 RICKER, 2 DERIVATIVE_RICKER,
+ * 3 MODIFIED_RICKER (par = centre time), 4 GAUSSIAN (par = plateau width),
+ * 5 HARMONIC (par = final time), 6 SINUS_GAUSSIAN (par = alpha) */
+double or_source_limit(int kind, double f0, double par) {
+  switch (kind) {
+    case 1: case 2: return 2.5 / f0;
+    case 3: return (2.5 + par) / (sqrt(2.0) * f0);
+    case 4: return 2.5 / f0 + par;
+    case 5: return par;
+    case 6: return 2.0 * sqrt(log(1e6) / par);
+  }
+  return 0.0;
+}
+
+/* the function each kind interpolates (VirtualTimeSource::Evaluate of
+ * Source/TimeSource.cxx: TimeRickerSource :33-51, DerivativeTimeRickerSource
+ * :121-124 -> :55-67, TimeModifiedRickerSource :152-160, TimeGaussianSource
+ * :198-213 with t_begin 0, TimeHarmonicSource :597-611,
+ * TimeSinusGaussianSource :667-679 with toff half the cut-off) */
+static double source_primitive(int kind, double f0, double par, double t) {
+  const double pi = 3.14159265358979323846;
+  if (kind == 1) return ricker_primitive(f0, t);
+  if (kind == 2) return or_ricker(f0, t);
+  if (kind == 3) {
+    double u = f0 * (t - par);
+    return -(t - par) * exp(-2.0 * (u * pi) * (u * pi));
+  }
+  if (kind == 4) {
+    double T0 = 1.0 / f0, s;
+    if (t < T0) s = t;
+    else if (t >= T0 + par) s = t - par;
+    else return 1.0;
+    double u = pi * (f0 * s - 1.0);
+    return exp(-u * u);
+  }
+  if (kind == 5) {
+    if (t <= 0.0 || t >= par) return 0.0;
+    double a = f0 * f0 * t * t, b = f0 * f0 * (t - par) * (t - par);
+    return exp(-1.0 / (pi * a)) * sin(2.0 * pi * f0 * t) * exp(-1.0 / (pi * b));
+  }
+  if (kind == 6) {
+    double toff = 0.5 * or_source_limit(6, f0, par), s = t - toff;
+    return sin(2.0 * pi * f0 * t) * exp(-par * (s * s));
+  }
+  return 0.0;
+}
+
+/*
+ * or_time_table for any kind: out[n] = f^(nb_deriv)(t_begin + n*dt +
+ * offset), the GL-24 interpolant of the kind's function on windows of
+ * 0.5/f0 (TimeSourceHyperbolic, Source/TimeSource.cxx:818-880), zero from
+ * the kind's cut-off on (AddScalarSourceAtTime, VarInstationary.cxx:2394).
+ * nb_deriv <= 3.
+ */
+void or_source_table(int kind, double f0, double par, double t_begin, double dt, int nsteps,
+                     int nb_deriv, double offset, double *out) {
+  const int R = 24, NP = 25;
+  double x[OR_MAXN1D], w[OR_MAXN1D], D[OR_MAXN1D * OR_MAXN1D], c[OR_MAXN1D];
+  or_gl(R, x, w);
+  or_gl_deriv(R, x, D);
+  cte_phi(R, x, c);
+  double eps = (nsteps * dt) * 1e-15;
+  double tlimit = or_source_limit(kind, f0, par);
+  double t0 = t_begin, t1 = t0 + 0.5 / f0;
+  double V[4][OR_MAXN1D];
+  if (nb_deriv < 0 || nb_deriv > 3) return;
+  int fresh = 1;
+  for (int n = 0; n < nsteps; n++) {
+    double ts = (t_begin + n * dt) + offset;
+    if (ts >= tlimit) { out[n] = 0.0; continue; }
+    if ((ts > t1 + eps) && (t1 > t0)) {
+      while (ts > t1 + eps) {
+        double tnext = t1 + (t1 - t0);
+        t0 = t1; t1 = tnext;
+      }
+      fresh = 1;
+    }
+    if (fresh) {
+      for (int i = 0; i < NP; i++)
+        V[0][i] = source_primitive(kind, f0, par, (1.0 - x[i]) * t0 + x[i] * t1);
+      for (int d = 1; d <= nb_deriv; d++)
+        for (int j = 0; j < NP; j++) {
+          double s = 0.0;
+          for (int k = 0; k < NP; k++) s += D[j * NP + k] * V[d - 1][k];
+          V[d][j] = s * (1.0 / (t1 - t0));
+        }
+      fresh = 0;
+    }
+    double p = 0.0, lambda = (ts - t0) / (t1 - t0);
+    for (int i = 0; i < NP; i++) {
+      double ph = 1.0;
+      for (int k = 0; k < NP; k++)
+        if (k != i) ph *= lambda - x[k];
+      p += (ph * c[i]) * V[nb_deriv][i];
+    }
+    out[n] = p;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* First-order split system with PML layers (LEAP_FROG_PML)                 */
 /* ------------------------------------------------------------------------ */
 

@@ -66,6 +66,10 @@ def lib():
     L.or_ricker.argtypes = [C.c_double, C.c_double]
     L.or_ricker.restype = C.c_double
     L.or_pulse_table.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+    L.or_source_table.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_int, C.c_int, C.c_double, _dp]
+    L.or_source_limit.argtypes = [C.c_int, C.c_double, C.c_double]
+    L.or_source_limit.restype = C.c_double
     L.or_dirac.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _ip, C.c_long, _bp, _dp, C.c_int,
                            _dp, _dp]
     L.or_dirac.restype = C.c_int
@@ -125,6 +129,27 @@ def phi(r: int, i: int, t: float) -> float:
 
 def ricker(f0: float, t: float) -> float:
     return lib().or_ricker(f0, t)
+
+
+SOURCE_KINDS = {"RICKER": 1, "DERIVATIVE_RICKER": 2, "MODIFIED_RICKER": 3, "GAUSSIAN": 4,
+                "HARMONIC": 5, "SINUS_GAUSSIAN": 6}
+
+
+def source_table(kind, f0: float, par: float, t_begin: float, dt: float, nsteps: int,
+                 nb_deriv: int = 1, offset: float | None = None) -> np.ndarray:
+    """or_source_table: the TemporalSource `kind` (keyword or number) with its
+    value `par`; nb_deriv = 1 and offset = dt (default) is the leapfrog's
+    pulse table, mjw_oracle.c:or_source_table."""
+    k = SOURCE_KINDS[kind] if isinstance(kind, str) else int(kind)
+    out = np.zeros(nsteps)
+    lib().or_source_table(k, f0, par, t_begin, dt, nsteps, nb_deriv,
+                          dt if offset is None else offset, out)
+    return out
+
+
+def source_limit(kind, f0: float, par: float) -> float:
+    k = SOURCE_KINDS[kind] if isinstance(kind, str) else int(kind)
+    return lib().or_source_limit(k, f0, par)
 
 
 def pulse_table(f0: float, t_begin: float, dt: float, nsteps: int) -> np.ndarray:

@@ -37,6 +37,12 @@ class Case:
     # AddPML + LEAP_FROG_PML: (thickness, layers, vsigma, axes, profile, offset,
     # transverse); layers = the count used (AUTO decks: the resolved count)
     pml: tuple | None = None
+    # the deck's TemporalSource line: a kind and its value
+    temporal: str = "RICKER"
+
+    def source_kind(self) -> tuple:
+        w = self.temporal.split()
+        return w[0], float(w[1]) if len(w) > 1 else 0.0
 
 
 def golden_names():
@@ -94,7 +100,25 @@ def load(name: str) -> Case:
         pml = (p[0], int(p[1]), p[2], axes, int(p[3]), p[4], p[5])
     return Case(name, dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]), elastic, mu_e,
                 rho_e, lam_e, float(z["f0"]), float(z["dt"]), int(z["nsteps"]), z["src"],
-                z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e, pml)
+                z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e, pml,
+                str(z["temporal"]) if "temporal" in z.files else "RICKER")
+
+
+def oracle_pulse(c: Case, nsteps: int) -> np.ndarray:
+    """The oracle's pulse table of the case's TemporalSource."""
+    kind, par = c.source_kind()
+    if kind == "RICKER":
+        return O.pulse_table(c.f0, 0.0, c.dt, nsteps)
+    return O.source_table(kind, c.f0, par, 0.0, c.dt, nsteps)
+
+
+def gpu_pulse(c: Case, nsteps: int) -> np.ndarray:
+    """The library's pulse table of the case's TemporalSource."""
+    import montjoie_amd as M
+    kind, par = c.source_kind()
+    if kind == "RICKER":
+        return M.ricker_pulse_table(c.f0, 0.0, c.dt, nsteps)
+    return M.temporal_source_table(kind, c.f0, par, 0.0, c.dt, nsteps)
 
 
 def pml_setup(c: Case):
@@ -146,7 +170,7 @@ def oracle_run(c: Case, nsteps: int | None = None):
         P = O.acoustic_problem(m, c.mu_e, c.rho_e)
         b = O.dirac(m, c.src)
         rc = None
-    pulse = O.pulse_table(c.f0, 0.0, c.dt, nsteps)
+    pulse = oracle_pulse(c, nsteps)
     S = O.damping(m, c.sigma_e) if c.sigma_e is not None and np.any(c.sigma_e) else None
     if c.elastic:
         # every receiver on the three components, component-major
@@ -174,7 +198,7 @@ def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
     re = np.repeat(el, d * d)
     rw = np.stack([Gw[r, a] for r in range(npt) for i in range(d) for a in range(d)])
     comp = np.array([i for r in range(npt) for i in range(d) for a in range(d)], np.int32)
-    pulse = O.pulse_table(c.f0, 0.0, c.dt, nsteps)
+    pulse = oracle_pulse(c, nsteps)
     S = O.damping(m, c.sigma_e) if c.sigma_e is not None and np.any(c.sigma_e) else None
     _, _, g = O.leapfrog(P, b, pulse, c.dt, nsteps, rec_comp=comp, rec_weights=(re, rw), S=S)
     g = g.reshape(nsteps + 1, npt, d, d)
@@ -265,7 +289,7 @@ def gpu_run(c: Case, precision="fp64", nsteps=None, graph=True, nshots=1, srcs=N
         # first-order system: the primitive at t_n + dt/2
         P.set_pulse(M.time_source_table(c.f0, 0.0, c.dt, nsteps, 0, 0.5 * c.dt))
     else:
-        P.set_pulse(M.ricker_pulse_table(c.f0, 0.0, c.dt, nsteps))
+        P.set_pulse(gpu_pulse(c, nsteps))
     P.leapfrog_init(c.dt)
     tr = P.run(nsteps, graph=graph)
     return P, tr

@@ -104,7 +104,8 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
                lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
                mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
-               output_format="BINARY DOUBLE", circle=None, pml=None, young=None):
+               output_format="BINARY DOUBLE", circle=None, pml=None, young=None,
+               temporal="RICKER"):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  circle = (x0, y0,
     radius, n): a SismoCircle line after the SismoPoint lines.  pml (pml_spec):
     AddPML = YES axes thickness layers, DampingPML (and TransverseDampingPML)
@@ -113,7 +114,8 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     save_reprise > 0: checkpoint files every save_reprise steps.  sigma_layers:
     damping of each layer (MateriauDielec's value after the stiffness).
     young = (E_layers, nu_layers): elastic media written as YOUNG_POISSON
-    rho E nu instead of ISOTROPE rho lambda mu."""
+    rho E nu instead of ISOTROPE rho lambda mu.  temporal: the TemporalSource
+    line's value (a kind and its parameter)."""
     os.makedirs(d, exist_ok=True)
     m = O.mesh(dim, nx, r, pert, seed)
     nl = len(layers)
@@ -207,7 +209,7 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     ini.append(f"TimeInterval = 0.0 {_f(T)}")
     ini.append("OrderTimeScheme = 2 " + ("LEAP_FROG_PML" if pml is not None and len(pml)
                                            else "LEAP_FROG"))
-    ini.append("TemporalSource = RICKER")
+    ini.append(f"TemporalSource = {temporal or 'RICKER'}")
     ini.append("PrintLevel = 0")
     with open(os.path.join(d, "case.ini"), "w") as fh:
         fh.write("\n".join(ini) + "\n")

@@ -51,14 +51,16 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
              lam_layers=None, rho_layers=None, pol=None, regular=False, sinus_rho=None,
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
              mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
-             output_format="BINARY DOUBLE", circle=None, pml=None, young=None):
+             output_format="BINARY DOUBLE", circle=None, pml=None, young=None,
+             temporal="RICKER"):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
                sinus_rho=sinus_rho, sinus_mu=sinus_mu, plane=plane, plane_times=plane_times,
                line=line, line_times=line_times, mesh_media=mesh_media,
                save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma,
-               output_format=output_format, circle=circle, pml=pml, young=young)
+               output_format=output_format, circle=circle, pml=pml, young=young,
+               temporal=temporal)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini", str(dim)]
     for f in os.listdir(d):
@@ -125,6 +127,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         circle=np.asarray(circle if circle is not None else [], float),
         young_E=np.asarray(young[0] if young is not None else [], float),
         young_nu=np.asarray(young[1] if young is not None else [], float),
+        temporal=temporal,
         **_pml_fields(pml, dim, nx),
         circle_files=circles,
         circle_names=names.get("totalCircle", np.zeros(0, "U1")),
@@ -227,6 +230,42 @@ def ref2d_el_q4_young_damped():
     run_case("ref2d_el_q4_young_damped", 2, 8, 4, 0.1, 24, list(mu), 10.0, 1e-3, 300,
              (0.48, 0.51), rec, elastic=True, lam_layers=list(lam), rho_layers=rho,
              pol=(0.8, 0.6), sigma_layers=[0.0, 4.0, 15.0], young=(E, nu))
+
+
+def _temporal_case(name, temporal, nsteps=250):
+    """8x8 Q3 perturbed quadrilaterals, two layers, Frequency 10, a Dirac
+    source driven by the TemporalSource line `temporal`."""
+    run_case(name, 2, 8, 3, 0.1, 48, [1.0, 2.0], 10.0, 1e-3, nsteps, (0.47, 0.52),
+             seeded_points(2, 5, 97), temporal=temporal)
+
+
+@case
+def ref2d_q3_src_derivative_ricker():
+    _temporal_case("ref2d_q3_src_derivative_ricker", "DERIVATIVE_RICKER")
+
+
+@case
+def ref2d_q3_src_modified_ricker():
+    """centre time 0.06: the cut-off (2.5 + 0.06)/(sqrt(2) 10) = 0.181 lies
+    past the pulse"""
+    _temporal_case("ref2d_q3_src_modified_ricker", "MODIFIED_RICKER 0.06")
+
+
+@case
+def ref2d_q3_src_gaussian():
+    """a plateau of 0.05 between the rise and the fall"""
+    _temporal_case("ref2d_q3_src_gaussian", "GAUSSIAN 0.05")
+
+
+@case
+def ref2d_q3_src_harmonic():
+    _temporal_case("ref2d_q3_src_harmonic", "HARMONIC 0.2")
+
+
+@case
+def ref2d_q3_src_sinus_gaussian():
+    """alpha 2000: cut-off 2 sqrt(ln(1e6)/2000) = 0.166, centred at 0.083"""
+    _temporal_case("ref2d_q3_src_sinus_gaussian", "SINUS_GAUSSIAN 2000")
 
 
 @case

@@ -69,6 +69,38 @@ def test_time_source_table_matches_oracle(nb_deriv, t_begin, offset):
     assert scale > 0 and np.abs(a - b).max() <= 1e-14 * scale
 
 
+SOURCES = [("RICKER", 0.0), ("DERIVATIVE_RICKER", 0.0), ("MODIFIED_RICKER", 0.06),
+           ("GAUSSIAN", 0.05), ("GAUSSIAN", 0.0), ("HARMONIC", 0.2), ("SINUS_GAUSSIAN", 2000.0)]
+
+
+@pytest.mark.parametrize("kind,par", SOURCES)
+@pytest.mark.parametrize("nb_deriv,t_begin,offset", [(1, 0.0, 1e-3), (0, 0.0, 0.5e-3),
+                                                     (2, 0.013, 1e-3)])
+def test_temporal_source_tables_match_oracle(kind, par, nb_deriv, t_begin, offset):
+    """Every TemporalSource kind (mjw_temporal_source_table) against the
+    oracle's restatement (or_source_table), the leapfrog's table (nb_deriv 1,
+    offset dt) included; zero from the same cut-off on."""
+    f0, dt, n = 10.0, 1e-3, 350
+    a = M.temporal_source_table(kind, f0, par, t_begin, dt, n, nb_deriv, offset)
+    b = O.source_table(kind, f0, par, t_begin, dt, n, nb_deriv, offset)
+    scale = np.abs(b).max()
+    assert scale > 0 and np.abs(a - b).max() <= 1e-13 * scale
+    t = t_begin + np.arange(n) * dt + offset
+    cut = t >= O.source_limit(kind, f0, par)
+    assert cut.any() and not a[cut].any() and np.array_equal(a == 0, b == 0)
+    if kind == "RICKER" and nb_deriv == 1 and offset == dt and t_begin == 0.0:
+        np.testing.assert_array_equal(a, M.ricker_pulse_table(f0, 0.0, dt, n))
+
+
+def test_temporal_source_rejects_bad_parameters():
+    with pytest.raises(M.MjwError):
+        M.temporal_source_table("SINUS_GAUSSIAN", 10.0, 0.0, 0.0, 1e-3, 10)
+    with pytest.raises(M.MjwError):
+        M.temporal_source_table("GAUSSIAN", 10.0, -1.0, 0.0, 1e-3, 10)
+    with pytest.raises(M.MjwError):
+        M.temporal_source_table(9, 10.0, 0.0, 0.0, 1e-3, 10)
+
+
 def test_create_rejects_bad_arguments_without_device():
     L = _lib.lib()
     h = C.c_void_p()

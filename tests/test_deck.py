@@ -44,7 +44,8 @@ def _write(c: CS.Case, d):
                      circle=z["circle"] if "circle" in z.files and len(z["circle"]) else None,
                      pml=CS.pml_deck(c) if c.pml is not None else None,
                      young=(z["young_E"], z["young_nu"])
-                     if "young_E" in z.files and len(z["young_E"]) else None)
+                     if "young_E" in z.files and len(z["young_E"]) else None,
+                     temporal=c.temporal)
 
 
 def _dump(path):

@@ -169,6 +169,32 @@ def test_add_scalar_time_source():
     assert np.array_equal(y, np.ones(m.N))
 
 
+@pytest.mark.parametrize("kind,par", [("DERIVATIVE_RICKER", 0.0), ("MODIFIED_RICKER", 0.06),
+                                      ("GAUSSIAN", 0.05), ("HARMONIC", 0.2),
+                                      ("SINUS_GAUSSIAN", 2000.0)])
+def test_add_scalar_time_source_every_kind(kind, par):
+    """AddScalarTimeSource with each TemporalSource kind
+    (mjw_set_temporal_source): at the step times the pulse times the source
+    vector equals the oracle's table entry; nothing from the kind's cut-off
+    on."""
+    m = M.regular_mesh(2, 4, 3, 0.1, seed=8)
+    o = O.mesh(2, 4, 3, 0.1, 8)
+    P = M.WaveProblem(m, 1.0, 1.0)
+    src = np.array([[0.43, 0.52]])
+    P.set_sources(src)
+    f0, dt, n = 10.0, 1e-3, 320
+    P.set_time_source(f0, 0.0, n * dt * 1e-15, kind=kind, param=par)
+    tab = O.source_table(kind, f0, par, 0.0, dt, n)
+    b0 = O.dirac(o, src[0])
+    scale = np.abs(tab).max()
+    for k in range(n):
+        y = P.add_scalar_time_source(k * dt + dt, None, alpha=1.0, nb_deriv=1)
+        assert np.abs(y - tab[k] * b0).max() <= 1e-13 * scale * np.abs(b0).max()
+        if tab[k] == 0 and k * dt + dt >= O.source_limit(kind, f0, par):
+            assert not y.any()
+    assert (np.arange(n) * dt + dt >= O.source_limit(kind, f0, par)).any()
+
+
 def test_two_sources_in_one_element_of_one_shot():
     """Two Dirac points of one shot in the same element: the shot's source
     vector is their sum (one entry per node, no write conflict); traces

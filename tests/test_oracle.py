@@ -218,6 +218,66 @@ def test_oracle_matches_montjoie(name):
     assert CS.rel_l2(tr, ref) < 1e-12
 
 
+def _pulse_closed_form(kind, f0, par, t):
+    """d/dt of each TemporalSource's function, differentiated by hand (the
+    EvaluateDerivative formulas of Source/TimeSource.cxx agree)."""
+    pi = np.pi
+    if kind == "DERIVATIVE_RICKER":
+        a = f0 * t - 1.0
+        q = (pi * a) ** 2
+        v = 2 * pi ** 2 * f0 * a * (3 - 2 * q) * np.exp(-q)
+        return np.where((t <= -0.9 / f0) | (t >= 2.9 / f0), 0.0, v)
+    if kind == "MODIFIED_RICKER":
+        g = 2 * (pi * f0 * (t - par)) ** 2
+        return (2 * g - 1) * np.exp(-g)
+    if kind == "GAUSSIAN":
+        s = np.where(t < 1 / f0, t, t - par)
+        u = pi * (f0 * s - 1)
+        v = -2 * pi * f0 * u * np.exp(-u * u)
+        return np.where((t >= 1 / f0) & (t < 1 / f0 + par), 0.0, v)
+    if kind == "HARMONIC":
+        tt = np.clip(t, 1e-12, par - 1e-12)
+        a, b = (f0 * tt) ** 2, (f0 * (tt - par)) ** 2
+        e = np.exp(-1 / (pi * a)) * np.exp(-1 / (pi * b))
+        w = 2 * pi * f0
+        v = e * (w * np.cos(w * tt) + np.sin(w * tt) * 2 * (1 / (pi * a * tt) +
+                                                              1 / (pi * b * (tt - par))))
+        return np.where((t <= 0) | (t >= par), 0.0, v)
+    if kind == "SINUS_GAUSSIAN":
+        toff = np.sqrt(np.log(1e6) / par)
+        w = 2 * pi * f0
+        return np.exp(-par * (t - toff) ** 2) * (w * np.cos(w * t) - 2 * par * (t - toff) *
+                                                 np.sin(w * t))
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind,par,tol", [("DERIVATIVE_RICKER", 0.0, 1e-12),
+                                          ("MODIFIED_RICKER", 0.06, 1e-12),
+                                          ("GAUSSIAN", 0.05, 1e-11),
+                                          ("HARMONIC", 0.2, 3e-5),
+                                          ("SINUS_GAUSSIAN", 2000.0, 1e-11)])
+def test_temporal_source_pulse_closed_forms(kind, par, tol):
+    """The leapfrog's table of each TemporalSource (the derivative of the
+    degree-24 interpolant over 0.5/f0 windows, at t_n + dt) against the
+    function's derivative by hand, relative to its peak, before the cut-off;
+    the cut-offs are GetNewTimeSource's.  The HARMONIC's onset
+    exp(-1/(pi f0^2 t^2)) is smooth but not analytic at t = 0, so the
+    polynomial of the first window departs from it by 1e-5 of the peak there
+    (Montjoie interpolates the same way)."""
+    f0, dt, n = 10.0, 1e-3, 300
+    tab = O.source_table(kind, f0, par, 0.0, dt, n)
+    t = np.arange(n) * dt + dt
+    lim = {"DERIVATIVE_RICKER": lambda: 2.5 / f0,
+           "MODIFIED_RICKER": lambda: (2.5 + par) / (np.sqrt(2) * f0),
+           "GAUSSIAN": lambda: 2.5 / f0 + par, "HARMONIC": lambda: par,
+           "SINUS_GAUSSIAN": lambda: 2 * np.sqrt(np.log(1e6) / par)}[kind]()
+    assert O.source_limit(kind, f0, par) == pytest.approx(lim, rel=1e-15)
+    live = t < lim
+    ref = _pulse_closed_form(kind, f0, par, t)
+    assert np.abs(tab[live] - ref[live]).max() <= tol * np.abs(ref[live]).max()
+    assert not tab[~live].any()
+
+
 def test_young_poisson_2d_is_plane_stress():
     """YOUNG_POISSON media of a 2-D deck: Montjoie's traces are matched by the
     plane-stress Lame coefficient lambda = E nu / (1 - nu^2) the fixture
