@@ -27,14 +27,16 @@
 // (2 LEAP_FROG), SismoPoint, FileOutputPoint, ParametersOutputPoint, SismoCircle,
 // FileOutputCircle, ParametersOutputCircle, SismoPlane, FileOutputPlane, ParametersOutputPlane,
 // SismoLine, FileOutputLine, ParametersOutputLine, OutputFormat (BINARY or ASCII, DOUBLE or FLOAT),
-// DisplayStress (YES), SaveReprise, PathReprise.  Other keywords are reported and ignored; a value
+// DisplayStress, SaveReprise, PathReprise.  Other keywords are reported and ignored; a value
 // this path does not implement (another scheme, source or condition) stops
 // the run with a message, as Montjoie's own parser does (cout + abort).
 //
 // Elastic decks (quadrilaterals or hexahedra): Montjoie's rows hold the
 // displacement of every point (d values per point, point-major) followed by
 // the stress tensor of every point (d*d values, row-major; DisplayStress =
-// YES, its default); this driver writes both.
+// YES, its default), or the raw gradient of u (DisplayStress = NO,
+// ModifyOutputUnknown returns early, Elliptic/Elastic/VarElastic.cxx:
+// 257-258); this driver writes both.
 //
 // --dump-mesh writes the assembled mesh (sizes, node positions, element ->
 // node map in local order q = i + n j + n^2 k, Dirichlet flags, media) as raw
@@ -1601,8 +1603,11 @@ int main(int argc, char **argv) {
       die("TemporalSource " + k + ": only RICKER, DERIVATIVE_RICKER, MODIFIED_RICKER, GAUSSIAN, "
           "HARMONIC and SINUS_GAUSSIAN");
   }
+  // DisplayStress = NO: the elastic outputs keep grad u instead of C grad u
+  // (VarElastic.cxx:134-139)
+  bool display_stress = true;
   if (const auto *s = d.get("DisplayStress"))
-    if (!s->empty() && (*s)[0] == "NO") die("DisplayStress = NO: only the stress output is written");
+    display_stress = s->empty() || (*s)[0] != "NO";
   if (const auto *s = d.get("LoadReprise"))
     if (!s->empty() && (*s)[0] == "YES") die("LoadReprise: restarting is not implemented");
   // SaveReprise = n: checkpoint every n steps (VarInstationary.cxx:321-333)
@@ -1863,6 +1868,7 @@ int main(int argc, char **argv) {
           for (size_t i = 0; i < n; ++i) {
             const int64_t e = probe_elem[sn.off + n + i];
             if (e < 0) { vals[i] = 0.0; continue; }
+            if (!display_stress) { vals[i] = grad(i, a, b); continue; }
             double div = 0.0;
             for (int c = 0; c < dim; ++c) div += grad(i, c, c);
             vals[i] = (a == b ? md.lam[e] * div : 0.0) + md.mu[e] * (grad(i, a, b) + grad(i, b, a));
@@ -1966,7 +1972,9 @@ int main(int argc, char **argv) {
           for (int i = 0; i < dim; ++i)
             for (int j = 0; j < dim; ++j)
               std::fprintf(f, fmt,
-                           (i == j ? lam * div : 0.0) + mu * (g[i * dim + j] + g[j * dim + i]));
+                           !display_stress ? g[i * dim + j]
+                                           : (i == j ? lam * div : 0.0) +
+                                                 mu * (g[i * dim + j] + g[j * dim + i]));
         }
       std::fprintf(f, "\n");
     }

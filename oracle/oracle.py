@@ -691,11 +691,12 @@ def snapshot(m: Mesh, u, pts):
     return out
 
 
-def snapshot_elastic(m: Mesh, u, pts, lam_e, mu_e):
+def snapshot_elastic(m: Mesh, u, pts, lam_e, mu_e, stress: bool = True):
     """Elastic snapshot fields at pts: [d, npts] displacement and [d*d, npts]
     stress sigma_ij = lambda tr(grad u) delta_ij + mu (du_i/dx_j + du_j/dx_i)
     (row-major, k = d i + j) with the medium of the first element holding each
-    point; 0 outside the mesh."""
+    point, or with stress = False (DisplayStress = NO, VarElastic.cxx:257-258)
+    d u_i / d x_j itself; 0 outside the mesh."""
     d = m.dim
     el, W, Gw = receiver_gradients(m, pts)
     npts = len(el)
@@ -708,6 +709,9 @@ def snapshot_elastic(m: Mesh, u, pts, lam_e, mu_e):
         ue = np.stack([u[c * m.N + en[el[k]]] for c in range(d)])
         disp[:, k] = ue @ W[k]
         g = ue @ Gw[k].T  # g[i, a] = d u_i / d x_a
+        if not stress:
+            sig[:, k] = g.ravel()
+            continue
         s = lam_e[el[k]] * np.trace(g) * np.eye(d) + mu_e[el[k]] * (g + g.T)
         sig[:, k] = s.ravel()
     return disp, sig

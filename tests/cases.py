@@ -39,6 +39,8 @@ class Case:
     pml: tuple | None = None
     # the deck's TemporalSource line: a kind and its value
     temporal: str = "RICKER"
+    # elastic outputs: the stress (DisplayStress = YES, the default) or grad u
+    display_stress: bool = True
 
     def source_kind(self) -> tuple:
         w = self.temporal.split()
@@ -101,7 +103,8 @@ def load(name: str) -> Case:
     return Case(name, dim, nx, int(z["r"]), float(z["pert"]), int(z["seed"]), elastic, mu_e,
                 rho_e, lam_e, float(z["f0"]), float(z["dt"]), int(z["nsteps"]), z["src"],
                 z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e, pml,
-                str(z["temporal"]) if "temporal" in z.files else "RICKER")
+                str(z["temporal"]) if "temporal" in z.files else "RICKER",
+                bool(z["display_stress"]) if "display_stress" in z.files else True)
 
 
 def oracle_pulse(c: Case, nsteps: int) -> np.ndarray:
@@ -185,7 +188,8 @@ def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
     receiver recorded by the leapfrog (gradient weights), then
     sigma = lambda tr(grad u) I + mu (grad u + grad u^T) with the medium of
     the element holding the point; rows n = 0..nsteps, d*d values per point
-    (row-major), point-major."""
+    (row-major), point-major.  DisplayStress = NO decks: grad u itself,
+    d u_i / d x_a at i d + a."""
     nsteps = c.nsteps if nsteps is None else nsteps
     d = c.dim
     m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
@@ -202,6 +206,8 @@ def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
     S = O.damping(m, c.sigma_e) if c.sigma_e is not None and np.any(c.sigma_e) else None
     _, _, g = O.leapfrog(P, b, pulse, c.dt, nsteps, rec_comp=comp, rec_weights=(re, rw), S=S)
     g = g.reshape(nsteps + 1, npt, d, d)
+    if not c.display_stress:
+        return g.reshape(nsteps + 1, npt * d * d)
     lam = c.lam_e[el][None, :, None, None]
     mu = c.mu_e[el][None, :, None, None]
     div = np.trace(g, axis1=2, axis2=3)[:, :, None, None]

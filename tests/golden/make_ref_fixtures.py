@@ -52,7 +52,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
              mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
              output_format="BINARY DOUBLE", circle=None, pml=None, young=None,
-             temporal="RICKER"):
+             temporal="RICKER", display_stress=True):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
@@ -60,7 +60,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
                line=line, line_times=line_times, mesh_media=mesh_media,
                save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma,
                output_format=output_format, circle=circle, pml=pml, young=young,
-               temporal=temporal)
+               temporal=temporal, display_stress=display_stress)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini", str(dim)]
     for f in os.listdir(d):
@@ -127,7 +127,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         circle=np.asarray(circle if circle is not None else [], float),
         young_E=np.asarray(young[0] if young is not None else [], float),
         young_nu=np.asarray(young[1] if young is not None else [], float),
-        temporal=temporal,
+        temporal=temporal, display_stress=display_stress,
         **_pml_fields(pml, dim, nx),
         circle_files=circles,
         circle_names=names.get("totalCircle", np.zeros(0, "U1")),
@@ -230,6 +230,32 @@ def ref2d_el_q4_young_damped():
     run_case("ref2d_el_q4_young_damped", 2, 8, 4, 0.1, 24, list(mu), 10.0, 1e-3, 300,
              (0.48, 0.51), rec, elastic=True, lam_layers=list(lam), rho_layers=rho,
              pol=(0.8, 0.6), sigma_layers=[0.0, 4.0, 15.0], young=(E, nu))
+
+
+@case
+def ref3d_el_q3_grad():
+    """DisplayStress = NO: the SismoPoint rows and the SismoLine files keep
+    grad u; 3^3 Q3 perturbed, two layers, a line leaving the cube."""
+    lam = [1.0 + 3.0 * O.uniform(2048, k) for k in range(2)]
+    mu = [1.0 + 3.0 * O.uniform(2049, k) for k in range(2)]
+    rho = [1.0 + O.uniform(2050, k) for k in range(2)]
+    run_case("ref3d_el_q3_grad", 3, 3, 3, 0.1, 25, mu, 8.0, 4e-3, 120, (0.47, 0.52, 0.5),
+             seeded_points(3, 4, 98), elastic=True, lam_layers=lam, rho_layers=rho,
+             pol=(0.2, 0.9, -0.4), line=[0.1, 0.3, 0.2, 1.15, 0.6, 0.7, 9],
+             line_times=[0.2, 0.4, 0.1], display_stress=False)
+
+
+@case
+def ref2d_el_q4_grad():
+    """DisplayStress = NO on quadrilaterals: SismoPoint rows and SismoPlane
+    files of grad u; 8x8 Q4 perturbed, two layers."""
+    lam = [1.0 + 3.0 * O.uniform(2051, k) for k in range(2)]
+    mu = [1.0 + 3.0 * O.uniform(2052, k) for k in range(2)]
+    rho = [1.0 + O.uniform(2053, k) for k in range(2)]
+    run_case("ref2d_el_q4_grad", 2, 8, 4, 0.1, 26, mu, 10.0, 1e-3, 200, (0.45, 0.55),
+             seeded_points(2, 5, 99), elastic=True, lam_layers=lam, rho_layers=rho,
+             pol=(0.5, 0.5), plane=[0.05, 0.95, 0.1, 0.9, 7, 6], plane_times=[0.05, 0.15, 0.05],
+             display_stress=False)
 
 
 def _temporal_case(name, temporal, nsteps=250):

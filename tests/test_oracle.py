@@ -530,8 +530,9 @@ def test_oracle_gradient_weights_exact_on_linear_fields():
         np.testing.assert_allclose(W[k] @ u, pts[k] @ g + 0.7, rtol=0, atol=1e-12)
 
 
-@pytest.mark.parametrize("name", ["ref3d_el_q3_plane", "ref2d_el_q4_plane"])
-@pytest.mark.parametrize("kind", ["plane", "line"])
+@pytest.mark.parametrize("name,kind", [("ref3d_el_q3_plane", "plane"), ("ref3d_el_q3_plane", "line"),
+                                       ("ref2d_el_q4_plane", "plane"), ("ref2d_el_q4_plane", "line"),
+                                       ("ref3d_el_q3_grad", "line"), ("ref2d_el_q4_grad", "plane")])
 def test_oracle_elastic_snapshots_match_montjoie(kind, name):
     """Elastic SismoPlane / SismoLine files written by Montjoie: one file per
     displacement component (_U<m>) and per stress component (_dU<k>, k = d i
@@ -552,7 +553,7 @@ def test_oracle_elastic_snapshots_match_montjoie(kind, name):
     m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     for k, s in enumerate(steps):
         uN, _, _ = CS.oracle_run(c, s)
-        disp, sig = O.snapshot_elastic(m, uN, pts, c.lam_e, c.mu_e)
+        disp, sig = O.snapshot_elastic(m, uN, pts, c.lam_e, c.mu_e, c.display_stress)
         fields = np.vstack([disp, sig])
         for f in range(nf):
             head, ref = CS.plane_file(files[nf * k + f], len(pts))
