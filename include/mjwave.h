@@ -198,6 +198,26 @@ int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, const double *p
  */
 int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table);
 /*
+ * Time order of the leapfrog (OrderTimeScheme = r LEAP_FROG,
+ * All_TimeScheme::SetInputData, Instationary/TimeSchemes.cxx:760-767, and
+ * ModifiedEquationIterator::SetOrder, ModifiedEquation.cxx:71-79): 2 (the
+ * default) or 4, the modified-equation scheme
+ *   (D + dt/2 S) U^{n+1} = 2 D U^n - (D - dt/2 S) U^{n-1} + dt^2 (F^n - A U^n)
+ *                          + dt^4/12 D d^4U/dt^4 + dt^4/6 S d^3U/dt^3
+ * with the derivatives from the operator (Advance, ModifiedEquation.cxx:
+ * 347-407), started by a fourth-order Taylor expansion from u0, du0
+ * (:124-170).  Each step applies the stiffness twice (three times with
+ * damping); its CFL limit is 2 sqrt(3) / sqrt(lambda_max) (GetCFL, :217-225).
+ * The source needs f and its first two derivatives at t_n + dt: mjw_set_pulse
+ * and mjw_set_pulse_derivatives (zero where not given).  Call before
+ * mjw_leapfrog_init.  PML layers, batches, the adjoint, mjw_energy and
+ * mjw_profile refuse order 4.
+ */
+int mjw_set_time_order(mjw_problem *p, int order);
+/* f'(t_n + dt) and f''(t_n + dt), n < nsteps: mjw_temporal_source_table with
+ * nb_deriv 2 and 3 (the fourth-order scheme's d^3U/dt^3 and d^4U/dt^4 terms) */
+int mjw_set_pulse_derivatives(mjw_problem *p, int nsteps, const double *d1, const double *d2);
+/*
  * Time function of the sources for the operator-level entry below
  * (GetNewTimeSource + InitSource, VarInstationary.cxx:409-420,2359-2365):
  * MJW_TIME_RICKER is the Ricker primitive interpolated by the degree-24

@@ -53,6 +53,8 @@ SIGNATURES = {
     "mjw_get_mass": (C.c_int, [_vp, _dp]),
     "mjw_set_sources": (C.c_int, [_vp, C.c_int, _dp, _dp, _i32p]),
     "mjw_set_pulse": (C.c_int, [_vp, C.c_int, _dp]),
+    "mjw_set_pulse_derivatives": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "mjw_set_time_order": (C.c_int, [_vp, C.c_int]),
     "mjw_ricker_pulse_table": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
     "mjw_set_receivers": (C.c_int, [_vp, C.c_int, _dp, _i32p]),
     "mjw_set_receivers_grad": (C.c_int, [_vp, C.c_int, _dp, _i32p, _i32p, _i64p]),

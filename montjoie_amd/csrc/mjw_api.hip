@@ -83,6 +83,15 @@ struct mjw_problem {
   void *d_D = nullptr;
   void *d_U[2] = {nullptr, nullptr};
   int cur = 0;
+  // fourth-order modified equation (mjw_set_time_order): U^{n-1}, work
+  // vectors V, X, Z, the source tables dt^2 f, dt^2 f', dt^2 f'' and S per node
+  int torder = 2;
+  void *d_U3 = nullptr;
+  void *d_me[3] = {nullptr, nullptr, nullptr};
+  double *d_pulse_me = nullptr;
+  double *d_Sme = nullptr;
+  int me_phase = 0;                   // steps mod 6: the cycle of the three levels
+  std::vector<double> h_pulse[3];     // f, f', f'' at t_n + dt (host copies)
   void *d_tmp = nullptr, *d_tmp2 = nullptr;
   double *d_stage = nullptr;          // host-vector staging (one FP64 vector)
   double *d_part = nullptr, *d_red = nullptr;
@@ -278,6 +287,10 @@ static void release(mjw_problem *p) {
   (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  dfree(p->d_U3);
+  for (void *&v : p->d_me) dfree(v);
+  dfree(p->d_pulse_me);
+  dfree(p->d_Sme);
   dfree(p->d_coef);
   dfree(p->d_glxw);
   dfree(p->d_stage);
@@ -1152,6 +1165,16 @@ extern "C" int mjw_add_scalar_time_source(mjw_problem *p, int shot, double alpha
   return MJW_OK;
 }
 
+extern "C" int mjw_set_pulse_derivatives(mjw_problem *p, int nsteps, const double *d1,
+                                         const double *d2) {
+  if (!p || nsteps < 0 || (nsteps > 0 && (!d1 || !d2)))
+    FAIL(p, MJW_ERR_ARG, "mjw_set_pulse_derivatives: bad args");
+  p->h_pulse[1].assign(d1, d1 + nsteps);
+  p->h_pulse[2].assign(d2, d2 + nsteps);
+  p->inited = p->inited && p->torder == 2;
+  return MJW_OK;
+}
+
 extern "C" int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table) {
   if (!p || nsteps < 0 || (nsteps > 0 && !table)) FAIL(p, MJW_ERR_ARG, "mjw_set_pulse: bad args");
   HIPCHK(p, hipSetDevice(p->device));
@@ -1163,6 +1186,8 @@ extern "C" int mjw_set_pulse(mjw_problem *p, int nsteps, const double *table) {
     HIPCHK(p, dalloc(p, &p->d_pulse, (size_t)nsteps * 8));
     HIPCHK(p, hipMemcpy(p->d_pulse, table, (size_t)nsteps * 8, hipMemcpyHostToDevice));
   }
+  p->h_pulse[0].assign(table, table + nsteps);
+  p->inited = p->inited && p->torder == 2;  // the fourth-order tables are scaled at init
   // both step graphs hold the old table
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   if (p->bexec) { (void)hipGraphExecDestroy(p->bexec); p->bexec = nullptr; }
@@ -1260,10 +1285,99 @@ extern "C" int mjw_set_receivers_grad(mjw_problem *p, int nrec, const double *pt
 static int pml_init(mjw_problem *p, double t_begin, double dt, const double *u0,
                     const double *du0);
 
+// Fourth-order modified equation (ModifiedEquationIterator, order 4): three
+// levels U^{n-1} (d_U3), U^n (d_U[cur]), U^{n+1} (d_U[cur^1]); the start
+// from u0, du0 by the fourth-order Taylor expansion of SetInitialConditionS
+// (ModifiedEquation.cxx:124-170): d^k U/dt^k = D^{-1}(-A d^{k-2} U - S d^{k-1} U),
+// k = 2, 3, 4, each zero on the Dirichlet nodes, no source near t_begin.
+static int me4_init(mjw_problem *p, double t_begin, double dt, const double *u0,
+                    const double *du0) {
+  HIPCHK(p, hipSetDevice(p->device));
+  hipStream_t st = p->stream;
+  const int64_t ns = nstate(p);
+  const size_t bytes = (size_t)ns * ssize(p);
+  const int S = p->nshots;
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }  // dt is baked in
+  if (!p->d_U3) HIPCHK(p, dalloc(p, &p->d_U3, bytes));
+  for (void *&v : p->d_me)
+    if (!v) HIPCHK(p, dalloc(p, &v, bytes));
+  // S per node (solver order, 0 on Dirichlet nodes), or none
+  dfree(p->d_Sme);
+  p->d_Sme = nullptr;
+  if (!p->damp_S.empty()) {
+    std::vector<double> sv(p->N);
+    for (int64_t g = 0; g < p->N; ++g) sv[p->nperm[g]] = p->dir[g] ? 0.0 : p->damp_S[p->nperm[g]];
+    HIPCHK(p, dalloc(p, &p->d_Sme, (size_t)p->N * 8));
+    HIPCHK(p, hipMemcpy(p->d_Sme, sv.data(), (size_t)p->N * 8, hipMemcpyHostToDevice));
+  }
+  // source tables scaled by dt^2 (f' and f'' zero where not given)
+  const int64_t np = p->npulse;
+  dfree(p->d_pulse_me);
+  p->d_pulse_me = nullptr;
+  if (np) {
+    std::vector<double> tab(3 * np, 0.0);
+    for (int k = 0; k < 3; ++k)
+      for (int64_t n = 0; n < np && n < (int64_t)p->h_pulse[k].size(); ++n)
+        tab[k * np + n] = dt * dt * p->h_pulse[k][n];
+    HIPCHK(p, dalloc(p, &p->d_pulse_me, tab.size() * 8));
+    HIPCHK(p, hipMemcpy(p->d_pulse_me, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+  }
+  KCHK(p, launch_make_sdiag(p->state, p->N, p->d_M, p->d_dir, dt * dt, p->d_sdiag, st));
+  p->dt = dt;
+  p->t_begin = t_begin;
+  p->zero_ic = (u0 == nullptr && du0 == nullptr);
+  p->adj_ready = false;
+  p->cur = 0;
+  p->me_phase = 0;
+  void *U1 = p->d_U[0], *U2 = p->d_U[1], *U0 = p->d_U3;
+  void *V = p->d_me[1], *W = p->d_tmp2, *X = p->d_me[2], *Z = p->d_me[0];
+  if (u0) KCHK(p, upload_vec(p, u0, U1, S));
+  else HIPCHK(p, hipMemsetAsync(U1, 0, bytes, st));
+  KCHK(p, launch_mask(p->state, ns, S, p->ilv, p->N, p->d_dir, U1, st));
+  if (du0) KCHK(p, upload_vec(p, du0, V, S));
+  else HIPCHK(p, hipMemsetAsync(V, 0, bytes, st));
+  auto node = [&](int op, const void *a, const void *b, const void *x, const void *u,
+                  const void *w, double c0, double c1, double c2, void *y) {
+    return launch_me4(p->state, op, ns, S, p->ilv, p->N, p->d_M, p->d_Sme, p->d_dir, a, b, x, u,
+                      w, c0, c1, c2, y, st);
+  };
+  // d2 = D^{-1}(-A U - S du0), d3 = D^{-1}(-A du0 - S d2), d4 = D^{-1}(-A d2 - S d3)
+  KCHK(p, apply_A(p, -1.0, U1, 0.0, W, S));
+  KCHK(p, node(kMeSolve, W, V, nullptr, nullptr, nullptr, 0, 0, 0, W));
+  KCHK(p, apply_A(p, -1.0, V, 0.0, X, S));
+  KCHK(p, node(kMeSolve, X, W, nullptr, nullptr, nullptr, 0, 0, 0, X));
+  KCHK(p, apply_A(p, -1.0, W, 0.0, Z, S));
+  KCHK(p, node(kMeSolve, Z, X, nullptr, nullptr, nullptr, 0, 0, 0, Z));
+  // U^{+-1} = U + dt(+-du0 + dt(d2/2 + dt(+-d3/6 + dt d4/24))); the Dirichlet
+  // rows of U^{-1} keep -dt du0 (SetInitialConditionS masks only U^0, U^1)
+  KCHK(p, node(kMeTaylor, V, W, X, Z, U1, dt, 1.0, 1.0, U2));
+  KCHK(p, node(kMeTaylor, V, W, X, Z, U1, dt, -1.0, 0.0, U0));
+  p->step = 0;
+  int64_t ctl[2] = {0, 0};
+  HIPCHK(p, hipMemcpyAsync(p->d_ctl, ctl, 16, hipMemcpyHostToDevice, st));
+  HIPCHK(p, hipStreamSynchronize(st));
+  p->inited = true;
+  return MJW_OK;
+}
+
+extern "C" int mjw_set_time_order(mjw_problem *p, int order) {
+  if (!p) FAIL(p, MJW_ERR_ARG, "mjw_set_time_order: NULL");
+  if (order != 2 && order != 4) FAIL(p, MJW_ERR_ARG, "mjw_set_time_order: order 2 or 4");
+  if (order == 4 && p->pml)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_time_order: the PML layers run the first-order system");
+  if (order != p->torder) {
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+    p->torder = order;
+    p->inited = false;
+  }
+  return MJW_OK;
+}
+
 extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, const double *u0,
                                  const double *du0) {
   if (!p || !(dt > 0.0) || !std::isfinite(dt)) FAIL(p, MJW_ERR_ARG, "mjw_leapfrog_init: bad dt");
   if (p->pml) return pml_init(p, t_begin, dt, u0, du0);
+  if (p->torder == 4) return me4_init(p, t_begin, dt, u0, du0);
   HIPCHK(p, hipSetDevice(p->device));
   hipStream_t st = p->stream;
   p->dt = dt;
@@ -1345,8 +1459,70 @@ extern "C" int mjw_leapfrog_init(mjw_problem *p, double t_begin, double dt, cons
 // source and records the traces of U^{n+1} (row n+1).
 static int enqueue_step_pml(mjw_problem *p, double *traces);
 
+// One step of the fourth-order modified equation (ModifiedEquationIterator::
+// Advance, order 4, ModifiedEquation.cxx:347-407) with Kh = -A:
+//   P = dt^2 (f(t+dt) b - A U^{n+1}),  V = dt/2 (3 U^{n+1} - 4 U^n + U^{n-1}),
+//   W = D^{-1}(P - S V),  X = D^{-1}(-A V + dt^2 f'(t+dt) b - S W),
+//   Z = -A W + dt^2 f''(t+dt) b - S X,
+//   (D + dt/2 S) U^{n+2} = P + dt^2/12 Z + dt^2/6 S X + 2 D U^{n+1} - (D - dt/2 S) U^n,
+// W, X and U^{n+2} zero on the Dirichlet nodes.  Undamped (S = 0) the V and
+// X terms vanish and are skipped.  U^{n+2} overwrites U^{n-1}; the caller
+// rotates the levels.  The first tail records U^{n+1} (row n+1), the last
+// one advances the step counter.
+static int enqueue_step_me4(mjw_problem *p, double *traces) {
+  hipStream_t st = p->stream;
+  const int S = p->nshots;
+  const int64_t ns = nstate(p);
+  const double dt = p->dt, dt2 = dt * dt;
+  void *Uprev = p->d_U[p->cur], *Ucur = p->d_U[p->cur ^ 1], *Unew = p->d_U3;
+  void *P = p->d_tmp, *W = p->d_tmp2, *Z = p->d_me[0], *V = p->d_me[1], *X = p->d_me[2];
+  const bool damped = p->d_Sme != nullptr;
+  auto node = [&](int op, const void *a, const void *b, const void *x, const void *u,
+                  const void *w, double c0, double c1, double c2, void *y) {
+    return launch_me4(p->state, op, ns, S, p->ilv, p->N, p->d_M, p->d_Sme, p->d_dir, a, b, x, u,
+                      w, c0, c1, c2, y, st);
+  };
+  auto tail = [&](int k, void *into, bool record, int advance) {
+    TailArgs t{};
+    t.state = p->state; t.ctl = p->d_ctl;
+    t.pulse = p->d_pulse_me ? p->d_pulse_me + (int64_t)k * p->npulse : nullptr;
+    t.npulse = p->npulse; t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot;
+    t.src_val = p->d_src_val; t.sdiag = nullptr; t.nrec = record && traces ? p->nrec : 0;
+    t.nd = p->nd; t.rec_gid = p->d_rec_gid; t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp;
+    t.U = Ucur; t.P = into; t.traces = (record && traces && p->nrec) ? traces : nullptr;
+    t.N = p->N; t.nshots = S; t.ilv = p->ilv; t.advance = advance; t.row_offset = 1;
+    return launch_tail_n(t, p->ncomp, st);
+  };
+  int rc = apply_A(p, -dt2, Ucur, 0.0, P, S);
+  if (!rc) rc = tail(0, P, true, 0);
+  if (damped) {
+    if (!rc) rc = node(kMeVel, Ucur, Uprev, Unew, nullptr, nullptr, 0.5 * dt, 0, 0, V);
+    if (!rc) rc = node(kMeSolve, P, V, nullptr, nullptr, nullptr, 0, 0, 0, W);
+    if (!rc) rc = apply_A(p, -1.0, V, 0.0, X, S);
+    if (!rc) rc = tail(1, X, false, 0);
+    if (!rc) rc = node(kMeSolve, X, W, nullptr, nullptr, nullptr, 0, 0, 0, X);
+  } else {
+    if (!rc) rc = node(kMeSolve, P, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, W);
+  }
+  if (!rc) rc = apply_A(p, -1.0, W, 0.0, Z, S);
+  if (!rc) rc = tail(2, Z, false, 1);
+  if (damped && !rc) rc = node(kMeSub, Z, X, nullptr, nullptr, nullptr, 0, 0, 0, Z);
+  if (!rc)
+    rc = node(kMeNew, P, Z, damped ? X : nullptr, Ucur, Uprev, 0.5 * dt, dt2 / 12.0, dt2 / 6.0,
+              Unew);
+  return rc;
+}
+
+// the levels after a fourth-order step: U^{n+1} stays, U^{n+2} (written over
+// U^{n-1}) becomes d_U[cur ^ 1] and U^n becomes the oldest level
+static void me4_rotate(mjw_problem *p) {
+  std::swap(p->d_U[p->cur], p->d_U3);
+  p->me_phase = (p->me_phase + 1) % 6;
+}
+
 static int enqueue_step(mjw_problem *p, int parity, double *traces) {
   if (p->pml) return enqueue_step_pml(p, traces);
+  if (p->torder == 4) return enqueue_step_me4(p, traces);
   void *Ucur = p->d_U[parity ^ 1], *Y = p->d_U[parity];
   if (p->ndamp) {
     // keep U^n on the damped nodes before the passes overwrite it
@@ -1397,6 +1573,10 @@ static int ensure_traces(mjw_problem *p, int64_t nsteps) {
 }
 
 static const int kGraphSteps = 16;  // even
+// fourth-order scheme: a multiple of the 6-step cycle of its three levels
+static const int kGraphSteps4 = 24;
+
+static int graph_steps(const mjw_problem *p) { return p->torder == 4 ? kGraphSteps4 : kGraphSteps; }
 
 static int build_graph(mjw_problem *p, bool with_traces) {
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
@@ -1404,15 +1584,25 @@ static int build_graph(mjw_problem *p, bool with_traces) {
   hipError_t e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return (int)e;
   int rc = 0;
-  for (int s = 0; s < kGraphSteps && !rc; ++s)
-    rc = enqueue_step(p, (p->cur + s) & 1, with_traces ? p->d_traces : nullptr);
+  const int gs = graph_steps(p);
+  for (int s = 0; s < gs && !rc; ++s) {
+    if (p->torder == 4) {
+      // the captured steps rotate the levels as the eager ones do; after
+      // the whole cycle the pointers are back where they started
+      rc = enqueue_step(p, p->cur, with_traces ? p->d_traces : nullptr);
+      me4_rotate(p);
+      p->cur ^= 1;
+    } else {
+      rc = enqueue_step(p, (p->cur + s) & 1, with_traces ? p->d_traces : nullptr);
+    }
+  }
   e = hipStreamEndCapture(p->stream, &g);
   if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
   if (e != hipSuccess) return (int)e;
   e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) { p->gexec = nullptr; return (int)e; }
-  p->g_steps = kGraphSteps;
+  p->g_steps = gs;
   p->g_parity = p->cur;
   p->g_traces = p->d_traces;
   p->g_with_traces = with_traces;
@@ -1436,26 +1626,31 @@ extern "C" int mjw_leapfrog_advance(mjw_problem *p, int nsteps, double *traces, 
   if (want && !p->pml) KCHK(p, enqueue_record(p, p->d_traces));
   HIPCHK(p, hipEventRecord(ev0, st));
   int done = 0;
-  if (use_graph && nsteps >= kGraphSteps) {
-    if (p->cur != 0) {  // graphs are captured at even parity
-      KCHK(p, enqueue_step(p, p->cur, want ? p->d_traces : nullptr));
-      p->cur ^= 1;
+  const int gs = graph_steps(p);
+  auto eager = [&]() -> int {
+    int rc = enqueue_step(p, p->cur, want ? p->d_traces : nullptr);
+    if (p->torder == 4) me4_rotate(p);
+    p->cur ^= 1;
+    return rc;
+  };
+  if (use_graph && nsteps >= gs) {
+    // graphs are captured at even parity (fourth order: at the start of the
+    // cycle of its levels)
+    while (nsteps - done > 0 && (p->cur != 0 || (p->torder == 4 && p->me_phase != 0))) {
+      KCHK(p, eager());
       ++done;
     }
-    if (nsteps - done >= kGraphSteps) {
+    if (nsteps - done >= gs) {
       if (!p->gexec || p->g_parity != p->cur || p->g_with_traces != want ||
           (want && p->g_traces != p->d_traces))
         KCHK(p, build_graph(p, want));
-      while (nsteps - done >= kGraphSteps) {
+      while (nsteps - done >= gs) {
         HIPCHK(p, hipGraphLaunch(p->gexec, st));
-        done += kGraphSteps;
+        done += gs;
       }
     }
   }
-  for (; done < nsteps; ++done) {
-    KCHK(p, enqueue_step(p, p->cur, want ? p->d_traces : nullptr));
-    p->cur ^= 1;
-  }
+  for (; done < nsteps; ++done) KCHK(p, eager());
   if (p->pml) p->cur = 0;  // one state buffer
   HIPCHK(p, hipEventRecord(ev1, st));
   HIPCHK(p, hipEventSynchronize(ev1));
@@ -1493,6 +1688,7 @@ extern "C" int mjw_leapfrog_advance_static(mjw_problem *p, int nsteps, double *t
   if (want && !p->pml) KCHK(p, enqueue_record(p, buf));  // row 0 = U^n
   for (int k = 0; k < nsteps; ++k) {
     KCHK(p, enqueue_step(p, p->cur, buf));
+    if (p->torder == 4) me4_rotate(p);
     p->cur ^= 1;
   }
   if (p->pml) p->cur = 0;
@@ -1697,6 +1893,8 @@ extern "C" int mjw_last_timing(const mjw_problem *p, double *ms2) {
 
 extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
   if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_profile: not available with PML layers");
+  if (p && p->torder == 4)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_profile: times the second-order step only");
   if (!p || !out || nsteps < 1) FAIL(p, MJW_ERR_ARG, "mjw_profile: bad args");
   if (!p->inited) FAIL(p, MJW_ERR_STATE, "not initialised");
   if (!p->damp_S.empty()) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_profile: undamped problems only");
@@ -1752,6 +1950,8 @@ extern "C" int mjw_profile(mjw_problem *p, int nsteps, double *out) {
 
 extern "C" int mjw_energy(mjw_problem *p, double *energy) {
   if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_energy: not available for the first-order system (PML layers)");
+  if (p && p->torder == 4)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_energy: the discrete energy of the second-order scheme only");
   if (!p || !energy) FAIL(p, MJW_ERR_ARG, "mjw_energy: NULL");
   if (!p->inited) FAIL(p, MJW_ERR_STATE, "not initialised");
   HIPCHK(p, hipSetDevice(p->device));
@@ -1979,6 +2179,8 @@ extern "C" int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const do
 
 extern "C" int mjw_adjoint_forward(mjw_problem *p, int nsteps, int nsnap, double *traces) {
   if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: not available with PML layers");
+  if (p && p->torder == 4)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: the second-order scheme only");
   if (!p || nsteps < 2 || nsnap < 1 || !traces) FAIL(p, MJW_ERR_ARG, "mjw_adjoint_forward: bad args");
   if (p->ncomp != 1 || p->nshots != 1)
     FAIL(p, MJW_ERR_UNSUPPORTED, "adjoint: single-shot acoustic problems only");
@@ -2208,6 +2410,8 @@ extern "C" int mjw_set_pml(mjw_problem *p, const uint8_t *pml, const double *tau
   if (!p || !pml || !tau) FAIL(p, MJW_ERR_ARG, "mjw_set_pml: NULL");
   if (p->ncomp != 1 || p->nshots != 1)
     FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_pml: single-shot acoustic problems only");
+  if (p->torder == 4)
+    FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_pml: the first-order system is of order two in time");
   if (!p->damp_S.empty())
     FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_pml: physical damping with PML layers is not supported "
                                  "(VarHelmholtz.cxx:411-422 refuses it inside the layers)");
@@ -2515,6 +2719,8 @@ extern "C" int mjw_batch_create(mjw_batch **out, int nprob, mjw_problem *const *
       BFAIL((mjw_batch *)nullptr, MJW_ERR_UNSUPPORTED, "batch: undamped problems only");
     if (p->pml)
       BFAIL((mjw_batch *)nullptr, MJW_ERR_UNSUPPORTED, "batch: problems without PML layers only");
+    if (p->torder == 4)
+      BFAIL((mjw_batch *)nullptr, MJW_ERR_UNSUPPORTED, "batch: second-order scheme only");
     if (p->device != probs[0]->device || p->state != probs[0]->state ||
         p->geom != probs[0]->geom)
       BFAIL((mjw_batch *)nullptr, MJW_ERR_ARG, "batch: problems must share device and precision");

@@ -207,6 +207,12 @@ int launch_from_solver_f64(int state, int S, int s, int ncomp, int64_t N, int il
 int launch_scale_by(int state, int64_t n, int S, int ilv, int64_t N, const double *d, double c,
                     void *y, void *stream);
 // y[i] = 0 where dir[node(i)]
+// node-wise steps of the fourth-order modified-equation leapfrog (k_me4)
+enum MeOp { kMeVel = 0, kMeSolve = 1, kMeSub = 2, kMeNew = 3, kMeTaylor = 4 };
+int launch_me4(int state, int op, int64_t n, int S, int ilv, int64_t N, const double *D,
+               const double *Sd, const uint8_t *dir, const void *a, const void *b, const void *x,
+               const void *u, const void *w, double c0, double c1, double c2, void *y,
+               void *stream);
 int launch_mask(int state, int64_t n, int S, int ilv, int64_t N, const uint8_t *dir, void *y,
                 void *stream);
 int launch_make_sdiag(int state, int64_t N, const double *M, const uint8_t *dir, double dt2,

@@ -442,6 +442,75 @@ int launch_mask(int state, int64_t n, int S, int ilv, int64_t N, const uint8_t *
   return (int)hipGetLastError();
 }
 
+// Node-wise steps of the fourth-order modified-equation leapfrog
+// (ModifiedEquationIterator::Advance with order 4 and SetInitialConditionS,
+// Instationary/ModifiedEquation.cxx:124-170,347-407), on whole state vectors
+// with the per-node mass D, damping S (null: none) and Dirichlet flags; all
+// arithmetic in FP64.  Operands by op (y is written):
+//   kMeVel    y = c0 (3 a - 4 b + x)                      (dt^2 dU/dt, c0 = dt/2)
+//   kMeSolve  y = mask(D^{-1} (a - S b))                  (b null: no S term)
+//   kMeSub    y = a - S b
+//   kMeNew    y = mask((D + c0 S)^{-1} (a + c1 b + c2 S x + 2 D u - (D - c0 S) w))
+//   kMeTaylor y = w + c0 (c1 a + c0 (b/2 + c0 (c1 x/6 + c0 u/24))), c1 = +-1,
+//             zero on the Dirichlet nodes when c2 != 0 (SetInitialConditionS
+//             masks U^n and U^{n+1} but leaves U^{n-1} as it is)
+template <typename T>
+__global__ void k_me4(int op, int64_t n, int64_t N, int64_t div, const double *__restrict__ D,
+                      const double *__restrict__ Sd, const uint8_t *__restrict__ dir,
+                      const T *__restrict__ a, const T *__restrict__ b, const T *__restrict__ x,
+                      const T *__restrict__ u, const T *__restrict__ w, double c0, double c1,
+                      double c2, T *__restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = (i / div) % N;
+    const double s = Sd ? Sd[g] : 0.0;
+    double r;
+    switch (op) {
+      case kMeVel:
+        r = c0 * (3.0 * (double)a[i] - 4.0 * (double)b[i] + (double)x[i]);
+        break;
+      case kMeSolve:
+        r = dir[g] ? 0.0 : ((double)a[i] - (b ? s * (double)b[i] : 0.0)) / D[g];
+        break;
+      case kMeSub:
+        r = (double)a[i] - s * (double)b[i];
+        break;
+      case kMeNew: {
+        const double dg = D[g];
+        const double rhs = (double)a[i] + c1 * (double)b[i] + (x ? c2 * s * (double)x[i] : 0.0) +
+                           2.0 * dg * (double)u[i] - (dg - c0 * s) * (double)w[i];
+        r = dir[g] ? 0.0 : rhs / (dg + c0 * s);
+        break;
+      }
+      default: {  // kMeTaylor
+        const double t4 = c0 * (double)u[i] / 24.0;
+        const double t3 = c0 * (c1 * (double)x[i] / 6.0 + t4);
+        const double t2 = c0 * ((double)b[i] / 2.0 + t3);
+        r = (c2 != 0.0 && dir[g]) ? 0.0 : (double)w[i] + c0 * (c1 * (double)a[i] + t2);
+        break;
+      }
+    }
+    y[i] = (T)r;
+  }
+}
+
+int launch_me4(int state, int op, int64_t n, int S, int ilv, int64_t N, const double *D,
+               const double *Sd, const uint8_t *dir, const void *a, const void *b, const void *x,
+               const void *u, const void *w, double c0, double c1, double c2, void *y,
+               void *stream) {
+  const int64_t div = ilv ? S : 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (state == kF64)
+    hipLaunchKernelGGL(k_me4<double>, dim3(grid_for(n)), dim3(256), 0, st, op, n, N, div, D, Sd,
+                       dir, (const double *)a, (const double *)b, (const double *)x,
+                       (const double *)u, (const double *)w, c0, c1, c2, (double *)y);
+  else
+    hipLaunchKernelGGL(k_me4<float>, dim3(grid_for(n)), dim3(256), 0, st, op, n, N, div, D, Sd,
+                       dir, (const float *)a, (const float *)b, (const float *)x,
+                       (const float *)u, (const float *)w, c0, c1, c2, (float *)y);
+  return (int)hipGetLastError();
+}
+
 // caller layout [S][ncomp][N] <-> solver layout (see mjw_internal.hpp)
 __device__ __forceinline__ int64_t solver_index(int64_t s, int64_t mN_g, int S, int64_t per,
                                                 int ilv) {

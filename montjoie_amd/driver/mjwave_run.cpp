@@ -23,8 +23,8 @@
 // (rho / mu / sigma, SINUS or MESH / SAME_MESH data files on the
 // computational mesh, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
 // TemporalSource (RICKER, DERIVATIVE_RICKER, MODIFIED_RICKER, GAUSSIAN, HARMONIC,
-// SINUS_GAUSSIAN), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme
-// (2 LEAP_FROG), SismoPoint, FileOutputPoint, ParametersOutputPoint, SismoCircle,
+// SINUS_GAUSSIAN), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme (2 or 4
+// LEAP_FROG, 2 LEAP_FROG_PML), SismoPoint, FileOutputPoint, ParametersOutputPoint, SismoCircle,
 // FileOutputCircle, ParametersOutputCircle, SismoPlane, FileOutputPlane, ParametersOutputPlane,
 // SismoLine, FileOutputLine, ParametersOutputLine, OutputFormat (BINARY or ASCII, DOUBLE or FLOAT),
 // DisplayStress, SaveReprise, PathReprise.  Other keywords are reported and ignored; a value
@@ -1563,13 +1563,19 @@ int main(int argc, char **argv) {
   if (!elastic && eq != "ACOUSTIC") die("TypeEquation " + eq + ": only ACOUSTIC / TIME_ELASTIC");
   const int ncomp = elastic ? dim : 1;
   const int order = (int)num(d.need("OrderDiscretization")[0]);
-  // OrderTimeScheme = 2 LEAP_FROG (second-order scheme, ModifiedEquationIterator)
-  // or 2 LEAP_FROG_PML (the first-order split system, ModifiedEquationSystemIterator)
+  // OrderTimeScheme = 2 LEAP_FROG or 4 LEAP_FROG (ModifiedEquationIterator of
+  // that order, TimeSchemes.cxx:760-767) or 2 LEAP_FROG_PML (the first-order
+  // split system, ModifiedEquationSystemIterator)
   bool first_order = false;
+  int time_order = 2;
   if (const auto *o = d.get("OrderTimeScheme")) {
-    if (o->size() < 2 || num((*o)[0]) != 2 || ((*o)[1] != "LEAP_FROG" && (*o)[1] != "LEAP_FROG_PML"))
-      die("OrderTimeScheme: only 2 LEAP_FROG or 2 LEAP_FROG_PML");
-    first_order = (*o)[1] == "LEAP_FROG_PML";
+    const int r = o->empty() ? 0 : (int)num((*o)[0]);
+    const bool lf = o->size() >= 2 && (*o)[1] == "LEAP_FROG";
+    const bool lfp = o->size() >= 2 && (*o)[1] == "LEAP_FROG_PML";
+    if (!((lf && (r == 2 || r == 4)) || (lfp && r == 2)))
+      die("OrderTimeScheme: only 2 LEAP_FROG, 4 LEAP_FROG or 2 LEAP_FROG_PML");
+    first_order = lfp;
+    time_order = r;
   }
   PmlSpec pml_spec = parse_pml(d, dim);
   const PmlDamping pml_damp = parse_pml_damping(d);
@@ -1616,6 +1622,8 @@ int main(int argc, char **argv) {
     if (s->empty()) die("SaveReprise needs a number of steps");
     save_every = (int64_t)num((*s)[0]);
     if (save_every <= 0) die("SaveReprise: the number of steps must be positive");
+    if (time_order == 4)
+      die("SaveReprise: checkpoints of the second-order scheme only (order 4 keeps three levels)");
   }
   const std::string path_reprise = d.get("PathReprise") ? d.need("PathReprise")[0] : "./";
   // OutputFormat = BINARY|ASCII|... DOUBLE|FLOAT (Output/OutputHarmonic.cxx:321-380): the
@@ -1704,10 +1712,13 @@ int main(int argc, char **argv) {
     check(mjw_set_damping_nodal(p, nm.sigma.data()), p, "PhysicalMedia sigma");
   else if (!md.sigma.empty())
     check(mjw_set_damping(p, md.sigma.data()), p, "MateriauDielec sigma");
+  if (time_order == 4) check(mjw_set_time_order(p, 4), p, "OrderTimeScheme");
   // time step: given, or TimeStep = AUTO: 0.999 times the leapfrog limit
-  // 2 / sqrt(lambda_max(D^-1 K)) (VarInstationary.cxx:91-103,1229-1240,
-  // EvaluateCFL_Generic, EvaluationCflWave.cxx:717-760, P(0) = 2), lambda_max
-  // by Lanczos run until it repeats to 1e-13 (ARPACK in the reference)
+  // c / sqrt(lambda_max(D^-1 K)) (VarInstationary.cxx:91-103,1229-1240,
+  // EvaluateCFL_Generic, EvaluationCflWave.cxx:717-760), c = 2 for order 2
+  // and 2 sqrt(3) for order 4 (ModifiedEquationIterator::GetCFL,
+  // ModifiedEquation.cxx:217-225), lambda_max by Lanczos run until it
+  // repeats to 1e-13 (ARPACK in the reference)
   double dt;
   if (auto_dt) {
     double lam = 0.0, prev = -1.0;
@@ -1718,8 +1729,9 @@ int main(int argc, char **argv) {
       if (std::fabs(lam - prev) <= 1e-13 * lam || it >= nfree) break;
       prev = lam;
     }
-    dt = 0.999 * 2.0 / std::sqrt(lam);
-    std::printf("Maximal time step = %.15e\n", 2.0 / std::sqrt(lam));
+    const double cfl = time_order == 4 ? 2.0 * std::sqrt(3.0) : 2.0;
+    dt = 0.999 * cfl / std::sqrt(lam);
+    std::printf("Maximal time step = %.15e\n", cfl / std::sqrt(lam));
   } else {
     dt = num(tsv[0]);
   }
@@ -1745,6 +1757,15 @@ int main(int argc, char **argv) {
                                   first_order ? 0.5 * dt : dt, tab.data()),
         nullptr, "TemporalSource");
   check(mjw_set_pulse(p, (int)nsteps, tab.data()), p, "pulse");
+  if (time_order == 4) {
+    // f' and f'' at t_n + dt for the d^3U/dt^3 and d^4U/dt^4 terms
+    std::vector<double> tab2(nsteps), tab3(nsteps);
+    check(mjw_temporal_source_table(tkind, f0, tpar, t0, dt, (int)nsteps, 2, dt, tab2.data()),
+          nullptr, "TemporalSource");
+    check(mjw_temporal_source_table(tkind, f0, tpar, t0, dt, (int)nsteps, 3, dt, tab3.data()),
+          nullptr, "TemporalSource");
+    check(mjw_set_pulse_derivatives(p, (int)nsteps, tab2.data(), tab3.data()), p, "pulse");
+  }
   // receivers: SismoPoint (SismoCircle is a snapshot grid, below)
   std::vector<double> pts;
   int npt = 0;

@@ -294,9 +294,23 @@ class WaveProblem:
         self._chk(_lib.lib().mjw_set_sources(self._h, k, dptr(pts), dptr(polv), iptr(sh)),
                   "set_sources")
 
-    def set_pulse(self, table):
+    def set_pulse(self, table, d1=None, d2=None):
+        """Source time factors f(t_n + dt); the fourth-order scheme also takes
+        f' and f'' at the same times (d1, d2; mjw_set_pulse_derivatives)."""
         t = np.ascontiguousarray(table, np.float64)
         self._chk(_lib.lib().mjw_set_pulse(self._h, len(t), dptr(t)), "set_pulse")
+        if d1 is not None or d2 is not None:
+            a = np.ascontiguousarray(np.zeros(len(t)) if d1 is None else d1, np.float64)
+            b = np.ascontiguousarray(np.zeros(len(t)) if d2 is None else d2, np.float64)
+            if len(a) != len(t) or len(b) != len(t):
+                raise ValueError("set_pulse: the derivative tables must match the pulse table")
+            self._chk(_lib.lib().mjw_set_pulse_derivatives(self._h, len(t), dptr(a), dptr(b)),
+                      "set_pulse_derivatives")
+
+    def set_time_order(self, order: int):
+        """2 (default) or 4: the modified-equation leapfrog of that order in
+        time (mjw_set_time_order); call leapfrog_init afterwards."""
+        self._chk(_lib.lib().mjw_set_time_order(self._h, int(order)), "set_time_order")
 
     def set_damping(self, sigma):
         """Damping sigma per element, or [E, nd] at the quadrature points

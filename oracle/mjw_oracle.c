@@ -1068,6 +1068,138 @@ void or_leapfrog(int dim, int r, int ncomp, long E, long N, const int *elem_node
   free(U0); free(U1); free(Y); free(invD); free(invDp); free(Dmin); free(bi);
 }
 
+/*
+ * Fourth-order modified-equation leapfrog (ModifiedEquationIterator, order
+ * 4), step for step as Montjoie runs it with Kh = -A:
+ * SetInitialConditionS (Instationary/ModifiedEquation.cxx:124-170): from
+ * U1 = u0 (Dirichlet rows 0) and du0, d2 = D^{-1}(Kh U1 - S du0),
+ * d3 = D^{-1}(Kh du0 - S d2), d4 = D^{-1}(Kh d2 - S d3) (each with Dirichlet
+ * rows 0), then U2 = U1 + dt du0 + dt^2/2 d2 + dt^3/6 d3 + dt^4/24 d4 and U0
+ * the same with -dt, in Montjoie's Horner order.  Advance (:347-407) at step
+ * it (t = it*dt):
+ *   P  = dt^2 (Kh U2 + f(t+dt) b)
+ *   V  = dt/2 (U0 - 4 U1 + 3 U2)
+ *   W  = D^{-1}(P - S V), Dirichlet 0
+ *   X  = D^{-1}(Kh V - S W + dt^2 f'(t+dt) b), Dirichlet 0
+ *   Z  = Kh W - S X + dt^2 f''(t+dt) b
+ *   Q  = P + dt^2/12 Z + dt^2/6 S X + 2 D U2 - (D - dt/2 S) U1
+ *   U0 <- U1, U1 <- U2, U2 <- (D + dt/2 S)^{-1} Q, Dirichlet 0.
+ * The iterate of step it is U1 (GetIterate): traces row it = R U1 before the
+ * step, row nsteps after the last.  pulse, d1, d2: f, f', f'' at t_it + dt
+ * (or_source_table with nb_deriv 1, 2, 3), each may be NULL (zero).
+ * uN, uN1 receive U1 and U2 at the end.
+ */
+void or_leapfrog4(int dim, int r, int ncomp, long E, long N, const int *elem_nodes,
+                  const double *G, const double *jw, const double *dfinv, const double *lam,
+                  const double *mu, const int *color, int ncolor, const double *Dm,
+                  const double *Sd, const unsigned char *dirichlet, const double *b,
+                  const double *pulse, const double *d1, const double *d2, double dt,
+                  int nsteps, const double *u0, const double *du0, int nrec,
+                  const long *rec_elem, const double *rec_w, const int *rec_comp, double *uN,
+                  double *uN1, double *traces) {
+  int n = r + 1, nd = (dim == 2) ? n * n : n * n * n;
+  long NT = N * ncomp;
+  size_t bytes = sizeof(double) * (size_t)NT;
+  double *U0 = (double *)calloc((size_t)NT, sizeof(double));
+  double *U1 = (double *)calloc((size_t)NT, sizeof(double));
+  double *U2 = (double *)calloc((size_t)NT, sizeof(double));
+  double *V = (double *)calloc((size_t)NT, sizeof(double));
+  double *P = (double *)calloc((size_t)NT, sizeof(double));
+  double *W = (double *)calloc((size_t)NT, sizeof(double));
+  double *X = (double *)calloc((size_t)NT, sizeof(double));
+  double *Z = (double *)calloc((size_t)NT, sizeof(double));
+  double dt2 = dt * dt;
+#define OR4_APPLY(uu, yy)                                                                   \
+  do {                                                                                      \
+    if (ncomp == 1)                                                                         \
+      or_stiffness(dim, r, E, N, elem_nodes, G, color, ncolor, -1.0, (uu), 0.0, (yy));      \
+    else                                                                                    \
+      or_stiffness_elastic(dim, r, E, N, elem_nodes, jw, dfinv, lam, mu, color, ncolor, -1.0, \
+                           (uu), 0.0, (yy));                                                \
+  } while (0)
+#define OR4_S(g) (Sd ? Sd[(g) % N] : 0.0)
+#define OR4_DIR(vv)                                         \
+  do {                                                      \
+    for (long g = 0; g < NT; g++)                           \
+      if (dirichlet[g % N]) (vv)[g] = 0.0;                  \
+  } while (0)
+#define OR4_SRC(tab, it, scale, yy)                          \
+  do {                                                       \
+    if ((tab) && (tab)[it] != 0.0 && b)                      \
+      for (long g = 0; g < NT; g++)                          \
+        if (b[g] != 0.0) (yy)[g] += (scale) * ((tab)[it] * b[g]); \
+  } while (0)
+#define OR4_RECORD(row)                                                         \
+  do {                                                                          \
+    for (int s = 0; s < nrec; s++) {                                            \
+      double acc = 0.0;                                                         \
+      long e = rec_elem[s];                                                     \
+      int m = rec_comp ? rec_comp[s] : 0;                                       \
+      for (int q = 0; q < nd; q++)                                              \
+        acc += rec_w[(long)s * nd + q] * U1[m * N + elem_nodes[e * nd + q]];    \
+      traces[(long)(row)*nrec + s] = acc;                                       \
+    }                                                                           \
+  } while (0)
+  /* initial condition */
+  if (u0) memcpy(U1, u0, bytes);
+  OR4_DIR(U1);
+  if (du0) memcpy(V, du0, bytes);
+  double *d2v = W, *d3v = X, *d4v = Z;
+  OR4_APPLY(U1, d2v);
+  for (long g = 0; g < NT; g++) d2v[g] = (d2v[g] - OR4_S(g) * V[g]) / Dm[g % N];
+  OR4_DIR(d2v);
+  OR4_APPLY(V, d3v);
+  for (long g = 0; g < NT; g++) d3v[g] = (d3v[g] - OR4_S(g) * d2v[g]) / Dm[g % N];
+  OR4_DIR(d3v);
+  OR4_APPLY(d2v, d4v);
+  for (long g = 0; g < NT; g++) d4v[g] = (d4v[g] - OR4_S(g) * d3v[g]) / Dm[g % N];
+  OR4_DIR(d4v);
+  for (long g = 0; g < NT; g++) {
+    double a4 = d4v[g] / 24.0, a3 = d3v[g] / 6.0, a2 = d2v[g] / 2.0;
+    double up = a4 * dt; up += a3; up *= dt; up += a2; up *= dt; up += V[g]; up *= dt;
+    U2[g] = up + U1[g];
+    double um = a4 * dt; um -= a3; um *= dt; um += a2; um *= dt; um -= V[g]; um *= dt;
+    U0[g] = um + U1[g];
+  }
+  OR4_DIR(U1);
+  OR4_DIR(U2);
+  for (int it = 0; it < nsteps; it++) {
+    OR4_RECORD(it);
+    OR4_APPLY(U2, P);
+    OR4_SRC(pulse, it, 1.0, P);
+    for (long g = 0; g < NT; g++) P[g] *= dt2;
+    for (long g = 0; g < NT; g++) V[g] = 0.5 * dt * (U0[g] - 4.0 * U1[g] + 3.0 * U2[g]);
+    for (long g = 0; g < NT; g++) W[g] = (P[g] - OR4_S(g) * V[g]) / Dm[g % N];
+    OR4_DIR(W);
+    OR4_APPLY(V, X);
+    for (long g = 0; g < NT; g++) X[g] -= OR4_S(g) * W[g];
+    OR4_SRC(d1, it, dt2, X);
+    for (long g = 0; g < NT; g++) X[g] /= Dm[g % N];
+    OR4_DIR(X);
+    OR4_APPLY(W, Z);
+    for (long g = 0; g < NT; g++) Z[g] -= OR4_S(g) * X[g];
+    OR4_SRC(d2, it, dt2, Z);
+    for (long g = 0; g < NT; g++) {
+      double q = P[g] + dt2 / 12.0 * Z[g];
+      q += dt2 / 6.0 * (OR4_S(g) * X[g]);
+      q += 2.0 * (Dm[g % N] * U2[g]);
+      q -= (Dm[g % N] - 0.5 * dt * OR4_S(g)) * U1[g];
+      U0[g] = q / (Dm[g % N] + 0.5 * dt * OR4_S(g)); /* the new level, in U0's storage */
+    }
+    double *tmp = U0; U0 = U1; U1 = U2; U2 = tmp;
+    OR4_DIR(U2);
+  }
+  OR4_RECORD(nsteps);
+  if (uN) memcpy(uN, U1, bytes);
+  if (uN1) memcpy(uN1, U2, bytes);
+#undef OR4_APPLY
+#undef OR4_S
+#undef OR4_DIR
+#undef OR4_SRC
+#undef OR4_RECORD
+  free(U0); free(U1); free(U2); free(V); free(P); free(W); free(X); free(Z);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Largest eigenvalue of D^{-1} A (CFL)                                     */
 /* ------------------------------------------------------------------------ */

@@ -83,6 +83,11 @@ def lib():
                               _ip, C.c_int, _dp, C.c_void_p, _bp, C.c_void_p, C.c_void_p,
                               C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_int, _lp, _dp,
                               C.c_void_p, _dp, _dp, _dp]
+    L.or_leapfrog4.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, _ip,
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               _ip, C.c_int, _dp, C.c_void_p, _bp, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
+                               C.c_void_p, C.c_int, _lp, _dp, C.c_void_p, _dp, _dp, _dp]
     L.or_lambda_max.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, C.c_long, _ip,
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 _ip, C.c_int, _dp, _bp, C.c_int, C.c_uint64]
@@ -794,6 +799,34 @@ def leapfrog(P: Problem, b, pulse, dt, nsteps, rec_pts=None, rec_comp=None, u0=N
                       m.dirichlet, _ptr(b), _ptr(pulse), dt, nsteps, _ptr(u0), _ptr(du0), nrec,
                       np.ascontiguousarray(re), np.ascontiguousarray(rw.reshape(-1)), _ptr(rc),
                       uN, uN1, tr)
+    return uN, uN1, tr[: (nsteps + 1) * nrec].reshape(nsteps + 1, nrec)
+
+
+def leapfrog4(P: Problem, b, tables, dt, nsteps, rec_pts=None, rec_comp=None, u0=None, du0=None,
+              S=None, rec_weights=None):
+    """Fourth-order modified-equation leapfrog (or_leapfrog4): tables = (f, f',
+    f'') at t_n + dt (any may be None).  Returns (U^N, U^{N+1}, traces)."""
+    m = P.mesh
+    c, nc = P.colors
+    NT = m.N * P.ncomp
+    if rec_weights is not None:
+        re = np.ascontiguousarray(rec_weights[0], np.int64)
+        rw = np.ascontiguousarray(rec_weights[1], np.float64)
+    elif rec_pts is not None and len(rec_pts):
+        re, rw = receivers(m, rec_pts)
+    else:
+        re, rw = np.zeros(0, np.int64), np.zeros(0)
+    nrec = len(re)
+    rc = None if rec_comp is None else np.ascontiguousarray(rec_comp, dtype=np.int32)
+    uN, uN1 = np.zeros(NT), np.zeros(NT)
+    tr = np.zeros(((nsteps + 1) * max(nrec, 1)))
+    tabs = [None if t is None else np.ascontiguousarray(t, np.float64) for t in tables]
+    lib().or_leapfrog4(m.dim, m.r, P.ncomp, m.E, m.N, m.elem_nodes.reshape(-1), _ptr(P.G),
+                       _ptr(P.jw), _ptr(P.dfi), _ptr(P.lam), _ptr(P.mu), c, nc, P.M,
+                       _ptr(None if S is None else np.ascontiguousarray(S, np.float64)),
+                       m.dirichlet, _ptr(b), _ptr(tabs[0]), _ptr(tabs[1]), _ptr(tabs[2]), dt,
+                       nsteps, _ptr(u0), _ptr(du0), nrec, np.ascontiguousarray(re),
+                       np.ascontiguousarray(rw.reshape(-1)), _ptr(rc), uN, uN1, tr)
     return uN, uN1, tr[: (nsteps + 1) * nrec].reshape(nsteps + 1, nrec)
 
 

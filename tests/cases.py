@@ -41,6 +41,8 @@ class Case:
     temporal: str = "RICKER"
     # elastic outputs: the stress (DisplayStress = YES, the default) or grad u
     display_stress: bool = True
+    # OrderTimeScheme = r LEAP_FROG: 2 or 4 (the modified equation)
+    time_order: int = 2
 
     def source_kind(self) -> tuple:
         w = self.temporal.split()
@@ -104,7 +106,8 @@ def load(name: str) -> Case:
                 rho_e, lam_e, float(z["f0"]), float(z["dt"]), int(z["nsteps"]), z["src"],
                 z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e, pml,
                 str(z["temporal"]) if "temporal" in z.files else "RICKER",
-                bool(z["display_stress"]) if "display_stress" in z.files else True)
+                bool(z["display_stress"]) if "display_stress" in z.files else True,
+                int(z["time_order"]) if "time_order" in z.files else 2)
 
 
 def oracle_pulse(c: Case, nsteps: int) -> np.ndarray:
@@ -115,6 +118,22 @@ def oracle_pulse(c: Case, nsteps: int) -> np.ndarray:
     return O.source_table(kind, c.f0, par, 0.0, c.dt, nsteps)
 
 
+def oracle_pulse_derivatives(c: Case, nsteps: int):
+    """f' and f'' of the case's TemporalSource at t_n + dt (the fourth-order
+    scheme's source terms)."""
+    kind, par = c.source_kind()
+    return tuple(O.source_table(kind, c.f0, par, 0.0, c.dt, nsteps, k) for k in (2, 3))
+
+
+def oracle_leapfrog(c: Case, P, b, nsteps: int, **kw):
+    """O.leapfrog, or O.leapfrog4 for OrderTimeScheme = 4 cases."""
+    pulse = oracle_pulse(c, nsteps)
+    if c.time_order == 4:
+        d1, d2 = oracle_pulse_derivatives(c, nsteps)
+        return O.leapfrog4(P, b, (pulse, d1, d2), c.dt, nsteps, **kw)
+    return O.leapfrog(P, b, pulse, c.dt, nsteps, **kw)
+
+
 def gpu_pulse(c: Case, nsteps: int) -> np.ndarray:
     """The library's pulse table of the case's TemporalSource."""
     import montjoie_amd as M
@@ -122,6 +141,19 @@ def gpu_pulse(c: Case, nsteps: int) -> np.ndarray:
     if kind == "RICKER":
         return M.ricker_pulse_table(c.f0, 0.0, c.dt, nsteps)
     return M.temporal_source_table(kind, c.f0, par, 0.0, c.dt, nsteps)
+
+
+def gpu_set_pulse(c: Case, P, nsteps: int):
+    """The case's source tables on a library problem (f' and f'' too for the
+    fourth-order scheme)."""
+    import montjoie_amd as M
+    if c.time_order == 4:
+        kind, par = c.source_kind()
+        P.set_pulse(gpu_pulse(c, nsteps),
+                    *(M.temporal_source_table(kind, c.f0, par, 0.0, c.dt, nsteps, k)
+                      for k in (2, 3)))
+    else:
+        P.set_pulse(gpu_pulse(c, nsteps))
 
 
 def pml_setup(c: Case):
@@ -173,14 +205,13 @@ def oracle_run(c: Case, nsteps: int | None = None):
         P = O.acoustic_problem(m, c.mu_e, c.rho_e)
         b = O.dirac(m, c.src)
         rc = None
-    pulse = oracle_pulse(c, nsteps)
     S = O.damping(m, c.sigma_e) if c.sigma_e is not None and np.any(c.sigma_e) else None
     if c.elastic:
         # every receiver on the three components, component-major
         pts = np.vstack([c.rec] * c.dim)
         comp = np.repeat(np.arange(c.dim, dtype=np.int32), len(c.rec))
-        return O.leapfrog(P, b, pulse, c.dt, nsteps, pts, comp, S=S)
-    return O.leapfrog(P, b, pulse, c.dt, nsteps, c.rec, rc, S=S)
+        return oracle_leapfrog(c, P, b, nsteps, rec_pts=pts, rec_comp=comp, S=S)
+    return oracle_leapfrog(c, P, b, nsteps, rec_pts=c.rec, rec_comp=rc, S=S)
 
 
 def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
@@ -202,9 +233,8 @@ def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
     re = np.repeat(el, d * d)
     rw = np.stack([Gw[r, a] for r in range(npt) for i in range(d) for a in range(d)])
     comp = np.array([i for r in range(npt) for i in range(d) for a in range(d)], np.int32)
-    pulse = oracle_pulse(c, nsteps)
     S = O.damping(m, c.sigma_e) if c.sigma_e is not None and np.any(c.sigma_e) else None
-    _, _, g = O.leapfrog(P, b, pulse, c.dt, nsteps, rec_comp=comp, rec_weights=(re, rw), S=S)
+    _, _, g = oracle_leapfrog(c, P, b, nsteps, rec_comp=comp, rec_weights=(re, rw), S=S)
     g = g.reshape(nsteps + 1, npt, d, d)
     if not c.display_stress:
         return g.reshape(nsteps + 1, npt * d * d)
@@ -275,6 +305,8 @@ def gpu_problem(c: Case, precision: str = "fp64", nshots: int = 1, srcs=None):
         P = M.WaveProblem(m, c.rho_e, c.mu_e, nshots=nshots, precision=precision)
     if c.sigma_e is not None and np.any(c.sigma_e):
         P.set_damping(c.sigma_e)
+    if c.time_order != 2:
+        P.set_time_order(c.time_order)
     if srcs is None:
         srcs = [c.src] * nshots
     pol = None if not c.elastic else np.vstack([c.pol] * len(srcs))
@@ -295,7 +327,7 @@ def gpu_run(c: Case, precision="fp64", nsteps=None, graph=True, nshots=1, srcs=N
         # first-order system: the primitive at t_n + dt/2
         P.set_pulse(M.time_source_table(c.f0, 0.0, c.dt, nsteps, 0, 0.5 * c.dt))
     else:
-        P.set_pulse(gpu_pulse(c, nsteps))
+        gpu_set_pulse(c, P, nsteps)
     P.leapfrog_init(c.dt)
     tr = P.run(nsteps, graph=graph)
     return P, tr

@@ -105,7 +105,7 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
                mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
                output_format="BINARY DOUBLE", circle=None, pml=None, young=None,
-               temporal="RICKER", display_stress=True):
+               temporal="RICKER", display_stress=True, time_order=2):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  circle = (x0, y0,
     radius, n): a SismoCircle line after the SismoPoint lines.  pml (pml_spec):
     AddPML = YES axes thickness layers, DampingPML (and TransverseDampingPML)
@@ -116,7 +116,8 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     young = (E_layers, nu_layers): elastic media written as YOUNG_POISSON
     rho E nu instead of ISOTROPE rho lambda mu.  temporal: the TemporalSource
     line's value (a kind and its parameter).  display_stress = False writes
-    DisplayStress = NO (elastic outputs hold grad u instead of the stress)."""
+    DisplayStress = NO (elastic outputs hold grad u instead of the stress).
+    time_order = 4: OrderTimeScheme = 4 LEAP_FROG (the modified equation)."""
     os.makedirs(d, exist_ok=True)
     m = O.mesh(dim, nx, r, pert, seed)
     nl = len(layers)
@@ -208,7 +209,7 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     ini.append("ElectricOrMagnetic = " + ("-1" if elastic else "0"))
     ini.append(f"TimeStep = {_f(dt)}")
     ini.append(f"TimeInterval = 0.0 {_f(T)}")
-    ini.append("OrderTimeScheme = 2 " + ("LEAP_FROG_PML" if pml is not None and len(pml)
+    ini.append(f"OrderTimeScheme = {int(time_order)} " + ("LEAP_FROG_PML" if pml is not None and len(pml)
                                            else "LEAP_FROG"))
     ini.append(f"TemporalSource = {temporal or 'RICKER'}")
     if elastic and not display_stress:

@@ -52,7 +52,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
              mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
              output_format="BINARY DOUBLE", circle=None, pml=None, young=None,
-             temporal="RICKER", display_stress=True):
+             temporal="RICKER", display_stress=True, time_order=2):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
@@ -60,7 +60,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
                line=line, line_times=line_times, mesh_media=mesh_media,
                save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma,
                output_format=output_format, circle=circle, pml=pml, young=young,
-               temporal=temporal, display_stress=display_stress)
+               temporal=temporal, display_stress=display_stress, time_order=time_order)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini", str(dim)]
     for f in os.listdir(d):
@@ -127,7 +127,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         circle=np.asarray(circle if circle is not None else [], float),
         young_E=np.asarray(young[0] if young is not None else [], float),
         young_nu=np.asarray(young[1] if young is not None else [], float),
-        temporal=temporal, display_stress=display_stress,
+        temporal=temporal, display_stress=display_stress, time_order=time_order,
         **_pml_fields(pml, dim, nx),
         circle_files=circles,
         circle_names=names.get("totalCircle", np.zeros(0, "U1")),
@@ -256,6 +256,47 @@ def ref2d_el_q4_grad():
              seeded_points(2, 5, 99), elastic=True, lam_layers=lam, rho_layers=rho,
              pol=(0.5, 0.5), plane=[0.05, 0.95, 0.1, 0.9, 7, 6], plane_times=[0.05, 0.15, 0.05],
              display_stress=False)
+
+
+@case
+def ref2d_q4_order4():
+    """OrderTimeScheme = 4 LEAP_FROG (fourth-order modified equation): 8x8 Q4
+    perturbed, three layers, at a time step past the second-order CFL limit
+    (dt = 1.5 x 2/sqrt(lambda_max), below 2 sqrt(3)/sqrt(lambda_max))."""
+    m = O.mesh(2, 8, 4, 0.1, 49)
+    lay = [1.0, 2.0, 1.5]
+    e = np.arange(m.E)
+    P = O.acoustic_problem(m, np.asarray(lay)[((e // 8) * 3) // 8], 1.0)
+    dt = 1.5 * 2.0 / np.sqrt(O.lambda_max(P, 200, 5))
+    run_case("ref2d_q4_order4", 2, 8, 4, 0.1, 49, lay, 10.0, dt, int(0.3 / dt), (0.47, 0.52),
+             seeded_points(2, 5, 100), time_order=4)
+
+
+@case
+def ref2d_q4_order4_damped():
+    """the fourth-order scheme with damping (sigma 0, 3, 12 by layer): the
+    S d^3U/dt^3 terms and the three time levels"""
+    run_case("ref2d_q4_order4_damped", 2, 8, 4, 0.1, 50, [1.0, 2.0, 1.5], 10.0, 1e-3, 300,
+             (0.47, 0.52), seeded_points(2, 5, 101), sigma_layers=[0.0, 3.0, 12.0], time_order=4)
+
+
+@case
+def ref3d_q3_order4():
+    """the fourth-order scheme on 4^3 Q3 perturbed hexahedra, two layers"""
+    run_case("ref3d_q3_order4", 3, 4, 3, 0.1, 51, [1.0, 2.5], 8.0, 5e-3, 150,
+             (0.43, 0.52, 0.47), seeded_points(3, 4, 102), time_order=4)
+
+
+@case
+def ref3d_el_q3_order4():
+    """the fourth-order scheme for isotropic elastodynamics: 3^3 Q3 perturbed
+    hexahedra, two layers"""
+    lam = [1.0 + 3.0 * O.uniform(2054, k) for k in range(2)]
+    mu = [1.0 + 3.0 * O.uniform(2055, k) for k in range(2)]
+    rho = [1.0 + O.uniform(2056, k) for k in range(2)]
+    run_case("ref3d_el_q3_order4", 3, 3, 3, 0.1, 27, mu, 8.0, 4e-3, 120, (0.47, 0.52, 0.5),
+             seeded_points(3, 4, 103), elastic=True, lam_layers=lam, rho_layers=rho,
+             pol=(0.3, 0.6, -0.7), time_order=4)
 
 
 def _temporal_case(name, temporal, nsteps=250):

@@ -45,7 +45,8 @@ def _write(c: CS.Case, d):
                      pml=CS.pml_deck(c) if c.pml is not None else None,
                      young=(z["young_E"], z["young_nu"])
                      if "young_E" in z.files and len(z["young_E"]) else None,
-                     temporal=c.temporal, display_stress=c.display_stress)
+                     temporal=c.temporal, display_stress=c.display_stress,
+                     time_order=c.time_order)
 
 
 def _dump(path):

@@ -134,7 +134,7 @@ def test_gradient_receivers_give_montjoie_stress(name):
     o = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     el, _, _ = O.receiver_gradients(o, c.rec)
     assert np.array_equal(elem[::d * d], el)
-    P.set_pulse(CS.gpu_pulse(c, c.nsteps))
+    CS.gpu_set_pulse(c, P, c.nsteps)
     P.leapfrog_init(c.dt)
     g = P.run(c.nsteps)[:, 0, :].reshape(c.nsteps + 1, npt, d, d)
     lam = c.lam_e[el][None, :, None, None]
