@@ -1622,8 +1622,6 @@ int main(int argc, char **argv) {
     if (s->empty()) die("SaveReprise needs a number of steps");
     save_every = (int64_t)num((*s)[0]);
     if (save_every <= 0) die("SaveReprise: the number of steps must be positive");
-    if (time_order == 4)
-      die("SaveReprise: checkpoints of the second-order scheme only (order 4 keeps three levels)");
   }
   const std::string path_reprise = d.get("PathReprise") ? d.need("PathReprise")[0] : "./";
   // OutputFormat = BINARY|ASCII|... DOUBLE|FLOAT (Output/OutputHarmonic.cxx:321-380): the

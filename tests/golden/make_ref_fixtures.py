@@ -273,6 +273,15 @@ def ref2d_q4_order4():
 
 
 @case
+def ref2d_q4_order4_reprise():
+    """checkpoint files of the fourth-order scheme (SaveReprise = 60: the
+    iterate U^n, as for the second-order one, WriteSnapshot ->
+    SaveDataReprise, VarInstationary.cxx:797-801,917-979)"""
+    run_case("ref2d_q4_order4_reprise", 2, 6, 4, 0.1, 52, [1.0, 2.0], 10.0, 1e-3, 200,
+             (0.47, 0.52), seeded_points(2, 3, 104), save_reprise=60, time_order=4)
+
+
+@case
 def ref2d_q4_order4_damped():
     """the fourth-order scheme with damping (sigma 0, 3, 12 by layer): the
     S d^3U/dt^3 terms and the three time levels"""

@@ -496,7 +496,7 @@ def test_oracle_checkpoints_match_montjoie(name):
     assert ints("IterationAcousSave0N1") == [3, saves[-2][0], -1, saves[-2][1]]
     m, dof = _montjoie_dofs(c, bool(z["regular"]))
     assert np.array_equal(np.sort(dof), np.arange(m.N))
-    ncomp = 3 if c.elastic else 1
+    ncomp = c.dim if c.elastic else 1
     for nt, b in saves[-2:]:
         raw = z[f"reprise_SolAcous{b}_P0N1"].tobytes()
         assert np.frombuffer(raw[:4], np.int32)[0] == ncomp * m.N
