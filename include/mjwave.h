@@ -231,6 +231,24 @@ int mjw_set_pulse_derivatives(mjw_problem *p, int nsteps, const double *d1, cons
 #define MJW_TIME_RICKER 1
 int mjw_set_time_source(mjw_problem *p, int kind, double f0, double t_init, double eps_time);
 /*
+ * Sources given as whole vectors (a volume source f b, TypeSource =
+ * SRC_VOLUME, DefineSourceElliptic.cxx:470-560): F[nshots][ncomp*n_nodes],
+ * caller's numbering, component-blocked; replaces the sources of every shot
+ * (as mjw_set_sources does), Dirichlet rows and entries below
+ * eps*||F_shot|| dropped (VarInstationary.cxx:637-658).
+ */
+int mjw_set_source_vectors(mjw_problem *p, const double *F);
+/*
+ * out[g] = integral of f phi_g over the mesh with the elements' own
+ * Gauss-Lobatto quadrature (the nodes): sum over the elements holding g of
+ * f(x_g) J w.  f_q[n_elem][(order+1)^dim] gives f at each element's
+ * quadrature points (caller's element order, local node order), so f may
+ * jump between elements; out has n_nodes entries, caller's numbering.  This
+ * is AddVolumetricSource's load of a volume source
+ * (DefineSourceElliptic.cxx:856-960).
+ */
+int mjw_lumped_integral(mjw_problem *p, const double *f_q, double *out);
+/*
  * The same for every TemporalSource of the explicit path (GetNewTimeSource,
  * VarInstationary.cxx:409-481; primitives Source/TimeSource.cxx), through the
  * same interpolator and windows of 0.5/f0, with `param` the keyword's value:

@@ -52,6 +52,8 @@ SIGNATURES = {
     "mjw_add_scalar_time_source": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, _dp]),
     "mjw_get_mass": (C.c_int, [_vp, _dp]),
     "mjw_set_sources": (C.c_int, [_vp, C.c_int, _dp, _dp, _i32p]),
+    "mjw_set_source_vectors": (C.c_int, [_vp, _dp]),
+    "mjw_lumped_integral": (C.c_int, [_vp, _dp, _dp]),
     "mjw_set_pulse": (C.c_int, [_vp, C.c_int, _dp]),
     "mjw_set_pulse_derivatives": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "mjw_set_time_order": (C.c_int, [_vp, C.c_int]),

@@ -1049,6 +1049,23 @@ static std::pair<const int32_t *, const int32_t *> candidates(mjw_problem *p, co
   return {p->loc_elem.data() + p->loc_ptr[k], p->loc_elem.data() + p->loc_ptr[k + 1]};
 }
 
+static int store_sources(mjw_problem *p, const std::vector<std::map<int64_t, double>> &acc);
+
+extern "C" int mjw_set_source_vectors(mjw_problem *p, const double *F) {
+  if (!p || !F) FAIL(p, MJW_ERR_ARG, "mjw_set_source_vectors: NULL");
+  HIPCHK(p, hipSetDevice(p->device));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  const int64_t n1 = (int64_t)p->ncomp * p->N;
+  std::vector<std::map<int64_t, double>> acc(p->nshots);
+  for (int s = 0; s < p->nshots; ++s)
+    for (int64_t i = 0; i < n1; ++i) {
+      const double v = F[(size_t)s * n1 + i];
+      if (!std::isfinite(v)) FAIL(p, MJW_ERR_ARG, "mjw_set_source_vectors: non-finite entry");
+      if (v != 0.0) acc[s][i] = v;
+    }
+  return store_sources(p, acc);
+}
+
 extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, const double *pol,
                                const int32_t *shot) {
   if (!p || nsrc < 0 || (nsrc > 0 && !pts)) FAIL(p, MJW_ERR_ARG, "mjw_set_sources: bad args");
@@ -1085,8 +1102,13 @@ extern "C" int mjw_set_sources(mjw_problem *p, int nsrc, const double *pts, cons
       }
     }
   }
-  // per shot: drop Dirichlet rows, then values below eps*||F||
-  // (VarInstationary.cxx:637-658)
+  return store_sources(p, acc);
+}
+
+// The device lists of the source vectors acc[shot] (caller numbering m*N + g):
+// per shot, Dirichlet rows dropped, then values below eps*||F||
+// (VarInstationary.cxx:637-658)
+static int store_sources(mjw_problem *p, const std::vector<std::map<int64_t, double>> &acc) {
   std::vector<int64_t> dof, hdof;
   std::vector<int32_t> sh;
   std::vector<double> val;
@@ -1768,6 +1790,9 @@ extern "C" int mjw_set_probes_grad(mjw_problem *p, int npts, const double *pts,
 // elastic: sigma I in the mass tensor, VarElastic.cxx:59-71, so every
 // component of a node takes the same S, assembled by TimeElastic.cxx:296-330)
 // sigma per element (nodal = false) or per quadrature point [E][nd] (nodal)
+static int lumped_integral(mjw_problem *p, const double *vals, bool nodal,
+                           std::vector<double> &out);
+
 static int set_damping(mjw_problem *p, const double *sigma, bool nodal) {
   if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_damping: physical damping with PML layers is not supported");
   if (!p || !sigma) FAIL(p, MJW_ERR_ARG, "mjw_set_damping: NULL");
@@ -1791,6 +1816,26 @@ static int set_damping(mjw_problem *p, const double *sigma, bool nodal) {
     p->damp_c.clear();
     return MJW_OK;
   }
+  std::vector<double> S;
+  const int rc = lumped_integral(p, ss.data(), nodal, S);
+  if (rc) {
+    p->damp_S.clear();
+    p->damp_c.clear();
+    return rc;
+  }
+  p->damp_S = S;
+  p->damp_c.assign(N, 0.0);
+  for (int64_t g = 0; g < N; ++g)
+    if (!p->dir[g]) p->damp_c[g] = p->damp_S[p->nperm[g]];
+  return MJW_OK;
+}
+
+// out[i] = sum_e sum_q v J w phi_i(x_q) (solver order): v per element
+// (nodal = false) or per quadrature point [E][nd] (sorted element order,
+// local order), through the geometry kernel with v in the place of rho
+static int lumped_integral(mjw_problem *p, const double *vals, bool nodal,
+                           std::vector<double> &out) {
+  const int64_t E = p->E, N = p->N, per = nodal ? p->nd : 1;
   double xg[kMaxN], wg[kMaxN], Dg[kMaxN * kMaxN];
   gl_rule(p->order, xg, wg);
   gl_deriv(p->order, xg, Dg);
@@ -1807,14 +1852,14 @@ static int set_damping(mjw_problem *p, const double *sigma, bool nodal) {
   ok(hipMalloc(&d_Dg, p->n * p->n * 8));
   ok(hipMalloc(&d_w, p->n * 8));
   ok(hipMalloc(&d_Mloc, (size_t)E * p->nd * 8));
-  ok(hipMalloc(&d_sig, ss.size() * 8));
+  ok(hipMalloc(&d_sig, (size_t)E * per * 8));
   ok(hipMalloc(&d_S, (size_t)N * 8));
   ok(hipMalloc(&d_bad, 4));
   ok(hipMemsetAsync(d_bad, 0, 4, st));
   ok(hipMemcpyAsync(d_xyz, xyz_s.data(), xyz_s.size() * 8, hipMemcpyHostToDevice, st));
   ok(hipMemcpyAsync(d_Dg, Dg, p->n * p->n * 8, hipMemcpyHostToDevice, st));
   ok(hipMemcpyAsync(d_w, wg, p->n * 8, hipMemcpyHostToDevice, st));
-  ok(hipMemcpyAsync(d_sig, ss.data(), ss.size() * 8, hipMemcpyHostToDevice, st));
+  ok(hipMemcpyAsync(d_sig, vals, (size_t)E * per * 8, hipMemcpyHostToDevice, st));
   // sigma in the place of rho: the kernel's "mass" output is sigma J w
   if (e == hipSuccess &&
       launch_geometry_full(p->dim, p->order, 1, E, p->d_map, d_xyz, d_Dg, d_w,
@@ -1825,18 +1870,31 @@ static int set_damping(mjw_problem *p, const double *sigma, bool nodal) {
     if (launch_mass_assemble(p->d_map, d_Mloc, E, p->dim, p->order, p->coff[c], p->coff[c + 1],
                              d_S, st))
       e = hipErrorLaunchFailure;
-  p->damp_S.assign(N, 0.0);
-  ok(hipMemcpyAsync(p->damp_S.data(), d_S, (size_t)N * 8, hipMemcpyDeviceToHost, st));
+  out.assign(N, 0.0);
+  ok(hipMemcpyAsync(out.data(), d_S, (size_t)N * 8, hipMemcpyDeviceToHost, st));
   ok(hipStreamSynchronize(st));
   dfree(d_xyz); dfree(d_Dg); dfree(d_w); dfree(d_Mloc); dfree(d_sig); dfree(d_S); dfree(d_bad);
-  if (e != hipSuccess) {
-    p->damp_S.clear();
-    p->damp_c.clear();
-    FAIL(p, MJW_ERR_HIP, std::string("mjw_set_damping: ") + hipGetErrorString(e));
-  }
-  p->damp_c.assign(N, 0.0);
-  for (int64_t g = 0; g < N; ++g)
-    if (!p->dir[g]) p->damp_c[g] = p->damp_S[p->nperm[g]];
+  if (e != hipSuccess) FAIL(p, MJW_ERR_HIP, std::string("lumped integral: ") + hipGetErrorString(e));
+  return MJW_OK;
+}
+
+extern "C" int mjw_lumped_integral(mjw_problem *p, const double *f_q, double *out) {
+  if (!p || !f_q || !out) FAIL(p, MJW_ERR_ARG, "mjw_lumped_integral: NULL");
+  HIPCHK(p, hipSetDevice(p->device));
+  HIPCHK(p, hipStreamSynchronize(p->stream));
+  const int64_t E = p->E, nd = p->nd;
+  // caller's element order -> sorted (colour) order
+  std::vector<double> v((size_t)E * nd);
+  for (int64_t es = 0; es < E; ++es)
+    for (int64_t q = 0; q < nd; ++q) {
+      const double x = f_q[p->perm[es] * nd + q];
+      if (!std::isfinite(x)) FAIL(p, MJW_ERR_ARG, "mjw_lumped_integral: non-finite value");
+      v[(size_t)es * nd + q] = x;
+    }
+  std::vector<double> b;
+  const int rc = lumped_integral(p, v.data(), true, b);
+  if (rc) return rc;
+  for (int64_t g = 0; g < p->N; ++g) out[g] = b[p->nperm[g]];
   return MJW_OK;
 }
 

@@ -21,7 +21,8 @@
 // OrderDiscretization, ConditionReference (DIRICHLET), MateriauDielec
 // (ISOTROPE, elastic also YOUNG_POISSON; damping sigma), PhysicalMedia
 // (rho / mu / sigma, SINUS or MESH / SAME_MESH data files on the
-// computational mesh, acoustic), TypeSource (SRC_DIRAC), OriginePhase, Polarization, Frequency,
+// computational mesh, acoustic), TypeSource (SRC_DIRAC, SRC_VOLUME GAUSSIAN), OriginePhase,
+// Polarization, Frequency,
 // TemporalSource (RICKER, DERIVATIVE_RICKER, MODIFIED_RICKER, GAUSSIAN, HARMONIC,
 // SINUS_GAUSSIAN), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme (2 or 4
 // LEAP_FROG, 2 LEAP_FROG_PML), SismoPoint, FileOutputPoint, ParametersOutputPoint, SismoCircle,
@@ -42,6 +43,7 @@
 // node map in local order q = i + n j + n^2 k, Dirichlet flags, media) as raw
 // little-endian arrays and exits without touching the GPU.
 #include <algorithm>
+#include <cctype>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -1588,8 +1590,35 @@ int main(int argc, char **argv) {
     if (d.get("PhysicalMedia")) die("PhysicalMedia: not available with LEAP_FROG_PML (per-reference media only)");
     if (d.need("TimeStep")[0] == "AUTO") die("TimeStep = AUTO: not available with LEAP_FROG_PML");
   }
-  if (const auto *s = d.get("TypeSource"))
-    if ((*s)[0] != "SRC_DIRAC") die("TypeSource " + (*s)[0] + ": only SRC_DIRAC");
+  // TypeSource = SRC_DIRAC (at OriginePhase), or SRC_VOLUME [references]
+  // GAUSSIAN x y [z] radius [cut-off radius] (DefineSourceElliptic.cxx:
+  // 470-540, InitGaussianParameter :169-195): f = beta exp(-alpha r^2),
+  // alpha = -ln(1e-6) / radius^2, beta = sqrt(alpha / pi), 0 from the cut-off
+  // radius on (the radius when not given; GaussianSource::GetAmplitude,
+  // Source/SourceSpatiale.cxx:72-97), on the elements of the references
+  // (every element when none is given), times the Polarization for elastic
+  bool volume_source = false;
+  std::vector<int> vol_refs;
+  double vol_x[3] = {0, 0, 0}, vol_r = 0, vol_rc = 0;
+  if (const auto *s = d.get("TypeSource")) {
+    const std::string k = s->empty() ? std::string() : (*s)[0];
+    if (k == "SRC_VOLUME") {
+      size_t w = 1;
+      while (w < s->size() && !std::isalpha((unsigned char)(*s)[w][0]))
+        vol_refs.push_back((int)num((*s)[w++]));
+      if (w >= s->size() || (*s)[w] != "GAUSSIAN")
+        die("TypeSource = SRC_VOLUME: only the GAUSSIAN volume source");
+      ++w;
+      if (s->size() < w + dim + 1) die("TypeSource = SRC_VOLUME GAUSSIAN needs the centre and a radius");
+      for (int a = 0; a < dim; ++a) vol_x[a] = num((*s)[w++]);
+      vol_r = num((*s)[w++]);
+      vol_rc = (w < s->size() && !std::isalpha((unsigned char)(*s)[w][0])) ? num((*s)[w]) : vol_r;
+      if (!(vol_r > 0.0)) die("TypeSource = SRC_VOLUME GAUSSIAN: the radius must be positive");
+      volume_source = true;
+    } else if (k != "SRC_DIRAC") {
+      die("TypeSource " + k + ": only SRC_DIRAC and SRC_VOLUME GAUSSIAN");
+    }
+  }
   // TemporalSource = KIND [value] (GetNewTimeSource, VarInstationary.cxx:409-481)
   int tkind = MJW_TIME_RICKER;
   double tpar = 0.0;
@@ -1736,7 +1765,34 @@ int main(int argc, char **argv) {
   const int64_t nsteps = (int64_t)std::ceil((t1 - t0) / dt - 1e-9);  // RunSecondOrderScheme
   if (nsteps < 1) die("TimeInterval shorter than one TimeStep");
   // source
-  {
+  if (volume_source) {
+    std::vector<double> pol(ncomp, 1.0);
+    if (elastic) {
+      const auto &pv = d.need("Polarization");
+      for (int a = 0; a < dim; ++a) pol[a] = num(pv.at(a));
+    }
+    const double pi = 3.14159265358979323846;
+    const double alpha = -std::log(1.e-6) / (vol_r * vol_r), beta = std::sqrt(alpha / pi);
+    const int64_t nd = qm.en.size() / qm.E;
+    std::vector<double> fq((size_t)qm.E * nd, 0.0);
+    for (int64_t e = 0; e < qm.E; ++e) {
+      if (!vol_refs.empty() &&
+          std::find(vol_refs.begin(), vol_refs.end(), q1.elem_ref[e]) == vol_refs.end())
+        continue;
+      for (int64_t q = 0; q < nd; ++q) {
+        const double *xq = &qm.xyz[(size_t)qm.en[e * nd + q] * dim];
+        double r2 = 0.0;
+        for (int a = 0; a < dim; ++a) r2 += (xq[a] - vol_x[a]) * (xq[a] - vol_x[a]);
+        const double r = std::sqrt(r2);
+        if (r < vol_rc) fq[e * nd + q] = beta * std::exp(-alpha * (r * r));
+      }
+    }
+    std::vector<double> b(qm.N), F((size_t)ncomp * qm.N);
+    check(mjw_lumped_integral(p, fq.data(), b.data()), p, "SRC_VOLUME");
+    for (int m = 0; m < ncomp; ++m)
+      for (int64_t g = 0; g < qm.N; ++g) F[(size_t)m * qm.N + g] = pol[m] * b[g];
+    check(mjw_set_source_vectors(p, F.data()), p, "SRC_VOLUME");
+  } else {
     const auto &op = d.need("OriginePhase");
     if ((int)op.size() < dim) die("OriginePhase needs " + std::to_string(dim) + " coordinates");
     double x[3];

@@ -285,6 +285,22 @@ class WaveProblem:
         return D
 
     # ---------------------------------------------------- sources/receivers
+    def set_source_vectors(self, F):
+        """Whole source vectors [nshots, ncomp*N] in place of Dirac points
+        (mjw_set_source_vectors)."""
+        F = np.ascontiguousarray(np.broadcast_to(np.asarray(F, np.float64),
+                                                 (self.nshots, self.ndof)))
+        self._chk(_lib.lib().mjw_set_source_vectors(self._h, dptr(F)), "set_source_vectors")
+
+    def lumped_integral(self, f_q) -> np.ndarray:
+        """integral of f phi_g with the Gauss-Lobatto quadrature, f at the
+        quadrature points [E, nd] (mjw_lumped_integral)."""
+        f = np.ascontiguousarray(np.broadcast_to(np.asarray(f_q, np.float64),
+                                                 (self.mesh.E, self.mesh.nd)))
+        out = np.zeros(self.mesh.N)
+        self._chk(_lib.lib().mjw_lumped_integral(self._h, dptr(f), dptr(out)), "lumped_integral")
+        return out
+
     def set_sources(self, pts, pol=None, shot=None):
         pts = np.ascontiguousarray(np.asarray(pts, np.float64).reshape(-1, self.mesh.dim))
         k = pts.shape[0]

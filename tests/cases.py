@@ -43,6 +43,8 @@ class Case:
     display_stress: bool = True
     # OrderTimeScheme = r LEAP_FROG: 2 or 4 (the modified equation)
     time_order: int = 2
+    # TypeSource = SRC_VOLUME [refs] GAUSSIAN centre radius cut-off, or None
+    volume: tuple | None = None
 
     def source_kind(self) -> tuple:
         w = self.temporal.split()
@@ -107,7 +109,42 @@ def load(name: str) -> Case:
                 z["pol"] if elastic else None, z["rec"], z["times"], z["traces"], sigma_e, pml,
                 str(z["temporal"]) if "temporal" in z.files else "RICKER",
                 bool(z["display_stress"]) if "display_stress" in z.files else True,
-                int(z["time_order"]) if "time_order" in z.files else 2)
+                int(z["time_order"]) if "time_order" in z.files else 2,
+                # the elements of the references (the layers, 1-based; all
+                # when none is given), centre, radius, cut-off
+                (np.isin(lay + 1, z["volume_refs"]) if len(z["volume_refs"]) else
+                 np.ones(E, bool), z["volume_centre"], float(z["volume_radius"][0]),
+                 float(z["volume_radius"][1]))
+                if "volume_centre" in z.files and len(z["volume_centre"]) else None)
+
+
+def gaussian_volume(pts, centre, radius, cut):
+    """The GAUSSIAN volume source's amplitude beta exp(-alpha r^2), alpha =
+    -ln(1e-6)/radius^2, beta = sqrt(alpha/pi), 0 from r = cut on
+    (GaussianSource::Init / GetAmplitude, Source/SourceSpatiale.cxx:72-97)."""
+    alpha = -np.log(1e-6) / radius ** 2
+    beta = np.sqrt(alpha / np.pi)
+    r = np.sqrt(((np.asarray(pts) - np.asarray(centre)) ** 2).sum(-1))
+    return np.where(r < cut, beta * np.exp(-alpha * r * r), 0.0)
+
+
+def volume_field(c: Case, pts) -> np.ndarray:
+    """f at the quadrature points pts [E, nd, dim] of the case's volume
+    source, zero on the elements outside its references."""
+    inside, centre, radius, cut = c.volume
+    return np.where(inside[:, None], gaussian_volume(pts, centre, radius, cut), 0.0)
+
+
+def oracle_source(c: Case, m) -> np.ndarray:
+    """The case's source vector from the oracle: the Dirac at src, or the
+    volume source's integral of f phi_i (component-blocked, times the
+    polarisation for elastic)."""
+    if c.volume is None:
+        return O.dirac(m, c.src, c.pol) if c.elastic else O.dirac(m, c.src)
+    b = O.damping(m, volume_field(c, O.quadrature_points(m)))
+    if c.elastic:
+        return np.concatenate([p * b for p in c.pol])
+    return b
 
 
 def oracle_pulse(c: Case, nsteps: int) -> np.ndarray:
@@ -199,12 +236,11 @@ def oracle_run(c: Case, nsteps: int | None = None):
     m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     if c.elastic:
         P = O.elastic_problem(m, c.lam_e, c.mu_e, c.rho_e)
-        b = O.dirac(m, c.src, c.pol)
         rc = np.zeros(0, np.int32)
     else:
         P = O.acoustic_problem(m, c.mu_e, c.rho_e)
-        b = O.dirac(m, c.src)
         rc = None
+    b = oracle_source(c, m)
     S = O.damping(m, c.sigma_e) if c.sigma_e is not None and np.any(c.sigma_e) else None
     if c.elastic:
         # every receiver on the three components, component-major
@@ -225,7 +261,7 @@ def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
     d = c.dim
     m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     P = O.elastic_problem(m, c.lam_e, c.mu_e, c.rho_e)
-    b = O.dirac(m, c.src, c.pol)
+    b = oracle_source(c, m)
     el, _, Gw = O.receiver_gradients(m, c.rec)
     assert (el >= 0).all()
     npt = len(c.rec)
@@ -307,10 +343,16 @@ def gpu_problem(c: Case, precision: str = "fp64", nshots: int = 1, srcs=None):
         P.set_damping(c.sigma_e)
     if c.time_order != 2:
         P.set_time_order(c.time_order)
-    if srcs is None:
-        srcs = [c.src] * nshots
-    pol = None if not c.elastic else np.vstack([c.pol] * len(srcs))
-    P.set_sources(np.asarray(srcs), pol=pol, shot=np.arange(len(srcs)) % nshots)
+    if c.volume is not None and srcs is None:
+        # the volume source through the library's own quadrature sum
+        b = P.lumped_integral(volume_field(c, m.xyz[m.elem_nodes]))
+        F = np.concatenate([p * b for p in c.pol]) if c.elastic else b
+        P.set_source_vectors(np.tile(F, (nshots, 1)))
+    else:
+        if srcs is None:
+            srcs = [c.src] * nshots
+        pol = None if not c.elastic else np.vstack([c.pol] * len(srcs))
+        P.set_sources(np.asarray(srcs), pol=pol, shot=np.arange(len(srcs)) % nshots)
     if c.elastic:
         P.set_receivers(np.vstack([c.rec] * c.dim),
                         comp=np.repeat(np.arange(c.dim), len(c.rec)))

@@ -105,7 +105,7 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
                sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
                mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
                output_format="BINARY DOUBLE", circle=None, pml=None, young=None,
-               temporal="RICKER", display_stress=True, time_order=2):
+               temporal="RICKER", display_stress=True, time_order=2, volume=None):
     """case.ini (+ mesh.mesh unless REGULAR) in directory d.  circle = (x0, y0,
     radius, n): a SismoCircle line after the SismoPoint lines.  pml (pml_spec):
     AddPML = YES axes thickness layers, DampingPML (and TransverseDampingPML)
@@ -117,7 +117,9 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     rho E nu instead of ISOTROPE rho lambda mu.  temporal: the TemporalSource
     line's value (a kind and its parameter).  display_stress = False writes
     DisplayStress = NO (elastic outputs hold grad u instead of the stress).
-    time_order = 4: OrderTimeScheme = 4 LEAP_FROG (the modified equation)."""
+    time_order = 4: OrderTimeScheme = 4 LEAP_FROG (the modified equation).
+    volume = (refs, centre, radius, cut-off): TypeSource = SRC_VOLUME [refs]
+    GAUSSIAN centre radius cut-off instead of the Dirac at src."""
     os.makedirs(d, exist_ok=True)
     m = O.mesh(dim, nx, r, pert, seed)
     nl = len(layers)
@@ -181,7 +183,12 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
         ini.append(sinus_line("mu", 1, sinus_mu))
     if sinus_sigma is not None and len(sinus_sigma):
         ini.append(sinus_line("sigma", 1, sinus_sigma))
-    ini.append("TypeSource = SRC_DIRAC")
+    if volume is not None and len(volume[1]):
+        refs, centre, rad, cut = volume
+        ini.append("TypeSource = SRC_VOLUME " + "".join(f"{int(k)} " for k in refs) + "GAUSSIAN "
+                   + " ".join(_f(c) for c in centre) + f" {_f(rad)} {_f(cut)}")
+    else:
+        ini.append("TypeSource = SRC_DIRAC")
     ini.append("OriginePhase = " + " ".join(_f(c) for c in src))
     if elastic:
         ini.append("Polarization = " + " ".join(_f(c) for c in pol))

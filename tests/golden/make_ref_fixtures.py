@@ -52,7 +52,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
              sinus_mu=None, plane=None, plane_times=None, line=None, line_times=None,
              mesh_media=None, save_reprise=0, sigma_layers=None, sinus_sigma=None,
              output_format="BINARY DOUBLE", circle=None, pml=None, young=None,
-             temporal="RICKER", display_stress=True, time_order=2):
+             temporal="RICKER", display_stress=True, time_order=2, volume=None):
     d = os.path.join(RUN, name)
     write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elastic=elastic,
                lam_layers=lam_layers, rho_layers=rho_layers, pol=pol, regular=regular,
@@ -60,7 +60,8 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
                line=line, line_times=line_times, mesh_media=mesh_media,
                save_reprise=save_reprise, sigma_layers=sigma_layers, sinus_sigma=sinus_sigma,
                output_format=output_format, circle=circle, pml=pml, young=young,
-               temporal=temporal, display_stress=display_stress, time_order=time_order)
+               temporal=temporal, display_stress=display_stress, time_order=time_order,
+               volume=volume)
     exe = os.path.join(REF, "mjref_elastic.x" if elastic else "mjref_acous.x")
     args = [exe, "case.ini", str(dim)]
     for f in os.listdir(d):
@@ -128,6 +129,9 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         young_E=np.asarray(young[0] if young is not None else [], float),
         young_nu=np.asarray(young[1] if young is not None else [], float),
         temporal=temporal, display_stress=display_stress, time_order=time_order,
+        volume_refs=np.asarray(volume[0] if volume else [], int),
+        volume_centre=np.asarray(volume[1] if volume else [], float),
+        volume_radius=np.asarray(volume[2:] if volume else [], float),
         **_pml_fields(pml, dim, nx),
         circle_files=circles,
         circle_names=names.get("totalCircle", np.zeros(0, "U1")),
@@ -270,6 +274,37 @@ def ref2d_q4_order4():
     dt = 1.5 * 2.0 / np.sqrt(O.lambda_max(P, 200, 5))
     run_case("ref2d_q4_order4", 2, 8, 4, 0.1, 49, lay, 10.0, dt, int(0.3 / dt), (0.47, 0.52),
              seeded_points(2, 5, 100), time_order=4)
+
+
+@case
+def ref2d_q4_src_volume():
+    """TypeSource = SRC_VOLUME GAUSSIAN 0.47 0.52 0.15 0.25: a Gaussian volume
+    source (radius 0.15, cut off at 0.25) on every element, 8x8 Q4 perturbed,
+    two layers"""
+    run_case("ref2d_q4_src_volume", 2, 8, 4, 0.1, 53, [1.0, 2.0], 10.0, 1e-3, 250,
+             (0.47, 0.52), seeded_points(2, 5, 105), volume=([], (0.47, 0.52), 0.15, 0.25))
+
+
+@case
+def ref3d_q3_src_volume_ref():
+    """SRC_VOLUME 1 GAUSSIAN ...: the volume source restricted to the elements
+    of reference 1 (the lower layer), its Gaussian crossing the layer
+    interface; 4^3 Q3 perturbed, two layers"""
+    run_case("ref3d_q3_src_volume_ref", 3, 4, 3, 0.1, 54, [1.0, 2.5], 8.0, 5e-3, 150,
+             (0.5, 0.5, 0.5), seeded_points(3, 4, 106),
+             volume=([1], (0.46, 0.52, 0.45), 0.2, 0.3))
+
+
+@case
+def ref2d_el_q4_src_volume():
+    """elastic: the Gaussian volume source times the Polarization; 8x8 Q4,
+    two layers"""
+    lam = [1.0 + 3.0 * O.uniform(2057, k) for k in range(2)]
+    mu = [1.0 + 3.0 * O.uniform(2058, k) for k in range(2)]
+    rho = [1.0 + O.uniform(2059, k) for k in range(2)]
+    run_case("ref2d_el_q4_src_volume", 2, 8, 4, 0.1, 55, mu, 10.0, 1e-3, 250, (0.5, 0.5),
+             seeded_points(2, 4, 107), elastic=True, lam_layers=lam, rho_layers=rho,
+             pol=(0.6, -0.8), volume=([], (0.52, 0.47), 0.12, 0.2))
 
 
 @case

@@ -46,7 +46,9 @@ def _write(c: CS.Case, d):
                      young=(z["young_E"], z["young_nu"])
                      if "young_E" in z.files and len(z["young_E"]) else None,
                      temporal=c.temporal, display_stress=c.display_stress,
-                     time_order=c.time_order)
+                     time_order=c.time_order,
+                     volume=(z["volume_refs"], z["volume_centre"], *z["volume_radius"])
+                     if "volume_centre" in z.files and len(z["volume_centre"]) else None)
 
 
 def _dump(path):
