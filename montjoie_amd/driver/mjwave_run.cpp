@@ -21,7 +21,7 @@
 // OrderDiscretization, ConditionReference (DIRICHLET), MateriauDielec
 // (ISOTROPE, elastic also YOUNG_POISSON; damping sigma), PhysicalMedia
 // (rho / mu / sigma, SINUS or MESH / SAME_MESH data files on the
-// computational mesh, acoustic), TypeSource (SRC_DIRAC, SRC_VOLUME GAUSSIAN), OriginePhase,
+// computational mesh, acoustic), TypeSource (SRC_DIRAC, SRC_VOLUME GAUSSIAN / UNIFORM), OriginePhase,
 // Polarization, Frequency,
 // TemporalSource (RICKER, DERIVATIVE_RICKER, MODIFIED_RICKER, GAUSSIAN, HARMONIC,
 // SINUS_GAUSSIAN), TimeStep (a value or AUTO), TimeInterval, OrderTimeScheme (2 or 4
@@ -1591,13 +1591,14 @@ int main(int argc, char **argv) {
     if (d.need("TimeStep")[0] == "AUTO") die("TimeStep = AUTO: not available with LEAP_FROG_PML");
   }
   // TypeSource = SRC_DIRAC (at OriginePhase), or SRC_VOLUME [references]
+  // UNIFORM (f = 1, UniformSourceField, Source/SourceSpatiale.cxx:188-204) or
   // GAUSSIAN x y [z] radius [cut-off radius] (DefineSourceElliptic.cxx:
-  // 470-540, InitGaussianParameter :169-195): f = beta exp(-alpha r^2),
+  // 470-560, InitGaussianParameter :169-195): f = beta exp(-alpha r^2),
   // alpha = -ln(1e-6) / radius^2, beta = sqrt(alpha / pi), 0 from the cut-off
   // radius on (the radius when not given; GaussianSource::GetAmplitude,
   // Source/SourceSpatiale.cxx:72-97), on the elements of the references
   // (every element when none is given), times the Polarization for elastic
-  bool volume_source = false;
+  bool volume_source = false, vol_uniform = false;
   std::vector<int> vol_refs;
   double vol_x[3] = {0, 0, 0}, vol_r = 0, vol_rc = 0;
   if (const auto *s = d.get("TypeSource")) {
@@ -1606,14 +1607,18 @@ int main(int argc, char **argv) {
       size_t w = 1;
       while (w < s->size() && !std::isalpha((unsigned char)(*s)[w][0]))
         vol_refs.push_back((int)num((*s)[w++]));
-      if (w >= s->size() || (*s)[w] != "GAUSSIAN")
-        die("TypeSource = SRC_VOLUME: only the GAUSSIAN volume source");
+      if (w >= s->size() || ((*s)[w] != "GAUSSIAN" && (*s)[w] != "UNIFORM"))
+        die("TypeSource = SRC_VOLUME: only the GAUSSIAN and UNIFORM volume sources");
+      vol_uniform = (*s)[w] == "UNIFORM";
       ++w;
-      if (s->size() < w + dim + 1) die("TypeSource = SRC_VOLUME GAUSSIAN needs the centre and a radius");
-      for (int a = 0; a < dim; ++a) vol_x[a] = num((*s)[w++]);
-      vol_r = num((*s)[w++]);
-      vol_rc = (w < s->size() && !std::isalpha((unsigned char)(*s)[w][0])) ? num((*s)[w]) : vol_r;
-      if (!(vol_r > 0.0)) die("TypeSource = SRC_VOLUME GAUSSIAN: the radius must be positive");
+      if (!vol_uniform) {
+        if (s->size() < w + dim + 1)
+          die("TypeSource = SRC_VOLUME GAUSSIAN needs the centre and a radius");
+        for (int a = 0; a < dim; ++a) vol_x[a] = num((*s)[w++]);
+        vol_r = num((*s)[w++]);
+        vol_rc = (w < s->size() && !std::isalpha((unsigned char)(*s)[w][0])) ? num((*s)[w]) : vol_r;
+        if (!(vol_r > 0.0)) die("TypeSource = SRC_VOLUME GAUSSIAN: the radius must be positive");
+      }
       volume_source = true;
     } else if (k != "SRC_DIRAC") {
       die("TypeSource " + k + ": only SRC_DIRAC and SRC_VOLUME GAUSSIAN");
@@ -1772,7 +1777,8 @@ int main(int argc, char **argv) {
       for (int a = 0; a < dim; ++a) pol[a] = num(pv.at(a));
     }
     const double pi = 3.14159265358979323846;
-    const double alpha = -std::log(1.e-6) / (vol_r * vol_r), beta = std::sqrt(alpha / pi);
+    const double alpha = vol_uniform ? 0.0 : -std::log(1.e-6) / (vol_r * vol_r);
+    const double beta = std::sqrt(alpha / pi);
     const int64_t nd = qm.en.size() / qm.E;
     std::vector<double> fq((size_t)qm.E * nd, 0.0);
     for (int64_t e = 0; e < qm.E; ++e) {
@@ -1780,6 +1786,10 @@ int main(int argc, char **argv) {
           std::find(vol_refs.begin(), vol_refs.end(), q1.elem_ref[e]) == vol_refs.end())
         continue;
       for (int64_t q = 0; q < nd; ++q) {
+        if (vol_uniform) {
+          fq[e * nd + q] = 1.0;
+          continue;
+        }
         const double *xq = &qm.xyz[(size_t)qm.en[e * nd + q] * dim];
         double r2 = 0.0;
         for (int a = 0; a < dim; ++a) r2 += (xq[a] - vol_x[a]) * (xq[a] - vol_x[a]);

@@ -113,9 +113,9 @@ def load(name: str) -> Case:
                 # the elements of the references (the layers, 1-based; all
                 # when none is given), centre, radius, cut-off
                 (np.isin(lay + 1, z["volume_refs"]) if len(z["volume_refs"]) else
-                 np.ones(E, bool), z["volume_centre"], float(z["volume_radius"][0]),
-                 float(z["volume_radius"][1]))
-                if "volume_centre" in z.files and len(z["volume_centre"]) else None)
+                 np.ones(E, bool), z["volume_centre"],
+                 *(list(z["volume_radius"]) if len(z["volume_radius"]) else [0.0, 0.0]))
+                if "volume_kind" in z.files and str(z["volume_kind"]) else None)
 
 
 def gaussian_volume(pts, centre, radius, cut):
@@ -132,6 +132,8 @@ def volume_field(c: Case, pts) -> np.ndarray:
     """f at the quadrature points pts [E, nd, dim] of the case's volume
     source, zero on the elements outside its references."""
     inside, centre, radius, cut = c.volume
+    if not len(centre):  # UNIFORM
+        return np.where(inside[:, None], 1.0, np.zeros(pts.shape[:2]))
     return np.where(inside[:, None], gaussian_volume(pts, centre, radius, cut), 0.0)
 
 

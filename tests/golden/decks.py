@@ -119,7 +119,8 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
     DisplayStress = NO (elastic outputs hold grad u instead of the stress).
     time_order = 4: OrderTimeScheme = 4 LEAP_FROG (the modified equation).
     volume = (refs, centre, radius, cut-off): TypeSource = SRC_VOLUME [refs]
-    GAUSSIAN centre radius cut-off instead of the Dirac at src."""
+    GAUSSIAN centre radius cut-off instead of the Dirac at src; an empty
+    centre writes SRC_VOLUME [refs] UNIFORM."""
     os.makedirs(d, exist_ok=True)
     m = O.mesh(dim, nx, r, pert, seed)
     nl = len(layers)
@@ -183,7 +184,10 @@ def write_deck(d, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, elas
         ini.append(sinus_line("mu", 1, sinus_mu))
     if sinus_sigma is not None and len(sinus_sigma):
         ini.append(sinus_line("sigma", 1, sinus_sigma))
-    if volume is not None and len(volume[1]):
+    if volume is not None and not len(volume[1]):
+        ini.append("TypeSource = SRC_VOLUME " + "".join(f"{int(k)} " for k in volume[0])
+                   + "UNIFORM")
+    elif volume is not None:
         refs, centre, rad, cut = volume
         ini.append("TypeSource = SRC_VOLUME " + "".join(f"{int(k)} " for k in refs) + "GAUSSIAN "
                    + " ".join(_f(c) for c in centre) + f" {_f(rad)} {_f(cut)}")

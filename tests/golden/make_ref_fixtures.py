@@ -132,6 +132,7 @@ def run_case(name, dim, nx, r, pert, seed, layers, f0, dt, nsteps, src, rec, ela
         volume_refs=np.asarray(volume[0] if volume else [], int),
         volume_centre=np.asarray(volume[1] if volume else [], float),
         volume_radius=np.asarray(volume[2:] if volume else [], float),
+        volume_kind="" if not volume else ("GAUSSIAN" if len(volume[1]) else "UNIFORM"),
         **_pml_fields(pml, dim, nx),
         circle_files=circles,
         circle_names=names.get("totalCircle", np.zeros(0, "U1")),
@@ -305,6 +306,14 @@ def ref2d_el_q4_src_volume():
     run_case("ref2d_el_q4_src_volume", 2, 8, 4, 0.1, 55, mu, 10.0, 1e-3, 250, (0.5, 0.5),
              seeded_points(2, 4, 107), elastic=True, lam_layers=lam, rho_layers=rho,
              pol=(0.6, -0.8), volume=([], (0.52, 0.47), 0.12, 0.2))
+
+
+@case
+def ref2d_q3_src_uniform_ref():
+    """TypeSource = SRC_VOLUME 2 UNIFORM: f = 1 on the elements of reference 2
+    (the middle layer); 8x8 Q3 perturbed, three layers"""
+    run_case("ref2d_q3_src_uniform_ref", 2, 8, 3, 0.1, 56, [1.0, 2.0, 1.5], 10.0, 1e-3, 250,
+             (0.5, 0.5), seeded_points(2, 5, 108), volume=([2], [], 0.0, 0.0))
 
 
 @case

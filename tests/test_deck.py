@@ -48,7 +48,7 @@ def _write(c: CS.Case, d):
                      temporal=c.temporal, display_stress=c.display_stress,
                      time_order=c.time_order,
                      volume=(z["volume_refs"], z["volume_centre"], *z["volume_radius"])
-                     if "volume_centre" in z.files and len(z["volume_centre"]) else None)
+                     if "volume_kind" in z.files and str(z["volume_kind"]) else None)
 
 
 def _dump(path):
