@@ -411,7 +411,7 @@ int mjw_adjoint_timing(const mjw_problem *p, double *ms2);
  * rebuilds the problem from the data file). */
 int mjw_set_mu(mjw_problem *p, const double *mu);
 
-/* Media varying inside the elements (acoustic): rho_q and mu_q hold rho and mu
+/* Media varying inside the elements: rho_q and mu_q hold rho and mu
  * at every quadrature point, [n_elem][(r+1)^dim] in the caller's element order
  * and the local node order of mjw_create.  Replaces ScalarPhysicalIndice /
  * TensorPhysicalIndice with a varying field (PhysicalMedia = rho|mu ref
@@ -421,7 +421,9 @@ int mjw_set_mu(mjw_problem *p, const double *mu);
  * tensor J w mu DF^{-1} DF^{-T}.  Rebuilds the geometric factors and the lumped
  * mass; the problem then uses stored geometry (MJW_GEOM_TRILINEAR is refused),
  * mjw_set_mu and the adjoint are refused, and mjw_leapfrog_init must be called
- * again. */
+ * again.  Elastic problems take rho_q only (mu_q = NULL): the density enters
+ * the mass rho J w alone (TimeElastic, ElasticEquation::GetTensorMass,
+ * VarElastic.cxx:59-71); a varying stiffness tensor C is not supported. */
 int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const double *mu_q);
 /*
  * Damping sigma >= 0 per element (acoustic: rho u'' + sigma u' - div(mu grad u)

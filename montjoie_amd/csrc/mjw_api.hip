@@ -1607,7 +1607,8 @@ static int build_graph(mjw_problem *p, bool with_traces) {
   if (e != hipSuccess) return (int)e;
   int rc = 0;
   const int gs = graph_steps(p);
-  for (int s = 0; s < gs && !rc; ++s) {
+  int s = 0;
+  for (; s < gs && !rc; ++s) {
     if (p->torder == 4) {
       // the captured steps rotate the levels as the eager ones do; after
       // the whole cycle the pointers are back where they started
@@ -1617,6 +1618,12 @@ static int build_graph(mjw_problem *p, bool with_traces) {
     } else {
       rc = enqueue_step(p, (p->cur + s) & 1, with_traces ? p->d_traces : nullptr);
     }
+  }
+  // a capture stopped part-way has run nothing: finish the cycle of the
+  // levels so that they are where they were
+  for (; p->torder == 4 && s < gs; ++s) {
+    me4_rotate(p);
+    p->cur ^= 1;
   }
   e = hipStreamEndCapture(p->stream, &g);
   if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -2166,8 +2173,33 @@ extern "C" int mjw_set_mu(mjw_problem *p, const double *mu) {
 // applies, so the problem switches to stored geometry for good.
 extern "C" int mjw_set_media_nodal(mjw_problem *p, const double *rho_q, const double *mu_q) {
   if (p && p->pml) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_media_nodal: not available with PML layers");
-  if (!p || !rho_q || !mu_q) FAIL(p, MJW_ERR_ARG, "mjw_set_media_nodal: NULL");
-  if (p->ncomp != 1) FAIL(p, MJW_ERR_UNSUPPORTED, "mjw_set_media_nodal: acoustic problems only");
+  if (!p || !rho_q) FAIL(p, MJW_ERR_ARG, "mjw_set_media_nodal: NULL");
+  if (p->ncomp != 1) {
+    // elastic: the density alone varies (it enters the mass only; a
+    // varying stiffness tensor is not supported)
+    if (mu_q)
+      FAIL(p, MJW_ERR_UNSUPPORTED,
+           "mjw_set_media_nodal: elastic problems take rho only (mu_q = NULL)");
+    HIPCHK(p, hipSetDevice(p->device));
+    const int64_t E = p->E, nd = p->nd;
+    std::vector<double> rq((size_t)E * nd);
+    for (int64_t es = 0; es < E; ++es)
+      for (int64_t q = 0; q < nd; ++q) {
+        const double r = rho_q[p->perm[es] * nd + q];
+        if (!(r > 0) || !std::isfinite(r))
+          FAIL(p, MJW_ERR_ARG, "mjw_set_media_nodal: rho must be finite, > 0");
+        rq[(size_t)es * nd + q] = r;
+      }
+    std::vector<double> ms;
+    const int rc = lumped_integral(p, rq.data(), true, ms);
+    if (rc) return rc;
+    p->mass = ms;  // solver order; finish_mass renumbers and uploads it
+    HIPCHK(p, finish_mass(p));
+    if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+    p->inited = false;
+    return MJW_OK;
+  }
+  if (!mu_q) FAIL(p, MJW_ERR_ARG, "mjw_set_media_nodal: NULL mu_q");
   HIPCHK(p, hipSetDevice(p->device));
   const int64_t E = p->E, nd = p->nd, N = p->N;
   std::vector<double> rq((size_t)E * nd), mq((size_t)E * nd);

@@ -1265,25 +1265,28 @@ NodalMedia nodal_media(const Deck &d, const Q1Mesh &m, const QrMesh &qm, const M
   NodalMedia nm;
   const auto lines = d.all("PhysicalMedia");
   if (lines.empty()) return nm;
-  if (elastic) die("PhysicalMedia: varying media are supported for ACOUSTIC decks only");
   const int64_t nd = (int64_t)qm.en.size() / qm.E;
   bool rhomu = false, sig = false;
   for (const auto &v : lines) {
     if (v.size() < 4) die("PhysicalMedia = name ref ISOTROPE SINUS|MESH|SAME_MESH ...");
-    if (v[0] != "rho" && v[0] != "mu" && v[0] != "sigma")
-      die("PhysicalMedia " + v[0] + ": only rho, mu and sigma");
+    // elastic media are rho, C and sigma (VarElastic.cxx:1752-1760); the
+    // stiffness tensor C does not vary on this path
+    if (elastic && v[0] == "C") die("PhysicalMedia C: a varying elastic stiffness is not supported");
+    if (elastic ? (v[0] != "rho" && v[0] != "sigma")
+                : (v[0] != "rho" && v[0] != "mu" && v[0] != "sigma"))
+      die("PhysicalMedia " + v[0] + (elastic ? ": only rho and sigma" : ": only rho, mu and sigma"));
     (v[0] == "sigma" ? sig : rhomu) = true;
   }
   if (rhomu) {
     nm.rho.resize(qm.en.size());
-    nm.mu.resize(qm.en.size());
+    if (!elastic) nm.mu.resize(qm.en.size());
   }
   if (sig) nm.sigma.resize(qm.en.size());
   for (int64_t e = 0; e < qm.E; ++e)
     for (int64_t q = 0; q < nd; ++q) {
       if (rhomu) {
         nm.rho[e * nd + q] = md.rho[e];
-        nm.mu[e * nd + q] = md.mu[e];
+        if (!elastic) nm.mu[e * nd + q] = md.mu[e];
       }
       if (sig) nm.sigma[e * nd + q] = md.sigma.empty() ? 0.0 : md.sigma[e];
     }
@@ -1712,7 +1715,15 @@ int main(int argc, char **argv) {
     // MateriauDielec's damping sigma per element (zeros when undamped)
     wr(o, md.sigma.empty() ? std::vector<double>((size_t)qm.E, 0.0) : md.sigma);
     wr(o, nm.rho);  // empty unless PhysicalMedia: E * nd each
-    wr(o, nm.mu);
+    // (elastic: mu per element expanded to the quadrature points)
+    std::vector<double> muq = nm.mu;
+    if (muq.empty() && !nm.rho.empty()) {
+      const int64_t nd = (int64_t)qm.en.size() / qm.E;
+      muq.resize(nm.rho.size());
+      for (int64_t e = 0; e < qm.E; ++e)
+        for (int64_t q = 0; q < nd; ++q) muq[e * nd + q] = md.mu[e];
+    }
+    wr(o, muq);
     wr(o, montjoie_dofs(q1, qm));
     if (first_order) {
       // the layers: element flags, then sigma at every quadrature point
@@ -1739,7 +1750,8 @@ int main(int argc, char **argv) {
   if (first_order)
     check(mjw_set_pml(p, pml_flag.data(), pml_tau_q.data()), p, "AddPML");
   if (!nm.rho.empty())
-    check(mjw_set_media_nodal(p, nm.rho.data(), nm.mu.data()), p, "PhysicalMedia");
+    check(mjw_set_media_nodal(p, nm.rho.data(), nm.mu.empty() ? nullptr : nm.mu.data()), p,
+          "PhysicalMedia");
   if (!nm.sigma.empty())
     check(mjw_set_damping_nodal(p, nm.sigma.data()), p, "PhysicalMedia sigma");
   else if (!md.sigma.empty())

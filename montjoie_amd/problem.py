@@ -166,13 +166,14 @@ class WaveProblem:
         mu = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, np.float64), (self.mesh.E,)))
         self._chk(_lib.lib().mjw_set_mu(self._h, dptr(mu)), "set_mu")
 
-    def set_media_nodal(self, rho_q, mu_q):
+    def set_media_nodal(self, rho_q, mu_q=None):
         """rho and mu at every quadrature point, shape (n_elem, (r+1)^dim) in
-        the local node order (mjw_set_media_nodal).  Call leapfrog_init
-        afterwards."""
+        the local node order (mjw_set_media_nodal); elastic problems take rho
+        only (mu_q None).  Call leapfrog_init afterwards."""
         shape = (self.mesh.E, self.mesh.elem_nodes.shape[1])
         rq = np.ascontiguousarray(np.broadcast_to(np.asarray(rho_q, np.float64), shape))
-        mq = np.ascontiguousarray(np.broadcast_to(np.asarray(mu_q, np.float64), shape))
+        mq = None if mu_q is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(mu_q, np.float64), shape))
         self._chk(_lib.lib().mjw_set_media_nodal(self._h, dptr(rq), dptr(mq)), "set_media_nodal")
 
     def set_geometry(self, mode: str):

@@ -749,6 +749,16 @@ def acoustic_problem(m: Mesh, mu_e, rho_e) -> Problem:
 
 
 def elastic_problem(m: Mesh, lam_e, mu_e, rho_e) -> Problem:
+    """lam_e, mu_e per element; rho_e per element or (E, nd) at the quadrature
+    points (the mass rho J w only)."""
+    rho_a = np.asarray(rho_e, float)
+    if rho_a.ndim == 2:
+        jw, dfi, Mloc = geometry(m, np.ones_like(rho_a), rho_a, elastic=True)
+        M, Mi = mass(m, Mloc)
+        return Problem(m, m.dim, jw=jw, dfi=dfi,
+                       lam=np.ascontiguousarray(np.broadcast_to(np.asarray(lam_e, float), (m.E,))),
+                       mu=np.ascontiguousarray(np.broadcast_to(np.asarray(mu_e, float), (m.E,))),
+                       rho=rho_a.copy(), M=M, Minv=Mi, colors=color(m))
     jw, dfi, Mloc = geometry(m, mu_e, rho_e, elastic=True)
     M, Mi = mass(m, Mloc)
     return Problem(m, m.dim, jw=jw, dfi=dfi,

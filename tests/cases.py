@@ -87,8 +87,10 @@ def load(name: str) -> Case:
         pts = O.quadrature_points(m)
         rho_e = sinus_field(srho, pts) if len(srho) else np.broadcast_to(rho_e[:, None],
                                                                          pts.shape[:2]).copy()
-        mu_e = sinus_field(smu, pts) if len(smu) else np.broadcast_to(mu_e[:, None],
-                                                                      pts.shape[:2]).copy()
+        if len(smu):
+            mu_e = sinus_field(smu, pts)
+        elif not elastic:  # elastic decks keep mu (and lambda) per element
+            mu_e = np.broadcast_to(mu_e[:, None], pts.shape[:2]).copy()
     sg = z["sigma_layers"] if "sigma_layers" in z.files else np.zeros(0)
     sigma_e = sg[lay] if len(sg) else None
     ssg = z["sinus_sigma"] if "sinus_sigma" in z.files else np.zeros(0)
@@ -333,8 +335,11 @@ def gpu_problem(c: Case, precision: str = "fp64", nshots: int = 1, srcs=None):
         return P
     m = M.regular_mesh(c.dim, c.nx, c.r, c.pert, c.seed)
     if c.elastic:
-        P = M.WaveProblem(m, c.rho_e, c.mu_e, lam=c.lam_e, ncomp=c.dim, nshots=nshots,
-                          precision=precision)
+        nodal_rho = c.rho_e.ndim == 2
+        P = M.WaveProblem(m, 1.0 if nodal_rho else c.rho_e, c.mu_e, lam=c.lam_e, ncomp=c.dim,
+                          nshots=nshots, precision=precision)
+        if nodal_rho:  # PhysicalMedia rho: the mass only
+            P.set_media_nodal(c.rho_e)
     elif c.mu_e.ndim == 2:
         # media varying inside the elements
         P = M.WaveProblem(m, 1.0, 1.0, nshots=nshots, precision=precision)

@@ -309,6 +309,18 @@ def ref2d_el_q4_src_volume():
 
 
 @case
+def ref2d_el_q4_sinus():
+    """elastic PhysicalMedia: rho and sigma SINUS fields on the one medium
+    (PhysicalMedia = rho 1 ... SINUS, sigma 1 ... SINUS; MateriauDielec's
+    sigma 1 elsewhere in the box); 6x6 Q4 perturbed"""
+    run_case("ref2d_el_q4_sinus", 2, 6, 4, 0.1, 57, [2.0], 10.0, 1e-3, 250, (0.48, 0.51),
+             seeded_points(2, 4, 109), elastic=True, lam_layers=[3.0], rho_layers=[1.5],
+             pol=(0.7, 0.3), sigma_layers=[1.0],
+             sinus_rho=[0.05, 0.1, 0.9, 0.95, 0.02, 0.01, 1.5, 0.4, 0.6, 0.7],
+             sinus_sigma=[0.1, 0.05, 0.8, 0.9, 0.01, 0.03, 3.0, 2.0, 0.5, 0.4])
+
+
+@case
 def ref2d_q3_src_uniform_ref():
     """TypeSource = SRC_VOLUME 2 UNIFORM: f = 1 on the elements of reference 2
     (the middle layer); 8x8 Q3 perturbed, three layers"""

@@ -134,12 +134,15 @@ def test_driver_mesh_is_the_oracle_mesh(name, tmp_path):
     assert np.array_equal(perm[oen], g["en"]) and np.unique(perm).size == o.N
     np.testing.assert_allclose(g["xyz"][perm], o.xyz.reshape(o.N, -1), rtol=0, atol=1e-14)
     assert np.array_equal(g["dir"][perm], o.dirichlet.astype(np.uint8))
-    if c.mu_e.ndim == 2:
+    if c.mu_e.ndim == 2 or c.rho_e.ndim == 2:
         # PhysicalMedia SINUS evaluated by the driver at its quadrature points
         # against the oracle's field at the oracle's (same elements and
-        # local order; positions equal to 1e-14)
-        assert g["mu_q"].shape == c.mu_e.shape
-        np.testing.assert_allclose(g["mu_q"], c.mu_e, rtol=1e-13, atol=0)
+        # local order; positions equal to 1e-14); elastic decks keep mu per
+        # element
+        nd = g["rho_q"].shape[1]
+        mu_q = c.mu_e if c.mu_e.ndim == 2 else np.repeat(c.mu_e[:, None], nd, axis=1)
+        assert g["mu_q"].shape == mu_q.shape
+        np.testing.assert_allclose(g["mu_q"], mu_q, rtol=1e-13, atol=0)
         np.testing.assert_allclose(g["rho_q"], c.rho_e, rtol=1e-13, atol=0)
     else:
         assert g["mu_q"].size == 0
