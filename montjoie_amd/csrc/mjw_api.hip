@@ -88,6 +88,7 @@ struct mjw_problem {
   int torder = 2;
   void *d_U3 = nullptr;
   void *d_me[3] = {nullptr, nullptr, nullptr};
+  void *d_sdiag4 = nullptr;           // dt^2 / (12 D), 0 on Dirichlet nodes
   double *d_pulse_me = nullptr;
   double *d_Sme = nullptr;
   int me_phase = 0;                   // steps mod 6: the cycle of the three levels
@@ -288,6 +289,7 @@ static void release(mjw_problem *p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
   dfree(p->d_U3);
+  dfree(p->d_sdiag4);
   for (void *&v : p->d_me) dfree(v);
   dfree(p->d_pulse_me);
   dfree(p->d_Sme);
@@ -1345,6 +1347,8 @@ static int me4_init(mjw_problem *p, double t_begin, double dt, const double *u0,
     HIPCHK(p, hipMemcpy(p->d_pulse_me, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
   }
   KCHK(p, launch_make_sdiag(p->state, p->N, p->d_M, p->d_dir, dt * dt, p->d_sdiag, st));
+  if (!p->d_sdiag4) HIPCHK(p, dalloc(p, &p->d_sdiag4, (size_t)p->N * ssize(p)));
+  KCHK(p, launch_make_sdiag(p->state, p->N, p->d_M, p->d_dir, dt * dt / 12.0, p->d_sdiag4, st));
   p->dt = dt;
   p->t_begin = t_begin;
   p->zero_ic = (u0 == nullptr && du0 == nullptr);
@@ -1504,12 +1508,12 @@ static int enqueue_step_me4(mjw_problem *p, double *traces) {
     return launch_me4(p->state, op, ns, S, p->ilv, p->N, p->d_M, p->d_Sme, p->d_dir, a, b, x, u,
                       w, c0, c1, c2, y, st);
   };
-  auto tail = [&](int k, void *into, bool record, int advance) {
+  auto tail = [&](int k, void *into, bool record, int advance, const void *sd = nullptr) {
     TailArgs t{};
     t.state = p->state; t.ctl = p->d_ctl;
     t.pulse = p->d_pulse_me ? p->d_pulse_me + (int64_t)k * p->npulse : nullptr;
     t.npulse = p->npulse; t.nsrc = p->nsrc; t.src_dof = p->d_src_dof; t.src_shot = p->d_src_shot;
-    t.src_val = p->d_src_val; t.sdiag = nullptr; t.nrec = record && traces ? p->nrec : 0;
+    t.src_val = p->d_src_val; t.sdiag = sd; t.nrec = record && traces ? p->nrec : 0;
     t.nd = p->nd; t.rec_gid = p->d_rec_gid; t.rec_w = p->d_rec_w; t.rec_comp = p->d_rec_comp;
     t.U = Ucur; t.P = into; t.traces = (record && traces && p->nrec) ? traces : nullptr;
     t.N = p->N; t.nshots = S; t.ilv = p->ilv; t.advance = advance; t.row_offset = 1;
@@ -1524,7 +1528,22 @@ static int enqueue_step_me4(mjw_problem *p, double *traces) {
     if (!rc) rc = tail(1, X, false, 0);
     if (!rc) rc = node(kMeSolve, X, W, nullptr, nullptr, nullptr, 0, 0, 0, X);
   } else {
+    // undamped: U^{n+2} = W + 2 U^{n+1} - U^n - dt^2/12 D^{-1} A W
+    // + dt^4/12 D^{-1} f'' b.  U^{n-1}'s storage takes W - 2 U^{n+1} + U^n;
+    // the fused leapfrog pass on W with s = dt^2/(12 D) (first toucher:
+    // 2 W - y - s A W, the others y - s A W) leaves the first three terms,
+    // and the tail adds the source with the same s
     if (!rc) rc = node(kMeSolve, P, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, W);
+    if (!rc) rc = node(kMePre, W, Ucur, Uprev, nullptr, nullptr, 0, 0, 0, Unew);
+    for (int c = 0; c < p->ncolor && !rc; ++c) {
+      StiffArgs a = stiff_args(p, kLeap, W, Unew, S);
+      a.sdiag = p->d_sdiag4;
+      a.e0 = p->coff[c];
+      a.e1 = p->coff[c + 1];
+      rc = launch_stiffness(a, st);
+    }
+    if (!rc) rc = tail(2, Unew, false, 1, p->d_sdiag4);
+    return rc;
   }
   if (!rc) rc = apply_A(p, -1.0, W, 0.0, Z, S);
   if (!rc) rc = tail(2, Z, false, 1);

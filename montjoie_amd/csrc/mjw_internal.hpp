@@ -208,7 +208,7 @@ int launch_scale_by(int state, int64_t n, int S, int ilv, int64_t N, const doubl
                     void *y, void *stream);
 // y[i] = 0 where dir[node(i)]
 // node-wise steps of the fourth-order modified-equation leapfrog (k_me4)
-enum MeOp { kMeVel = 0, kMeSolve = 1, kMeSub = 2, kMeNew = 3, kMeTaylor = 4 };
+enum MeOp { kMeVel = 0, kMeSolve = 1, kMeSub = 2, kMeNew = 3, kMeTaylor = 4, kMePre = 5 };
 int launch_me4(int state, int op, int64_t n, int S, int ilv, int64_t N, const double *D,
                const double *Sd, const uint8_t *dir, const void *a, const void *b, const void *x,
                const void *u, const void *w, double c0, double c1, double c2, void *y,

@@ -451,6 +451,7 @@ int launch_mask(int state, int64_t n, int S, int ilv, int64_t N, const uint8_t *
 //   kMeSolve  y = mask(D^{-1} (a - S b))                  (b null: no S term)
 //   kMeSub    y = a - S b
 //   kMeNew    y = mask((D + c0 S)^{-1} (a + c1 b + c2 S x + 2 D u - (D - c0 S) w))
+//   kMePre    y = a - 2 b + x
 //   kMeTaylor y = w + c0 (c1 a + c0 (b/2 + c0 (c1 x/6 + c0 u/24))), c1 = +-1,
 //             zero on the Dirichlet nodes when c2 != 0 (SetInitialConditionS
 //             masks U^n and U^{n+1} but leaves U^{n-1} as it is)
@@ -474,6 +475,9 @@ __global__ void k_me4(int op, int64_t n, int64_t N, int64_t div, const double *_
         break;
       case kMeSub:
         r = (double)a[i] - s * (double)b[i];
+        break;
+      case kMePre:
+        r = (double)a[i] - 2.0 * (double)b[i] + (double)x[i];
         break;
       case kMeNew: {
         const double dg = D[g];
