@@ -146,10 +146,17 @@ class HipWaveSystem {
   // any TemporalSource kind (MJW_TIME_*) with its value, for both
   void SetTemporalSource(int kind, double f0, double param, double t_begin, double dt,
                          int nsteps) {
-    std::vector<double> tab(nsteps);
+    std::vector<double> tab(nsteps), tab2(nsteps), tab3(nsteps);
     check(mjw_temporal_source_table(kind, f0, param, t_begin, dt, nsteps, 1, dt, tab.data()),
           nullptr, "TemporalSource");
     check(mjw_set_pulse(p_, nsteps, tab.data()), p_, "SetTemporalSource");
+    // f' and f'' for the fourth-order scheme (unused at order 2)
+    check(mjw_temporal_source_table(kind, f0, param, t_begin, dt, nsteps, 2, dt, tab2.data()),
+          nullptr, "TemporalSource");
+    check(mjw_temporal_source_table(kind, f0, param, t_begin, dt, nsteps, 3, dt, tab3.data()),
+          nullptr, "TemporalSource");
+    check(mjw_set_pulse_derivatives(p_, nsteps, tab2.data(), tab3.data()), p_,
+          "SetTemporalSource");
     check(mjw_set_temporal_source(p_, kind, f0, param, t_begin, (nsteps * dt) * 1e-15), p_,
           "SetTemporalSource");
   }
@@ -184,6 +191,13 @@ class HipLeapFrogScheme {
  public:
   explicit HipLeapFrogScheme(HipWaveSystem &sys) : sys_(sys) {}
 
+  // ModifiedEquationIterator::SetOrder (ModifiedEquation.cxx:71-79): 2 or 4;
+  // order 4 also needs the source's f' and f'' (SetTemporalSource sets them)
+  void SetOrder(int r) {
+    check(mjw_set_time_order(sys_.handle(), r), sys_.handle(), "SetOrder");
+    order_ = r;
+  }
+
   // U^0 = u0, U^1 = U^0 + dt du0 + dt^2/2 D^{-1} K U^0 (+ Dirichlet)
   template <class V>
   void SetInitialConditionS(double t, double dt, const V &u0, const V &du0) {
@@ -217,13 +231,15 @@ class HipLeapFrogScheme {
     check(mjw_energy(sys_.handle(), e.data()), sys_.handle(), "Energy");
     return e;
   }
-  static double GetCFL() { return 2.0; }
-  static int GetNumberOfIterates() { return 2; }
+  // GetCFL, GetNumberOfIterates of ModifiedEquationIterator (:217-225, 264-270)
+  double GetCFL() const { return order_ == 4 ? 2.0 * std::sqrt(3.0) : 2.0; }
+  int GetNumberOfIterates() const { return order_ == 4 ? 3 : 2; }
   double GetTimeStep() const { return dt_; }
 
  private:
   HipWaveSystem &sys_;
   double dt_ = 0.0;
+  int order_ = 2;
 };
 
 }  // namespace mjw

@@ -27,5 +27,11 @@ int main() {
   std::vector<double> traces(nsteps * sys.GetNbReceivers());
   lf.AdvanceN(nsteps, traces.data());
   for (int n = 100; n < 103; ++n) std::printf("%.15e %.15e\n", traces[2 * n], traces[2 * n + 1]);
+  // the same run with the fourth-order scheme (OrderTimeScheme = 4 LEAP_FROG)
+  sys.SetTemporalSource(MJW_TIME_RICKER, 10.0, 0.0, 0.0, dt, nsteps);
+  lf.SetOrder(4);
+  lf.SetInitialConditionZero(0.0, dt);
+  lf.AdvanceN(nsteps, traces.data());
+  for (int n = 100; n < 103; ++n) std::printf("%.15e %.15e\n", traces[2 * n], traces[2 * n + 1]);
   return 0;
 }

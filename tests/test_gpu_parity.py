@@ -282,7 +282,8 @@ def test_xcd_block_order_is_result_neutral():
 
 def test_cpp_adapter_runs_on_gpu(tmp_path):
     """The C++ adapter (HipWaveSystem + HipLeapFrogScheme) reproduces the
-    oracle's traces (2-D Q4 8x8, 200 steps)."""
+    oracle's traces (2-D Q4 8x8, 200 steps), with the second- and the
+    fourth-order scheme."""
     import os
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -297,4 +298,8 @@ def test_cpp_adapter_runs_on_gpu(tmp_path):
     P = O.acoustic_problem(m, 1.0, 1.0)
     _, _, tr = O.leapfrog(P, O.dirac(m, (0.5, 0.5)), O.pulse_table(10.0, 0.0, 1e-3, 200), 1e-3,
                           200, [[0.8, 0.5], [0.3, 0.4]])
-    np.testing.assert_allclose(got, tr[100:103], rtol=1e-12, atol=1e-20)
+    np.testing.assert_allclose(got[:3], tr[100:103], rtol=1e-12, atol=1e-20)
+    tabs = tuple(O.source_table("RICKER", 10.0, 0.0, 0.0, 1e-3, 200, k) for k in (1, 2, 3))
+    _, _, tr4 = O.leapfrog4(P, O.dirac(m, (0.5, 0.5)), tabs, 1e-3, 200, [[0.8, 0.5], [0.3, 0.4]])
+    np.testing.assert_allclose(got[3:], tr4[100:103], rtol=1e-12, atol=1e-20)
+    assert not np.allclose(got[3:], got[:3], rtol=1e-9, atol=0)
