@@ -1850,25 +1850,40 @@ int main(int argc, char **argv) {
     for (int a = 0; a < dim; ++a) pts.push_back(num(v[a]));
     ++npt;
   }
-  // receivers: every component of u at every point (component-major); for
-  // elastic decks also d u_i / d x_a at every point (point-major, i outer),
-  // from which the stress columns are formed
+  // a SismoPoint outside the mesh keeps its columns, written as 0 (the point
+  // is never located, so its interpolation row stays empty:
+  // Output/GridInterpolation.cxx TranslatePoint); only the points inside
+  // become receivers, slot[r] = the point's index among them or -1
+  std::vector<uint8_t> inside(npt, 1);
+  std::vector<int> slot(npt, -1);
+  if (npt) {
+    check(mjw_set_probes(p, npt, pts.data(), inside.data()), p, "SismoPoint");
+    check(mjw_set_probes(p, 0, nullptr, nullptr), p, "SismoPoint");
+  }
+  int nin = 0;
+  for (int r = 0; r < npt; ++r)
+    if (inside[r]) slot[r] = nin++;
+  // receivers: every component of u at every point inside (component-major);
+  // for elastic decks also d u_i / d x_a at every point (point-major, i
+  // outer), from which the stress columns are formed
   std::vector<double> rpts;
   std::vector<int32_t> comp, deriv;
   for (int m = 0; m < ncomp; ++m)
     for (int r = 0; r < npt; ++r) {
+      if (!inside[r]) continue;
       for (int a = 0; a < dim; ++a) rpts.push_back(pts[(size_t)r * dim + a]);
       comp.push_back(m);
       deriv.push_back(-1);
     }
   if (elastic)
     for (int r = 0; r < npt; ++r)
-      for (int m = 0; m < dim; ++m)
-        for (int a = 0; a < dim; ++a) {
-          for (int b = 0; b < dim; ++b) rpts.push_back(pts[(size_t)r * dim + b]);
-          comp.push_back(m);
-          deriv.push_back(a);
-        }
+      if (inside[r])
+        for (int m = 0; m < dim; ++m)
+          for (int a = 0; a < dim; ++a) {
+            for (int b = 0; b < dim; ++b) rpts.push_back(pts[(size_t)r * dim + b]);
+            comp.push_back(m);
+            deriv.push_back(a);
+          }
   const int nrec = (int)comp.size();
   std::vector<int64_t> rec_elem(nrec, -1);
   if (nrec)
@@ -2034,7 +2049,7 @@ int main(int argc, char **argv) {
   const double t_loop = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
 
   // seismograms
-  if (nrec && d.get("FileOutputPoint")) {
+  if (npt && d.get("FileOutputPoint")) {
     const auto &fo = d.need("FileOutputPoint");
     const std::string out = fo.size() >= 2 ? fo[1] : fo[0];
     double ts0 = t0, ts1 = t0 + nsteps * dt, tsd = dt;
@@ -2053,7 +2068,7 @@ int main(int argc, char **argv) {
       // point-major: the components of point 0, then of point 1, ...
       for (int r = 0; r < npt; ++r)
         for (int m = 0; m < ncomp; ++m)
-          std::fprintf(f, fmt, tr[(size_t)s * nrec + (size_t)m * npt + r]);
+          std::fprintf(f, fmt, slot[r] < 0 ? 0.0 : tr[(size_t)s * nrec + (size_t)m * nin + slot[r]]);
       // elastic: then the stress C grad u of each point, d*d values row-major
       // (sigma_ij = lambda tr(grad u) delta_ij + mu (du_i/dx_j + du_j/dx_i),
       // ModifyOutputUnknown, Elliptic/Elastic/VarElastic.cxx:257-282), with
@@ -2061,8 +2076,12 @@ int main(int argc, char **argv) {
       if (elastic)
         for (int r = 0; r < npt; ++r) {
           const size_t dd = (size_t)dim * dim;
-          const double *g = &tr[(size_t)s * nrec + (size_t)dim * npt + (size_t)r * dd];
-          const int64_t e = rec_elem[(size_t)dim * npt + (size_t)r * dd];
+          if (slot[r] < 0) {
+            for (size_t k = 0; k < dd; ++k) std::fprintf(f, fmt, 0.0);
+            continue;
+          }
+          const double *g = &tr[(size_t)s * nrec + (size_t)dim * nin + (size_t)slot[r] * dd];
+          const int64_t e = rec_elem[(size_t)dim * nin + (size_t)slot[r] * dd];
           const double lam = md.lam[e], mu = md.mu[e];
           double div = 0.0;
           for (int c = 0; c < dim; ++c) div += g[c * dim + c];

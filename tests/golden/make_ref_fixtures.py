@@ -321,6 +321,15 @@ def ref2d_el_q4_sinus():
 
 
 @case
+def ref2d_q4_rec_outside():
+    """SismoPoint receivers outside the mesh (beyond x = 1 and below y = 0)
+    between ones inside: Montjoie writes 0 for them and keeps the columns"""
+    rec = np.array([[0.3, 0.4], [1.2, 0.5], [0.71, 0.52], [0.5, -0.1], [0.6, 0.8]])
+    run_case("ref2d_q4_rec_outside", 2, 6, 4, 0.1, 58, [1.0, 2.0], 10.0, 1e-3, 200, (0.47, 0.52),
+             rec)
+
+
+@case
 def ref2d_q3_src_uniform_ref():
     """TypeSource = SRC_VOLUME 2 UNIFORM: f = 1 on the elements of reference 2
     (the middle layer); 8x8 Q3 perturbed, three layers"""
