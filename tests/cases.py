@@ -289,6 +289,13 @@ def oracle_stress(c: Case, nsteps: int | None = None) -> np.ndarray:
     the element holding the point; rows n = 0..nsteps, d*d values per point
     (row-major), point-major.  DisplayStress = NO decks: grad u itself,
     d u_i / d x_a at i d + a."""
+    inside = receivers_inside(c)
+    if not inside.all():
+        # a point outside the mesh: d*d zero columns
+        g = oracle_stress(replace(c, rec=c.rec[inside]), nsteps)
+        out = np.zeros((g.shape[0], len(c.rec), c.dim * c.dim))
+        out[:, inside] = g.reshape(g.shape[0], int(inside.sum()), c.dim * c.dim)
+        return out.reshape(g.shape[0], -1)
     nsteps = c.nsteps if nsteps is None else nsteps
     d = c.dim
     m = O.mesh(c.dim, c.nx, c.r, c.pert, c.seed)

@@ -373,6 +373,20 @@ def ref3d_el_q3_order4():
              pol=(0.3, 0.6, -0.7), time_order=4)
 
 
+@case
+def ref2d_el_q5_order4_damped():
+    """the fourth-order scheme for 2-D elastodynamics with damping: 6x6 Q5
+    perturbed quadrilaterals, three layers, sigma 0, 3 and 9; one SismoPoint
+    outside the mesh (0 in its displacement and stress columns)"""
+    lam = [1.0 + 3.0 * O.uniform(2060, k) for k in range(3)]
+    mu = [1.0 + 3.0 * O.uniform(2061, k) for k in range(3)]
+    rho = [1.0 + O.uniform(2062, k) for k in range(3)]
+    rec = np.vstack([seeded_points(2, 4, 107), [[1.3, 0.45]]])
+    run_case("ref2d_el_q5_order4_damped", 2, 6, 5, 0.1, 29, mu, 8.0, 2e-3, 200, (0.47, 0.53),
+             rec, elastic=True, lam_layers=lam, rho_layers=rho, pol=(0.7, -0.4),
+             sigma_layers=[0.0, 3.0, 9.0], time_order=4)
+
+
 def _temporal_case(name, temporal, nsteps=250):
     """8x8 Q3 perturbed quadrilaterals, two layers, Frequency 10, a Dirac
     source driven by the TemporalSource line `temporal`."""
