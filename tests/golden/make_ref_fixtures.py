@@ -555,6 +555,17 @@ def ref2d_q4_sinus():
 
 
 @case
+def ref2d_q4_order4_sinus():
+    """the fourth-order scheme with rho and mu varying inside the elements
+    (PhysicalMedia SINUS) and damping by layer: 8x8 Q4 unit square"""
+    mu = [0.1, 0.15, 0.9, 0.85, 0.03, 0.02, 1.8, 1.0, 0.6, 1.1]
+    rho = [0.0, 0.0, 1.0, 1.0, 0.0, 0.0, 1.4, 0.3, 0.9, 1.7]
+    run_case("ref2d_q4_order4_sinus", 2, 8, 4, 0.0, 0, [1.0], 10.0, 1.5e-3, 160, (0.45, 0.55),
+             seeded_points(2, 6, 109), regular=True, sinus_rho=rho, sinus_mu=mu,
+             sigma_layers=[3.0], time_order=4)
+
+
+@case
 def ref3d_q3_sinus():
     """4^3 Q3 perturbed hexahedra, rho and mu varying inside the elements."""
     rec = seeded_points(3, 10, 80)
