@@ -287,6 +287,16 @@ def ref2d_q4_src_volume():
 
 
 @case
+def ref3d_q3_order4_src_volume():
+    """the fourth-order scheme driven by a Gaussian volume source (f, f' and
+    f'' times the same load vector): 4^3 Q3 perturbed hexahedra, two layers,
+    a MODIFIED_RICKER pulse"""
+    run_case("ref3d_q3_order4_src_volume", 3, 4, 3, 0.1, 57, [1.0, 2.5], 8.0, 2e-3, 150,
+             (0.5, 0.48, 0.52), seeded_points(3, 5, 111), temporal="MODIFIED_RICKER 0.06",
+             time_order=4, volume=([], (0.5, 0.48, 0.52), 0.2, 0.3))
+
+
+@case
 def ref3d_q3_src_volume_ref():
     """SRC_VOLUME 1 GAUSSIAN ...: the volume source restricted to the elements
     of reference 1 (the lower layer), its Gaussian crossing the layer
