@@ -689,6 +689,17 @@ def ref3d_el_q3_reprise():
 
 
 @case
+def ref2d_el_q4_order4_reprise():
+    """2-D elastic fourth-order scheme with SaveReprise every 30 of 120 steps
+    (the checkpoints hold U^n only): 6x6 Q4 perturbed, two layers"""
+    lam = [1.5 + 2.0 * O.uniform(2064, k) for k in range(2)]
+    rho = [1.0 + O.uniform(2065, k) for k in range(2)]
+    run_case("ref2d_el_q4_order4_reprise", 2, 6, 4, 0.1, 31, [1.0, 2.0], 8.0, 2e-3, 120,
+             (0.52, 0.47), seeded_points(2, 4, 113), elastic=True, lam_layers=lam,
+             rho_layers=rho, pol=(-0.5, 0.9), save_reprise=30, time_order=4)
+
+
+@case
 def ref3d_el_q3_plane():
     """Elastic 4^3 Q3 perturbed hexahedra with a tilted SismoPlane partly
     outside the cube every 30 steps and a SismoLine every 45 steps: one file
